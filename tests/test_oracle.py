@@ -1,0 +1,154 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+* the reference's own regression decks (final U vs U.ref, ew/_cli/_run_tests_edelweissfe.py:96-109)
+* element invariants: symmetry, rigid-body null space, patch test
+* tangents against central finite differences of the residual (J2 plastic, Neo-Hooke TL)
+* J2 closed form for linear hardening and yield-surface consistency
+* Neo-Hooke homogeneous closed form and stress-free reference state
+* CSR pattern: scipy canonical format vs brute-force dense assembly, and the cube closed form 9(3N-2)^3
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import decks
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build():
+    O.build()
+
+
+@pytest.mark.parametrize("name", ["TensionBarHexa8", "CantileverBeamHexa8", "C3D20", "C3D8", "SimpleBeamHexa8"])
+def test_reference_decks(name):
+    fx = decks.load(name)
+    X, conn, sets = decks.mesh(fx, O.boxgen)
+    r = O.solve_step(dict(el_kind=fx["el_type"], mat_kind=fx["material"], mp=fx["mp"], X=X, conn=conn),
+                     decks.step(fx, sets))
+    err = np.abs(r["U"] - np.array(fx["U_ref"])).max()
+    assert err < 1e-6  # the reference's own bar
+    assert err < 1e-7 * max(1.0, np.abs(fx["U_ref"]).max())
+
+
+def _jittered_hex(nn, seed=0, amp=0.08):
+    X, conn, _ = O.boxgen(1, 1, 1, nnodes=nn)
+    rng = np.random.default_rng(seed)
+    Xe = X[conn[0]] + rng.uniform(-amp, amp, size=(nn, 3))
+    return Xe
+
+
+@pytest.mark.parametrize("ek,nn", [("C3D8", 8), ("C3D20", 20), ("C3D8TL", 8), ("C3D20TL", 20)])
+def test_symmetry_rigid_body_patch(ek, nn):
+    Xe = _jittered_hex(nn)
+    nd = 3 * nn
+    mk, mp = ("neohookeanwa", [80.0, 200.0]) if "TL" in ek else ("linearelastic", [210000.0, 0.3])
+    nq = 8 if nn == 8 else 27
+    st = np.zeros(nq * O.NSTATE[O.MAT_KINDS[mk]])
+    K, P, _, _ = O.element(ek, mk, mp, Xe.ravel(), np.zeros(nd), np.zeros(nd), st)
+    scale = np.abs(K).max()
+    assert np.abs(K - K.T).max() < 1e-12 * scale
+    w = np.linalg.eigvalsh(0.5 * (K + K.T))
+    assert np.all(np.abs(w[:6]) < 1e-10 * scale) and w[6] > 1e-3 * scale
+    assert np.abs(P).max() < 1e-12 * scale
+    if "TL" not in ek:  # patch test: linear field -> constant strain, K u = -P exactly
+        G = np.array([[1e-3, 2e-4, 0], [0, -3e-4, 1e-4], [5e-4, 0, 2e-4]])
+        u = (Xe @ G.T).ravel()
+        K, P, st1, _ = O.element(ek, mk, mp, Xe.ravel(), u, u, st)
+        assert np.abs(K @ u + P).max() < 1e-10 * np.abs(P).max()
+        eps = np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[0, 2] + G[2, 0], G[1, 2] + G[2, 1]])
+        for q in range(nq):
+            np.testing.assert_allclose(st1.reshape(nq, -1)[q, 6:12], eps, rtol=1e-10, atol=1e-16)
+
+
+def _fd_tangent(ek, mk, mp, Xe, U0, dU0, st, h=1e-7):
+    nd = len(U0)
+    Kfd = np.zeros((nd, nd))
+    for j in range(nd):
+        e = np.zeros(nd)
+        e[j] = h
+        _, Pp, _, _ = O.element(ek, mk, mp, Xe, U0 + e, dU0 + e, st)
+        _, Pm, _, _ = O.element(ek, mk, mp, Xe, U0 - e, dU0 - e, st)
+        Kfd[:, j] = -(Pp - Pm) / (2 * h)
+    return Kfd
+
+
+def test_j2_tangent_vs_fd_plastic():
+    Xe = _jittered_hex(8, seed=3)
+    mp = [210000.0, 0.3, 250.0, 1000.0, 50.0, 20.0]
+    rng = np.random.default_rng(1)
+    G = np.array([[4e-3, 1e-3, 0], [0, -2e-3, 5e-4], [1e-3, 0, -1e-3]])
+    u = (Xe @ G.T).ravel() + rng.uniform(-1e-4, 1e-4, 24)
+    st = np.zeros(8 * 13)
+    K, P, st1, pdt = O.element("C3D8", "vonmises", mp, Xe.ravel(), u, u, st)
+    assert pdt > 1 and np.all(st1.reshape(8, 13)[:, 12] > 0)  # every QP yielded
+    Kfd = _fd_tangent("C3D8", "vonmises", mp, Xe.ravel(), u, u, st, h=1e-9)
+    assert np.abs(K - Kfd).max() < 1e-5 * np.abs(K).max()
+    assert np.abs(K - K.T).max() < 1e-10 * np.abs(K).max()  # associative J2: symmetric tangent
+
+
+@pytest.mark.parametrize("ek,nn", [("C3D8TL", 8), ("C3D20TL", 20)])
+def test_neohooke_tangent_vs_fd(ek, nn):
+    Xe = _jittered_hex(nn, seed=5)
+    rng = np.random.default_rng(2)
+    u = rng.uniform(-0.08, 0.08, 3 * nn)
+    nq = 8 if nn == 8 else 27
+    st = np.zeros(nq * 13)
+    K, P, _, _ = O.element(ek, "neohookeanwa", [80.0, 200.0], Xe.ravel(), u, u, st)
+    Kfd = _fd_tangent(ek, "neohookeanwa", [80.0, 200.0], Xe.ravel(), u, u, st, h=1e-6)
+    assert np.abs(K - Kfd).max() < 1e-6 * np.abs(K).max()
+
+
+def test_j2_linear_hardening_closed_form():
+    """Homogeneous strain, one element: kappa follows the closed form and the returned stress sits on the
+    yield surface (vonmises.py:121, 237-240)."""
+    E, nu, fy0, H = 210000.0, 0.3, 250.0, 1000.0
+    G = E / (2 * (1 + nu))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    Xe = _jittered_hex(8, seed=7, amp=0.05)
+    Gd = np.array([[3e-3, 0, 0], [0, -1e-3, 0], [0, 0, -1e-3]]) + np.array([[0, 1e-3, 0], [1e-3, 0, 0], [0, 0, 0]])
+    u = (Xe @ Gd.T).ravel()
+    K, P, st1, _ = O.element("C3D8", "vonmises", [E, nu, fy0, H, 0, 0], Xe.ravel(), u, u, np.zeros(8 * 13))
+    eps = np.array([3e-3, -1e-3, -1e-3, 2e-3, 0, 0])
+    tr = lam * eps[:3].sum() * np.array([1, 1, 1, 0, 0, 0]) + 2 * G * np.array([*eps[:3], *(eps[3:] / 2)])
+    s = tr - np.array([1, 1, 1, 0, 0, 0]) * tr[:3].mean()
+    ns = np.sqrt((s[:3] ** 2).sum() + 2 * (s[3:] ** 2).sum())
+    dk = (ns - np.sqrt(2 / 3) * fy0) / (np.sqrt(6) * G + np.sqrt(2 / 3) * H)
+    for q in range(8):
+        sq = st1.reshape(8, 13)[q]
+        assert abs(sq[12] - dk) < 1e-12 * dk
+        sd = sq[:6] - np.array([1, 1, 1, 0, 0, 0]) * sq[:3].mean()
+        nsd = np.sqrt((sd[:3] ** 2).sum() + 2 * (sd[3:] ** 2).sum())
+        assert abs(nsd - np.sqrt(2 / 3) * (fy0 + H * sq[12])) < 1e-10 * fy0
+
+
+def test_neohooke_homogeneous_closed_form():
+    mu, Kb = 80.0, 200.0
+    Xe = _jittered_hex(8, seed=9, amp=0.05)
+    Fh = np.array([[1.3, 0.1, 0.0], [0.05, 0.9, 0.02], [0.0, -0.1, 1.1]])
+    u = (Xe @ (Fh - np.eye(3)).T).ravel()
+    _, _, st1, _ = O.element("C3D8TL", "neohookeanwa", [mu, Kb], Xe.ravel(), u, u, np.zeros(8 * 13))
+    J = np.linalg.det(Fh)
+    T = mu * Fh @ Fh.T + ((Kb - 2 * mu / 3) * (J * J - J) - mu) * np.eye(3)
+    ref = np.array([T[0, 0], T[1, 1], T[2, 2], T[0, 1], T[1, 2], T[0, 2]])
+    for q in range(8):
+        np.testing.assert_allclose(st1.reshape(8, 13)[q, :6], ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,nn", [(2, 8), (3, 8), (2, 20)])
+def test_csr_pattern_bruteforce(n, nn):
+    X, conn, _ = O.boxgen(n, n, n, nnodes=nn)
+    base, nDof, edofs = O.dof_numbering(len(X), conn)
+    I, J = O.vij_pattern(edofs)
+    indptr, indices, x = O.csr_pattern(I, J, nDof)
+    dense = np.zeros((nDof, nDof), dtype=bool)
+    for e in range(len(conn)):
+        d = edofs[e]
+        dense[np.ix_(d, d)] = True
+    m = sp.csr_matrix(dense)
+    assert np.array_equal(m.indptr, indptr) and np.array_equal(m.indices, indices)
+    rows = np.repeat(np.arange(nDof), np.diff(indptr))
+    assert np.array_equal(rows[x], I) and np.array_equal(indices[x], J)
+    if nn == 8:
+        # nnz of a structured hex mesh: 9 * prod over axes of (3*nodes - 2)
+        assert len(indices) == 9 * (3 * (n + 1) - 2) ** 3
