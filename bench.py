@@ -1,0 +1,199 @@
+"""Benchmark of the element evaluation + CSR assembly path on one MI355X.
+
+Workload (default): C3D8 small-strain J2 plasticity on a 128x128x128 structured hex mesh of the unit cube
+(2,097,152 elements, 8 quadrature points each, state = plastic strain 6 + kappa per point, structure of
+arrays), nodes jittered by U(-5%, +5%) of h (seed 0; faces kept planar), E=210000, nu=0.3, fy0=250,
+H=1000. The state is taken half way through the uniaxial loading to 1 % strain: every quadrature point is
+plastic and runs the radial return. One "step" = one Newton iteration's element evaluation + assembly
+(the reference's timing keys "elements" + "CSR generation"): P, F, trial state and all CSR values of the
+6,440,067-dof system, inputs resident in HBM. The metric is elements per second.
+
+Also reported: the FP32-tangent variant of the same step, the roofline fraction of the step (algorithmic
+bytes per element: connectivity + nodal X/U/dU + state read/write + P/F + CSR values) and the CPU
+baseline: the oracle (C, OpenMP) on a bounded slab of the same workload, on this host's cores.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+J2 = [210000.0, 0.3, 250.0, 1000.0, 0.0, 0.0]
+
+
+def uniaxial(X, strain):
+    u = np.zeros_like(X)
+    u[:, 2] = strain * X[:, 2]
+    u[:, 0] = -0.5 * strain * X[:, 0]
+    u[:, 1] = -0.5 * strain * X[:, 1]
+    return u.ravel()
+
+
+def build_workload(n, seed=0):
+    from edelweissfe_amd.model import ElementBlock, FEModel, Material, boxGenArrays, jitterNodes
+
+    X, conn, sets, grid = boxGenArrays(n, n, n, elType="C3D8")
+    X = jitterNodes(X, grid, (n + 1,) * 3, np.full(3, 1.0 / n), 0.05, seed)
+    m = FEModel()
+    m.addNodes(X)
+    b = ElementBlock("C3D8", conn)
+    b.setMaterial(Material("vonmises", J2))
+    m.addElementBlock(b, "all")
+    m.nodeSets.update(sets)
+    return m, b, X, conn
+
+
+def algorithmic_bytes_per_element(n_el, n_nodes, nnz, n_state, n_qp, val_bytes, npe=8):
+    return (4 * npe + 72.0 * n_nodes / n_el + 16.0 * n_state * n_qp + 48.0 * n_nodes / n_el
+            + val_bytes * nnz / n_el)
+
+
+def cpu_baseline(X, conn, U_n, dU, layers, n_per_layer, reps=3):
+    """The oracle (C + OpenMP restatement of NISTParallel's element loop, serial P/F scatter and serial
+    VIJ->CSR scatter) on the first `layers` x-layers of the same mesh and inputs."""
+    from oracle import oracle as O
+
+    O.build()
+    ns = layers * n_per_layer
+    c = conn[:ns]
+    used = np.unique(c)
+    remap = -np.ones(len(X), dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    cs = remap[c].astype(np.int32)
+    Xs = X[used]
+    base, nDof, edofs = O.dof_numbering(len(Xs), cs)
+    I, J = O.vij_pattern(edofs)
+    indptr, indices, x = O.csr_pattern(I, J, nDof)
+    del I, J
+    Un = U_n.reshape(-1, 3)[used].ravel()
+    dUs = dU.reshape(-1, 3)[used].ravel()
+    threads = min(os.cpu_count() or 1, 16)
+    st0 = np.zeros(ns * 8 * 13)
+    r0 = O.eval_assemble("C3D8", "vonmises", J2, cs, Xs, edofs, Un, Un, st0, x, len(indices), n_threads=threads)
+    ts = []
+    for _ in range(reps):
+        r = O.eval_assemble("C3D8", "vonmises", J2, cs, Xs, edofs, Un + dUs, dUs, r0["state"], x, len(indices),
+                            n_threads=threads)
+        ts.append(r["t_elements"] + r["t_csr"])
+    t = float(np.median(ts))
+    return dict(value=ns / t, unit="elements/s", cores=threads, kind="port",
+                sample=f"first {layers} x-layers ({ns} elements) of the same jittered 128^3 J2 mesh and state, "
+                       f"{reps} timed evaluations, median ('elements' + 'CSR generation' = {t:.3f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128, help="elements per cube edge")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-layers", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from edelweissfe_amd.assembly import ElementAssembly
+    from edelweissfe_amd.dofmanager import DofManager
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda:0")
+    n = args.n
+    t0 = time.time()
+    m, b, X, conn = build_workload(n)
+    dm = DofManager(m, device="cuda")
+    asm = ElementAssembly(b, dm, m, device="cuda")
+    t_setup = time.time() - t0
+    print(f"setup {n}^3: {len(conn)} elements, {dm.nDof} dofs, nnz {asm.nnz}, colours {asm.nColors}, "
+          f"{t_setup:.1f}s", file=sys.stderr, flush=True)
+
+    U_n = uniaxial(X, 5e-3)
+    dU = uniaxial(X, 1e-3)
+    U_n_t = torch.as_tensor(U_n, device=dev)
+    dU_t = torch.as_tensor(dU, device=dev)
+    U_np_t = U_n_t + dU_t
+    P, F = dm.constructDofVector(), dm.constructDofVector()
+    K64 = asm.newValues("fp64")
+    K32 = asm.newValues("fp32")
+    # converged state at U_n (first loading from the virgin state), then accepted
+    asm.evaluate(U_n_t, U_n_t, P, F, K64)
+    b.acceptLastState()
+    torch.cuda.synchronize()
+    kappa = b.stateVars.view(7, 8, -1)[6]
+    plastic_frac = float((kappa > 0).double().mean().item())
+
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(K, steps, warmup):
+        for _ in range(warmup):
+            asm.evaluate(U_np_t, dU_t, P, F, K)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t_wall = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(steps):
+            asm.evaluate(U_np_t, dU_t, P, F, K)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_wall = time.perf_counter() - t_wall
+        return ev0.elapsed_time(ev1) / steps, t_wall * 1e3 / steps
+
+    ms64, wall64 = timed(K64, args.steps, args.warmup)
+    pdt = float(asm.pnewdt.item())
+    ms32, wall32 = timed(K32, args.steps, args.warmup)
+    print(f"fp64 step {ms64:.3f} ms (wall {wall64:.3f}), fp32-tangent step {ms32:.3f} ms (wall {wall32:.3f}), "
+          f"plastic QPs {plastic_frac:.3f}, pNewDT {pdt:g}", file=sys.stderr, flush=True)
+
+    n_el = len(conn)
+    B64 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 8)
+    B32 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 4)
+    ach64 = B64 * n_el / (ms64 * 1e-3) / 1e9
+    ach32 = B32 * n_el / (ms32 * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(X, conn, U_n, dU, args.cpu_layers, n * n)
+        print(f"cpu baseline {cpu['value']:.4g} el/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "C3D8 J2 elem evals+assembly/s",
+        "value": n_el / (ms64 * 1e-3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"C3D8 J2 small strain {n}^3 jittered unit cube, one Newton iteration's element "
+                               f"evaluation + CSR assembly, FP64", "elements": n_el, "dofs": dm.nDof,
+                   "nnz": asm.nnz, "qp_per_element": 8, "state_per_qp": 7, "plastic_qp_fraction": plastic_frac},
+        "roofline": {"bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach64 / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_element": B64,
+                     "scope": "one evaluation call: CSR/P/F zero fill + 8 colour launches of k_hex8"},
+        "cpu_baseline": cpu,
+        "fp32_tangent": {"value": n_el / (ms32 * 1e-3), "ms_per_step": ms32, "speedup_vs_fp64": ms64 / ms32,
+                         "roofline_achieved_GBs": ach32, "roofline_frac": ach32 / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_element": B32},
+        "setup_s": t_setup,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+"""GPU element evaluation + CSR assembly of one element block through libewfe.so.
+
+``ElementAssembly`` is the device-side counterpart of the reference's per-step CSRGenerator plus the
+element loop of NISTParallel:
+  * construction = DofManager VIJ pattern + CSRGenerator.__init__ (dofmanager.py:522-557,
+    csrgenerator.pyx:47-98): ewfe_plan_create builds the canonical CSR pattern, the element -> slot map and
+    the colouring;
+  * ``evaluate`` = computeElements + assembleStiffnessCSR (nonlinearimplicitstatic.py:794-849, 753-769,
+    mk2.pyx:111-198, csrgenerator.pyx:100-115) in one stream-ordered call;
+  * ``acceptLastState`` = MarmotElementWrapper.acceptLastState (element.pyx:278-281).
+Quadrature-point state is kept on the device in plan order, [component][qp][element].
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import EWFE_F32, EWFE_F64, check, lib, ptr
+
+
+class ElementAssembly:
+    def __init__(self, block, dofManager, model, device="cuda"):
+        import torch
+
+        self.torch = torch
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError("ElementAssembly runs on a HIP device only")
+        torch.cuda.set_device(self.device)
+        self.block = block
+        self.nDof = dofManager.nDof
+        L = lib()
+        plan = ctypes.c_void_p()
+        nnz = ctypes.c_int64()
+        conn = np.ascontiguousarray(block.conn, dtype=np.int32)
+        node_dof = np.ascontiguousarray(dofManager.nodeDofBase, dtype=np.int64)
+        check(L.ewfe_plan_create(ctypes.byref(plan), block.kind, len(conn), ptr(conn), len(node_dof), ptr(node_dof),
+                                 self.nDof, ctypes.byref(nnz)))
+        self._plan = plan
+        self.nnz = nnz.value
+        order = np.zeros(len(conn), dtype=np.int64)
+        ncol = ctypes.c_int32()
+        check(L.ewfe_plan_element_order(plan, ptr(order), ctypes.byref(ncol)))
+        self.elementOrder = order
+        self.nColors = ncol.value
+        ip, ix = ctypes.c_void_p(), ctypes.c_void_p()
+        check(L.ewfe_plan_pattern_dev(plan, ctypes.byref(ip), ctypes.byref(ix)))
+        self.X = dofManager.coordinatesAsDofVector(model)
+        self.matKind = block.material.kind
+        mp = np.zeros(8)
+        mp[: len(block.material.materialProperties)] = block.material.materialProperties[:8]
+        self.matParams = mp
+        self.elParams = (None if block.elParams is None else
+                         torch.as_tensor(block.elParams, dtype=torch.float64, device=self.device))
+        self.nState = L.ewfe_state_size(self.matKind)
+        nInt = block.nInt
+        if self.nState > 0:
+            block.stateVars = torch.zeros(self.nState * nInt * len(conn), dtype=torch.float64, device=self.device)
+            block.stateVarsTemp = torch.zeros_like(block.stateVars)
+        else:
+            block.stateVars = block.stateVarsTemp = None
+        block.planOrder = order
+        self.pnewdt = torch.empty(1, dtype=torch.float64, device=self.device)
+        self._pcg_ws = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_plan", None):
+                lib().ewfe_plan_destroy(self._plan)
+                self._plan = None
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------------------------------------
+    def pattern(self):
+        """Host copy of the CSR pattern (indptr, indices), int32."""
+        indptr = np.zeros(self.nDof + 1, dtype=np.int32)
+        indices = np.zeros(self.nnz, dtype=np.int32)
+        check(lib().ewfe_plan_pattern(self._plan, ptr(indptr), ptr(indices)))
+        return indptr, indices
+
+    def newValues(self, dtype="fp64"):
+        tdt = self.torch.float64 if dtype == "fp64" else self.torch.float32
+        return self.torch.zeros(self.nnz, dtype=tdt, device=self.device)
+
+    @staticmethod
+    def _dtype_code(vals):
+        import torch
+
+        if vals.dtype == torch.float64:
+            return EWFE_F64
+        if vals.dtype == torch.float32:
+            return EWFE_F32
+        raise TypeError("CSR values must be float64 or float32")
+
+    def _stream(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def evaluate(self, U_np, dU, P, F, vals):
+        """Fills P (= sum Pe), F (= sum |Pe|), the CSR values and the trial state. Returns pNewDT (device
+        tensor, 1e36 or < 1 when a quadrature point asks for a cutback)."""
+        for t in (U_np, dU, P, F):
+            if t.numel() != self.nDof or t.dtype != self.torch.float64 or not t.is_contiguous():
+                raise ValueError("dof vectors must be contiguous float64 tensors of length nDof")
+        if vals.numel() != self.nnz:
+            raise ValueError("CSR value array has the wrong length")
+        b = self.block
+        check(lib().ewfe_eval_assemble(self._plan, self.matKind, ptr(self.matParams), ptr(self.elParams),
+                                       ptr(self.X), ptr(U_np), ptr(dU), ptr(b.stateVars), ptr(b.stateVarsTemp),
+                                       ptr(P), ptr(F), ptr(vals), self._dtype_code(vals), ptr(self.pnewdt),
+                                       self._stream()))
+        return self.pnewdt
+
+    def dirichletRows(self, rows_dev, vals):
+        check(lib().ewfe_dirichlet_rows(self._plan, ptr(rows_dev), rows_dev.numel(), ptr(vals),
+                                        self._dtype_code(vals), self._stream()))
+
+    def spmv(self, vals, x, y):
+        check(lib().ewfe_spmv(self._plan, ptr(vals), self._dtype_code(vals), ptr(x), ptr(y), self._stream()))
+        return y
+
+    def pcg(self, vals, b, x, rows_dev, rtol=1e-10, maxit=20000, check_every=32):
+        """Jacobi-PCG for the row-eliminated system; returns (iterations, relative residual)."""
+        if self._pcg_ws is None:
+            nbytes = lib().ewfe_pcg_workspace_size(self._plan)
+            self._pcg_ws = self.torch.empty((nbytes + 7) // 8, dtype=self.torch.float64, device=self.device)
+        it = ctypes.c_int()
+        rr = ctypes.c_double()
+        n_rows = 0 if rows_dev is None else rows_dev.numel()
+        check(lib().ewfe_pcg(self._plan, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev), n_rows,
+                             float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws), ctypes.byref(it),
+                             ctypes.byref(rr), self._stream()))
+        return it.value, rr.value
+
+    def stateInElementOrder(self, which="trial"):
+        """[nEl, nInt, nState] host array of the state in original element order."""
+        t = self.block.stateVarsTemp if which == "trial" else self.block.stateVars
+        if t is None:
+            return np.zeros((len(self.block), self.block.nInt, 0))
+        a = t.detach().cpu().numpy().reshape(self.nState, self.block.nInt, -1)  # [s][q][p]
+        out = np.empty((len(self.block), self.block.nInt, self.nState))
+        out[self.elementOrder] = np.transpose(a, (2, 1, 0))
+        return out
+
+    def setStateFromElementOrder(self, state):
+        """Inverse of stateInElementOrder for the converged state."""
+        if self.nState == 0:
+            return
+        s = np.asarray(state, dtype=np.float64)[self.elementOrder]  # [p][q][s]
+        t = self.torch.as_tensor(np.ascontiguousarray(np.transpose(s, (2, 1, 0))).ravel(), device=self.device)
+        self.block.stateVars.copy_(t)
+
+
+_ = _lib  # re-exported for callers that need the enum values

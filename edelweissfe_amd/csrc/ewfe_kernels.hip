@@ -1,0 +1,944 @@
+// libewfe.so -- element evaluation + CSR assembly kernels for gfx950 (MI355X), and the C ABI.
+//
+// What one ewfe_eval_assemble call computes is the reference's per-iteration element loop
+// (edelweissfe/solvers/nonlinearimplicitstaticparallelmk2.pyx:111-198 -> MarmotElement::computeYourself)
+// followed by CSRGenerator.updateCSR (edelweissfe/numerics/csrgenerator.pyx:100-115):
+//   P = sum_e Pe (Pe = -f_int,e), F = sum_e |Pe|, CSR values = sum_e Ke scattered through the slot map.
+// The element formulations follow the in-tree specs of the reference:
+//   small strain: edelweissfe/elements/displacementelement/element.py:290-346
+//   total Lagrangian: edelweissfe/elements/displacementtlelement/element.py:346-427
+//   J2: edelweissfe/materials/vonmises/vonmises.py:186-254; linear elastic: linearelastic.py:185-210
+//   Neo-Hooke Wa: edelweissfe/materials/neohooke/neohookepencegouformulationa.py:107-141
+//
+// MI355X design
+//  * Elements are processed colour by colour (no two elements of a colour share a node), so the
+//    scatter into CSR is a plain read-add-write with a fixed, run-independent accumulation order:
+//    deterministic FP64 without atomics.
+//  * Quadrature-point state is structure-of-arrays [component][qp][element] in plan order, so
+//    consecutive lanes touch consecutive addresses.
+//  * C3D8 family: a 64-lane workgroup holds 8 elements. Phase 0 stages nodal X/U/dU in LDS (lane = node),
+//    phase 1 runs one quadrature point per lane (Jacobian, gradients, material update) and leaves the
+//    per-QP tangent "recipe" in LDS, phase 2 runs one node per lane: the lane for node a builds the
+//    3x3 blocks K_ab for b in {a, a+1, a+2, a+3 (, a+4 if a<4)} mod 8 -- the 36 blocks of the symmetric
+//    24x24 Ke, each exactly once -- and scatters K_ab and K_ab^T.
+//  * The tangent uses the rank structure of the isotropic/J2/Neo-Hooke tangents instead of a dense
+//    B^T C B:  K_ab,ik = sum_q [ alpha p_a,i p_b,k + beta p_a,k p_b,i + gamma delta_ik (r_a . r_b)
+//                                - c v_a,i v_b,k ] ,  v = N p
+//    with p = r = dN/dx (small strain) or p = dN/dx (spatial), r = dN/dX (total Lagrangian).
+//  * C3D20 family: one element per 64-lane workgroup; lanes 0..26 run the quadrature points, lanes
+//    0..59 build one row (node a, component i) of Ke each.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "ewfe.h"
+#include "ewfe_internal.h"
+
+// ------------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+void ewfe::set_error(const std::string &m) { g_err = m; }
+extern "C" const char *ewfe_last_error(void) { return g_err.c_str(); }
+extern "C" int ewfe_version(void) { return 1; }
+
+#define HIP_CHECK(expr)                                                                          \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess) {                                                                  \
+            ewfe::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+            return EWFE_EHIP;                                                                    \
+        }                                                                                        \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------
+// shape-function tables (Abaqus node order; Gauss points q = i + n*j + n*n*k over (xi, eta, zeta))
+// ------------------------------------------------------------------------------------------------
+__constant__ double c_dN8[8][8][3];
+__constant__ double c_w8[8];
+__constant__ double c_dN20[27][20][3];
+__constant__ double c_w27[27];
+
+static const double kHexXi[20][3] = {
+    {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
+    {0, -1, -1},  {1, 0, -1},  {0, 1, -1}, {-1, 0, -1}, {0, -1, 1},  {1, 0, 1},  {0, 1, 1}, {-1, 0, 1},
+    {-1, -1, 0},  {1, -1, 0},  {1, 1, 0},  {-1, 1, 0}};
+
+static void host_dN(int nn, const double x[3], double dN[][3])
+{
+    for (int a = 0; a < nn; ++a) {
+        const double xa = kHexXi[a][0], ya = kHexXi[a][1], za = kHexXi[a][2];
+        if (nn == 8) {
+            dN[a][0] = 0.125 * xa * (1 + ya * x[1]) * (1 + za * x[2]);
+            dN[a][1] = 0.125 * (1 + xa * x[0]) * ya * (1 + za * x[2]);
+            dN[a][2] = 0.125 * (1 + xa * x[0]) * (1 + ya * x[1]) * za;
+        } else if (a < 8) {  // serendipity corner: 1/8 (1+xa x)(1+ya y)(1+za z)(xa x + ya y + za z - 2)
+            const double fx = 1 + xa * x[0], fy = 1 + ya * x[1], fz = 1 + za * x[2];
+            const double s = xa * x[0] + ya * x[1] + za * x[2] - 2;
+            dN[a][0] = 0.125 * xa * fy * fz * (s + fx);
+            dN[a][1] = 0.125 * ya * fx * fz * (s + fy);
+            dN[a][2] = 0.125 * za * fx * fy * (s + fz);
+        } else {  // mid-side: 1/4 (1 - t^2) along the zero axis
+            double f[3], df[3];
+            const double c[3] = {xa, ya, za};
+            for (int d = 0; d < 3; ++d) {
+                if (c[d] == 0) {
+                    f[d] = 1 - x[d] * x[d];
+                    df[d] = -2 * x[d];
+                } else {
+                    f[d] = 1 + c[d] * x[d];
+                    df[d] = c[d];
+                }
+            }
+            dN[a][0] = 0.25 * df[0] * f[1] * f[2];
+            dN[a][1] = 0.25 * f[0] * df[1] * f[2];
+            dN[a][2] = 0.25 * f[0] * f[1] * df[2];
+        }
+    }
+}
+
+static int upload_tables(int device)
+{
+    static std::mutex mu;
+    static bool done[256] = {false};
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0 || device >= 256) return EWFE_EINVAL;
+    if (done[device]) return 0;
+    double dN8[8][8][3], w8[8], dN20[27][20][3], w27[27];
+    const double g2 = 1.0 / std::sqrt(3.0), g3 = std::sqrt(0.6);
+    for (int q = 0; q < 8; ++q) {
+        const double x[3] = {(q & 1) ? g2 : -g2, (q & 2) ? g2 : -g2, (q & 4) ? g2 : -g2};
+        host_dN(8, x, dN8[q]);
+        w8[q] = 1.0;
+    }
+    const double pts[3] = {-g3, 0.0, g3}, wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    for (int q = 0; q < 27; ++q) {
+        const int i = q % 3, j = (q / 3) % 3, k = q / 9;
+        const double x[3] = {pts[i], pts[j], pts[k]};
+        host_dN(20, x, dN20[q]);
+        w27[q] = wts[i] * wts[j] * wts[k];
+    }
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN8), dN8, sizeof(dN8)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_w8), w8, sizeof(w8)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN20), dN20, sizeof(dN20)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_w27), w27, sizeof(w27)));
+    done[device] = true;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// per-quadrature-point material update -> stress S (wd folded in later) + tangent recipe
+// ------------------------------------------------------------------------------------------------
+struct MatParams {
+    double p[8];
+};
+
+enum { FORM_SS = 0, FORM_TL = 1 };
+enum { M_LE = 0, M_J2 = 1, M_NH = 2 };
+
+struct Recipe {
+    double S[6];  // stress, tensor components [11,22,33,12,13,23]
+    double alpha, beta, gam, c;
+    double N[6];  // J2 flow direction (tensor, unit norm), used when c != 0
+};
+
+__device__ __forceinline__ double det3(const double A[3][3])
+{
+    return A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+           A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+}
+
+__device__ __forceinline__ double inv3(const double A[3][3], double B[3][3])
+{
+    const double d = det3(A);
+    const double r = 1.0 / d;
+    B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) * r;
+    B[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * r;
+    B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * r;
+    B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) * r;
+    B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * r;
+    B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * r;
+    B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) * r;
+    B[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * r;
+    B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * r;
+    return d;
+}
+
+// Linear elasticity (linearelastic.py:95-117): lambda = f nu, mu = f (1 - 2 nu) / 2,
+// f = E / ((1 + nu)(1 - 2 nu)) -- the entries of the reference's Voigt matrix.
+__device__ __forceinline__ void le_recipe(const MatParams &m, double E, const double H[3][3], Recipe &r)
+{
+    const double nu = m.p[1];
+    const double f = E / ((1 + nu) * (1 - 2 * nu));
+    const double lam = f * nu, mu = f * (1 - 2 * nu) / 2;
+    const double e[6] = {H[0][0], H[1][1], H[2][2], 0.5 * (H[0][1] + H[1][0]), 0.5 * (H[0][2] + H[2][0]),
+                         0.5 * (H[1][2] + H[2][1])};
+    const double tr = e[0] + e[1] + e[2];
+    for (int i = 0; i < 3; ++i) r.S[i] = lam * tr + 2 * mu * e[i];
+    for (int i = 3; i < 6; ++i) r.S[i] = 2 * mu * e[i];
+    r.alpha = lam;
+    r.beta = mu;
+    r.gam = mu;
+    r.c = 0;
+}
+
+// J2 radial return with isotropic hardening fy = fy0 + H k + dfy (1 - exp(-delta k)), consistent tangent
+// (vonmises.py:186-254). State: plastic strain (engineering Voigt [11,22,33,12,13,23]) + kappa.
+// Returns false when the local Newton does not converge in 15 iterations (the reference's CutbackRequest).
+__device__ bool j2_recipe(const MatParams &m, double E, const double H[3][3], const double dH[3][3],
+                          const double ep[6], double kappa, double ep_t[6], double &kappa_t, Recipe &r)
+{
+    const double nu = m.p[1], fy0 = m.p[2], HL = m.p[3], dfy = m.p[4], delta = m.p[5];
+    const double f = E / ((1 + nu) * (1 - 2 * nu));
+    const double lam = f * nu, mu = f * (1 - 2 * nu) / 2;
+    const double G = E / (2 * (1.0 + nu));
+    // total and incremental strain (tensor components)
+    const double e[6] = {H[0][0], H[1][1], H[2][2], 0.5 * (H[0][1] + H[1][0]), 0.5 * (H[0][2] + H[2][0]),
+                         0.5 * (H[1][2] + H[2][1])};
+    const double de[6] = {dH[0][0], dH[1][1], dH[2][2], 0.5 * (dH[0][1] + dH[1][0]), 0.5 * (dH[0][2] + dH[2][0]),
+                          0.5 * (dH[1][2] + dH[2][1])};
+    const double epT[6] = {ep[0], ep[1], ep[2], 0.5 * ep[3], 0.5 * ep[4], 0.5 * ep[5]};
+    for (int i = 0; i < 6; ++i) ep_t[i] = ep[i];
+    kappa_t = kappa;
+    r.alpha = lam;
+    r.beta = mu;
+    r.gam = mu;
+    r.c = 0;
+    // |dE| of the engineering-Voigt increment (vonmises.py:211)
+    const double dn = sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2] +
+                           4 * (de[3] * de[3] + de[4] * de[4] + de[5] * de[5]));
+    double ee[6];
+    if (dn < 1e-14) {  // stress stays at the converged state, elastic tangent
+        for (int i = 0; i < 6; ++i) ee[i] = (e[i] - de[i]) - epT[i];
+    } else {
+        for (int i = 0; i < 6; ++i) ee[i] = e[i] - epT[i];
+    }
+    const double tr = ee[0] + ee[1] + ee[2];
+    double s[6];
+    for (int i = 0; i < 3; ++i) r.S[i] = lam * tr + 2 * mu * ee[i];
+    for (int i = 3; i < 6; ++i) r.S[i] = 2 * mu * ee[i];
+    if (dn < 1e-14) return true;
+    const double p3 = (r.S[0] + r.S[1] + r.S[2]) / 3.0;
+    for (int i = 0; i < 3; ++i) s[i] = r.S[i] - p3;
+    for (int i = 3; i < 6; ++i) s[i] = r.S[i];
+    const double devNorm = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+    const double sq23 = sqrt(2.0 / 3.0), sq6 = sqrt(6.0);
+    const double fy_n = fy0 + HL * kappa + dfy * (1.0 - exp(-delta * kappa));
+    if (!(devNorm - sq23 * fy_n > 0.0)) return true;
+    double dK = 0;
+    int counter = 0;
+    for (;;) {
+        const double k = kappa + dK;
+        const double R = devNorm - sq6 * G * dK - sq23 * (fy0 + HL * k + dfy * (1.0 - exp(-delta * k)));
+        if (!(fabs(R) > 1e-12)) break;
+        if (counter == 15) return false;
+        const double dR = -sq6 * G - sq23 * (HL + dfy * delta * exp(-delta * k));
+        dK -= R / dR;
+        counter++;
+    }
+    const double dLam = sqrt(1.5) * dK;
+    kappa_t = kappa + dK;
+    for (int i = 0; i < 6; ++i) r.N[i] = s[i] / devNorm;
+    for (int i = 0; i < 6; ++i) r.S[i] -= 2.0 * G * dLam * r.N[i];
+    for (int i = 0; i < 3; ++i) ep_t[i] = ep[i] + dLam * r.N[i];
+    for (int i = 3; i < 6; ++i) ep_t[i] = ep[i] + 2.0 * dLam * r.N[i];
+    const double hp = HL + dfy * delta * exp(-delta * kappa_t);
+    const double theta = 2.0 * G * dLam / devNorm;
+    const double a1 = 2.0 * G * (1.0 / (1.0 + hp / (3.0 * G)) - theta);
+    const double a2 = 2.0 * G * theta;
+    r.alpha = lam + a2 / 3.0;
+    r.beta = mu - 0.5 * a2;
+    r.gam = r.beta;
+    r.c = a1;
+    return true;
+}
+
+// Neo-Hooke Wa (Pence-Gou): tau = mu B + lamBar I, lamBar = (K - 2mu/3)(J^2 - J) - mu,
+// muBar = (K - 2mu/3)(2J^2 - J) (neohookepencegouformulationa.py:130-141). With p = dN/dx, r = dN/dX the
+// total-Lagrangian tangent of element.py:405-427 reduces to alpha = muBar, beta = -lamBar, gamma = mu.
+__device__ __forceinline__ void nh_recipe(double mu, double Kb, const double F[3][3], double J, Recipe &r)
+{
+    const double lamBar = (Kb - 2.0 / 3.0 * mu) * (J * J - J) - mu;
+    const double muBar = (Kb - 2.0 / 3.0 * mu) * (2 * J * J - J);
+    double Bc[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Bc[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
+    r.S[0] = mu * Bc[0][0] + lamBar;
+    r.S[1] = mu * Bc[1][1] + lamBar;
+    r.S[2] = mu * Bc[2][2] + lamBar;
+    r.S[3] = mu * Bc[0][1];
+    r.S[4] = mu * Bc[0][2];
+    r.S[5] = mu * Bc[1][2];
+    r.alpha = muBar;
+    r.beta = -lamBar;
+    r.gam = mu;
+    r.c = 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// kernel arguments
+// ------------------------------------------------------------------------------------------------
+struct EvalArgs {
+    const int32_t *edof;    // [npe][n_el]
+    const int32_t *colpos;  // [npe*npe][n_el]
+    const int32_t *order;   // [n_el]
+    const int32_t *indptr;  // [n_dof+1]
+    const double *el_param;  // [n_el] original element order, or null
+    const double *X, *U, *dU;
+    const double *state_n;
+    double *state_t;
+    double *P, *F;
+    void *vals;
+    double *pnewdt;
+    int64_t n_el;  // SoA stride
+    int64_t p_begin, p_end;
+    MatParams mat;
+};
+
+__device__ __forceinline__ void atomic_min_pos(double *addr, double v)
+{
+    // positive doubles order like their bit patterns
+    atomicMin(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename VT>
+__device__ __forceinline__ void add_to(VT *vals, int64_t idx, double v)
+{
+    vals[idx] = vals[idx] + (VT)v;
+}
+
+// Phase 1 shared by both element families: one quadrature point. Streams over the nodes staged in LDS
+// (accessor nd(f, a) gives field f = X0..2, U0..2, dU0..2 of node a), writes per-node gradients through
+// put_g(a, j, v) / put_p(a, j, v) and returns wd = w detJ plus the recipe. For the small-strain form
+// p = g and put_p is not used.
+template <int NN, int FORM, int MAT, typename ND, typename PG, typename GG, typename PP>
+__device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd, PG put_g, GG get_g, PP put_p,
+                                         const EvalArgs &A, double Epar, int64_t pos, int q, int NQ, Recipe &r,
+                                         double &wd)
+{
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3];
+    for (int a = 0; a < NN; ++a)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[i][j] += dN[a][i] * nd(j, a);
+    const double detJ = inv3(J, Ji);
+    wd = w * detJ;
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, dH[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < NN; ++a) {
+        double g[3];
+        for (int j = 0; j < 3; ++j) g[j] = Ji[j][0] * dN[a][0] + Ji[j][1] * dN[a][1] + Ji[j][2] * dN[a][2];
+        for (int j = 0; j < 3; ++j) put_g(a, j, g[j]);
+        for (int i = 0; i < 3; ++i) {
+            const double u = nd(3 + i, a);
+            const double du = nd(6 + i, a);
+            for (int j = 0; j < 3; ++j) {
+                H[i][j] += u * g[j];
+                if (MAT == M_J2) dH[i][j] += du * g[j];
+            }
+        }
+    }
+    bool ok = true;
+    if (FORM == FORM_SS) {
+        if (MAT == M_LE) {
+            le_recipe(A.mat, Epar, H, r);
+        } else {
+            double ep[6], ep_t[6], kt;
+            const int64_t n = A.n_el;
+            for (int s = 0; s < 6; ++s) ep[s] = A.state_n[((int64_t)s * NQ + q) * n + pos];
+            const double kn = A.state_n[((int64_t)6 * NQ + q) * n + pos];
+            ok = j2_recipe(A.mat, Epar, H, dH, ep, kn, ep_t, kt, r);
+            for (int s = 0; s < 6; ++s) A.state_t[((int64_t)s * NQ + q) * n + pos] = ep_t[s];
+            A.state_t[((int64_t)6 * NQ + q) * n + pos] = kt;
+        }
+    } else {
+        double F[3][3], Fi[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
+        const double Jd = inv3(F, Fi);
+        nh_recipe(Epar, A.mat.p[1], F, Jd, r);
+        // spatial gradients NA_a = F^-T g_a
+        for (int a = 0; a < NN; ++a) {
+            const double g0 = get_g(a, 0), g1 = get_g(a, 1), g2 = get_g(a, 2);
+            for (int m = 0; m < 3; ++m) put_p(a, m, g0 * Fi[0][m] + g1 * Fi[1][m] + g2 * Fi[2][m]);
+        }
+    }
+    return ok;
+}
+
+// Number of LDS fields per quadrature point: g (3 NN) [+ p (3 NN) for TL] + alpha,beta,gam,c + N (6) + S (6)
+template <int NN, int FORM>
+struct QPLayout {
+    static constexpr int G = 0;
+    static constexpr int P = (FORM == FORM_TL) ? 3 * NN : 0;  // offset of p (aliases g for small strain)
+    static constexpr int COEF = (FORM == FORM_TL) ? 6 * NN : 3 * NN;
+    static constexpr int NV = COEF + 4;
+    static constexpr int S = NV + 6;
+    static constexpr int NF = S + 6;
+};
+
+// ------------------------------------------------------------------------------------------------
+// C3D8 / C3D8TL: 8 elements per 64-lane workgroup
+// ------------------------------------------------------------------------------------------------
+template <int FORM, int MAT, typename VT>
+__global__ __launch_bounds__(64) void k_hex8(EvalArgs A)
+{
+    constexpr int NN = 8, NQ = 8, EPB = 8;
+    using L = QPLayout<NN, FORM>;
+    __shared__ double s_nd[9 * NN * EPB];      // [field][node][le]
+    __shared__ double s_qp[L::NF * NQ * EPB];  // [field][q][le]
+    const int lane = threadIdx.x;
+    const int le = lane >> 3, t = lane & 7;
+    const int64_t pos = A.p_begin + (int64_t)blockIdx.x * EPB + le;
+    const bool active = pos < A.p_end;
+    const int64_t n = A.n_el;
+
+    // phase 0: lane (le, t) stages node t of its element
+    int32_t my_dof = 0;
+    if (active) {
+        my_dof = A.edof[(int64_t)t * n + pos];
+        for (int c = 0; c < 3; ++c) {
+            s_nd[((0 + c) * NN + t) * EPB + le] = A.X[my_dof + c];
+            s_nd[((3 + c) * NN + t) * EPB + le] = A.U[my_dof + c];
+            s_nd[((6 + c) * NN + t) * EPB + le] = A.dU[my_dof + c];
+        }
+    }
+    __syncthreads();
+
+    // phase 1: lane (le, q=t) evaluates one quadrature point
+    double Epar = A.mat.p[0];
+    if (active && A.el_param) Epar = A.el_param[A.order[pos]];
+    if (active) {
+        const int q = t;
+        auto nd = [&](int f, int a) { return s_nd[(f * NN + a) * EPB + le]; };
+        auto put_g = [&](int a, int j, double v) { s_qp[((L::G + 3 * a + j) * NQ + q) * EPB + le] = v; };
+        auto get_g = [&](int a, int j) { return s_qp[((L::G + 3 * a + j) * NQ + q) * EPB + le]; };
+        auto put_p = [&](int a, int j, double v) { s_qp[((L::P + 3 * a + j) * NQ + q) * EPB + le] = v; };
+        Recipe r;
+        double wd;
+        const bool ok = qp_phase<NN, FORM, MAT>(c_dN8[q], c_w8[q], nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd);
+        if (!ok) atomic_min_pos(A.pnewdt, 0.5);
+        auto put = [&](int f, double v) { s_qp[(f * NQ + q) * EPB + le] = v; };
+        put(L::COEF + 0, wd * r.alpha);
+        put(L::COEF + 1, wd * r.beta);
+        put(L::COEF + 2, wd * r.gam);
+        put(L::COEF + 3, wd * r.c);
+        for (int i = 0; i < 6; ++i) put(L::NV + i, (MAT == M_J2) ? r.N[i] : 0.0);
+        for (int i = 0; i < 6; ++i) put(L::S + i, wd * r.S[i]);
+    }
+    __syncthreads();
+    if (!active) return;
+
+    // phase 2: lane (le, a=t) builds blocks K_ab, b = a + j (mod 8), j < nb, and f_a
+    const int a = t;
+    const int nb = (a < 4) ? 5 : 4;
+    double K[5][3][3];
+    for (int j = 0; j < 5; ++j)
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) K[j][i][k] = 0.0;
+    double fa[3] = {0, 0, 0};
+    auto ld = [&](int f, int q) { return s_qp[(f * NQ + q) * EPB + le]; };
+    for (int q = 0; q < NQ; ++q) {
+        const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q), cc = ld(L::COEF + 3, q);
+        double pa[3], ra[3], va[3] = {0, 0, 0}, Nq[6];
+        for (int i = 0; i < 3; ++i) {
+            ra[i] = ld(L::G + 3 * a + i, q);
+            pa[i] = ld(L::P + 3 * a + i, q);
+        }
+        if (MAT == M_J2) {
+            for (int i = 0; i < 6; ++i) Nq[i] = ld(L::NV + i, q);
+            va[0] = Nq[0] * pa[0] + Nq[3] * pa[1] + Nq[4] * pa[2];
+            va[1] = Nq[3] * pa[0] + Nq[1] * pa[1] + Nq[5] * pa[2];
+            va[2] = Nq[4] * pa[0] + Nq[5] * pa[1] + Nq[2] * pa[2];
+        }
+        {  // residual f_a = S p_a
+            const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+            const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+            fa[0] += S0 * pa[0] + S3 * pa[1] + S4 * pa[2];
+            fa[1] += S3 * pa[0] + S1 * pa[1] + S5 * pa[2];
+            fa[2] += S4 * pa[0] + S5 * pa[1] + S2 * pa[2];
+        }
+        double Aa[3], Ba[3], Va[3];
+        for (int i = 0; i < 3; ++i) {
+            Aa[i] = al * pa[i];
+            Ba[i] = be * pa[i];
+            Va[i] = cc * va[i];
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            if (j < nb) {
+                const int b = (a + j) & 7;
+                double pb[3], rb[3], vb[3] = {0, 0, 0};
+                for (int i = 0; i < 3; ++i) {
+                    rb[i] = ld(L::G + 3 * b + i, q);
+                    pb[i] = ld(L::P + 3 * b + i, q);
+                }
+                if (MAT == M_J2) {
+                    vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
+                    vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
+                    vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
+                }
+                const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                for (int i = 0; i < 3; ++i)
+                    for (int k = 0; k < 3; ++k) {
+                        double v = Aa[i] * pb[k] + Ba[k] * pb[i];
+                        if (MAT == M_J2) v -= Va[i] * vb[k];
+                        if (i == k) v += d;
+                        K[j][i][k] += v;
+                    }
+            }
+        }
+    }
+
+    // scatter: P = -f, F = |f| for node a; K_ab into node a's rows and K_ab^T into node b's rows
+    for (int i = 0; i < 3; ++i) {
+        A.P[my_dof + i] -= fa[i];
+        A.F[my_dof + i] += fabs(fa[i]);
+    }
+    VT *vals = reinterpret_cast<VT *>(A.vals);
+    int32_t rs_a[3];
+    for (int i = 0; i < 3; ++i) rs_a[i] = A.indptr[my_dof + i];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        if (j < nb) {
+            const int b = (a + j) & 7;
+            const int32_t cab = A.colpos[(int64_t)(a * NN + b) * n + pos];
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)rs_a[i] + cab + k, K[j][i][k]);
+            if (j > 0) {
+                const int32_t db = A.edof[(int64_t)b * n + pos];
+                const int32_t cba = A.colpos[(int64_t)(b * NN + a) * n + pos];
+                for (int k = 0; k < 3; ++k) {
+                    const int64_t rs = A.indptr[db + k];
+                    for (int i = 0; i < 3; ++i) add_to(vals, rs + cba + i, K[j][i][k]);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// C3D20 / C3D20TL: one element per 64-lane workgroup
+// ------------------------------------------------------------------------------------------------
+template <int FORM, int MAT, typename VT>
+__global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
+{
+    constexpr int NN = 20, NQ = 27;
+    using L = QPLayout<NN, FORM>;
+    __shared__ double s_nd[9 * NN];      // [field][node]
+    __shared__ double s_qp[L::NF * NQ];  // [field][q]
+    __shared__ int32_t s_dof[NN];
+    const int lane = threadIdx.x;
+    const int64_t pos = A.p_begin + blockIdx.x;
+    if (pos >= A.p_end) return;  // uniform over the workgroup
+    const int64_t n = A.n_el;
+    if (lane < NN) {
+        const int32_t d = A.edof[(int64_t)lane * n + pos];
+        s_dof[lane] = d;
+        for (int c = 0; c < 3; ++c) {
+            s_nd[(0 + c) * NN + lane] = A.X[d + c];
+            s_nd[(3 + c) * NN + lane] = A.U[d + c];
+            s_nd[(6 + c) * NN + lane] = A.dU[d + c];
+        }
+    }
+    __syncthreads();
+    double Epar = A.mat.p[0];
+    if (A.el_param) Epar = A.el_param[A.order[pos]];
+    if (lane < NQ) {
+        const int q = lane;
+        auto nd = [&](int f, int a) { return s_nd[f * NN + a]; };
+        auto put_g = [&](int a, int j, double v) { s_qp[(L::G + 3 * a + j) * NQ + q] = v; };
+        auto get_g = [&](int a, int j) { return s_qp[(L::G + 3 * a + j) * NQ + q]; };
+        auto put_p = [&](int a, int j, double v) { s_qp[(L::P + 3 * a + j) * NQ + q] = v; };
+        Recipe r;
+        double wd;
+        const bool ok =
+            qp_phase<NN, FORM, MAT>(c_dN20[q], c_w27[q], nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd);
+        if (!ok) atomic_min_pos(A.pnewdt, 0.5);
+        auto put = [&](int f, double v) { s_qp[f * NQ + q] = v; };
+        put(L::COEF + 0, wd * r.alpha);
+        put(L::COEF + 1, wd * r.beta);
+        put(L::COEF + 2, wd * r.gam);
+        put(L::COEF + 3, wd * r.c);
+        for (int i = 0; i < 6; ++i) put(L::NV + i, (MAT == M_J2) ? r.N[i] : 0.0);
+        for (int i = 0; i < 6; ++i) put(L::S + i, wd * r.S[i]);
+    }
+    __syncthreads();
+    if (lane >= 3 * NN) return;
+    // lane = (a, i): row i of node a
+    const int a = lane / 3, i = lane % 3;
+    double K[NN][3];
+    for (int b = 0; b < NN; ++b)
+        for (int k = 0; k < 3; ++k) K[b][k] = 0.0;
+    double fi = 0.0;
+    auto ld = [&](int f, int q) { return s_qp[f * NQ + q]; };
+    for (int q = 0; q < NQ; ++q) {
+        const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q), cc = ld(L::COEF + 3, q);
+        double pa[3], ra[3], Nq[6], vai = 0.0;
+        for (int m = 0; m < 3; ++m) {
+            ra[m] = ld(L::G + 3 * a + m, q);
+            pa[m] = ld(L::P + 3 * a + m, q);
+        }
+        {
+            double Srow[3];
+            const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+            const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+            Srow[0] = (i == 0) ? S0 : (i == 1 ? S3 : S4);
+            Srow[1] = (i == 0) ? S3 : (i == 1 ? S1 : S5);
+            Srow[2] = (i == 0) ? S4 : (i == 1 ? S5 : S2);
+            fi += Srow[0] * pa[0] + Srow[1] * pa[1] + Srow[2] * pa[2];
+        }
+        if (MAT == M_J2) {
+            for (int m = 0; m < 6; ++m) Nq[m] = ld(L::NV + m, q);
+            const double Nr0 = (i == 0) ? Nq[0] : (i == 1 ? Nq[3] : Nq[4]);
+            const double Nr1 = (i == 0) ? Nq[3] : (i == 1 ? Nq[1] : Nq[5]);
+            const double Nr2 = (i == 0) ? Nq[4] : (i == 1 ? Nq[5] : Nq[2]);
+            vai = cc * (Nr0 * pa[0] + Nr1 * pa[1] + Nr2 * pa[2]);
+        }
+        const double Aai = al * pa[i];
+        double Ba[3];
+        for (int m = 0; m < 3; ++m) Ba[m] = be * pa[m];
+        for (int b = 0; b < NN; ++b) {
+            double pb[3], rb[3];
+            for (int m = 0; m < 3; ++m) {
+                rb[m] = ld(L::G + 3 * b + m, q);
+                pb[m] = ld(L::P + 3 * b + m, q);
+            }
+            const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+            double vb[3] = {0, 0, 0};
+            if (MAT == M_J2) {
+                vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
+                vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
+                vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
+            }
+            for (int k = 0; k < 3; ++k) {
+                double v = Aai * pb[k] + Ba[k] * pb[i];
+                if (MAT == M_J2) v -= vai * vb[k];
+                if (i == k) v += d;
+                K[b][k] += v;
+            }
+        }
+    }
+    const int32_t row = s_dof[a] + i;
+    A.P[row] -= fi;
+    A.F[row] += fabs(fi);
+    VT *vals = reinterpret_cast<VT *>(A.vals);
+    const int64_t rs = A.indptr[row];
+    for (int b = 0; b < NN; ++b) {
+        const int32_t cab = A.colpos[(int64_t)(a * NN + b) * n + pos];
+        for (int k = 0; k < 3; ++k) add_to(vals, rs + cab + k, K[b][k]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// small helper kernels
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T *p, int64_t n, T v)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+template <typename VT>
+__global__ void k_dirichlet_rows(const int32_t *rows, int64_t n_rows, const int32_t *indptr, const int32_t *indices,
+                                 VT *vals)
+{
+    // one wavefront per Dirichlet row
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x / 64);
+    if (r >= n_rows) return;
+    const int32_t row = rows[r];
+    for (int32_t k = indptr[row] + (threadIdx.x & 63); k < indptr[row + 1]; k += 64)
+        vals[k] = (indices[k] == row) ? (VT)1 : (VT)0;
+}
+
+template <typename T>
+static int fill(T *p, int64_t n, T v, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------------
+static int npe_of(int k) { return (k == EWFE_C3D8 || k == EWFE_C3D8TL) ? 8 : 20; }
+static int nqp_of(int k) { return (k == EWFE_C3D8 || k == EWFE_C3D8TL) ? 8 : 27; }
+static bool is_tl(int k) { return k == EWFE_C3D8TL || k == EWFE_C3D20TL; }
+
+extern "C" int ewfe_state_size(int mat_kind)
+{
+    switch (mat_kind) {
+    case EWFE_LINEAR_ELASTIC: return 0;
+    case EWFE_VON_MISES: return 7;
+    case EWFE_NEO_HOOKE_WA: return 0;
+    default: ewfe::set_error("unknown material kind"); return EWFE_EINVAL;
+    }
+}
+
+static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32_t *conn_host, int64_t n_nodes,
+                       const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out, bool upload)
+{
+    if (!out || !conn_host || !node_dof_host || n_el <= 0 || n_nodes <= 0 || n_dof <= 0) {
+        ewfe::set_error("ewfe_plan_create: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    if (elem_kind < EWFE_C3D8 || elem_kind > EWFE_C3D20TL) {
+        ewfe::set_error("ewfe_plan_create: unknown element kind");
+        return EWFE_EINVAL;
+    }
+    *out = nullptr;
+    ewfe_plan *p = new (std::nothrow) ewfe_plan();
+    if (!p) {
+        ewfe::set_error("out of host memory");
+        return EWFE_ENOMEM;
+    }
+    p->elem_kind = elem_kind;
+    p->npe = npe_of(elem_kind);
+    p->nqp = nqp_of(elem_kind);
+    p->n_el = n_el;
+    p->n_nodes = n_nodes;
+    p->n_dof = n_dof;
+    if ((int64_t)p->npe * p->npe * n_el >= ((int64_t)1 << 40)) {
+        delete p;
+        ewfe::set_error("mesh too large");
+        return EWFE_ERANGE;
+    }
+    std::vector<int32_t> edof, colpos;
+    int rc;
+    try {
+        rc = ewfe::build_plan_host(*p, conn_host, node_dof_host, edof, colpos);
+    } catch (const std::bad_alloc &) {
+        rc = EWFE_ENOMEM;
+        ewfe::set_error("out of host memory while building the plan");
+    }
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    if (!upload) {
+        if (nnz_out) *nnz_out = p->nnz;
+        *out = p;
+        return 0;
+    }
+    auto fail = [&](hipError_t e, const char *what) {
+        ewfe::set_error(std::string(what) + ": " + hipGetErrorString(e));
+        ewfe_plan_destroy(p);
+        return EWFE_EHIP;
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&p->device)) != hipSuccess) return fail(e, "hipGetDevice");
+    if ((rc = upload_tables(p->device))) {
+        ewfe_plan_destroy(p);
+        return rc;
+    }
+    std::vector<int32_t> order32(n_el);
+    for (int64_t i = 0; i < n_el; ++i) order32[i] = (int32_t)p->order[i];
+    struct Up {
+        int32_t **dst;
+        const void *src;
+        size_t bytes;
+    } ups[] = {
+        {&p->d_indptr, p->indptr.data(), sizeof(int32_t) * (n_dof + 1)},
+        {&p->d_indices, p->indices.data(), sizeof(int32_t) * p->nnz},
+        {&p->d_edof, edof.data(), sizeof(int32_t) * edof.size()},
+        {&p->d_colpos, colpos.data(), sizeof(int32_t) * colpos.size()},
+        {&p->d_order, order32.data(), sizeof(int32_t) * n_el},
+    };
+    for (auto &u : ups) {
+        if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
+        if (u.bytes && (e = hipMemcpy(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy");
+    }
+    if (nnz_out) *nnz_out = p->nnz;
+    *out = p;
+    return 0;
+}
+
+extern "C" int ewfe_plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32_t *conn_host,
+                                int64_t n_nodes, const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out)
+{
+    return plan_create(out, elem_kind, n_el, conn_host, n_nodes, node_dof_host, n_dof, nnz_out, true);
+}
+
+extern "C" int ewfe_plan_create_host(ewfe_plan **out, int elem_kind, int64_t n_el, const int32_t *conn_host,
+                                     int64_t n_nodes, const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out)
+{
+    return plan_create(out, elem_kind, n_el, conn_host, n_nodes, node_dof_host, n_dof, nnz_out, false);
+}
+
+extern "C" int ewfe_plan_pattern(const ewfe_plan *p, int32_t *indptr_host, int32_t *indices_host)
+{
+    if (!p || !indptr_host || !indices_host) {
+        ewfe::set_error("ewfe_plan_pattern: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    std::memcpy(indptr_host, p->indptr.data(), sizeof(int32_t) * (p->n_dof + 1));
+    std::memcpy(indices_host, p->indices.data(), sizeof(int32_t) * p->nnz);
+    return 0;
+}
+
+extern "C" int ewfe_plan_pattern_dev(const ewfe_plan *p, const int32_t **indptr_dev, const int32_t **indices_dev)
+{
+    if (!p || !indptr_dev || !indices_dev || !p->d_indptr) {
+        ewfe::set_error("ewfe_plan_pattern_dev: invalid arguments or plan without device data");
+        return EWFE_EINVAL;
+    }
+    *indptr_dev = p->d_indptr;
+    *indices_dev = p->d_indices;
+    return 0;
+}
+
+extern "C" int ewfe_plan_element_order(const ewfe_plan *p, int64_t *order_host, int32_t *n_colors_out)
+{
+    if (!p || !order_host) {
+        ewfe::set_error("ewfe_plan_element_order: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    std::memcpy(order_host, p->order.data(), sizeof(int64_t) * p->n_el);
+    if (n_colors_out) *n_colors_out = p->n_colors;
+    return 0;
+}
+
+extern "C" int ewfe_plan_info(const ewfe_plan *p, int64_t *n_el, int32_t *npe, int32_t *nqp, int64_t *n_dof)
+{
+    if (!p) {
+        ewfe::set_error("ewfe_plan_info: null plan");
+        return EWFE_EINVAL;
+    }
+    if (n_el) *n_el = p->n_el;
+    if (npe) *npe = p->npe;
+    if (nqp) *nqp = p->nqp;
+    if (n_dof) *n_dof = p->n_dof;
+    return 0;
+}
+
+extern "C" int ewfe_plan_destroy(ewfe_plan *p)
+{
+    if (!p) return 0;
+    int32_t *bufs[] = {p->d_indptr, p->d_indices, p->d_edof, p->d_colpos, p->d_order};
+    for (auto b : bufs)
+        if (b) (void)hipFree(b);
+    delete p;
+    return 0;
+}
+
+template <int FORM, int MAT, typename VT>
+static int launch_family(const ewfe_plan *p, EvalArgs A, hipStream_t s)
+{
+    for (int c = 0; c < p->n_colors; ++c) {
+        A.p_begin = p->color_begin[c];
+        A.p_end = p->color_begin[c + 1];
+        const int64_t cnt = A.p_end - A.p_begin;
+        if (cnt <= 0) continue;
+        if (p->npe == 8) {
+            const int64_t blocks = (cnt + 7) / 8;
+            hipLaunchKernelGGL((k_hex8<FORM, MAT, VT>), dim3((unsigned)blocks), dim3(64), 0, s, A);
+        } else {
+            hipLaunchKernelGGL((k_hex20<FORM, MAT, VT>), dim3((unsigned)cnt), dim3(64), 0, s, A);
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+template <typename VT>
+static int dispatch(const ewfe_plan *p, int mat_kind, EvalArgs A, hipStream_t s)
+{
+    const bool tl = is_tl(p->elem_kind);
+    if (!tl && mat_kind == EWFE_LINEAR_ELASTIC) return launch_family<FORM_SS, M_LE, VT>(p, A, s);
+    if (!tl && mat_kind == EWFE_VON_MISES) return launch_family<FORM_SS, M_J2, VT>(p, A, s);
+    if (tl && mat_kind == EWFE_NEO_HOOKE_WA) return launch_family<FORM_TL, M_NH, VT>(p, A, s);
+    ewfe::set_error("unsupported element/material combination (small-strain elements take linear elastic or "
+                    "von Mises, total-Lagrangian elements take Neo-Hooke)");
+    return EWFE_EINVAL;
+}
+
+extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double *mat_params_host,
+                                  const double *el_param_dev, const double *X_dev, const double *U_np_dev,
+                                  const double *dU_dev, const double *state_n_dev, double *state_t_dev, double *P_dev,
+                                  double *F_dev, void *csr_vals_dev, int vals_dtype, double *pnewdt_dev, void *stream)
+{
+    if (!p || !mat_params_host || !X_dev || !U_np_dev || !dU_dev || !P_dev || !F_dev || !csr_vals_dev || !pnewdt_dev) {
+        ewfe::set_error("ewfe_eval_assemble: null argument");
+        return EWFE_EINVAL;
+    }
+    const int ns = ewfe_state_size(mat_kind);
+    if (ns < 0) return ns;
+    if (ns > 0 && (!state_n_dev || !state_t_dev)) {
+        ewfe::set_error("ewfe_eval_assemble: material needs state_n/state_t buffers");
+        return EWFE_EINVAL;
+    }
+    if (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) {
+        ewfe::set_error("ewfe_eval_assemble: vals_dtype must be EWFE_F64 or EWFE_F32");
+        return EWFE_EINVAL;
+    }
+    if (!p->d_edof) {
+        ewfe::set_error("ewfe_eval_assemble: plan was built without device upload (ewfe_plan_create_host)");
+        return EWFE_EINVAL;
+    }
+    int dev;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev != p->device) {
+        ewfe::set_error("ewfe_eval_assemble: current device differs from the plan's device");
+        return EWFE_EINVAL;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int rc;
+    if ((rc = fill<double>(P_dev, p->n_dof, 0.0, s))) return rc;
+    if ((rc = fill<double>(F_dev, p->n_dof, 0.0, s))) return rc;
+    if ((rc = fill<double>(pnewdt_dev, 1, 1e36, s))) return rc;
+    if (vals_dtype == EWFE_F64) {
+        if ((rc = fill<double>(reinterpret_cast<double *>(csr_vals_dev), p->nnz, 0.0, s))) return rc;
+    } else {
+        if ((rc = fill<float>(reinterpret_cast<float *>(csr_vals_dev), p->nnz, 0.0f, s))) return rc;
+    }
+    EvalArgs A;
+    A.edof = p->d_edof;
+    A.colpos = p->d_colpos;
+    A.order = p->d_order;
+    A.indptr = p->d_indptr;
+    A.el_param = el_param_dev;
+    A.X = X_dev;
+    A.U = U_np_dev;
+    A.dU = dU_dev;
+    A.state_n = state_n_dev;
+    A.state_t = state_t_dev;
+    A.P = P_dev;
+    A.F = F_dev;
+    A.vals = csr_vals_dev;
+    A.pnewdt = pnewdt_dev;
+    A.n_el = p->n_el;
+    A.p_begin = A.p_end = 0;
+    for (int i = 0; i < 8; ++i) A.mat.p[i] = mat_params_host[i];
+    return vals_dtype == EWFE_F64 ? dispatch<double>(p, mat_kind, A, s) : dispatch<float>(p, mat_kind, A, s);
+}
+
+extern "C" int ewfe_dirichlet_rows(const ewfe_plan *p, const int32_t *rows_dev, int64_t n_rows, void *csr_vals_dev,
+                                   int vals_dtype, void *stream)
+{
+    if (!p || (n_rows > 0 && !rows_dev) || !csr_vals_dev) {
+        ewfe::set_error("ewfe_dirichlet_rows: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    if (n_rows <= 0) return 0;
+    if (!p->d_indptr) {
+        ewfe::set_error("ewfe_dirichlet_rows: plan has no device data");
+        return EWFE_EINVAL;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int64_t blocks = (n_rows + 3) / 4;
+    if (vals_dtype == EWFE_F64)
+        hipLaunchKernelGGL(k_dirichlet_rows<double>, dim3((unsigned)blocks), dim3(256), 0, s, rows_dev, n_rows,
+                           p->d_indptr, p->d_indices, reinterpret_cast<double *>(csr_vals_dev));
+    else
+        hipLaunchKernelGGL(k_dirichlet_rows<float>, dim3((unsigned)blocks), dim3(256), 0, s, rows_dev, n_rows,
+                           p->d_indptr, p->d_indices, reinterpret_cast<float *>(csr_vals_dev));
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}

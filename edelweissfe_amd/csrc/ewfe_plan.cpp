@@ -1,0 +1,206 @@
+// Host-side plan construction for libewfe.so.
+//
+// Replaces, for the element-assembly path, the reference's per-step set-up:
+//   * DofManager's VIJ pattern (edelweissfe/numerics/dofmanager.py:522-557): every element couples all
+//     of its dofs with each other, Ke stored column-major in the VIJ slice;
+//   * CSRGenerator.__init__ (edelweissfe/numerics/csrgenerator.pyx:47-98): canonical CSR of that COO
+//     pattern (rows = dofs, sorted unique columns, explicit zeros kept, int32 indices) and, per VIJ
+//     entry, the CSR slot it accumulates into.
+// Instead of a slot per VIJ entry (nDof_el^2 per element) the plan keeps, per element and node pair
+// (a, b), the offset of node b's 3 columns inside node a's rows: the 3 rows of one node have the same
+// columns, and a node's 3 dofs are consecutive, so slot(a,i ; b,k) = indptr[dof(a)+i] + colpos(a,b) + k.
+// It also colours the elements (no two elements of one colour share a node) so that the kernels can
+// accumulate into CSR without atomics and in a fixed order.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <thread>
+
+#include "ewfe_internal.h"
+
+namespace ewfe {
+
+template <typename F>
+static void parallel_for(int64_t n, F &&f)
+{
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > 16) nt = 16;
+    if (n < 4096 || nt == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([&f, b, e] { f(b, e); });
+    }
+    for (auto &t : th) t.join();
+}
+
+int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,
+                    std::vector<int32_t> &colpos_host)
+{
+    const int npe = plan.npe;
+    const int64_t n_el = plan.n_el, n_nodes = plan.n_nodes, n_dof = plan.n_dof;
+
+    // --- validate topology and the dof layout --------------------------------------------------
+    for (int64_t e = 0; e < n_el; ++e)
+        for (int a = 0; a < npe; ++a) {
+            const int64_t nd = conn[e * npe + a];
+            if (nd < 0 || nd >= n_nodes) {
+                set_error("element " + std::to_string(e) + " references node " + std::to_string(nd) +
+                          " outside [0, n_nodes)");
+                return -1;
+            }
+            const int64_t d = node_dof[nd];
+            if (d < 0 || d + 3 > n_dof) {
+                set_error("node " + std::to_string(nd) + " of element " + std::to_string(e) +
+                          " has no valid displacement dofs");
+                return -1;
+            }
+        }
+    if (n_dof >= (int64_t)INT32_MAX) {
+        set_error("n_dof beyond int32 indexing");
+        return -4;
+    }
+    std::vector<int32_t> node_of_dof(n_dof, -1);
+    for (int64_t n = 0; n < n_nodes; ++n) {
+        const int64_t d = node_dof[n];
+        if (d < 0) continue;
+        if (d + 3 > n_dof) {
+            set_error("node dof range beyond n_dof");
+            return -1;
+        }
+        for (int c = 0; c < 3; ++c) {
+            if (node_of_dof[d + c] != -1) {
+                set_error("two nodes share dof " + std::to_string(d + c));
+                return -1;
+            }
+            node_of_dof[d + c] = (int32_t)n;
+        }
+    }
+
+    // --- node -> element incidence ---------------------------------------------------------------
+    std::vector<int64_t> inc_ptr(n_nodes + 1, 0);
+    for (int64_t i = 0; i < n_el * npe; ++i) inc_ptr[conn[i] + 1]++;
+    for (int64_t n = 0; n < n_nodes; ++n) inc_ptr[n + 1] += inc_ptr[n];
+    std::vector<int64_t> inc(n_el * npe);
+    {
+        std::vector<int64_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (int64_t e = 0; e < n_el; ++e)
+            for (int a = 0; a < npe; ++a) inc[fill[conn[e * npe + a]]++] = e;
+    }
+
+    // --- node neighbour lists, sorted by first dof (= column order) ------------------------------
+    std::vector<int32_t> deg(n_nodes, 0);
+    parallel_for(n_nodes, [&](int64_t b, int64_t e) {
+        std::vector<int64_t> tmp;
+        for (int64_t n = b; n < e; ++n) {
+            tmp.clear();
+            for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
+                const int64_t el = inc[k];
+                for (int a = 0; a < npe; ++a) tmp.push_back(node_dof[conn[el * npe + a]]);
+            }
+            std::sort(tmp.begin(), tmp.end());
+            deg[n] = (int32_t)(std::unique(tmp.begin(), tmp.end()) - tmp.begin());
+        }
+    });
+    std::vector<int64_t> nb_ptr(n_nodes + 1, 0);
+    for (int64_t n = 0; n < n_nodes; ++n) nb_ptr[n + 1] = nb_ptr[n] + deg[n];
+    std::vector<int32_t> nb_dof(nb_ptr[n_nodes]);  // first dof of each neighbour, ascending
+    parallel_for(n_nodes, [&](int64_t b, int64_t e) {
+        std::vector<int64_t> tmp;
+        for (int64_t n = b; n < e; ++n) {
+            tmp.clear();
+            for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
+                const int64_t el = inc[k];
+                for (int a = 0; a < npe; ++a) tmp.push_back(node_dof[conn[el * npe + a]]);
+            }
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            for (size_t i = 0; i < tmp.size(); ++i) nb_dof[nb_ptr[n] + i] = (int32_t)tmp[i];
+        }
+    });
+
+    // --- CSR pattern over dof rows ---------------------------------------------------------------
+    plan.indptr.assign(n_dof + 1, 0);
+    int64_t acc = 0;
+    for (int64_t d = 0; d < n_dof; ++d) {
+        plan.indptr[d] = (int32_t)acc;
+        const int32_t n = node_of_dof[d];
+        acc += (n >= 0) ? 3 * (int64_t)deg[n] : 0;
+        if (acc >= (int64_t)INT32_MAX) {
+            set_error("nnz beyond int32 CSR indexing");
+            return -4;
+        }
+    }
+    plan.indptr[n_dof] = (int32_t)acc;
+    plan.nnz = acc;
+    plan.indices.resize(acc);
+    parallel_for(n_dof, [&](int64_t b, int64_t e) {
+        for (int64_t d = b; d < e; ++d) {
+            const int32_t n = node_of_dof[d];
+            if (n < 0) continue;
+            int32_t *out = plan.indices.data() + plan.indptr[d];
+            for (int64_t k = nb_ptr[n]; k < nb_ptr[n + 1]; ++k)
+                for (int c = 0; c < 3; ++c) *out++ = nb_dof[k] + c;
+        }
+    });
+
+    // --- greedy element colouring (elements of one colour share no node) -------------------------
+    std::vector<uint64_t> node_mask(n_nodes, 0);
+    std::vector<int8_t> color(n_el);
+    int n_colors = 0;
+    for (int64_t e = 0; e < n_el; ++e) {
+        uint64_t used = 0;
+        for (int a = 0; a < npe; ++a) used |= node_mask[conn[e * npe + a]];
+        if (used == ~0ull) {
+            set_error("element colouring needs more than 64 colours");
+            return -1;
+        }
+        const int c = __builtin_ctzll(~used);
+        color[e] = (int8_t)c;
+        n_colors = std::max(n_colors, c + 1);
+        for (int a = 0; a < npe; ++a) node_mask[conn[e * npe + a]] |= (1ull << c);
+    }
+    plan.n_colors = n_colors;
+    plan.color_begin.assign(n_colors + 1, 0);
+    for (int64_t e = 0; e < n_el; ++e) plan.color_begin[color[e] + 1]++;
+    for (int c = 0; c < n_colors; ++c) plan.color_begin[c + 1] += plan.color_begin[c];
+    plan.order.resize(n_el);
+    {
+        std::vector<int64_t> fill(plan.color_begin.begin(), plan.color_begin.end() - 1);
+        for (int64_t e = 0; e < n_el; ++e) plan.order[fill[color[e]]++] = e;
+    }
+
+    // --- SoA device images: element dof bases and the node-pair slot map -------------------------
+    edof_host.resize((size_t)npe * n_el);
+    colpos_host.resize((size_t)npe * npe * n_el);
+    std::atomic<int> bad{0};
+    parallel_for(n_el, [&](int64_t b, int64_t e) {
+        for (int64_t p = b; p < e; ++p) {
+            const int64_t el = plan.order[p];
+            for (int a = 0; a < npe; ++a) {
+                const int32_t na = conn[el * npe + a];
+                edof_host[(size_t)a * n_el + p] = (int32_t)node_dof[na];
+                const int32_t *lo = nb_dof.data() + nb_ptr[na], *hi = nb_dof.data() + nb_ptr[na + 1];
+                for (int bb = 0; bb < npe; ++bb) {
+                    const int32_t db = (int32_t)node_dof[conn[el * npe + bb]];
+                    const int32_t *it = std::lower_bound(lo, hi, db);
+                    if (it == hi || *it != db) bad = 1;
+                    colpos_host[((size_t)a * npe + bb) * n_el + p] = (int32_t)(3 * (it - lo));
+                }
+            }
+        }
+    });
+    if (bad) {
+        set_error("internal error: slot map lookup failed");
+        return -1;
+    }
+    return 0;
+}
+
+}  // namespace ewfe

@@ -1,0 +1,237 @@
+"""Model tree for the GPU path: nodes, node sets, element blocks with their material and quadrature-point
+state, and the boxGen structured-mesh generator.
+
+Mirrors the parts of the reference's model that the element-assembly path reads:
+  * FEModel (edelweissfe/models/femodel.py:45-297): nodes in insertion order, nodeSets, elements,
+    ``advanceToTime`` -> every element accepts its last state (femodel.py:245-258,
+    marmotelement/element.pyx:278-281);
+  * boxGen (edelweissfe/generators/boxgen.py:91-448): node order ix -> iy -> iz, C3D8 / C3D20
+    connectivity in Abaqus order, the face / edge / vertex node sets and element sets;
+  * sections with per-element material parameters (edelweissfe/sections/base/sectionbase.py:86-128).
+
+Elements are held as blocks (one connectivity array per element type + material) rather than one Python
+object per element, so a 2-million-element mesh is built in seconds; ``ElementBlock`` carries the
+per-element API the solver needs (nNodes, nDof, fields, acceptLastState).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import ELEMENT_KINDS, MATERIAL_KINDS
+
+ELEMENT_LIBRARY = {
+    # nNodes, nInt (quadrature points), formulation (edelweissfe/elements/library.py:212-227, 260-275, 453-516)
+    "C3D8": dict(nNodes=8, nInt=8, tl=False),
+    "C3D20": dict(nNodes=20, nInt=27, tl=False),
+    "C3D8TL": dict(nNodes=8, nInt=8, tl=True),
+    "C3D20TL": dict(nNodes=20, nInt=27, tl=True),
+}
+
+MATERIAL_PARAMETERS = {
+    # name -> names of the material properties (the reference material classes' constructors)
+    "linearelastic": ("E", "nu"),  # linearelastic.py
+    "vonmises": ("E", "nu", "fy0", "HLin", "dfy", "delta"),  # vonmises.py:105-124
+    "neohookeanwa": ("mu", "K"),  # neohookepencegouformulationa.py:72-78
+}
+
+
+class Material:
+    """A material definition: name + property vector (the reference's ``*material, name=..`` block)."""
+
+    def __init__(self, name: str, materialProperties):
+        key = name.lower()
+        if key not in MATERIAL_PARAMETERS:
+            raise AttributeError(f"material {name} is not available on the GPU path "
+                                 f"(available: {', '.join(MATERIAL_PARAMETERS)})")
+        props = np.asarray(materialProperties, dtype=np.float64).ravel()
+        need = len(MATERIAL_PARAMETERS[key])
+        if key == "vonmises" and len(props) == 4:  # linear hardening given without the nonlinear terms
+            props = np.concatenate([props, [0.0, 0.0]])
+        if len(props) < need:
+            raise ValueError(f"material {name} needs {need} properties {MATERIAL_PARAMETERS[key]}, got {len(props)}")
+        self.name = key
+        self.materialProperties = props
+        self.kind = MATERIAL_KINDS[key]
+
+
+class ElementBlock:
+    """All elements of one type sharing one material (a section applied to an element set).
+
+    conn      int32 [nEl, nNodes]   model node indices, element-local (Abaqus) node order
+    labels    int64 [nEl]           element labels
+    elParams  float64 [nEl] or None per-element value of the material's first property
+              (section ``materialParameterFromField``, sectionbase.py:86-128)
+    """
+
+    def __init__(self, elType: str, conn, labels=None, material: Material | None = None, elParams=None):
+        et = elType.upper()
+        if et not in ELEMENT_LIBRARY:
+            raise AttributeError(f"element type {elType} is not available on the GPU path")
+        self.elType = et
+        self.kind = ELEMENT_KINDS[et]
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32)
+        lib = ELEMENT_LIBRARY[et]
+        if self.conn.ndim != 2 or self.conn.shape[1] != lib["nNodes"]:
+            raise ValueError(f"{et} connectivity must have shape [nEl, {lib['nNodes']}]")
+        self.nNodes = lib["nNodes"]
+        self.nDof = 3 * self.nNodes
+        self.nInt = lib["nInt"]
+        self.labels = np.arange(1, len(self.conn) + 1) if labels is None else np.asarray(labels)
+        self.material = material
+        self.elParams = None if elParams is None else np.ascontiguousarray(elParams, dtype=np.float64)
+        self.fields = [["displacement"]] * self.nNodes
+        # quadrature-point state (device tensors, plan order), owned here like the reference's elements own
+        # theirs; created by the solver's assembly on first use
+        self.stateVars = None
+        self.stateVarsTemp = None
+        self.planOrder = None
+
+    def __len__(self):
+        return len(self.conn)
+
+    @property
+    def hasMaterial(self):
+        return self.material is not None
+
+    def setMaterial(self, material: Material):
+        if self.elType.endswith("TL") != (material.name == "neohookeanwa"):
+            raise AttributeError(f"{self.elType} cannot take material {material.name}: small-strain elements take "
+                                 "linearelastic / vonmises, total-Lagrangian elements take neohookeanwa")
+        self.material = material
+
+    def acceptLastState(self):
+        """state_n <- state_trial (marmotelement/element.pyx:278-281)."""
+        if self.stateVarsTemp is not None:
+            self.stateVars.copy_(self.stateVarsTemp)
+
+
+class FEModel:
+    """Nodes (insertion order), node sets, element blocks, element sets, time."""
+
+    def __init__(self, domainSize: int = 3):
+        if domainSize != 3:
+            raise NotImplementedError("the GPU path covers 3D solid elements only")
+        self.domainSize = 3
+        self.nodeCoordinates = np.zeros((0, 3))
+        self.nodeLabels = np.zeros(0, dtype=np.int64)
+        self.nodeSets: dict[str, np.ndarray] = {}
+        self.elementBlocks: list[ElementBlock] = []
+        self.elementSets: dict[str, tuple[int, np.ndarray]] = {}  # name -> (block index, element indices)
+        self.time = 0.0
+        self.constraints = {}
+        self.scalarVariables = {}
+
+    def addNodes(self, coords, labels=None):
+        coords = np.asarray(coords, dtype=np.float64).reshape(-1, 3)
+        start = len(self.nodeCoordinates)
+        if labels is None:
+            first = int(self.nodeLabels.max()) + 1 if len(self.nodeLabels) else 1
+            labels = np.arange(first, first + len(coords))
+        self.nodeCoordinates = np.concatenate([self.nodeCoordinates, coords])
+        self.nodeLabels = np.concatenate([self.nodeLabels, np.asarray(labels, dtype=np.int64)])
+        return np.arange(start, start + len(coords))
+
+    def addElementBlock(self, block: ElementBlock, setName: str | None = None):
+        self.elementBlocks.append(block)
+        if setName:
+            self.elementSets[setName] = (len(self.elementBlocks) - 1, np.arange(len(block)))
+        return block
+
+    @property
+    def nNodes(self):
+        return len(self.nodeCoordinates)
+
+    def advanceToTime(self, time: float):
+        """femodel.py:245-258."""
+        self.time = time
+        for b in self.elementBlocks:
+            b.acceptLastState()
+
+
+# --------------------------------------------------------------------------------------------------
+# boxGen
+# --------------------------------------------------------------------------------------------------
+def boxGenArrays(nX, nY, nZ, lX=1.0, lY=1.0, lZ=1.0, x0=0.0, y0=0.0, z0=0.0, elType="C3D8", name="gen"):
+    """Vectorised boxGen (boxgen.py:125-371). Returns (coords [nNodes,3], conn [nEl,npe] int32, nodeSets,
+    gridIndex [nNodes,3] int (ix, iy, iz) on the generator's node grid)."""
+    npe = ELEMENT_LIBRARY[elType.upper()]["nNodes"]
+    if npe == 8:
+        nx, ny, nz = nX + 1, nY + 1, nZ + 1
+    else:
+        nx, ny, nz = 2 * nX + 1, 2 * nY + 1, 2 * nZ + 1
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    if npe == 8:
+        present = np.ones((nx, ny, nz), dtype=bool)
+    else:  # C3D20 meshes skip nodes with two or more odd grid indices (boxgen.py:139)
+        present = ((ix % 2) + (iy % 2) + (iz % 2)) < 2
+    g2m = np.full((nx, ny, nz), -1, dtype=np.int64)
+    g2m[present] = np.arange(int(present.sum()))
+    xs, ys, zs = np.linspace(x0, x0 + lX, nx), np.linspace(y0, y0 + lY, ny), np.linspace(z0, z0 + lZ, nz)
+    grid = np.stack([ix[present], iy[present], iz[present]], axis=1)
+    coords = np.stack([xs[grid[:, 0]], ys[grid[:, 1]], zs[grid[:, 2]]], axis=1)
+
+    ex, ey, ez = (a.ravel() for a in np.meshgrid(np.arange(nX), np.arange(nY), np.arange(nZ), indexing="ij"))
+    if npe == 8:
+        offs = [(0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0), (0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0)]
+        base = (ex, ey, ez)
+    else:
+        offs = [(0, 0, 0), (0, 0, 2), (2, 0, 2), (2, 0, 0), (0, 2, 0), (0, 2, 2), (2, 2, 2), (2, 2, 0),
+                (0, 0, 1), (1, 0, 2), (2, 0, 1), (1, 0, 0), (0, 2, 1), (1, 2, 2), (2, 2, 1), (1, 2, 0),
+                (0, 1, 0), (0, 1, 2), (2, 1, 2), (2, 1, 0)]
+        base = (2 * ex, 2 * ey, 2 * ez)
+    conn = np.stack([g2m[base[0] + a, base[1] + b, base[2] + c] for a, b, c in offs], axis=1).astype(np.int32)
+
+    def filt(sl):
+        sub = g2m[sl]
+        return sub[sub >= 0].ravel()
+
+    sets = {}
+    for nm, sl in [("top", np.s_[:, -1, :]), ("bottom", np.s_[:, 0, :]), ("right", np.s_[-1, :, :]),
+                   ("left", np.s_[0, :, :]), ("front", np.s_[:, :, -1]), ("back", np.s_[:, :, 0]),
+                   ("bottomRight", np.s_[-1, 0, :]), ("bottomLeft", np.s_[0, 0, :]),
+                   ("bottomFront", np.s_[:, 0, -1]), ("bottomBack", np.s_[:, 0, 0]),
+                   ("topRight", np.s_[-1, -1, :]), ("topLeft", np.s_[0, -1, :]), ("topFront", np.s_[:, -1, -1]),
+                   ("topBack", np.s_[:, -1, 0]), ("rightBack", np.s_[-1, :, 0]), ("rightFront", np.s_[-1, :, -1]),
+                   ("leftBack", np.s_[0, :, 0]), ("leftFront", np.s_[0, :, -1]),
+                   ("centerX", np.s_[nx // 2, :, :]), ("centerY", np.s_[:, ny // 2, :]),
+                   ("centerZ", np.s_[:, :, nz // 2])]:
+        sets[f"{name}_{nm}"] = filt(sl)
+    for nm, (i, j, k) in [("bottomRightFront", (-1, 0, -1)), ("bottomRightBack", (-1, 0, 0)),
+                          ("bottomLeftFront", (0, 0, -1)), ("bottomLeftBack", (0, 0, 0)),
+                          ("topRightFront", (-1, -1, -1)), ("topRightBack", (-1, -1, 0)),
+                          ("topLeftFront", (0, -1, -1)), ("topLeftBack", (0, -1, 0))]:
+        sets[f"{name}_{nm}"] = np.array([g2m[i, j, k]])
+    return coords, conn, sets, grid
+
+
+def jitterNodes(coords, grid, gridShape, h, amplitude, seed):
+    """Moves every node by U(-amplitude*h, +amplitude*h) per component, except along the normal of a
+    boundary face it lies on, so loaded and fixed faces stay planar. Seeded numpy.random.default_rng."""
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-1.0, 1.0, size=coords.shape) * (amplitude * np.asarray(h))[None, :]
+    for c in range(3):
+        onFace = (grid[:, c] == 0) | (grid[:, c] == gridShape[c] - 1)
+        d[onFace, c] = 0.0
+    return coords + d
+
+
+def generateModelData(generatorDefinition: dict, model: FEModel, journal=None) -> FEModel:
+    """The boxGen generator plugin (boxgen.py:91) for this package: options nX, nY, nZ, lX, lY, lZ, x0, y0,
+    z0, elType; adds nodes, one element block (material assigned later by a section) and the node /
+    element sets ``<name>_<face|edge|vertex>``, ``<name>_all``."""
+    o = {k: v for k, v in generatorDefinition.get("data", {}).items()}
+    name = generatorDefinition.get("name", "boxGen")
+    coords, conn, sets, _ = boxGenArrays(int(o.get("nX", 1)), int(o.get("nY", 1)), int(o.get("nZ", 1)),
+                                         float(o.get("lX", 1.0)), float(o.get("lY", 1.0)), float(o.get("lZ", 1.0)),
+                                         float(o.get("x0", 0.0)), float(o.get("y0", 0.0)), float(o.get("z0", 0.0)),
+                                         o["elType"], name)
+    first = model.nNodes
+    model.addNodes(coords)
+    block = ElementBlock(o["elType"], conn + first)
+    model.addElementBlock(block, f"{name}_all")
+    for k, v in sets.items():
+        model.nodeSets[k] = v + first
+    model.nodeSets.setdefault("all", np.arange(model.nNodes))
+    model.nodeSets["all"] = np.arange(model.nNodes)
+    return model

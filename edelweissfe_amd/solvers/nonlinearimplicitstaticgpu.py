@@ -1,0 +1,356 @@
+"""NISTGPU: the nonlinear implicit static Newton solver with element evaluation, CSR assembly, Dirichlet
+rows and the linear solve on an MI355X.
+
+Method for method the same contract as the reference's NIST solver
+(edelweissfe/solvers/nonlinearimplicitstatic.py:60-1084):
+
+  solveStep(step, model, fieldOutputController, outputmanagers)                    :110-332
+  solveIncrement(U_n, dU, P, K, stepActions, model, timeStep, prevTimeStep,
+                 extrapolation, maxIter, maxGrowingIter)                          :334-458
+  computeElements(elements, U_np, dU, P, K, F, timeStep) -> (P, K, F)              :794-849
+  assembleStiffnessCSR(K)                                                          :753-769
+  applyDirichletK(K, dirichlets) / applyDirichlet(timeStep, R, dirichlets)         :559-623
+  checkConvergence(R, ddU, F, iterationCounter, residualHistory)                   :625-725
+  linearSolve(A, b)                                                                :727-751
+  computeSpatialAveragedFluxes(F)                                                  :771-792
+  extrapolateLastIncrement(...) / checkDivergingSolution(...)                       :947-1006
+  findDirichletIndices(dirichlet)                                                  :1077-1084
+
+Differences that are part of the design (not of the semantics):
+  * dof vectors are float64 device tensors; "K" is the CSR value array of the plan's pattern (the VIJ
+    array is never materialised), filled by computeElements in the same launch as P and F, so the
+    "CSR generation" phase is fused into "elements" and its timer stays 0;
+  * the linear solve is device PCG on the row-eliminated matrix (option ``linsolver="pcg"``), converged to
+    ``pcgTolerance`` relative residual; the Newton loop's own flux/correction tolerances decide convergence
+    exactly as in the reference;
+  * ``tangentPrecision="fp32"`` builds and stores the tangent in FP32 (residual, return mapping and update
+    stay FP64): Newton then converges to the same FP64 residual tolerance, possibly in more iterations;
+  * the 1e-7 absolute floor of the flux test is the option ``fluxResidualFloor`` (default 1e-7).
+"""
+
+from __future__ import annotations
+
+import json
+import time as _time
+
+import numpy as np
+
+from ..assembly import ElementAssembly
+from ..dofmanager import DofManager
+from ..stepping import (ConditionalStop, CutbackRequest, DivergingSolution, ReachedMaxIncrements,
+                        ReachedMaxIterations, ReachedMinIncrementSize, StepFailed)
+
+TIMING_KEYS = ["elements", "constraints", "distributed loads", "body forces", "dirichlet R", "CSR generation",
+               "dirichlet K", "linear solve", "convergence check", "output"]  # edelweissfe/config/timing.py:31-46
+
+DEFAULT_JOB_INFO = {  # edelweissfe/config/phenomena.py:59-93, displacement field
+    "fieldCorrectionTolerance": {"displacement": 1e-8},
+    "fluxResidualTolerance": {"displacement": 1e-8},
+    "fluxResidualToleranceAlternative": {"displacement": 5e-3},
+}
+
+
+class _Journal:
+    def __init__(self, verbose=False):
+        self.verbose = verbose
+        self.lines = []
+
+    def message(self, msg, ident="", level=0):
+        self.lines.append(msg)
+        if self.verbose:
+            print(f"{ident:>20}: {msg}", flush=True)
+
+    def errorMessage(self, msg, ident=""):
+        self.message("ERROR " + msg, ident)
+
+
+class NISTGPU:
+    identification = "NISTGPUSolver"
+
+    NISTOptions = {
+        "defaultMaxIter": 10,
+        "defaultCriticalIter": 5,
+        "defaultMaxGrowingIter": 10,
+        "extrapolation": "linear",
+        "linsolver": "pcg",
+        "linsolverConfigFile": "",
+        "tangentPrecision": "fp64",
+        "pcgTolerance": 1e-10,
+        "pcgMaxIter": 50000,
+        "fluxResidualFloor": 1e-7,
+        "device": "cuda",
+    }
+
+    def __init__(self, jobInfo=None, journal=None, **kwargs):
+        jobInfo = jobInfo or DEFAULT_JOB_INFO
+        self.journal = journal or _Journal()
+        self.fieldCorrectionTolerances = jobInfo["fieldCorrectionTolerance"]
+        self.fluxResidualTolerances = jobInfo["fluxResidualTolerance"]
+        self.fluxResidualTolerancesAlt = jobInfo["fluxResidualToleranceAlternative"]
+        self.options = self.NISTOptions.copy()
+        self._updateOptions(kwargs, self.journal)
+        self.assemblies = {}
+        self.history = []  # per converged increment: list of (|R|inf, |ddU|inf) per iteration
+        self.linearSolveIterations = []
+
+    def _updateOptions(self, updatedOptions, journal):
+        for k, v in updatedOptions.items():
+            if k in self.NISTOptions:
+                self.options[k] = type(self.NISTOptions[k])(v)
+            else:
+                raise AttributeError("Invalid option {:} for {:}".format(k, self.identification))
+
+    # ------------------------------------------------------------------------------------------------
+    def _sync(self):
+        self.torch.cuda.synchronize()
+
+    def solveStep(self, step, model, fieldOutputController=None, outputmanagers=()):
+        import torch
+
+        self.torch = torch
+        try:
+            self._updateOptions(step.actions["options"]["NISTSolver"].options, self.journal)
+        except KeyError:
+            pass
+        if self.options["linsolverConfigFile"]:
+            with open(self.options["linsolverConfigFile"]) as f:
+                cfg = json.load(f)
+            self._updateOptions({k: v for k, v in cfg.items() if k in self.NISTOptions}, self.journal)
+        if self.options["linsolver"].lower() != "pcg":
+            raise AttributeError("invalid linear solver {:} requested".format(self.options["linsolver"]))
+        if len(model.elementBlocks) != 1:
+            raise NotImplementedError("NISTGPU assembles one element block (one element type and section) per model")
+        device = self.options["device"]
+        self.theDofManager = DofManager(model, device=device)
+        self.journal.message("total size of eq. system: {:}".format(self.theDofManager.nDof), self.identification)
+        block = model.elementBlocks[0]
+        if not block.hasMaterial:
+            raise Exception("No material was assigned to the element block")
+        if id(block) not in self.assemblies or self.assemblies[id(block)].nDof != self.theDofManager.nDof:
+            self.assemblies[id(block)] = ElementAssembly(block, self.theDofManager, model, device=device)
+        self.assembly = self.assemblies[id(block)]
+        self.computationTimes = dict.fromkeys(TIMING_KEYS, 0.0)
+        K = self.assembly.newValues("fp32" if self.options["tangentPrecision"] == "fp32" else "fp64")
+
+        extrapolation = self.options["extrapolation"]
+        maxIter, criticalIter = step.maxIter, step.criticalIter
+        maxGrowingIter, cutbackFactor = step.maxGrowIter, step.cutbackFactor
+
+        U = getattr(model, "U", None)
+        U = self.theDofManager.constructDofVector() if U is None else U.clone()
+        P = self.theDofManager.constructDofVector()
+        dU = self.theDofManager.constructDofVector()
+        self._dirichletRowsCache = {}
+        prevTimeStep = None
+        self.applyStepActionsAtStepStart(model, step.actions)
+        try:
+            for timeStep in step.getTimeStep():
+                try:
+                    U, dU, P, iterationCounter, incrementResidualHistory = self.solveIncrement(
+                        U, dU, P, K, step.actions, model, timeStep, prevTimeStep, extrapolation, maxIter,
+                        maxGrowingIter)
+                except CutbackRequest as e:
+                    self.journal.message(str(e), self.identification, 1)
+                    step.discardAndChangeIncrement(max(e.cutbackSize, cutbackFactor))
+                    prevTimeStep = None
+                except (ReachedMaxIterations, DivergingSolution) as e:
+                    self.journal.message(str(e), self.identification, 1)
+                    step.discardAndChangeIncrement(cutbackFactor)
+                    prevTimeStep = None
+                else:
+                    prevTimeStep = timeStep
+                    if iterationCounter >= criticalIter:
+                        step.preventIncrementIncrease()
+                    model.U, model.P, model.dU = U, P, dU
+                    model.advanceToTime(timeStep.totalTime)
+                    self.history.append(list(self._currentHistory))
+                    self.journal.message("Converged in {:} iteration(s)".format(iterationCounter),
+                                         self.identification, 1)
+        except (ReachedMaxIncrements, ReachedMinIncrementSize):
+            self.journal.errorMessage("Incrementation failed", self.identification)
+            raise StepFailed()
+        except ConditionalStop:
+            self.applyStepActionsAtStepEnd(model, step.actions)
+        else:
+            self.applyStepActionsAtStepEnd(model, step.actions)
+        return True, model
+
+    def solveIncrement(self, U_n, dU, P, K, stepActions, model, timeStep, prevTimeStep, extrapolation, maxIter,
+                       maxGrowingIter):
+        iterationCounter = 0
+        incrementResidualHistory = dict.fromkeys(self.theDofManager.idcsOfFieldsInDofVector, (0.0, 0))
+        self._currentHistory = []
+        dm = self.theDofManager
+        R, F, PExt, U_np = (dm.constructDofVector() for _ in range(4))
+        ddU = None
+        dirichlets = list(stepActions["dirichlet"].values())
+        nodeforces = list(stepActions["nodeforces"].values())
+        if stepActions.get("distributedload") or stepActions.get("bodyforce"):
+            raise NotImplementedError("distributed loads and body forces are not on the GPU path yet")
+        self.applyStepActionsAtIncrementStart(model, timeStep, stepActions)
+        dU, isExtrapolatedIncrement = self.extrapolateLastIncrement(extrapolation, timeStep, dU, dirichlets,
+                                                                    prevTimeStep, model)
+        elements = model.elementBlocks
+        while True:
+            U_np.copy_(U_n)
+            U_np += dU
+            PExt.zero_()
+            P, K, F = self.computeElements(elements, U_np, dU, P, K, F, timeStep)
+            PExt, K = self.assembleLoads(nodeforces, [], [], U_np, PExt, K, timeStep)
+            R.copy_(P)
+            R += PExt
+            if iterationCounter == 0 and not isExtrapolatedIncrement and dirichlets:
+                R = self.applyDirichlet(timeStep, R, dirichlets)
+            else:
+                for dirichlet in dirichlets:
+                    R[self._rows(dirichlet)] = 0.0
+                converged, _ = self.checkConvergence(R, ddU, F, iterationCounter, incrementResidualHistory)
+                if converged:
+                    break
+                if self.checkDivergingSolution(incrementResidualHistory, maxGrowingIter):
+                    raise DivergingSolution("Residual grew {:} times, cutting back".format(maxGrowingIter))
+                if iterationCounter == maxIter:
+                    raise ReachedMaxIterations("Reached max. iterations in current increment, cutting back")
+            K_ = self.assembleStiffnessCSR(K)
+            K_ = self.applyDirichletK(K_, dirichlets)
+            ddU = self.linearSolve(K_, R)
+            dU += ddU
+            iterationCounter += 1
+        return U_np, dU, P, iterationCounter, incrementResidualHistory
+
+    # ------------------------------------------------------------------------------------------------
+    def computeElements(self, elements, U_np, dU, P, K, F, timeStep):
+        tic = _time.perf_counter()
+        pnewdt = self.assembly.evaluate(U_np, dU, P, F, K)
+        pdt = float(pnewdt.item())  # synchronises
+        self.computationTimes["elements"] += _time.perf_counter() - tic
+        if pdt < 1.0:
+            raise CutbackRequest("An element requests for a cutback", pdt)
+        return P, K, F
+
+    def assembleStiffnessCSR(self, K):
+        # the CSR values were accumulated by computeElements (same launch); nothing left to do
+        return K
+
+    def _rows(self, dirichlet):
+        key = id(dirichlet)
+        if key not in self._dirichletRowsCache:
+            idx = self.findDirichletIndices(dirichlet)
+            self._dirichletRowsCache[key] = self.torch.as_tensor(idx, dtype=self.torch.int64,
+                                                                 device=self.assembly.device)
+        return self._dirichletRowsCache[key]
+
+    def _allRows32(self, dirichlets):
+        key = ("all",) + tuple(id(d) for d in dirichlets)
+        if key not in self._dirichletRowsCache:
+            rows = np.concatenate([self.findDirichletIndices(d) for d in dirichlets]) if dirichlets else \
+                np.zeros(0, dtype=np.int64)
+            self._dirichletRowsCache[key] = self.torch.as_tensor(rows.astype(np.int32), device=self.assembly.device)
+        return self._dirichletRowsCache[key]
+
+    def applyDirichletK(self, K, dirichlets):
+        if dirichlets:
+            tic = _time.perf_counter()
+            self.assembly.dirichletRows(self._allRows32(dirichlets), K)
+            self.computationTimes["dirichlet K"] += _time.perf_counter() - tic
+        return K
+
+    def applyDirichlet(self, timeStep, R, dirichlets):
+        tic = _time.perf_counter()
+        for dirichlet in dirichlets:
+            delta = dirichlet.getDelta(timeStep)
+            R[self._rows(dirichlet)] = self.torch.as_tensor(delta.flatten(), dtype=self.torch.float64,
+                                                            device=R.device)
+        self.computationTimes["dirichlet R"] += _time.perf_counter() - tic
+        return R
+
+    def computeSpatialAveragedFluxes(self, F):
+        out = {}
+        for field, nAcc in self.theDofManager.nAccumulatedNodalFluxesFieldwise.items():
+            sl = self.theDofManager.idcsOfFieldsInDofVector[field]
+            out[field] = max(1e-10, float(F[sl].abs().sum().item()) / nAcc)
+        return out
+
+    def checkConvergence(self, R, ddU, F, iterationCounter, residualHistory):
+        tic = _time.perf_counter()
+        convergedAtAll = True
+        nodesWithLargestResidual = {}
+        avg = self.computeSpatialAveragedFluxes(F)
+        tolFlux = self.fluxResidualTolerances if iterationCounter < 15 else self.fluxResidualTolerancesAlt
+        floor = self.options["fluxResidualFloor"]
+        for field, sl in self.theDofManager.idcsOfFieldsInDofVector.items():
+            absR = R[sl].abs()
+            idx = int(absR.argmax().item())
+            fluxResidual = float(absR[idx].item())
+            nodesWithLargestResidual[field] = self.theDofManager.getNodeForIndexInDofVector(idx)
+            fieldCorrection = float(ddU[sl].abs().max().item()) if ddU is not None else 0.0
+            convergedCorrection = fieldCorrection < self.fieldCorrectionTolerances[field]
+            convergedFlux = fluxResidual <= max(tolFlux[field] * avg[field], floor)
+            prev, nGrew = residualHistory[field]
+            if fluxResidual > prev:
+                nGrew += 1
+            residualHistory[field] = (fluxResidual, nGrew)
+            self._currentHistory.append((fluxResidual, fieldCorrection))
+            self.journal.message("{:11.2e}{:1}{:11.2e}{:1}".format(fluxResidual, "v" if convergedFlux else " ",
+                                                                   fieldCorrection,
+                                                                   "v" if convergedCorrection else " "),
+                                 self.identification, 2)
+            convergedAtAll = convergedAtAll and convergedCorrection and convergedFlux
+        self.computationTimes["convergence check"] += _time.perf_counter() - tic
+        return convergedAtAll, nodesWithLargestResidual
+
+    def linearSolve(self, A, b):
+        tic = _time.perf_counter()
+        x = self.theDofManager.constructDofVector()
+        rows = self._allRows32(self._activeDirichlets) if self._activeDirichlets else None
+        it, rel = self.assembly.pcg(A, b, x, rows, rtol=self.options["pcgTolerance"],
+                                    maxit=self.options["pcgMaxIter"])
+        self.linearSolveIterations.append((it, rel))
+        self.computationTimes["linear solve"] += _time.perf_counter() - tic
+        if not np.isfinite(rel) or bool(self.torch.isnan(x).any().item()):
+            raise DivergingSolution("Obtained NaN in linear solve")
+        return x
+
+    def assembleLoads(self, nodeForces, distributedLoads, bodyForces, U_np, PExt, K, timeStep):
+        for cLoad in nodeForces:
+            idx = self.theDofManager.idcsOfFieldsOnNodeSetsInDofVector[cLoad.field][cLoad.nSetName]
+            load = cLoad.getCurrentLoad(timeStep).flatten()
+            PExt.index_add_(0, self.torch.as_tensor(idx, device=PExt.device),
+                            self.torch.as_tensor(load, dtype=self.torch.float64, device=PExt.device))
+        return PExt, K
+
+    def extrapolateLastIncrement(self, extrapolation, timeStep, dU, dirichlets, prevTimeStep, model):
+        if extrapolation == "linear" and prevTimeStep and prevTimeStep.timeIncrement:
+            dU *= timeStep.stepProgressIncrement / prevTimeStep.stepProgressIncrement
+            dU = self.applyDirichlet(timeStep, dU, dirichlets)
+            isExtrapolatedIncrement = True
+        else:
+            isExtrapolatedIncrement = False
+            dU.zero_()
+        return dU, isExtrapolatedIncrement
+
+    def checkDivergingSolution(self, incrementResidualHistory, maxGrowingIter):
+        for _, nGrew in incrementResidualHistory.values():
+            if nGrew > maxGrowingIter:
+                return True
+        return False
+
+    def findDirichletIndices(self, dirichlet):
+        fieldIndices = self.theDofManager.idcsOfFieldsOnNodeSetsInDofVector[dirichlet.field][dirichlet.nSetName]
+        return fieldIndices.reshape((len(dirichlet.nSet), -1))[:, dirichlet.components].flatten()
+
+    def applyStepActionsAtStepStart(self, model, stepActions):
+        self._activeDirichlets = list(stepActions["dirichlet"].values())
+        for typ in stepActions.values():
+            for action in typ.values():
+                action.applyAtStepStart(model)
+
+    def applyStepActionsAtStepEnd(self, model, stepActions):
+        for typ in stepActions.values():
+            for action in typ.values():
+                action.applyAtStepEnd(model)
+
+    def applyStepActionsAtIncrementStart(self, model, timeStep, stepActions):
+        for typ in stepActions.values():
+            for action in typ.values():
+                action.applyAtIncrementStart(model, timeStep)

@@ -1,0 +1,255 @@
+"""Steps, increments and step actions of the nonlinear implicit static path.
+
+Restates, with the reference's names and semantics:
+  * TimeStep                     edelweissfe/timesteppers/timestep.py
+  * AdaptiveTimeStepper          edelweissfe/timesteppers/adaptivetimestepper.py:60-183 (zero-size first
+                                 increment, growth x1.1 after 3 good increments, cutback, limits)
+  * AdaptiveStep                 edelweissfe/steps/adaptivestep.py:56-156 (defaults and options)
+  * Dirichlet step action        edelweissfe/stepactions/dirichlet.py:53-158 (delta per node set,
+                                 linear amplitude, components)
+  * NodeForces step action       edelweissfe/stepactions/nodeforces.py:53-152
+  * exceptions                   edelweissfe/utils/exceptions.py:36-73
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+class CutbackRequest(Exception):
+    def __init__(self, message, cutbackSize):
+        super().__init__(message)
+        self.cutbackSize = cutbackSize
+
+
+class ReachedMaxIterations(Exception):
+    pass
+
+
+class DivergingSolution(Exception):
+    pass
+
+
+class ReachedMaxIncrements(Exception):
+    pass
+
+
+class ReachedMinIncrementSize(Exception):
+    pass
+
+
+class StepFailed(Exception):
+    pass
+
+
+class ConditionalStop(Exception):
+    pass
+
+
+class TimeStep:
+    def __init__(self, number, stepProgressIncrement, stepProgress, timeIncrement, stepTime, totalTime):
+        self.number = number
+        self.stepProgressIncrement = stepProgressIncrement
+        self.stepProgress = stepProgress
+        self.timeIncrement = timeIncrement
+        self.stepTime = stepTime
+        self.totalTime = totalTime
+
+
+class AdaptiveTimeStepper:
+    identification = "AdaptiveTimeStepper"
+
+    def __init__(self, currentTime, stepLength, startIncrement, maxIncrement, minIncrement, maxNumberIncrements,
+                 journal=None, increaseFactor=1.1):
+        self.nPassedGoodIncrements = 0
+        self.incrementCounter = 0
+        self.startIncrement = startIncrement
+        self.maxIncrement = maxIncrement
+        self.minIncrement = minIncrement
+        self.maxNumberIncrements = maxNumberIncrements
+        self.finishedStepProgress = 0.0
+        self.increment = min(startIncrement, maxIncrement)
+        self.allowedToIncreasedNext = True
+        self.currentTime = currentTime
+        self.stepLength = stepLength
+        self.journal = journal
+        self.increaseFactor = increaseFactor
+
+    def generateTimeStep(self):
+        while self.finishedStepProgress < (1.0 - 1e-15):
+            remainder = 1.0 - self.finishedStepProgress
+            if remainder < self.increment:
+                self.increment = remainder
+            theIncrement = self.increment if self.incrementCounter > 0 else 0.0
+            dT = self.stepLength * theIncrement
+            self.finishedStepProgress += theIncrement
+            endInStep = self.stepLength * self.finishedStepProgress
+            yield TimeStep(self.incrementCounter, theIncrement, self.finishedStepProgress, dT, endInStep,
+                           self.currentTime + endInStep)
+            if self.incrementCounter > self.maxNumberIncrements:
+                raise ReachedMaxIncrements()
+            if self.nPassedGoodIncrements >= 3 and self.allowedToIncreasedNext:
+                self.increment *= self.increaseFactor
+                if self.increment > self.maxIncrement:
+                    self.increment = self.maxIncrement
+            self.allowedToIncreasedNext = True
+            self.incrementCounter += 1
+            self.nPassedGoodIncrements += 1
+
+    def preventIncrementIncrease(self):
+        self.allowedToIncreasedNext = False
+
+    def reduceNextIncrement(self, scaleFactor):
+        if self.increment == self.minIncrement:
+            raise ReachedMinIncrementSize()
+        newIncrement = self.increment * scaleFactor
+        if newIncrement > self.maxIncrement:
+            self.increment = self.maxIncrement
+        elif newIncrement < self.minIncrement:
+            self.increment = self.minIncrement
+        else:
+            self.increment = newIncrement
+
+    def discardAndChangeIncrement(self, scaleFactor):
+        if self.incrementCounter == 0:
+            raise ReachedMinIncrementSize()
+        self.finishedStepProgress -= self.increment
+        self.incrementCounter -= 1
+        self.nPassedGoodIncrements = 0
+        self.reduceNextIncrement(scaleFactor)
+
+
+class AdaptiveStep:
+    """A step with its actions; ``actions`` maps action type -> {name: action} like the reference."""
+
+    defaultStartInc = 1.0
+    defaultMaxInc = 1.0
+    defaultMinInc = 1e-4
+    defaultMaxNumInc = 1000
+    defaultMaxIter = 10
+    defaultCriticalIter = 5
+    defaultMaxGrowingIter = 10
+    defaultCutbackFactor = 0.25
+
+    def __init__(self, number, model, stepActions=None, **kwargs):
+        kw = {k.lower(): v for k, v in kwargs.items()}
+        self.number = number
+        self.model = model
+        self.length = float(kw.get("steplength", 1.0))
+        self.startIncrementSize = float(kw.get("startinc", self.defaultStartInc))
+        self.maxIncrementSize = float(kw.get("maxinc", self.defaultMaxInc))
+        self.minIncrementSize = float(kw.get("mininc", self.defaultMinInc))
+        self.maxNumberIncrements = int(kw.get("maxnuminc", self.defaultMaxNumInc))
+        self.maxIter = int(kw.get("maxiter", self.defaultMaxIter))
+        self.criticalIter = int(kw.get("criticaliter", self.defaultCriticalIter))
+        self.maxGrowIter = int(kw.get("maxgrowiter", self.defaultMaxGrowingIter))
+        self.cutbackFactor = float(kw.get("cutbackfactor", self.defaultCutbackFactor))
+        self.incrementGenerator = AdaptiveTimeStepper(model.time, self.length, self.startIncrementSize,
+                                                      self.maxIncrementSize, self.minIncrementSize,
+                                                      self.maxNumberIncrements)
+        self.actions = {k: {} for k in ("dirichlet", "nodeforces", "distributedload", "bodyforce", "geostatic",
+                                        "options", "modelupdate")}
+        for typ, acts in (stepActions or {}).items():
+            self.actions.setdefault(typ, {}).update(acts)
+
+    def getTimeStep(self):
+        return self.incrementGenerator.generateTimeStep()
+
+    def discardAndChangeIncrement(self, cutbackFactor):
+        return self.incrementGenerator.discardAndChangeIncrement(cutbackFactor)
+
+    def preventIncrementIncrease(self):
+        return self.incrementGenerator.preventIncrementIncrease()
+
+
+def _linear(x):
+    return x
+
+
+class Dirichlet:
+    """``dirichlet, nSet=.., field=displacement, 1=.., 2=.., 3=..`` (dirichlet.py:53-158). ``components``
+    maps 0-based component -> prescribed value at the end of the step; ``amplitude`` is f(stepProgress)."""
+
+    def __init__(self, name, model, nSet: str, components: dict, field="displacement", amplitude=None):
+        if field != "displacement":
+            raise NotImplementedError("the GPU path carries the displacement field only")
+        self.name = name
+        self.field = field
+        self.nSetName = nSet
+        self.nSet = np.asarray(model.nodeSets[nSet], dtype=np.int64)
+        comps = sorted(int(c) for c in components)
+        self._components = comps
+        self.delta = np.tile([float(components[c]) for c in comps], (len(self.nSet), 1))
+        self.amplitude = amplitude or _linear
+        self.active = True
+
+    @property
+    def components(self):
+        return self._components
+
+    def applyAtStepStart(self, model):
+        pass
+
+    def applyAtIncrementStart(self, model, timeStep):
+        pass
+
+    def applyAtStepEnd(self, model):
+        self.active = False
+
+    def getDelta(self, timeStep: TimeStep):
+        if self.active:
+            return self.delta * (self.amplitude(timeStep.stepProgress) -
+                                 self.amplitude(timeStep.stepProgress - timeStep.stepProgressIncrement))
+        return self.delta * 0.0
+
+
+class NodeForces:
+    """``nodeforces, nSet=.., components='fx, fy, fz'`` (nodeforces.py:53-152): a force vector per node,
+    scaled by the amplitude of the step progress."""
+
+    def __init__(self, name, model, nSet: str, vector, field="displacement", amplitude=None):
+        if field != "displacement":
+            raise NotImplementedError("the GPU path carries the displacement field only")
+        self.name = name
+        self.field = field
+        self.nSetName = nSet
+        self.nodeSet = np.asarray(model.nodeSets[nSet], dtype=np.int64)
+        self.nodeForcesStepStart = np.zeros((len(self.nodeSet), 3))
+        self.nodeForcesDelta = np.tile(np.asarray(vector, dtype=np.float64), (len(self.nodeSet), 1))
+        self.amplitude = amplitude or _linear
+        self._idle = False
+
+    def applyAtStepStart(self, model):
+        pass
+
+    def applyAtIncrementStart(self, model, timeStep):
+        pass
+
+    def applyAtStepEnd(self, model):
+        if not self._idle:
+            self.nodeForcesStepStart += self.nodeForcesDelta * self.amplitude(1.0)
+            self.nodeForcesDelta[:] = 0
+            self._idle = True
+
+    def getCurrentLoad(self, timeStep: TimeStep):
+        if self._idle:
+            return self.nodeForcesStepStart
+        return self.nodeForcesStepStart + self.nodeForcesDelta * self.amplitude(timeStep.stepProgress)
+
+
+class Options:
+    """``options, category=NISTSolver, key=value`` (edelweissfe/stepactions/options.py:38-61)."""
+
+    def __init__(self, name, category, **options):
+        self.name = name
+        self.category = category
+        self.options = options
+
+    def applyAtStepStart(self, model):
+        pass
+
+    def applyAtIncrementStart(self, model, timeStep):
+        pass
+
+    def applyAtStepEnd(self, model):
+        pass

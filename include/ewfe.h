@@ -1,0 +1,131 @@
+/*
+ * ewfe.h -- C ABI of libewfe.so: MI355X (gfx950) element evaluation + CSR assembly for the
+ * nonlinear implicit static Newton loop of EdelweissFE.
+ *
+ * Every entry point returns 0 on success and a negative code on failure; ewfe_last_error()
+ * then holds a message. No C++ exception crosses this boundary. Buffers named *_dev are device
+ * pointers (e.g. torch tensors' data_ptr()), owned by the caller; buffers named *_host are host
+ * memory. A plan owns only its own index structures (element order, slot map, colouring).
+ * Calls are ordered on the HIP stream passed in (NULL = default stream); distinct plans may be
+ * used concurrently from different host threads.
+ *
+ * Reference interfaces these entry points replace (paths relative to the EdelweissFE tree):
+ *   ewfe_plan_create / ewfe_plan_pattern
+ *       DofManager VIJ pattern + CSRGenerator.__init__:
+ *       edelweissfe/numerics/dofmanager.py:522-557, edelweissfe/numerics/csrgenerator.pyx:47-98
+ *   ewfe_eval_assemble
+ *       NIST.computeElements + NIST.assembleStiffnessCSR (CSRGenerator.updateCSR), i.e. the
+ *       per-element MarmotElement::computeYourself(QTotal, dQ, Pe, Ke, time, dT, pNewDT) loop:
+ *       edelweissfe/solvers/nonlinearimplicitstatic.py:794-849, 753-769;
+ *       edelweissfe/solvers/nonlinearimplicitstaticparallelmk2.pyx:111-198;
+ *       edelweissfe/elements/marmotelement/element.pxd:97-103; csrgenerator.pyx:100-115
+ *   ewfe_dirichlet_rows
+ *       NIST.applyDirichletK (row elimination + unit diagonal): nonlinearimplicitstatic.py:559-593
+ *   ewfe_spmv / ewfe_pcg / ewfe_pcg_workspace_size
+ *       NIST.linearSolve (nonlinearimplicitstatic.py:727-751) and the linear-solver registry
+ *       (edelweissfe/config/linsolve.py:46-86)
+ */
+#ifndef EWFE_H
+#define EWFE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ewfe_plan ewfe_plan;
+
+enum ewfe_elem_kind { EWFE_C3D8 = 0, EWFE_C3D20 = 1, EWFE_C3D8TL = 2, EWFE_C3D20TL = 3 };
+enum ewfe_mat_kind { EWFE_LINEAR_ELASTIC = 0, EWFE_VON_MISES = 1, EWFE_NEO_HOOKE_WA = 2 };
+enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1 };
+
+/* Error codes */
+#define EWFE_OK 0
+#define EWFE_EINVAL -1   /* bad argument */
+#define EWFE_EHIP -2     /* HIP runtime error */
+#define EWFE_ENOMEM -3   /* allocation failed */
+#define EWFE_ERANGE -4   /* size beyond int32 CSR indexing */
+
+const char *ewfe_last_error(void);
+int ewfe_version(void);
+
+/* Builds the CSR pattern, the element -> CSR slot map and the element colouring on the host,
+ * then uploads the plan's index structures to the current device.
+ *   conn_host      int32 [n_el * npe]  node indices of each element, element-local node order
+ *   node_dof_host  int64 [n_nodes]     first dof of each node (its 3 dofs are consecutive), -1 = none
+ *   n_dof          size of the dof vector
+ * Outputs nnz; the pattern itself is read with ewfe_plan_pattern. */
+int ewfe_plan_create(ewfe_plan **plan, int elem_kind, int64_t n_el, const int32_t *conn_host, int64_t n_nodes,
+                     const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out);
+
+/* Same as ewfe_plan_create without touching any device: the plan answers ewfe_plan_pattern,
+ * ewfe_plan_element_order and ewfe_plan_info only (host-side checks and tools). */
+int ewfe_plan_create_host(ewfe_plan **plan, int elem_kind, int64_t n_el, const int32_t *conn_host, int64_t n_nodes,
+                          const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out);
+
+/* Copies the CSR pattern to host buffers: indptr int32[n_dof+1], indices int32[nnz]. It equals
+ * scipy's canonical COO->CSR of the VIJ pattern bit for bit (csrgenerator.pyx:68-77). */
+int ewfe_plan_pattern(const ewfe_plan *plan, int32_t *indptr_host, int32_t *indices_host);
+
+/* Device copies of the pattern (owned by the plan, valid until destroy). */
+int ewfe_plan_pattern_dev(const ewfe_plan *plan, const int32_t **indptr_dev, const int32_t **indices_dev);
+
+/* Element order of the plan's quadrature-point state arrays: order_host[p] = element index stored at
+ * position p (state layout is [component][qp][p], structure of arrays). n_colors_out may be NULL. */
+int ewfe_plan_element_order(const ewfe_plan *plan, int64_t *order_host, int32_t *n_colors_out);
+
+/* Number of state doubles per quadrature point the kernels keep for a material:
+ * linear elastic 0, von Mises 7 (plastic strain 6 in engineering Voigt [11,22,33,12,13,23] + kappa),
+ * Neo-Hooke 0 (path independent). */
+int ewfe_state_size(int mat_kind);
+int ewfe_plan_info(const ewfe_plan *plan, int64_t *n_el, int32_t *nodes_per_el, int32_t *qp_per_el,
+                   int64_t *n_dof);
+int ewfe_plan_destroy(ewfe_plan *plan);
+
+/* One Newton iteration's element evaluation + assembly (the reference zeroes P, F and the CSR values
+ * first, nonlinearimplicitstatic.py:419, csrgenerator.pyx:111; so does this call).
+ *   mat_params_host  f64[8]  linear elastic [E, nu]; von Mises [E, nu, fy0, H, dfy, delta];
+ *                            Neo-Hooke Wa [mu, K]
+ *   el_param_dev     f64[n_el] or NULL: per-element override of mat_params[0] (original element order)
+ *   X_dev            f64[n_dof] reference coordinates laid out like a dof vector (X[dof(node)+c])
+ *   U_np_dev, dU_dev f64[n_dof] current total displacement and increment displacement
+ *   state_n_dev      f64[nstate * nqp * n_el] converged state, plan order; NULL if nstate == 0
+ *   state_t_dev      same layout, receives the trial state
+ *   P_dev, F_dev     f64[n_dof] out: P = -f_int (element "P"), F = sum |Pe|
+ *   csr_vals_dev     f64 or f32 [nnz] out (vals_dtype); the tangent is computed in that precision
+ *   pnewdt_dev       f64[1] out: 1e36, or < 1 if a quadrature point requests a cutback
+ */
+int ewfe_eval_assemble(const ewfe_plan *plan, int mat_kind, const double *mat_params_host,
+                       const double *el_param_dev, const double *X_dev, const double *U_np_dev,
+                       const double *dU_dev, const double *state_n_dev, double *state_t_dev, double *P_dev,
+                       double *F_dev, void *csr_vals_dev, int vals_dtype, double *pnewdt_dev, void *stream);
+
+/* Dirichlet rows of the assembled matrix: zero the rows listed in rows_dev (int32 [n_rows]) and put
+ * 1 on their diagonal (nonlinearimplicitstatic.py:576-584; the reference's eliminate_zeros acts on a
+ * copy and does not change the generator's pattern). */
+int ewfe_dirichlet_rows(const ewfe_plan *plan, const int32_t *rows_dev, int64_t n_rows, void *csr_vals_dev,
+                        int vals_dtype, void *stream);
+
+
+/* y = K x over the plan's CSR pattern (K in f64 or f32, x and y f64). */
+int ewfe_spmv(const ewfe_plan *plan, const void *csr_vals_dev, int vals_dtype, const double *x_dev, double *y_dev,
+              void *stream);
+
+/* Bytes of device workspace ewfe_pcg needs for this plan. */
+int64_t ewfe_pcg_workspace_size(const ewfe_plan *plan);
+
+/* Solves K x = b for the Newton correction, K being the assembled matrix after ewfe_dirichlet_rows with
+ * the same rows (NIST.linearSolve, nonlinearimplicitstatic.py:727-751; the reference factorises the
+ * row-eliminated matrix, this call runs Jacobi-preconditioned CG on its symmetric free-free block, which
+ * has the same solution). Stops when ||b - K x||_2 <= rtol ||b||_2 or after maxit iterations; the host
+ * polls for convergence every check_every iterations. Reports iterations and the final relative residual.
+ * Fails with EWFE_EINVAL if p.Kp <= 0 (matrix not positive definite on the free dofs). */
+int ewfe_pcg(const ewfe_plan *plan, const void *csr_vals_dev, int vals_dtype, const double *b_dev, double *x_dev,
+             const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every, void *workspace_dev,
+             int *iters_out, double *relres_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,198 @@
+"""GPU parity: libewfe.so (through its C ABI) against the oracle on the same seeded inputs.
+
+Tolerances (normwise, relative to the largest entry):
+  FP64 tangent, residual, state           1e-12
+  FP32 tangent vs the FP64 oracle          2e-6
+  CSR pattern                              bit-exact
+  decks: final U vs the reference's U.ref  1e-6 absolute (the reference's bar), vs the oracle 1e-9 relative
+"""
+import numpy as np
+import pytest
+
+import decks
+from gpu_util import GpuEval, j2_plastic_strain_from_oracle, jittered_box, one_block_model, oracle_eval, rel
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+J2 = [210000.0, 0.3, 250.0, 1000.0, 0.0, 0.0]
+J2NL = [210000.0, 0.3, 550.0, 1000.0, 200.0, 1400.0]
+LE = [210000.0, 0.3]
+NH = [80.0, 200.0]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _setup():
+    O.build()
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    print("device:", torch.cuda.get_device_name(0), flush=True)
+
+
+def _uniaxial_field(X, strain, rng, noise):
+    u = np.zeros_like(X)
+    u[:, 2] = strain * X[:, 2]
+    u[:, 0] = -0.3 * strain * X[:, 0]
+    u[:, 1] = -0.3 * strain * X[:, 1]
+    return (u + noise * rng.uniform(-1, 1, X.shape)).ravel()
+
+
+@pytest.mark.parametrize("el,mat,mp,n", [("C3D8", "linearelastic", LE, 4), ("C3D20", "linearelastic", LE, 2),
+                                         ("C3D8TL", "neohookeanwa", NH, 4), ("C3D20TL", "neohookeanwa", NH, 2)])
+def test_eval_assemble_matches_oracle(el, mat, mp, n):
+    print(f"case {el} {mat} n={n}", flush=True)
+    X, conn, _ = jittered_box(n, el, seed=1)
+    rng = np.random.default_rng(2)
+    amp = 0.08 if "TL" in el else 2e-3
+    U = _uniaxial_field(X, amp, rng, amp / 10)
+    dU = 0.5 * U
+    g = GpuEval(el, mat, mp, X, conn)
+    out = g(U, dU)
+    ref = oracle_eval(el, mat, mp, X, conn, U, dU)
+    ip, ix = g.asm.pattern()
+    assert np.array_equal(ip, ref["indptr"]) and np.array_equal(ix, ref["indices"])
+    assert rel(out["data"], ref["data"]) < 1e-12
+    assert rel(out["P"], ref["P"]) < 1e-12
+    assert rel(out["F"], ref["F"]) < 1e-12
+    assert out["pNewDT"] == 1e36
+    out2 = g(U, dU)  # deterministic: bitwise identical on a second call
+    assert np.array_equal(out2["data"], out["data"]) and np.array_equal(out2["P"], out["P"])
+
+
+@pytest.mark.parametrize("el,n,mp", [("C3D8", 4, J2), ("C3D8", 3, J2NL), ("C3D20", 2, J2)])
+def test_j2_two_increments_match_oracle(el, n, mp):
+    """Increment 1 from the virgin state into the plastic range, then a second increment from the
+    converged state: P, F, CSR values, plastic strain and kappa against the oracle."""
+    print(f"case J2 {el} n={n} mp={mp}", flush=True)
+    X, conn, _ = jittered_box(n, el, seed=3)
+    rng = np.random.default_rng(4)
+    U1 = _uniaxial_field(X, 4e-3, rng, 2e-4)
+    g = GpuEval(el, "vonmises", mp, X, conn)
+    o1 = g(U1, U1)
+    r1 = oracle_eval(el, "vonmises", mp, X, conn, U1, U1)
+    assert rel(o1["data"], r1["data"]) < 1e-12 and rel(o1["P"], r1["P"]) < 1e-12
+    ep_r, k_r = j2_plastic_strain_from_oracle(r1["state"], mp[0], mp[1])
+    st = o1["state"].reshape(-1, 7)
+    assert np.all(k_r > 0)  # every quadrature point yielded
+    assert rel(st[:, 6], k_r) < 1e-11 and rel(st[:, :6], ep_r) < 1e-11
+    g.block.acceptLastState()
+    dU = _uniaxial_field(X, 1e-3, rng, 1e-4)
+    U2 = U1 + dU
+    o2 = g(U2, dU)
+    r2 = oracle_eval(el, "vonmises", mp, X, conn, U2, dU, state_n=r1["state"])
+    assert rel(o2["data"], r2["data"]) < 1e-11 and rel(o2["P"], r2["P"]) < 1e-11
+    ep_r, k_r = j2_plastic_strain_from_oracle(r2["state"], mp[0], mp[1])
+    st = o2["state"].reshape(-1, 7)
+    assert rel(st[:, 6], k_r) < 1e-11 and rel(st[:, :6], ep_r) < 1e-11
+    # zero increment -> elastic tangent, stress of the converged state (vonmises.py:210-213)
+    o3 = g(U1, np.zeros_like(U1))
+    r3 = oracle_eval(el, "vonmises", mp, X, conn, U1, np.zeros_like(U1), state_n=r1["state"])
+    assert rel(o3["data"], r3["data"]) < 1e-12 and rel(o3["P"], r3["P"]) < 1e-11
+
+
+def test_fp32_tangent_close_to_fp64():
+    X, conn, _ = jittered_box(4, "C3D8", seed=5)
+    rng = np.random.default_rng(6)
+    U = _uniaxial_field(X, 4e-3, rng, 2e-4)
+    g = GpuEval("C3D8", "vonmises", J2, X, conn)
+    o64 = g(U, U, "fp64")
+    o32 = g(U, U, "fp32")
+    assert rel(o32["data"], o64["data"]) < 2e-6
+    assert np.array_equal(o32["P"], o64["P"])  # residual stays FP64
+
+
+def test_per_element_parameter_field():
+    X, conn, _ = jittered_box(3, "C3D8TL", seed=7)
+    rng = np.random.default_rng(8)
+    mu_e = 80.0 * np.exp(0.2 * rng.standard_normal(len(conn)))
+    U = rng.uniform(-0.03, 0.03, X.size)
+    g = GpuEval("C3D8TL", "neohookeanwa", NH, X, conn, elParams=mu_e)
+    out = g(U, U)
+    ref = oracle_eval("C3D8TL", "neohookeanwa", NH, X, conn, U, U, el_mp=mu_e)
+    assert rel(out["data"], ref["data"]) < 1e-12 and rel(out["P"], ref["P"]) < 1e-12
+
+
+def test_j2_cutback_request():
+    """A hardening law the local Newton cannot resolve in 15 iterations -> pNewDT = 0.5 (vonmises.py:230-231)."""
+    X, conn, _ = jittered_box(2, "C3D8", seed=9)
+    mp = [210000.0, 0.3, 250.0, 0.0, 5e5, 1e-3]  # strongly nonlinear saturation
+    U = _uniaxial_field(X, 5e-2, np.random.default_rng(0), 0)
+    r = oracle_eval("C3D8", "vonmises", mp, X, conn, U, U)
+    g = GpuEval("C3D8", "vonmises", mp, X, conn)
+    out = g(U, U)
+    assert (r["pNewDT"] < 1) == (out["pNewDT"] < 1)
+
+
+def test_pcg_solves_row_eliminated_system():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+
+    X, conn, sets = jittered_box(5, "C3D8", seed=11)
+    g = GpuEval("C3D8", "linearelastic", LE, X, conn)
+    n = g.dm.nDof
+    vals = g.asm.newValues("fp64")
+    P, F = g.dm.constructDofVector(), g.dm.constructDofVector()
+    z = g.vec(np.zeros(n))
+    g.asm.evaluate(z, z, P, F, vals)
+    rows = np.concatenate([(g.dm.nodeDofBase[sets["gen_back"]][:, None] + np.arange(3)).ravel()]).astype(np.int32)
+    rows_t = torch.as_tensor(rows, device="cuda")
+    g.asm.dirichletRows(rows_t, vals)
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal(n)
+    b[rows] = 1e-3
+    x = g.dm.constructDofVector()
+    it, rr = g.asm.pcg(vals, g.vec(b), x, rows_t, rtol=1e-12, maxit=5000)
+    ip, ix = g.asm.pattern()
+    K = sp.csr_matrix((vals.cpu().numpy(), ix, ip), shape=(n, n))
+    xr = spla.spsolve(K.tocsc(), b)
+    assert rr < 1e-12 and it > 0
+    assert rel(x.cpu().numpy(), xr) < 1e-9
+    y = g.dm.constructDofVector()
+    g.asm.spmv(vals, x, y)
+    assert rel(y.cpu().numpy(), K @ x.cpu().numpy()) < 1e-13
+
+
+def _deck_run(name, tangent="fp64"):
+    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
+    from edelweissfe_amd.stepping import AdaptiveStep, Dirichlet, NodeForces, Options
+
+    fx = decks.load(name)
+    X, conn, sets = decks.mesh(fx, O.boxgen)
+    model, block = one_block_model(fx["el_type"], fx["material"], fx["mp"], X, conn, sets=sets)
+    st = fx["step"]
+    acts = {"dirichlet": {}, "nodeforces": {}, "options": {}}
+    for i, d in enumerate(st["dirichlet"]):
+        acts["dirichlet"][f"d{i}"] = Dirichlet(f"d{i}", model, d["nSet"], {int(k): v for k, v in d["components"].items()})
+    for i, d in enumerate(st["nodeforces"]):
+        acts["nodeforces"][f"f{i}"] = NodeForces(f"f{i}", model, d["nSet"], d["vector"])
+    acts["options"]["NISTSolver"] = Options("o", "NISTSolver", extrapolation=st["extrapolation"])
+    kw = {k: st[k] for k in ("maxInc", "minInc", "maxNumInc", "maxIter", "stepLength", "startInc") if k in st}
+    step = AdaptiveStep(1, model, acts, **kw)
+    solver = NISTGPU(tangentPrecision=tangent, pcgTolerance=1e-13)
+    solver.solveStep(step, model)
+    return fx, model.U.cpu().numpy(), solver
+
+
+@pytest.mark.parametrize("name", ["TensionBarHexa8", "CantileverBeamHexa8", "C3D20", "C3D8", "SimpleBeamHexa8"])
+def test_reference_decks_on_gpu(name):
+    print(f"deck {name}", flush=True)
+    fx, U, solver = _deck_run(name)
+    Uref = np.array(fx["U_ref"])
+    assert np.abs(U - Uref).max() < 1e-6
+    X, conn, sets = decks.mesh(fx, O.boxgen)
+    r = O.solve_step(dict(el_kind=fx["el_type"], mat_kind=fx["material"], mp=fx["mp"], X=X, conn=conn),
+                     decks.step(fx, sets))
+    assert rel(U, r["U"]) < 1e-9
+    assert [len(h) for h in solver.history] == [len(h) for h in r["history"]]
+
+
+def test_fp32_tangent_newton_reaches_fp64_tolerance():
+    fx, U32, s32 = _deck_run("C3D8", "fp32")
+    _, U64, s64 = _deck_run("C3D8", "fp64")
+    assert np.abs(U32 - np.array(fx["U_ref"])).max() < 1e-6
+    assert rel(U32, U64) < 1e-8
+    it32 = sum(len(h) for h in s32.history)
+    it64 = sum(len(h) for h in s64.history)
+    print(f"Newton iterations fp64 {it64} fp32-tangent {it32}", flush=True)

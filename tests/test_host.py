@@ -1,0 +1,190 @@
+"""CPU tests of the product's host side: the C ABI library loads and exports every declared symbol, the
+host-built CSR pattern / slot map / colouring equal the oracle's scipy restatement bit for bit, boxGen,
+DofManager and the time stepper follow the reference's ordering and semantics."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import decks
+from edelweissfe_amd import _lib
+from edelweissfe_amd.model import FEModel, boxGenArrays, jitterNodes
+from edelweissfe_amd.stepping import AdaptiveTimeStepper, ReachedMinIncrementSize
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "ewfe.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(ewfe_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 14
+    for s in syms:
+        assert hasattr(L, s), s
+        assert s in _lib.SIGNATURES, s
+    assert L.ewfe_version() >= 1
+    assert L.ewfe_state_size(_lib.EWFE_VON_MISES) == 7
+
+
+def host_plan(kind, conn, node_dof, n_dof):
+    L = _lib.lib()
+    plan = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    node_dof = np.ascontiguousarray(node_dof, dtype=np.int64)
+    _lib.check(L.ewfe_plan_create_host(ctypes.byref(plan), kind, len(conn), _lib.ptr(conn), len(node_dof),
+                                       _lib.ptr(node_dof), n_dof, ctypes.byref(nnz)))
+    indptr = np.zeros(n_dof + 1, dtype=np.int32)
+    indices = np.zeros(nnz.value, dtype=np.int32)
+    _lib.check(L.ewfe_plan_pattern(plan, _lib.ptr(indptr), _lib.ptr(indices)))
+    order = np.zeros(len(conn), dtype=np.int64)
+    nc = ctypes.c_int32()
+    _lib.check(L.ewfe_plan_element_order(plan, _lib.ptr(order), ctypes.byref(nc)))
+    L.ewfe_plan_destroy(plan)
+    return indptr, indices, order, nc.value
+
+
+def _check_pattern(kind, X, conn):
+    base, nDof, edofs = O.dof_numbering(len(X), conn)
+    I, J = O.vij_pattern(edofs)
+    ip_ref, ix_ref, _ = O.csr_pattern(I, J, nDof)
+    ip, ix, order, ncol = host_plan(kind, conn, base, nDof)
+    assert np.array_equal(ip, ip_ref) and np.array_equal(ix, ix_ref)
+    assert np.array_equal(np.sort(order), np.arange(len(conn)))
+    return order, ncol
+
+
+@pytest.mark.parametrize("el,n", [("C3D8", 3), ("C3D8", (5, 2, 4)), ("C3D20", 2), ("C3D20", (1, 3, 2))])
+def test_pattern_bitexact_boxgen(el, n):
+    n = (n, n, n) if isinstance(n, int) else n
+    X, conn, _ = O.boxgen(*n, nnodes=8 if el == "C3D8" else 20)
+    order, ncol = _check_pattern(_lib.ELEMENT_KINDS[el], X, conn)
+    if el == "C3D8" and min(n) >= 2:
+        assert ncol == 8  # parity colouring of a structured grid
+
+
+@pytest.mark.parametrize("deck", ["C3D8", "C3D20", "CantileverBeamHexa8"])
+def test_pattern_bitexact_explicit_meshes(deck):
+    fx = decks.load(deck)
+    X, conn, _ = decks.mesh(fx, O.boxgen)
+    _check_pattern(_lib.ELEMENT_KINDS[fx["el_type"]], X, conn)
+
+
+def test_pattern_shuffled_numbering_and_unused_nodes():
+    X, conn, _ = O.boxgen(3, 2, 2)
+    rng = np.random.default_rng(4)
+    perm = rng.permutation(len(X))  # new node order
+    inv = np.argsort(perm)
+    X2 = np.concatenate([X[perm], np.zeros((3, 3))])  # three trailing nodes without elements
+    conn2 = inv[conn].astype(np.int32)
+    conn2 = conn2[rng.permutation(len(conn2))]
+    _check_pattern(_lib.EWFE_C3D8, X2, conn2)
+
+
+def test_colouring_is_conflict_free():
+    X, conn, _ = O.boxgen(4, 3, 5)
+    base, nDof, _ = O.dof_numbering(len(X), conn)
+    _, _, order, ncol = host_plan(_lib.EWFE_C3D8, conn, base, nDof)
+    # recover colour boundaries: the plan keeps each colour contiguous and in element order
+    sizes = []
+    start = 0
+    for p in range(1, len(order) + 1):
+        if p == len(order) or order[p] < order[p - 1]:
+            sizes.append(p - start)
+            start = p
+    assert len(sizes) == ncol
+    s = 0
+    for sz in sizes:
+        nodes = conn[order[s: s + sz]].ravel()
+        assert len(np.unique(nodes)) == len(nodes)
+        s += sz
+
+
+def test_plan_rejects_bad_input():
+    L = _lib.lib()
+    plan = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    conn = np.array([[0, 1, 2, 3, 4, 5, 6, 99]], dtype=np.int32)
+    node_dof = np.arange(0, 24, 3, dtype=np.int64)
+    rc = L.ewfe_plan_create_host(ctypes.byref(plan), 0, 1, _lib.ptr(conn), 8, _lib.ptr(node_dof), 24,
+                                 ctypes.byref(nnz))
+    assert rc == -1 and b"outside" in L.ewfe_last_error()
+    with pytest.raises(_lib.EwfeError):
+        _lib.check(rc)
+
+
+@pytest.mark.parametrize("el,n", [("C3D8", (3, 2, 4)), ("C3D20", (2, 3, 1))])
+def test_boxgen_matches_reference_order(el, n):
+    npe = 8 if el == "C3D8" else 20
+    Xo, conno, setso = O.boxgen(*n, lX=2.0, lY=3.0, lZ=0.5, nnodes=npe)
+    X, conn, sets, grid = boxGenArrays(*n, lX=2.0, lY=3.0, lZ=0.5, elType=el)
+    assert np.array_equal(X, Xo) and np.array_equal(conn, conno)
+    for k, v in setso.items():
+        if k != "all":
+            assert np.array_equal(sets[k], v), k
+
+
+def test_jitter_keeps_faces_planar():
+    X, conn, sets, grid = boxGenArrays(4, 4, 4, elType="C3D8")
+    Xj = jitterNodes(X, grid, (5, 5, 5), np.array([0.25] * 3), 0.05, seed=0)
+    assert np.abs(Xj - X).max() <= 0.05 * 0.25 + 1e-15
+    assert np.all(Xj[sets["gen_back"], 2] == 0.0) and np.all(Xj[sets["gen_front"], 2] == 1.0)
+    assert np.all(Xj[sets["gen_left"], 0] == 0.0) and np.all(Xj[sets["gen_right"], 0] == 1.0)
+
+
+def test_dofmanager_matches_oracle_numbering():
+    from edelweissfe_amd.dofmanager import DofManager
+    from edelweissfe_amd.model import ElementBlock
+
+    fx = decks.load("C3D8")
+    X, conn, sets = decks.mesh(fx, O.boxgen)
+    m = FEModel()
+    m.addNodes(X)
+    m.addElementBlock(ElementBlock("C3D8", conn))
+    m.nodeSets.update({k: np.asarray(v) for k, v in sets.items()})
+    dm = DofManager(m, device="cpu")
+    base, nDof, _ = O.dof_numbering(len(X), conn)
+    assert dm.nDof == nDof and np.array_equal(dm.nodeDofBase, base)
+    assert dm.nAccumulatedNodalFluxesFieldwise["displacement"] == 3 * conn.size
+    idx = dm.idcsOfFieldsOnNodeSetsInDofVector["displacement"]["nSetRight"]
+    assert np.array_equal(idx, (base[sets["nSetRight"]][:, None] + np.arange(3)).ravel())
+
+
+def test_time_stepper_matches_oracle_with_cutbacks():
+    seq_a, seq_b = [], []
+    ts = AdaptiveTimeStepper(0.0, 2.0, 0.3, 0.25, 1e-3, 50)
+    ora = O.time_steps(2.0, 0.3, 0.25, 1e-3, 50)
+    ga, gb = ts.generateTimeStep(), ora.gen()
+    for k in range(40):
+        try:
+            a = next(ga)
+        except StopIteration:
+            a = None
+        try:
+            b = next(gb)
+        except StopIteration:
+            b = None
+        if a is None or b is None:
+            assert a is None and b is None
+            break
+        seq_a.append((a.number, a.stepProgressIncrement, a.stepProgress, a.timeIncrement))
+        seq_b.append((b["number"], b["stepProgressIncrement"], b["stepProgress"], b["timeIncrement"]))
+        if k in (3, 4, 9):
+            ts.discardAndChangeIncrement(0.25)
+            ora.discard(0.25)
+        if k == 6:
+            ts.preventIncrementIncrease()
+            ora.allowInc = False
+    assert seq_a == seq_b and len(seq_a) > 8
+    t2 = AdaptiveTimeStepper(0.0, 1.0, 1.0, 1.0, 1.0, 10)
+    g = t2.generateTimeStep()
+    next(g)
+    with pytest.raises(ReachedMinIncrementSize):
+        t2.discardAndChangeIncrement(0.5)
