@@ -29,6 +29,8 @@ class ElementAssembly:
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise RuntimeError("ElementAssembly runs on a HIP device only")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         torch.cuda.set_device(self.device)
         self.block = block
         self.nDof = dofManager.nDof

@@ -202,6 +202,7 @@ __device__ bool j2_recipe(const MatParams &m, double E, const double H[3][3], co
                           0.5 * (dH[1][2] + dH[2][1])};
     const double epT[6] = {ep[0], ep[1], ep[2], 0.5 * ep[3], 0.5 * ep[4], 0.5 * ep[5]};
     for (int i = 0; i < 6; ++i) ep_t[i] = ep[i];
+    for (int i = 0; i < 6; ++i) r.N[i] = 0.0;  // flow direction only on plastic points
     kappa_t = kappa;
     r.alpha = lam;
     r.beta = mu;
