@@ -185,7 +185,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach64 / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_element": B64,
-                     "scope": "one evaluation call: CSR/P/F zero fill + 8 colour launches of k_hex8"},
+                     "scope": "one evaluation call: k_hex8_elem (element records) + k_gather8 (node-row CSR assembly)"},
         "cpu_baseline": cpu,
         "fp32_tangent": {"value": n_el / (ms32 * 1e-3), "ms_per_step": ms32, "speedup_vs_fp64": ms64 / ms32,
                          "roofline_achieved_GBs": ach32, "roofline_frac": ach32 / HBM_PEAK_GBS,

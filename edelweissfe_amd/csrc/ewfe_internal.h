@@ -25,12 +25,28 @@ struct ewfe_plan {
     int32_t *d_edof = nullptr;    // [npe][n_el]   first dof of element node a, plan order
     int32_t *d_colpos = nullptr;  // [npe*npe][n_el] column offset of node b's block inside node a's rows
     int32_t *d_order = nullptr;   // [n_el] plan position -> element index
+
+    // 8-node elements: node-row gather structures (element records are summed per row node)
+    int64_t n_rownodes = 0, n_inc = 0;
+    int32_t max_deg = 0;
+    int32_t *d_rownode_dof = nullptr;  // [n_rownodes] first dof of each row node, ascending
+    int32_t *d_inc_ptr = nullptr;      // [n_rownodes + 1]
+    int32_t *d_inc = nullptr;          // [n_inc] 8 * element + local node index, ascending element
+    uint8_t *d_inc_rank = nullptr;     // [n_inc * 8] rank of element node b among the row node's neighbours
+    // element records (allocated on first evaluation): 36 unique 3x3 blocks per element, and nodal forces
+    void *d_kbuf = nullptr;
+    int kbuf_dtype = -1;
+    double *d_fe = nullptr;
 };
 
 namespace ewfe {
 void set_error(const std::string &msg);
 // Builds pattern, colouring and slot map on the host; fills everything but the device pointers.
 // Returns 0 or a negative error code. edof_host / colpos_host receive the SoA device images.
+struct GatherHost {
+    std::vector<int32_t> rownode_dof, inc_ptr, inc;
+    std::vector<uint8_t> inc_rank;
+};
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,
-                    std::vector<int32_t> &colpos_host);
+                    std::vector<int32_t> &colpos_host, GatherHost &gather);
 }  // namespace ewfe

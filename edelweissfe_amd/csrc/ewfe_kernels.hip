@@ -380,25 +380,40 @@ struct QPLayout {
 };
 
 // ------------------------------------------------------------------------------------------------
-// C3D8 / C3D8TL: 8 elements per 64-lane workgroup
+// C3D8 / C3D8TL, pass 1: element records. 8 elements per 64-lane workgroup.
+// Record of element p: kbuf[p][36][3][3] = the 36 unique 3x3 blocks K_ab (a <= b, packed upper triangle,
+// row-major), fe[p][8][3] = internal nodal forces f_a (P_e = -f).
 // ------------------------------------------------------------------------------------------------
+constexpr int kPairs8 = 36;
+constexpr int kRec8 = kPairs8 * 9;  // 324
+
+__host__ __device__ constexpr int pair8(int lo, int hi) { return lo * 8 - lo * (lo + 1) / 2 + hi; }
+
+struct RecordArgs {
+    void *kbuf;
+    double *fe;
+};
+
 template <int FORM, int MAT, typename VT>
-__global__ __launch_bounds__(64) void k_hex8(EvalArgs A)
+__global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 {
     constexpr int NN = 8, NQ = 8, EPB = 8;
     using L = QPLayout<NN, FORM>;
-    __shared__ double s_nd[9 * NN * EPB];      // [field][node][le]
-    __shared__ double s_qp[L::NF * NQ * EPB];  // [field][q][le]
+    constexpr int QP_WORDS = L::NF * NQ * EPB;
+    constexpr int STAGE_WORDS = (EPB * kRec8 * (int)sizeof(VT) + 7) / 8;
+    __shared__ double s_nd[9 * NN * EPB];                                            // [field][node][le]
+    __shared__ double s_qp[QP_WORDS > STAGE_WORDS ? QP_WORDS : STAGE_WORDS];         // [field][q][le]
+    __shared__ double s_fe[EPB * 24];
     const int lane = threadIdx.x;
     const int le = lane >> 3, t = lane & 7;
-    const int64_t pos = A.p_begin + (int64_t)blockIdx.x * EPB + le;
+    const int64_t pos0 = A.p_begin + (int64_t)blockIdx.x * EPB;
+    const int64_t pos = pos0 + le;
     const bool active = pos < A.p_end;
     const int64_t n = A.n_el;
 
     // phase 0: lane (le, t) stages node t of its element
-    int32_t my_dof = 0;
     if (active) {
-        my_dof = A.edof[(int64_t)t * n + pos];
+        const int32_t my_dof = A.edof[(int64_t)t * n + pos];
         for (int c = 0; c < 3; ++c) {
             s_nd[((0 + c) * NN + t) * EPB + le] = A.X[my_dof + c];
             s_nd[((3 + c) * NN + t) * EPB + le] = A.U[my_dof + c];
@@ -429,7 +444,6 @@ __global__ __launch_bounds__(64) void k_hex8(EvalArgs A)
         for (int i = 0; i < 6; ++i) put(L::S + i, wd * r.S[i]);
     }
     __syncthreads();
-    if (!active) return;
 
     // phase 2: lane (le, a=t) builds blocks K_ab, b = a + j (mod 8), j < nb, and f_a
     const int a = t;
@@ -439,83 +453,218 @@ __global__ __launch_bounds__(64) void k_hex8(EvalArgs A)
         for (int i = 0; i < 3; ++i)
             for (int k = 0; k < 3; ++k) K[j][i][k] = 0.0;
     double fa[3] = {0, 0, 0};
-    auto ld = [&](int f, int q) { return s_qp[(f * NQ + q) * EPB + le]; };
-    for (int q = 0; q < NQ; ++q) {
-        const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q), cc = ld(L::COEF + 3, q);
-        double pa[3], ra[3], va[3] = {0, 0, 0}, Nq[6];
-        for (int i = 0; i < 3; ++i) {
-            ra[i] = ld(L::G + 3 * a + i, q);
-            pa[i] = ld(L::P + 3 * a + i, q);
+    if (active && FORM == FORM_SS && MAT == M_LE) {
+        // linear elasticity: alpha = lambda wd, beta = gamma = mu wd with constant lambda, mu, so with the
+        // single sum T_ab = sum_q mu wd g_a g_b^T:  K_ab = (lambda / mu) T + T^T + tr(T) I
+        auto ld = [&](int f, int q) { return s_qp[(f * NQ + q) * EPB + le]; };
+#pragma unroll 2
+        for (int q = 0; q < NQ; ++q) {
+            const double muwd = ld(L::COEF + 2, q);
+            double wa[3];
+            for (int i = 0; i < 3; ++i) wa[i] = muwd * ld(L::G + 3 * a + i, q);
+            {
+                const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+                const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+                const double ga0 = ld(L::G + 3 * a, q), ga1 = ld(L::G + 3 * a + 1, q), ga2 = ld(L::G + 3 * a + 2, q);
+                fa[0] += S0 * ga0 + S3 * ga1 + S4 * ga2;
+                fa[1] += S3 * ga0 + S1 * ga1 + S5 * ga2;
+                fa[2] += S4 * ga0 + S5 * ga1 + S2 * ga2;
+            }
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                if (j < nb) {
+                    const int b = (a + j) & 7;
+                    const double gb0 = ld(L::G + 3 * b, q), gb1 = ld(L::G + 3 * b + 1, q), gb2 = ld(L::G + 3 * b + 2, q);
+                    for (int i = 0; i < 3; ++i) {
+                        K[j][i][0] += wa[i] * gb0;
+                        K[j][i][1] += wa[i] * gb1;
+                        K[j][i][2] += wa[i] * gb2;
+                    }
+                }
+            }
         }
-        if (MAT == M_J2) {
-            for (int i = 0; i < 6; ++i) Nq[i] = ld(L::NV + i, q);
-            va[0] = Nq[0] * pa[0] + Nq[3] * pa[1] + Nq[4] * pa[2];
-            va[1] = Nq[3] * pa[0] + Nq[1] * pa[1] + Nq[5] * pa[2];
-            va[2] = Nq[4] * pa[0] + Nq[5] * pa[1] + Nq[2] * pa[2];
+        const double nu = A.mat.p[1];
+        const double ratio = 2.0 * nu / (1.0 - 2.0 * nu);  // lambda / mu
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            double T[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) T[i][k] = K[j][i][k];
+            const double tr = T[0][0] + T[1][1] + T[2][2];
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) K[j][i][k] = ratio * T[i][k] + T[k][i] + (i == k ? tr : 0.0);
         }
-        {  // residual f_a = S p_a
-            const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
-            const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
-            fa[0] += S0 * pa[0] + S3 * pa[1] + S4 * pa[2];
-            fa[1] += S3 * pa[0] + S1 * pa[1] + S5 * pa[2];
-            fa[2] += S4 * pa[0] + S5 * pa[1] + S2 * pa[2];
+    } else if (active) {
+        auto ld = [&](int f, int q) { return s_qp[(f * NQ + q) * EPB + le]; };
+        for (int q = 0; q < NQ; ++q) {
+            const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q),
+                         cc = ld(L::COEF + 3, q);
+            double pa[3], ra[3], va[3] = {0, 0, 0}, Nq[6];
+            for (int i = 0; i < 3; ++i) {
+                ra[i] = ld(L::G + 3 * a + i, q);
+                pa[i] = ld(L::P + 3 * a + i, q);
+            }
+            if (MAT == M_J2) {
+                for (int i = 0; i < 6; ++i) Nq[i] = ld(L::NV + i, q);
+                va[0] = Nq[0] * pa[0] + Nq[3] * pa[1] + Nq[4] * pa[2];
+                va[1] = Nq[3] * pa[0] + Nq[1] * pa[1] + Nq[5] * pa[2];
+                va[2] = Nq[4] * pa[0] + Nq[5] * pa[1] + Nq[2] * pa[2];
+            }
+            {  // internal force f_a = S p_a
+                const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+                const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+                fa[0] += S0 * pa[0] + S3 * pa[1] + S4 * pa[2];
+                fa[1] += S3 * pa[0] + S1 * pa[1] + S5 * pa[2];
+                fa[2] += S4 * pa[0] + S5 * pa[1] + S2 * pa[2];
+            }
+            double Aa[3], Ba[3], Va[3];
+            for (int i = 0; i < 3; ++i) {
+                Aa[i] = al * pa[i];
+                Ba[i] = be * pa[i];
+                Va[i] = cc * va[i];
+            }
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                if (j < nb) {
+                    const int b = (a + j) & 7;
+                    double pb[3], rb[3], vb[3] = {0, 0, 0};
+                    for (int i = 0; i < 3; ++i) {
+                        rb[i] = ld(L::G + 3 * b + i, q);
+                        pb[i] = ld(L::P + 3 * b + i, q);
+                    }
+                    if (MAT == M_J2) {
+                        vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
+                        vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
+                        vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
+                    }
+                    const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                    for (int i = 0; i < 3; ++i)
+                        for (int k = 0; k < 3; ++k) {
+                            double v = Aa[i] * pb[k] + Ba[k] * pb[i];
+                            if (MAT == M_J2) v -= Va[i] * vb[k];
+                            if (i == k) v += d;
+                            K[j][i][k] += v;
+                        }
+                }
+            }
         }
-        double Aa[3], Ba[3], Va[3];
-        for (int i = 0; i < 3; ++i) {
-            Aa[i] = al * pa[i];
-            Ba[i] = be * pa[i];
-            Va[i] = cc * va[i];
-        }
+    }
+    __syncthreads();  // every lane is done reading s_qp: reuse it as the record staging area
+
+    VT *stage = reinterpret_cast<VT *>(s_qp);
+    if (active) {
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
             if (j < nb) {
                 const int b = (a + j) & 7;
-                double pb[3], rb[3], vb[3] = {0, 0, 0};
-                for (int i = 0; i < 3; ++i) {
-                    rb[i] = ld(L::G + 3 * b + i, q);
-                    pb[i] = ld(L::P + 3 * b + i, q);
-                }
-                if (MAT == M_J2) {
-                    vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
-                    vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
-                    vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
-                }
-                const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                VT *blk = stage + le * kRec8 + 9 * (a <= b ? pair8(a, b) : pair8(b, a));
                 for (int i = 0; i < 3; ++i)
                     for (int k = 0; k < 3; ++k) {
-                        double v = Aa[i] * pb[k] + Ba[k] * pb[i];
-                        if (MAT == M_J2) v -= Va[i] * vb[k];
-                        if (i == k) v += d;
-                        K[j][i][k] += v;
+                        if (a <= b)
+                            blk[3 * i + k] = (VT)K[j][i][k];
+                        else
+                            blk[3 * k + i] = (VT)K[j][i][k];  // store K_ba = K_ab^T
                     }
             }
         }
+        for (int i = 0; i < 3; ++i) s_fe[le * 24 + 3 * a + i] = fa[i];
     }
+    __syncthreads();
+    const int64_t valid = (A.p_end - pos0) < EPB ? (A.p_end - pos0) : EPB;
+    VT *kout = reinterpret_cast<VT *>(R.kbuf) + pos0 * kRec8;
+    for (int64_t w = lane; w < valid * kRec8; w += 64) kout[w] = stage[w];
+    double *fout = R.fe + pos0 * 24;
+    for (int64_t w = lane; w < valid * 24; w += 64) fout[w] = s_fe[w];
+}
 
-    // scatter: P = -f, F = |f| for node a; K_ab into node a's rows and K_ab^T into node b's rows
-    for (int i = 0; i < 3; ++i) {
-        A.P[my_dof + i] -= fa[i];
-        A.F[my_dof + i] += fabs(fa[i]);
-    }
-    VT *vals = reinterpret_cast<VT *>(A.vals);
-    int32_t rs_a[3];
-    for (int i = 0; i < 3; ++i) rs_a[i] = A.indptr[my_dof + i];
+// ------------------------------------------------------------------------------------------------
+// C3D8 / C3D8TL, pass 2: one 64-lane workgroup per row node. Sums the node's rows of every incident
+// element's record in ascending element order (the order of the reference's serial VIJ -> CSR scatter,
+// csrgenerator.pyx:112-113, and P/F scatter, mk2.pyx:187-193) in LDS, then writes the node's 3 CSR rows
+// (contiguous: the rows of one node have equal length) and P, F once.
+// ------------------------------------------------------------------------------------------------
+constexpr int kMaxDeg8 = 64;
+constexpr int kXcds = 8;  // MI355X: workgroups are dealt round-robin to 8 XCDs, each with its own L2
+
+// Workgroup w runs on XCD w % 8; give every XCD one contiguous range of row nodes so that neighbouring
+// nodes -- which read the same element records -- share an L2.
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t w, int64_t n)
+{
+    const int64_t per = (n + kXcds - 1) / kXcds;
+    return (w % kXcds) * per + w / kXcds;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(64) void k_gather8(const int32_t *__restrict__ rownode_dof,
+                                                const int32_t *__restrict__ inc_ptr, const int32_t *__restrict__ inc,
+                                                const uint8_t *__restrict__ inc_rank, const int32_t *__restrict__ indptr,
+                                                const VT *__restrict__ kbuf, const double *__restrict__ fe,
+                                                VT *__restrict__ vals, double *__restrict__ P, double *__restrict__ F,
+                                                int64_t n_rownodes)
+{
+    __shared__ uint8_t s_rank[64];
+    __shared__ int32_t s_code[8];
+    const int64_t r = xcd_contiguous(blockIdx.x, n_rownodes);
+    if (r >= n_rownodes) return;  // uniform over the workgroup
+    const int lane = threadIdx.x;
+    const int32_t d = rownode_dof[r];
+    const int32_t base = indptr[d];
+    const int32_t L = indptr[d + 1] - base;  // 3 * degree
+    const int m = lane;                      // this lane owns neighbour block (row node, neighbour of rank m)
+    const bool owner = 3 * m < L;
+    double acc[9];
 #pragma unroll
-    for (int j = 0; j < 5; ++j) {
-        if (j < nb) {
-            const int b = (a + j) & 7;
-            const int32_t cab = A.colpos[(int64_t)(a * NN + b) * n + pos];
-            for (int i = 0; i < 3; ++i)
-                for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)rs_a[i] + cab + k, K[j][i][k]);
-            if (j > 0) {
-                const int32_t db = A.edof[(int64_t)b * n + pos];
-                const int32_t cba = A.colpos[(int64_t)(b * NN + a) * n + pos];
-                for (int k = 0; k < 3; ++k) {
-                    const int64_t rs = A.indptr[db + k];
-                    for (int i = 0; i < 3; ++i) add_to(vals, rs + cba + i, K[j][i][k]);
+    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+    double f = 0.0, fa = 0.0;
+    const int32_t k0 = inc_ptr[r], k1 = inc_ptr[r + 1];
+    for (int32_t c0 = k0; c0 < k1; c0 += 8) {
+        const int nk = (k1 - c0) < 8 ? (k1 - c0) : 8;
+        if (lane < 8 * nk) s_rank[lane] = inc_rank[(int64_t)c0 * 8 + lane];
+        if (lane < nk) s_code[lane] = inc[c0 + lane];
+        __syncthreads();
+        // contributions to block m, in ascending element order (incidents are sorted by element)
+#pragma unroll
+        for (int sl = 0; sl < 8; ++sl) {
+            if (sl < nk) {
+                int b = -1;
+#pragma unroll
+                for (int bb = 0; bb < 8; ++bb)
+                    if (s_rank[sl * 8 + bb] == m) b = bb;
+                if (owner && b >= 0) {
+                    const int32_t code = s_code[sl];
+                    const int64_t e = code >> 3;
+                    const int a = code & 7;
+                    const int lo = a < b ? a : b, hi = a < b ? b : a;
+                    const VT *blk = kbuf + e * kRec8 + 9 * pair8(lo, hi);
+                    if (a <= b) {
+#pragma unroll
+                        for (int t = 0; t < 9; ++t) acc[t] += (double)blk[t];
+                    } else {  // stored block is K_ba: add its transpose
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int kk = 0; kk < 3; ++kk) acc[3 * i + kk] += (double)blk[3 * kk + i];
+                    }
+                }
+                if (lane < 3) {
+                    const int32_t code = s_code[sl];
+                    const double x = fe[(int64_t)(code >> 3) * 24 + 3 * (code & 7) + lane];
+                    f -= x;
+                    fa += fabs(x);
                 }
             }
         }
+        __syncthreads();
+    }
+    if (owner) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) vals[(int64_t)base + i * L + 3 * m + kk] = (VT)acc[3 * i + kk];
+    }
+    if (lane < 3) {
+        P[d + lane] = f;
+        F[d + lane] = fa;
     }
 }
 
@@ -711,9 +860,10 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
         return EWFE_ERANGE;
     }
     std::vector<int32_t> edof, colpos;
+    ewfe::GatherHost gh;
     int rc;
     try {
-        rc = ewfe::build_plan_host(*p, conn_host, node_dof_host, edof, colpos);
+        rc = ewfe::build_plan_host(*p, conn_host, node_dof_host, edof, colpos, gh);
     } catch (const std::bad_alloc &) {
         rc = EWFE_ENOMEM;
         ewfe::set_error("out of host memory while building the plan");
@@ -755,6 +905,19 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
         if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
         if (u.bytes && (e = hipMemcpy(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess)
             return fail(e, "hipMemcpy");
+    }
+    if (p->npe == 8) {
+        Up gups[] = {
+            {&p->d_rownode_dof, gh.rownode_dof.data(), sizeof(int32_t) * gh.rownode_dof.size()},
+            {&p->d_inc_ptr, gh.inc_ptr.data(), sizeof(int32_t) * gh.inc_ptr.size()},
+            {&p->d_inc, gh.inc.data(), sizeof(int32_t) * gh.inc.size()},
+            {(int32_t **)&p->d_inc_rank, gh.inc_rank.data(), gh.inc_rank.size()},
+        };
+        for (auto &u : gups) {
+            if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
+            if (u.bytes && (e = hipMemcpy(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess)
+                return fail(e, "hipMemcpy");
+        }
     }
     if (nnz_out) *nnz_out = p->nnz;
     *out = p;
@@ -822,34 +985,59 @@ extern "C" int ewfe_plan_info(const ewfe_plan *p, int64_t *n_el, int32_t *npe, i
 extern "C" int ewfe_plan_destroy(ewfe_plan *p)
 {
     if (!p) return 0;
-    int32_t *bufs[] = {p->d_indptr, p->d_indices, p->d_edof, p->d_colpos, p->d_order};
+    void *bufs[] = {p->d_indptr, p->d_indices,     p->d_edof,     p->d_colpos,  p->d_order, p->d_rownode_dof,
+                    p->d_inc_ptr, p->d_inc,        p->d_inc_rank, p->d_kbuf,    p->d_fe};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     delete p;
     return 0;
 }
 
-template <int FORM, int MAT, typename VT>
-static int launch_family(const ewfe_plan *p, EvalArgs A, hipStream_t s)
+template <typename VT>
+static int ensure_records(ewfe_plan *p, int dtype)
 {
+    if (p->d_kbuf && p->kbuf_dtype == dtype) return 0;
+    if (p->d_kbuf) (void)hipFree(p->d_kbuf);
+    p->d_kbuf = nullptr;
+    HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(VT) * (size_t)kRec8 * p->n_el));
+    p->kbuf_dtype = dtype;
+    if (!p->d_fe) HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 24 * (size_t)p->n_el));
+    return 0;
+}
+
+template <int FORM, int MAT, typename VT>
+static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s)
+{
+    if (p->npe == 8) {
+        int rc = ensure_records<VT>(p, sizeof(VT) == 8 ? EWFE_F64 : EWFE_F32);
+        if (rc) return rc;
+        RecordArgs R{p->d_kbuf, p->d_fe};
+        A.p_begin = 0;
+        A.p_end = p->n_el;
+        const int64_t blocks = (p->n_el + 7) / 8;
+        hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT>), dim3((unsigned)blocks), dim3(64), 0, s, A, R);
+        HIP_CHECK(hipGetLastError());
+        // xcd_contiguous() maps workgroup w to node (w % 8) * ceil(n / 8) + w / 8: launch 8 * ceil(n / 8)
+        const int64_t gblocks = kXcds * ((p->n_rownodes + kXcds - 1) / kXcds);
+        hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)gblocks), dim3(64), 0, s, p->d_rownode_dof,
+                           p->d_inc_ptr, p->d_inc, p->d_inc_rank, p->d_indptr, (const VT *)p->d_kbuf,
+                           (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->n_rownodes);
+        HIP_CHECK(hipGetLastError());
+        return 0;
+    }
     for (int c = 0; c < p->n_colors; ++c) {
         A.p_begin = p->color_begin[c];
         A.p_end = p->color_begin[c + 1];
         const int64_t cnt = A.p_end - A.p_begin;
         if (cnt <= 0) continue;
-        if (p->npe == 8) {
-            const int64_t blocks = (cnt + 7) / 8;
-            hipLaunchKernelGGL((k_hex8<FORM, MAT, VT>), dim3((unsigned)blocks), dim3(64), 0, s, A);
-        } else {
-            hipLaunchKernelGGL((k_hex20<FORM, MAT, VT>), dim3((unsigned)cnt), dim3(64), 0, s, A);
-        }
+        hipLaunchKernelGGL((k_hex20<FORM, MAT, VT>), dim3((unsigned)cnt), dim3(64), 0, s, A);
         HIP_CHECK(hipGetLastError());
     }
     return 0;
 }
 
 template <typename VT>
-static int dispatch(const ewfe_plan *p, int mat_kind, EvalArgs A, hipStream_t s)
+static int dispatch(ewfe_plan *p, int mat_kind, EvalArgs A, hipStream_t s)
 {
     const bool tl = is_tl(p->elem_kind);
     if (!tl && mat_kind == EWFE_LINEAR_ELASTIC) return launch_family<FORM_SS, M_LE, VT>(p, A, s);
@@ -891,13 +1079,19 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     int rc;
-    if ((rc = fill<double>(P_dev, p->n_dof, 0.0, s))) return rc;
-    if ((rc = fill<double>(F_dev, p->n_dof, 0.0, s))) return rc;
+    // the 8-node gather writes P, F and every CSR value of the row nodes once; other dofs must read zero
+    const bool gather = p->npe == 8;
+    if (!gather || 3 * p->n_rownodes != p->n_dof) {
+        if ((rc = fill<double>(P_dev, p->n_dof, 0.0, s))) return rc;
+        if ((rc = fill<double>(F_dev, p->n_dof, 0.0, s))) return rc;
+    }
     if ((rc = fill<double>(pnewdt_dev, 1, 1e36, s))) return rc;
-    if (vals_dtype == EWFE_F64) {
-        if ((rc = fill<double>(reinterpret_cast<double *>(csr_vals_dev), p->nnz, 0.0, s))) return rc;
-    } else {
-        if ((rc = fill<float>(reinterpret_cast<float *>(csr_vals_dev), p->nnz, 0.0f, s))) return rc;
+    if (!gather) {
+        if (vals_dtype == EWFE_F64) {
+            if ((rc = fill<double>(reinterpret_cast<double *>(csr_vals_dev), p->nnz, 0.0, s))) return rc;
+        } else {
+            if ((rc = fill<float>(reinterpret_cast<float *>(csr_vals_dev), p->nnz, 0.0f, s))) return rc;
+        }
     }
     EvalArgs A;
     A.edof = p->d_edof;
@@ -917,7 +1111,8 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
     A.n_el = p->n_el;
     A.p_begin = A.p_end = 0;
     for (int i = 0; i < 8; ++i) A.mat.p[i] = mat_params_host[i];
-    return vals_dtype == EWFE_F64 ? dispatch<double>(p, mat_kind, A, s) : dispatch<float>(p, mat_kind, A, s);
+    ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // element records are allocated lazily by the plan
+    return vals_dtype == EWFE_F64 ? dispatch<double>(pm, mat_kind, A, s) : dispatch<float>(pm, mat_kind, A, s);
 }
 
 extern "C" int ewfe_dirichlet_rows(const ewfe_plan *p, const int32_t *rows_dev, int64_t n_rows, void *csr_vals_dev,

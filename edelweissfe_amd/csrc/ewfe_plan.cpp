@@ -41,7 +41,7 @@ static void parallel_for(int64_t n, F &&f)
 }
 
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,
-                    std::vector<int32_t> &colpos_host)
+                    std::vector<int32_t> &colpos_host, GatherHost &gather)
 {
     const int npe = plan.npe;
     const int64_t n_el = plan.n_el, n_nodes = plan.n_nodes, n_dof = plan.n_dof;
@@ -150,11 +150,12 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
         }
     });
 
-    // --- greedy element colouring (elements of one colour share no node) -------------------------
-    std::vector<uint64_t> node_mask(n_nodes, 0);
-    std::vector<int8_t> color(n_el);
-    int n_colors = 0;
-    for (int64_t e = 0; e < n_el; ++e) {
+    // --- greedy element colouring (elements of one colour share no node); 8-node elements need none:
+    //     their records are summed per row node by the gather kernel ------------------------------
+    std::vector<uint64_t> node_mask(npe == 8 ? 0 : n_nodes, 0);
+    std::vector<int8_t> color(n_el, 0);
+    int n_colors = 1;
+    for (int64_t e = 0; npe != 8 && e < n_el; ++e) {
         uint64_t used = 0;
         for (int a = 0; a < npe; ++a) used |= node_mask[conn[e * npe + a]];
         if (used == ~0ull) {
@@ -178,7 +179,8 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
 
     // --- SoA device images: element dof bases and the node-pair slot map -------------------------
     edof_host.resize((size_t)npe * n_el);
-    colpos_host.resize((size_t)npe * npe * n_el);
+    const bool need_colpos = npe != 8;  // the 8-node path uses the row-node gather lists instead
+    colpos_host.resize(need_colpos ? (size_t)npe * npe * n_el : 0);
     std::atomic<int> bad{0};
     parallel_for(n_el, [&](int64_t b, int64_t e) {
         for (int64_t p = b; p < e; ++p) {
@@ -186,6 +188,7 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
             for (int a = 0; a < npe; ++a) {
                 const int32_t na = conn[el * npe + a];
                 edof_host[(size_t)a * n_el + p] = (int32_t)node_dof[na];
+                if (!need_colpos) continue;
                 const int32_t *lo = nb_dof.data() + nb_ptr[na], *hi = nb_dof.data() + nb_ptr[na + 1];
                 for (int bb = 0; bb < npe; ++bb) {
                     const int32_t db = (int32_t)node_dof[conn[el * npe + bb]];
@@ -199,6 +202,52 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
     if (bad) {
         set_error("internal error: slot map lookup failed");
         return -1;
+    }
+
+    // --- row-node gather lists (8-node elements) ---------------------------------------------------
+    if (npe == 8) {
+        for (int64_t n = 0; n < n_nodes; ++n) plan.max_deg = std::max(plan.max_deg, deg[n]);
+        if (plan.max_deg > 64) {
+            set_error("a node couples with more than 64 nodes; the 8-node gather path supports at most 64");
+            return -1;
+        }
+        std::vector<int32_t> rownodes;
+        for (int64_t d = 0; d < n_dof; d += 1) {
+            const int32_t n = node_of_dof[d];
+            if (n >= 0 && node_dof[n] == d && inc_ptr[n + 1] > inc_ptr[n]) rownodes.push_back(n);
+        }
+        const int64_t nr = (int64_t)rownodes.size();
+        plan.n_rownodes = nr;
+        gather.rownode_dof.resize(nr);
+        gather.inc_ptr.assign(nr + 1, 0);
+        for (int64_t r = 0; r < nr; ++r) {
+            gather.rownode_dof[r] = (int32_t)node_dof[rownodes[r]];
+            gather.inc_ptr[r + 1] = gather.inc_ptr[r] + (int32_t)(inc_ptr[rownodes[r] + 1] - inc_ptr[rownodes[r]]);
+        }
+        plan.n_inc = gather.inc_ptr[nr];
+        gather.inc.resize(plan.n_inc);
+        gather.inc_rank.resize((size_t)plan.n_inc * 8);
+        parallel_for(nr, [&](int64_t b, int64_t e) {
+            for (int64_t r = b; r < e; ++r) {
+                const int32_t n = rownodes[r];
+                const int32_t *lo = nb_dof.data() + nb_ptr[n], *hi = nb_dof.data() + nb_ptr[n + 1];
+                int64_t w = gather.inc_ptr[r];
+                for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k, ++w) {
+                    const int64_t el = inc[k];  // ascending element order (incidence built in element order)
+                    int a = 0;
+                    while (a < 8 && conn[el * 8 + a] != n) ++a;
+                    gather.inc[w] = (int32_t)(el * 8 + a);
+                    for (int bb = 0; bb < 8; ++bb) {
+                        const int32_t db = (int32_t)node_dof[conn[el * 8 + bb]];
+                        gather.inc_rank[(size_t)w * 8 + bb] = (uint8_t)(std::lower_bound(lo, hi, db) - lo);
+                    }
+                }
+            }
+        });
+        if (n_el >= ((int64_t)1 << 28)) {
+            set_error("too many elements for the gather index packing");
+            return -4;
+        }
     }
     return 0;
 }

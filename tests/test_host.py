@@ -66,8 +66,8 @@ def test_pattern_bitexact_boxgen(el, n):
     n = (n, n, n) if isinstance(n, int) else n
     X, conn, _ = O.boxgen(*n, nnodes=8 if el == "C3D8" else 20)
     order, ncol = _check_pattern(_lib.ELEMENT_KINDS[el], X, conn)
-    if el == "C3D8" and min(n) >= 2:
-        assert ncol == 8  # parity colouring of a structured grid
+    if el == "C3D8":  # 8-node elements are gathered per row node: no colouring, element order kept
+        assert ncol == 1 and np.array_equal(order, np.arange(len(conn)))
 
 
 @pytest.mark.parametrize("deck", ["C3D8", "C3D20", "CantileverBeamHexa8"])
@@ -89,9 +89,9 @@ def test_pattern_shuffled_numbering_and_unused_nodes():
 
 
 def test_colouring_is_conflict_free():
-    X, conn, _ = O.boxgen(4, 3, 5)
+    X, conn, _ = O.boxgen(4, 3, 5, nnodes=20)
     base, nDof, _ = O.dof_numbering(len(X), conn)
-    _, _, order, ncol = host_plan(_lib.EWFE_C3D8, conn, base, nDof)
+    _, _, order, ncol = host_plan(_lib.EWFE_C3D20, conn, base, nDof)
     # recover colour boundaries: the plan keeps each colour contiguous and in element order
     sizes = []
     start = 0
