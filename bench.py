@@ -55,6 +55,46 @@ def algorithmic_bytes_per_element(n_el, n_nodes, nnz, n_state, n_qp, val_bytes, 
             + val_bytes * nnz / n_el)
 
 
+def newton_increment(n, tangent, strain=2e-3, pcg_tol=1e-10, seed=0):
+    """One implicit static increment of the same J2 cube from the virgin state to `strain` (into the plastic
+    range), solved by NISTGPU: uz = 0 on z = 0, uz = strain on z = 1, ux = uy = 0 at (0,0,0), uy = 0 at
+    (1,0,0). Newton to the reference's criteria |R|inf <= 1e-8 * averaged flux (without the 1e-7 absolute
+    floor) and |ddU|inf < 1e-8; returns timings per phase, iterations and PCG iterations."""
+    import torch
+
+    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
+    from edelweissfe_amd.stepping import AdaptiveStep, Dirichlet
+
+    m, b, X, conn = build_workload(n, seed)
+    acts = {"dirichlet": {
+        "zbottom": Dirichlet("zbottom", m, "gen_back", {2: 0.0}),
+        "ztop": Dirichlet("ztop", m, "gen_front", {2: strain}),
+        "origin": Dirichlet("origin", m, "gen_bottomLeftBack", {0: 0.0, 1: 0.0}),
+        "xaxis": Dirichlet("xaxis", m, "gen_bottomRightBack", {1: 0.0}),
+    }}
+    step = AdaptiveStep(1, m, acts, maxInc=1.0, startInc=1.0, minInc=1e-3, maxIter=20)
+    jobInfo = {"fieldCorrectionTolerance": {"displacement": 1e-8},
+               "fluxResidualTolerance": {"displacement": 1e-8},
+               "fluxResidualToleranceAlternative": {"displacement": 1e-8}}
+    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import _Journal
+
+    solver = NISTGPU(jobInfo, _Journal(verbose=True, stream=sys.stderr), tangentPrecision=tangent,
+                     pcgTolerance=pcg_tol, fluxResidualFloor=0.0, extrapolation="off")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.solveStep(step, m)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    its = [len(h) for h in solver.history]
+    newton_iters = sum(its)  # linear solves = convergence checks of the increments that converged
+    pcg_its = [it for it, _ in solver.linearSolveIterations]
+    U = m.U.detach().cpu().numpy()
+    return dict(seconds=t, newton_iterations=newton_iters, iterations_per_increment=its, pcg_iterations=pcg_its,
+                ms_per_newton_iteration=1e3 * t / max(1, newton_iters),
+                phases_s={k: v for k, v in solver.computationTimes.items() if v > 0},
+                residual_history=[h for h in solver.history], U=U)
+
+
 def cpu_baseline(X, conn, U_n, dU, layers, n_per_layer, reps=3):
     """The oracle (C + OpenMP restatement of NISTParallel's element loop, serial P/F scatter and serial
     VIJ->CSR scatter) on the first `layers` x-layers of the same mesh and inputs."""
@@ -96,6 +136,8 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="elements per cube edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-layers", type=int, default=4)
+    ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--newton-n", type=int, default=None, help="mesh of the Newton increment (default --n)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -158,6 +200,7 @@ def main():
     n_el = len(conn)
     B64 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 8)
     B32 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 4)
+    nnz = asm.nnz
     ach64 = B64 * n_el / (ms64 * 1e-3) / 1e9
     ach32 = B32 * n_el / (ms32 * 1e-3) / 1e9
 
@@ -165,6 +208,26 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(X, conn, U_n, dU, args.cpu_layers, n * n)
         print(f"cpu baseline {cpu['value']:.4g} el/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    nnz, ncolors = asm.nnz, asm.nColors
+    newton = None
+    if not args.no_newton:
+        del asm, K64, K32
+        torch.cuda.empty_cache()
+        nn_ = args.newton_n or n
+        res = {}
+        for tan in ("fp64", "fp32"):
+            r = newton_increment(nn_, tan)
+            res[tan] = r
+            print(f"newton {tan}: {r['seconds']:.2f}s, {r['newton_iterations']} iterations, pcg {r['pcg_iterations']}, "
+                  f"phases {r['phases_s']}", file=sys.stderr, flush=True)
+            torch.cuda.empty_cache()
+        U64, U32 = res["fp64"].pop("U"), res["fp32"].pop("U")
+        newton = {"mesh": f"{nn_}^3", "load": "one increment from the virgin state to 0.2% uniaxial strain",
+                  "fp64": res["fp64"], "fp32_tangent": res["fp32"],
+                  "increment_speedup_fp32_vs_fp64": res["fp64"]["seconds"] / res["fp32"]["seconds"],
+                  "max_rel_displacement_difference_fp32_vs_fp64":
+                      float(np.abs(U32 - U64).max() / np.abs(U64).max())}
 
     line = {
         "metric": "C3D8 J2 elem evals+assembly/s",
@@ -181,7 +244,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"C3D8 J2 small strain {n}^3 jittered unit cube, one Newton iteration's element "
                                f"evaluation + CSR assembly, FP64", "elements": n_el, "dofs": dm.nDof,
-                   "nnz": asm.nnz, "qp_per_element": 8, "state_per_qp": 7, "plastic_qp_fraction": plastic_frac},
+                   "nnz": nnz, "qp_per_element": 8, "state_per_qp": 7, "plastic_qp_fraction": plastic_frac},
         "roofline": {"bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach64 / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_element": B64,
@@ -191,6 +254,7 @@ def main():
                          "roofline_achieved_GBs": ach32, "roofline_frac": ach32 / HBM_PEAK_GBS,
                          "algorithmic_bytes_per_element": B32},
         "setup_s": t_setup,
+        "newton_increment": newton,
     }
     print(json.dumps(line), flush=True)
 

@@ -26,10 +26,15 @@ struct ewfe_plan {
     int32_t *d_colpos = nullptr;  // [npe*npe][n_el] column offset of node b's block inside node a's rows
     int32_t *d_order = nullptr;   // [n_el] plan position -> element index
 
-    // 8-node elements: node-row gather structures (element records are summed per row node)
-    int64_t n_rownodes = 0, n_inc = 0;
+    // row nodes (nodes with dofs and elements, ascending first dof) and their neighbour blocks: node-block
+    // SpMV / block-Jacobi of the solver, and the 8-node gather
+    int64_t n_rownodes = 0, n_inc = 0, n_nbblocks = 0;
     int32_t max_deg = 0;
+    bool rows_cover_all_dofs = false;  // every dof is one of the 3 dofs of a row node
     int32_t *d_rownode_dof = nullptr;  // [n_rownodes] first dof of each row node, ascending
+    int32_t *d_nb_ptr = nullptr;       // [n_rownodes + 1] neighbour-block offsets
+    int32_t *d_nbcol = nullptr;        // [n_nbblocks] first dof (column) of each neighbour block, ascending
+    int32_t *d_diag_rank = nullptr;    // [n_rownodes] rank of the node itself among its neighbours
     int32_t *d_inc_ptr = nullptr;      // [n_rownodes + 1]
     int32_t *d_inc = nullptr;          // [n_inc] 8 * element + local node index, ascending element
     uint8_t *d_inc_rank = nullptr;     // [n_inc * 8] rank of element node b among the row node's neighbours
@@ -44,7 +49,7 @@ void set_error(const std::string &msg);
 // Builds pattern, colouring and slot map on the host; fills everything but the device pointers.
 // Returns 0 or a negative error code. edof_host / colpos_host receive the SoA device images.
 struct GatherHost {
-    std::vector<int32_t> rownode_dof, inc_ptr, inc;
+    std::vector<int32_t> rownode_dof, inc_ptr, inc, nb_ptr, nbcol, diag_rank;
     std::vector<uint8_t> inc_rank;
 };
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,

@@ -584,15 +584,6 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 // (contiguous: the rows of one node have equal length) and P, F once.
 // ------------------------------------------------------------------------------------------------
 constexpr int kMaxDeg8 = 64;
-constexpr int kXcds = 8;  // MI355X: workgroups are dealt round-robin to 8 XCDs, each with its own L2
-
-// Workgroup w runs on XCD w % 8; give every XCD one contiguous range of row nodes so that neighbouring
-// nodes -- which read the same element records -- share an L2.
-__device__ __forceinline__ int64_t xcd_contiguous(int64_t w, int64_t n)
-{
-    const int64_t per = (n + kXcds - 1) / kXcds;
-    return (w % kXcds) * per + w / kXcds;
-}
 
 template <typename VT>
 __global__ __launch_bounds__(64) void k_gather8(const int32_t *__restrict__ rownode_dof,
@@ -602,70 +593,148 @@ __global__ __launch_bounds__(64) void k_gather8(const int32_t *__restrict__ rown
                                                 VT *__restrict__ vals, double *__restrict__ P, double *__restrict__ F,
                                                 int64_t n_rownodes)
 {
-    __shared__ uint8_t s_rank[64];
-    __shared__ int32_t s_code[8];
-    const int64_t r = xcd_contiguous(blockIdx.x, n_rownodes);
-    if (r >= n_rownodes) return;  // uniform over the workgroup
+    __shared__ double acc[3 * 3 * kMaxDeg8];
+    __shared__ double s_fe[8 * 3];
+    const int64_t r = blockIdx.x;  // consecutive nodes run concurrently on all XCDs and share records via MALL
+    if (r >= n_rownodes) return;   // uniform over the workgroup
     const int lane = threadIdx.x;
     const int32_t d = rownode_dof[r];
     const int32_t base = indptr[d];
     const int32_t L = indptr[d + 1] - base;  // 3 * degree
-    const int m = lane;                      // this lane owns neighbour block (row node, neighbour of rank m)
-    const bool owner = 3 * m < L;
-    double acc[9];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+    for (int t = lane; t < 3 * L; t += 64) acc[t] = 0.0;
     double f = 0.0, fa = 0.0;
     const int32_t k0 = inc_ptr[r], k1 = inc_ptr[r + 1];
+    const int slot = lane >> 3, b = lane & 7;  // lane = (incident slot, element node b)
     for (int32_t c0 = k0; c0 < k1; c0 += 8) {
-        const int nk = (k1 - c0) < 8 ? (k1 - c0) : 8;
-        if (lane < 8 * nk) s_rank[lane] = inc_rank[(int64_t)c0 * 8 + lane];
-        if (lane < nk) s_code[lane] = inc[c0 + lane];
+        // every lane fetches one whole 3x3 block K_ab of one incident element: all loads in flight at once
+        const int32_t k = c0 + slot;
+        const bool has = k < k1;
+        double v[9];
+        int rank = 0, a = 0;
+        double fev = 0.0;
+        if (has) {
+            const int32_t code = inc[k];
+            const int64_t e = code >> 3;
+            a = code & 7;
+            rank = inc_rank[(int64_t)k * 8 + b];
+            const int lo = a < b ? a : b, hi = a < b ? b : a;
+            const VT *blk = kbuf + e * kRec8 + 9 * pair8(lo, hi);
+#pragma unroll
+            for (int m = 0; m < 9; ++m) v[m] = (double)blk[m];
+            if (b < 3) fev = fe[e * 24 + 3 * a + b];
+        }
+        if (b < 3) s_fe[slot * 3 + b] = fev;
         __syncthreads();
-        // contributions to block m, in ascending element order (incidents are sorted by element)
+        // ordered accumulation: incident slots in ascending element order (the reference's scatter order)
 #pragma unroll
-        for (int sl = 0; sl < 8; ++sl) {
-            if (sl < nk) {
-                int b = -1;
+        for (int round = 0; round < 8; ++round) {
+            if (slot == round && has) {
+                const bool tr = a > b;  // stored block is K_ba: read it transposed
 #pragma unroll
-                for (int bb = 0; bb < 8; ++bb)
-                    if (s_rank[sl * 8 + bb] == m) b = bb;
-                if (owner && b >= 0) {
-                    const int32_t code = s_code[sl];
-                    const int64_t e = code >> 3;
-                    const int a = code & 7;
-                    const int lo = a < b ? a : b, hi = a < b ? b : a;
-                    const VT *blk = kbuf + e * kRec8 + 9 * pair8(lo, hi);
-                    if (a <= b) {
+                for (int i = 0; i < 3; ++i)
 #pragma unroll
-                        for (int t = 0; t < 9; ++t) acc[t] += (double)blk[t];
-                    } else {  // stored block is K_ba: add its transpose
-#pragma unroll
-                        for (int i = 0; i < 3; ++i)
-#pragma unroll
-                            for (int kk = 0; kk < 3; ++kk) acc[3 * i + kk] += (double)blk[3 * kk + i];
-                    }
-                }
-                if (lane < 3) {
-                    const int32_t code = s_code[sl];
-                    const double x = fe[(int64_t)(code >> 3) * 24 + 3 * (code & 7) + lane];
-                    f -= x;
-                    fa += fabs(x);
-                }
+                    for (int kk = 0; kk < 3; ++kk) acc[i * L + 3 * rank + kk] += tr ? v[3 * kk + i] : v[3 * i + kk];
+            }
+            __syncthreads();
+        }
+        if (lane < 3) {
+            const int nk = (k1 - c0) < 8 ? (k1 - c0) : 8;
+            for (int sl = 0; sl < nk; ++sl) {
+                const double x = s_fe[sl * 3 + lane];
+                f -= x;
+                fa += fabs(x);
             }
         }
         __syncthreads();
     }
-    if (owner) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk) vals[(int64_t)base + i * L + 3 * m + kk] = (VT)acc[3 * i + kk];
-    }
+    for (int t = lane; t < 3 * L; t += 64) vals[(int64_t)base + t] = (VT)acc[t];
     if (lane < 3) {
         P[d + lane] = f;
         F[d + lane] = fa;
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Adjoint: grad[e] = lambda^T dR/dtheta_e for a per-element material parameter theta_e (the first material
+// property: mu of Neo-Hooke Wa on total-Lagrangian hexahedra, E of linear elasticity on small-strain
+// hexahedra). R = P + PExt with P = -f_int, so grad[e] = -sum_a lambda_a . df_a/dtheta_e. The element
+// kinematics are recomputed (nothing of the forward evaluation is stored).
+//   Neo-Hooke Wa: tau = mu B + lamBar I, lamBar = (K - 2mu/3)(J^2 - J) - mu  =>
+//                 dtau/dmu = B - ((2/3)(J^2 - J) + 1) I,  f_a = sum_q wd tau NA_a
+//   linear elastic: sigma is linear in E  =>  dsigma/dE = sigma / E,  f_a = sum_q wd sigma g_a
+// One 64-lane workgroup holds 8 elements, lane = (element, quadrature point); the 8 point contributions
+// are summed with a fixed butterfly (deterministic).
+// ------------------------------------------------------------------------------------------------
+template <int FORM, int MAT>
+__global__ __launch_bounds__(64) void k_adjoint8(EvalArgs A, const double *__restrict__ lam, double *__restrict__ grad)
+{
+    constexpr int NN = 8, EPB = 8;
+    __shared__ double s_nd[9 * NN * EPB];  // [X, U, lambda][node][le]
+    const int lane = threadIdx.x;
+    const int le = lane >> 3, t = lane & 7;
+    const int64_t pos = (int64_t)blockIdx.x * EPB + le;
+    const bool active = pos < A.n_el;
+    const int64_t n = A.n_el;
+    if (active) {
+        const int32_t dof = A.edof[(int64_t)t * n + pos];
+        for (int c = 0; c < 3; ++c) {
+            s_nd[((0 + c) * NN + t) * EPB + le] = A.X[dof + c];
+            s_nd[((3 + c) * NN + t) * EPB + le] = A.U[dof + c];
+            s_nd[((6 + c) * NN + t) * EPB + le] = lam[dof + c];
+        }
+    }
+    __syncthreads();
+    double contrib = 0.0;
+    if (active) {
+        const int q = t;
+        auto nd = [&](int f, int a) { return s_nd[(f * NN + a) * EPB + le]; };
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3];
+        for (int a = 0; a < NN; ++a)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) J[i][j] += c_dN8[q][a][i] * nd(j, a);
+        const double wd = c_w8[q] * inv3(J, Ji);
+        double g[NN][3];
+        double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int a = 0; a < NN; ++a) {
+            for (int j = 0; j < 3; ++j)
+                g[a][j] = Ji[j][0] * c_dN8[q][a][0] + Ji[j][1] * c_dN8[q][a][1] + Ji[j][2] * c_dN8[q][a][2];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) H[i][j] += nd(3 + i, a) * g[a][j];
+        }
+        double D[3][3];  // d(stress)/d(theta), symmetric
+        double theta = A.mat.p[0];
+        if (A.el_param) theta = A.el_param[A.order[pos]];
+        if (FORM == FORM_TL) {
+            double F[3][3], Fi[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
+            const double Jd = inv3(F, Fi);
+            const double c = 2.0 / 3.0 * (Jd * Jd - Jd) + 1.0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    D[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2] - (i == j ? c : 0.0);
+            for (int a = 0; a < NN; ++a) {  // p_a = NA_a = F^-T g_a
+                double p[3];
+                for (int m = 0; m < 3; ++m) p[m] = g[a][0] * Fi[0][m] + g[a][1] * Fi[1][m] + g[a][2] * Fi[2][m];
+                for (int i = 0; i < 3; ++i)
+                    contrib += nd(6 + i, a) * wd * (D[i][0] * p[0] + D[i][1] * p[1] + D[i][2] * p[2]);
+            }
+        } else {
+            Recipe r;
+            le_recipe(A.mat, theta, H, r);
+            const double S[3][3] = {{r.S[0], r.S[3], r.S[4]}, {r.S[3], r.S[1], r.S[5]}, {r.S[4], r.S[5], r.S[2]}};
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) D[i][j] = S[i][j] / theta;
+            for (int a = 0; a < NN; ++a)
+                for (int i = 0; i < 3; ++i)
+                    contrib += nd(6 + i, a) * wd * (D[i][0] * g[a][0] + D[i][1] * g[a][1] + D[i][2] * g[a][2]);
+        }
+    }
+    // sum over the 8 quadrature-point lanes of each element (fixed butterfly)
+    contrib += __shfl_xor(contrib, 1, 8);
+    contrib += __shfl_xor(contrib, 2, 8);
+    contrib += __shfl_xor(contrib, 4, 8);
+    if (active && t == 0) grad[A.order[pos]] = -contrib;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -906,9 +975,21 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
         if (u.bytes && (e = hipMemcpy(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess)
             return fail(e, "hipMemcpy");
     }
+    {
+        Up rups[] = {
+            {&p->d_rownode_dof, gh.rownode_dof.data(), sizeof(int32_t) * gh.rownode_dof.size()},
+            {&p->d_nb_ptr, gh.nb_ptr.data(), sizeof(int32_t) * gh.nb_ptr.size()},
+            {&p->d_nbcol, gh.nbcol.data(), sizeof(int32_t) * gh.nbcol.size()},
+            {&p->d_diag_rank, gh.diag_rank.data(), sizeof(int32_t) * gh.diag_rank.size()},
+        };
+        for (auto &u : rups) {
+            if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
+            if (u.bytes && (e = hipMemcpy(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess)
+                return fail(e, "hipMemcpy");
+        }
+    }
     if (p->npe == 8) {
         Up gups[] = {
-            {&p->d_rownode_dof, gh.rownode_dof.data(), sizeof(int32_t) * gh.rownode_dof.size()},
             {&p->d_inc_ptr, gh.inc_ptr.data(), sizeof(int32_t) * gh.inc_ptr.size()},
             {&p->d_inc, gh.inc.data(), sizeof(int32_t) * gh.inc.size()},
             {(int32_t **)&p->d_inc_rank, gh.inc_rank.data(), gh.inc_rank.size()},
@@ -985,8 +1066,9 @@ extern "C" int ewfe_plan_info(const ewfe_plan *p, int64_t *n_el, int32_t *npe, i
 extern "C" int ewfe_plan_destroy(ewfe_plan *p)
 {
     if (!p) return 0;
-    void *bufs[] = {p->d_indptr, p->d_indices,     p->d_edof,     p->d_colpos,  p->d_order, p->d_rownode_dof,
-                    p->d_inc_ptr, p->d_inc,        p->d_inc_rank, p->d_kbuf,    p->d_fe};
+    void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
+                    p->d_rownode_dof, p->d_inc_ptr, p->d_inc,   p->d_inc_rank,  p->d_kbuf,
+                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -1017,9 +1099,7 @@ static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s)
         const int64_t blocks = (p->n_el + 7) / 8;
         hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT>), dim3((unsigned)blocks), dim3(64), 0, s, A, R);
         HIP_CHECK(hipGetLastError());
-        // xcd_contiguous() maps workgroup w to node (w % 8) * ceil(n / 8) + w / 8: launch 8 * ceil(n / 8)
-        const int64_t gblocks = kXcds * ((p->n_rownodes + kXcds - 1) / kXcds);
-        hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)gblocks), dim3(64), 0, s, p->d_rownode_dof,
+        hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)p->n_rownodes), dim3(64), 0, s, p->d_rownode_dof,
                            p->d_inc_ptr, p->d_inc, p->d_inc_rank, p->d_indptr, (const VT *)p->d_kbuf,
                            (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->n_rownodes);
         HIP_CHECK(hipGetLastError());
@@ -1113,6 +1193,46 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
     for (int i = 0; i < 8; ++i) A.mat.p[i] = mat_params_host[i];
     ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // element records are allocated lazily by the plan
     return vals_dtype == EWFE_F64 ? dispatch<double>(pm, mat_kind, A, s) : dispatch<float>(pm, mat_kind, A, s);
+}
+
+extern "C" int ewfe_adjoint_param_grad(const ewfe_plan *p, int mat_kind, const double *mat_params_host,
+                                       const double *el_param_dev, const double *X_dev, const double *U_dev,
+                                       const double *lambda_dev, double *grad_dev, void *stream)
+{
+    if (!p || !mat_params_host || !X_dev || !U_dev || !lambda_dev || !grad_dev) {
+        ewfe::set_error("ewfe_adjoint_param_grad: null argument");
+        return EWFE_EINVAL;
+    }
+    if (!p->d_edof) {
+        ewfe::set_error("ewfe_adjoint_param_grad: plan has no device data");
+        return EWFE_EINVAL;
+    }
+    if (p->npe != 8) {
+        ewfe::set_error("ewfe_adjoint_param_grad: 8-node elements only (C3D8, C3D8TL)");
+        return EWFE_EINVAL;
+    }
+    const bool tl = is_tl(p->elem_kind);
+    if (!((tl && mat_kind == EWFE_NEO_HOOKE_WA) || (!tl && mat_kind == EWFE_LINEAR_ELASTIC))) {
+        ewfe::set_error("ewfe_adjoint_param_grad: supported are C3D8TL + Neo-Hooke (mu_e) and C3D8 + linear "
+                        "elastic (E_e)");
+        return EWFE_EINVAL;
+    }
+    EvalArgs A{};
+    A.edof = p->d_edof;
+    A.order = p->d_order;
+    A.el_param = el_param_dev;
+    A.X = X_dev;
+    A.U = U_dev;
+    A.n_el = p->n_el;
+    for (int i = 0; i < 8; ++i) A.mat.p[i] = mat_params_host[i];
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int64_t blocks = (p->n_el + 7) / 8;
+    if (tl)
+        hipLaunchKernelGGL((k_adjoint8<FORM_TL, M_NH>), dim3((unsigned)blocks), dim3(64), 0, s, A, lambda_dev, grad_dev);
+    else
+        hipLaunchKernelGGL((k_adjoint8<FORM_SS, M_LE>), dim3((unsigned)blocks), dim3(64), 0, s, A, lambda_dev, grad_dev);
+    HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 extern "C" int ewfe_dirichlet_rows(const ewfe_plan *p, const int32_t *rows_dev, int64_t n_rows, void *csr_vals_dev,

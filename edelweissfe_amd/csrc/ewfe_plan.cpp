@@ -204,26 +204,44 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
         return -1;
     }
 
+    // --- row nodes and their neighbour blocks (all element kinds) ------------------------------------
+    for (int64_t n = 0; n < n_nodes; ++n) plan.max_deg = std::max(plan.max_deg, deg[n]);
+    std::vector<int32_t> rownodes;
+    for (int64_t d = 0; d < n_dof; d += 1) {
+        const int32_t n = node_of_dof[d];
+        if (n >= 0 && node_dof[n] == d && inc_ptr[n + 1] > inc_ptr[n]) rownodes.push_back(n);
+    }
+    const int64_t nr = (int64_t)rownodes.size();
+    plan.n_rownodes = nr;
+    plan.rows_cover_all_dofs = (3 * nr == n_dof);
+    gather.rownode_dof.resize(nr);
+    gather.nb_ptr.assign(nr + 1, 0);
+    gather.diag_rank.resize(nr);
+    for (int64_t r = 0; r < nr; ++r) {
+        gather.rownode_dof[r] = (int32_t)node_dof[rownodes[r]];
+        gather.nb_ptr[r + 1] = gather.nb_ptr[r] + deg[rownodes[r]];
+    }
+    plan.n_nbblocks = gather.nb_ptr[nr];
+    gather.nbcol.resize(plan.n_nbblocks);
+    parallel_for(nr, [&](int64_t b, int64_t e) {
+        for (int64_t r = b; r < e; ++r) {
+            const int32_t n = rownodes[r];
+            for (int32_t k = 0; k < deg[n]; ++k) {
+                gather.nbcol[gather.nb_ptr[r] + k] = nb_dof[nb_ptr[n] + k];
+                if (nb_dof[nb_ptr[n] + k] == node_dof[n]) gather.diag_rank[r] = k;
+            }
+        }
+    });
+
     // --- row-node gather lists (8-node elements) ---------------------------------------------------
     if (npe == 8) {
-        for (int64_t n = 0; n < n_nodes; ++n) plan.max_deg = std::max(plan.max_deg, deg[n]);
         if (plan.max_deg > 64) {
             set_error("a node couples with more than 64 nodes; the 8-node gather path supports at most 64");
             return -1;
         }
-        std::vector<int32_t> rownodes;
-        for (int64_t d = 0; d < n_dof; d += 1) {
-            const int32_t n = node_of_dof[d];
-            if (n >= 0 && node_dof[n] == d && inc_ptr[n + 1] > inc_ptr[n]) rownodes.push_back(n);
-        }
-        const int64_t nr = (int64_t)rownodes.size();
-        plan.n_rownodes = nr;
-        gather.rownode_dof.resize(nr);
         gather.inc_ptr.assign(nr + 1, 0);
-        for (int64_t r = 0; r < nr; ++r) {
-            gather.rownode_dof[r] = (int32_t)node_dof[rownodes[r]];
+        for (int64_t r = 0; r < nr; ++r)
             gather.inc_ptr[r + 1] = gather.inc_ptr[r] + (int32_t)(inc_ptr[rownodes[r] + 1] - inc_ptr[rownodes[r]]);
-        }
         plan.n_inc = gather.inc_ptr[nr];
         gather.inc.resize(plan.n_inc);
         gather.inc_rank.resize((size_t)plan.n_inc * 8);

@@ -1,13 +1,19 @@
-// libewfe.so -- CSR SpMV and a device-resident preconditioned conjugate-gradient solver for the Newton
-// correction K ddU = R (the reference's NIST.linearSolve, edelweissfe/solvers/nonlinearimplicitstatic.py:727-751,
-// which hands the row-eliminated matrix of applyDirichletK (:559-593) to a direct solver).
+// libewfe.so -- SpMV and a device-resident preconditioned conjugate-gradient solver for the Newton correction
+// K ddU = R (the reference's NIST.linearSolve, edelweissfe/solvers/nonlinearimplicitstatic.py:727-751, which
+// hands the row-eliminated matrix of applyDirichletK (:559-593) to a direct solver).
 //
-// The Dirichlet rows of the matrix are unit rows (ewfe_dirichlet_rows). Starting from x0 = b on those rows
-// and 0 elsewhere, the initial residual vanishes on them, every search direction keeps them at zero, and
-// the iteration is plain CG on the symmetric positive definite free-free block: its solution is the
-// solution of the row-eliminated system. Jacobi preconditioning. All scalars stay on the device; the host
-// only polls a "done" flag every few iterations. Reductions are blockwise partial sums followed by a
-// single-block sum in fixed order, so the iteration history is bitwise reproducible.
+// The Dirichlet rows of the matrix are unit rows (ewfe_dirichlet_rows). Starting from x0 = b on those rows and 0
+// elsewhere, the initial residual vanishes on them, every search direction keeps them at zero, and the iteration
+// is plain CG on the symmetric positive definite free-free block: its solution is the solution of the
+// row-eliminated system.
+//
+// MI355X layout: the 3 CSR rows of a node have one column pattern, made of 3-column blocks (one per neighbour
+// node). The node-block SpMV gives each row node a 32-lane group, one lane per neighbour block: one column index
+// per 3x3 block (the plan's neighbour list) instead of one per value, and the 3 rows' values are read as
+// contiguous runs. Preconditioner: node-block Jacobi (inverse 3x3 diagonal blocks). All scalars stay on the
+// device; the host only polls a "done" flag every few iterations. Reductions are blockwise partial sums followed
+// by a single-block sum in fixed order, so the iteration history is bitwise reproducible. Meshes whose dofs are
+// not all node dofs fall back to a row-wise CSR SpMV with point Jacobi.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -30,8 +36,17 @@ constexpr int kBlocks = 1024;  // fixed grid of the row kernels -> fixed number 
 constexpr int kThreads = 256;
 constexpr int kLanesPerRow = 16;
 constexpr int kRowsPerBlock = kThreads / kLanesPerRow;
+constexpr int kLanesPerNode = 32;
+constexpr int kNodesPerBlock = kThreads / kLanesPerNode;
 
 enum Scalar { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BB, S_DONE, S_ITERS, S_FAIL, S_COUNT = 16 };
+
+struct Mat {
+    const int32_t *indptr, *indices;                      // CSR
+    const int32_t *rn_dof, *nb_ptr, *nbcol, *diag_rank;  // row nodes
+    int64_t n, n_rownodes;
+    bool blocked;
+};
 
 __device__ __forceinline__ double group_sum16(double v)
 {
@@ -40,6 +55,12 @@ __device__ __forceinline__ double group_sum16(double v)
     v += __shfl_xor(v, 2, 16);
     v += __shfl_xor(v, 1, 16);
     return v;
+}
+
+__device__ __forceinline__ double group_sum32(double v)
+{
+    v += __shfl_xor(v, 16, 32);
+    return group_sum16(v);
 }
 
 template <int NV>
@@ -59,44 +80,113 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *partials
     }
 }
 
-// row r of K times x, by a 16-lane group; returns the full sum on every lane of the group
+// --- row-wise CSR (fallback) -----------------------------------------------------------------------
 template <typename VT>
-__device__ __forceinline__ double row_dot(const int32_t *indptr, const int32_t *indices, const VT *vals,
-                                          const double *x, int64_t r, int lane16)
+__device__ __forceinline__ double row_dot(const Mat &M, const VT *vals, const double *x, int64_t r, int lane16)
 {
     double acc = 0.0;
-    const int32_t b = indptr[r], e = indptr[r + 1];
-    for (int32_t k = b + lane16; k < e; k += kLanesPerRow) acc += (double)vals[k] * x[indices[k]];
+    const int32_t b = M.indptr[r], e = M.indptr[r + 1];
+    for (int32_t k = b + lane16; k < e; k += kLanesPerRow) acc += (double)vals[k] * x[M.indices[k]];
     return group_sum16(acc);
 }
 
+// --- node-block rows: (K x)[d..d+2] of row node rn, summed over the 32-lane group ----------------------------
 template <typename VT>
-__global__ __launch_bounds__(kThreads) void k_spmv(int64_t n, const int32_t *indptr, const int32_t *indices,
-                                                   const VT *vals, const double *x, double *y)
+__device__ __forceinline__ void node_dot(const Mat &M, const VT *__restrict__ vals, const double *__restrict__ x,
+                                         int64_t rn, int lane32, double &y0, double &y1, double &y2)
 {
-    const int lane16 = threadIdx.x % kLanesPerRow;
-    for (int64_t r = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLanesPerRow; r < n;
-         r += (int64_t)gridDim.x * kRowsPerBlock) {
-        const double s = row_dot(indptr, indices, vals, x, r, lane16);
-        if (lane16 == 0) y[r] = s;
+    const int32_t d = M.rn_dof[rn];
+    const int32_t base = M.indptr[d];
+    const int32_t L = M.indptr[d + 1] - base;
+    const int32_t nb0 = M.nb_ptr[rn], deg = M.nb_ptr[rn + 1] - nb0;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int m = lane32; m < deg; m += kLanesPerNode) {
+        const int32_t c = M.nbcol[nb0 + m];
+        const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+        const VT *v0 = vals + base + 3 * m;
+        const VT *v1 = v0 + L;
+        const VT *v2 = v1 + L;
+        a0 += (double)v0[0] * x0 + (double)v0[1] * x1 + (double)v0[2] * x2;
+        a1 += (double)v1[0] * x0 + (double)v1[1] * x1 + (double)v1[2] * x2;
+        a2 += (double)v2[0] * x0 + (double)v2[1] * x1 + (double)v2[2] * x2;
+    }
+    y0 = group_sum32(a0);
+    y1 = group_sum32(a1);
+    y2 = group_sum32(a2);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void k_spmv(Mat M, const VT *vals, const double *x, double *y)
+{
+    if (M.blocked) {
+        const int lane32 = threadIdx.x % kLanesPerNode;
+        for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
+             rn += (int64_t)gridDim.x * kNodesPerBlock) {
+            double y0, y1, y2;
+            node_dot(M, vals, x, rn, lane32, y0, y1, y2);
+            if (lane32 == 0) {
+                const int32_t d = M.rn_dof[rn];
+                y[d] = y0;
+                y[d + 1] = y1;
+                y[d + 2] = y2;
+            }
+        }
+    } else {
+        const int lane16 = threadIdx.x % kLanesPerRow;
+        for (int64_t r = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLanesPerRow; r < M.n;
+             r += (int64_t)gridDim.x * kRowsPerBlock) {
+            const double s = row_dot(M, vals, x, r, lane16);
+            if (lane16 == 0) y[r] = s;
+        }
     }
 }
 
-// inverse of the diagonal; x0 = 0 (Dirichlet rows get b in k_set_rows)
+// preconditioner: node blocks -> inverse 3x3 diagonal block (9 values per node, row-major, pre[9 rn .. 9 rn + 8]);
+// fallback -> inverse diagonal (pre[r])
 template <typename VT>
-__global__ __launch_bounds__(kThreads) void k_pcg_setup(int64_t n, const int32_t *indptr, const int32_t *indices,
-                                                        const VT *vals, double *dinv, double *x)
+__global__ __launch_bounds__(kThreads) void k_pcg_setup(Mat M, const VT *vals, double *pre, double *x)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
-        double d = 1.0;
-        for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k)
-            if (indices[k] == r) {
-                d = (double)vals[k];
-                break;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M.n; i += stride) x[i] = 0.0;
+    if (M.blocked) {
+        for (int64_t rn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; rn < M.n_rownodes; rn += stride) {
+            const int32_t d = M.rn_dof[rn];
+            const int32_t base = M.indptr[d];
+            const int32_t L = M.indptr[d + 1] - base;
+            const int32_t off = base + 3 * M.diag_rank[rn];
+            double B[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) B[i][k] = (double)vals[off + i * L + k];
+            const double det = B[0][0] * (B[1][1] * B[2][2] - B[1][2] * B[2][1]) -
+                               B[0][1] * (B[1][0] * B[2][2] - B[1][2] * B[2][0]) +
+                               B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0]);
+            double *Q = pre + 9 * rn;
+            if (det != 0.0 && det == det) {
+                const double r = 1.0 / det;
+                Q[0] = (B[1][1] * B[2][2] - B[1][2] * B[2][1]) * r;
+                Q[1] = (B[0][2] * B[2][1] - B[0][1] * B[2][2]) * r;
+                Q[2] = (B[0][1] * B[1][2] - B[0][2] * B[1][1]) * r;
+                Q[3] = (B[1][2] * B[2][0] - B[1][0] * B[2][2]) * r;
+                Q[4] = (B[0][0] * B[2][2] - B[0][2] * B[2][0]) * r;
+                Q[5] = (B[0][2] * B[1][0] - B[0][0] * B[1][2]) * r;
+                Q[6] = (B[1][0] * B[2][1] - B[1][1] * B[2][0]) * r;
+                Q[7] = (B[0][1] * B[2][0] - B[0][0] * B[2][1]) * r;
+                Q[8] = (B[0][0] * B[1][1] - B[0][1] * B[1][0]) * r;
+            } else {  // singular block: point Jacobi on it
+                for (int t = 0; t < 9; ++t) Q[t] = 0.0;
+                for (int i = 0; i < 3; ++i) Q[4 * i] = (B[i][i] != 0.0) ? 1.0 / B[i][i] : 1.0;
             }
-        dinv[r] = (d != 0.0) ? 1.0 / d : 1.0;
-        x[r] = 0.0;
+        }
+    } else {
+        for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < M.n; r += stride) {
+            double dg = 1.0;
+            for (int32_t k = M.indptr[r]; k < M.indptr[r + 1]; ++k)
+                if (M.indices[k] == r) {
+                    dg = (double)vals[k];
+                    break;
+                }
+            pre[r] = (dg != 0.0) ? 1.0 / dg : 1.0;
+        }
     }
 }
 
@@ -106,28 +196,57 @@ __global__ void k_set_rows(const int32_t *rows, int64_t n_rows, const double *b,
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_rows; i += stride) x[rows[i]] = b[rows[i]];
 }
 
-// r = b - K x, z = dinv r, partials of (r.z, r.r, b.b)
+__device__ __forceinline__ void apply_block(const double *Q, const double r0, const double r1, const double r2,
+                                            double &z0, double &z1, double &z2)
+{
+    z0 = Q[0] * r0 + Q[1] * r1 + Q[2] * r2;
+    z1 = Q[3] * r0 + Q[4] * r1 + Q[5] * r2;
+    z2 = Q[6] * r0 + Q[7] * r1 + Q[8] * r2;
+}
+
+// r = b - K x, z = M^-1 r, p = z, partials of (r.z, r.r, b.b)
 template <typename VT>
-__global__ __launch_bounds__(kThreads) void k_pcg_residual(int64_t n, const int32_t *indptr, const int32_t *indices,
-                                                           const VT *vals, const double *b, const double *x,
-                                                           const double *dinv, double *r, double *z, double *p,
+__global__ __launch_bounds__(kThreads) void k_pcg_residual(Mat M, const VT *vals, const double *b, const double *x,
+                                                           const double *pre, double *r, double *z, double *p,
                                                            double *partials)
 {
-    const int lane16 = threadIdx.x % kLanesPerRow;
     double acc[3] = {0, 0, 0};
-    for (int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLanesPerRow; row < n;
-         row += (int64_t)gridDim.x * kRowsPerBlock) {
-        const double kx = row_dot(indptr, indices, vals, x, row, lane16);
-        if (lane16 == 0) {
-            const double bi = b[row];
-            const double ri = bi - kx;
-            const double zi = dinv[row] * ri;
-            r[row] = ri;
-            z[row] = zi;
-            p[row] = zi;
-            acc[0] += ri * zi;
-            acc[1] += ri * ri;
-            acc[2] += bi * bi;
+    if (M.blocked) {
+        const int lane32 = threadIdx.x % kLanesPerNode;
+        for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
+             rn += (int64_t)gridDim.x * kNodesPerBlock) {
+            double k0, k1, k2;
+            node_dot(M, vals, x, rn, lane32, k0, k1, k2);
+            if (lane32 == 0) {
+                const int32_t d = M.rn_dof[rn];
+                const double b0 = b[d], b1 = b[d + 1], b2 = b[d + 2];
+                const double r0 = b0 - k0, r1 = b1 - k1, r2 = b2 - k2;
+                double z0, z1, z2;
+                apply_block(pre + 9 * rn, r0, r1, r2, z0, z1, z2);
+                r[d] = r0, r[d + 1] = r1, r[d + 2] = r2;
+                z[d] = z0, z[d + 1] = z1, z[d + 2] = z2;
+                p[d] = z0, p[d + 1] = z1, p[d + 2] = z2;
+                acc[0] += r0 * z0 + r1 * z1 + r2 * z2;
+                acc[1] += r0 * r0 + r1 * r1 + r2 * r2;
+                acc[2] += b0 * b0 + b1 * b1 + b2 * b2;
+            }
+        }
+    } else {
+        const int lane16 = threadIdx.x % kLanesPerRow;
+        for (int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLanesPerRow; row < M.n;
+             row += (int64_t)gridDim.x * kRowsPerBlock) {
+            const double kx = row_dot(M, vals, x, row, lane16);
+            if (lane16 == 0) {
+                const double bi = b[row];
+                const double ri = bi - kx;
+                const double zi = pre[row] * ri;
+                r[row] = ri;
+                z[row] = zi;
+                p[row] = zi;
+                acc[0] += ri * zi;
+                acc[1] += ri * ri;
+                acc[2] += bi * bi;
+            }
         }
     }
     block_partials<3>(acc, partials, kBlocks);
@@ -135,41 +254,71 @@ __global__ __launch_bounds__(kThreads) void k_pcg_residual(int64_t n, const int3
 
 // Ap = K p, partial of p.Ap
 template <typename VT>
-__global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t n, const int32_t *indptr, const int32_t *indices,
-                                                       const VT *vals, const double *p, double *Ap, double *partials,
-                                                       const double *scal)
+__global__ __launch_bounds__(kThreads) void k_pcg_spmv(Mat M, const VT *vals, const double *p, double *Ap,
+                                                       double *partials, const double *scal)
 {
     if (scal[S_DONE] != 0.0) return;
-    const int lane16 = threadIdx.x % kLanesPerRow;
     double acc[1] = {0};
-    for (int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLanesPerRow; row < n;
-         row += (int64_t)gridDim.x * kRowsPerBlock) {
-        const double s = row_dot(indptr, indices, vals, p, row, lane16);
-        if (lane16 == 0) {
-            Ap[row] = s;
-            acc[0] += s * p[row];
+    if (M.blocked) {
+        const int lane32 = threadIdx.x % kLanesPerNode;
+        for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
+             rn += (int64_t)gridDim.x * kNodesPerBlock) {
+            double y0, y1, y2;
+            node_dot(M, vals, p, rn, lane32, y0, y1, y2);
+            if (lane32 == 0) {
+                const int32_t d = M.rn_dof[rn];
+                Ap[d] = y0, Ap[d + 1] = y1, Ap[d + 2] = y2;
+                acc[0] += y0 * p[d] + y1 * p[d + 1] + y2 * p[d + 2];
+            }
+        }
+    } else {
+        const int lane16 = threadIdx.x % kLanesPerRow;
+        for (int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLanesPerRow; row < M.n;
+             row += (int64_t)gridDim.x * kRowsPerBlock) {
+            const double s = row_dot(M, vals, p, row, lane16);
+            if (lane16 == 0) {
+                Ap[row] = s;
+                acc[0] += s * p[row];
+            }
         }
     }
     block_partials<1>(acc, partials, kBlocks);
 }
 
-// x += alpha p, r -= alpha Ap, z = dinv r; partials of (r.z, r.r)
-__global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, double *x, double *r, double *z, const double *p,
-                                                         const double *Ap, const double *dinv, double *partials,
+// x += alpha p, r -= alpha Ap, z = M^-1 r; partials of (r.z, r.r)
+__global__ __launch_bounds__(kThreads) void k_pcg_update(Mat M, double *x, double *r, double *z, const double *p,
+                                                         const double *Ap, const double *pre, double *partials,
                                                          const double *scal)
 {
     if (scal[S_DONE] != 0.0) return;
     const double alpha = scal[S_ALPHA];
     double acc[2] = {0, 0};
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * Ap[i];
-        const double zi = dinv[i] * ri;
-        r[i] = ri;
-        z[i] = zi;
-        acc[0] += ri * zi;
-        acc[1] += ri * ri;
+    if (M.blocked) {
+        for (int64_t rn = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; rn < M.n_rownodes; rn += stride) {
+            const int32_t d = M.rn_dof[rn];
+            double rr[3];
+            for (int c = 0; c < 3; ++c) {
+                x[d + c] += alpha * p[d + c];
+                rr[c] = r[d + c] - alpha * Ap[d + c];
+                r[d + c] = rr[c];
+            }
+            double z0, z1, z2;
+            apply_block(pre + 9 * rn, rr[0], rr[1], rr[2], z0, z1, z2);
+            z[d] = z0, z[d + 1] = z1, z[d + 2] = z2;
+            acc[0] += rr[0] * z0 + rr[1] * z1 + rr[2] * z2;
+            acc[1] += rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+        }
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M.n; i += stride) {
+            x[i] += alpha * p[i];
+            const double ri = r[i] - alpha * Ap[i];
+            const double zi = pre[i] * ri;
+            r[i] = ri;
+            z[i] = zi;
+            acc[0] += ri * zi;
+            acc[1] += ri * ri;
+        }
     }
     block_partials<2>(acc, partials, kBlocks);
 }
@@ -239,20 +388,34 @@ __global__ __launch_bounds__(1024) void k_pcg_reduce(int mode, const double *par
     }
 }
 
+Mat make_mat(const ewfe_plan *p)
+{
+    Mat M;
+    M.indptr = p->d_indptr;
+    M.indices = p->d_indices;
+    M.rn_dof = p->d_rownode_dof;
+    M.nb_ptr = p->d_nb_ptr;
+    M.nbcol = p->d_nbcol;
+    M.diag_rank = p->d_diag_rank;
+    M.n = p->n_dof;
+    M.n_rownodes = p->n_rownodes;
+    M.blocked = p->rows_cover_all_dofs && p->d_nbcol != nullptr;
+    return M;
+}
+
 template <typename VT>
 int pcg(const ewfe_plan *p, const VT *vals, const double *b, double *x, const int32_t *rows, int64_t n_rows,
         double rtol, int maxit, int check_every, double *ws, int *iters_out, double *relres_out, hipStream_t s)
 {
+    const Mat M = make_mat(p);
     const int64_t n = p->n_dof;
-    double *r = ws, *z = ws + n, *pp = ws + 2 * n, *Ap = ws + 3 * n, *dinv = ws + 4 * n;
-    double *partials = ws + 5 * n, *scal = partials + 3 * kBlocks;
+    double *r = ws, *z = ws + n, *pp = ws + 2 * n, *Ap = ws + 3 * n, *pre = ws + 4 * n;
+    double *partials = pre + 3 * n, *scal = partials + 3 * kBlocks;
     const double rtol2 = rtol * rtol;
-    hipLaunchKernelGGL(k_pcg_setup<VT>, dim3(kBlocks), dim3(kThreads), 0, s, n, p->d_indptr, p->d_indices, vals,
-                       dinv, x);
-    if (n_rows > 0)
-        hipLaunchKernelGGL(k_set_rows, dim3(256), dim3(256), 0, s, rows, n_rows, b, x);
-    hipLaunchKernelGGL(k_pcg_residual<VT>, dim3(kBlocks), dim3(kThreads), 0, s, n, p->d_indptr, p->d_indices, vals,
-                       b, (const double *)x, (const double *)dinv, r, z, pp, partials);
+    hipLaunchKernelGGL(k_pcg_setup<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, pre, x);
+    if (n_rows > 0) hipLaunchKernelGGL(k_set_rows, dim3(256), dim3(256), 0, s, rows, n_rows, b, x);
+    hipLaunchKernelGGL(k_pcg_residual<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, b, (const double *)x,
+                       (const double *)pre, r, z, pp, partials);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 0, (const double *)partials, scal, rtol2, maxit);
     HIP_CHECK(hipGetLastError());
     double host_scal[S_COUNT];
@@ -262,12 +425,12 @@ int pcg(const ewfe_plan *p, const VT *vals, const double *b, double *x, const in
         HIP_CHECK(hipStreamSynchronize(s));
         if (host_scal[S_DONE] != 0.0 || it >= maxit) break;
         for (int k = 0; k < check_every && it < maxit; ++k, ++it) {
-            hipLaunchKernelGGL(k_pcg_spmv<VT>, dim3(kBlocks), dim3(kThreads), 0, s, n, p->d_indptr, p->d_indices,
-                               vals, (const double *)pp, Ap, partials, (const double *)scal);
+            hipLaunchKernelGGL(k_pcg_spmv<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, (const double *)pp, Ap,
+                               partials, (const double *)scal);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 1, (const double *)partials, scal, rtol2,
                                maxit);
-            hipLaunchKernelGGL(k_pcg_update, dim3(kBlocks), dim3(kThreads), 0, s, n, x, r, z, (const double *)pp,
-                               (const double *)Ap, (const double *)dinv, partials, (const double *)scal);
+            hipLaunchKernelGGL(k_pcg_update, dim3(kBlocks), dim3(kThreads), 0, s, M, x, r, z, (const double *)pp,
+                               (const double *)Ap, (const double *)pre, partials, (const double *)scal);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 2, (const double *)partials, scal, rtol2,
                                maxit);
             hipLaunchKernelGGL(k_pcg_pupdate, dim3(kBlocks), dim3(kThreads), 0, s, n, pp, (const double *)z,
@@ -290,7 +453,7 @@ int pcg(const ewfe_plan *p, const VT *vals, const double *b, double *x, const in
 extern "C" int64_t ewfe_pcg_workspace_size(const ewfe_plan *p)
 {
     if (!p) return 0;
-    return (int64_t)sizeof(double) * (5 * p->n_dof + 3 * kBlocks + S_COUNT);
+    return (int64_t)sizeof(double) * (7 * p->n_dof + 3 * kBlocks + S_COUNT);
 }
 
 extern "C" int ewfe_spmv(const ewfe_plan *p, const void *vals, int vals_dtype, const double *x, double *y,
@@ -301,12 +464,11 @@ extern "C" int ewfe_spmv(const ewfe_plan *p, const void *vals, int vals_dtype, c
         return EWFE_EINVAL;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const Mat M = make_mat(p);
     if (vals_dtype == EWFE_F64)
-        hipLaunchKernelGGL(k_spmv<double>, dim3(kBlocks), dim3(kThreads), 0, s, p->n_dof, p->d_indptr, p->d_indices,
-                           (const double *)vals, x, y);
+        hipLaunchKernelGGL(k_spmv<double>, dim3(kBlocks), dim3(kThreads), 0, s, M, (const double *)vals, x, y);
     else
-        hipLaunchKernelGGL(k_spmv<float>, dim3(kBlocks), dim3(kThreads), 0, s, p->n_dof, p->d_indptr, p->d_indices,
-                           (const float *)vals, x, y);
+        hipLaunchKernelGGL(k_spmv<float>, dim3(kBlocks), dim3(kThreads), 0, s, M, (const float *)vals, x, y);
     HIP_CHECK(hipGetLastError());
     return 0;
 }

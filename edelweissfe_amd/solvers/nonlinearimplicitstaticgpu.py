@@ -51,14 +51,18 @@ DEFAULT_JOB_INFO = {  # edelweissfe/config/phenomena.py:59-93, displacement fiel
 
 
 class _Journal:
-    def __init__(self, verbose=False):
+    """Minimal stand-in for the reference's Journal (ew/journal/journal.py): keeps the lines, optionally
+    prints them (to ``stream``, default stdout)."""
+
+    def __init__(self, verbose=False, stream=None):
         self.verbose = verbose
+        self.stream = stream
         self.lines = []
 
     def message(self, msg, ident="", level=0):
         self.lines.append(msg)
         if self.verbose:
-            print(f"{ident:>20}: {msg}", flush=True)
+            print(f"{ident:>20}: {msg}", file=self.stream, flush=True)
 
     def errorMessage(self, msg, ident=""):
         self.message("ERROR " + msg, ident)
@@ -306,6 +310,8 @@ class NISTGPU:
         it, rel = self.assembly.pcg(A, b, x, rows, rtol=self.options["pcgTolerance"],
                                     maxit=self.options["pcgMaxIter"])
         self.linearSolveIterations.append((it, rel))
+        self.journal.message("linear solve: {:} PCG iterations, relative residual {:.2e}".format(it, rel),
+                             self.identification, 2)
         self.computationTimes["linear solve"] += _time.perf_counter() - tic
         if not np.isfinite(rel) or bool(self.torch.isnan(x).any().item()):
             raise DivergingSolution("Obtained NaN in linear solve")

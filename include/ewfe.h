@@ -21,6 +21,9 @@
  *       edelweissfe/elements/marmotelement/element.pxd:97-103; csrgenerator.pyx:100-115
  *   ewfe_dirichlet_rows
  *       NIST.applyDirichletK (row elimination + unit diagonal): nonlinearimplicitstatic.py:559-593
+ *   ewfe_adjoint_param_grad
+ *       no reference counterpart (adjoint of the per-element parameter field of
+ *       edelweissfe/sections/base/sectionbase.py:86-128)
  *   ewfe_spmv / ewfe_pcg / ewfe_pcg_workspace_size
  *       NIST.linearSolve (nonlinearimplicitstatic.py:727-751) and the linear-solver registry
  *       (edelweissfe/config/linsolve.py:46-86)
@@ -107,6 +110,15 @@ int ewfe_eval_assemble(const ewfe_plan *plan, int mat_kind, const double *mat_pa
 int ewfe_dirichlet_rows(const ewfe_plan *plan, const int32_t *rows_dev, int64_t n_rows, void *csr_vals_dev,
                         int vals_dtype, void *stream);
 
+/* Adjoint sensitivity of a per-element material parameter (no counterpart in the reference, whose solvers
+ * have no adjoint; the per-element parameter mechanism is ew/sections/base/sectionbase.py:86-128):
+ *   grad_dev[e] = lambda^T dR/dtheta_e,  R = P + PExt the Newton residual, theta_e = the element's first
+ *   material property (el_param_dev[e] if given, else mat_params[0]): mu of Neo-Hooke Wa on C3D8TL, E of
+ *   linear elasticity on C3D8. With K lambda = dJ/dU on the free dofs (lambda = 0 on Dirichlet dofs) this
+ *   is dJ/dtheta_e of an objective J(U) at equilibrium. Element kinematics are recomputed. */
+int ewfe_adjoint_param_grad(const ewfe_plan *plan, int mat_kind, const double *mat_params_host,
+                            const double *el_param_dev, const double *X_dev, const double *U_dev,
+                            const double *lambda_dev, double *grad_dev, void *stream);
 
 /* y = K x over the plan's CSR pattern (K in f64 or f32, x and y f64). */
 int ewfe_spmv(const ewfe_plan *plan, const void *csr_vals_dev, int vals_dtype, const double *x_dev, double *y_dev,
