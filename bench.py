@@ -128,6 +128,145 @@ def cpu_baseline(X, conn, U_n, dU, layers, n_per_layer, reps=3):
                        f"{reps} timed evaluations, median ('elements' + 'CSR generation' = {t:.3f} s)")
 
 
+def _timed_evals(asm, U, dU, P, F, K, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        asm.evaluate(U, dU, P, F, K)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        asm.evaluate(U, dU, P, F, K)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def workload_c3d20_tl(args):
+    """Finite-strain total-Lagrangian compressible Neo-Hooke (mu=80, K=200) on 40^3 C3D20 hexahedra (64,000
+    elements, 27 quadrature points, 60 dofs each), nodes jittered by U(-3%, 3%) of h (seed 0): one evaluation +
+    CSR assembly at the state of a 1.25 homogeneous stretch along z (lambda = 1.5 is reached in 20
+    increments; this is increment 10), FP64 and FP32 values."""
+    import torch
+
+    from edelweissfe_amd.assembly import ElementAssembly
+    from edelweissfe_amd.dofmanager import DofManager
+    from edelweissfe_amd.model import ElementBlock, FEModel, Material, boxGenArrays, jitterNodes
+
+    n = args.n if args.n != 128 else 40
+    X, conn, sets, grid = boxGenArrays(n, n, n, elType="C3D20TL")
+    X = jitterNodes(X, grid, (2 * n + 1,) * 3, np.full(3, 1.0 / n), 0.03, 0)
+    m = FEModel()
+    m.addNodes(X)
+    b = ElementBlock("C3D20TL", conn)
+    b.setMaterial(Material("neohookeanwa", [80.0, 200.0]))
+    m.addElementBlock(b, "all")
+    dm = DofManager(m)
+    asm = ElementAssembly(b, dm, m)
+    u = np.zeros_like(X)
+    u[:, 2] = 0.25 * X[:, 2]
+    u[:, 0] = -0.1 * X[:, 0]
+    u[:, 1] = -0.1 * X[:, 1]
+    U = torch.as_tensor(u.ravel(), device="cuda")
+    P, F = dm.constructDofVector(), dm.constructDofVector()
+    K64, K32 = asm.newValues("fp64"), asm.newValues("fp32")
+    ms64 = _timed_evals(asm, U, U, P, F, K64, args.steps, args.warmup)
+    ms32 = _timed_evals(asm, U, U, P, F, K32, args.steps, args.warmup)
+    n_el = len(conn)
+    B64 = 4 * 20 + 120.0 * m.nNodes / n_el + 8.0 * asm.nnz / n_el  # no quadrature-point state
+    B32 = 4 * 20 + 120.0 * m.nNodes / n_el + 4.0 * asm.nnz / n_el
+    ach = B64 * n_el / (ms64 * 1e-3) / 1e9
+    return {
+        "metric": "C3D20-TL Neo-Hooke elem evals+assembly/s", "value": n_el / (ms64 * 1e-3), "unit": "elements/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"C3D20TL Neo-Hooke Wa {n}^3 jittered cube, eval + CSR assembly", "elements": n_el,
+                   "dofs": dm.nDof, "nnz": asm.nnz},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_element": B64,
+                     "scope": "fill + colour launches of k_hex20 (FLOP-bound: dense-per-row tangent)"},
+        "cpu_baseline": None,
+        "fp32_tangent": {"value": n_el / (ms32 * 1e-3), "ms_per_step": ms32,
+                         "algorithmic_bytes_per_element": B32},
+    }
+
+
+def workload_adjoint(args):
+    """48^3 C3D8TL Neo-Hooke block (mu_e = 80 exp(0.2 z_e), z_e ~ N(0,1) seed 1, K=200), bottom (z=0) fixed,
+    top pulled by nodal forces; converged forward solve, then the adjoint gradient dJ/dmu_e of
+    J = 1/2 |u_top - u_target|^2 (u_target from seed 2): tangent at the converged state, PCG for lambda, and
+    the adjoint kernel. Reports the forward solve, the adjoint linear solve and the adjoint kernel times."""
+    import torch
+
+    from edelweissfe_amd.adjoint import parameter_gradient
+    from edelweissfe_amd.model import ElementBlock, FEModel, Material, boxGenArrays, jitterNodes
+    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU, _Journal
+    from edelweissfe_amd.stepping import AdaptiveStep, Dirichlet, NodeForces
+
+    n = args.n if args.n != 128 else 48
+    X, conn, sets, grid = boxGenArrays(n, n, n, elType="C3D8TL")
+    X = jitterNodes(X, grid, (n + 1,) * 3, np.full(3, 1.0 / n), 0.05, 0)
+    mu = 80.0 * np.exp(0.2 * np.random.default_rng(1).standard_normal(len(conn)))
+    m = FEModel()
+    m.addNodes(X)
+    b = ElementBlock("C3D8TL", conn, elParams=mu)
+    b.setMaterial(Material("neohookeanwa", [80.0, 200.0]))
+    m.addElementBlock(b, "all")
+    m.nodeSets.update(sets)
+    top = sets["gen_front"]
+    fz = 8.0 / len(top)  # ~10 % stretch of the unit cube
+    acts = {"dirichlet": {"fix": Dirichlet("fix", m, "gen_back", {0: 0.0, 1: 0.0, 2: 0.0})},
+            "nodeforces": {"pull": NodeForces("pull", m, "gen_front", [0.0, 0.0, fz])}}
+    step = AdaptiveStep(1, m, acts, maxInc=0.5, startInc=0.5, maxIter=20)
+    solver = NISTGPU(None, _Journal(verbose=True, stream=sys.stderr), pcgTolerance=1e-10, fluxResidualFloor=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.solveStep(step, m)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    base = solver.theDofManager.nodeDofBase
+    top_dofs = torch.as_tensor((base[top][:, None] + np.arange(3)).ravel(), device="cuda")
+    target = m.U[top_dofs] * (1 + 0.1 * torch.as_tensor(np.random.default_rng(2).standard_normal(len(top_dofs)),
+                                                        device="cuda"))
+    dJdU = torch.zeros_like(m.U)
+    dJdU[top_dofs] = m.U[top_dofs] - target
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grad, lam, (it, rel) = parameter_gradient(solver, m, dJdU, rtol=1e-10)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    # the adjoint kernel alone
+    from edelweissfe_amd._lib import check, lib, ptr
+
+    asm = solver.assembly
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream = torch.cuda.current_stream()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        check(lib().ewfe_adjoint_param_grad(asm._plan, asm.matKind, ptr(asm.matParams), ptr(asm.elParams),
+                                            ptr(asm.X), ptr(m.U), ptr(lam), ptr(grad), asm._stream()))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_kernel = ev0.elapsed_time(ev1) / args.steps
+    n_el = len(conn)
+    B = 4 * 8 + 72.0 * m.nNodes / n_el + 8.0  # X, U, lambda gathered + one gradient value
+    ach = B * n_el / (ms_kernel * 1e-3) / 1e9
+    return {
+        "metric": "adjoint dJ/dmu_e gradient evaluations/s (C3D8TL Neo-Hooke)", "value": 1e3 / ms_kernel,
+        "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": ms_kernel,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"C3D8TL Neo-Hooke {n}^3, per-element mu field ({n_el} parameters), adjoint kernel",
+                   "forward_solve_s": t_fwd, "adjoint_solve_and_kernel_s": t_adj, "adjoint_pcg_iterations": it,
+                   "adjoint_pcg_relres": rel, "newton_iterations": [len(h) for h in solver.history],
+                   "grad_norm": float(grad.norm().item())},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_element": B, "scope": "k_adjoint8"},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +277,7 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=4)
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--newton-n", type=int, default=None, help="mesh of the Newton increment (default --n)")
+    ap.add_argument("--workload", default="c3d8-j2", choices=["c3d8-j2", "c3d20-nh-tl", "c3d8-nh-adjoint"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -150,6 +290,10 @@ def main():
     from edelweissfe_amd.dofmanager import DofManager
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.workload != "c3d8-j2":
+        fn = workload_c3d20_tl if args.workload == "c3d20-nh-tl" else workload_adjoint
+        print(json.dumps(fn(args)), flush=True)
+        return
     dev = torch.device("cuda:0")
     n = args.n
     t0 = time.time()

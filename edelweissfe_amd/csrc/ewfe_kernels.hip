@@ -227,26 +227,30 @@ __device__ bool j2_recipe(const MatParams &m, double E, const double H[3][3], co
     for (int i = 3; i < 6; ++i) s[i] = r.S[i];
     const double devNorm = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
     const double sq23 = sqrt(2.0 / 3.0), sq6 = sqrt(6.0);
-    const double fy_n = fy0 + HL * kappa + dfy * (1.0 - exp(-delta * kappa));
-    if (!(devNorm - sq23 * fy_n > 0.0)) return true;
+    // the exponential saturation term; identically 0 (and its slope 0) without nonlinear hardening
+    const bool sat = (dfy != 0.0) && (delta != 0.0);
+    auto fy = [&](double k) { return fy0 + HL * k + (sat ? dfy * (1.0 - exp(-delta * k)) : 0.0); };
+    auto dfy_dk = [&](double k) { return HL + (sat ? dfy * delta * exp(-delta * k) : 0.0); };
+    if (!(devNorm - sq23 * fy(kappa) > 0.0)) return true;
     double dK = 0;
     int counter = 0;
     for (;;) {
         const double k = kappa + dK;
-        const double R = devNorm - sq6 * G * dK - sq23 * (fy0 + HL * k + dfy * (1.0 - exp(-delta * k)));
+        const double R = devNorm - sq6 * G * dK - sq23 * fy(k);
         if (!(fabs(R) > 1e-12)) break;
         if (counter == 15) return false;
-        const double dR = -sq6 * G - sq23 * (HL + dfy * delta * exp(-delta * k));
+        const double dR = -sq6 * G - sq23 * dfy_dk(k);
         dK -= R / dR;
         counter++;
     }
     const double dLam = sqrt(1.5) * dK;
     kappa_t = kappa + dK;
-    for (int i = 0; i < 6; ++i) r.N[i] = s[i] / devNorm;
+    const double inv_norm = 1.0 / devNorm;
+    for (int i = 0; i < 6; ++i) r.N[i] = s[i] * inv_norm;
     for (int i = 0; i < 6; ++i) r.S[i] -= 2.0 * G * dLam * r.N[i];
     for (int i = 0; i < 3; ++i) ep_t[i] = ep[i] + dLam * r.N[i];
     for (int i = 3; i < 6; ++i) ep_t[i] = ep[i] + 2.0 * dLam * r.N[i];
-    const double hp = HL + dfy * delta * exp(-delta * kappa_t);
+    const double hp = dfy_dk(kappa_t);
     const double theta = 2.0 * G * dLam / devNorm;
     const double a1 = 2.0 * G * (1.0 / (1.0 + hp / (3.0 * G)) - theta);
     const double a2 = 2.0 * G * theta;
