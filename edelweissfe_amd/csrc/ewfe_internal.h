@@ -35,6 +35,7 @@ struct ewfe_plan {
     int32_t *d_nb_ptr = nullptr;       // [n_rownodes + 1] neighbour-block offsets
     int32_t *d_nbcol = nullptr;        // [n_nbblocks] first dof (column) of each neighbour block, ascending
     int32_t *d_diag_rank = nullptr;    // [n_rownodes] rank of the node itself among its neighbours
+    int32_t *d_rn_meta = nullptr;      // [n_rownodes][4] {first dof, CSR offset of its rows, nb_ptr, degree}
     int32_t *d_inc_ptr = nullptr;      // [n_rownodes + 1]
     int32_t *d_inc = nullptr;          // [n_inc] 8 * element + local node index, ascending element
     uint8_t *d_inc_rank = nullptr;     // [n_inc * 8] rank of element node b among the row node's neighbours
@@ -49,7 +50,7 @@ void set_error(const std::string &msg);
 // Builds pattern, colouring and slot map on the host; fills everything but the device pointers.
 // Returns 0 or a negative error code. edof_host / colpos_host receive the SoA device images.
 struct GatherHost {
-    std::vector<int32_t> rownode_dof, inc_ptr, inc, nb_ptr, nbcol, diag_rank;
+    std::vector<int32_t> rownode_dof, inc_ptr, inc, nb_ptr, nbcol, diag_rank, rn_meta;
     std::vector<uint8_t> inc_rank;
 };
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,

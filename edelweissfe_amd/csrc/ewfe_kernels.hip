@@ -985,6 +985,7 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
             {&p->d_nb_ptr, gh.nb_ptr.data(), sizeof(int32_t) * gh.nb_ptr.size()},
             {&p->d_nbcol, gh.nbcol.data(), sizeof(int32_t) * gh.nbcol.size()},
             {&p->d_diag_rank, gh.diag_rank.data(), sizeof(int32_t) * gh.diag_rank.size()},
+            {&p->d_rn_meta, gh.rn_meta.data(), sizeof(int32_t) * gh.rn_meta.size()},
         };
         for (auto &u : rups) {
             if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
@@ -1072,7 +1073,7 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
     if (!p) return 0;
     void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
                     p->d_rownode_dof, p->d_inc_ptr, p->d_inc,   p->d_inc_rank,  p->d_kbuf,
-                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank};
+                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     delete p;

@@ -222,6 +222,14 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
         gather.nb_ptr[r + 1] = gather.nb_ptr[r] + deg[rownodes[r]];
     }
     plan.n_nbblocks = gather.nb_ptr[nr];
+    gather.rn_meta.resize((size_t)4 * nr);
+    for (int64_t r = 0; r < nr; ++r) {
+        const int32_t d = gather.rownode_dof[r];
+        gather.rn_meta[4 * r + 0] = d;
+        gather.rn_meta[4 * r + 1] = plan.indptr[d];
+        gather.rn_meta[4 * r + 2] = gather.nb_ptr[r];
+        gather.rn_meta[4 * r + 3] = gather.nb_ptr[r + 1] - gather.nb_ptr[r];
+    }
     gather.nbcol.resize(plan.n_nbblocks);
     parallel_for(nr, [&](int64_t b, int64_t e) {
         for (int64_t r = b; r < e; ++r) {

@@ -32,7 +32,7 @@
 
 namespace {
 
-constexpr int kBlocks = 1024;  // fixed grid of the row kernels -> fixed number of partial sums
+constexpr int kBlocks = 4096;  // fixed grid of the row kernels -> fixed number of partial sums
 constexpr int kThreads = 256;
 constexpr int kLanesPerRow = 16;
 constexpr int kRowsPerBlock = kThreads / kLanesPerRow;
@@ -44,6 +44,7 @@ enum Scalar { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BB, S_DONE, S_ITERS, S_F
 struct Mat {
     const int32_t *indptr, *indices;                      // CSR
     const int32_t *rn_dof, *nb_ptr, *nbcol, *diag_rank;  // row nodes
+    const int4 *rn_meta;                                  // {first dof, CSR offset, nb_ptr, degree} per row node
     int64_t n, n_rownodes;
     bool blocked;
 };
@@ -95,10 +96,9 @@ template <typename VT>
 __device__ __forceinline__ void node_dot(const Mat &M, const VT *__restrict__ vals, const double *__restrict__ x,
                                          int64_t rn, int lane32, double &y0, double &y1, double &y2)
 {
-    const int32_t d = M.rn_dof[rn];
-    const int32_t base = M.indptr[d];
-    const int32_t L = M.indptr[d + 1] - base;
-    const int32_t nb0 = M.nb_ptr[rn], deg = M.nb_ptr[rn + 1] - nb0;
+    const int4 meta = M.rn_meta[rn];  // one load instead of a chain rn -> dof -> indptr
+    const int32_t base = meta.y, nb0 = meta.z, deg = meta.w;
+    const int32_t L = 3 * deg;
     double a0 = 0, a1 = 0, a2 = 0;
     for (int m = lane32; m < deg; m += kLanesPerNode) {
         const int32_t c = M.nbcol[nb0 + m];
@@ -397,9 +397,10 @@ Mat make_mat(const ewfe_plan *p)
     M.nb_ptr = p->d_nb_ptr;
     M.nbcol = p->d_nbcol;
     M.diag_rank = p->d_diag_rank;
+    M.rn_meta = reinterpret_cast<const int4 *>(p->d_rn_meta);
     M.n = p->n_dof;
     M.n_rownodes = p->n_rownodes;
-    M.blocked = p->rows_cover_all_dofs && p->d_nbcol != nullptr;
+    M.blocked = p->rows_cover_all_dofs && p->d_nbcol != nullptr && p->d_rn_meta != nullptr;
     return M;
 }
 
