@@ -39,6 +39,10 @@ struct ewfe_plan {
     int32_t *d_inc_ptr = nullptr;      // [n_rownodes + 1]
     int32_t *d_inc = nullptr;          // [n_inc] 8 * element + local node index, ascending element
     uint8_t *d_inc_rank = nullptr;     // [n_inc * 8] rank of element node b among the row node's neighbours
+    // per row node, at byte offset d_cl_ptr[r]: cstart[deg + 1] then 8 * n_inc entries (slot << 3 | b), grouped by
+    // neighbour rank m (entries of m: cstart[m] .. cstart[m + 1]), ascending incident slot inside each group
+    int32_t *d_cl_ptr = nullptr;       // [n_rownodes + 1]
+    uint8_t *d_cl = nullptr;
     // element records (allocated on first evaluation): 36 unique 3x3 blocks per element, and nodal forces
     void *d_kbuf = nullptr;
     int kbuf_dtype = -1;
@@ -51,7 +55,8 @@ void set_error(const std::string &msg);
 // Returns 0 or a negative error code. edof_host / colpos_host receive the SoA device images.
 struct GatherHost {
     std::vector<int32_t> rownode_dof, inc_ptr, inc, nb_ptr, nbcol, diag_rank, rn_meta;
-    std::vector<uint8_t> inc_rank;
+    std::vector<uint8_t> inc_rank, cl;
+    std::vector<int32_t> cl_ptr;
 };
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,
                     std::vector<int32_t> &colpos_host, GatherHost &gather);

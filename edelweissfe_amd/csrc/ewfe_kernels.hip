@@ -592,58 +592,70 @@ constexpr int kMaxDeg8 = 64;
 template <typename VT>
 __global__ __launch_bounds__(64) void k_gather8(const int32_t *__restrict__ rownode_dof,
                                                 const int32_t *__restrict__ inc_ptr, const int32_t *__restrict__ inc,
-                                                const uint8_t *__restrict__ inc_rank, const int32_t *__restrict__ indptr,
-                                                const VT *__restrict__ kbuf, const double *__restrict__ fe,
-                                                VT *__restrict__ vals, double *__restrict__ P, double *__restrict__ F,
-                                                int64_t n_rownodes)
+                                                const int32_t *__restrict__ cl_ptr, const uint8_t *__restrict__ cl,
+                                                const int32_t *__restrict__ indptr, const VT *__restrict__ kbuf,
+                                                const double *__restrict__ fe, VT *__restrict__ vals,
+                                                double *__restrict__ P, double *__restrict__ F, int64_t n_rownodes)
 {
-    __shared__ double acc[3 * 3 * kMaxDeg8];
+    __shared__ double s_blk[64 * 9];  // [slot][b][3x3]: block K_ab oriented for the row node (a)
     __shared__ double s_fe[8 * 3];
+    __shared__ uint8_t s_cl[65 + 8 * 31];
     const int64_t r = blockIdx.x;  // consecutive nodes run concurrently on all XCDs and share records via MALL
     if (r >= n_rownodes) return;   // uniform over the workgroup
     const int lane = threadIdx.x;
     const int32_t d = rownode_dof[r];
     const int32_t base = indptr[d];
     const int32_t L = indptr[d + 1] - base;  // 3 * degree
-    for (int t = lane; t < 3 * L; t += 64) acc[t] = 0.0;
+    const int deg = L / 3;
+    const int32_t k0 = inc_ptr[r], nk = inc_ptr[r + 1] - k0;
+    {  // stage this node's contribution lists
+        const int32_t c0 = cl_ptr[r], nbytes = cl_ptr[r + 1] - c0;
+        for (int t = lane; t < nbytes; t += 64) s_cl[t] = cl[c0 + t];
+    }
+    const int m = lane;  // output: neighbour block of rank m
+    double acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
     double f = 0.0, fa = 0.0;
-    const int32_t k0 = inc_ptr[r], k1 = inc_ptr[r + 1];
-    const int slot = lane >> 3, b = lane & 7;  // lane = (incident slot, element node b)
-    for (int32_t c0 = k0; c0 < k1; c0 += 8) {
-        // every lane fetches one whole 3x3 block K_ab of one incident element: all loads in flight at once
-        const int32_t k = c0 + slot;
-        const bool has = k < k1;
-        double v[9];
-        int rank = 0, a = 0;
-        double fev = 0.0;
-        if (has) {
-            const int32_t code = inc[k];
+    const int slot = lane >> 3, b = lane & 7;
+    __syncthreads();
+    int cur = 0, cend = 0;
+    if (m < deg) {
+        cur = s_cl[m];
+        cend = s_cl[m + 1];
+    }
+    const uint8_t *ent = s_cl + deg + 1;
+    for (int32_t c0 = 0; c0 < nk; c0 += 8) {
+        // 1) every lane fetches one whole block of one incident element (all loads in flight at once)
+        if (c0 + slot < nk) {
+            const int32_t code = inc[k0 + c0 + slot];
             const int64_t e = code >> 3;
-            a = code & 7;
-            rank = inc_rank[(int64_t)k * 8 + b];
+            const int a = code & 7;
             const int lo = a < b ? a : b, hi = a < b ? b : a;
             const VT *blk = kbuf + e * kRec8 + 9 * pair8(lo, hi);
+            double v[9];
 #pragma unroll
-            for (int m = 0; m < 9; ++m) v[m] = (double)blk[m];
-            if (b < 3) fev = fe[e * 24 + 3 * a + b];
+            for (int t = 0; t < 9; ++t) v[t] = (double)blk[t];
+            const bool tr = a > b;  // stored block is K_ba: its transpose is K_ab
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) s_blk[lane * 9 + 3 * i + kk] = tr ? v[3 * kk + i] : v[3 * i + kk];
+            if (b < 3) s_fe[slot * 3 + b] = fe[e * 24 + 3 * a + b];
         }
-        if (b < 3) s_fe[slot * 3 + b] = fev;
         __syncthreads();
-        // ordered accumulation: incident slots in ascending element order (the reference's scatter order)
+        // 2) lane m adds its contributions of this chunk, in ascending element order
+        for (; cur < cend; ++cur) {
+            const int e8 = ent[cur];
+            const int sl = (e8 >> 3) - c0;
+            if (sl >= 8) break;
+            const double *src = s_blk + (sl * 8 + (e8 & 7)) * 9;
 #pragma unroll
-        for (int round = 0; round < 8; ++round) {
-            if (slot == round && has) {
-                const bool tr = a > b;  // stored block is K_ba: read it transposed
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int kk = 0; kk < 3; ++kk) acc[i * L + 3 * rank + kk] += tr ? v[3 * kk + i] : v[3 * i + kk];
-            }
-            __syncthreads();
+            for (int t = 0; t < 9; ++t) acc[t] += src[t];
         }
         if (lane < 3) {
-            const int nk = (k1 - c0) < 8 ? (k1 - c0) : 8;
-            for (int sl = 0; sl < nk; ++sl) {
+            const int nc = (nk - c0) < 8 ? (nk - c0) : 8;
+            for (int sl = 0; sl < nc; ++sl) {
                 const double x = s_fe[sl * 3 + lane];
                 f -= x;
                 fa += fabs(x);
@@ -651,7 +663,12 @@ __global__ __launch_bounds__(64) void k_gather8(const int32_t *__restrict__ rown
         }
         __syncthreads();
     }
-    for (int t = lane; t < 3 * L; t += 64) vals[(int64_t)base + t] = (VT)acc[t];
+    if (m < deg) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) vals[(int64_t)base + i * L + 3 * m + kk] = (VT)acc[3 * i + kk];
+    }
     if (lane < 3) {
         P[d + lane] = f;
         F[d + lane] = fa;
@@ -998,6 +1015,8 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
             {&p->d_inc_ptr, gh.inc_ptr.data(), sizeof(int32_t) * gh.inc_ptr.size()},
             {&p->d_inc, gh.inc.data(), sizeof(int32_t) * gh.inc.size()},
             {(int32_t **)&p->d_inc_rank, gh.inc_rank.data(), gh.inc_rank.size()},
+            {&p->d_cl_ptr, gh.cl_ptr.data(), sizeof(int32_t) * gh.cl_ptr.size()},
+            {(int32_t **)&p->d_cl, gh.cl.data(), gh.cl.size()},
         };
         for (auto &u : gups) {
             if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
@@ -1073,7 +1092,8 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
     if (!p) return 0;
     void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
                     p->d_rownode_dof, p->d_inc_ptr, p->d_inc,   p->d_inc_rank,  p->d_kbuf,
-                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta};
+                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta,
+                    p->d_cl_ptr,   p->d_cl};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -1105,7 +1125,7 @@ static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s)
         hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT>), dim3((unsigned)blocks), dim3(64), 0, s, A, R);
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)p->n_rownodes), dim3(64), 0, s, p->d_rownode_dof,
-                           p->d_inc_ptr, p->d_inc, p->d_inc_rank, p->d_indptr, (const VT *)p->d_kbuf,
+                           p->d_inc_ptr, p->d_inc, p->d_cl_ptr, p->d_cl, p->d_indptr, (const VT *)p->d_kbuf,
                            (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->n_rownodes);
         HIP_CHECK(hipGetLastError());
         return 0;

@@ -274,6 +274,38 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
             set_error("too many elements for the gather index packing");
             return -4;
         }
+        // contribution lists: for every neighbour rank m of the row node, the (slot, b) pairs whose element node
+        // b is that neighbour, in ascending slot (= element) order
+        gather.cl_ptr.assign(nr + 1, 0);
+        for (int64_t r = 0; r < nr; ++r) {
+            const int32_t nk = gather.inc_ptr[r + 1] - gather.inc_ptr[r];
+            const int32_t dg = gather.nb_ptr[r + 1] - gather.nb_ptr[r];
+            if (8 * nk > 255 || nk > 31) {
+                set_error("a node belongs to more than 31 elements; the 8-node gather path supports at most 31");
+                return -1;
+            }
+            gather.cl_ptr[r + 1] = gather.cl_ptr[r] + (dg + 1) + 8 * nk;
+        }
+        gather.cl.resize(gather.cl_ptr[nr]);
+        parallel_for(nr, [&](int64_t b0, int64_t e0) {
+            std::vector<int> cnt;
+            for (int64_t r = b0; r < e0; ++r) {
+                const int32_t nk = gather.inc_ptr[r + 1] - gather.inc_ptr[r];
+                const int32_t dg = gather.nb_ptr[r + 1] - gather.nb_ptr[r];
+                uint8_t *cs = gather.cl.data() + gather.cl_ptr[r];
+                uint8_t *ent = cs + dg + 1;
+                cnt.assign(dg + 1, 0);
+                for (int32_t sl = 0; sl < nk; ++sl)
+                    for (int bb = 0; bb < 8; ++bb) cnt[gather.inc_rank[(size_t)(gather.inc_ptr[r] + sl) * 8 + bb] + 1]++;
+                for (int32_t m = 0; m < dg; ++m) cnt[m + 1] += cnt[m];
+                for (int32_t m = 0; m <= dg; ++m) cs[m] = (uint8_t)cnt[m];
+                for (int32_t sl = 0; sl < nk; ++sl)
+                    for (int bb = 0; bb < 8; ++bb) {
+                        const int m = gather.inc_rank[(size_t)(gather.inc_ptr[r] + sl) * 8 + bb];
+                        ent[cnt[m]++] = (uint8_t)((sl << 3) | bb);
+                    }
+            }
+        });
     }
     return 0;
 }

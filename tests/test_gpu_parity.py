@@ -172,7 +172,7 @@ def _deck_run(name, tangent="fp64"):
     acts["options"]["NISTSolver"] = Options("o", "NISTSolver", extrapolation=st["extrapolation"])
     kw = {k: st[k] for k in ("maxInc", "minInc", "maxNumInc", "maxIter", "stepLength", "startInc") if k in st}
     step = AdaptiveStep(1, model, acts, **kw)
-    solver = NISTGPU(tangentPrecision=tangent, pcgTolerance=1e-13)
+    solver = NISTGPU(tangentPrecision=tangent, pcgTolerance=1e-13, pcgToleranceFP32=1e-8)
     solver.solveStep(step, model)
     return fx, model.U.cpu().numpy(), solver
 
