@@ -128,6 +128,21 @@ def cpu_baseline(X, conn, U_n, dU, layers, n_per_layer, reps=3):
                        f"{reps} timed evaluations, median ('elements' + 'CSR generation' = {t:.3f} s)")
 
 
+def copy_calibration(dev, nbytes=1 << 30):
+    """A fill and a plain device copy of `nbytes` each (torch's elementwise kernels). Under rocprofv3 --pmc their
+    FETCH_SIZE / WRITE_SIZE per dispatch, read against the known byte counts, give the scale that turns the same
+    counters of the assembly kernels into HBM bytes."""
+    import torch
+
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device=dev)
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
+
+
 def _timed_evals(asm, U, dU, P, F, K, steps, warmup):
     import torch
 
@@ -296,6 +311,7 @@ def main():
         return
     dev = torch.device("cuda:0")
     n = args.n
+    copy_calibration(dev)
     t0 = time.time()
     m, b, X, conn = build_workload(n)
     dm = DofManager(m, device="cuda")

@@ -595,13 +595,17 @@ __global__ __launch_bounds__(64) void k_gather8(const int32_t *__restrict__ rown
                                                 const int32_t *__restrict__ cl_ptr, const uint8_t *__restrict__ cl,
                                                 const int32_t *__restrict__ indptr, const VT *__restrict__ kbuf,
                                                 const double *__restrict__ fe, VT *__restrict__ vals,
-                                                double *__restrict__ P, double *__restrict__ F, int64_t n_rownodes)
+                                                double *__restrict__ P, double *__restrict__ F, int64_t n_rownodes,
+                                                int64_t xcd_chunk)
 {
     __shared__ double s_blk[64 * 9];  // [slot][b][3x3]: block K_ab oriented for the row node (a)
     __shared__ double s_fe[8 * 3];
     __shared__ uint8_t s_cl[65 + 8 * 31];
-    const int64_t r = blockIdx.x;  // consecutive nodes run concurrently on all XCDs and share records via MALL
-    if (r >= n_rownodes) return;   // uniform over the workgroup
+    // workgroups are dealt to the 8 XCDs round robin: XCD x takes the contiguous row-node range
+    // [x * xcd_chunk, (x + 1) * xcd_chunk), so nodes that share elements share that XCD's L2
+    // (grid = 8 * xcd_chunk >= n_rownodes: a bijection of [0, grid))
+    const int64_t r = (int64_t)(blockIdx.x & 7) * xcd_chunk + (blockIdx.x >> 3);
+    if (r >= n_rownodes) return;  // uniform over the workgroup
     const int lane = threadIdx.x;
     const int32_t d = rownode_dof[r];
     const int32_t base = indptr[d];
@@ -1124,9 +1128,10 @@ static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s)
         const int64_t blocks = (p->n_el + 7) / 8;
         hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT>), dim3((unsigned)blocks), dim3(64), 0, s, A, R);
         HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)p->n_rownodes), dim3(64), 0, s, p->d_rownode_dof,
+        const int64_t chunk = (p->n_rownodes + 7) / 8;
+        hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, s, p->d_rownode_dof,
                            p->d_inc_ptr, p->d_inc, p->d_cl_ptr, p->d_cl, p->d_indptr, (const VT *)p->d_kbuf,
-                           (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->n_rownodes);
+                           (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->n_rownodes, chunk);
         HIP_CHECK(hipGetLastError());
         return 0;
     }
