@@ -67,9 +67,14 @@ class ElementAssembly:
         block.planOrder = order
         self.pnewdt = torch.empty(1, dtype=torch.float64, device=self.device)
         self._pcg_ws = None
+        self._mg = None
+        self.mgLevels = None
 
     def __del__(self):
         try:
+            if getattr(self, "_mg", None):
+                lib().ewfe_mg_destroy(self._mg)
+                self._mg = None
             if getattr(self, "_plan", None):
                 lib().ewfe_plan_destroy(self._plan)
                 self._plan = None
@@ -124,17 +129,43 @@ class ElementAssembly:
         check(lib().ewfe_spmv(self._plan, ptr(vals), self._dtype_code(vals), ptr(x), ptr(y), self._stream()))
         return y
 
-    def pcg(self, vals, b, x, rows_dev, rtol=1e-10, maxit=20000, check_every=32):
-        """Jacobi-PCG for the row-eliminated system; returns (iterations, relative residual)."""
+    def multigrid(self, degree=2, ratio=20.0):
+        """The smoothed-aggregation hierarchy of this plan (built on first use; patterns only)."""
+        if self._mg is None:
+            mg = ctypes.c_void_p()
+            nl = ctypes.c_int()
+            check(lib().ewfe_mg_create(self._plan, int(degree), float(ratio), ctypes.byref(mg), ctypes.byref(nl)))
+            self._mg = mg
+            sizes = []
+            for lvl in range(nl.value):
+                nn, nb = ctypes.c_int64(), ctypes.c_int64()
+                check(lib().ewfe_mg_level_info(mg, lvl, ctypes.byref(nn), ctypes.byref(nb)))
+                sizes.append((nn.value, nb.value))
+            self.mgLevels = sizes
+        return self._mg
+
+    def pcg(self, vals, b, x, rows_dev, rtol=1e-10, maxit=20000, check_every=32, precond="jacobi"):
+        """PCG for the row-eliminated system, preconditioned by node-block Jacobi (``precond="jacobi"``) or by
+        the smoothed-aggregation V-cycle (``precond="amg"``, set up here for `vals`); returns (iterations,
+        relative residual)."""
         if self._pcg_ws is None:
             nbytes = lib().ewfe_pcg_workspace_size(self._plan)
             self._pcg_ws = self.torch.empty((nbytes + 7) // 8, dtype=self.torch.float64, device=self.device)
         it = ctypes.c_int()
         rr = ctypes.c_double()
         n_rows = 0 if rows_dev is None else rows_dev.numel()
-        check(lib().ewfe_pcg(self._plan, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev), n_rows,
-                             float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws), ctypes.byref(it),
-                             ctypes.byref(rr), self._stream()))
+        if precond == "amg":
+            mg = self.multigrid()
+            check(lib().ewfe_mg_setup(mg, ptr(vals), self._dtype_code(vals), ptr(rows_dev), n_rows, self._stream()))
+            check(lib().ewfe_pcg_mg(mg, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev), n_rows,
+                                    float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws), ctypes.byref(it),
+                                    ctypes.byref(rr), self._stream()))
+        elif precond == "jacobi":
+            check(lib().ewfe_pcg(self._plan, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev),
+                                 n_rows, float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws),
+                                 ctypes.byref(it), ctypes.byref(rr), self._stream()))
+        else:
+            raise ValueError(f"unknown preconditioner {precond!r}")
         return it.value, rr.value
 
     def stateInElementOrder(self, which="trial"):

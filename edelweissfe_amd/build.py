@@ -10,7 +10,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libewfe.so")
-SOURCES = ["ewfe_kernels.hip", "ewfe_solver.hip", "ewfe_plan.cpp"]
+SOURCES = ["ewfe_kernels.hip", "ewfe_solver.hip", "ewfe_amg.hip", "ewfe_plan.cpp", "ewfe_amg_host.cpp"]
 ARCH = os.environ.get("EWFE_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1,0 +1,1042 @@
+// libewfe.so -- smoothed-aggregation multigrid (SA-AMG) preconditioner for the Newton correction K ddU = R.
+//
+// The reference's iterative option is GMRES preconditioned by pyamg's smoothed-aggregation solver
+// (edelweissfe/linsolve/gmres/gmres.py:70-88, registered in edelweissfe/config/linsolve.py:46-86). Here the same
+// method runs on the device as a symmetric V-cycle inside CG (ewfe_pcg_mg in ewfe_solver.hip):
+//   * unknowns grouped by node (3x3 blocks); near-null space = the three translations, so the tentative
+//     prolongator P0 is a block identity per aggregate;
+//   * P = (I - w D^-1 A) P0, w = 4 / (3 lambda_max(D^-1 A)), D = 3x3 block diagonal, lambda_max by power
+//     iteration on the device;
+//   * A_c = P^T (A P), two sparse products with patterns built once per plan on the host
+//     (ewfe_amg_host.cpp), each output block summed by one lane in a fixed order (bitwise reproducible);
+//   * smoother: Chebyshev polynomial of degree nu in D^-1 A on [lambda_max / ratio, 1.1 lambda_max]
+//     (Adams, Brezina, Hu, Tuminaro 2003), the same polynomial before and after the coarse correction, so the
+//     V-cycle is a symmetric positive definite preconditioner;
+//   * coarsest level (<= a few hundred nodes): dense inverse, computed on the host by Cholesky.
+// Dirichlet dofs: the fine matrix has unit rows there (row elimination). The prolongator's rows of Dirichlet
+// dofs are zero (P0 masked, and the masked columns of A dropped), so every level works on the free dofs only;
+// the smoother keeps vectors at zero on Dirichlet dofs because D^-1 has identity rows there.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "ewfe.h"
+#include "ewfe_amg.h"
+#include "ewfe_internal.h"
+
+#define HIP_CHECK(expr)                                                                          \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess) {                                                                  \
+            ewfe::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+            return EWFE_EHIP;                                                                    \
+        }                                                                                        \
+    } while (0)
+
+namespace {
+
+constexpr int kBlocks = 4096;
+constexpr int kThreads = 256;
+constexpr int kGroup = 32;  // lanes per node in the SpMV-type kernels
+constexpr int kNodesPerBlock = kThreads / kGroup;
+constexpr int kPowerIters = 15;
+using ewfe::kAmgMaxP;
+using ewfe::kAmgMaxRow;
+
+struct MgLevel {
+    int64_t n = 0;  // nodes
+    // matrix of levels >= 1 (block CSR, row-major 3x3 blocks)
+    int32_t *aptr = nullptr, *acol = nullptr, *adiag = nullptr;
+    double *aval = nullptr;
+    int64_t nnzb = 0;
+    double *dinv = nullptr;  // [n][9]
+    // transfer to the next level
+    int64_t n_agg = 0;
+    int32_t *pptr = nullptr, *pcol = nullptr;
+    uint8_t *pslot = nullptr;
+    double *pval = nullptr;
+    int32_t *wptr = nullptr, *wcol = nullptr;
+    double *wval = nullptr;
+    int32_t *rptr = nullptr, *ri = nullptr, *rt = nullptr;
+    // V-cycle vectors (3n each); level 0 uses the caller's right-hand side / result for b / x
+    double *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;
+};
+
+}  // namespace
+
+struct ewfe_mg {
+    const ewfe_plan *plan = nullptr;
+    int device = 0;
+    std::vector<MgLevel> lv;
+    // coarsest level: host pattern for the dense assembly, device dense inverse
+    std::vector<int32_t> c_aptr, c_acol;
+    double *ainv = nullptr;
+    int64_t nc3 = 0;
+    double *lam = nullptr;       // [levels] lambda_max(D^-1 A)
+    double *partials = nullptr;  // [kBlocks]
+    uint8_t *mask = nullptr;     // [n_dof] 1 on Dirichlet dofs
+    double *pw_x = nullptr, *pw_y = nullptr;
+    int degree = 2;
+    double ratio = 20.0;
+    bool ready = false;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+__device__ __forceinline__ double group_sum32(double v)
+{
+    v += __shfl_xor(v, 16, 32);
+    v += __shfl_xor(v, 8, 32);
+    v += __shfl_xor(v, 4, 32);
+    v += __shfl_xor(v, 2, 32);
+    v += __shfl_xor(v, 1, 32);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum64(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ void block_partial1(double v, double *partials)
+{
+    __shared__ double red[kThreads / 64];
+    v = wave_sum64(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+        partials[blockIdx.x] = s;
+    }
+}
+
+// ---- level operators --------------------------------------------------------------------------------
+// Fine level: the plan's CSR; row node i owns dofs 3i..3i+2; rn_meta = {first dof, CSR offset, nb_ptr, degree};
+// nbcol = first dof of each neighbour block. Coarse levels: block CSR.
+template <typename VT>
+struct FineOp {
+    const int4 *meta;
+    const int32_t *nbcol, *diag_rank;
+    const VT *vals;
+    const uint8_t *mask;
+    int64_t n;
+};
+
+struct BsrOp {
+    const int32_t *aptr, *acol, *adiag;
+    const double *aval;
+    int64_t n;
+};
+
+template <typename VT>
+__device__ __forceinline__ void row_apply(const FineOp<VT> &A, int64_t i, int lane, const double *__restrict__ x,
+                                          double &y0, double &y1, double &y2)
+{
+    const int4 m4 = A.meta[i];
+    const int32_t base = m4.y, nb0 = m4.z, deg = m4.w, L = 3 * deg;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int m = lane; m < deg; m += kGroup) {
+        const int32_t c = A.nbcol[nb0 + m];
+        const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+        const VT *v0 = A.vals + base + 3 * m;
+        a0 += (double)v0[0] * x0 + (double)v0[1] * x1 + (double)v0[2] * x2;
+        a1 += (double)v0[L] * x0 + (double)v0[L + 1] * x1 + (double)v0[L + 2] * x2;
+        a2 += (double)v0[2 * L] * x0 + (double)v0[2 * L + 1] * x1 + (double)v0[2 * L + 2] * x2;
+    }
+    y0 = group_sum32(a0);
+    y1 = group_sum32(a1);
+    y2 = group_sum32(a2);
+}
+
+__device__ __forceinline__ void row_apply(const BsrOp &A, int64_t i, int lane, const double *__restrict__ x,
+                                          double &y0, double &y1, double &y2)
+{
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int32_t e = A.aptr[i] + lane; e < A.aptr[i + 1]; e += kGroup) {
+        const int32_t c = 3 * A.acol[e];
+        const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+        const double *v = A.aval + 9 * (int64_t)e;
+        a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+        a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+        a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    y0 = group_sum32(a0);
+    y1 = group_sum32(a1);
+    y2 = group_sum32(a2);
+}
+
+__device__ __forceinline__ void apply3(const double *__restrict__ Q, double r0, double r1, double r2, double &z0,
+                                       double &z1, double &z2)
+{
+    z0 = Q[0] * r0 + Q[1] * r1 + Q[2] * r2;
+    z1 = Q[3] * r0 + Q[4] * r1 + Q[5] * r2;
+    z2 = Q[6] * r0 + Q[7] * r1 + Q[8] * r2;
+}
+
+// inverse of a 3x3 diagonal block; a component whose row and column vanish (a coarse dof that only Dirichlet
+// dofs feed) gets a unit diagonal, a singular remainder falls back to point Jacobi
+__device__ void block_inverse(double B[3][3], double *Q)
+{
+    for (int c = 0; c < 3; ++c) {
+        bool zero = true;
+        for (int t = 0; t < 3; ++t) zero = zero && B[c][t] == 0.0 && B[t][c] == 0.0;
+        if (zero) B[c][c] = 1.0;
+    }
+    const double det = B[0][0] * (B[1][1] * B[2][2] - B[1][2] * B[2][1]) -
+                       B[0][1] * (B[1][0] * B[2][2] - B[1][2] * B[2][0]) +
+                       B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0]);
+    if (det != 0.0 && det == det) {
+        const double r = 1.0 / det;
+        Q[0] = (B[1][1] * B[2][2] - B[1][2] * B[2][1]) * r;
+        Q[1] = (B[0][2] * B[2][1] - B[0][1] * B[2][2]) * r;
+        Q[2] = (B[0][1] * B[1][2] - B[0][2] * B[1][1]) * r;
+        Q[3] = (B[1][2] * B[2][0] - B[1][0] * B[2][2]) * r;
+        Q[4] = (B[0][0] * B[2][2] - B[0][2] * B[2][0]) * r;
+        Q[5] = (B[0][2] * B[1][0] - B[0][0] * B[1][2]) * r;
+        Q[6] = (B[1][0] * B[2][1] - B[1][1] * B[2][0]) * r;
+        Q[7] = (B[0][1] * B[2][0] - B[0][0] * B[2][1]) * r;
+        Q[8] = (B[0][0] * B[1][1] - B[0][1] * B[1][0]) * r;
+    } else {
+        for (int t = 0; t < 9; ++t) Q[t] = 0.0;
+        for (int c = 0; c < 3; ++c) Q[4 * c] = B[c][c] != 0.0 ? 1.0 / B[c][c] : 1.0;
+    }
+}
+
+// ---- setup kernels ----------------------------------------------------------------------------------
+__global__ void k_mg_mask(const int32_t *rows, int64_t n_rows, uint8_t *mask)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_rows; i += stride) mask[rows[i]] = 1;
+}
+
+template <typename VT>
+__global__ void k_mg_dinv_fine(FineOp<VT> A, double *dinv)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {
+        const int4 m4 = A.meta[i];
+        const int32_t L = 3 * m4.w, off = m4.y + 3 * A.diag_rank[i];
+        double B[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) B[r][c] = (double)A.vals[off + r * L + c];
+        block_inverse(B, dinv + 9 * i);
+    }
+}
+
+__global__ void k_mg_dinv_bsr(BsrOp A, double *dinv)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {
+        const double *v = A.aval + 9 * (int64_t)A.adiag[i];
+        double B[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) B[r][c] = v[3 * r + c];
+        block_inverse(B, dinv + 9 * i);
+    }
+}
+
+// power iteration for lambda_max(D^-1 A): start vector in [0.5, 1.5) (a fixed hash), zero on Dirichlet dofs
+__global__ void k_mg_power_init(int64_t n3, const uint8_t *mask, double *x)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += stride) {
+        uint64_t h = (uint64_t)k * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 29;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 32;
+        const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+        x[k] = (mask && mask[k]) ? 0.0 : 0.5 + u;
+    }
+}
+
+template <typename OP>
+__global__ __launch_bounds__(kThreads) void k_mg_power(OP A, const double *dinv, const double *x, double *y,
+                                                       double *partials)
+{
+    const int lane = threadIdx.x % kGroup;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kGroup; i < A.n;
+         i += (int64_t)gridDim.x * kNodesPerBlock) {
+        double a0, a1, a2;
+        row_apply(A, i, lane, x, a0, a1, a2);
+        if (lane == 0) {
+            double z0, z1, z2;
+            apply3(dinv + 9 * i, a0, a1, a2, z0, z1, z2);
+            y[3 * i] = z0, y[3 * i + 1] = z1, y[3 * i + 2] = z2;
+            acc += z0 * z0 + z1 * z1 + z2 * z2;
+        }
+    }
+    block_partial1(acc, partials);
+}
+
+__global__ __launch_bounds__(1024) void k_mg_norm(const double *partials, double *out)
+{
+    __shared__ double red[1024];
+    double s = 0;
+    for (int i = threadIdx.x; i < kBlocks; i += blockDim.x) s += partials[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sqrt(red[0]);
+}
+
+__global__ void k_mg_scale(int64_t n3, const double *y, double *x, const double *nrm)
+{
+    const double s = *nrm > 0.0 ? 1.0 / *nrm : 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += stride) x[k] = y[k] * s;
+}
+
+// P = (I - w D^-1 A) P0 for row node i (one 64-lane wave per node). Fine level: columns of Dirichlet dofs are
+// dropped from A and the Dirichlet rows of P0 are zero, so P vanishes on Dirichlet rows.
+template <typename VT>
+__device__ __forceinline__ void load_block_fine(const FineOp<VT> &A, int32_t base, int32_t L, int m, int32_t col,
+                                                double *dst)
+{
+    const VT *v = A.vals + base + 3 * m;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) dst[3 * r + c] = A.mask[col + c] ? 0.0 : (double)v[r * L + c];
+}
+
+template <bool FINE, typename VT>
+__global__ __launch_bounds__(64) void k_mg_prolongator(FineOp<VT> F, BsrOp B, const double *__restrict__ dinv,
+                                                       const int32_t *__restrict__ pptr,
+                                                       const uint8_t *__restrict__ pslot, double *__restrict__ pval,
+                                                       const double *__restrict__ lam)
+{
+    __shared__ double s_a[kAmgMaxRow * 9];
+    __shared__ uint8_t s_slot[kAmgMaxRow];
+    __shared__ double s_s[kAmgMaxP * 9];
+    const int64_t i = blockIdx.x;
+    const int lane = threadIdx.x;
+    int32_t deg, e0, self;
+    if (FINE) {
+        const int4 m4 = F.meta[i];
+        deg = m4.w;
+        e0 = m4.z;
+        self = F.diag_rank[i];
+        for (int m = lane; m < deg; m += 64) load_block_fine(F, m4.y, 3 * deg, m, F.nbcol[e0 + m], s_a + 9 * m);
+    } else {
+        e0 = B.aptr[i];
+        deg = B.aptr[i + 1] - e0;
+        self = B.adiag[i] - e0;
+        for (int t = lane; t < 9 * deg; t += 64) s_a[t] = B.aval[9 * (int64_t)e0 + t];
+    }
+    for (int m = lane; m < deg; m += 64) s_slot[m] = pslot[e0 + m];
+    const int32_t p0 = pptr[i], np = pptr[i + 1] - p0;
+    __syncthreads();
+    // S_t = sum of the (masked) blocks whose column node lies in aggregate t, ascending neighbour order
+    for (int q = lane; q < 9 * np; q += 64) {
+        const int t = q / 9, rc = q % 9;
+        double s = 0.0;
+        for (int m = 0; m < deg; ++m)
+            if (s_slot[m] == t) s += s_a[9 * m + rc];
+        s_s[q] = s;
+    }
+    __syncthreads();
+    const double w = 4.0 / (3.0 * lam[0]);
+    const int tself = s_slot[self];
+    const double *Q = dinv + 9 * i;
+    for (int q = lane; q < 9 * np; q += 64) {
+        const int t = q / 9, r = (q % 9) / 3, c = q % 3;
+        double v = -w * (Q[3 * r] * s_s[9 * t + c] + Q[3 * r + 1] * s_s[9 * t + 3 + c] + Q[3 * r + 2] * s_s[9 * t + 6 + c]);
+        if (t == tself && r == c && !(FINE && F.mask[3 * i + r])) v += 1.0;
+        pval[9 * (int64_t)(p0 + t) + 3 * r + c] = v;
+    }
+}
+
+// W = A P, row node i (one 64-lane wave): lane u owns output block u and sums A_ik P_kJ over the neighbours k in
+// ascending order. The neighbours' P rows are staged in LDS (at most MAXSTAGE blocks, MAXROW neighbours).
+template <bool FINE, typename VT, int MAXROW, int MAXSTAGE>
+__global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32_t *__restrict__ pptr,
+                                              const int32_t *__restrict__ pcol, const double *__restrict__ pval,
+                                              const int32_t *__restrict__ wptr, const int32_t *__restrict__ wcol,
+                                              double *__restrict__ wval)
+{
+    __shared__ double s_a[MAXROW * 9];
+    __shared__ int32_t s_k[MAXROW];
+    __shared__ int32_t s_off[MAXROW + 1];
+    __shared__ int32_t s_pc[MAXSTAGE];
+    __shared__ double s_pv[MAXSTAGE * 9];
+    const int64_t i = blockIdx.x;
+    const int lane = threadIdx.x;
+    int32_t deg;
+    if (FINE) {
+        const int4 m4 = F.meta[i];
+        deg = m4.w;
+        const int32_t L = 3 * deg;
+        for (int m = lane; m < deg; m += 64) {
+            const VT *v = F.vals + m4.y + 3 * m;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) s_a[9 * m + 3 * r + c] = (double)v[r * L + c];
+            const int32_t k = F.nbcol[m4.z + m] / 3;
+            s_k[m] = k;
+            s_off[m + 1] = pptr[k + 1] - pptr[k];
+        }
+    } else {
+        const int32_t e0 = B.aptr[i];
+        deg = B.aptr[i + 1] - e0;
+        for (int t = lane; t < 9 * deg; t += 64) s_a[t] = B.aval[9 * (int64_t)e0 + t];
+        for (int m = lane; m < deg; m += 64) {
+            const int32_t k = B.acol[e0 + m];
+            s_k[m] = k;
+            s_off[m + 1] = pptr[k + 1] - pptr[k];
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        s_off[0] = 0;
+        for (int m = 0; m < deg; ++m) s_off[m + 1] += s_off[m];
+    }
+    __syncthreads();
+    const int32_t total = s_off[deg];
+    for (int32_t g = lane; g < total; g += 64) {  // stage the neighbours' P rows, one block per lane
+        int lo = 0, hi = deg;                      // m with s_off[m] <= g < s_off[m + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        const int32_t t = pptr[s_k[lo]] + (g - s_off[lo]);
+        s_pc[g] = pcol[t];
+        for (int q = 0; q < 9; ++q) s_pv[9 * g + q] = pval[9 * (int64_t)t + q];
+    }
+    __syncthreads();
+    const int32_t w0 = wptr[i], nw = wptr[i + 1] - w0;
+    for (int u = lane; u < nw; u += 64) {
+        const int32_t J = wcol[w0 + u];
+        double acc[9];
+        for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+        for (int m = 0; m < deg; ++m) {
+            for (int32_t o = s_off[m]; o < s_off[m + 1]; ++o) {
+                const int32_t c = s_pc[o];
+                if (c < J) continue;
+                if (c == J) {
+                    const double *a = s_a + 9 * m, *p = s_pv + 9 * o;
+                    for (int r = 0; r < 3; ++r)
+                        for (int cc = 0; cc < 3; ++cc)
+                            acc[3 * r + cc] += a[3 * r] * p[cc] + a[3 * r + 1] * p[3 + cc] + a[3 * r + 2] * p[6 + cc];
+                }
+                break;  // P rows are sorted
+            }
+        }
+        for (int t = 0; t < 9; ++t) wval[9 * (int64_t)(w0 + u) + t] = acc[t];
+    }
+}
+constexpr int kFineMaxRow = 64, kFineMaxStage = 256;      // level 0: LDS ~24 KB per workgroup
+constexpr int kCoarseMaxRow = 128, kCoarseMaxStage = 512;  // levels >= 1: LDS ~49 KB
+
+// A_c = P^T W, aggregate I (one 64-lane wave): lane v owns output block v (column J) and sums P_tI^T W_iJ over
+// the rows i of R(I) in ascending order; W_iJ is found by binary search in W's sorted row.
+__global__ __launch_bounds__(64) void k_mg_rap(int64_t n_agg, const int32_t *__restrict__ rptr,
+                                               const int32_t *__restrict__ ri, const int32_t *__restrict__ rt,
+                                               const double *__restrict__ pval, const int32_t *__restrict__ wptr,
+                                               const int32_t *__restrict__ wcol, const double *__restrict__ wval,
+                                               const int32_t *__restrict__ captr, const int32_t *__restrict__ cacol,
+                                               double *__restrict__ caval)
+{
+    const int64_t I = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int32_t c0 = captr[I], nc = captr[I + 1] - c0;
+    const int32_t q0 = rptr[I], q1 = rptr[I + 1];
+    for (int v = lane; v < nc; v += 64) {
+        const int32_t J = cacol[c0 + v];
+        double acc[9];
+        for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+        for (int32_t q = q0; q < q1; ++q) {
+            const int32_t i = ri[q];
+            int32_t lo = wptr[i], hi = wptr[i + 1];
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (wcol[mid] < J) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo == wptr[i + 1] || wcol[lo] != J) continue;
+            const double *p = pval + 9 * (int64_t)rt[q], *w = wval + 9 * (int64_t)lo;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) acc[3 * r + c] += p[r] * w[c] + p[3 + r] * w[3 + c] + p[6 + r] * w[6 + c];
+        }
+        for (int t = 0; t < 9; ++t) caval[9 * (int64_t)(c0 + v) + t] = acc[t];
+    }
+}
+
+// ---- V-cycle kernels ------------------------------------------------------------------------------------
+struct Cheb {
+    double theta, delta, sigma;
+};
+
+__device__ __forceinline__ Cheb cheb_params(double lam, double ratio)
+{
+    const double hi = 1.1 * lam, lo = hi / ratio;
+    Cheb c;
+    c.theta = 0.5 * (hi + lo);
+    c.delta = 0.5 * (hi - lo);
+    c.sigma = c.theta / c.delta;
+    return c;
+}
+
+// rho_k of the Chebyshev recurrence: rho_0 = 1 / sigma, rho_k = 1 / (2 sigma - rho_{k-1})
+__device__ __forceinline__ double cheb_rho(const Cheb &c, int k)
+{
+    double rho = 1.0 / c.sigma;
+    for (int j = 1; j <= k; ++j) rho = 1.0 / (2.0 * c.sigma - rho);
+    return rho;
+}
+
+// first step from x = 0: r = b, d = D^-1 b / theta, x = d
+__global__ void k_mg_cheb0_zero(int64_t n, const double *dinv, const double *b, double *r, double *d, double *x,
+                                const double *lam, double ratio, const double *done)
+{
+    if (done && *done != 0.0) return;
+    const Cheb c = cheb_params(*lam, ratio);
+    const double s = 1.0 / c.theta;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
+        double z0, z1, z2;
+        apply3(dinv + 9 * i, b0, b1, b2, z0, z1, z2);
+        r[3 * i] = b0, r[3 * i + 1] = b1, r[3 * i + 2] = b2;
+        d[3 * i] = s * z0, d[3 * i + 1] = s * z1, d[3 * i + 2] = s * z2;
+        x[3 * i] = s * z0, x[3 * i + 1] = s * z1, x[3 * i + 2] = s * z2;
+    }
+}
+
+// first step from a given x (r = b - A x already computed): d = D^-1 r / theta, x += d
+__global__ void k_mg_cheb0(int64_t n, const double *dinv, const double *r, double *d, double *x, const double *lam,
+                           double ratio, const double *done)
+{
+    if (done && *done != 0.0) return;
+    const Cheb c = cheb_params(*lam, ratio);
+    const double s = 1.0 / c.theta;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double z0, z1, z2;
+        apply3(dinv + 9 * i, r[3 * i], r[3 * i + 1], r[3 * i + 2], z0, z1, z2);
+        d[3 * i] = s * z0, d[3 * i + 1] = s * z1, d[3 * i + 2] = s * z2;
+        x[3 * i] += s * z0, x[3 * i + 1] += s * z1, x[3 * i + 2] += s * z2;
+    }
+}
+
+// r = b - A x (b may alias r: each node reads only its own entries of b)
+template <typename OP>
+__global__ __launch_bounds__(kThreads) void k_mg_resid(OP A, const double *b, const double *__restrict__ x, double *r,
+                                                       const double *done)
+{
+    if (done && *done != 0.0) return;
+    const int lane = threadIdx.x % kGroup;
+    for (int64_t i = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kGroup; i < A.n;
+         i += (int64_t)gridDim.x * kNodesPerBlock) {
+        double y0, y1, y2;
+        row_apply(A, i, lane, x, y0, y1, y2);
+        if (lane == 0) {
+            const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
+            r[3 * i] = b0 - y0, r[3 * i + 1] = b1 - y1, r[3 * i + 2] = b2 - y2;
+        }
+    }
+}
+
+// Chebyshev step k >= 1: r -= A d_old, d_new = rho_k rho_{k-1} d_old + (2 rho_k / delta) D^-1 r, x += d_new
+template <typename OP>
+__global__ __launch_bounds__(kThreads) void k_mg_cheb_step(OP A, const double *dinv, double *r,
+                                                           const double *__restrict__ d_old, double *d_new, double *x,
+                                                           const double *lam, double ratio, int k, const double *done)
+{
+    if (done && *done != 0.0) return;
+    const Cheb c = cheb_params(*lam, ratio);
+    const double rho_prev = cheb_rho(c, k - 1), rho = cheb_rho(c, k);
+    const double ca = rho * rho_prev, cb = 2.0 * rho / c.delta;
+    const int lane = threadIdx.x % kGroup;
+    for (int64_t i = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kGroup; i < A.n;
+         i += (int64_t)gridDim.x * kNodesPerBlock) {
+        double y0, y1, y2;
+        row_apply(A, i, lane, d_old, y0, y1, y2);
+        if (lane == 0) {
+            const double r0 = r[3 * i] - y0, r1 = r[3 * i + 1] - y1, r2 = r[3 * i + 2] - y2;
+            r[3 * i] = r0, r[3 * i + 1] = r1, r[3 * i + 2] = r2;
+            double z0, z1, z2;
+            apply3(dinv + 9 * i, r0, r1, r2, z0, z1, z2);
+            const double e0 = ca * d_old[3 * i] + cb * z0, e1 = ca * d_old[3 * i + 1] + cb * z1,
+                         e2 = ca * d_old[3 * i + 2] + cb * z2;
+            d_new[3 * i] = e0, d_new[3 * i + 1] = e1, d_new[3 * i + 2] = e2;
+            x[3 * i] += e0, x[3 * i + 1] += e1, x[3 * i + 2] += e2;
+        }
+    }
+}
+
+// b_c[I] = sum over R(I) of P_t^T r[i] (one 64-lane wave per aggregate, fixed-order butterfly)
+__global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t *__restrict__ rptr,
+                                                    const int32_t *__restrict__ ri, const int32_t *__restrict__ rt,
+                                                    const double *__restrict__ pval, const double *__restrict__ r,
+                                                    double *__restrict__ bc, const double *done)
+{
+    if (done && *done != 0.0) return;
+    const int64_t I = blockIdx.x;
+    const int lane = threadIdx.x;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int32_t q = rptr[I] + lane; q < rptr[I + 1]; q += 64) {
+        const int32_t i = ri[q];
+        const double *p = pval + 9 * (int64_t)rt[q];
+        const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+        a0 += p[0] * r0 + p[3] * r1 + p[6] * r2;
+        a1 += p[1] * r0 + p[4] * r1 + p[7] * r2;
+        a2 += p[2] * r0 + p[5] * r1 + p[8] * r2;
+    }
+    a0 = wave_sum64(a0);
+    a1 = wave_sum64(a1);
+    a2 = wave_sum64(a2);
+    if (lane == 0) bc[3 * I] = a0, bc[3 * I + 1] = a1, bc[3 * I + 2] = a2;
+}
+
+// x[i] += sum_t P_t x_c[J_t]
+__global__ void k_mg_prolong(int64_t n, const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
+                             const double *__restrict__ pval, const double *__restrict__ xc, double *__restrict__ x,
+                             const double *done)
+{
+    if (done && *done != 0.0) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (int32_t t = pptr[i]; t < pptr[i + 1]; ++t) {
+            const int32_t J = 3 * pcol[t];
+            const double *p = pval + 9 * (int64_t)t;
+            const double c0 = xc[J], c1 = xc[J + 1], c2 = xc[J + 2];
+            a0 += p[0] * c0 + p[1] * c1 + p[2] * c2;
+            a1 += p[3] * c0 + p[4] * c1 + p[5] * c2;
+            a2 += p[6] * c0 + p[7] * c1 + p[8] * c2;
+        }
+        x[3 * i] += a0, x[3 * i + 1] += a1, x[3 * i + 2] += a2;
+    }
+}
+
+// x = A_c^-1 b (dense, one 64-lane wave per row)
+__global__ __launch_bounds__(64) void k_mg_coarse(int64_t n3, const double *__restrict__ ainv,
+                                                  const double *__restrict__ b, double *__restrict__ x,
+                                                  const double *done)
+{
+    if (done && *done != 0.0) return;
+    const int64_t row = blockIdx.x;
+    double s = 0.0;
+    for (int64_t k = threadIdx.x; k < n3; k += 64) s += ainv[row * n3 + k] * b[k];
+    s = wave_sum64(s);
+    if (threadIdx.x == 0) x[row] = s;
+}
+
+// ---- host helpers ------------------------------------------------------------------------------------
+template <typename T>
+int upload(ewfe_mg *mg, T *&dst, const std::vector<T> &src)
+{
+    dst = nullptr;
+    if (src.empty()) return 0;
+    HIP_CHECK(hipMalloc((void **)&dst, sizeof(T) * src.size()));
+    mg->allocs.push_back((void *)dst);
+    HIP_CHECK(hipMemcpy(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+int alloc(ewfe_mg *mg, T *&dst, size_t count)
+{
+    dst = nullptr;
+    if (count == 0) return 0;
+    HIP_CHECK(hipMalloc((void **)&dst, sizeof(T) * count));
+    mg->allocs.push_back((void *)dst);
+    return 0;
+}
+
+#define RC(expr)                 \
+    do {                         \
+        const int _rc = (expr);  \
+        if (_rc) return _rc;     \
+    } while (0)
+
+template <typename VT>
+FineOp<VT> fine_op(const ewfe_mg *mg, const void *vals)
+{
+    const ewfe_plan *p = mg->plan;
+    FineOp<VT> F;
+    F.meta = reinterpret_cast<const int4 *>(p->d_rn_meta);
+    F.nbcol = p->d_nbcol;
+    F.diag_rank = p->d_diag_rank;
+    F.vals = (const VT *)vals;
+    F.mask = mg->mask;
+    F.n = p->n_rownodes;
+    return F;
+}
+
+BsrOp bsr_op(const MgLevel &L)
+{
+    BsrOp B;
+    B.aptr = L.aptr;
+    B.acol = L.acol;
+    B.adiag = L.adiag;
+    B.aval = L.aval;
+    B.n = L.n;
+    return B;
+}
+
+// lambda_max(D^-1 A_l) into mg->lam[l]
+template <typename OP>
+int power_iteration(ewfe_mg *mg, int l, const OP &A, const double *dinv, const uint8_t *mask, hipStream_t s)
+{
+    const int64_t n3 = 3 * A.n;
+    hipLaunchKernelGGL(k_mg_power_init, dim3(1024), dim3(256), 0, s, n3, mask, mg->pw_x);
+    for (int it = 0; it < kPowerIters; ++it) {
+        hipLaunchKernelGGL((k_mg_power<OP>), dim3(kBlocks), dim3(kThreads), 0, s, A, dinv, (const double *)mg->pw_x,
+                           mg->pw_y, mg->partials);
+        hipLaunchKernelGGL(k_mg_norm, dim3(1), dim3(1024), 0, s, (const double *)mg->partials, mg->lam + l);
+        if (it + 1 < kPowerIters)
+            hipLaunchKernelGGL(k_mg_scale, dim3(1024), dim3(256), 0, s, n3, (const double *)mg->pw_y, mg->pw_x,
+                               (const double *)(mg->lam + l));
+    }
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// dense inverse of the coarsest operator (host, Cholesky; zero rows/columns get a unit diagonal)
+int coarse_inverse(ewfe_mg *mg, hipStream_t s)
+{
+    const MgLevel &C = mg->lv.back();
+    const int64_t n3 = 3 * C.n;
+    std::vector<double> av(9 * (size_t)C.nnzb);
+    HIP_CHECK(hipMemcpyAsync(av.data(), C.aval, sizeof(double) * av.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<double> A((size_t)n3 * n3, 0.0);
+    for (int64_t I = 0; I < C.n; ++I)
+        for (int32_t e = mg->c_aptr[I]; e < mg->c_aptr[I + 1]; ++e) {
+            const int64_t J = mg->c_acol[e];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) A[(3 * I + r) * n3 + 3 * J + c] = av[9 * (size_t)e + 3 * r + c];
+        }
+    for (int64_t k = 0; k < n3; ++k) {
+        bool zero = true;
+        for (int64_t t = 0; t < n3 && zero; ++t) zero = A[k * n3 + t] == 0.0 && A[t * n3 + k] == 0.0;
+        if (zero) A[k * n3 + k] = 1.0;
+    }
+    // symmetrise (A_c is symmetric up to rounding), Cholesky A = L L^T in the lower triangle
+    for (int64_t i = 0; i < n3; ++i)
+        for (int64_t j = 0; j < i; ++j) {
+            const double v = 0.5 * (A[i * n3 + j] + A[j * n3 + i]);
+            A[i * n3 + j] = A[j * n3 + i] = v;
+        }
+    for (int64_t j = 0; j < n3; ++j) {
+        double dg = A[j * n3 + j];
+        for (int64_t k = 0; k < j; ++k) dg -= A[j * n3 + k] * A[j * n3 + k];
+        if (!(dg > 0.0)) {
+            ewfe::set_error("multigrid: coarsest operator is not positive definite");
+            return EWFE_EINVAL;
+        }
+        const double ljj = std::sqrt(dg);
+        A[j * n3 + j] = ljj;
+        for (int64_t i = j + 1; i < n3; ++i) {
+            double v = A[i * n3 + j];
+            for (int64_t k = 0; k < j; ++k) v -= A[i * n3 + k] * A[j * n3 + k];
+            A[i * n3 + j] = v / ljj;
+        }
+    }
+    // inverse = L^-T L^-1, column by column
+    std::vector<double> inv((size_t)n3 * n3), y(n3);
+    for (int64_t col = 0; col < n3; ++col) {
+        for (int64_t i = 0; i < n3; ++i) {  // L y = e_col
+            double v = i == col ? 1.0 : 0.0;
+            for (int64_t k = 0; k < i; ++k) v -= A[i * n3 + k] * y[k];
+            y[i] = v / A[i * n3 + i];
+        }
+        for (int64_t i = n3 - 1; i >= 0; --i) {  // L^T z = y
+            double v = y[i];
+            for (int64_t k = i + 1; k < n3; ++k) v -= A[k * n3 + i] * inv[k * n3 + col];
+            inv[i * n3 + col] = v / A[i * n3 + i];
+        }
+    }
+    HIP_CHECK(hipMemcpyAsync(mg->ainv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+template <typename VT>
+int mg_setup(ewfe_mg *mg, const void *vals, const int32_t *rows, int64_t n_rows, hipStream_t s)
+{
+    const ewfe_plan *p = mg->plan;
+    HIP_CHECK(hipMemsetAsync(mg->mask, 0, (size_t)p->n_dof, s));
+    if (n_rows > 0) hipLaunchKernelGGL(k_mg_mask, dim3(256), dim3(256), 0, s, rows, n_rows, mg->mask);
+    const FineOp<VT> F = fine_op<VT>(mg, vals);
+    const int nl = (int)mg->lv.size();
+    for (int l = 0; l < nl; ++l) {
+        MgLevel &L = mg->lv[l];
+        const BsrOp B = bsr_op(L);
+        if (l == 0) {
+            hipLaunchKernelGGL((k_mg_dinv_fine<VT>), dim3(1024), dim3(256), 0, s, F, L.dinv);
+            RC(power_iteration(mg, 0, F, L.dinv, mg->mask, s));
+        } else {
+            hipLaunchKernelGGL(k_mg_dinv_bsr, dim3(1024), dim3(256), 0, s, B, L.dinv);
+            RC(power_iteration(mg, l, B, L.dinv, (const uint8_t *)nullptr, s));
+        }
+        if (l + 1 == nl) break;
+        MgLevel &N = mg->lv[l + 1];
+        if (l == 0) {
+            hipLaunchKernelGGL((k_mg_prolongator<true, VT>), dim3((unsigned)L.n), dim3(64), 0, s, F, B,
+                               (const double *)L.dinv, (const int32_t *)L.pptr, (const uint8_t *)L.pslot, L.pval,
+                               (const double *)(mg->lam + l));
+            hipLaunchKernelGGL((k_mg_ap<true, VT, kFineMaxRow, kFineMaxStage>), dim3((unsigned)L.n), dim3(64), 0, s, F, B,
+                               (const int32_t *)L.pptr, (const int32_t *)L.pcol, (const double *)L.pval,
+                               (const int32_t *)L.wptr, (const int32_t *)L.wcol, L.wval);
+        } else {
+            hipLaunchKernelGGL((k_mg_prolongator<false, double>), dim3((unsigned)L.n), dim3(64), 0, s,
+                               FineOp<double>{}, B, (const double *)L.dinv, (const int32_t *)L.pptr,
+                               (const uint8_t *)L.pslot, L.pval, (const double *)(mg->lam + l));
+            hipLaunchKernelGGL((k_mg_ap<false, double, kCoarseMaxRow, kCoarseMaxStage>), dim3((unsigned)L.n), dim3(64),
+                               0, s, FineOp<double>{}, B,
+                               (const int32_t *)L.pptr, (const int32_t *)L.pcol, (const double *)L.pval,
+                               (const int32_t *)L.wptr, (const int32_t *)L.wcol, L.wval);
+        }
+        hipLaunchKernelGGL(k_mg_rap, dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
+                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const double *)L.pval,
+                           (const int32_t *)L.wptr, (const int32_t *)L.wcol, (const double *)L.wval,
+                           (const int32_t *)N.aptr, (const int32_t *)N.acol, N.aval);
+        HIP_CHECK(hipGetLastError());
+    }
+    RC(coarse_inverse(mg, s));
+    mg->ready = true;
+    return 0;
+}
+
+// one V-cycle on level l: x_l = M_l^-1 b_l
+template <typename VT>
+int vcycle(ewfe_mg *mg, int l, const void *vals, const double *done, hipStream_t s)
+{
+    MgLevel &L = mg->lv[l];
+    const int nl = (int)mg->lv.size();
+    if (l + 1 == nl) {
+        hipLaunchKernelGGL(k_mg_coarse, dim3((unsigned)(3 * L.n)), dim3(64), 0, s, 3 * L.n, (const double *)mg->ainv,
+                           (const double *)L.b, L.x, done);
+        return 0;
+    }
+    MgLevel &N = mg->lv[l + 1];
+    const double *lam = mg->lam + l;
+    const double ratio = mg->ratio;
+    const int nu = mg->degree;
+    const unsigned ew = 1024;  // grid of the node-wise kernels
+    auto smooth_steps = [&](double *&dcur, double *&dnext, auto &&op) {
+        for (int k = 1; k < nu; ++k) {
+            hipLaunchKernelGGL((k_mg_cheb_step<std::decay_t<decltype(op)>>), dim3(kBlocks), dim3(kThreads), 0, s, op,
+                               (const double *)L.dinv, L.r, (const double *)dcur, dnext, L.x, lam, ratio, k, done);
+            std::swap(dcur, dnext);
+        }
+    };
+    auto level_body = [&](auto &&op) {
+        double *dc = L.d0, *dn = L.d1;
+        // pre-smoothing from x = 0
+        hipLaunchKernelGGL(k_mg_cheb0_zero, dim3(ew), dim3(256), 0, s, L.n, (const double *)L.dinv,
+                           (const double *)L.b, L.r, dc, L.x, lam, ratio, done);
+        smooth_steps(dc, dn, op);
+        // residual of the smoothed iterate: r -= A d_last
+        hipLaunchKernelGGL((k_mg_resid<std::decay_t<decltype(op)>>), dim3(kBlocks), dim3(kThreads), 0, s, op,
+                           (const double *)L.r, (const double *)dc, L.r, done);
+        hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
+                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const double *)L.pval,
+                           (const double *)L.r, N.b, done);
+        return 0;
+    };
+    auto post = [&](auto &&op) {
+        hipLaunchKernelGGL(k_mg_prolong, dim3(ew), dim3(256), 0, s, L.n, (const int32_t *)L.pptr,
+                           (const int32_t *)L.pcol, (const double *)L.pval, (const double *)N.x, L.x, done);
+        double *dc = L.d0, *dn = L.d1;
+        hipLaunchKernelGGL((k_mg_resid<std::decay_t<decltype(op)>>), dim3(kBlocks), dim3(kThreads), 0, s, op,
+                           (const double *)L.b, (const double *)L.x, L.r, done);
+        hipLaunchKernelGGL(k_mg_cheb0, dim3(ew), dim3(256), 0, s, L.n, (const double *)L.dinv, (const double *)L.r,
+                           dc, L.x, lam, ratio, done);
+        smooth_steps(dc, dn, op);
+    };
+    if (l == 0) {
+        const FineOp<VT> F = fine_op<VT>(mg, vals);
+        level_body(F);
+        RC(vcycle<VT>(mg, l + 1, vals, done, s));
+        post(F);
+    } else {
+        const BsrOp B = bsr_op(L);
+        level_body(B);
+        RC(vcycle<VT>(mg, l + 1, vals, done, s));
+        post(B);
+    }
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+namespace ewfe {
+
+int mg_apply(ewfe_mg *mg, const void *vals, int vals_dtype, const double *r, double *z, const double *done,
+             void *stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    MgLevel &L0 = mg->lv[0];
+    L0.b = const_cast<double *>(r);
+    L0.x = z;
+    if (vals_dtype == EWFE_F64) return vcycle<double>(mg, 0, vals, done, s);
+    return vcycle<float>(mg, 0, vals, done, s);
+}
+
+bool mg_ready(const ewfe_mg *mg) { return mg && mg->ready; }
+const ewfe_plan *mg_plan(const ewfe_mg *mg) { return mg ? mg->plan : nullptr; }
+
+}  // namespace ewfe
+
+extern "C" void ewfe_mg_destroy(ewfe_mg *mg)
+{
+    if (!mg) return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(mg->device);
+    for (void *q : mg->allocs) (void)hipFree(q);
+    (void)hipSetDevice(prev);
+    delete mg;
+}
+
+extern "C" int ewfe_mg_create(const ewfe_plan *p, int degree, double ratio, ewfe_mg **out, int *n_levels_out)
+{
+    if (!p || !out || degree < 1 || degree > 8 || !(ratio > 1.0)) {
+        ewfe::set_error("ewfe_mg_create: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    *out = nullptr;
+    if (!p->rows_cover_all_dofs || !p->d_rn_meta || !p->d_nbcol || p->n_rownodes * 3 != p->n_dof) {
+        ewfe::set_error("ewfe_mg_create: the multigrid needs 3 dofs on every node of the plan");
+        return EWFE_EINVAL;
+    }
+    if (p->max_deg > kAmgMaxRow) {
+        ewfe::set_error("ewfe_mg_create: a node has more than " + std::to_string(ewfe::kAmgMaxRow) + " neighbours");
+        return EWFE_EINVAL;
+    }
+    // fine node graph from the plan's neighbour lists (row node r owns dofs 3r .. 3r+2)
+    const int64_t n = p->n_rownodes;
+    std::vector<int32_t> meta(4 * (size_t)n), nbcol(p->n_nbblocks);
+    int prev = 0;
+    HIP_CHECK(hipGetDevice(&prev));
+    HIP_CHECK(hipSetDevice(p->device));
+    HIP_CHECK(hipMemcpy(meta.data(), p->d_rn_meta, sizeof(int32_t) * meta.size(), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(nbcol.data(), p->d_nbcol, sizeof(int32_t) * nbcol.size(), hipMemcpyDeviceToHost));
+    std::vector<int32_t> aptr(n + 1), acol(p->n_nbblocks);
+    for (int64_t i = 0; i < n; ++i) {
+        if (meta[4 * i] != 3 * i) {
+            (void)hipSetDevice(prev);
+            ewfe::set_error("ewfe_mg_create: row nodes do not own consecutive dof triples");
+            return EWFE_EINVAL;
+        }
+        aptr[i] = meta[4 * i + 2];
+        aptr[i + 1] = meta[4 * i + 2] + meta[4 * i + 3];
+        for (int32_t e = aptr[i]; e < aptr[i + 1]; ++e) acol[e] = nbcol[e] / 3;
+    }
+    ewfe::AmgHost H;
+    int rc = ewfe::amg_build_host(n, aptr, acol, 200, 10, H);
+    if (rc) {
+        (void)hipSetDevice(prev);
+        return rc;
+    }
+    ewfe_mg *mg = new ewfe_mg;
+    mg->plan = p;
+    mg->device = p->device;
+    mg->degree = degree;
+    mg->ratio = ratio;
+    auto fail = [&](int code) {
+        ewfe_mg_destroy(mg);
+        (void)hipSetDevice(prev);
+        return code;
+    };
+    const int nl = (int)H.levels.size();
+    mg->lv.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+        const ewfe::AmgLevelHost &h = H.levels[l];
+        MgLevel &L = mg->lv[l];
+        L.n = h.n;
+        L.nnzb = h.aptr[h.n];
+        if (l > 0) {
+            if ((rc = upload(mg, L.aptr, h.aptr)) || (rc = upload(mg, L.acol, h.acol)) ||
+                (rc = upload(mg, L.adiag, h.adiag)) || (rc = alloc(mg, L.aval, 9 * (size_t)L.nnzb)))
+                return fail(rc);
+        }
+        if ((rc = alloc(mg, L.dinv, 9 * (size_t)L.n))) return fail(rc);
+        if (l > 0) {
+            if ((rc = alloc(mg, L.b, 3 * (size_t)L.n)) || (rc = alloc(mg, L.x, 3 * (size_t)L.n))) return fail(rc);
+        }
+        if ((rc = alloc(mg, L.r, 3 * (size_t)L.n)) || (rc = alloc(mg, L.d0, 3 * (size_t)L.n)) ||
+            (rc = alloc(mg, L.d1, 3 * (size_t)L.n)))
+            return fail(rc);
+        if (l + 1 < nl) {
+            L.n_agg = h.n_agg;
+            if ((rc = upload(mg, L.pptr, h.pptr)) || (rc = upload(mg, L.pcol, h.pcol)) ||
+                (rc = upload(mg, L.pslot, h.pslot)) || (rc = alloc(mg, L.pval, 9 * h.pcol.size())) ||
+                (rc = upload(mg, L.wptr, h.wptr)) || (rc = upload(mg, L.wcol, h.wcol)) ||
+                (rc = alloc(mg, L.wval, 9 * h.wcol.size())) || (rc = upload(mg, L.rptr, h.rptr)) ||
+                (rc = upload(mg, L.ri, h.ri)) || (rc = upload(mg, L.rt, h.rt)))
+                return fail(rc);
+            // the LDS staging of k_mg_ap bounds the row length and the neighbours' P rows per row
+            const int64_t max_row = l == 0 ? kFineMaxRow : kCoarseMaxRow;
+            const int64_t max_stage = l == 0 ? kFineMaxStage : kCoarseMaxStage;
+            for (int64_t i = 0; i < h.n; ++i) {
+                int64_t tot = 0;
+                for (int32_t e = h.aptr[i]; e < h.aptr[i + 1]; ++e)
+                    tot += h.pptr[h.acol[e] + 1] - h.pptr[h.acol[e]];
+                if (tot > max_stage || h.aptr[i + 1] - h.aptr[i] > max_row) {
+                    ewfe::set_error("ewfe_mg_create: prolongator rows of a node's neighbours exceed the staging limit");
+                    return fail(EWFE_EINVAL);
+                }
+            }
+        }
+    }
+    mg->c_aptr = H.levels.back().aptr;
+    mg->c_acol = H.levels.back().acol;
+    mg->nc3 = 3 * H.levels.back().n;
+    if ((rc = alloc(mg, mg->ainv, (size_t)mg->nc3 * mg->nc3)) || (rc = alloc(mg, mg->lam, (size_t)nl)) ||
+        (rc = alloc(mg, mg->partials, (size_t)kBlocks)) || (rc = alloc(mg, mg->mask, (size_t)p->n_dof)) ||
+        (rc = alloc(mg, mg->pw_x, 3 * (size_t)n)) || (rc = alloc(mg, mg->pw_y, 3 * (size_t)n)))
+        return fail(rc);
+    (void)hipSetDevice(prev);
+    *out = mg;
+    if (n_levels_out) *n_levels_out = nl;
+    return 0;
+}
+
+extern "C" int ewfe_mg_level_info(const ewfe_mg *mg, int level, int64_t *n_nodes, int64_t *nnz_blocks)
+{
+    if (!mg || level < 0 || level >= (int)mg->lv.size()) {
+        ewfe::set_error("ewfe_mg_level_info: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    if (n_nodes) *n_nodes = mg->lv[level].n;
+    if (nnz_blocks) *nnz_blocks = level == 0 ? mg->plan->n_nbblocks : mg->lv[level].nnzb;
+    return 0;
+}
+
+extern "C" int ewfe_mg_setup(ewfe_mg *mg, const void *vals, int vals_dtype, const int32_t *rows_dev, int64_t n_rows,
+                             void *stream)
+{
+    if (!mg || !vals || (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) || (n_rows > 0 && !rows_dev)) {
+        ewfe::set_error("ewfe_mg_setup: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    mg->ready = false;
+    if (vals_dtype == EWFE_F64) return mg_setup<double>(mg, vals, rows_dev, n_rows, s);
+    return mg_setup<float>(mg, vals, rows_dev, n_rows, s);
+}
+
+extern "C" int ewfe_mg_lambda(const ewfe_mg *mg, double *lam_out, int n)
+{
+    if (!mg || !lam_out || n < (int)mg->lv.size()) {
+        ewfe::set_error("ewfe_mg_lambda: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    HIP_CHECK(hipMemcpy(lam_out, mg->lam, sizeof(double) * mg->lv.size(), hipMemcpyDeviceToHost));
+    return 0;
+}

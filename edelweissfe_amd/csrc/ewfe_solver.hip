@@ -19,6 +19,7 @@
 #include <string>
 
 #include "ewfe.h"
+#include "ewfe_amg.h"
 #include "ewfe_internal.h"
 
 #define HIP_CHECK(expr)                                                                          \
@@ -188,6 +189,12 @@ __global__ __launch_bounds__(kThreads) void k_pcg_setup(Mat M, const VT *vals, d
             pre[r] = (dg != 0.0) ? 1.0 / dg : 1.0;
         }
     }
+}
+
+__global__ void k_fill0(int64_t n, double *x)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = 0.0;
 }
 
 __global__ void k_set_rows(const int32_t *rows, int64_t n_rows, const double *b, double *x)
@@ -388,6 +395,57 @@ __global__ __launch_bounds__(1024) void k_pcg_reduce(int mode, const double *par
     }
 }
 
+// multigrid-preconditioned variant: r = b - K x with partials of (r.r, b.b) at partials + kBlocks
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void k_pcg_residual_nz(Mat M, const VT *vals, const double *b,
+                                                              const double *x, double *r, double *partials)
+{
+    double acc[2] = {0, 0};
+    const int lane32 = threadIdx.x % kLanesPerNode;
+    for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
+         rn += (int64_t)gridDim.x * kNodesPerBlock) {
+        double k0, k1, k2;
+        node_dot(M, vals, x, rn, lane32, k0, k1, k2);
+        if (lane32 == 0) {
+            const int32_t d = M.rn_dof[rn];
+            const double b0 = b[d], b1 = b[d + 1], b2 = b[d + 2];
+            const double r0 = b0 - k0, r1 = b1 - k1, r2 = b2 - k2;
+            r[d] = r0, r[d + 1] = r1, r[d + 2] = r2;
+            acc[0] += r0 * r0 + r1 * r1 + r2 * r2;
+            acc[1] += b0 * b0 + b1 * b1 + b2 * b2;
+        }
+    }
+    block_partials<2>(acc, partials + kBlocks, kBlocks);
+}
+
+// x += alpha p, r -= alpha Ap; partial of r.r at partials + kBlocks
+__global__ __launch_bounds__(kThreads) void k_pcg_update_nz(int64_t n, double *x, double *r, const double *p,
+                                                            const double *Ap, double *partials, const double *scal)
+{
+    if (scal[S_DONE] != 0.0) return;
+    const double alpha = scal[S_ALPHA];
+    double acc[1] = {0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * Ap[i];
+        r[i] = ri;
+        acc[0] += ri * ri;
+    }
+    block_partials<1>(acc, partials + kBlocks, kBlocks);
+}
+
+// partial of r.z at partials (scal may be null: no early exit)
+__global__ __launch_bounds__(kThreads) void k_pcg_dot(int64_t n, const double *r, const double *z, double *partials,
+                                                      const double *scal)
+{
+    if (scal && scal[S_DONE] != 0.0) return;
+    double acc[1] = {0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) acc[0] += r[i] * z[i];
+    block_partials<1>(acc, partials, kBlocks);
+}
+
 Mat make_mat(const ewfe_plan *p)
 {
     Mat M;
@@ -449,7 +507,86 @@ int pcg(const ewfe_plan *p, const VT *vals, const double *b, double *x, const in
     return 0;
 }
 
+template <typename VT>
+int pcg_mg(ewfe_mg *mg, const ewfe_plan *p, int vals_dtype, const VT *vals, const double *b, double *x,
+           const int32_t *rows, int64_t n_rows, double rtol, int maxit, int check_every, double *ws, int *iters_out,
+           double *relres_out, hipStream_t s)
+{
+    const Mat M = make_mat(p);
+    const int64_t n = p->n_dof;
+    double *r = ws, *z = ws + n, *pp = ws + 2 * n, *Ap = ws + 3 * n;
+    double *partials = ws + 7 * n, *scal = partials + 3 * kBlocks;
+    const double rtol2 = rtol * rtol;
+    void *sv = reinterpret_cast<void *>(s);
+    hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, s, n, x);
+    if (n_rows > 0) hipLaunchKernelGGL(k_set_rows, dim3(256), dim3(256), 0, s, rows, n_rows, b, x);
+    hipLaunchKernelGGL(k_pcg_residual_nz<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, b, (const double *)x, r,
+                       partials);
+    int rc = ewfe::mg_apply(mg, vals, vals_dtype, r, z, nullptr, sv);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pcg_dot, dim3(kBlocks), dim3(kThreads), 0, s, n, (const double *)r, (const double *)z,
+                       partials, (const double *)nullptr);
+    hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 0, (const double *)partials, scal, rtol2, maxit);
+    HIP_CHECK(hipMemcpyAsync(pp, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipGetLastError());
+    double host_scal[S_COUNT];
+    int it = 0;
+    for (;;) {
+        HIP_CHECK(hipMemcpyAsync(host_scal, scal, sizeof(host_scal), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (host_scal[S_DONE] != 0.0 || it >= maxit) break;
+        for (int k = 0; k < check_every && it < maxit; ++k, ++it) {
+            hipLaunchKernelGGL(k_pcg_spmv<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, (const double *)pp, Ap,
+                               partials, (const double *)scal);
+            hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 1, (const double *)partials, scal, rtol2,
+                               maxit);
+            hipLaunchKernelGGL(k_pcg_update_nz, dim3(kBlocks), dim3(kThreads), 0, s, n, x, r, (const double *)pp,
+                               (const double *)Ap, partials, (const double *)scal);
+            rc = ewfe::mg_apply(mg, vals, vals_dtype, r, z, scal + S_DONE, sv);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_pcg_dot, dim3(kBlocks), dim3(kThreads), 0, s, n, (const double *)r,
+                               (const double *)z, partials, (const double *)scal);
+            hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 2, (const double *)partials, scal, rtol2,
+                               maxit);
+            hipLaunchKernelGGL(k_pcg_pupdate, dim3(kBlocks), dim3(kThreads), 0, s, n, pp, (const double *)z,
+                               (const double *)scal);
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+    if (iters_out) *iters_out = (int)host_scal[S_ITERS];
+    if (relres_out)
+        *relres_out = host_scal[S_BB] > 0 ? std::sqrt(host_scal[S_RR] / host_scal[S_BB]) : std::sqrt(host_scal[S_RR]);
+    if (host_scal[S_FAIL] != 0.0) {
+        ewfe::set_error("ewfe_pcg_mg: matrix or preconditioner is not positive definite on the free dofs");
+        return EWFE_EINVAL;
+    }
+    return 0;
+}
+
 }  // namespace
+
+extern "C" int ewfe_pcg_mg(ewfe_mg *mg, const void *vals, int vals_dtype, const double *b, double *x,
+                           const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every,
+                           void *workspace, int *iters_out, double *relres_out, void *stream)
+{
+    const ewfe_plan *p = ewfe::mg_plan(mg);
+    if (!p || !vals || !b || !x || !workspace || maxit < 0 || check_every <= 0 ||
+        (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) || !(rtol >= 0.0) || (n_rows > 0 && !rows_dev)) {
+        ewfe::set_error("ewfe_pcg_mg: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    if (!ewfe::mg_ready(mg)) {
+        ewfe::set_error("ewfe_pcg_mg: the multigrid hierarchy is not set up (ewfe_mg_setup)");
+        return EWFE_EINVAL;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    double *ws = reinterpret_cast<double *>(workspace);
+    if (vals_dtype == EWFE_F64)
+        return pcg_mg<double>(mg, p, vals_dtype, (const double *)vals, b, x, rows_dev, n_rows, rtol, maxit,
+                              check_every, ws, iters_out, relres_out, s);
+    return pcg_mg<float>(mg, p, vals_dtype, (const float *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every,
+                         ws, iters_out, relres_out, s);
+}
 
 extern "C" int64_t ewfe_pcg_workspace_size(const ewfe_plan *p)
 {

@@ -20,7 +20,9 @@ Differences that are part of the design (not of the semantics):
   * dof vectors are float64 device tensors; "K" is the CSR value array of the plan's pattern (the VIJ
     array is never materialised), filled by computeElements in the same launch as P and F, so the
     "CSR generation" phase is fused into "elements" and its timer stays 0;
-  * the linear solve is device PCG on the row-eliminated matrix (option ``linsolver="pcg"``), converged to
+  * the linear solve is device PCG on the row-eliminated matrix, preconditioned by node-block Jacobi
+    (option ``linsolver="pcg"``) or by a smoothed-aggregation multigrid V-cycle (``linsolver="amg"``, the
+    device counterpart of the reference's pyamg-preconditioned "gmres" solver), converged to
     ``pcgTolerance`` relative residual; the Newton loop's own flux/correction tolerances decide convergence
     exactly as in the reference;
   * ``tangentPrecision="fp32"`` builds and stores the tangent in FP32 (residual, return mapping and update
@@ -122,7 +124,7 @@ class NISTGPU:
             with open(self.options["linsolverConfigFile"]) as f:
                 cfg = json.load(f)
             self._updateOptions({k: v for k, v in cfg.items() if k in self.NISTOptions}, self.journal)
-        if self.options["linsolver"].lower() != "pcg":
+        if self.options["linsolver"].lower() not in ("pcg", "amg"):
             raise AttributeError("invalid linear solver {:} requested".format(self.options["linsolver"]))
         if len(model.elementBlocks) != 1:
             raise NotImplementedError("NISTGPU assembles one element block (one element type and section) per model")
@@ -312,7 +314,9 @@ class NISTGPU:
         # an FP32 tangent is accurate to ~1e-7: Newton then contracts linearly at a rate set by that error, and
         # solving its system further than pcgToleranceFP32 does not change the Newton iterates' quality
         rtol = self.options["pcgToleranceFP32"] if A.dtype == self.torch.float32 else self.options["pcgTolerance"]
-        it, rel = self.assembly.pcg(A, b, x, rows, rtol=rtol, maxit=self.options["pcgMaxIter"])
+        amg = self.options["linsolver"].lower() == "amg"
+        it, rel = self.assembly.pcg(A, b, x, rows, rtol=rtol, maxit=self.options["pcgMaxIter"],
+                                    check_every=4 if amg else 32, precond="amg" if amg else "jacobi")
         self.linearSolveIterations.append((it, rel))
         self.journal.message("linear solve: {:} PCG iterations, relative residual {:.2e}".format(it, rel),
                              self.identification, 2)

@@ -38,6 +38,7 @@ extern "C" {
 #endif
 
 typedef struct ewfe_plan ewfe_plan;
+typedef struct ewfe_mg ewfe_mg;
 
 enum ewfe_elem_kind { EWFE_C3D8 = 0, EWFE_C3D20 = 1, EWFE_C3D8TL = 2, EWFE_C3D20TL = 3 };
 enum ewfe_mat_kind { EWFE_LINEAR_ELASTIC = 0, EWFE_VON_MISES = 1, EWFE_NEO_HOOKE_WA = 2 };
@@ -136,6 +137,30 @@ int64_t ewfe_pcg_workspace_size(const ewfe_plan *plan);
 int ewfe_pcg(const ewfe_plan *plan, const void *csr_vals_dev, int vals_dtype, const double *b_dev, double *x_dev,
              const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every, void *workspace_dev,
              int *iters_out, double *relres_out, void *stream);
+
+/* Smoothed-aggregation multigrid preconditioner for the same systems (the reference's iterative solver is
+ * GMRES with a pyamg smoothed-aggregation preconditioner, edelweissfe/linsolve/gmres/gmres.py:70-88, selected
+ * by name in edelweissfe/config/linsolve.py:46-86; here: a device-resident SA V-cycle inside CG).
+ * ewfe_mg_create builds the aggregation and every sparsity pattern on the host (once per plan; needs 3 dofs
+ * on every node) and allocates the hierarchy on the plan's device; degree = Chebyshev smoother degree,
+ * ratio = lambda_max / lambda_min of the smoothing interval. The plan must outlive the hierarchy. */
+int ewfe_mg_create(const ewfe_plan *plan, int degree, double ratio, ewfe_mg **mg_out, int *n_levels_out);
+void ewfe_mg_destroy(ewfe_mg *mg);
+int ewfe_mg_level_info(const ewfe_mg *mg, int level, int64_t *n_nodes, int64_t *nnz_blocks);
+
+/* Numeric setup for one matrix (after ewfe_dirichlet_rows with the same rows): block-Jacobi inverses,
+ * lambda_max(D^-1 A) per level by power iteration, prolongators, Galerkin products, dense coarsest inverse
+ * (one host synchronisation). */
+int ewfe_mg_setup(ewfe_mg *mg, const void *csr_vals_dev, int vals_dtype, const int32_t *rows_dev, int64_t n_rows,
+                  void *stream);
+/* lambda_max(D^-1 A) of every level, after ewfe_mg_setup (host array of n >= levels). */
+int ewfe_mg_lambda(const ewfe_mg *mg, double *lambda_out, int n);
+
+/* ewfe_pcg with the multigrid V-cycle as preconditioner; the hierarchy must be set up for csr_vals_dev.
+ * Same arguments, stopping rule, workspace (ewfe_pcg_workspace_size) and errors as ewfe_pcg. */
+int ewfe_pcg_mg(ewfe_mg *mg, const void *csr_vals_dev, int vals_dtype, const double *b_dev, double *x_dev,
+                const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every,
+                void *workspace_dev, int *iters_out, double *relres_out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -156,7 +156,7 @@ def test_pcg_solves_row_eliminated_system():
     assert rel(y.cpu().numpy(), K @ x.cpu().numpy()) < 1e-13
 
 
-def _deck_run(name, tangent="fp64"):
+def _deck_run(name, tangent="fp64", solver_options=None):
     from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
     from edelweissfe_amd.stepping import AdaptiveStep, Dirichlet, NodeForces, Options
 
@@ -172,7 +172,7 @@ def _deck_run(name, tangent="fp64"):
     acts["options"]["NISTSolver"] = Options("o", "NISTSolver", extrapolation=st["extrapolation"])
     kw = {k: st[k] for k in ("maxInc", "minInc", "maxNumInc", "maxIter", "stepLength", "startInc") if k in st}
     step = AdaptiveStep(1, model, acts, **kw)
-    solver = NISTGPU(tangentPrecision=tangent, pcgTolerance=1e-13, pcgToleranceFP32=1e-8)
+    solver = NISTGPU(tangentPrecision=tangent, pcgTolerance=1e-13, pcgToleranceFP32=1e-8, **(solver_options or {}))
     solver.solveStep(step, model)
     return fx, model.U.cpu().numpy(), solver
 
