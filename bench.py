@@ -55,7 +55,7 @@ def algorithmic_bytes_per_element(n_el, n_nodes, nnz, n_state, n_qp, val_bytes, 
             + val_bytes * nnz / n_el)
 
 
-def newton_increment(n, tangent, strain=2e-3, pcg_tol=1e-10, seed=0):
+def newton_increment(n, tangent, strain=2e-3, pcg_tol=1e-10, seed=0, linsolver="amg"):
     """One implicit static increment of the same J2 cube from the virgin state to `strain` (into the plastic
     range), solved by NISTGPU: uz = 0 on z = 0, uz = strain on z = 1, ux = uy = 0 at (0,0,0), uy = 0 at
     (1,0,0). Newton to the reference's criteria |R|inf <= 1e-8 * averaged flux (without the 1e-7 absolute
@@ -79,7 +79,7 @@ def newton_increment(n, tangent, strain=2e-3, pcg_tol=1e-10, seed=0):
     from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import _Journal
 
     solver = NISTGPU(jobInfo, _Journal(verbose=True, stream=sys.stderr), tangentPrecision=tangent,
-                     pcgTolerance=pcg_tol, fluxResidualFloor=0.0, extrapolation="off")
+                     pcgTolerance=pcg_tol, fluxResidualFloor=0.0, extrapolation="off", linsolver=linsolver)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solver.solveStep(step, m)
@@ -89,8 +89,12 @@ def newton_increment(n, tangent, strain=2e-3, pcg_tol=1e-10, seed=0):
     newton_iters = sum(its)  # linear solves = convergence checks of the increments that converged
     pcg_its = [it for it, _ in solver.linearSolveIterations]
     U = m.U.detach().cpu().numpy()
-    return dict(seconds=t, newton_iterations=newton_iters, iterations_per_increment=its, pcg_iterations=pcg_its,
-                ms_per_newton_iteration=1e3 * t / max(1, newton_iters),
+    levels = solver.assembly.mgLevels
+    t_setup = solver.computationTimes["setup"]
+    return dict(seconds=t - t_setup, setup_seconds=t_setup, linsolver=linsolver, multigrid_levels=levels,
+                newton_iterations=newton_iters,
+                iterations_per_increment=its, pcg_iterations=pcg_its,
+                ms_per_newton_iteration=1e3 * (t - t_setup) / max(1, newton_iters),
                 phases_s={k: v for k, v in solver.computationTimes.items() if v > 0},
                 residual_history=[h for h in solver.history], U=U)
 
@@ -292,6 +296,8 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=4)
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--newton-n", type=int, default=None, help="mesh of the Newton increment (default --n)")
+    ap.add_argument("--linsolver", default="amg", choices=["amg", "pcg"],
+                    help="Newton increment's linear solver: multigrid- or node-block-Jacobi-preconditioned CG")
     ap.add_argument("--workload", default="c3d8-j2", choices=["c3d8-j2", "c3d20-nh-tl", "c3d8-nh-adjoint"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -377,7 +383,7 @@ def main():
         nn_ = args.newton_n or n
         res = {}
         for tan in ("fp64", "fp32"):
-            r = newton_increment(nn_, tan)
+            r = newton_increment(nn_, tan, linsolver=args.linsolver)
             res[tan] = r
             print(f"newton {tan}: {r['seconds']:.2f}s, {r['newton_iterations']} iterations, pcg {r['pcg_iterations']}, "
                   f"phases {r['phases_s']}", file=sys.stderr, flush=True)

@@ -144,10 +144,16 @@ class ElementAssembly:
             self.mgLevels = sizes
         return self._mg
 
-    def pcg(self, vals, b, x, rows_dev, rtol=1e-10, maxit=20000, check_every=32, precond="jacobi"):
+    def multigridSetup(self, vals, rows_dev):
+        """Numeric multigrid setup for the row-eliminated matrix `vals` (Dirichlet rows `rows_dev`)."""
+        n_rows = 0 if rows_dev is None else rows_dev.numel()
+        check(lib().ewfe_mg_setup(self.multigrid(), ptr(vals), self._dtype_code(vals), ptr(rows_dev), n_rows,
+                                  self._stream()))
+
+    def pcg(self, vals, b, x, rows_dev, rtol=1e-10, maxit=20000, check_every=32, precond="jacobi", setup=True):
         """PCG for the row-eliminated system, preconditioned by node-block Jacobi (``precond="jacobi"``) or by
-        the smoothed-aggregation V-cycle (``precond="amg"``, set up here for `vals`); returns (iterations,
-        relative residual)."""
+        the smoothed-aggregation V-cycle (``precond="amg"``, set up here for `vals` unless ``setup=False``);
+        returns (iterations, relative residual)."""
         if self._pcg_ws is None:
             nbytes = lib().ewfe_pcg_workspace_size(self._plan)
             self._pcg_ws = self.torch.empty((nbytes + 7) // 8, dtype=self.torch.float64, device=self.device)
@@ -155,8 +161,9 @@ class ElementAssembly:
         rr = ctypes.c_double()
         n_rows = 0 if rows_dev is None else rows_dev.numel()
         if precond == "amg":
+            if setup:
+                self.multigridSetup(vals, rows_dev)
             mg = self.multigrid()
-            check(lib().ewfe_mg_setup(mg, ptr(vals), self._dtype_code(vals), ptr(rows_dev), n_rows, self._stream()))
             check(lib().ewfe_pcg_mg(mg, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev), n_rows,
                                     float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws), ctypes.byref(it),
                                     ctypes.byref(rr), self._stream()))

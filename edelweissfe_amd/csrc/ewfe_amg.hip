@@ -18,6 +18,7 @@
 // the smoother keeps vectors at zero on Dirichlet dofs because D^-1 has identity rows there.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -266,12 +267,12 @@ __global__ __launch_bounds__(kThreads) void k_mg_power(OP A, const double *dinv,
     for (int64_t i = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kGroup; i < A.n;
          i += (int64_t)gridDim.x * kNodesPerBlock) {
         double a0, a1, a2;
-        row_apply(A, i, lane, x, a0, a1, a2);
-        if (lane == 0) {
-            double z0, z1, z2;
-            apply3(dinv + 9 * i, a0, a1, a2, z0, z1, z2);
-            y[3 * i] = z0, y[3 * i + 1] = z1, y[3 * i + 2] = z2;
-            acc += z0 * z0 + z1 * z1 + z2 * z2;
+        row_apply(A, i, lane, x, a0, a1, a2);  // every lane of the group holds the sums
+        if (lane < 3) {                        // lane c: component c
+            const double *Q = dinv + 9 * i + 3 * lane;
+            const double z = Q[0] * a0 + Q[1] * a1 + Q[2] * a2;
+            y[3 * i + lane] = z;
+            acc += z * z;
         }
     }
     block_partial1(acc, partials);
@@ -356,20 +357,36 @@ __global__ __launch_bounds__(64) void k_mg_prolongator(FineOp<VT> F, BsrOp B, co
     }
 }
 
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ int32_t wave_scan64(int32_t v)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
 // W = A P, row node i (one 64-lane wave): lane u owns output block u and sums A_ik P_kJ over the neighbours k in
-// ascending order. The neighbours' P rows are staged in LDS (at most MAXSTAGE blocks, MAXROW neighbours).
-template <bool FINE, typename VT, int MAXROW, int MAXSTAGE>
+// ascending order. The row of A and the neighbours' P column indices are staged in LDS; the P blocks are staged
+// too when STAGE_PV (coarse levels), else read from L2 for the matching entries only (fine level: the small LDS
+// footprint keeps many workgroups per CU in flight). Rows whose neighbours' P rows exceed MAXSTAGE read P from
+// L2 altogether. The grid walks XCD-contiguous node ranges (workgroups are dealt to the 8 XCDs round robin).
+template <bool FINE, typename VT, int MAXROW, int MAXSTAGE, bool STAGE_PV>
 __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32_t *__restrict__ pptr,
                                               const int32_t *__restrict__ pcol, const double *__restrict__ pval,
                                               const int32_t *__restrict__ wptr, const int32_t *__restrict__ wcol,
-                                              double *__restrict__ wval)
+                                              double *__restrict__ wval, int64_t n, int64_t xcd_chunk)
 {
     __shared__ double s_a[MAXROW * 9];
     __shared__ int32_t s_k[MAXROW];
     __shared__ int32_t s_off[MAXROW + 1];
     __shared__ int32_t s_pc[MAXSTAGE];
-    __shared__ double s_pv[MAXSTAGE * 9];
-    const int64_t i = blockIdx.x;
+    __shared__ int32_t s_t[MAXSTAGE];
+    __shared__ double s_pv[STAGE_PV ? MAXSTAGE * 9 : 1];
+    const int64_t i = (int64_t)(blockIdx.x & 7) * xcd_chunk + (blockIdx.x >> 3);
+    if (i >= n) return;  // uniform over the workgroup
     const int lane = threadIdx.x;
     int32_t deg;
     if (FINE) {
@@ -380,27 +397,55 @@ __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32
             const VT *v = F.vals + m4.y + 3 * m;
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) s_a[9 * m + 3 * r + c] = (double)v[r * L + c];
-            const int32_t k = F.nbcol[m4.z + m] / 3;
-            s_k[m] = k;
-            s_off[m + 1] = pptr[k + 1] - pptr[k];
+            s_k[m] = F.nbcol[m4.z + m] / 3;
         }
     } else {
         const int32_t e0 = B.aptr[i];
         deg = B.aptr[i + 1] - e0;
         for (int t = lane; t < 9 * deg; t += 64) s_a[t] = B.aval[9 * (int64_t)e0 + t];
-        for (int m = lane; m < deg; m += 64) {
-            const int32_t k = B.acol[e0 + m];
-            s_k[m] = k;
-            s_off[m + 1] = pptr[k + 1] - pptr[k];
-        }
+        for (int m = lane; m < deg; m += 64) s_k[m] = B.acol[e0 + m];
     }
-    __syncthreads();
-    if (lane == 0) {
-        s_off[0] = 0;
-        for (int m = 0; m < deg; ++m) s_off[m + 1] += s_off[m];
+    // neighbour m's P row length, prefix-summed (rows of up to MAXROW <= 128 neighbours: two wave passes)
+    int32_t carry = 0;
+    if (lane == 0) s_off[0] = 0;
+    for (int m0 = 0; m0 < deg; m0 += 64) {
+        const int m = m0 + lane;
+        int32_t cnt = 0;
+        if (m < deg) {
+            const int32_t k = FINE ? F.nbcol[F.meta[i].z + m] / 3 : B.acol[B.aptr[i] + m];
+            cnt = pptr[k + 1] - pptr[k];
+        }
+        const int32_t incl = wave_scan64(cnt) + carry;
+        if (m < deg) s_off[m + 1] = incl;
+        carry = __shfl(incl, 63, 64);
     }
     __syncthreads();
     const int32_t total = s_off[deg];
+    const int32_t w0 = wptr[i], nw = wptr[i + 1] - w0;
+    if (total > MAXSTAGE) {  // rows beyond the staging capacity read the neighbours' P rows from L2
+        for (int u = lane; u < nw; u += 64) {
+            const int32_t J = wcol[w0 + u];
+            double acc[9];
+            for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+            for (int m = 0; m < deg; ++m) {
+                const int32_t k = s_k[m];
+                for (int32_t t = pptr[k]; t < pptr[k + 1]; ++t) {
+                    const int32_t c = pcol[t];
+                    if (c < J) continue;
+                    if (c == J) {
+                        const double *a = s_a + 9 * m, *p = pval + 9 * (int64_t)t;
+                        for (int r = 0; r < 3; ++r)
+                            for (int cc = 0; cc < 3; ++cc)
+                                acc[3 * r + cc] +=
+                                    a[3 * r] * p[cc] + a[3 * r + 1] * p[3 + cc] + a[3 * r + 2] * p[6 + cc];
+                    }
+                    break;
+                }
+            }
+            for (int t = 0; t < 9; ++t) wval[9 * (int64_t)(w0 + u) + t] = acc[t];
+        }
+        return;  // uniform over the workgroup
+    }
     for (int32_t g = lane; g < total; g += 64) {  // stage the neighbours' P rows, one block per lane
         int lo = 0, hi = deg;                      // m with s_off[m] <= g < s_off[m + 1]
         while (hi - lo > 1) {
@@ -410,10 +455,11 @@ __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32
         }
         const int32_t t = pptr[s_k[lo]] + (g - s_off[lo]);
         s_pc[g] = pcol[t];
-        for (int q = 0; q < 9; ++q) s_pv[9 * g + q] = pval[9 * (int64_t)t + q];
+        s_t[g] = t;
+        if (STAGE_PV)
+            for (int q = 0; q < 9; ++q) s_pv[9 * g + q] = pval[9 * (int64_t)t + q];
     }
     __syncthreads();
-    const int32_t w0 = wptr[i], nw = wptr[i + 1] - w0;
     for (int u = lane; u < nw; u += 64) {
         const int32_t J = wcol[w0 + u];
         double acc[9];
@@ -423,7 +469,8 @@ __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32
                 const int32_t c = s_pc[o];
                 if (c < J) continue;
                 if (c == J) {
-                    const double *a = s_a + 9 * m, *p = s_pv + 9 * o;
+                    const double *a = s_a + 9 * m;
+                    const double *p = STAGE_PV ? s_pv + 9 * o : pval + 9 * (int64_t)s_t[o];
                     for (int r = 0; r < 3; ++r)
                         for (int cc = 0; cc < 3; ++cc)
                             acc[3 * r + cc] += a[3 * r] * p[cc] + a[3 * r + 1] * p[3 + cc] + a[3 * r + 2] * p[6 + cc];
@@ -434,8 +481,8 @@ __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32
         for (int t = 0; t < 9; ++t) wval[9 * (int64_t)(w0 + u) + t] = acc[t];
     }
 }
-constexpr int kFineMaxRow = 64, kFineMaxStage = 256;      // level 0: LDS ~24 KB per workgroup
-constexpr int kCoarseMaxRow = 128, kCoarseMaxStage = 512;  // levels >= 1: LDS ~49 KB
+constexpr int kFineMaxRow = 64, kFineMaxStage = 256;      // level 0: LDS ~7 KB per workgroup
+constexpr int kCoarseMaxRow = 128, kCoarseMaxStage = 512;  // levels >= 1: LDS ~51 KB
 
 // A_c = P^T W, aggregate I (one 64-lane wave): lane v owns output block v (column J) and sums P_tI^T W_iJ over
 // the rows i of R(I) in ascending order; W_iJ is found by binary search in W's sorted row.
@@ -539,10 +586,7 @@ __global__ __launch_bounds__(kThreads) void k_mg_resid(OP A, const double *b, co
          i += (int64_t)gridDim.x * kNodesPerBlock) {
         double y0, y1, y2;
         row_apply(A, i, lane, x, y0, y1, y2);
-        if (lane == 0) {
-            const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
-            r[3 * i] = b0 - y0, r[3 * i + 1] = b1 - y1, r[3 * i + 2] = b2 - y2;
-        }
+        if (lane < 3) r[3 * i + lane] = b[3 * i + lane] - (lane == 0 ? y0 : (lane == 1 ? y1 : y2));
     }
 }
 
@@ -561,15 +605,17 @@ __global__ __launch_bounds__(kThreads) void k_mg_cheb_step(OP A, const double *d
          i += (int64_t)gridDim.x * kNodesPerBlock) {
         double y0, y1, y2;
         row_apply(A, i, lane, d_old, y0, y1, y2);
-        if (lane == 0) {
-            const double r0 = r[3 * i] - y0, r1 = r[3 * i + 1] - y1, r2 = r[3 * i + 2] - y2;
-            r[3 * i] = r0, r[3 * i + 1] = r1, r[3 * i + 2] = r2;
-            double z0, z1, z2;
-            apply3(dinv + 9 * i, r0, r1, r2, z0, z1, z2);
-            const double e0 = ca * d_old[3 * i] + cb * z0, e1 = ca * d_old[3 * i + 1] + cb * z1,
-                         e2 = ca * d_old[3 * i + 2] + cb * z2;
-            d_new[3 * i] = e0, d_new[3 * i + 1] = e1, d_new[3 * i + 2] = e2;
-            x[3 * i] += e0, x[3 * i + 1] += e1, x[3 * i + 2] += e2;
+        // lanes 0..2 each own one component; the 3x3 block-Jacobi product takes the three residual
+        // components from those lanes by shuffles (the other lanes carry copies of component 2)
+        const int c = lane < 3 ? lane : 2;
+        const double rc = r[3 * i + c] - (c == 0 ? y0 : (c == 1 ? y1 : y2));
+        const double r0 = __shfl(rc, 0, kGroup), r1 = __shfl(rc, 1, kGroup), r2 = __shfl(rc, 2, kGroup);
+        if (lane < 3) {
+            const double *Q = dinv + 9 * i + 3 * c;
+            const double e = ca * d_old[3 * i + c] + cb * (Q[0] * r0 + Q[1] * r1 + Q[2] * r2);
+            r[3 * i + c] = rc;
+            d_new[3 * i + c] = e;
+            x[3 * i + c] += e;
         }
     }
 }
@@ -744,20 +790,34 @@ int coarse_inverse(ewfe_mg *mg, hipStream_t s)
             A[i * n3 + j] = v / ljj;
         }
     }
-    // inverse = L^-T L^-1, column by column
-    std::vector<double> inv((size_t)n3 * n3), y(n3);
-    for (int64_t col = 0; col < n3; ++col) {
-        for (int64_t i = 0; i < n3; ++i) {  // L y = e_col
-            double v = i == col ? 1.0 : 0.0;
-            for (int64_t k = 0; k < i; ++k) v -= A[i * n3 + k] * y[k];
-            y[i] = v / A[i * n3 + i];
+    // M = L^-1 (lower triangular), row by row with contiguous row updates: M_i = -(sum_k<i L_ik M_k) / L_ii + e_i/L_ii
+    std::vector<double> Mi((size_t)n3 * n3, 0.0), tmp(n3);
+    for (int64_t i = 0; i < n3; ++i) {
+        std::fill(tmp.begin(), tmp.begin() + i + 1, 0.0);
+        for (int64_t k = 0; k < i; ++k) {
+            const double l = A[i * n3 + k];
+            if (l == 0.0) continue;
+            const double *mk = Mi.data() + k * n3;
+            for (int64_t j = 0; j <= k; ++j) tmp[j] += l * mk[j];
         }
-        for (int64_t i = n3 - 1; i >= 0; --i) {  // L^T z = y
-            double v = y[i];
-            for (int64_t k = i + 1; k < n3; ++k) v -= A[k * n3 + i] * inv[k * n3 + col];
-            inv[i * n3 + col] = v / A[i * n3 + i];
+        const double dinv = 1.0 / A[i * n3 + i];
+        double *mi = Mi.data() + i * n3;
+        for (int64_t j = 0; j < i; ++j) mi[j] = -tmp[j] * dinv;
+        mi[i] = dinv;
+    }
+    // A^-1 = M^T M = sum_k (row k of M)^T (row k of M), lower triangle accumulated, then mirrored
+    std::vector<double> inv((size_t)n3 * n3, 0.0);
+    for (int64_t k = 0; k < n3; ++k) {
+        const double *mk = Mi.data() + k * n3;
+        for (int64_t i = 0; i <= k; ++i) {
+            const double a = mk[i];
+            if (a == 0.0) continue;
+            double *ri = inv.data() + i * n3;
+            for (int64_t j = 0; j <= i; ++j) ri[j] += a * mk[j];
         }
     }
+    for (int64_t i = 0; i < n3; ++i)
+        for (int64_t j = 0; j < i; ++j) inv[j * n3 + i] = inv[i * n3 + j];
     HIP_CHECK(hipMemcpyAsync(mg->ainv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipStreamSynchronize(s));
     return 0;
@@ -787,17 +847,20 @@ int mg_setup(ewfe_mg *mg, const void *vals, const int32_t *rows, int64_t n_rows,
             hipLaunchKernelGGL((k_mg_prolongator<true, VT>), dim3((unsigned)L.n), dim3(64), 0, s, F, B,
                                (const double *)L.dinv, (const int32_t *)L.pptr, (const uint8_t *)L.pslot, L.pval,
                                (const double *)(mg->lam + l));
-            hipLaunchKernelGGL((k_mg_ap<true, VT, kFineMaxRow, kFineMaxStage>), dim3((unsigned)L.n), dim3(64), 0, s, F, B,
-                               (const int32_t *)L.pptr, (const int32_t *)L.pcol, (const double *)L.pval,
-                               (const int32_t *)L.wptr, (const int32_t *)L.wcol, L.wval);
+            const int64_t chunk = (L.n + 7) / 8;
+            hipLaunchKernelGGL((k_mg_ap<true, VT, kFineMaxRow, kFineMaxStage, false>), dim3((unsigned)(8 * chunk)),
+                               dim3(64), 0, s, F, B, (const int32_t *)L.pptr, (const int32_t *)L.pcol,
+                               (const double *)L.pval, (const int32_t *)L.wptr, (const int32_t *)L.wcol, L.wval, L.n,
+                               chunk);
         } else {
             hipLaunchKernelGGL((k_mg_prolongator<false, double>), dim3((unsigned)L.n), dim3(64), 0, s,
                                FineOp<double>{}, B, (const double *)L.dinv, (const int32_t *)L.pptr,
                                (const uint8_t *)L.pslot, L.pval, (const double *)(mg->lam + l));
-            hipLaunchKernelGGL((k_mg_ap<false, double, kCoarseMaxRow, kCoarseMaxStage>), dim3((unsigned)L.n), dim3(64),
-                               0, s, FineOp<double>{}, B,
+            const int64_t chunk = (L.n + 7) / 8;
+            hipLaunchKernelGGL((k_mg_ap<false, double, kCoarseMaxRow, kCoarseMaxStage, true>),
+                               dim3((unsigned)(8 * chunk)), dim3(64), 0, s, FineOp<double>{}, B,
                                (const int32_t *)L.pptr, (const int32_t *)L.pcol, (const double *)L.pval,
-                               (const int32_t *)L.wptr, (const int32_t *)L.wcol, L.wval);
+                               (const int32_t *)L.wptr, (const int32_t *)L.wcol, L.wval, L.n, chunk);
         }
         hipLaunchKernelGGL(k_mg_rap, dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
                            (const int32_t *)L.ri, (const int32_t *)L.rt, (const double *)L.pval,
@@ -980,18 +1043,14 @@ extern "C" int ewfe_mg_create(const ewfe_plan *p, int degree, double ratio, ewfe
                 (rc = alloc(mg, L.wval, 9 * h.wcol.size())) || (rc = upload(mg, L.rptr, h.rptr)) ||
                 (rc = upload(mg, L.ri, h.ri)) || (rc = upload(mg, L.rt, h.rt)))
                 return fail(rc);
-            // the LDS staging of k_mg_ap bounds the row length and the neighbours' P rows per row
+            // k_mg_ap keeps a row of A_l in LDS
             const int64_t max_row = l == 0 ? kFineMaxRow : kCoarseMaxRow;
-            const int64_t max_stage = l == 0 ? kFineMaxStage : kCoarseMaxStage;
-            for (int64_t i = 0; i < h.n; ++i) {
-                int64_t tot = 0;
-                for (int32_t e = h.aptr[i]; e < h.aptr[i + 1]; ++e)
-                    tot += h.pptr[h.acol[e] + 1] - h.pptr[h.acol[e]];
-                if (tot > max_stage || h.aptr[i + 1] - h.aptr[i] > max_row) {
-                    ewfe::set_error("ewfe_mg_create: prolongator rows of a node's neighbours exceed the staging limit");
+            for (int64_t i = 0; i < h.n; ++i)
+                if (h.aptr[i + 1] - h.aptr[i] > max_row) {
+                    ewfe::set_error("ewfe_mg_create: level " + std::to_string(l) + " has a row with more than " +
+                                    std::to_string(max_row) + " blocks");
                     return fail(EWFE_EINVAL);
                 }
-            }
         }
     }
     mg->c_aptr = H.levels.back().aptr;

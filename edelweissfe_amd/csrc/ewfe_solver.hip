@@ -272,10 +272,11 @@ __global__ __launch_bounds__(kThreads) void k_pcg_spmv(Mat M, const VT *vals, co
              rn += (int64_t)gridDim.x * kNodesPerBlock) {
             double y0, y1, y2;
             node_dot(M, vals, p, rn, lane32, y0, y1, y2);
-            if (lane32 == 0) {
-                const int32_t d = M.rn_dof[rn];
-                Ap[d] = y0, Ap[d + 1] = y1, Ap[d + 2] = y2;
-                acc[0] += y0 * p[d] + y1 * p[d + 1] + y2 * p[d + 2];
+            if (lane32 < 3) {  // lane c: component c (every lane of the group holds the sums)
+                const int32_t d = M.rn_dof[rn] + lane32;
+                const double y = lane32 == 0 ? y0 : (lane32 == 1 ? y1 : y2);
+                Ap[d] = y;
+                acc[0] += y * p[d];
             }
         }
     } else {
@@ -406,13 +407,12 @@ __global__ __launch_bounds__(kThreads) void k_pcg_residual_nz(Mat M, const VT *v
          rn += (int64_t)gridDim.x * kNodesPerBlock) {
         double k0, k1, k2;
         node_dot(M, vals, x, rn, lane32, k0, k1, k2);
-        if (lane32 == 0) {
-            const int32_t d = M.rn_dof[rn];
-            const double b0 = b[d], b1 = b[d + 1], b2 = b[d + 2];
-            const double r0 = b0 - k0, r1 = b1 - k1, r2 = b2 - k2;
-            r[d] = r0, r[d + 1] = r1, r[d + 2] = r2;
-            acc[0] += r0 * r0 + r1 * r1 + r2 * r2;
-            acc[1] += b0 * b0 + b1 * b1 + b2 * b2;
+        if (lane32 < 3) {
+            const int32_t d = M.rn_dof[rn] + lane32;
+            const double bc = b[d], rc = bc - (lane32 == 0 ? k0 : (lane32 == 1 ? k1 : k2));
+            r[d] = rc;
+            acc[0] += rc * rc;
+            acc[1] += bc * bc;
         }
     }
     block_partials<2>(acc, partials + kBlocks, kBlocks);

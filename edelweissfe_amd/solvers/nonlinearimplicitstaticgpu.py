@@ -129,6 +129,7 @@ class NISTGPU:
         if len(model.elementBlocks) != 1:
             raise NotImplementedError("NISTGPU assembles one element block (one element type and section) per model")
         device = self.options["device"]
+        tic_setup = _time.perf_counter()
         self.theDofManager = DofManager(model, device=device)
         self.journal.message("total size of eq. system: {:}".format(self.theDofManager.nDof), self.identification)
         block = model.elementBlocks[0]
@@ -137,7 +138,11 @@ class NISTGPU:
         if id(block) not in self.assemblies or self.assemblies[id(block)].nDof != self.theDofManager.nDof:
             self.assemblies[id(block)] = ElementAssembly(block, self.theDofManager, model, device=device)
         self.assembly = self.assemblies[id(block)]
-        self.computationTimes = dict.fromkeys(TIMING_KEYS, 0.0)
+        if self.options["linsolver"].lower() == "amg":
+            self.assembly.multigrid()  # aggregation and sparsity patterns: once per mesh
+        self.computationTimes = dict.fromkeys(TIMING_KEYS + ["multigrid setup", "setup"], 0.0)
+        # dof numbering, CSR pattern, element plan, multigrid patterns: once per step (mesh), not per increment
+        self.computationTimes["setup"] = _time.perf_counter() - tic_setup
         K = self.assembly.newValues("fp32" if self.options["tangentPrecision"] == "fp32" else "fp64")
 
         extrapolation = self.options["extrapolation"]
@@ -311,12 +316,22 @@ class NISTGPU:
         tic = _time.perf_counter()
         x = self.theDofManager.constructDofVector()
         rows = self._allRows32(self._activeDirichlets) if self._activeDirichlets else None
+        if not bool(b.any().item()):  # zero right-hand side (e.g. the zero-size first increment): x = 0
+            self.linearSolveIterations.append((0, 0.0))
+            self.computationTimes["linear solve"] += _time.perf_counter() - tic
+            return x
         # an FP32 tangent is accurate to ~1e-7: Newton then contracts linearly at a rate set by that error, and
         # solving its system further than pcgToleranceFP32 does not change the Newton iterates' quality
         rtol = self.options["pcgToleranceFP32"] if A.dtype == self.torch.float32 else self.options["pcgTolerance"]
         amg = self.options["linsolver"].lower() == "amg"
+        if amg:
+            self.assembly.multigridSetup(A, rows)
+            self.torch.cuda.synchronize(self.assembly.device)
+            toc = _time.perf_counter()
+            self.computationTimes["multigrid setup"] += toc - tic
+            tic = toc
         it, rel = self.assembly.pcg(A, b, x, rows, rtol=rtol, maxit=self.options["pcgMaxIter"],
-                                    check_every=4 if amg else 32, precond="amg" if amg else "jacobi")
+                                    check_every=4 if amg else 32, precond="amg" if amg else "jacobi", setup=False)
         self.linearSolveIterations.append((it, rel))
         self.journal.message("linear solve: {:} PCG iterations, relative residual {:.2e}".format(it, rel),
                              self.identification, 2)
