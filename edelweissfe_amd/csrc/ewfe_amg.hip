@@ -12,7 +12,10 @@
 //   * smoother: Chebyshev polynomial of degree nu in D^-1 A on [lambda_max / ratio, 1.1 lambda_max]
 //     (Adams, Brezina, Hu, Tuminaro 2003), the same polynomial before and after the coarse correction, so the
 //     V-cycle is a symmetric positive definite preconditioner;
-//   * coarsest level (<= a few hundred nodes): dense inverse, computed on the host by Cholesky.
+//   * coarsest level (<= a few hundred nodes): dense inverse, computed on the host by Cholesky;
+//   * the prolongators are stored in the precision of the fine matrix (FP32 for an FP32 tangent: the
+//     preconditioner stays a fixed symmetric positive definite operator, at half the transfer traffic);
+//     Galerkin products, coarse operators, smoother inverses and all vectors are FP64.
 // Dirichlet dofs: the fine matrix has unit rows there (row elimination). The prolongator's rows of Dirichlet
 // dofs are zero (P0 masked, and the masked columns of A dropped), so every level works on the free dofs only;
 // the smoother keeps vectors at zero on Dirichlet dofs because D^-1 has identity rows there.
@@ -85,6 +88,7 @@ struct ewfe_mg {
     int degree = 2;
     double ratio = 20.0;
     bool ready = false;
+    int setup_dtype = -1;  // precision of the matrix the hierarchy was set up for (= prolongator storage)
     std::vector<void *> allocs;
 };
 
@@ -310,10 +314,10 @@ __device__ __forceinline__ void load_block_fine(const FineOp<VT> &A, int32_t bas
         for (int c = 0; c < 3; ++c) dst[3 * r + c] = A.mask[col + c] ? 0.0 : (double)v[r * L + c];
 }
 
-template <bool FINE, typename VT>
+template <bool FINE, typename VT, typename PT>
 __global__ __launch_bounds__(64) void k_mg_prolongator(FineOp<VT> F, BsrOp B, const double *__restrict__ dinv,
                                                        const int32_t *__restrict__ pptr,
-                                                       const uint8_t *__restrict__ pslot, double *__restrict__ pval,
+                                                       const uint8_t *__restrict__ pslot, PT *__restrict__ pval,
                                                        const double *__restrict__ lam)
 {
     __shared__ double s_a[kAmgMaxRow * 9];
@@ -353,7 +357,7 @@ __global__ __launch_bounds__(64) void k_mg_prolongator(FineOp<VT> F, BsrOp B, co
         const int t = q / 9, r = (q % 9) / 3, c = q % 3;
         double v = -w * (Q[3 * r] * s_s[9 * t + c] + Q[3 * r + 1] * s_s[9 * t + 3 + c] + Q[3 * r + 2] * s_s[9 * t + 6 + c]);
         if (t == tself && r == c && !(FINE && F.mask[3 * i + r])) v += 1.0;
-        pval[9 * (int64_t)(p0 + t) + 3 * r + c] = v;
+        pval[9 * (int64_t)(p0 + t) + 3 * r + c] = (PT)v;
     }
 }
 
@@ -373,9 +377,9 @@ __device__ __forceinline__ int32_t wave_scan64(int32_t v)
 // too when STAGE_PV (coarse levels), else read from L2 for the matching entries only (fine level: the small LDS
 // footprint keeps many workgroups per CU in flight). Rows whose neighbours' P rows exceed MAXSTAGE read P from
 // L2 altogether. The grid walks XCD-contiguous node ranges (workgroups are dealt to the 8 XCDs round robin).
-template <bool FINE, typename VT, int MAXROW, int MAXSTAGE, bool STAGE_PV>
+template <bool FINE, typename VT, typename PT, int MAXROW, int MAXSTAGE, bool STAGE_PV>
 __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32_t *__restrict__ pptr,
-                                              const int32_t *__restrict__ pcol, const double *__restrict__ pval,
+                                              const int32_t *__restrict__ pcol, const PT *__restrict__ pval,
                                               const int32_t *__restrict__ wptr, const int32_t *__restrict__ wcol,
                                               double *__restrict__ wval, int64_t n, int64_t xcd_chunk)
 {
@@ -433,11 +437,12 @@ __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32
                     const int32_t c = pcol[t];
                     if (c < J) continue;
                     if (c == J) {
-                        const double *a = s_a + 9 * m, *p = pval + 9 * (int64_t)t;
+                        const double *a = s_a + 9 * m;
+                        const PT *p = pval + 9 * (int64_t)t;
                         for (int r = 0; r < 3; ++r)
                             for (int cc = 0; cc < 3; ++cc)
-                                acc[3 * r + cc] +=
-                                    a[3 * r] * p[cc] + a[3 * r + 1] * p[3 + cc] + a[3 * r + 2] * p[6 + cc];
+                                acc[3 * r + cc] += a[3 * r] * (double)p[cc] + a[3 * r + 1] * (double)p[3 + cc] +
+                                                   a[3 * r + 2] * (double)p[6 + cc];
                     }
                     break;
                 }
@@ -457,7 +462,7 @@ __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32
         s_pc[g] = pcol[t];
         s_t[g] = t;
         if (STAGE_PV)
-            for (int q = 0; q < 9; ++q) s_pv[9 * g + q] = pval[9 * (int64_t)t + q];
+            for (int q = 0; q < 9; ++q) s_pv[9 * g + q] = (double)pval[9 * (int64_t)t + q];
     }
     __syncthreads();
     for (int u = lane; u < nw; u += 64) {
@@ -470,7 +475,8 @@ __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32
                 if (c < J) continue;
                 if (c == J) {
                     const double *a = s_a + 9 * m;
-                    const double *p = STAGE_PV ? s_pv + 9 * o : pval + 9 * (int64_t)s_t[o];
+                    double p[9];
+                    for (int q = 0; q < 9; ++q) p[q] = STAGE_PV ? s_pv[9 * o + q] : (double)pval[9 * (int64_t)s_t[o] + q];
                     for (int r = 0; r < 3; ++r)
                         for (int cc = 0; cc < 3; ++cc)
                             acc[3 * r + cc] += a[3 * r] * p[cc] + a[3 * r + 1] * p[3 + cc] + a[3 * r + 2] * p[6 + cc];
@@ -486,9 +492,10 @@ constexpr int kCoarseMaxRow = 128, kCoarseMaxStage = 512;  // levels >= 1: LDS ~
 
 // A_c = P^T W, aggregate I (one 64-lane wave): lane v owns output block v (column J) and sums P_tI^T W_iJ over
 // the rows i of R(I) in ascending order; W_iJ is found by binary search in W's sorted row.
+template <typename PT>
 __global__ __launch_bounds__(64) void k_mg_rap(int64_t n_agg, const int32_t *__restrict__ rptr,
                                                const int32_t *__restrict__ ri, const int32_t *__restrict__ rt,
-                                               const double *__restrict__ pval, const int32_t *__restrict__ wptr,
+                                               const PT *__restrict__ pval, const int32_t *__restrict__ wptr,
                                                const int32_t *__restrict__ wcol, const double *__restrict__ wval,
                                                const int32_t *__restrict__ captr, const int32_t *__restrict__ cacol,
                                                double *__restrict__ caval)
@@ -510,9 +517,11 @@ __global__ __launch_bounds__(64) void k_mg_rap(int64_t n_agg, const int32_t *__r
                 else hi = mid;
             }
             if (lo == wptr[i + 1] || wcol[lo] != J) continue;
-            const double *p = pval + 9 * (int64_t)rt[q], *w = wval + 9 * (int64_t)lo;
+            const PT *p = pval + 9 * (int64_t)rt[q];
+            const double *w = wval + 9 * (int64_t)lo;
             for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) acc[3 * r + c] += p[r] * w[c] + p[3 + r] * w[3 + c] + p[6 + r] * w[6 + c];
+                for (int c = 0; c < 3; ++c)
+                    acc[3 * r + c] += (double)p[r] * w[c] + (double)p[3 + r] * w[3 + c] + (double)p[6 + r] * w[6 + c];
         }
         for (int t = 0; t < 9; ++t) caval[9 * (int64_t)(c0 + v) + t] = acc[t];
     }
@@ -621,9 +630,10 @@ __global__ __launch_bounds__(kThreads) void k_mg_cheb_step(OP A, const double *d
 }
 
 // b_c[I] = sum over R(I) of P_t^T r[i] (one 64-lane wave per aggregate, fixed-order butterfly)
+template <typename PT>
 __global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t *__restrict__ rptr,
                                                     const int32_t *__restrict__ ri, const int32_t *__restrict__ rt,
-                                                    const double *__restrict__ pval, const double *__restrict__ r,
+                                                    const PT *__restrict__ pval, const double *__restrict__ r,
                                                     double *__restrict__ bc, const double *done)
 {
     if (done && *done != 0.0) return;
@@ -632,11 +642,11 @@ __global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t
     double a0 = 0, a1 = 0, a2 = 0;
     for (int32_t q = rptr[I] + lane; q < rptr[I + 1]; q += 64) {
         const int32_t i = ri[q];
-        const double *p = pval + 9 * (int64_t)rt[q];
+        const PT *p = pval + 9 * (int64_t)rt[q];
         const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
-        a0 += p[0] * r0 + p[3] * r1 + p[6] * r2;
-        a1 += p[1] * r0 + p[4] * r1 + p[7] * r2;
-        a2 += p[2] * r0 + p[5] * r1 + p[8] * r2;
+        a0 += (double)p[0] * r0 + (double)p[3] * r1 + (double)p[6] * r2;
+        a1 += (double)p[1] * r0 + (double)p[4] * r1 + (double)p[7] * r2;
+        a2 += (double)p[2] * r0 + (double)p[5] * r1 + (double)p[8] * r2;
     }
     a0 = wave_sum64(a0);
     a1 = wave_sum64(a1);
@@ -645,8 +655,9 @@ __global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t
 }
 
 // x[i] += sum_t P_t x_c[J_t]
+template <typename PT>
 __global__ void k_mg_prolong(int64_t n, const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
-                             const double *__restrict__ pval, const double *__restrict__ xc, double *__restrict__ x,
+                             const PT *__restrict__ pval, const double *__restrict__ xc, double *__restrict__ x,
                              const double *done)
 {
     if (done && *done != 0.0) return;
@@ -655,11 +666,11 @@ __global__ void k_mg_prolong(int64_t n, const int32_t *__restrict__ pptr, const 
         double a0 = 0, a1 = 0, a2 = 0;
         for (int32_t t = pptr[i]; t < pptr[i + 1]; ++t) {
             const int32_t J = 3 * pcol[t];
-            const double *p = pval + 9 * (int64_t)t;
+            const PT *p = pval + 9 * (int64_t)t;
             const double c0 = xc[J], c1 = xc[J + 1], c2 = xc[J + 2];
-            a0 += p[0] * c0 + p[1] * c1 + p[2] * c2;
-            a1 += p[3] * c0 + p[4] * c1 + p[5] * c2;
-            a2 += p[6] * c0 + p[7] * c1 + p[8] * c2;
+            a0 += (double)p[0] * c0 + (double)p[1] * c1 + (double)p[2] * c2;
+            a1 += (double)p[3] * c0 + (double)p[4] * c1 + (double)p[5] * c2;
+            a2 += (double)p[6] * c0 + (double)p[7] * c1 + (double)p[8] * c2;
         }
         x[3 * i] += a0, x[3 * i + 1] += a1, x[3 * i + 2] += a2;
     }
@@ -844,32 +855,33 @@ int mg_setup(ewfe_mg *mg, const void *vals, const int32_t *rows, int64_t n_rows,
         if (l + 1 == nl) break;
         MgLevel &N = mg->lv[l + 1];
         if (l == 0) {
-            hipLaunchKernelGGL((k_mg_prolongator<true, VT>), dim3((unsigned)L.n), dim3(64), 0, s, F, B,
-                               (const double *)L.dinv, (const int32_t *)L.pptr, (const uint8_t *)L.pslot, L.pval,
-                               (const double *)(mg->lam + l));
+            hipLaunchKernelGGL((k_mg_prolongator<true, VT, VT>), dim3((unsigned)L.n), dim3(64), 0, s, F, B,
+                               (const double *)L.dinv, (const int32_t *)L.pptr, (const uint8_t *)L.pslot,
+                               (VT *)L.pval, (const double *)(mg->lam + l));
             const int64_t chunk = (L.n + 7) / 8;
-            hipLaunchKernelGGL((k_mg_ap<true, VT, kFineMaxRow, kFineMaxStage, false>), dim3((unsigned)(8 * chunk)),
-                               dim3(64), 0, s, F, B, (const int32_t *)L.pptr, (const int32_t *)L.pcol,
-                               (const double *)L.pval, (const int32_t *)L.wptr, (const int32_t *)L.wcol, L.wval, L.n,
-                               chunk);
+            hipLaunchKernelGGL((k_mg_ap<true, VT, VT, kFineMaxRow, kFineMaxStage, false>),
+                               dim3((unsigned)(8 * chunk)), dim3(64), 0, s, F, B, (const int32_t *)L.pptr,
+                               (const int32_t *)L.pcol, (const VT *)L.pval, (const int32_t *)L.wptr,
+                               (const int32_t *)L.wcol, L.wval, L.n, chunk);
         } else {
-            hipLaunchKernelGGL((k_mg_prolongator<false, double>), dim3((unsigned)L.n), dim3(64), 0, s,
+            hipLaunchKernelGGL((k_mg_prolongator<false, double, VT>), dim3((unsigned)L.n), dim3(64), 0, s,
                                FineOp<double>{}, B, (const double *)L.dinv, (const int32_t *)L.pptr,
-                               (const uint8_t *)L.pslot, L.pval, (const double *)(mg->lam + l));
+                               (const uint8_t *)L.pslot, (VT *)L.pval, (const double *)(mg->lam + l));
             const int64_t chunk = (L.n + 7) / 8;
-            hipLaunchKernelGGL((k_mg_ap<false, double, kCoarseMaxRow, kCoarseMaxStage, true>),
+            hipLaunchKernelGGL((k_mg_ap<false, double, VT, kCoarseMaxRow, kCoarseMaxStage, true>),
                                dim3((unsigned)(8 * chunk)), dim3(64), 0, s, FineOp<double>{}, B,
-                               (const int32_t *)L.pptr, (const int32_t *)L.pcol, (const double *)L.pval,
+                               (const int32_t *)L.pptr, (const int32_t *)L.pcol, (const VT *)L.pval,
                                (const int32_t *)L.wptr, (const int32_t *)L.wcol, L.wval, L.n, chunk);
         }
-        hipLaunchKernelGGL(k_mg_rap, dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
-                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const double *)L.pval,
+        hipLaunchKernelGGL(k_mg_rap<VT>, dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
+                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const VT *)L.pval,
                            (const int32_t *)L.wptr, (const int32_t *)L.wcol, (const double *)L.wval,
                            (const int32_t *)N.aptr, (const int32_t *)N.acol, N.aval);
         HIP_CHECK(hipGetLastError());
     }
     RC(coarse_inverse(mg, s));
     mg->ready = true;
+    mg->setup_dtype = std::is_same<VT, float>::value ? EWFE_F32 : EWFE_F64;
     return 0;
 }
 
@@ -905,14 +917,14 @@ int vcycle(ewfe_mg *mg, int l, const void *vals, const double *done, hipStream_t
         // residual of the smoothed iterate: r -= A d_last
         hipLaunchKernelGGL((k_mg_resid<std::decay_t<decltype(op)>>), dim3(kBlocks), dim3(kThreads), 0, s, op,
                            (const double *)L.r, (const double *)dc, L.r, done);
-        hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
-                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const double *)L.pval,
+        hipLaunchKernelGGL(k_mg_restrict<VT>, dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
+                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const VT *)L.pval,
                            (const double *)L.r, N.b, done);
         return 0;
     };
     auto post = [&](auto &&op) {
-        hipLaunchKernelGGL(k_mg_prolong, dim3(ew), dim3(256), 0, s, L.n, (const int32_t *)L.pptr,
-                           (const int32_t *)L.pcol, (const double *)L.pval, (const double *)N.x, L.x, done);
+        hipLaunchKernelGGL(k_mg_prolong<VT>, dim3(ew), dim3(256), 0, s, L.n, (const int32_t *)L.pptr,
+                           (const int32_t *)L.pcol, (const VT *)L.pval, (const double *)N.x, L.x, done);
         double *dc = L.d0, *dn = L.d1;
         hipLaunchKernelGGL((k_mg_resid<std::decay_t<decltype(op)>>), dim3(kBlocks), dim3(kThreads), 0, s, op,
                            (const double *)L.b, (const double *)L.x, L.r, done);
@@ -943,6 +955,10 @@ int mg_apply(ewfe_mg *mg, const void *vals, int vals_dtype, const double *r, dou
              void *stream)
 {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (vals_dtype != mg->setup_dtype) {
+        ewfe::set_error("multigrid: the hierarchy was set up for a matrix of another precision");
+        return EWFE_EINVAL;
+    }
     MgLevel &L0 = mg->lv[0];
     L0.b = const_cast<double *>(r);
     L0.x = z;
