@@ -384,6 +384,16 @@ def main():
     ach64 = B64 * n_el / (ms64 * 1e-3) / 1e9
     ach32 = B32 * n_el / (ms32 * 1e-3) / 1e9
 
+    # HBM bytes per evaluation from the committed rocprofv3 --pmc summary of this workload (FETCH_SIZE and
+    # WRITE_SIZE, copy-calibrated); counters cannot be read live here
+    traffic64, traffic_src = None, None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_c3d8_j2_128cube.json")
+    if n == 128 and os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+        traffic64 = tj["bytes_per_evaluation"]["fp64"]["total"]
+        traffic_src = tj["source"] + " (k_hex8_elem + k_gather8, bytes per evaluation)"
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(X, conn, U_n, dU, args.cpu_layers, n * n)
@@ -447,7 +457,8 @@ def main():
                                f"evaluation + CSR assembly, FP64", "elements": n_el, "dofs": dm.nDof,
                    "nnz": nnz, "qp_per_element": 8, "state_per_qp": 7, "plastic_qp_fraction": plastic_frac},
         "roofline": {"bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach64 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach64 / HBM_PEAK_GBS, "traffic": traffic64,
+                     "traffic_source": traffic_src,
                      "algorithmic_bytes_per_element": B64,
                      "scope": "one evaluation call: k_hex8_elem (element records) + k_gather8 (node-row CSR assembly)"},
         "cpu_baseline": cpu,
