@@ -313,6 +313,8 @@ def main():
     ap.add_argument("--no-newton-controls", action="store_true",
                     help="skip the control increments (FP32 tangent at 1e-6, FP64 at the FP32 tolerance)")
     ap.add_argument("--workload", default="c3d8-j2", choices=["c3d8-j2", "c3d20-nh-tl", "c3d8-nh-adjoint"])
+    ap.add_argument("--chunks", type=int, default=0,
+                    help="pipeline chunks of the 8-node evaluation (0 = the library's automatic choice)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -335,10 +337,11 @@ def main():
     t0 = time.time()
     m, b, X, conn = build_workload(n)
     dm = DofManager(m, device="cuda")
-    asm = ElementAssembly(b, dm, m, device="cuda")
+    asm = ElementAssembly(b, dm, m, device="cuda", pipelineChunks=args.chunks)
     t_setup = time.time() - t0
     print(f"setup {n}^3: {len(conn)} elements, {dm.nDof} dofs, nnz {asm.nnz}, colours {asm.nColors}, "
-          f"{t_setup:.1f}s", file=sys.stderr, flush=True)
+          f"pipeline chunks {asm.pipelineChunks} (structured mesh: {asm.structuredMesh}), {t_setup:.1f}s",
+          file=sys.stderr, flush=True)
 
     U_n = uniaxial(X, 5e-3)
     dU = uniaxial(X, 1e-3)
@@ -399,7 +402,7 @@ def main():
         cpu = cpu_baseline(X, conn, U_n, dU, args.cpu_layers, n * n)
         print(f"cpu baseline {cpu['value']:.4g} el/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
 
-    nnz, ncolors = asm.nnz, asm.nColors
+    nnz, ncolors, n_chunks = asm.nnz, asm.nColors, asm.pipelineChunks
     newton = None
     if not args.no_newton:
         del asm, K64, K32
@@ -455,7 +458,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"C3D8 J2 small strain {n}^3 jittered unit cube, one Newton iteration's element "
                                f"evaluation + CSR assembly, FP64", "elements": n_el, "dofs": dm.nDof,
-                   "nnz": nnz, "qp_per_element": 8, "state_per_qp": 7, "plastic_qp_fraction": plastic_frac},
+                   "nnz": nnz, "qp_per_element": 8, "state_per_qp": 7, "plastic_qp_fraction": plastic_frac,
+                   "pipeline_chunks": n_chunks},
         "roofline": {"bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach64 / HBM_PEAK_GBS, "traffic": traffic64,
                      "traffic_source": traffic_src,

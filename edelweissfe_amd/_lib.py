@@ -25,7 +25,12 @@ SIGNATURES = {
     "ewfe_last_error": (ctypes.c_char_p, []),
     "ewfe_version": (ctypes.c_int, []),
     "ewfe_plan_create": (ctypes.c_int, [_P(_vp), ctypes.c_int, _i64, _vp, _i64, _vp, _i64, _P(_i64)]),
-    "ewfe_plan_create_host": (ctypes.c_int, [_P(_vp), ctypes.c_int, _i64, _vp, _i64, _vp, _i64, _P(_i64)]),
+    "ewfe_plan_create_chunked": (ctypes.c_int, [_P(_vp), ctypes.c_int, _i64, _vp, _i64, _vp, _i64, _P(_i64),
+                                                ctypes.c_int]),
+    "ewfe_plan_pipeline_info": (ctypes.c_int, [_vp, _P(_i32), _P(_i32)]),
+    "ewfe_plan_create_host": (ctypes.c_int, [_P(_vp), ctypes.c_int, _i64, _vp, _i64, _vp, _i64, _P(_i64),
+                                             ctypes.c_int]),
+    "ewfe_plan_pipeline_orders": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ewfe_plan_pattern": (ctypes.c_int, [_vp, _vp, _vp]),
     "ewfe_plan_pattern_dev": (ctypes.c_int, [_vp, _P(_vp), _P(_vp)]),
     "ewfe_plan_element_order": (ctypes.c_int, [_vp, _vp, _P(_i32)]),

@@ -22,7 +22,7 @@ from ._lib import EWFE_F32, EWFE_F64, check, lib, ptr
 
 
 class ElementAssembly:
-    def __init__(self, block, dofManager, model, device="cuda"):
+    def __init__(self, block, dofManager, model, device="cuda", pipelineChunks=0):
         import torch
 
         self.torch = torch
@@ -39,10 +39,14 @@ class ElementAssembly:
         nnz = ctypes.c_int64()
         conn = np.ascontiguousarray(block.conn, dtype=np.int32)
         node_dof = np.ascontiguousarray(dofManager.nodeDofBase, dtype=np.int64)
-        check(L.ewfe_plan_create(ctypes.byref(plan), block.kind, len(conn), ptr(conn), len(node_dof), ptr(node_dof),
-                                 self.nDof, ctypes.byref(nnz)))
+        check(L.ewfe_plan_create_chunked(ctypes.byref(plan), block.kind, len(conn), ptr(conn), len(node_dof),
+                                         ptr(node_dof), self.nDof, ctypes.byref(nnz), int(pipelineChunks)))
         self._plan = plan
         self.nnz = nnz.value
+        nch, structured = ctypes.c_int32(), ctypes.c_int32()
+        check(L.ewfe_plan_pipeline_info(plan, ctypes.byref(nch), ctypes.byref(structured)))
+        self.pipelineChunks = nch.value
+        self.structuredMesh = bool(structured.value)
         order = np.zeros(len(conn), dtype=np.int64)
         ncol = ctypes.c_int32()
         check(L.ewfe_plan_element_order(plan, ptr(order), ctypes.byref(ncol)))

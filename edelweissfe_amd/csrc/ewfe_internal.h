@@ -36,17 +36,30 @@ struct ewfe_plan {
     int32_t *d_nbcol = nullptr;        // [n_nbblocks] first dof (column) of each neighbour block, ascending
     int32_t *d_diag_rank = nullptr;    // [n_rownodes] rank of the node itself among its neighbours
     int32_t *d_rn_meta = nullptr;      // [n_rownodes][4] {first dof, CSR offset of its rows, nb_ptr, degree}
-    int32_t *d_inc_ptr = nullptr;      // [n_rownodes + 1]
-    int32_t *d_inc = nullptr;          // [n_inc] 8 * element + local node index, ascending element
-    uint8_t *d_inc_rank = nullptr;     // [n_inc * 8] rank of element node b among the row node's neighbours
-    // per row node, at byte offset d_cl_ptr[r]: cstart[deg + 1] then 8 * n_inc entries (slot << 3 | b), grouped by
-    // neighbour rank m (entries of m: cstart[m] .. cstart[m + 1]), ascending incident slot inside each group
-    int32_t *d_cl_ptr = nullptr;       // [n_rownodes + 1]
-    uint8_t *d_cl = nullptr;
+    // 8-node gather: per row node (gather order) its incident elements (8 * element + local node index,
+    // ascending element) and contribution lists: cstart[deg + 1] then 8 * n_inc entries (slot << 3 | b),
+    // grouped by neighbour rank m (entries of m: cstart[m] .. cstart[m + 1]), ascending incident slot inside
+    // each group
+    int32_t *d_gdesc = nullptr;        // [n_rownodes][4] gather descriptors in gather order
+    int32_t *d_gblob = nullptr;        // gather blobs
     // element records (allocated on first evaluation): 36 unique 3x3 blocks per element, and nodal forces
     void *d_kbuf = nullptr;
     int kbuf_dtype = -1;
     double *d_fe = nullptr;
+
+    // pipelined 8-node evaluation: the elements are split into n_chunks chunks, evaluated in order on the
+    // caller's stream while a second stream gathers the row nodes whose incident elements are all evaluated.
+    // eorder lists element positions chunk by chunk (inside a chunk in an L2-friendly order), rorder lists row
+    // nodes chunk by chunk; chunk c is eorder[pipe_el[c], pipe_el[c+1]) and rorder[pipe_row[c], pipe_row[c+1]).
+    int n_chunks = 0;  // requested at creation (0 = automatic), the number used after it
+    bool structured = false;  // boxGen-ordered structured hexahedral mesh (tile-streaming orders)
+    int64_t grid_el[3] = {0, 0, 0};  // elements per axis (slowest first) when structured
+    std::vector<int64_t> pipe_el, pipe_row;
+    std::vector<int32_t> eorder, rorder;  // host copies
+    int32_t *d_eorder = nullptr;
+    int32_t *d_rorder = nullptr;
+    void *aux_stream = nullptr;       // hipStream_t, created on first pipelined evaluation (default priority)
+    std::vector<void *> chunk_events;  // hipEvent_t per chunk + 1 join event
 };
 
 namespace ewfe {
@@ -57,6 +70,11 @@ struct GatherHost {
     std::vector<int32_t> rownode_dof, inc_ptr, inc, nb_ptr, nbcol, diag_rank, rn_meta;
     std::vector<uint8_t> inc_rank, cl;
     std::vector<int32_t> cl_ptr;
+    // the gather's inputs in gather order (position q of the plan's rorder): per row node one descriptor
+    // {first dof, CSR offset of its rows, word offset of its blob, degree | (incident elements << 8)}, and
+    // the blob: incident element codes (8 * element + local node, ascending element) followed by the
+    // contribution lists (bytes, padded to a word)
+    std::vector<int32_t> gdesc, gblob;
 };
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,
                     std::vector<int32_t> &colpos_host, GatherHost &gather);

@@ -396,55 +396,90 @@ __host__ __device__ constexpr int pair8(int lo, int hi) { return lo * 8 - lo * (
 struct RecordArgs {
     void *kbuf;
     double *fe;
+    // this launch evaluates the element positions eorder[q_begin, q_end), 8 per workgroup; the batches are
+    // dealt to the 8 XCDs in contiguous runs of xcd_batches (workgroups go to XCDs round robin)
+    const int32_t *eorder;
+    int64_t q_begin, q_end, xcd_batches;
 };
+
+// LDS layout of the per-quadrature-point data of k_hex8_elem, per field f: 64 doubles, one per (element le,
+// quadrature point q). Fields: per-node gradients g (and p for the total-Lagrangian form), 3 per node, then
+// the tangent coefficients alpha wd, beta wd [, gamma wd for the total-Lagrangian form; gamma = beta for small
+// strain] [, c wd and the flow direction N for J2] and the stress S wd.
+template <int FORM, int MAT>
+struct Hex8Layout {
+    static constexpr bool TL = FORM == FORM_TL, HAS_N = MAT == M_J2;
+    static constexpr int G = 0;
+    static constexpr int P = TL ? 24 : 0;  // aliases g for small strain
+    static constexpr int AL = TL ? 48 : 24;
+    static constexpr int BE = AL + 1;
+    static constexpr int GA = TL ? AL + 2 : BE;
+    static constexpr int CC = TL ? AL + 3 : AL + 2;  // only with HAS_N
+    static constexpr int NV = CC + 1;                // 6, only with HAS_N
+    static constexpr int S = HAS_N ? NV + 6 : CC;    // 6
+    static constexpr int NF = S + 6;
+};
+
+// A 64-lane ds_*_b64 is served 16 lanes at a time by 32 four-byte banks, i.e. 16 double slots. Phase 1 writes
+// a field with lanes (le, q) (fixed field), phase 2 reads node fields with lanes (le, a) at a fixed q, where
+// lane a reads node b = (a + j) mod 8: a 16-lane group (le in {2g, 2g + 1}) then touches all 8 nodes b. Node
+// fields are XOR-swizzled by 2 b inside their 16-slot group so that both access patterns hit 16 different
+// slots; scalar fields are read as broadcasts (one address per element) and need no swizzle.
+__device__ __forceinline__ int slot_scalar(int f, int q, int le) { return f * 64 + 16 * (le >> 1) + 2 * q + (le & 1); }
+__device__ __forceinline__ int slot_node(int base, int b, int i, int q, int le)
+{
+    return (base + 3 * b + i) * 64 + 16 * (le >> 1) + ((2 * q + (le & 1)) ^ (2 * b));
+}
 
 template <int FORM, int MAT, typename VT>
 __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 {
     constexpr int NN = 8, NQ = 8, EPB = 8;
-    using L = QPLayout<NN, FORM>;
-    constexpr int QP_WORDS = L::NF * NQ * EPB;
-    constexpr int STAGE_WORDS = (EPB * kRec8 * (int)sizeof(VT) + 7) / 8;
-    __shared__ double s_nd[9 * NN * EPB];                                            // [field][node][le]
-    __shared__ double s_qp[QP_WORDS > STAGE_WORDS ? QP_WORDS : STAGE_WORDS];         // [field][q][le]
-    __shared__ double s_fe[EPB * 24];
+    using L = Hex8Layout<FORM, MAT>;
+    constexpr int QP_WORDS = L::NF * 64;
+    constexpr int HALF_STAGE_WORDS = ((EPB / 2) * kRec8 * (int)sizeof(VT) + 7) / 8;
+    static_assert(HALF_STAGE_WORDS <= QP_WORDS, "record staging must fit in the quadrature-point area");
+    __shared__ double s_qp[QP_WORDS];
+    __shared__ int32_t s_pos[EPB];
     const int lane = threadIdx.x;
     const int le = lane >> 3, t = lane & 7;
-    const int64_t pos0 = A.p_begin + (int64_t)blockIdx.x * EPB;
-    const int64_t pos = pos0 + le;
-    const bool active = pos < A.p_end;
+    const int64_t xb = blockIdx.x >> 3;
+    if (xb >= R.xcd_batches) return;  // uniform over the workgroup
+    const int64_t q0 = R.q_begin + ((int64_t)(blockIdx.x & 7) * R.xcd_batches + xb) * EPB;
+    if (q0 >= R.q_end) return;  // uniform over the workgroup
+    const bool active = q0 + le < R.q_end;
+    const int64_t pos = active ? (int64_t)R.eorder[q0 + le] : 0;
+    if (t == 0) s_pos[le] = (int32_t)pos;
     const int64_t n = A.n_el;
 
-    // phase 0: lane (le, t) stages node t of its element
-    if (active) {
-        const int32_t my_dof = A.edof[(int64_t)t * n + pos];
-        for (int c = 0; c < 3; ++c) {
-            s_nd[((0 + c) * NN + t) * EPB + le] = A.X[my_dof + c];
-            s_nd[((3 + c) * NN + t) * EPB + le] = A.U[my_dof + c];
-            s_nd[((6 + c) * NN + t) * EPB + le] = A.dU[my_dof + c];
-        }
-    }
-    __syncthreads();
-
-    // phase 1: lane (le, q=t) evaluates one quadrature point
+    // phase 1: lane (le, q=t) evaluates one quadrature point; the 8 lanes of an element read its nodal
+    // X, U, dU straight from global memory (same addresses: one fetch per cache line)
     double Epar = A.mat.p[0];
     if (active && A.el_param) Epar = A.el_param[A.order[pos]];
     if (active) {
         const int q = t;
-        auto nd = [&](int f, int a) { return s_nd[(f * NN + a) * EPB + le]; };
-        auto put_g = [&](int a, int j, double v) { s_qp[((L::G + 3 * a + j) * NQ + q) * EPB + le] = v; };
-        auto get_g = [&](int a, int j) { return s_qp[((L::G + 3 * a + j) * NQ + q) * EPB + le]; };
-        auto put_p = [&](int a, int j, double v) { s_qp[((L::P + 3 * a + j) * NQ + q) * EPB + le] = v; };
+        int32_t dofs[NN];
+#pragma unroll
+        for (int a = 0; a < NN; ++a) dofs[a] = A.edof[(int64_t)a * n + pos];
+        auto nd = [&](int f, int a) {
+            const double *src = f < 3 ? A.X : (f < 6 ? A.U : A.dU);
+            return src[dofs[a] + (f % 3)];
+        };
+        auto put_g = [&](int a, int j, double v) { s_qp[slot_node(L::G, a, j, q, le)] = v; };
+        auto get_g = [&](int a, int j) { return s_qp[slot_node(L::G, a, j, q, le)]; };
+        auto put_p = [&](int a, int j, double v) { s_qp[slot_node(L::P, a, j, q, le)] = v; };
         Recipe r;
         double wd;
         const bool ok = qp_phase<NN, FORM, MAT>(c_dN8[q], c_w8[q], nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd);
         if (!ok) atomic_min_pos(A.pnewdt, 0.5);
-        auto put = [&](int f, double v) { s_qp[(f * NQ + q) * EPB + le] = v; };
-        put(L::COEF + 0, wd * r.alpha);
-        put(L::COEF + 1, wd * r.beta);
-        put(L::COEF + 2, wd * r.gam);
-        put(L::COEF + 3, wd * r.c);
-        for (int i = 0; i < 6; ++i) put(L::NV + i, (MAT == M_J2) ? r.N[i] : 0.0);
+        auto put = [&](int f, double v) { s_qp[slot_scalar(f, q, le)] = v; };
+        put(L::AL, wd * r.alpha);
+        put(L::BE, wd * r.beta);
+        if (L::TL) put(L::GA, wd * r.gam);
+        if (L::HAS_N) {
+            put(L::CC, wd * r.c);
+            for (int i = 0; i < 6; ++i) put(L::NV + i, r.N[i]);
+        }
         for (int i = 0; i < 6; ++i) put(L::S + i, wd * r.S[i]);
     }
     __syncthreads();
@@ -457,19 +492,19 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
         for (int i = 0; i < 3; ++i)
             for (int k = 0; k < 3; ++k) K[j][i][k] = 0.0;
     double fa[3] = {0, 0, 0};
+    auto ld = [&](int f, int q) { return s_qp[slot_scalar(f, q, le)]; };
+    auto ldg = [&](int base, int b, int i, int q) { return s_qp[slot_node(base, b, i, q, le)]; };
     if (active && FORM == FORM_SS && MAT == M_LE) {
         // linear elasticity: alpha = lambda wd, beta = gamma = mu wd with constant lambda, mu, so with the
         // single sum T_ab = sum_q mu wd g_a g_b^T:  K_ab = (lambda / mu) T + T^T + tr(T) I
-        auto ld = [&](int f, int q) { return s_qp[(f * NQ + q) * EPB + le]; };
 #pragma unroll 2
         for (int q = 0; q < NQ; ++q) {
-            const double muwd = ld(L::COEF + 2, q);
-            double wa[3];
-            for (int i = 0; i < 3; ++i) wa[i] = muwd * ld(L::G + 3 * a + i, q);
+            const double muwd = ld(L::BE, q);
+            const double ga0 = ldg(L::G, a, 0, q), ga1 = ldg(L::G, a, 1, q), ga2 = ldg(L::G, a, 2, q);
+            const double wa[3] = {muwd * ga0, muwd * ga1, muwd * ga2};
             {
                 const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
                 const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
-                const double ga0 = ld(L::G + 3 * a, q), ga1 = ld(L::G + 3 * a + 1, q), ga2 = ld(L::G + 3 * a + 2, q);
                 fa[0] += S0 * ga0 + S3 * ga1 + S4 * ga2;
                 fa[1] += S3 * ga0 + S1 * ga1 + S5 * ga2;
                 fa[2] += S4 * ga0 + S5 * ga1 + S2 * ga2;
@@ -478,7 +513,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             for (int j = 0; j < 5; ++j) {
                 if (j < nb) {
                     const int b = (a + j) & 7;
-                    const double gb0 = ld(L::G + 3 * b, q), gb1 = ld(L::G + 3 * b + 1, q), gb2 = ld(L::G + 3 * b + 2, q);
+                    const double gb0 = ldg(L::G, b, 0, q), gb1 = ldg(L::G, b, 1, q), gb2 = ldg(L::G, b, 2, q);
                     for (int i = 0; i < 3; ++i) {
                         K[j][i][0] += wa[i] * gb0;
                         K[j][i][1] += wa[i] * gb1;
@@ -499,16 +534,15 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                 for (int k = 0; k < 3; ++k) K[j][i][k] = ratio * T[i][k] + T[k][i] + (i == k ? tr : 0.0);
         }
     } else if (active) {
-        auto ld = [&](int f, int q) { return s_qp[(f * NQ + q) * EPB + le]; };
         for (int q = 0; q < NQ; ++q) {
-            const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q),
-                         cc = ld(L::COEF + 3, q);
+            const double al = ld(L::AL, q), be = ld(L::BE, q), ga = ld(L::GA, q);
+            const double cc = L::HAS_N ? ld(L::CC, q) : 0.0;
             double pa[3], ra[3], va[3] = {0, 0, 0}, Nq[6];
             for (int i = 0; i < 3; ++i) {
-                ra[i] = ld(L::G + 3 * a + i, q);
-                pa[i] = ld(L::P + 3 * a + i, q);
+                ra[i] = ldg(L::G, a, i, q);
+                pa[i] = L::TL ? ldg(L::P, a, i, q) : ra[i];
             }
-            if (MAT == M_J2) {
+            if (L::HAS_N) {
                 for (int i = 0; i < 6; ++i) Nq[i] = ld(L::NV + i, q);
                 va[0] = Nq[0] * pa[0] + Nq[3] * pa[1] + Nq[4] * pa[2];
                 va[1] = Nq[3] * pa[0] + Nq[1] * pa[1] + Nq[5] * pa[2];
@@ -533,10 +567,10 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                     const int b = (a + j) & 7;
                     double pb[3], rb[3], vb[3] = {0, 0, 0};
                     for (int i = 0; i < 3; ++i) {
-                        rb[i] = ld(L::G + 3 * b + i, q);
-                        pb[i] = ld(L::P + 3 * b + i, q);
+                        rb[i] = ldg(L::G, b, i, q);
+                        pb[i] = L::TL ? ldg(L::P, b, i, q) : rb[i];
                     }
-                    if (MAT == M_J2) {
+                    if (L::HAS_N) {
                         vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
                         vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
                         vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
@@ -545,7 +579,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                     for (int i = 0; i < 3; ++i)
                         for (int k = 0; k < 3; ++k) {
                             double v = Aa[i] * pb[k] + Ba[k] * pb[i];
-                            if (MAT == M_J2) v -= Va[i] * vb[k];
+                            if (L::HAS_N) v -= Va[i] * vb[k];
                             if (i == k) v += d;
                             K[j][i][k] += v;
                         }
@@ -553,129 +587,169 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             }
         }
     }
-    __syncthreads();  // every lane is done reading s_qp: reuse it as the record staging area
+    // nodal forces straight from the lanes: element le's 24 values are contiguous over its 8 lanes
+    if (active)
+        for (int i = 0; i < 3; ++i) R.fe[pos * 24 + 3 * a + i] = fa[i];
 
+    // records: staged through the (now free) quadrature-point area, 4 elements at a time, then written
+    // contiguously per element
     VT *stage = reinterpret_cast<VT *>(s_qp);
-    if (active) {
+    const int valid = (R.q_end - q0) < EPB ? (int)(R.q_end - q0) : EPB;
+    for (int half = 0; half < 2; ++half) {
+        __syncthreads();  // every lane is done with s_qp (phase 2) or with the previous half's copy-out
+        if (active && (le >> 2) == half) {
+            const int ls = le & 3;
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            if (j < nb) {
-                const int b = (a + j) & 7;
-                VT *blk = stage + le * kRec8 + 9 * (a <= b ? pair8(a, b) : pair8(b, a));
-                for (int i = 0; i < 3; ++i)
-                    for (int k = 0; k < 3; ++k) {
-                        if (a <= b)
-                            blk[3 * i + k] = (VT)K[j][i][k];
-                        else
-                            blk[3 * k + i] = (VT)K[j][i][k];  // store K_ba = K_ab^T
-                    }
+            for (int j = 0; j < 5; ++j) {
+                if (j < nb) {
+                    const int b = (a + j) & 7;
+                    VT *blk = stage + ls * kRec8 + 9 * (a <= b ? pair8(a, b) : pair8(b, a));
+                    for (int i = 0; i < 3; ++i)
+                        for (int k = 0; k < 3; ++k) {
+                            if (a <= b)
+                                blk[3 * i + k] = (VT)K[j][i][k];
+                            else
+                                blk[3 * k + i] = (VT)K[j][i][k];  // store K_ba = K_ab^T
+                        }
+                }
             }
         }
-        for (int i = 0; i < 3; ++i) s_fe[le * 24 + 3 * a + i] = fa[i];
+        __syncthreads();
+        for (int l = 4 * half; l < 4 * half + 4 && l < valid; ++l) {
+            VT *kout = reinterpret_cast<VT *>(R.kbuf) + (int64_t)s_pos[l] * kRec8;
+            for (int w = lane; w < kRec8; w += 64) kout[w] = stage[(l & 3) * kRec8 + w];
+        }
     }
-    __syncthreads();
-    const int64_t valid = (A.p_end - pos0) < EPB ? (A.p_end - pos0) : EPB;
-    VT *kout = reinterpret_cast<VT *>(R.kbuf) + pos0 * kRec8;
-    for (int64_t w = lane; w < valid * kRec8; w += 64) kout[w] = stage[w];
-    double *fout = R.fe + pos0 * 24;
-    for (int64_t w = lane; w < valid * 24; w += 64) fout[w] = s_fe[w];
 }
 
 // ------------------------------------------------------------------------------------------------
-// C3D8 / C3D8TL, pass 2: one 64-lane workgroup per row node. Sums the node's rows of every incident
-// element's record in ascending element order (the order of the reference's serial VIJ -> CSR scatter,
-// csrgenerator.pyx:112-113, and P/F scatter, mk2.pyx:187-193) in LDS, then writes the node's 3 CSR rows
-// (contiguous: the rows of one node have equal length) and P, F once.
+// C3D8 / C3D8TL, pass 2: two row nodes per 64-lane workgroup, one per 32-lane half. Sums the node's rows of
+// every incident element's record in ascending element order (the order of the reference's serial
+// VIJ -> CSR scatter, csrgenerator.pyx:112-113, and P/F scatter, mk2.pyx:187-193) and writes the node's 3
+// CSR rows (contiguous: the rows of one node have equal length) and P, F once. A row node's inputs are one
+// descriptor and one blob (incident elements + contribution lists), both read in gather order, so a node
+// costs three dependent global loads: descriptor, blob, element blocks.
 // ------------------------------------------------------------------------------------------------
 constexpr int kMaxDeg8 = 64;
+constexpr int kMaxInc8 = 31;
+constexpr int kMaxBlobWords8 = kMaxInc8 + (kMaxDeg8 + 1 + 8 * kMaxInc8 + 3) / 4;
+constexpr int kBlk = 10;  // doubles per staged 3x3 block (16-byte aligned rows of the accumulation reads)
 
 template <typename VT>
-__global__ __launch_bounds__(64) void k_gather8(const int32_t *__restrict__ rownode_dof,
-                                                const int32_t *__restrict__ inc_ptr, const int32_t *__restrict__ inc,
-                                                const int32_t *__restrict__ cl_ptr, const uint8_t *__restrict__ cl,
-                                                const int32_t *__restrict__ indptr, const VT *__restrict__ kbuf,
-                                                const double *__restrict__ fe, VT *__restrict__ vals,
-                                                double *__restrict__ P, double *__restrict__ F, int64_t n_rownodes,
-                                                int64_t xcd_chunk)
+__global__ __launch_bounds__(64) void k_gather8(const int4 *__restrict__ gdesc, const int32_t *__restrict__ gblob,
+                                                const VT *__restrict__ kbuf, const double *__restrict__ fe,
+                                                VT *__restrict__ vals, double *__restrict__ P, double *__restrict__ F,
+                                                int64_t q_begin, int64_t count, int64_t xcd_chunk)
 {
-    __shared__ double s_blk[64 * 9];  // [slot][b][3x3]: block K_ab oriented for the row node (a)
-    __shared__ double s_fe[8 * 3];
-    __shared__ uint8_t s_cl[65 + 8 * 31];
-    // workgroups are dealt to the 8 XCDs round robin: XCD x takes the contiguous row-node range
-    // [x * xcd_chunk, (x + 1) * xcd_chunk), so nodes that share elements share that XCD's L2
-    // (grid = 8 * xcd_chunk >= n_rownodes: a bijection of [0, grid))
-    const int64_t r = (int64_t)(blockIdx.x & 7) * xcd_chunk + (blockIdx.x >> 3);
-    if (r >= n_rownodes) return;  // uniform over the workgroup
-    const int lane = threadIdx.x;
-    const int32_t d = rownode_dof[r];
-    const int32_t base = indptr[d];
-    const int32_t L = indptr[d + 1] - base;  // 3 * degree
-    const int deg = L / 3;
-    const int32_t k0 = inc_ptr[r], nk = inc_ptr[r + 1] - k0;
-    {  // stage this node's contribution lists
-        const int32_t c0 = cl_ptr[r], nbytes = cl_ptr[r + 1] - c0;
-        for (int t = lane; t < nbytes; t += 64) s_cl[t] = cl[c0 + t];
+    __shared__ double s_blk[2][64 * kBlk];  // per half: blocks K_ab (oriented for the row node a) of 8 elements
+    __shared__ double s_fe[2][8 * 3];
+    __shared__ int32_t s_blob[2][kMaxBlobWords8 + 1];
+    __shared__ int32_t s_nk[2];
+    // workgroups are dealt to the 8 XCDs round robin: XCD x takes the contiguous run of node pairs
+    // [x * xcd_chunk, (x + 1) * xcd_chunk) of this launch's range rorder[q_begin, q_begin + count), so row nodes
+    // that are close in the gather order (and share elements) share that XCD's L2
+    const int64_t k = blockIdx.x >> 3;
+    const int64_t pair = (int64_t)(blockIdx.x & 7) * xcd_chunk + k;
+    if (k >= xcd_chunk || 2 * pair >= count) return;  // uniform over the workgroup
+    const int lane = threadIdx.x, h = lane >> 5, hl = lane & 31;
+    const int64_t idx = 2 * pair + h;
+    const bool live = idx < count;  // the second half may lie past the end of the range
+    const int4 ds = live ? gdesc[q_begin + idx] : make_int4(0, 0, 0, 0);
+    const int32_t d = ds.x, base = ds.y, boff = ds.z;
+    const int deg = ds.w & 255, nk = ds.w >> 8;
+    const int L = 3 * deg;
+    {
+        const int nwords = nk + (deg + 1 + 8 * nk + 3) / 4;
+        for (int w = hl; w < nwords; w += 32) s_blob[h][w] = gblob[boff + w];
+        if (hl == 0) s_nk[h] = nk;
     }
-    const int m = lane;  // output: neighbour block of rank m
-    double acc[9];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
-    double f = 0.0, fa = 0.0;
-    const int slot = lane >> 3, b = lane & 7;
     __syncthreads();
-    int cur = 0, cend = 0;
-    if (m < deg) {
-        cur = s_cl[m];
-        cend = s_cl[m + 1];
+    const int nk_max = s_nk[0] > s_nk[1] ? s_nk[0] : s_nk[1];
+    const int32_t *codes = s_blob[h];
+    const uint8_t *cl = reinterpret_cast<const uint8_t *>(s_blob[h] + nk);
+    const uint8_t *ent = cl + deg + 1;
+    // outputs: lane hl owns the neighbour blocks of rank hl and hl + 32 (a hexahedral node has at most 64
+    // neighbours here; 27 on structured meshes)
+    double acc[2][9];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc[u][t] = 0.0;
+    double f = 0.0, fa = 0.0;
+    int cur[2] = {0, 0}, cend[2] = {0, 0};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int m = hl + 32 * u;
+        if (live && m < deg) {
+            cur[u] = cl[m];
+            cend[u] = cl[m + 1];
+        }
     }
-    const uint8_t *ent = s_cl + deg + 1;
-    for (int32_t c0 = 0; c0 < nk; c0 += 8) {
-        // 1) every lane fetches one whole block of one incident element (all loads in flight at once)
-        if (c0 + slot < nk) {
-            const int32_t code = inc[k0 + c0 + slot];
-            const int64_t e = code >> 3;
-            const int a = code & 7;
-            const int lo = a < b ? a : b, hi = a < b ? b : a;
-            const VT *blk = kbuf + e * kRec8 + 9 * pair8(lo, hi);
-            double v[9];
+    for (int c0 = 0; c0 < nk_max; c0 += 8) {
+        // 1) lane hl fetches blocks hl and hl + 32 of this chunk (element slot = block >> 3, node b = block & 7):
+        //    all 64 loads of a node in flight at once
 #pragma unroll
-            for (int t = 0; t < 9; ++t) v[t] = (double)blk[t];
-            const bool tr = a > b;  // stored block is K_ba: its transpose is K_ab
+        for (int rr = 0; rr < 2; ++rr) {
+            const int blk = hl + 32 * rr;
+            const int sl = blk >> 3, b = blk & 7;
+            if (live && c0 + sl < nk) {
+                const int32_t code = codes[c0 + sl];
+                const int64_t e = code >> 3;
+                const int a = code & 7;
+                const int lo = a < b ? a : b, hi = a < b ? b : a;
+                const VT *src = kbuf + e * kRec8 + 9 * pair8(lo, hi);
+                double v[9];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+                for (int t = 0; t < 9; ++t) v[t] = (double)src[t];
+                double *dst = s_blk[h] + blk * kBlk;
+                if (a > b) {  // the stored block is K_ba: its transpose is K_ab
 #pragma unroll
-                for (int kk = 0; kk < 3; ++kk) s_blk[lane * 9 + 3 * i + kk] = tr ? v[3 * kk + i] : v[3 * i + kk];
-            if (b < 3) s_fe[slot * 3 + b] = fe[e * 24 + 3 * a + b];
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int kk = 0; kk < 3; ++kk) dst[3 * i + kk] = v[3 * kk + i];
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 9; ++t) dst[t] = v[t];
+                }
+                if (b < 3) s_fe[h][sl * 3 + b] = fe[e * 24 + 3 * a + b];
+            }
         }
         __syncthreads();
-        // 2) lane m adds its contributions of this chunk, in ascending element order
-        for (; cur < cend; ++cur) {
-            const int e8 = ent[cur];
-            const int sl = (e8 >> 3) - c0;
-            if (sl >= 8) break;
-            const double *src = s_blk + (sl * 8 + (e8 & 7)) * 9;
+        // 2) the owner of neighbour block m adds its contributions of this chunk, in ascending element order
 #pragma unroll
-            for (int t = 0; t < 9; ++t) acc[t] += src[t];
+        for (int u = 0; u < 2; ++u) {
+            for (; cur[u] < cend[u]; ++cur[u]) {
+                const int e8 = ent[cur[u]];
+                const int sl = (e8 >> 3) - c0;
+                if (sl >= 8) break;
+                const double *src = s_blk[h] + (sl * 8 + (e8 & 7)) * kBlk;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) acc[u][t] += src[t];
+            }
         }
-        if (lane < 3) {
+        if (live && hl < 3) {
             const int nc = (nk - c0) < 8 ? (nk - c0) : 8;
             for (int sl = 0; sl < nc; ++sl) {
-                const double x = s_fe[sl * 3 + lane];
+                const double x = s_fe[h][sl * 3 + hl];
                 f -= x;
                 fa += fabs(x);
             }
         }
         __syncthreads();
     }
-    if (m < deg) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+    for (int u = 0; u < 2; ++u) {
+        const int m = hl + 32 * u;
+        if (live && m < deg) {
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) vals[(int64_t)base + i * L + 3 * m + kk] = (VT)acc[3 * i + kk];
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) vals[(int64_t)base + i * L + 3 * m + kk] = (VT)acc[u][3 * i + kk];
+        }
     }
-    if (lane < 3) {
-        P[d + lane] = f;
-        F[d + lane] = fa;
+    if (live && hl < 3) {
+        P[d + hl] = f;
+        F[d + hl] = fa;
     }
 }
 
@@ -926,7 +1000,7 @@ extern "C" int ewfe_state_size(int mat_kind)
 }
 
 static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32_t *conn_host, int64_t n_nodes,
-                       const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out, bool upload)
+                       const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out, bool upload, int chunks)
 {
     if (!out || !conn_host || !node_dof_host || n_el <= 0 || n_nodes <= 0 || n_dof <= 0) {
         ewfe::set_error("ewfe_plan_create: invalid arguments");
@@ -948,6 +1022,7 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
     p->n_el = n_el;
     p->n_nodes = n_nodes;
     p->n_dof = n_dof;
+    p->n_chunks = chunks > 0 ? chunks : 0;
     if ((int64_t)p->npe * p->npe * n_el >= ((int64_t)1 << 40)) {
         delete p;
         ewfe::set_error("mesh too large");
@@ -1016,11 +1091,10 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
     }
     if (p->npe == 8) {
         Up gups[] = {
-            {&p->d_inc_ptr, gh.inc_ptr.data(), sizeof(int32_t) * gh.inc_ptr.size()},
-            {&p->d_inc, gh.inc.data(), sizeof(int32_t) * gh.inc.size()},
-            {(int32_t **)&p->d_inc_rank, gh.inc_rank.data(), gh.inc_rank.size()},
-            {&p->d_cl_ptr, gh.cl_ptr.data(), sizeof(int32_t) * gh.cl_ptr.size()},
-            {(int32_t **)&p->d_cl, gh.cl.data(), gh.cl.size()},
+            {&p->d_gdesc, gh.gdesc.data(), sizeof(int32_t) * gh.gdesc.size()},
+            {&p->d_gblob, gh.gblob.data(), sizeof(int32_t) * gh.gblob.size()},
+            {&p->d_eorder, p->eorder.data(), sizeof(int32_t) * p->eorder.size()},
+            {&p->d_rorder, p->rorder.data(), sizeof(int32_t) * p->rorder.size()},
         };
         for (auto &u : gups) {
             if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
@@ -1036,13 +1110,48 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
 extern "C" int ewfe_plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32_t *conn_host,
                                 int64_t n_nodes, const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out)
 {
-    return plan_create(out, elem_kind, n_el, conn_host, n_nodes, node_dof_host, n_dof, nnz_out, true);
+    return plan_create(out, elem_kind, n_el, conn_host, n_nodes, node_dof_host, n_dof, nnz_out, true, 0);
+}
+
+extern "C" int ewfe_plan_create_chunked(ewfe_plan **out, int elem_kind, int64_t n_el, const int32_t *conn_host,
+                                        int64_t n_nodes, const int64_t *node_dof_host, int64_t n_dof,
+                                        int64_t *nnz_out, int pipeline_chunks)
+{
+    return plan_create(out, elem_kind, n_el, conn_host, n_nodes, node_dof_host, n_dof, nnz_out, true,
+                       pipeline_chunks);
 }
 
 extern "C" int ewfe_plan_create_host(ewfe_plan **out, int elem_kind, int64_t n_el, const int32_t *conn_host,
-                                     int64_t n_nodes, const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out)
+                                     int64_t n_nodes, const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out,
+                                     int pipeline_chunks)
 {
-    return plan_create(out, elem_kind, n_el, conn_host, n_nodes, node_dof_host, n_dof, nnz_out, false);
+    return plan_create(out, elem_kind, n_el, conn_host, n_nodes, node_dof_host, n_dof, nnz_out, false,
+                       pipeline_chunks);
+}
+
+extern "C" int ewfe_plan_pipeline_orders(const ewfe_plan *p, int64_t *pipe_el_host, int64_t *pipe_row_host,
+                                         int32_t *eorder_host, int32_t *rorder_host)
+{
+    if (!p || p->npe != 8) {
+        ewfe::set_error("ewfe_plan_pipeline_orders: null plan or not an 8-node element plan");
+        return EWFE_EINVAL;
+    }
+    if (pipe_el_host) std::memcpy(pipe_el_host, p->pipe_el.data(), sizeof(int64_t) * p->pipe_el.size());
+    if (pipe_row_host) std::memcpy(pipe_row_host, p->pipe_row.data(), sizeof(int64_t) * p->pipe_row.size());
+    if (eorder_host) std::memcpy(eorder_host, p->eorder.data(), sizeof(int32_t) * p->eorder.size());
+    if (rorder_host) std::memcpy(rorder_host, p->rorder.data(), sizeof(int32_t) * p->rorder.size());
+    return 0;
+}
+
+extern "C" int ewfe_plan_pipeline_info(const ewfe_plan *p, int32_t *n_chunks, int32_t *structured)
+{
+    if (!p) {
+        ewfe::set_error("ewfe_plan_pipeline_info: null plan");
+        return EWFE_EINVAL;
+    }
+    if (n_chunks) *n_chunks = p->npe == 8 ? p->n_chunks : 0;
+    if (structured) *structured = p->structured ? 1 : 0;
+    return 0;
 }
 
 extern "C" int ewfe_plan_pattern(const ewfe_plan *p, int32_t *indptr_host, int32_t *indices_host)
@@ -1095,11 +1204,14 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
 {
     if (!p) return 0;
     void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
-                    p->d_rownode_dof, p->d_inc_ptr, p->d_inc,   p->d_inc_rank,  p->d_kbuf,
+                    p->d_rownode_dof, p->d_gdesc, p->d_gblob, p->d_kbuf,
                     p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta,
-                    p->d_cl_ptr,   p->d_cl};
+                    p->d_eorder,   p->d_rorder};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
+    for (void *ev : p->chunk_events)
+        if (ev) (void)hipEventDestroy(reinterpret_cast<hipEvent_t>(ev));
+    if (p->aux_stream) (void)hipStreamDestroy(reinterpret_cast<hipStream_t>(p->aux_stream));
     delete p;
     return 0;
 }
@@ -1116,25 +1228,72 @@ static int ensure_records(ewfe_plan *p, int dtype)
     return 0;
 }
 
+// The second stream and the chunk events of the pipelined 8-node evaluation (created once per plan, on the
+// plan's device, default priority).
+static int ensure_pipeline(ewfe_plan *p)
+{
+    if (p->n_chunks <= 1 || p->aux_stream) return 0;
+    // events first: the stream is set last, so a failed set-up is retried in full by the next call
+    p->chunk_events.resize(p->n_chunks + 1, nullptr);
+    for (auto &ev : p->chunk_events) {
+        if (ev) continue;
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ev = e;
+    }
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    p->aux_stream = st;
+    return 0;
+}
+
+// 8-node elements: chunk by chunk, the element pass writes the records of chunk c on the caller's stream
+// and the gather of chunk c's row nodes runs on the plan's second stream once that pass is done, so the
+// element pass of chunk c + 1 (arithmetic bound) overlaps the gather of chunk c (memory bound). The caller's
+// stream waits for the last gather: seen from the caller, the call is ordered on its stream as before.
+template <int FORM, int MAT, typename VT>
+static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s)
+{
+    int rc = ensure_records<VT>(p, sizeof(VT) == 8 ? EWFE_F64 : EWFE_F32);
+    if (rc) return rc;
+    if ((rc = ensure_pipeline(p))) return rc;
+    const int K = p->n_chunks;
+    hipStream_t g = K > 1 ? reinterpret_cast<hipStream_t>(p->aux_stream) : s;
+    for (int c = 0; c < K; ++c) {
+        RecordArgs R{p->d_kbuf, p->d_fe, p->d_eorder, p->pipe_el[c], p->pipe_el[c + 1], 0};
+        const int64_t batches = (R.q_end - R.q_begin + 7) / 8;
+        R.xcd_batches = (batches + 7) / 8;
+        if (batches > 0) {
+            hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT>), dim3((unsigned)(8 * R.xcd_batches)), dim3(64), 0, s, A,
+                               R);
+            HIP_CHECK(hipGetLastError());
+        }
+        if (K > 1) {
+            hipEvent_t ev = reinterpret_cast<hipEvent_t>(p->chunk_events[c]);
+            HIP_CHECK(hipEventRecord(ev, s));
+            HIP_CHECK(hipStreamWaitEvent(g, ev, 0));
+        }
+        const int64_t count = p->pipe_row[c + 1] - p->pipe_row[c];
+        if (count <= 0) continue;
+        const int64_t pairs = (count + 1) / 2;
+        const int64_t chunk = (pairs + 7) / 8;
+        hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, g,
+                           reinterpret_cast<const int4 *>(p->d_gdesc), p->d_gblob, (const VT *)p->d_kbuf,
+                           (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->pipe_row[c], count, chunk);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (K > 1) {
+        hipEvent_t ev = reinterpret_cast<hipEvent_t>(p->chunk_events[K]);
+        HIP_CHECK(hipEventRecord(ev, g));
+        HIP_CHECK(hipStreamWaitEvent(s, ev, 0));
+    }
+    return 0;
+}
+
 template <int FORM, int MAT, typename VT>
 static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s)
 {
-    if (p->npe == 8) {
-        int rc = ensure_records<VT>(p, sizeof(VT) == 8 ? EWFE_F64 : EWFE_F32);
-        if (rc) return rc;
-        RecordArgs R{p->d_kbuf, p->d_fe};
-        A.p_begin = 0;
-        A.p_end = p->n_el;
-        const int64_t blocks = (p->n_el + 7) / 8;
-        hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT>), dim3((unsigned)blocks), dim3(64), 0, s, A, R);
-        HIP_CHECK(hipGetLastError());
-        const int64_t chunk = (p->n_rownodes + 7) / 8;
-        hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, s, p->d_rownode_dof,
-                           p->d_inc_ptr, p->d_inc, p->d_cl_ptr, p->d_cl, p->d_indptr, (const VT *)p->d_kbuf,
-                           (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->n_rownodes, chunk);
-        HIP_CHECK(hipGetLastError());
-        return 0;
-    }
+    if (p->npe == 8) return launch_hex8<FORM, MAT, VT>(p, A, s);
     for (int c = 0; c < p->n_colors; ++c) {
         A.p_begin = p->color_begin[c];
         A.p_end = p->color_begin[c + 1];

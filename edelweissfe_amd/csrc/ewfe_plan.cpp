@@ -40,6 +40,166 @@ static void parallel_for(int64_t n, F &&f)
     for (auto &t : th) t.join();
 }
 
+// --- pipeline orders for 8-node elements -----------------------------------------------------------
+// The evaluation is cut into chunks of elements. Chunk c's elements are evaluated (element records) after
+// those of chunks < c; the row nodes gathered with chunk c are the ones whose incident elements all lie in
+// chunks <= c, so a row node's records are complete when its chunk is gathered, whatever the order inside.
+//
+// boxGen meshes (edelweissfe/generators/boxgen.py:125-185: nodes numbered with the third grid index
+// fastest, elements likewise, C3D8 local nodes at the offsets below) get tile-streaming orders: every
+// chunk is a slab of element layers along the slowest axis; inside it, the two fast axes are cut into
+// tiles of about 16 x 16 columns and each tile is walked layer by layer. Consecutive workgroups then touch
+// the same nodes (element pass) and the same element records (gather pass) while they are in L2.
+static const int kBoxOff8[8][3] = {{0, 0, 0}, {0, 0, 1}, {1, 0, 1}, {1, 0, 0},
+                                   {0, 1, 0}, {0, 1, 1}, {1, 1, 1}, {1, 1, 0}};
+
+static bool detect_boxgen8(const int32_t *conn, int64_t n_el, int64_t n_nodes, const int64_t *node_dof,
+                           int64_t E[3])
+{
+    if (n_el < 1) return false;
+    const int64_t n2 = (int64_t)conn[4] - conn[0];   // local node 4 sits one step along axis 1
+    const int64_t n12 = (int64_t)conn[3] - conn[0];  // local node 3 sits one step along axis 0
+    if (conn[1] - conn[0] != 1 || n2 < 2 || n12 < 2 * n2 || n12 % n2) return false;
+    const int64_t n1 = n12 / n2;
+    const int64_t e2 = n2 - 1, e1 = n1 - 1;
+    if (n_el % (e1 * e2)) return false;
+    const int64_t e0 = n_el / (e1 * e2);
+    if ((e0 + 1) * n12 != n_nodes) return false;
+    for (int64_t n = 0; n < n_nodes; ++n)
+        if (node_dof[n] != 3 * n) return false;
+    std::atomic<int> bad{0};
+    parallel_for(n_el, [&](int64_t b, int64_t e) {
+        for (int64_t el = b; el < e && !bad; ++el) {
+            const int64_t i0 = el / (e1 * e2), i1 = (el / e2) % e1, i2 = el % e2;
+            for (int a = 0; a < 8; ++a) {
+                const int64_t nd = (i0 + kBoxOff8[a][0]) * n12 + (i1 + kBoxOff8[a][1]) * n2 + (i2 + kBoxOff8[a][2]);
+                if (conn[el * 8 + a] != nd) {
+                    bad = 1;
+                    break;
+                }
+            }
+        }
+    });
+    if (bad) return false;
+    E[0] = e0;
+    E[1] = e1;
+    E[2] = e2;
+    return true;
+}
+
+// [begin, end) ranges of about `t` indices covering [0, n)
+static std::vector<std::pair<int64_t, int64_t>> tiles_of(int64_t n, int64_t t)
+{
+    const int64_t nt = std::max<int64_t>(1, (n + t / 2) / t);
+    std::vector<std::pair<int64_t, int64_t>> v;
+    for (int64_t k = 0; k < nt; ++k) v.emplace_back(k * n / nt, (k + 1) * n / nt);
+    return v;
+}
+
+int build_pipeline(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, const std::vector<int32_t> &rownodes,
+                   const std::vector<int64_t> &inc_ptr, const std::vector<int64_t> &inc)
+{
+    const int64_t n_el = plan.n_el, nr = (int64_t)rownodes.size();
+    int64_t E[3] = {0, 0, 0};
+    plan.structured = detect_boxgen8(conn, n_el, plan.n_nodes, node_dof, E);
+    // automatic: one chunk (no second stream). Overlapping the element pass with the gather measured slower
+    // on MI355X at 128^3 (both passes then compete for the same CU slots). A requested count is capped at
+    // one element (structured: one element layer) per chunk.
+    int64_t K = plan.n_chunks > 0 ? plan.n_chunks : 1;
+    K = std::min<int64_t>(K, plan.structured ? E[0] : n_el);
+    plan.n_chunks = (int)K;
+    plan.pipe_el.assign(K + 1, 0);
+    plan.pipe_row.assign(K + 1, 0);
+    plan.eorder.clear();
+    plan.rorder.clear();
+    plan.eorder.reserve(n_el);
+    plan.rorder.reserve(nr);
+    std::vector<int32_t> el_chunk(n_el, -1);
+    if (plan.structured) {
+        for (int i = 0; i < 3; ++i) plan.grid_el[i] = E[i];
+        const int64_t N1 = E[1] + 1, N2 = E[2] + 1;
+        const auto et1 = tiles_of(E[1], 16), et2 = tiles_of(E[2], 16);
+        const auto nt1 = tiles_of(N1, 16), nt2 = tiles_of(N2, 16);
+        for (int64_t c = 0; c < K; ++c) {
+            const int64_t l0 = c * E[0] / K, l1 = (c + 1) * E[0] / K;
+            for (const auto &t1 : et1)
+                for (const auto &t2 : et2)
+                    for (int64_t i0 = l0; i0 < l1; ++i0)
+                        for (int64_t i1 = t1.first; i1 < t1.second; ++i1)
+                            for (int64_t i2 = t2.first; i2 < t2.second; ++i2) {
+                                const int64_t el = (i0 * E[1] + i1) * E[2] + i2;
+                                plan.eorder.push_back((int32_t)el);
+                                el_chunk[el] = (int32_t)c;
+                            }
+            plan.pipe_el[c + 1] = (int64_t)plan.eorder.size();
+            // node layers whose incident element layers (i0 - 1 and i0) end in this chunk
+            const int64_t m1 = (c == K - 1) ? l1 + 1 : l1;
+            for (const auto &t1 : nt1)
+                for (const auto &t2 : nt2)
+                    for (int64_t i0 = l0; i0 < m1; ++i0)
+                        for (int64_t i1 = t1.first; i1 < t1.second; ++i1)
+                            for (int64_t i2 = t2.first; i2 < t2.second; ++i2)
+                                plan.rorder.push_back((int32_t)((i0 * N1 + i1) * N2 + i2));
+            plan.pipe_row[c + 1] = (int64_t)plan.rorder.size();
+        }
+        // structured meshes have every node as a row node, row index = node index
+        if ((int64_t)plan.rorder.size() != nr) {
+            set_error("internal error: structured row order does not cover the row nodes");
+            return -1;
+        }
+    } else {
+        for (int64_t c = 0; c < K; ++c) {
+            plan.pipe_el[c + 1] = (c + 1) * n_el / K;
+            for (int64_t p = plan.pipe_el[c]; p < plan.pipe_el[c + 1]; ++p) {
+                plan.eorder.push_back((int32_t)p);
+                el_chunk[plan.order[p]] = (int32_t)c;
+            }
+        }
+        std::vector<int32_t> rchunk(nr);
+        std::vector<int64_t> cnt(K + 1, 0);
+        for (int64_t r = 0; r < nr; ++r) {
+            int32_t m = 0;
+            for (int64_t k = inc_ptr[rownodes[r]]; k < inc_ptr[rownodes[r] + 1]; ++k) m = std::max(m, el_chunk[inc[k]]);
+            rchunk[r] = m;
+            cnt[m + 1]++;
+        }
+        for (int64_t c = 0; c < K; ++c) cnt[c + 1] += cnt[c];
+        for (int64_t c = 0; c <= K; ++c) plan.pipe_row[c] = cnt[c];
+        plan.rorder.resize(nr);
+        for (int64_t r = 0; r < nr; ++r) plan.rorder[cnt[rchunk[r]]++] = (int32_t)r;
+    }
+    // every element once; every row node once, in the chunk where its last incident element is evaluated
+    if ((int64_t)plan.eorder.size() != n_el) {
+        set_error("internal error: element order does not cover the elements");
+        return -1;
+    }
+    std::vector<uint8_t> seen(std::max<int64_t>(n_el, nr), 0);
+    for (int32_t p : plan.eorder) {
+        if (p < 0 || p >= n_el || seen[p]) {
+            set_error("internal error: element order is not a permutation");
+            return -1;
+        }
+        seen[p] = 1;
+    }
+    std::fill(seen.begin(), seen.end(), 0);
+    for (int64_t c = 0; c < K; ++c)
+        for (int64_t q = plan.pipe_row[c]; q < plan.pipe_row[c + 1]; ++q) {
+            const int32_t r = plan.rorder[q];
+            if (r < 0 || r >= nr || seen[r]) {
+                set_error("internal error: row order is not a permutation");
+                return -1;
+            }
+            seen[r] = 1;
+            int32_t m = 0;
+            for (int64_t k = inc_ptr[rownodes[r]]; k < inc_ptr[rownodes[r] + 1]; ++k) m = std::max(m, el_chunk[inc[k]]);
+            if (m != c) {
+                set_error("internal error: a row node is gathered in another chunk than its last element");
+                return -1;
+            }
+        }
+    return 0;
+}
+
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,
                     std::vector<int32_t> &colpos_host, GatherHost &gather)
 {
@@ -304,6 +464,38 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
                         const int m = gather.inc_rank[(size_t)(gather.inc_ptr[r] + sl) * 8 + bb];
                         ent[cnt[m]++] = (uint8_t)((sl << 3) | bb);
                     }
+            }
+        });
+        const int rc = build_pipeline(plan, conn, node_dof, rownodes, inc_ptr, inc);
+        if (rc) return rc;
+        // descriptors and blobs in gather order
+        gather.gdesc.resize((size_t)4 * nr);
+        std::vector<int64_t> woff(nr + 1, 0);
+        for (int64_t q = 0; q < nr; ++q) {
+            const int32_t r = plan.rorder[q];
+            const int64_t nk = gather.inc_ptr[r + 1] - gather.inc_ptr[r];
+            const int64_t nbytes = gather.cl_ptr[r + 1] - gather.cl_ptr[r];
+            woff[q + 1] = woff[q] + nk + (nbytes + 3) / 4;
+        }
+        if (woff[nr] >= (int64_t)INT32_MAX) {
+            set_error("gather lists beyond int32 indexing");
+            return -4;
+        }
+        gather.gblob.assign(woff[nr], 0);
+        parallel_for(nr, [&](int64_t b0, int64_t e0) {
+            for (int64_t q = b0; q < e0; ++q) {
+                const int32_t r = plan.rorder[q];
+                const int32_t nk = gather.inc_ptr[r + 1] - gather.inc_ptr[r];
+                const int32_t dg = gather.nb_ptr[r + 1] - gather.nb_ptr[r];
+                const int32_t d = gather.rownode_dof[r];
+                int32_t *g = gather.gdesc.data() + 4 * q;
+                g[0] = d;
+                g[1] = plan.indptr[d];
+                g[2] = (int32_t)woff[q];
+                g[3] = dg | (nk << 8);
+                int32_t *w = gather.gblob.data() + woff[q];
+                for (int32_t k = 0; k < nk; ++k) w[k] = gather.inc[gather.inc_ptr[r] + k];
+                std::memcpy(w + nk, gather.cl.data() + gather.cl_ptr[r], gather.cl_ptr[r + 1] - gather.cl_ptr[r]);
             }
         });
     }

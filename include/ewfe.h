@@ -63,10 +63,30 @@ int ewfe_version(void);
 int ewfe_plan_create(ewfe_plan **plan, int elem_kind, int64_t n_el, const int32_t *conn_host, int64_t n_nodes,
                      const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out);
 
-/* Same as ewfe_plan_create without touching any device: the plan answers ewfe_plan_pattern,
- * ewfe_plan_element_order and ewfe_plan_info only (host-side checks and tools). */
+/* ewfe_plan_create with a chosen number of pipeline chunks for 8-node elements (0 = automatic, currently
+ * one chunk). With more than one chunk the element pass of chunk c + 1 runs beside the CSR gather of chunk
+ * c on a second stream; results are bitwise independent of the count. Same role as ewfe_plan_create
+ * (dofmanager.py:522-557, csrgenerator.pyx:47-98); the count itself has no counterpart in the reference. */
+int ewfe_plan_create_chunked(ewfe_plan **plan, int elem_kind, int64_t n_el, const int32_t *conn_host,
+                             int64_t n_nodes, const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out,
+                             int pipeline_chunks);
+
+/* Pipeline chunks in use (0 for 20-node elements) and whether the mesh was recognised as a boxGen
+ * structured hexahedral grid (boxgen.py:125-185), which selects the tile-streaming work orders. */
+int ewfe_plan_pipeline_info(const ewfe_plan *plan, int32_t *n_chunks, int32_t *structured);
+
+/* Same as ewfe_plan_create_chunked without touching any device: the plan answers ewfe_plan_pattern,
+ * ewfe_plan_element_order, ewfe_plan_info, ewfe_plan_pipeline_info and ewfe_plan_pipeline_orders only
+ * (host-side checks and tools). */
 int ewfe_plan_create_host(ewfe_plan **plan, int elem_kind, int64_t n_el, const int32_t *conn_host, int64_t n_nodes,
-                          const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out);
+                          const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out, int pipeline_chunks);
+
+/* Host copies of the 8-node pipeline orders (tools and tests; no counterpart in the reference):
+ * pipe_el, pipe_row int64[n_chunks + 1] chunk boundaries; eorder int32[n_el] element positions in
+ * evaluation order; rorder int32[n_row_nodes] row nodes (ascending first dof = row node index) in gather
+ * order. Any pointer may be NULL. */
+int ewfe_plan_pipeline_orders(const ewfe_plan *plan, int64_t *pipe_el_host, int64_t *pipe_row_host,
+                              int32_t *eorder_host, int32_t *rorder_host);
 
 /* Copies the CSR pattern to host buffers: indptr int32[n_dof+1], indices int32[nnz]. It equals
  * scipy's canonical COO->CSR of the VIJ pattern bit for bit (csrgenerator.pyx:68-77). */

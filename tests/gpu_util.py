@@ -18,7 +18,7 @@ def one_block_model(el, mat, mp, X, conn, elParams=None, sets=None):
 
 
 class GpuEval:
-    def __init__(self, el, mat, mp, X, conn, elParams=None):
+    def __init__(self, el, mat, mp, X, conn, elParams=None, pipelineChunks=0):
         import torch
 
         from edelweissfe_amd.assembly import ElementAssembly
@@ -27,7 +27,7 @@ class GpuEval:
         self.torch = torch
         self.model, self.block = one_block_model(el, mat, mp, X, conn, elParams)
         self.dm = DofManager(self.model)
-        self.asm = ElementAssembly(self.block, self.dm, self.model)
+        self.asm = ElementAssembly(self.block, self.dm, self.model, pipelineChunks=pipelineChunks)
 
     def vec(self, a):
         return self.torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device="cuda")

@@ -93,6 +93,32 @@ def test_j2_two_increments_match_oracle(el, n, mp):
     assert rel(o3["data"], r3["data"]) < 1e-12 and rel(o3["P"], r3["P"]) < 1e-11
 
 
+@pytest.mark.parametrize("mesh,chunks", [("box", 3), ("box", 6), ("shuffled", 3)])
+def test_pipelined_chunks_match_single_pass(mesh, chunks):
+    """The chunked evaluation (element pass of chunk c + 1 beside the gather of chunk c on a second stream)
+    gives bitwise the CSR values, P, F and state of the one-chunk evaluation, and matches the oracle, on a
+    boxGen mesh (tile-streaming orders) and on a shuffled numbering of it (generic orders)."""
+    print(f"case pipelined J2 {mesh} chunks={chunks}", flush=True)
+    X, conn, _ = jittered_box((6, 4, 5), "C3D8", seed=8)
+    if mesh == "shuffled":
+        rng = np.random.default_rng(9)
+        perm = rng.permutation(len(X))
+        X, conn = X[perm], np.argsort(perm)[conn].astype(np.int32)[rng.permutation(len(conn))]
+    rng = np.random.default_rng(10)
+    U = _uniaxial_field(X, 4e-3, rng, 2e-4)
+    one = GpuEval("C3D8", "vonmises", J2, X, conn, pipelineChunks=1)
+    many = GpuEval("C3D8", "vonmises", J2, X, conn, pipelineChunks=chunks)
+    assert one.asm.pipelineChunks == 1 and many.asm.pipelineChunks == chunks
+    assert many.asm.structuredMesh == (mesh == "box")
+    for dtype in ("fp64", "fp32"):
+        a, b = one(U, U, dtype), many(U, U, dtype)
+        for k in ("data", "P", "F", "state"):
+            assert np.array_equal(a[k], b[k]), (dtype, k)
+    ref = oracle_eval("C3D8", "vonmises", J2, X, conn, U, U)
+    out = many(U, U)
+    assert rel(out["data"], ref["data"]) < 1e-12 and rel(out["P"], ref["P"]) < 1e-12
+
+
 def test_fp32_tangent_close_to_fp64():
     X, conn, _ = jittered_box(4, "C3D8", seed=5)
     rng = np.random.default_rng(6)

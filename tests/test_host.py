@@ -40,7 +40,7 @@ def host_plan(kind, conn, node_dof, n_dof):
     conn = np.ascontiguousarray(conn, dtype=np.int32)
     node_dof = np.ascontiguousarray(node_dof, dtype=np.int64)
     _lib.check(L.ewfe_plan_create_host(ctypes.byref(plan), kind, len(conn), _lib.ptr(conn), len(node_dof),
-                                       _lib.ptr(node_dof), n_dof, ctypes.byref(nnz)))
+                                       _lib.ptr(node_dof), n_dof, ctypes.byref(nnz), 0))
     indptr = np.zeros(n_dof + 1, dtype=np.int32)
     indices = np.zeros(nnz.value, dtype=np.int32)
     _lib.check(L.ewfe_plan_pattern(plan, _lib.ptr(indptr), _lib.ptr(indices)))
@@ -114,7 +114,7 @@ def test_plan_rejects_bad_input():
     conn = np.array([[0, 1, 2, 3, 4, 5, 6, 99]], dtype=np.int32)
     node_dof = np.arange(0, 24, 3, dtype=np.int64)
     rc = L.ewfe_plan_create_host(ctypes.byref(plan), 0, 1, _lib.ptr(conn), 8, _lib.ptr(node_dof), 24,
-                                 ctypes.byref(nnz))
+                                 ctypes.byref(nnz), 0)
     assert rc == -1 and b"outside" in L.ewfe_last_error()
     with pytest.raises(_lib.EwfeError):
         _lib.check(rc)
@@ -188,3 +188,68 @@ def test_time_stepper_matches_oracle_with_cutbacks():
     next(g)
     with pytest.raises(ReachedMinIncrementSize):
         t2.discardAndChangeIncrement(0.5)
+
+
+def _pipeline_orders(conn, node_dof, n_dof, chunks):
+    L = _lib.lib()
+    plan = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    node_dof = np.ascontiguousarray(node_dof, dtype=np.int64)
+    _lib.check(L.ewfe_plan_create_host(ctypes.byref(plan), _lib.EWFE_C3D8, len(conn), _lib.ptr(conn), len(node_dof),
+                                       _lib.ptr(node_dof), n_dof, ctypes.byref(nnz), chunks))
+    nch, st = ctypes.c_int32(), ctypes.c_int32()
+    _lib.check(L.ewfe_plan_pipeline_info(plan, ctypes.byref(nch), ctypes.byref(st)))
+    rows = np.unique(node_dof[np.unique(conn)])  # row nodes: nodes with elements, ascending first dof
+    pe, pr = np.zeros(nch.value + 1, np.int64), np.zeros(nch.value + 1, np.int64)
+    eo, ro = np.zeros(len(conn), np.int32), np.zeros(len(rows), np.int32)
+    _lib.check(L.ewfe_plan_pipeline_orders(plan, _lib.ptr(pe), _lib.ptr(pr), _lib.ptr(eo), _lib.ptr(ro)))
+    L.ewfe_plan_destroy(plan)
+    return nch.value, bool(st.value), pe, pr, eo, ro, rows
+
+
+def _check_pipeline(conn, node_dof, n_dof, chunks):
+    K, structured, pe, pr, eo, ro, rows = _pipeline_orders(conn, node_dof, n_dof, chunks)
+    assert 1 <= K and pe[0] == 0 and pe[-1] == len(conn) and pr[0] == 0 and pr[-1] == len(rows)
+    assert np.all(np.diff(pe) >= 0) and np.all(np.diff(pr) >= 0)
+    assert np.array_equal(np.sort(eo), np.arange(len(conn)))  # every element evaluated once
+    assert np.array_equal(np.sort(ro), np.arange(len(rows)))  # every row node gathered once
+    el_chunk = np.empty(len(conn), np.int64)
+    for c in range(K):
+        el_chunk[eo[pe[c]:pe[c + 1]]] = c
+    # a row node is gathered with the chunk that evaluates the last of its elements
+    node_of_row = {int(d): i for i, d in enumerate(rows)}
+    last = np.zeros(len(rows), np.int64)
+    for e, nodes in enumerate(conn):
+        for nd in nodes:
+            r = node_of_row[int(node_dof[nd])]
+            last[r] = max(last[r], el_chunk[e])
+    for c in range(K):
+        assert np.all(last[ro[pr[c]:pr[c + 1]]] == c)
+    return K, structured, pe, eo
+
+
+@pytest.mark.parametrize("n,chunks", [((4, 3, 5), 1), ((4, 3, 5), 3), ((7, 20, 18), 4), ((6, 2, 2), 6),
+                                      ((2, 2, 2), 9)])
+def test_pipeline_orders_boxgen(n, chunks):
+    X, conn, _ = O.boxgen(*n)
+    base, nDof, _ = O.dof_numbering(len(X), conn)
+    K, structured, pe, eo = _check_pipeline(conn, base, nDof, chunks)
+    assert structured and K == min(chunks, n[0])
+    # structured chunks are slabs of element layers along the slowest axis
+    layer = eo // (n[1] * n[2])
+    for c in range(K):
+        lay = np.unique(layer[pe[c]:pe[c + 1]])
+        assert np.array_equal(lay, np.arange(lay[0], lay[-1] + 1))
+
+
+@pytest.mark.parametrize("chunks", [1, 2, 5])
+def test_pipeline_orders_unstructured(chunks):
+    X, conn, _ = O.boxgen(4, 3, 3)
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(len(X))
+    inv = np.argsort(perm)
+    conn2 = inv[conn].astype(np.int32)[rng.permutation(len(conn))]
+    base, nDof, _ = O.dof_numbering(len(X), conn2)
+    K, structured, _, _ = _check_pipeline(conn2, base, nDof, chunks)
+    assert not structured and K == chunks
