@@ -47,17 +47,15 @@ struct ewfe_plan {
     int kbuf_dtype = -1;
     double *d_fe = nullptr;
 
-    // pipelined 8-node evaluation: the elements are split into n_chunks chunks, evaluated in order on the
-    // caller's stream while a second stream gathers the row nodes whose incident elements are all evaluated.
-    // eorder lists element positions chunk by chunk (inside a chunk in an L2-friendly order), rorder lists row
-    // nodes chunk by chunk; chunk c is eorder[pipe_el[c], pipe_el[c+1]) and rorder[pipe_row[c], pipe_row[c+1]).
+    // 8-node evaluation in chunks: plan positions [pipe_el[c], pipe_el[c+1]) are evaluated in order (the plan
+    // order itself is the evaluation order: tile-streaming for structured meshes), and the row nodes
+    // rorder[pipe_row[c], pipe_row[c+1]) -- those whose incident elements are all evaluated by then -- are
+    // gathered after them; with more than one chunk on a second stream, beside the next chunk's element pass.
     int n_chunks = 0;  // requested at creation (0 = automatic), the number used after it
     bool structured = false;  // boxGen-ordered structured hexahedral mesh (tile-streaming orders)
     int64_t grid_el[3] = {0, 0, 0};  // elements per axis (slowest first) when structured
     std::vector<int64_t> pipe_el, pipe_row;
-    std::vector<int32_t> eorder, rorder;  // host copies
-    int32_t *d_eorder = nullptr;
-    int32_t *d_rorder = nullptr;
+    std::vector<int32_t> rorder;  // host copy of the row order
     void *aux_stream = nullptr;       // hipStream_t, created on first pipelined evaluation (default priority)
     std::vector<void *> chunk_events;  // hipEvent_t per chunk + 1 join event
 };

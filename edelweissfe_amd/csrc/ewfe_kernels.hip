@@ -324,6 +324,13 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
                                          const EvalArgs &A, double Epar, int64_t pos, int q, int NQ, Recipe &r,
                                          double &wd)
 {
+    // converged state first: its loads do not depend on anything below
+    double ep[6] = {0, 0, 0, 0, 0, 0}, kn = 0.0;
+    if (FORM == FORM_SS && MAT == M_J2) {
+        const int64_t n = A.n_el;
+        for (int s = 0; s < 6; ++s) ep[s] = A.state_n[((int64_t)s * NQ + q) * n + pos];
+        kn = A.state_n[((int64_t)6 * NQ + q) * n + pos];
+    }
     double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3];
     for (int a = 0; a < NN; ++a)
         for (int i = 0; i < 3; ++i)
@@ -349,10 +356,8 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
         if (MAT == M_LE) {
             le_recipe(A.mat, Epar, H, r);
         } else {
-            double ep[6], ep_t[6], kt;
+            double ep_t[6], kt;
             const int64_t n = A.n_el;
-            for (int s = 0; s < 6; ++s) ep[s] = A.state_n[((int64_t)s * NQ + q) * n + pos];
-            const double kn = A.state_n[((int64_t)6 * NQ + q) * n + pos];
             ok = j2_recipe(A.mat, Epar, H, dH, ep, kn, ep_t, kt, r);
             for (int s = 0; s < 6; ++s) A.state_t[((int64_t)s * NQ + q) * n + pos] = ep_t[s];
             A.state_t[((int64_t)6 * NQ + q) * n + pos] = kt;
@@ -396,9 +401,8 @@ __host__ __device__ constexpr int pair8(int lo, int hi) { return lo * 8 - lo * (
 struct RecordArgs {
     void *kbuf;
     double *fe;
-    // this launch evaluates the element positions eorder[q_begin, q_end), 8 per workgroup; the batches are
-    // dealt to the 8 XCDs in contiguous runs of xcd_batches (workgroups go to XCDs round robin)
-    const int32_t *eorder;
+    // this launch evaluates the plan positions [q_begin, q_end), 8 per workgroup; the batches are dealt to
+    // the 8 XCDs in contiguous runs of xcd_batches (workgroups go to XCDs round robin)
     int64_t q_begin, q_end, xcd_batches;
 };
 
@@ -440,31 +444,35 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     constexpr int HALF_STAGE_WORDS = ((EPB / 2) * kRec8 * (int)sizeof(VT) + 7) / 8;
     static_assert(HALF_STAGE_WORDS <= QP_WORDS, "record staging must fit in the quadrature-point area");
     __shared__ double s_qp[QP_WORDS];
-    __shared__ int32_t s_pos[EPB];
     const int lane = threadIdx.x;
     const int le = lane >> 3, t = lane & 7;
     const int64_t xb = blockIdx.x >> 3;
     if (xb >= R.xcd_batches) return;  // uniform over the workgroup
     const int64_t q0 = R.q_begin + ((int64_t)(blockIdx.x & 7) * R.xcd_batches + xb) * EPB;
     if (q0 >= R.q_end) return;  // uniform over the workgroup
-    const bool active = q0 + le < R.q_end;
-    const int64_t pos = active ? (int64_t)R.eorder[q0 + le] : 0;
-    if (t == 0) s_pos[le] = (int32_t)pos;
+    const int64_t pos = q0 + le;
+    const bool active = pos < R.q_end;
     const int64_t n = A.n_el;
 
     // phase 1: lane (le, q=t) evaluates one quadrature point; the 8 lanes of an element read its nodal
-    // X, U, dU straight from global memory (same addresses: one fetch per cache line)
+    // X, U (, dU) straight from global memory (same addresses: one fetch per cache line), all loads issued
+    // before any arithmetic
     double Epar = A.mat.p[0];
     if (active && A.el_param) Epar = A.el_param[A.order[pos]];
     if (active) {
         const int q = t;
-        int32_t dofs[NN];
+        double xn[NN][3], un[NN][3], dun[NN][3];
 #pragma unroll
-        for (int a = 0; a < NN; ++a) dofs[a] = A.edof[(int64_t)a * n + pos];
-        auto nd = [&](int f, int a) {
-            const double *src = f < 3 ? A.X : (f < 6 ? A.U : A.dU);
-            return src[dofs[a] + (f % 3)];
-        };
+        for (int a = 0; a < NN; ++a) {
+            const int32_t da = A.edof[(int64_t)a * n + pos];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                xn[a][c] = A.X[da + c];
+                un[a][c] = A.U[da + c];
+                dun[a][c] = (MAT == M_J2) ? A.dU[da + c] : 0.0;
+            }
+        }
+        auto nd = [&](int f, int a) { return f < 3 ? xn[a][f] : (f < 6 ? un[a][f - 3] : dun[a][f - 6]); };
         auto put_g = [&](int a, int j, double v) { s_qp[slot_node(L::G, a, j, q, le)] = v; };
         auto get_g = [&](int a, int j) { return s_qp[slot_node(L::G, a, j, q, le)]; };
         auto put_p = [&](int a, int j, double v) { s_qp[slot_node(L::P, a, j, q, le)] = v; };
@@ -561,8 +569,20 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                 Ba[i] = be * pa[i];
                 Va[i] = cc * va[i];
             }
+            {  // diagonal block K_aa = (alpha + beta) p_a p_a^T + gamma |r_a|^2 I - c v_a v_a^T: symmetric, the
+               // upper triangle is accumulated and mirrored after the loop
+                const double dd = ga * (ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
+                double sa[3];
+                for (int i = 0; i < 3; ++i) sa[i] = Aa[i] + Ba[i];
+                for (int i = 0; i < 3; ++i)
+                    for (int k = i; k < 3; ++k) {
+                        K[0][i][k] = fma(sa[i], pa[k], K[0][i][k]);
+                        if (L::HAS_N) K[0][i][k] = fma(-Va[i], va[k], K[0][i][k]);
+                    }
+                for (int i = 0; i < 3; ++i) K[0][i][i] += dd;
+            }
 #pragma unroll
-            for (int j = 0; j < 5; ++j) {
+            for (int j = 1; j < 5; ++j) {
                 if (j < nb) {
                     const int b = (a + j) & 7;
                     double pb[3], rb[3], vb[3] = {0, 0, 0};
@@ -578,21 +598,23 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                     const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
                     for (int i = 0; i < 3; ++i)
                         for (int k = 0; k < 3; ++k) {
-                            double v = Aa[i] * pb[k] + Ba[k] * pb[i];
-                            if (L::HAS_N) v -= Va[i] * vb[k];
-                            if (i == k) v += d;
-                            K[j][i][k] += v;
+                            K[j][i][k] = fma(Aa[i], pb[k], K[j][i][k]);
+                            K[j][i][k] = fma(Ba[k], pb[i], K[j][i][k]);
+                            if (L::HAS_N) K[j][i][k] = fma(-Va[i], vb[k], K[j][i][k]);
                         }
+                    for (int i = 0; i < 3; ++i) K[j][i][i] += d;
                 }
             }
         }
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < i; ++k) K[0][i][k] = K[0][k][i];
     }
     // nodal forces straight from the lanes: element le's 24 values are contiguous over its 8 lanes
     if (active)
         for (int i = 0; i < 3; ++i) R.fe[pos * 24 + 3 * a + i] = fa[i];
 
     // records: staged through the (now free) quadrature-point area, 4 elements at a time, then written
-    // contiguously per element
+    // contiguously (the workgroup's elements are consecutive plan positions)
     VT *stage = reinterpret_cast<VT *>(s_qp);
     const int valid = (R.q_end - q0) < EPB ? (int)(R.q_end - q0) : EPB;
     for (int half = 0; half < 2; ++half) {
@@ -615,9 +637,11 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             }
         }
         __syncthreads();
-        for (int l = 4 * half; l < 4 * half + 4 && l < valid; ++l) {
-            VT *kout = reinterpret_cast<VT *>(R.kbuf) + (int64_t)s_pos[l] * kRec8;
-            for (int w = lane; w < kRec8; w += 64) kout[w] = stage[(l & 3) * kRec8 + w];
+        const int nv = valid - 4 * half;  // elements of this half
+        if (nv > 0) {
+            VT *kout = reinterpret_cast<VT *>(R.kbuf) + (q0 + 4 * half) * kRec8;
+            const int nw = (nv < 4 ? nv : 4) * kRec8;
+            for (int w = lane; w < nw; w += 64) kout[w] = stage[w];
         }
     }
 }
@@ -1093,8 +1117,7 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
         Up gups[] = {
             {&p->d_gdesc, gh.gdesc.data(), sizeof(int32_t) * gh.gdesc.size()},
             {&p->d_gblob, gh.gblob.data(), sizeof(int32_t) * gh.gblob.size()},
-            {&p->d_eorder, p->eorder.data(), sizeof(int32_t) * p->eorder.size()},
-            {&p->d_rorder, p->rorder.data(), sizeof(int32_t) * p->rorder.size()},
+
         };
         for (auto &u : gups) {
             if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
@@ -1138,7 +1161,8 @@ extern "C" int ewfe_plan_pipeline_orders(const ewfe_plan *p, int64_t *pipe_el_ho
     }
     if (pipe_el_host) std::memcpy(pipe_el_host, p->pipe_el.data(), sizeof(int64_t) * p->pipe_el.size());
     if (pipe_row_host) std::memcpy(pipe_row_host, p->pipe_row.data(), sizeof(int64_t) * p->pipe_row.size());
-    if (eorder_host) std::memcpy(eorder_host, p->eorder.data(), sizeof(int32_t) * p->eorder.size());
+    if (eorder_host)
+        for (int64_t i = 0; i < p->n_el; ++i) eorder_host[i] = (int32_t)p->order[i];
     if (rorder_host) std::memcpy(rorder_host, p->rorder.data(), sizeof(int32_t) * p->rorder.size());
     return 0;
 }
@@ -1205,8 +1229,7 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
     if (!p) return 0;
     void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
                     p->d_rownode_dof, p->d_gdesc, p->d_gblob, p->d_kbuf,
-                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta,
-                    p->d_eorder,   p->d_rorder};
+                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     for (void *ev : p->chunk_events)
@@ -1260,7 +1283,7 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s)
     const int K = p->n_chunks;
     hipStream_t g = K > 1 ? reinterpret_cast<hipStream_t>(p->aux_stream) : s;
     for (int c = 0; c < K; ++c) {
-        RecordArgs R{p->d_kbuf, p->d_fe, p->d_eorder, p->pipe_el[c], p->pipe_el[c + 1], 0};
+        RecordArgs R{p->d_kbuf, p->d_fe, p->pipe_el[c], p->pipe_el[c + 1], 0};
         const int64_t batches = (R.q_end - R.q_begin + 7) / 8;
         R.xcd_batches = (batches + 7) / 8;
         if (batches > 0) {

@@ -96,10 +96,12 @@ static std::vector<std::pair<int64_t, int64_t>> tiles_of(int64_t n, int64_t t)
     return v;
 }
 
-int build_pipeline(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, const std::vector<int32_t> &rownodes,
-                   const std::vector<int64_t> &inc_ptr, const std::vector<int64_t> &inc)
+// Element order of an 8-node plan (plan position -> element): structured meshes in tile-streaming order,
+// others in element order; also fixes the chunks (plan.n_chunks, plan.pipe_el). The element pass evaluates
+// the plan positions in order, so the state and records of consecutive workgroups are contiguous.
+static void element_order8(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof)
 {
-    const int64_t n_el = plan.n_el, nr = (int64_t)rownodes.size();
+    const int64_t n_el = plan.n_el;
     int64_t E[3] = {0, 0, 0};
     plan.structured = detect_boxgen8(conn, n_el, plan.n_nodes, node_dof, E);
     // automatic: one chunk (no second stream). Overlapping the element pass with the gather measured slower
@@ -109,29 +111,55 @@ int build_pipeline(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof
     K = std::min<int64_t>(K, plan.structured ? E[0] : n_el);
     plan.n_chunks = (int)K;
     plan.pipe_el.assign(K + 1, 0);
-    plan.pipe_row.assign(K + 1, 0);
-    plan.eorder.clear();
-    plan.rorder.clear();
-    plan.eorder.reserve(n_el);
-    plan.rorder.reserve(nr);
-    std::vector<int32_t> el_chunk(n_el, -1);
+    plan.order.clear();
+    plan.order.reserve(n_el);
     if (plan.structured) {
         for (int i = 0; i < 3; ++i) plan.grid_el[i] = E[i];
-        const int64_t N1 = E[1] + 1, N2 = E[2] + 1;
         const auto et1 = tiles_of(E[1], 16), et2 = tiles_of(E[2], 16);
-        const auto nt1 = tiles_of(N1, 16), nt2 = tiles_of(N2, 16);
         for (int64_t c = 0; c < K; ++c) {
             const int64_t l0 = c * E[0] / K, l1 = (c + 1) * E[0] / K;
             for (const auto &t1 : et1)
                 for (const auto &t2 : et2)
                     for (int64_t i0 = l0; i0 < l1; ++i0)
                         for (int64_t i1 = t1.first; i1 < t1.second; ++i1)
-                            for (int64_t i2 = t2.first; i2 < t2.second; ++i2) {
-                                const int64_t el = (i0 * E[1] + i1) * E[2] + i2;
-                                plan.eorder.push_back((int32_t)el);
-                                el_chunk[el] = (int32_t)c;
-                            }
-            plan.pipe_el[c + 1] = (int64_t)plan.eorder.size();
+                            for (int64_t i2 = t2.first; i2 < t2.second; ++i2)
+                                plan.order.push_back((i0 * E[1] + i1) * E[2] + i2);
+            plan.pipe_el[c + 1] = (int64_t)plan.order.size();
+        }
+    } else {
+        for (int64_t e = 0; e < n_el; ++e) plan.order.push_back(e);
+        for (int64_t c = 0; c < K; ++c) plan.pipe_el[c + 1] = (c + 1) * n_el / K;
+    }
+}
+
+// Row order of an 8-node plan: chunk by chunk, the row nodes whose last incident element is evaluated in the
+// chunk; structured meshes walk node tiles layer by layer like the element order.
+static int row_order8(ewfe_plan &plan, const std::vector<int32_t> &rownodes, const std::vector<int64_t> &inc_ptr,
+                      const std::vector<int64_t> &inc)
+{
+    const int64_t n_el = plan.n_el, nr = (int64_t)rownodes.size(), K = plan.n_chunks;
+    plan.pipe_row.assign(K + 1, 0);
+    plan.rorder.clear();
+    plan.rorder.reserve(nr);
+    std::vector<int32_t> el_chunk(n_el, -1);
+    for (int64_t c = 0; c < K; ++c)
+        for (int64_t p = plan.pipe_el[c]; p < plan.pipe_el[c + 1]; ++p) el_chunk[plan.order[p]] = (int32_t)c;
+    for (int64_t e = 0; e < n_el; ++e)
+        if (el_chunk[e] < 0) {
+            set_error("internal error: element order does not cover the elements");
+            return -1;
+        }
+    auto last_chunk = [&](int64_t r) {
+        int32_t m = 0;
+        for (int64_t k = inc_ptr[rownodes[r]]; k < inc_ptr[rownodes[r] + 1]; ++k) m = std::max(m, el_chunk[inc[k]]);
+        return m;
+    };
+    if (plan.structured) {
+        const int64_t *E = plan.grid_el;
+        const int64_t N1 = E[1] + 1, N2 = E[2] + 1;
+        const auto nt1 = tiles_of(N1, 16), nt2 = tiles_of(N2, 16);
+        for (int64_t c = 0; c < K; ++c) {
+            const int64_t l0 = c * E[0] / K, l1 = (c + 1) * E[0] / K;
             // node layers whose incident element layers (i0 - 1 and i0) end in this chunk
             const int64_t m1 = (c == K - 1) ? l1 + 1 : l1;
             for (const auto &t1 : nt1)
@@ -148,40 +176,19 @@ int build_pipeline(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof
             return -1;
         }
     } else {
-        for (int64_t c = 0; c < K; ++c) {
-            plan.pipe_el[c + 1] = (c + 1) * n_el / K;
-            for (int64_t p = plan.pipe_el[c]; p < plan.pipe_el[c + 1]; ++p) {
-                plan.eorder.push_back((int32_t)p);
-                el_chunk[plan.order[p]] = (int32_t)c;
-            }
-        }
         std::vector<int32_t> rchunk(nr);
         std::vector<int64_t> cnt(K + 1, 0);
         for (int64_t r = 0; r < nr; ++r) {
-            int32_t m = 0;
-            for (int64_t k = inc_ptr[rownodes[r]]; k < inc_ptr[rownodes[r] + 1]; ++k) m = std::max(m, el_chunk[inc[k]]);
-            rchunk[r] = m;
-            cnt[m + 1]++;
+            rchunk[r] = last_chunk(r);
+            cnt[rchunk[r] + 1]++;
         }
         for (int64_t c = 0; c < K; ++c) cnt[c + 1] += cnt[c];
         for (int64_t c = 0; c <= K; ++c) plan.pipe_row[c] = cnt[c];
         plan.rorder.resize(nr);
         for (int64_t r = 0; r < nr; ++r) plan.rorder[cnt[rchunk[r]]++] = (int32_t)r;
     }
-    // every element once; every row node once, in the chunk where its last incident element is evaluated
-    if ((int64_t)plan.eorder.size() != n_el) {
-        set_error("internal error: element order does not cover the elements");
-        return -1;
-    }
-    std::vector<uint8_t> seen(std::max<int64_t>(n_el, nr), 0);
-    for (int32_t p : plan.eorder) {
-        if (p < 0 || p >= n_el || seen[p]) {
-            set_error("internal error: element order is not a permutation");
-            return -1;
-        }
-        seen[p] = 1;
-    }
-    std::fill(seen.begin(), seen.end(), 0);
+    // every row node once, in the chunk where its last incident element is evaluated
+    std::vector<uint8_t> seen(nr, 0);
     for (int64_t c = 0; c < K; ++c)
         for (int64_t q = plan.pipe_row[c]; q < plan.pipe_row[c + 1]; ++q) {
             const int32_t r = plan.rorder[q];
@@ -190,9 +197,7 @@ int build_pipeline(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof
                 return -1;
             }
             seen[r] = 1;
-            int32_t m = 0;
-            for (int64_t k = inc_ptr[rownodes[r]]; k < inc_ptr[rownodes[r] + 1]; ++k) m = std::max(m, el_chunk[inc[k]]);
-            if (m != c) {
+            if (last_chunk(r) != c) {
                 set_error("internal error: a row node is gathered in another chunk than its last element");
                 return -1;
             }
@@ -331,11 +336,15 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
     plan.color_begin.assign(n_colors + 1, 0);
     for (int64_t e = 0; e < n_el; ++e) plan.color_begin[color[e] + 1]++;
     for (int c = 0; c < n_colors; ++c) plan.color_begin[c + 1] += plan.color_begin[c];
-    plan.order.resize(n_el);
-    {
+    if (npe == 8) {
+        element_order8(plan, conn, node_dof);
+    } else {
+        plan.order.resize(n_el);
         std::vector<int64_t> fill(plan.color_begin.begin(), plan.color_begin.end() - 1);
         for (int64_t e = 0; e < n_el; ++e) plan.order[fill[color[e]]++] = e;
     }
+    std::vector<int64_t> pos_of_el(n_el);
+    for (int64_t p = 0; p < n_el; ++p) pos_of_el[plan.order[p]] = p;
 
     // --- SoA device images: element dof bases and the node-pair slot map -------------------------
     edof_host.resize((size_t)npe * n_el);
@@ -422,7 +431,7 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
                     const int64_t el = inc[k];  // ascending element order (incidence built in element order)
                     int a = 0;
                     while (a < 8 && conn[el * 8 + a] != n) ++a;
-                    gather.inc[w] = (int32_t)(el * 8 + a);
+                    gather.inc[w] = (int32_t)(pos_of_el[el] * 8 + a);  // records are stored by plan position
                     for (int bb = 0; bb < 8; ++bb) {
                         const int32_t db = (int32_t)node_dof[conn[el * 8 + bb]];
                         gather.inc_rank[(size_t)w * 8 + bb] = (uint8_t)(std::lower_bound(lo, hi, db) - lo);
@@ -466,7 +475,7 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
                     }
             }
         });
-        const int rc = build_pipeline(plan, conn, node_dof, rownodes, inc_ptr, inc);
+        const int rc = row_order8(plan, rownodes, inc_ptr, inc);
         if (rc) return rc;
         // descriptors and blobs in gather order
         gather.gdesc.resize((size_t)4 * nr);

@@ -82,9 +82,9 @@ int ewfe_plan_create_host(ewfe_plan **plan, int elem_kind, int64_t n_el, const i
                           const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out, int pipeline_chunks);
 
 /* Host copies of the 8-node pipeline orders (tools and tests; no counterpart in the reference):
- * pipe_el, pipe_row int64[n_chunks + 1] chunk boundaries; eorder int32[n_el] element positions in
- * evaluation order; rorder int32[n_row_nodes] row nodes (ascending first dof = row node index) in gather
- * order. Any pointer may be NULL. */
+ * pipe_el, pipe_row int64[n_chunks + 1] chunk boundaries; eorder int32[n_el] element indices in
+ * evaluation order (= the plan order of ewfe_plan_element_order); rorder int32[n_row_nodes] row nodes
+ * (ascending first dof = row node index) in gather order. Any pointer may be NULL. */
 int ewfe_plan_pipeline_orders(const ewfe_plan *plan, int64_t *pipe_el_host, int64_t *pipe_row_host,
                               int32_t *eorder_host, int32_t *rorder_host);
 
