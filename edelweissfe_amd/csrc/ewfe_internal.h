@@ -55,7 +55,8 @@ struct ewfe_plan {
     bool structured = false;  // boxGen-ordered structured hexahedral mesh (tile-streaming orders)
     int64_t grid_el[3] = {0, 0, 0};  // elements per axis (slowest first) when structured
     std::vector<int64_t> pipe_el, pipe_row;
-    std::vector<int32_t> rorder;  // host copy of the row order
+    std::vector<int32_t> rorder;  // host copy of the row order (row node index per gather position)
+    int32_t *d_rorder = nullptr;
     void *aux_stream = nullptr;       // hipStream_t, created on first pipelined evaluation (default priority)
     std::vector<void *> chunk_events;  // hipEvent_t per chunk + 1 join event
 };

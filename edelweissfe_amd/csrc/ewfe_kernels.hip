@@ -444,6 +444,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     constexpr int HALF_STAGE_WORDS = ((EPB / 2) * kRec8 * (int)sizeof(VT) + 7) / 8;
     static_assert(HALF_STAGE_WORDS <= QP_WORDS, "record staging must fit in the quadrature-point area");
     __shared__ double s_qp[QP_WORDS];
+    __shared__ int32_t s_el[EPB];
     const int lane = threadIdx.x;
     const int le = lane >> 3, t = lane & 7;
     const int64_t xb = blockIdx.x >> 3;
@@ -453,6 +454,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     const int64_t pos = q0 + le;
     const bool active = pos < R.q_end;
     const int64_t n = A.n_el;
+    const int64_t el = active ? (int64_t)A.order[pos] : 0;  // records and nodal forces are stored by element
 
     // phase 1: lane (le, q=t) evaluates one quadrature point; the 8 lanes of an element read its nodal
     // X, U (, dU) straight from global memory (same addresses: one fetch per cache line), all loads issued
@@ -609,12 +611,13 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
         for (int i = 0; i < 3; ++i)
             for (int k = 0; k < i; ++k) K[0][i][k] = K[0][k][i];
     }
-    // nodal forces straight from the lanes: element le's 24 values are contiguous over its 8 lanes
+    // nodal forces straight from the lanes (element le's 24 values are contiguous over its 8 lanes)
+    if (t == 0) s_el[le] = (int32_t)el;
     if (active)
-        for (int i = 0; i < 3; ++i) R.fe[pos * 24 + 3 * a + i] = fa[i];
+        for (int i = 0; i < 3; ++i) R.fe[el * 24 + 3 * a + i] = fa[i];
 
     // records: staged through the (now free) quadrature-point area, 4 elements at a time, then written
-    // contiguously (the workgroup's elements are consecutive plan positions)
+    // contiguously per element (consecutive plan positions are mostly consecutive elements)
     VT *stage = reinterpret_cast<VT *>(s_qp);
     const int valid = (R.q_end - q0) < EPB ? (int)(R.q_end - q0) : EPB;
     for (int half = 0; half < 2; ++half) {
@@ -638,10 +641,10 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
         }
         __syncthreads();
         const int nv = valid - 4 * half;  // elements of this half
-        if (nv > 0) {
-            VT *kout = reinterpret_cast<VT *>(R.kbuf) + (q0 + 4 * half) * kRec8;
-            const int nw = (nv < 4 ? nv : 4) * kRec8;
-            for (int w = lane; w < nw; w += 64) kout[w] = stage[w];
+        const int nw = (nv < 4 ? nv : 4) * kRec8;
+        for (int w = lane; w < nw; w += 64) {
+            const int l = w / kRec8;
+            reinterpret_cast<VT *>(R.kbuf)[(int64_t)s_el[4 * half + l] * kRec8 + (w - l * kRec8)] = stage[w];
         }
     }
 }
@@ -657,7 +660,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 constexpr int kMaxDeg8 = 64;
 constexpr int kMaxInc8 = 31;
 constexpr int kMaxBlobWords8 = kMaxInc8 + (kMaxDeg8 + 1 + 8 * kMaxInc8 + 3) / 4;
-constexpr int kBlk = 10;  // doubles per staged 3x3 block (16-byte aligned rows of the accumulation reads)
+constexpr int kBlk = 10;  // doubles per staged 3x3 block (16-byte aligned: the accumulation reads use b128 loads)
 
 template <typename VT>
 __global__ __launch_bounds__(64) void k_gather8(const int4 *__restrict__ gdesc, const int32_t *__restrict__ gblob,
@@ -774,6 +777,133 @@ __global__ __launch_bounds__(64) void k_gather8(const int4 *__restrict__ gdesc, 
     if (live && hl < 3) {
         P[d + hl] = f;
         F[d + hl] = fa;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// C3D8 / C3D8TL, pass 2 on boxGen structured grids (boxgen.py:125-185: node (i0, i1, i2) has index
+// (i0 N1 + i1) N2 + i2 and dofs 3 * index, element (e0, e1, e2) likewise, C3D8 local nodes at fixed corner
+// offsets). Everything the descriptor gather reads from its lists follows from the indices here:
+//   * neighbours of a node: offsets d in {-1, 0, 1}^3 inside the grid; the CSR columns are sorted, so the
+//     rank of d is its lexicographic rank among the valid offsets;
+//   * degree = c0 c1 c2 with c_k = number of valid offsets on axis k, and the CSR offset of the node's rows
+//     = 9 x (sum of the degrees of all lower nodes), a product of per-axis prefix sums;
+//   * the elements holding both the node and neighbour d are the incident elements (corner offset o) with
+//     o_k = 0 where d_k = -1 and o_k = 1 where d_k = +1; ascending o is ascending element index.
+// One row node per 32-lane half, lane m owns neighbour block m and loads its own contributions: a node costs
+// one dependent load (its index in the gather order) before the element blocks. Sums run in ascending
+// element order, as in the descriptor gather: both give bitwise the same CSR values, P and F.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int local_of_offset(int x, int y, int z)
+{
+    // inverse of the C3D8 corner table {(0,0,0),(0,0,1),(1,0,1),(1,0,0),(0,1,0),(0,1,1),(1,1,1),(1,1,0)}
+    constexpr int lut = (0 << 0) | (1 << 3) | (4 << 6) | (5 << 9) | (3 << 12) | (2 << 15) | (7 << 18) | (6 << 21);
+    return (lut >> (3 * (4 * x + 2 * y + z))) & 7;
+}
+
+// sum of c(j) = 1 + [j > 0] + [j < n - 1] over j < i, for a grid axis of n >= 2 nodes
+__device__ __forceinline__ int64_t axis_prefix(int64_t i, int64_t n) { return i == 0 ? 0 : (i >= n ? 3 * n - 2 : 3 * i - 1); }
+
+struct GridArgs {
+    int64_t E0, E1, E2;  // elements per axis (slowest first)
+};
+
+template <typename VT>
+__global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__restrict__ rorder,
+                                                 const VT *__restrict__ kbuf, const double *__restrict__ fe,
+                                                 VT *__restrict__ vals, double *__restrict__ P,
+                                                 double *__restrict__ F, int64_t q_begin, int64_t count,
+                                                 int64_t xcd_chunk)
+{
+    __shared__ double s_blk[2][64 * kBlk];  // per half: block (a, b) of the incident element at corner o, slot 8 o + b
+    __shared__ double s_fe[2][8 * 3];
+    const int64_t k = blockIdx.x >> 3;
+    const int64_t pair = (int64_t)(blockIdx.x & 7) * xcd_chunk + k;
+    if (k >= xcd_chunk || 2 * pair >= count) return;  // uniform over the workgroup
+    const int lane = threadIdx.x, h = lane >> 5, hl = lane & 31;
+    const int64_t idx = 2 * pair + h;
+    const bool live = idx < count;  // the second half may lie past the end of the range
+    const int64_t N1 = G.E1 + 1, N2 = G.E2 + 1, N0 = G.E0 + 1;
+    const int64_t r = live ? (int64_t)rorder[q_begin + idx] : 0;
+    const int64_t i0 = r / (N1 * N2), i1 = (r / N2) % N1, i2 = r % N2;
+    const int lo0 = i0 > 0, lo1 = i1 > 0, lo2 = i2 > 0;
+    const int64_t n12 = G.E1 * G.E2;
+    auto elem = [&](int o, int64_t &e) {  // incident element at corner offset o (0..7); false outside the grid
+        const int64_t e0 = i0 - 1 + (o >> 2), e1 = i1 - 1 + ((o >> 1) & 1), e2 = i2 - 1 + (o & 1);
+        e = e0 * n12 + e1 * G.E2 + e2;
+        return e0 >= 0 && e0 < G.E0 && e1 >= 0 && e1 < G.E1 && e2 >= 0 && e2 < G.E2;
+    };
+    // 1) lane hl fetches blocks hl and hl + 32 (corner o = block >> 3, element node b = block & 7)
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+        const int blk = hl + 32 * rr;
+        const int o = blk >> 3, b = blk & 7;
+        int64_t e;
+        if (live && elem(o, e)) {
+            const int a = local_of_offset(1 - (o >> 2), 1 - ((o >> 1) & 1), 1 - (o & 1));
+            const int lo = a < b ? a : b, hi = a < b ? b : a;
+            const VT *src = kbuf + e * kRec8 + 9 * pair8(lo, hi);
+            double v[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) v[t] = (double)src[t];
+            double *dst = s_blk[h] + blk * kBlk;
+            if (a > b) {  // the stored block is K_ba: its transpose is K_ab
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk) dst[3 * i + kk] = v[3 * kk + i];
+            } else {
+#pragma unroll
+                for (int t = 0; t < 9; ++t) dst[t] = v[t];
+            }
+            if (b < 3) s_fe[h][o * 3 + b] = fe[e * 24 + 3 * a + b];
+        }
+    }
+    __syncthreads();
+    if (!live) return;  // no barrier below
+    const int c0 = 1 + lo0 + (i0 < N0 - 1), c1 = 1 + lo1 + (i1 < N1 - 1), c2 = 1 + lo2 + (i2 < N2 - 1);
+    const int deg = c0 * c1 * c2, L = 3 * deg;
+    const int m = hl;
+    if (m < deg) {
+        // 2) neighbour offset of rank m; its contributions in ascending element (= corner) order
+        const int d0 = m / (c1 * c2) - lo0, d1 = (m / c2) % c1 - lo1, d2 = m % c2 - lo2;
+        const int64_t base = 9 * (axis_prefix(i0, N0) * (3 * N1 - 2) * (3 * N2 - 2) +
+                                  (int64_t)c0 * axis_prefix(i1, N1) * (3 * N2 - 2) +
+                                  (int64_t)c0 * c1 * axis_prefix(i2, N2));
+        double acc[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            const int o0 = o >> 2, o1 = (o >> 1) & 1, o2 = o & 1;
+            int64_t e;
+            if ((d0 == 0 || d0 == 2 * o0 - 1) && (d1 == 0 || d1 == 2 * o1 - 1) && (d2 == 0 || d2 == 2 * o2 - 1) &&
+                elem(o, e)) {
+                const int b = local_of_offset(1 - o0 + d0, 1 - o1 + d1, 1 - o2 + d2);
+                const double *src = s_blk[h] + (8 * o + b) * kBlk;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) acc[t] += src[t];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) vals[base + i * L + 3 * m + kk] = (VT)acc[3 * i + kk];
+    }
+    if (m >= 29) {  // P and F of component m - 29, in ascending element order
+        const int c = m - 29;
+        double f = 0.0, fa = 0.0;
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            int64_t e;
+            if (elem(o, e)) {
+                const double x = s_fe[h][o * 3 + c];
+                f -= x;
+                fa += fabs(x);
+            }
+        }
+        P[3 * r + c] = f;
+        F[3 * r + c] = fa;
     }
 }
 
@@ -1117,6 +1247,7 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
         Up gups[] = {
             {&p->d_gdesc, gh.gdesc.data(), sizeof(int32_t) * gh.gdesc.size()},
             {&p->d_gblob, gh.gblob.data(), sizeof(int32_t) * gh.gblob.size()},
+            {&p->d_rorder, p->rorder.data(), sizeof(int32_t) * p->rorder.size()},
 
         };
         for (auto &u : gups) {
@@ -1229,7 +1360,8 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
     if (!p) return 0;
     void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
                     p->d_rownode_dof, p->d_gdesc, p->d_gblob, p->d_kbuf,
-                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta};
+                    p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta,
+                    p->d_rorder};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     for (void *ev : p->chunk_events)
@@ -1300,9 +1432,16 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s)
         if (count <= 0) continue;
         const int64_t pairs = (count + 1) / 2;
         const int64_t chunk = (pairs + 7) / 8;
-        hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, g,
-                           reinterpret_cast<const int4 *>(p->d_gdesc), p->d_gblob, (const VT *)p->d_kbuf,
-                           (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->pipe_row[c], count, chunk);
+        if (p->structured) {
+            GridArgs G{p->grid_el[0], p->grid_el[1], p->grid_el[2]};
+            hipLaunchKernelGGL((k_gather8s<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, g, G, p->d_rorder,
+                               (const VT *)p->d_kbuf, (const double *)p->d_fe, (VT *)A.vals, A.P, A.F,
+                               p->pipe_row[c], count, chunk);
+        } else {
+            hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, g,
+                               reinterpret_cast<const int4 *>(p->d_gdesc), p->d_gblob, (const VT *)p->d_kbuf,
+                               (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, p->pipe_row[c], count, chunk);
+        }
         HIP_CHECK(hipGetLastError());
     }
     if (K > 1) {

@@ -343,8 +343,7 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
         std::vector<int64_t> fill(plan.color_begin.begin(), plan.color_begin.end() - 1);
         for (int64_t e = 0; e < n_el; ++e) plan.order[fill[color[e]]++] = e;
     }
-    std::vector<int64_t> pos_of_el(n_el);
-    for (int64_t p = 0; p < n_el; ++p) pos_of_el[plan.order[p]] = p;
+
 
     // --- SoA device images: element dof bases and the node-pair slot map -------------------------
     edof_host.resize((size_t)npe * n_el);
@@ -431,7 +430,7 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
                     const int64_t el = inc[k];  // ascending element order (incidence built in element order)
                     int a = 0;
                     while (a < 8 && conn[el * 8 + a] != n) ++a;
-                    gather.inc[w] = (int32_t)(pos_of_el[el] * 8 + a);  // records are stored by plan position
+                    gather.inc[w] = (int32_t)(el * 8 + a);
                     for (int bb = 0; bb < 8; ++bb) {
                         const int32_t db = (int32_t)node_dof[conn[el * 8 + bb]];
                         gather.inc_rank[(size_t)w * 8 + bb] = (uint8_t)(std::lower_bound(lo, hi, db) - lo);

@@ -119,6 +119,24 @@ def test_pipelined_chunks_match_single_pass(mesh, chunks):
     assert rel(out["data"], ref["data"]) < 1e-12 and rel(out["P"], ref["P"]) < 1e-12
 
 
+def test_structured_gather_matches_descriptor_gather():
+    """The boxGen grid gather (index arithmetic) and the list-driven gather give bitwise the same CSR values,
+    P and F: the same mesh with one extra node without elements is not recognised as a grid."""
+    print("case structured vs descriptor gather", flush=True)
+    X, conn, _ = jittered_box((5, 3, 4), "C3D8", seed=11)
+    rng = np.random.default_rng(12)
+    U = _uniaxial_field(X, 4e-3, rng, 2e-4)
+    grid = GpuEval("C3D8", "vonmises", J2, X, conn)
+    X2 = np.concatenate([X, [[9.0, 9.0, 9.0]]])
+    lists = GpuEval("C3D8", "vonmises", J2, X2, conn)
+    assert grid.asm.structuredMesh and not lists.asm.structuredMesh
+    assert lists.dm.nDof == grid.dm.nDof  # the extra node has no element, hence no dofs
+    for dtype in ("fp64", "fp32"):
+        a, b = grid(U, U, dtype), lists(U, U, dtype)
+        for k in ("data", "P", "F", "state"):
+            assert np.array_equal(a[k], b[k]), (dtype, k)
+
+
 def test_fp32_tangent_close_to_fp64():
     X, conn, _ = jittered_box(4, "C3D8", seed=5)
     rng = np.random.default_rng(6)

@@ -253,3 +253,58 @@ def test_pipeline_orders_unstructured(chunks):
     base, nDof, _ = O.dof_numbering(len(X), conn2)
     K, structured, _, _ = _check_pipeline(conn2, base, nDof, chunks)
     assert not structured and K == chunks
+
+
+_CORNER_LUT = [0, 1, 4, 5, 3, 2, 7, 6]  # corner offset index 4x + 2y + z -> C3D8 local node (boxgen.py:171-185)
+
+
+def _axis_prefix(i, n):
+    return 0 if i == 0 else (3 * n - 2 if i >= n else 3 * i - 1)
+
+
+def _structured_gather_plan(E, r):
+    """The structured gather kernel's index arithmetic for row node r of an E[0] x E[1] x E[2] boxGen grid:
+    CSR offset, row length, and per neighbour rank m the neighbour node and the contributing
+    (element, local a, local b) in summation order."""
+    N = [e + 1 for e in E]
+    i = [r // (N[1] * N[2]), (r // N[2]) % N[1], r % N[2]]
+    lo = [int(i[k] > 0) for k in range(3)]
+    c = [1 + lo[k] + int(i[k] < N[k] - 1) for k in range(3)]
+    deg = c[0] * c[1] * c[2]
+    base = 9 * (_axis_prefix(i[0], N[0]) * (3 * N[1] - 2) * (3 * N[2] - 2)
+                + c[0] * _axis_prefix(i[1], N[1]) * (3 * N[2] - 2) + c[0] * c[1] * _axis_prefix(i[2], N[2]))
+    blocks = []
+    for m in range(deg):
+        d = [m // (c[1] * c[2]) - lo[0], (m // c[2]) % c[1] - lo[1], m % c[2] - lo[2]]
+        nb = ((i[0] + d[0]) * N[1] + i[1] + d[1]) * N[2] + i[2] + d[2]
+        contrib = []
+        for o in range(8):
+            oo = [o >> 2, (o >> 1) & 1, o & 1]
+            e = [i[k] - 1 + oo[k] for k in range(3)]
+            if all((d[k] == 0 or d[k] == 2 * oo[k] - 1) and 0 <= e[k] < E[k] for k in range(3)):
+                a = _CORNER_LUT[4 * (1 - oo[0]) + 2 * (1 - oo[1]) + (1 - oo[2])]
+                b = _CORNER_LUT[4 * (1 - oo[0] + d[0]) + 2 * (1 - oo[1] + d[1]) + (1 - oo[2] + d[2])]
+                contrib.append(((e[0] * E[1] + e[1]) * E[2] + e[2], a, b))
+        blocks.append((nb, contrib))
+    return base, 3 * deg, blocks
+
+
+@pytest.mark.parametrize("E", [(1, 1, 1), (3, 2, 4), (2, 5, 1), (4, 4, 4)])
+def test_structured_gather_index_math_matches_pattern(E):
+    X, conn, _ = O.boxgen(*E)
+    base_dof, nDof, _ = O.dof_numbering(len(X), conn)
+    ip, ix, _, _ = host_plan(_lib.EWFE_C3D8, conn, base_dof, nDof)
+    conn = np.asarray(conn)
+    inc = [[] for _ in range(len(X))]
+    for e, nodes in enumerate(conn):
+        for a, nd in enumerate(nodes):
+            inc[nd].append((e, a))
+    for r in range(len(X)):
+        base, L, blocks = _structured_gather_plan(E, r)
+        assert base == ip[3 * r] and L == ip[3 * r + 1] - ip[3 * r] == ip[3 * r + 3] - ip[3 * r + 2]
+        assert len(blocks) * 3 == L
+        for m, (nb, contrib) in enumerate(blocks):
+            assert ix[base + 3 * m] == 3 * nb  # column of neighbour block m
+            # contributions: every element holding both nodes, ascending element, with their local indices
+            ref = [(e, a, list(conn[e]).index(nb)) for e, a in inc[r] if nb in conn[e]]
+            assert contrib == sorted(ref)
