@@ -390,13 +390,38 @@ struct QPLayout {
 
 // ------------------------------------------------------------------------------------------------
 // C3D8 / C3D8TL, pass 1: element records. 8 elements per 64-lane workgroup.
-// Record of element p: kbuf[p][36][3][3] = the 36 unique 3x3 blocks K_ab (a <= b, packed upper triangle,
-// row-major), fe[p][8][3] = internal nodal forces f_a (P_e = -f).
+// Record of element e: kbuf[e][300] = the 36 unique 3x3 blocks K_ab (a <= b) in pair order
+// pair8(a, b) = a 8 - a (a + 1) / 2 + b; off-diagonal blocks row-major (9 values), diagonal blocks (symmetric)
+// as their upper triangle (00, 01, 02, 11, 12, 22: 6 values); fe[e][8][3] = internal nodal forces f_a
+// (P_e = -f).
 // ------------------------------------------------------------------------------------------------
 constexpr int kPairs8 = 36;
-constexpr int kRec8 = kPairs8 * 9;  // 324
+constexpr int kRec8 = 28 * 9 + 8 * 6;  // 300
 
 __host__ __device__ constexpr int pair8(int lo, int hi) { return lo * 8 - lo * (lo + 1) / 2 + hi; }
+// offset of block (lo, hi), lo <= hi, in a record: the diagonal blocks before it are (i, i) for i < lo, and
+// (lo, lo) itself when hi > lo
+__host__ __device__ constexpr int block_off8(int lo, int hi) { return 9 * pair8(lo, hi) - 3 * (lo + (hi > lo ? 1 : 0)); }
+constexpr int kSym6[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};  // (i, k) -> upper-triangle slot
+
+// Reads block K_ab of a record oriented for row node a (transposed when the record holds K_ba). Every lane
+// loads 9 values, whatever the block kind, so the loads of a wave do not diverge; for a diagonal block the
+// last 3 belong to the next block (or to the padding after the last record) and are not used.
+constexpr int kRecPad8 = 3;  // values after the last record of the buffer
+template <typename VT>
+__device__ __forceinline__ void load_block8(const VT *rec, int a, int b, double v[9])
+{
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const VT *src = rec + block_off8(lo, hi);
+    double u[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) u[t] = (double)src[t];
+    const bool diag = a == b, tr = a > b;  // tr: the stored block is K_ba, its transpose is K_ab
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[3 * i + k] = diag ? u[kSym6[i][k]] : (tr ? u[3 * k + i] : u[3 * i + k]);
+}
 
 struct RecordArgs {
     void *kbuf;
@@ -628,14 +653,19 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             for (int j = 0; j < 5; ++j) {
                 if (j < nb) {
                     const int b = (a + j) & 7;
-                    VT *blk = stage + ls * kRec8 + 9 * (a <= b ? pair8(a, b) : pair8(b, a));
-                    for (int i = 0; i < 3; ++i)
-                        for (int k = 0; k < 3; ++k) {
-                            if (a <= b)
-                                blk[3 * i + k] = (VT)K[j][i][k];
-                            else
-                                blk[3 * k + i] = (VT)K[j][i][k];  // store K_ba = K_ab^T
-                        }
+                    VT *blk = stage + ls * kRec8 + (a <= b ? block_off8(a, b) : block_off8(b, a));
+                    if (j == 0) {  // diagonal block: upper triangle
+                        for (int i = 0; i < 3; ++i)
+                            for (int k = i; k < 3; ++k) blk[kSym6[i][k]] = (VT)K[0][i][k];
+                    } else {
+                        for (int i = 0; i < 3; ++i)
+                            for (int k = 0; k < 3; ++k) {
+                                if (a <= b)
+                                    blk[3 * i + k] = (VT)K[j][i][k];
+                                else
+                                    blk[3 * k + i] = (VT)K[j][i][k];  // store K_ba = K_ab^T
+                            }
+                    }
                 }
             }
         }
@@ -723,21 +753,11 @@ __global__ __launch_bounds__(64) void k_gather8(const int4 *__restrict__ gdesc, 
                 const int32_t code = codes[c0 + sl];
                 const int64_t e = code >> 3;
                 const int a = code & 7;
-                const int lo = a < b ? a : b, hi = a < b ? b : a;
-                const VT *src = kbuf + e * kRec8 + 9 * pair8(lo, hi);
                 double v[9];
-#pragma unroll
-                for (int t = 0; t < 9; ++t) v[t] = (double)src[t];
+                load_block8(kbuf + e * kRec8, a, b, v);
                 double *dst = s_blk[h] + blk * kBlk;
-                if (a > b) {  // the stored block is K_ba: its transpose is K_ab
 #pragma unroll
-                    for (int i = 0; i < 3; ++i)
-#pragma unroll
-                        for (int kk = 0; kk < 3; ++kk) dst[3 * i + kk] = v[3 * kk + i];
-                } else {
-#pragma unroll
-                    for (int t = 0; t < 9; ++t) dst[t] = v[t];
-                }
+                for (int t = 0; t < 9; ++t) dst[t] = v[t];
                 if (b < 3) s_fe[h][sl * 3 + b] = fe[e * 24 + 3 * a + b];
             }
         }
@@ -841,21 +861,11 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
         int64_t e;
         if (live && elem(o, e)) {
             const int a = local_of_offset(1 - (o >> 2), 1 - ((o >> 1) & 1), 1 - (o & 1));
-            const int lo = a < b ? a : b, hi = a < b ? b : a;
-            const VT *src = kbuf + e * kRec8 + 9 * pair8(lo, hi);
             double v[9];
-#pragma unroll
-            for (int t = 0; t < 9; ++t) v[t] = (double)src[t];
+            load_block8(kbuf + e * kRec8, a, b, v);
             double *dst = s_blk[h] + blk * kBlk;
-            if (a > b) {  // the stored block is K_ba: its transpose is K_ab
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int kk = 0; kk < 3; ++kk) dst[3 * i + kk] = v[3 * kk + i];
-            } else {
-#pragma unroll
-                for (int t = 0; t < 9; ++t) dst[t] = v[t];
-            }
+            for (int t = 0; t < 9; ++t) dst[t] = v[t];
             if (b < 3) s_fe[h][o * 3 + b] = fe[e * 24 + 3 * a + b];
         }
     }
@@ -1377,7 +1387,7 @@ static int ensure_records(ewfe_plan *p, int dtype)
     if (p->d_kbuf && p->kbuf_dtype == dtype) return 0;
     if (p->d_kbuf) (void)hipFree(p->d_kbuf);
     p->d_kbuf = nullptr;
-    HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(VT) * (size_t)kRec8 * p->n_el));
+    HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(VT) * ((size_t)kRec8 * p->n_el + kRecPad8)));
     p->kbuf_dtype = dtype;
     if (!p->d_fe) HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 24 * (size_t)p->n_el));
     return 0;
