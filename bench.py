@@ -395,7 +395,7 @@ def main():
         with open(tfile) as f:
             tj = json.load(f)
         traffic64 = tj["bytes_per_evaluation"]["fp64"]["total"]
-        traffic_src = tj["source"] + " (k_hex8_elem + k_gather8, bytes per evaluation)"
+        traffic_src = tj["source"] + " (k_hex8_elem + k_gather8s, bytes per evaluation)"
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -464,7 +464,8 @@ def main():
                      "frac": ach64 / HBM_PEAK_GBS, "traffic": traffic64,
                      "traffic_source": traffic_src,
                      "algorithmic_bytes_per_element": B64,
-                     "scope": "one evaluation call: k_hex8_elem (element records) + k_gather8 (node-row CSR assembly)"},
+                     "scope": "one evaluation call: k_hex8_elem (element records) + k_gather8s (row-node CSR "
+                              "assembly on the boxGen grid; k_gather8 on other meshes)"},
         "cpu_baseline": cpu,
         "fp32_tangent": {"value": n_el / (ms32 * 1e-3), "ms_per_step": ms32, "speedup_vs_fp64": ms64 / ms32,
                          "roofline_achieved_GBs": ach32, "roofline_frac": ach32 / HBM_PEAK_GBS,
