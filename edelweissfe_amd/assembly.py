@@ -125,6 +125,22 @@ class ElementAssembly:
                                        self._stream()))
         return self.pnewdt
 
+    def bodyForce(self, PExt, force, elements=None):
+        """PExt += the body force `force` (3 values per unit volume) integrated over `elements` (element indices
+        of this block; all when None), reference configuration (displacementelement/element.py:348-371)."""
+        if PExt.numel() != self.nDof or PExt.dtype != self.torch.float64 or not PExt.is_contiguous():
+            raise ValueError("PExt must be a contiguous float64 tensor of length nDof")
+        mask = None
+        if elements is not None and len(elements) != len(self.block):
+            if getattr(self, "_maskKey", None) is not elements:
+                m = np.zeros(len(self.block), dtype=np.uint8)
+                m[np.asarray(elements, dtype=np.int64)] = 1
+                self._mask = self.torch.as_tensor(m, device=self.device)
+                self._maskKey = elements
+            mask = self._mask
+        f = np.ascontiguousarray(np.asarray(force, dtype=np.float64).ravel()[:3])
+        check(lib().ewfe_body_force(self._plan, ptr(self.X), ptr(mask), ptr(f), ptr(PExt), self._stream()))
+
     def dirichletRows(self, rows_dev, vals):
         check(lib().ewfe_dirichlet_rows(self._plan, ptr(rows_dev), rows_dev.numel(), ptr(vals),
                                         self._dtype_code(vals), self._stream()))

@@ -40,6 +40,9 @@ struct ewfe_plan {
     // ascending element) and contribution lists: cstart[deg + 1] then 8 * n_inc entries (slot << 3 | b),
     // grouped by neighbour rank m (entries of m: cstart[m] .. cstart[m + 1]), ascending incident slot inside
     // each group
+    int32_t *d_ninc_ptr = nullptr;     // [n_rownodes + 1] incident elements of every row node (all kinds)
+    int32_t *d_ninc = nullptr;         // 32 * element + local node, ascending element
+    double *d_bweights = nullptr;      // body-force scratch: nodal volume weights per element (lazy)
     int32_t *d_gdesc = nullptr;        // [n_rownodes][4] gather descriptors in gather order
     int32_t *d_gblob = nullptr;        // gather blobs
     // element records (allocated on first evaluation): 36 unique 3x3 blocks per element, and nodal forces
@@ -74,6 +77,7 @@ struct GatherHost {
     // the blob: incident element codes (8 * element + local node, ascending element) followed by the
     // contribution lists (bytes, padded to a word)
     std::vector<int32_t> gdesc, gblob;
+    std::vector<int32_t> ninc_ptr, ninc;  // row node -> incident elements (32 * element + local node)
 };
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,
                     std::vector<int32_t> &colpos_host, GatherHost &gather);

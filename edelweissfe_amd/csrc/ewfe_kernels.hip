@@ -61,6 +61,8 @@ __constant__ double c_dN8[8][8][3];
 __constant__ double c_w8[8];
 __constant__ double c_dN20[27][20][3];
 __constant__ double c_w27[27];
+__constant__ double c_N8[8][8];    // shape function values at the quadrature points
+__constant__ double c_N20[27][20];
 
 static const double kHexXi[20][3] = {
     {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
@@ -100,6 +102,24 @@ static void host_dN(int nn, const double x[3], double dN[][3])
     }
 }
 
+// Shape function values: trilinear (8 nodes) and serendipity (20 nodes, Abaqus node order)
+static void host_N(int nn, const double x[3], double N[])
+{
+    for (int a = 0; a < nn; ++a) {
+        const double xa = kHexXi[a][0], ya = kHexXi[a][1], za = kHexXi[a][2];
+        if (nn == 8) {
+            N[a] = 0.125 * (1 + xa * x[0]) * (1 + ya * x[1]) * (1 + za * x[2]);
+        } else if (a < 8) {
+            N[a] = 0.125 * (1 + xa * x[0]) * (1 + ya * x[1]) * (1 + za * x[2]) * (xa * x[0] + ya * x[1] + za * x[2] - 2);
+        } else {
+            const double c[3] = {xa, ya, za};
+            double f = 0.25;
+            for (int d = 0; d < 3; ++d) f *= (c[d] == 0) ? (1 - x[d] * x[d]) : (1 + c[d] * x[d]);
+            N[a] = f;
+        }
+    }
+}
+
 static int upload_tables(int device)
 {
     static std::mutex mu;
@@ -107,11 +127,12 @@ static int upload_tables(int device)
     std::lock_guard<std::mutex> lock(mu);
     if (device < 0 || device >= 256) return EWFE_EINVAL;
     if (done[device]) return 0;
-    double dN8[8][8][3], w8[8], dN20[27][20][3], w27[27];
+    double dN8[8][8][3], w8[8], dN20[27][20][3], w27[27], N8[8][8], N20[27][20];
     const double g2 = 1.0 / std::sqrt(3.0), g3 = std::sqrt(0.6);
     for (int q = 0; q < 8; ++q) {
         const double x[3] = {(q & 1) ? g2 : -g2, (q & 2) ? g2 : -g2, (q & 4) ? g2 : -g2};
         host_dN(8, x, dN8[q]);
+        host_N(8, x, N8[q]);
         w8[q] = 1.0;
     }
     const double pts[3] = {-g3, 0.0, g3}, wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
@@ -119,8 +140,11 @@ static int upload_tables(int device)
         const int i = q % 3, j = (q / 3) % 3, k = q / 9;
         const double x[3] = {pts[i], pts[j], pts[k]};
         host_dN(20, x, dN20[q]);
+        host_N(20, x, N20[q]);
         w27[q] = wts[i] * wts[j] * wts[k];
     }
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_N8), N8, sizeof(N8)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_N20), N20, sizeof(N20)));
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN8), dN8, sizeof(dN8)));
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_w8), w8, sizeof(w8)));
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN20), dN20, sizeof(dN20)));
@@ -918,6 +942,71 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
 }
 
 // ------------------------------------------------------------------------------------------------
+// Body force (displacementelement/element.py:348-371, assembled by nonlinearimplicitstatic.py:903-945):
+// P_e,a += sum_q N_a(xi_q) b detJ_q w_q for the elements of the load's element set. The load vector b is
+// constant over the set, so each element contributes b times its nodal volume weights
+// W_e,a = sum_q N_a(xi_q) detJ_q w_q (reference configuration); the second kernel sums them per node in
+// ascending element order and adds b times the sum to PExt.
+// ------------------------------------------------------------------------------------------------
+template <int NN, int NQ>
+__global__ __launch_bounds__(256) void k_body_weights(const int32_t *__restrict__ edof, const int32_t *__restrict__ order,
+                                                      const double *__restrict__ X, const uint8_t *__restrict__ mask,
+                                                      double *__restrict__ W, int64_t n_el)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_el) return;
+    const int64_t e = order[p];
+    double w[NN];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) w[a] = 0.0;
+    if (!mask || mask[e]) {
+        double x[NN][3];
+#pragma unroll
+        for (int a = 0; a < NN; ++a) {
+            const int32_t d = edof[(int64_t)a * n_el + p];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[a][c] = X[d + c];
+        }
+        for (int q = 0; q < NQ; ++q) {
+            const double(*dN)[3] = NN == 8 ? c_dN8[q] : c_dN20[q];
+            const double *Nq = NN == 8 ? c_N8[q] : c_N20[q];
+            const double wq = NN == 8 ? c_w8[q] : c_w27[q];
+            double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int a = 0; a < NN; ++a)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) J[i][j] += dN[a][i] * x[a][j];
+            const double wd = det3(J) * wq;
+#pragma unroll
+            for (int a = 0; a < NN; ++a) w[a] += Nq[a] * wd;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a) W[e * NN + a] = w[a];
+}
+
+__global__ __launch_bounds__(256) void k_body_nodes(const int32_t *__restrict__ rownode_dof,
+                                                    const int32_t *__restrict__ ninc_ptr,
+                                                    const int32_t *__restrict__ ninc, const double *__restrict__ W,
+                                                    int npe, double b0, double b1, double b2,
+                                                    double *__restrict__ PExt, int64_t n_rownodes)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rownodes) return;
+    double v = 0.0;
+    for (int32_t k = ninc_ptr[r]; k < ninc_ptr[r + 1]; ++k) {
+        const int32_t code = ninc[k];  // 32 * element + local node, ascending element
+        v += W[(int64_t)(code >> 5) * npe + (code & 31)];
+    }
+    const int32_t d = rownode_dof[r];
+    PExt[d + 0] += v * b0;
+    PExt[d + 1] += v * b1;
+    PExt[d + 2] += v * b2;
+}
+
+// ------------------------------------------------------------------------------------------------
 // Adjoint: grad[e] = lambda^T dR/dtheta_e for a per-element material parameter theta_e (the first material
 // property: mu of Neo-Hooke Wa on total-Lagrangian hexahedra, E of linear elasticity on small-strain
 // hexahedra). R = P + PExt with P = -f_int, so grad[e] = -sum_a lambda_a . df_a/dtheta_e. The element
@@ -1246,6 +1335,8 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
             {&p->d_nbcol, gh.nbcol.data(), sizeof(int32_t) * gh.nbcol.size()},
             {&p->d_diag_rank, gh.diag_rank.data(), sizeof(int32_t) * gh.diag_rank.size()},
             {&p->d_rn_meta, gh.rn_meta.data(), sizeof(int32_t) * gh.rn_meta.size()},
+            {&p->d_ninc_ptr, gh.ninc_ptr.data(), sizeof(int32_t) * gh.ninc_ptr.size()},
+            {&p->d_ninc, gh.ninc.data(), sizeof(int32_t) * gh.ninc.size()},
         };
         for (auto &u : rups) {
             if ((e = hipMalloc((void **)u.dst, u.bytes ? u.bytes : 4)) != hipSuccess) return fail(e, "hipMalloc");
@@ -1371,7 +1462,7 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
     void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
                     p->d_rownode_dof, p->d_gdesc, p->d_gblob, p->d_kbuf,
                     p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta,
-                    p->d_rorder};
+                    p->d_rorder,   p->d_ninc_ptr, p->d_ninc,   p->d_bweights};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     for (void *ev : p->chunk_events)
@@ -1616,6 +1707,35 @@ extern "C" int ewfe_dirichlet_rows(const ewfe_plan *p, const int32_t *rows_dev, 
     else
         hipLaunchKernelGGL(k_dirichlet_rows<float>, dim3((unsigned)blocks), dim3(256), 0, s, rows_dev, n_rows,
                            p->d_indptr, p->d_indices, reinterpret_cast<float *>(csr_vals_dev));
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ewfe_body_force(const ewfe_plan *p, const double *X_dev, const uint8_t *el_mask_dev,
+                               const double *force_host, double *PExt_dev, void *stream)
+{
+    if (!p || !X_dev || !force_host || !PExt_dev) {
+        ewfe::set_error("ewfe_body_force: null argument");
+        return EWFE_EINVAL;
+    }
+    if (!p->d_edof || !p->d_ninc) {
+        ewfe::set_error("ewfe_body_force: plan has no device data");
+        return EWFE_EINVAL;
+    }
+    ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // scratch weights are allocated lazily by the plan
+    if (!pm->d_bweights) HIP_CHECK(hipMalloc((void **)&pm->d_bweights, sizeof(double) * (size_t)p->npe * p->n_el));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const unsigned eb = (unsigned)((p->n_el + 255) / 256);
+    if (p->npe == 8)
+        hipLaunchKernelGGL((k_body_weights<8, 8>), dim3(eb), dim3(256), 0, s, p->d_edof, p->d_order, X_dev, el_mask_dev,
+                           pm->d_bweights, p->n_el);
+    else
+        hipLaunchKernelGGL((k_body_weights<20, 27>), dim3(eb), dim3(256), 0, s, p->d_edof, p->d_order, X_dev,
+                           el_mask_dev, pm->d_bweights, p->n_el);
+    HIP_CHECK(hipGetLastError());
+    const unsigned nb = (unsigned)((p->n_rownodes + 255) / 256);
+    hipLaunchKernelGGL(k_body_nodes, dim3(nb), dim3(256), 0, s, p->d_rownode_dof, p->d_ninc_ptr, p->d_ninc,
+                       pm->d_bweights, p->npe, force_host[0], force_host[1], force_host[2], PExt_dev, p->n_rownodes);
     HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -409,6 +409,28 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
         }
     });
 
+    // --- row-node incidence (all element kinds): incident elements in ascending element order ----------
+    if (n_el >= ((int64_t)1 << 26)) {
+        set_error("too many elements for the incidence packing");
+        return -4;
+    }
+    gather.ninc_ptr.assign(nr + 1, 0);
+    for (int64_t r = 0; r < nr; ++r)
+        gather.ninc_ptr[r + 1] = gather.ninc_ptr[r] + (int32_t)(inc_ptr[rownodes[r] + 1] - inc_ptr[rownodes[r]]);
+    gather.ninc.resize(gather.ninc_ptr[nr]);
+    parallel_for(nr, [&](int64_t b, int64_t e) {
+        for (int64_t r = b; r < e; ++r) {
+            const int32_t n = rownodes[r];
+            int64_t w = gather.ninc_ptr[r];
+            for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k, ++w) {
+                const int64_t el = inc[k];
+                int a = 0;
+                while (a < npe && conn[el * npe + a] != n) ++a;
+                gather.ninc[w] = (int32_t)(el * 32 + a);
+            }
+        }
+    });
+
     // --- row-node gather lists (8-node elements) ---------------------------------------------------
     if (npe == 8) {
         if (plan.max_deg > 64) {

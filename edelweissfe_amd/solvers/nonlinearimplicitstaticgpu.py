@@ -200,8 +200,9 @@ class NISTGPU:
         ddU = None
         dirichlets = list(stepActions["dirichlet"].values())
         nodeforces = list(stepActions["nodeforces"].values())
-        if stepActions.get("distributedload") or stepActions.get("bodyforce"):
-            raise NotImplementedError("distributed loads and body forces are not on the GPU path yet")
+        bodyforces = list(stepActions.get("bodyforce", {}).values())
+        if stepActions.get("distributedload"):
+            raise NotImplementedError("distributed loads are not on the GPU path yet")
         self.applyStepActionsAtIncrementStart(model, timeStep, stepActions)
         dU, isExtrapolatedIncrement = self.extrapolateLastIncrement(extrapolation, timeStep, dU, dirichlets,
                                                                     prevTimeStep, model)
@@ -211,7 +212,7 @@ class NISTGPU:
             U_np += dU
             PExt.zero_()
             P, K, F = self.computeElements(elements, U_np, dU, P, K, F, timeStep)
-            PExt, K = self.assembleLoads(nodeforces, [], [], U_np, PExt, K, timeStep)
+            PExt, K = self.assembleLoads(nodeforces, [], bodyforces, U_np, PExt, K, timeStep)
             R.copy_(P)
             R += PExt
             if iterationCounter == 0 and not isExtrapolatedIncrement and dirichlets:
@@ -343,11 +344,27 @@ class NISTGPU:
         return x
 
     def assembleLoads(self, nodeForces, distributedLoads, bodyForces, U_np, PExt, K, timeStep):
+        """nonlinearimplicitstatic.py:903-945: nodal forces, then distributed loads, then body forces."""
         for cLoad in nodeForces:
             idx = self.theDofManager.idcsOfFieldsOnNodeSetsInDofVector[cLoad.field][cLoad.nSetName]
             load = cLoad.getCurrentLoad(timeStep).flatten()
             PExt.index_add_(0, self.torch.as_tensor(idx, device=PExt.device),
                             self.torch.as_tensor(load, dtype=self.torch.float64, device=PExt.device))
+        PExt, K = self.computeBodyForces(bodyForces, U_np, PExt, K, timeStep)
+        return PExt, K
+
+    def computeBodyForces(self, bodyForces, U_np, PExt, K, timeStep):
+        """nonlinearimplicitstatic.py:503-557 on the device: every body force adds its integral over its
+        element set (reference configuration) to PExt."""
+        if not bodyForces:
+            return PExt, K
+        tic = _time.perf_counter()
+        for bForce in bodyForces:
+            if bForce.blockIndex != 0:
+                raise NotImplementedError("body forces act on the element block of the GPU path")
+            self.assembly.bodyForce(PExt, bForce.getCurrentLoad(timeStep), bForce.elementSet)
+        self.torch.cuda.synchronize(self.assembly.device)
+        self.computationTimes["body forces"] += _time.perf_counter() - tic
         return PExt, K
 
     def extrapolateLastIncrement(self, extrapolation, timeStep, dU, dirichlets, prevTimeStep, model):

@@ -237,6 +237,54 @@ class NodeForces:
         return self.nodeForcesStepStart + self.nodeForcesDelta * self.amplitude(timeStep.stepProgress)
 
 
+class BodyForce:
+    """``bodyforce, elSet=.., forceVector='bx, by, bz'`` (edelweissfe/stepactions/bodyforce.py:49-116): a force
+    per unit volume on an element set, held at its step-end value afterwards; ``amplitude`` is f(stepProgress).
+    A later step changes it with ``updateStepAction(forceVector=...)`` (total) or ``(delta=...)``."""
+
+    def __init__(self, name, model, elSet: str, vector, amplitude=None):
+        self.name = name
+        if elSet not in model.elementSets:
+            raise KeyError(f"BodyForce {name}: unknown element set {elSet}")
+        self.elSetName = elSet
+        self.blockIndex, self.elements = model.elementSets[elSet]
+        load = np.asarray(vector, dtype=np.float64).ravel()
+        if len(load) < model.domainSize:
+            raise ValueError(f"BodyForce {name}: force vector has wrong dimension!")
+        self.forceAtStepStart = np.zeros(model.domainSize)
+        self.delta = load[: model.domainSize]
+        self.amplitude = amplitude or _linear
+        self._idle = False
+
+    def applyAtStepStart(self, model):
+        pass
+
+    def applyAtIncrementStart(self, model, timeStep):
+        pass
+
+    def applyAtStepEnd(self, model):
+        if not self._idle:
+            self.forceAtStepStart = self.forceAtStepStart + self.delta * self.amplitude(1.0)
+            self.delta = self.delta * 0.0
+            self._idle = True
+
+    def updateStepAction(self, forceVector=None, delta=None, amplitude=None):
+        if forceVector is not None:
+            self.delta = np.asarray(forceVector, dtype=np.float64).ravel()[:3] - self.forceAtStepStart
+        elif delta is not None:
+            self.delta = np.asarray(delta, dtype=np.float64).ravel()[:3]
+        self.amplitude = amplitude or _linear
+        self._idle = False
+
+    def getCurrentLoad(self, timeStep: TimeStep):
+        t = 1.0 if self._idle else timeStep.stepProgress
+        return self.forceAtStepStart + self.delta * self.amplitude(t)
+
+    @property
+    def elementSet(self):
+        return self.elements
+
+
 class Options:
     """``options, category=NISTSolver, key=value`` (edelweissfe/stepactions/options.py:38-61)."""
 

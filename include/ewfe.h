@@ -182,6 +182,14 @@ int ewfe_pcg_mg(ewfe_mg *mg, const void *csr_vals_dev, int vals_dtype, const dou
                 const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every,
                 void *workspace_dev, int *iters_out, double *relres_out, void *stream);
 
+/* Body force (edelweissfe/elements/displacementelement/element.py:348-371, assembled per element set by
+ * edelweissfe/solvers/nonlinearimplicitstatic.py:903-945): PExt += sum over the elements with
+ * el_mask_dev[element] != 0 (all elements when NULL; mask indexed by element) of
+ * P_e,a = sum_q N_a(xi_q) b detJ_q w_q, b = force_host[0..2], reference configuration, summed per node in
+ * ascending element order. */
+int ewfe_body_force(const ewfe_plan *plan, const double *X_dev, const uint8_t *el_mask_dev, const double *force_host,
+                    double *PExt_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -484,3 +484,34 @@ int64_t oracle_eval_assemble(int ek, int mk, const double *mp, const double *el_
     free(Pe_all);
     return first_fail;
 }
+
+/* Body force, restating DisplacementElement.computeBodyForce (edelweissfe/elements/displacementelement/
+ * element.py:348-371): P_e += sum_q outer(N(xi_q), b) detJ_q w_q (reference configuration), and its assembly
+ * NIST.computeBodyForces (edelweissfe/solvers/nonlinearimplicitstatic.py:503-557): PExt[el] += Pe over the
+ * load's element set, in element order. mask: element -> 0/1 (NULL: every element).
+ * Pinned by the closed forms of a rectangular brick (V/8 per C3D8 node; -V/8 per C3D20 corner and V/6 per
+ * mid-side node) and by the reference's WallShearHexa8 / WallShearHexa20 decks (U.ref). */
+void oracle_body_force(int ek, int64_t n_el, const int32_t *conn, const double *X, const int64_t *edofs,
+                       const uint8_t *mask, const double *b, double *PExt)
+{
+    const int nn = el_nnodes(ek), nq = el_nqp(ek), nd = 3 * nn;
+    for (int64_t e = 0; e < n_el; ++e) {
+        if (mask && !mask[e]) continue;
+        double Xe[3 * MAXN], Pe[MAXD];
+        for (int a = 0; a < nn; ++a)
+            for (int c = 0; c < 3; ++c) Xe[3 * a + c] = X[3 * (int64_t)conn[e * nn + a] + c];
+        for (int k = 0; k < nd; ++k) Pe[k] = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            double p[3], w, dN[MAXN][3], N[MAXN], J[3][3] = {{0}};
+            gauss_point(nq, q, p, &w);
+            shape_deriv(nn, p, dN, N);
+            for (int a = 0; a < nn; ++a)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) J[i][j] += dN[a][i] * Xe[3 * a + j];
+            const double detJ = det3(J);
+            for (int a = 0; a < nn; ++a)
+                for (int c = 0; c < 3; ++c) Pe[3 * a + c] += N[a] * b[c] * detJ * w;
+        }
+        for (int k = 0; k < nd; ++k) PExt[edofs[e * nd + k]] += Pe[k];
+    }
+}

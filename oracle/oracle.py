@@ -70,6 +70,9 @@ def lib():
             ctypes.c_int, ctypes.c_int, d, d, ctypes.c_int64, i32, d, i64, d, d, d, d, ctypes.c_int, d, d, d, i32,
             ctypes.c_int64, d, ctypes.c_int, d, d,
         ]
+        L.oracle_body_force.restype = None
+        L.oracle_body_force.argtypes = [ctypes.c_int, ctypes.c_int64, i32, d, i64, ctypes.POINTER(ctypes.c_uint8), d,
+                                        d]
         _lib = L
     return _lib
 
@@ -227,6 +230,24 @@ def eval_assemble(el_kind, mat_kind, mp, conn, X, edofs, U_np, dU, state_n, x_sl
     return out
 
 
+def body_force(el_kind, conn, X, edofs, nDof, b, elements=None):
+    """DisplacementElement.computeBodyForce + NIST.computeBodyForces (element.py:348-371,
+    nonlinearimplicitstatic.py:503-557): the external load vector of a body force b on `elements` (all when
+    None). Pinned by brick closed forms and the WallShear decks (tests/test_oracle.py)."""
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    mask = None
+    if elements is not None:
+        mask = np.zeros(len(conn), dtype=np.uint8)
+        mask[np.asarray(elements, dtype=np.int64)] = 1
+    PExt = np.zeros(nDof)
+    lib().oracle_body_force(EL_KINDS[el_kind], len(conn), _p(conn, ctypes.c_int32),
+                            _p(np.ascontiguousarray(X, dtype=np.float64)),
+                            _p(np.ascontiguousarray(edofs, dtype=np.int64), ctypes.c_int64),
+                            _p(mask, ctypes.c_uint8) if mask is not None else None,
+                            _p(np.ascontiguousarray(b, dtype=np.float64)), _p(PExt))
+    return PExt
+
+
 # ----------------------------------------------------------------------------------------------
 # Newton loop (NIST).  Pinned by the reference's U.ref decks (tests/golden, 1e-6 abs).
 # ----------------------------------------------------------------------------------------------
@@ -363,6 +384,9 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
                 PExt = np.zeros(nDof)
                 for idx, load in nodeforces:
                     PExt[idx] += load * tstep["stepProgress"]
+                for bvec in step.get("bodyforces", []):
+                    PExt += body_force(model["el_kind"], conn, X, edofs, nDof,
+                                       np.asarray(bvec, float) * tstep["stepProgress"])
                 R = Pv + PExt
                 if it == 0 and not isExtrap and dirichlets:
                     for idx, delta in dirichlets:

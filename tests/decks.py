@@ -39,6 +39,10 @@ def step(fx, sets):
     st = dict(fx["step"])
     st["dirichlet"] = [(sets[d["nSet"]], {int(k): v for k, v in d["components"].items()}) for d in st["dirichlet"]]
     st["nodeforces"] = [(sets[d["nSet"]], d["vector"]) for d in st["nodeforces"]]
+    for d in st.get("bodyforces", []):
+        if d["elSet"] != "all":
+            raise NotImplementedError("deck body forces on element sets other than 'all'")
+    st["bodyforces"] = [d["vector"] for d in st.get("bodyforces", [])]
     return st
 
 

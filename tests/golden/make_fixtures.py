@@ -15,7 +15,8 @@ import json
 import os
 import sys
 
-DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C3D8", "C3D20"]
+DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C3D8", "C3D20", "WallShearHexa8",
+         "WallShearHexa20"]
 
 
 def read_keywords(path):
@@ -105,7 +106,7 @@ def convert(deck_dir, name):
                   if k.lower() in o}
             st = {k: (int(float(v)) if k in ("maxNumInc", "maxIter") else float(v)) for k, v in st.items()}
             st["extrapolation"] = "linear"
-            st["dirichlet"], st["nodeforces"] = [], []
+            st["dirichlet"], st["nodeforces"], st["bodyforces"] = [], [], []
             for ln in b["data"]:
                 kind, lo = parse_line_opts(ln)
                 if kind == "options" and "extrapolation" in lo:
@@ -116,6 +117,9 @@ def convert(deck_dir, name):
                 elif kind == "nodeforces":
                     vec = [float(t) for t in lo["components"].split(",")]
                     st["nodeforces"].append({"nSet": lo["nSet"], "vector": vec})
+                elif kind == "bodyforce":
+                    vec = [float(t) for t in lo["forceVector"].split(",")]
+                    st["bodyforces"].append({"elSet": lo["elSet"], "vector": vec})
             fx["step"] = st
     if nodes:
         fx["nodes"] = nodes

@@ -202,17 +202,19 @@ def test_pcg_solves_row_eliminated_system():
 
 def _deck_run(name, tangent="fp64", solver_options=None):
     from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
-    from edelweissfe_amd.stepping import AdaptiveStep, Dirichlet, NodeForces, Options
+    from edelweissfe_amd.stepping import AdaptiveStep, BodyForce, Dirichlet, NodeForces, Options
 
     fx = decks.load(name)
     X, conn, sets = decks.mesh(fx, O.boxgen)
     model, block = one_block_model(fx["el_type"], fx["material"], fx["mp"], X, conn, sets=sets)
     st = fx["step"]
-    acts = {"dirichlet": {}, "nodeforces": {}, "options": {}}
+    acts = {"dirichlet": {}, "nodeforces": {}, "bodyforce": {}, "options": {}}
     for i, d in enumerate(st["dirichlet"]):
         acts["dirichlet"][f"d{i}"] = Dirichlet(f"d{i}", model, d["nSet"], {int(k): v for k, v in d["components"].items()})
     for i, d in enumerate(st["nodeforces"]):
         acts["nodeforces"][f"f{i}"] = NodeForces(f"f{i}", model, d["nSet"], d["vector"])
+    for i, d in enumerate(st.get("bodyforces", [])):
+        acts["bodyforce"][f"b{i}"] = BodyForce(f"b{i}", model, d["elSet"], d["vector"])
     acts["options"]["NISTSolver"] = Options("o", "NISTSolver", extrapolation=st["extrapolation"])
     kw = {k: st[k] for k in ("maxInc", "minInc", "maxNumInc", "maxIter", "stepLength", "startInc") if k in st}
     step = AdaptiveStep(1, model, acts, **kw)
@@ -221,7 +223,8 @@ def _deck_run(name, tangent="fp64", solver_options=None):
     return fx, model.U.cpu().numpy(), solver
 
 
-@pytest.mark.parametrize("name", ["TensionBarHexa8", "CantileverBeamHexa8", "C3D20", "C3D8", "SimpleBeamHexa8"])
+@pytest.mark.parametrize("name", ["TensionBarHexa8", "CantileverBeamHexa8", "C3D20", "C3D8", "SimpleBeamHexa8",
+                                  "WallShearHexa8", "WallShearHexa20"])
 def test_reference_decks_on_gpu(name):
     print(f"deck {name}", flush=True)
     fx, U, solver = _deck_run(name)
@@ -230,12 +233,39 @@ def test_reference_decks_on_gpu(name):
     X, conn, sets = decks.mesh(fx, O.boxgen)
     r = O.solve_step(dict(el_kind=fx["el_type"], mat_kind=fx["material"], mp=fx["mp"], X=X, conn=conn),
                      decks.step(fx, sets))
-    assert rel(U, r["U"]) < 1e-9
+    # the WallShear decks are thin walls (elements 50 x 50 x 2.5): the oracle's direct solve and the PCG solve
+    # agree to ~1e-7 there (the direct solve itself leaves ~1e-12 on the fixed dofs); elsewhere to 1e-9
+    tol = 1e-6 if name.startswith("WallShear") else 1e-9
+    assert rel(U, r["U"]) < tol
     its_gpu = [len(h) for h in solver.history]
     its_ref = [len(h) for h in r["history"]]
     print("iterations per increment gpu", its_gpu, "oracle", its_ref, flush=True)
     # a convergence check that lands within rounding of a tolerance may flip by one iteration
     assert len(its_gpu) == len(its_ref) and max(abs(a - b) for a, b in zip(its_gpu, its_ref)) <= 1
+
+
+@pytest.mark.parametrize("el", ["C3D8", "C3D20"])
+def test_body_force_matches_oracle(el):
+    """ewfe_body_force on a jittered mesh (all elements, then every third element) against the oracle's
+    restatement of computeBodyForce + computeBodyForces; PExt is accumulated, not overwritten."""
+    import torch
+
+    print(f"case body force {el}", flush=True)
+    X, conn, _ = jittered_box((3, 2, 4) if el == "C3D8" else (2, 2, 1), el, seed=13)
+    g = GpuEval(el, "linearelastic", LE, X, conn)
+    base, nDof, edofs = O.dof_numbering(len(X), conn)
+    b = np.array([0.7, -2.1, 1.3])
+    PExt = g.dm.constructDofVector()
+    PExt += 1.0  # body forces add to what is there (nodal forces come first)
+    g.asm.bodyForce(PExt, b)
+    torch.cuda.synchronize()
+    ref = O.body_force(el, conn, X, edofs, nDof, b) + 1.0
+    assert rel(PExt.cpu().numpy(), ref) < 1e-13
+    sub = np.arange(0, len(conn), 3)
+    PExt.zero_()
+    g.asm.bodyForce(PExt, b, sub)
+    torch.cuda.synchronize()
+    assert rel(PExt.cpu().numpy(), O.body_force(el, conn, X, edofs, nDof, b, sub)) < 1e-13
 
 
 def test_fp32_tangent_newton_reaches_fp64_tolerance():

@@ -20,7 +20,8 @@ def _build():
     O.build()
 
 
-@pytest.mark.parametrize("name", ["TensionBarHexa8", "CantileverBeamHexa8", "C3D20", "C3D8", "SimpleBeamHexa8"])
+@pytest.mark.parametrize("name", ["TensionBarHexa8", "CantileverBeamHexa8", "C3D20", "C3D8", "SimpleBeamHexa8",
+                                  "WallShearHexa8", "WallShearHexa20"])
 def test_reference_decks(name):
     fx = decks.load(name)
     X, conn, sets = decks.mesh(fx, O.boxgen)
@@ -152,3 +153,32 @@ def test_csr_pattern_bruteforce(n, nn):
     if nn == 8:
         # nnz of a structured hex mesh: 9 * prod over axes of (3*nodes - 2)
         assert len(indices) == 9 * (3 * (n + 1) - 2) ** 3
+
+
+@pytest.mark.parametrize("nn", [8, 20])
+def test_body_force_brick_closed_form(nn):
+    """Consistent nodal loads of a uniform body force on rectangular bricks: V/8 at every C3D8 node;
+    -V/8 at C3D20 corners and V/6 at mid-side nodes; the loads of a mesh sum to b times its volume."""
+    L = (2.0, 3.0, 0.5)
+    X, conn, _ = O.boxgen(1, 1, 1, *L, nnodes=nn)
+    base, nDof, edofs = O.dof_numbering(len(X), conn)
+    b = np.array([0.3, -1.7, 2.5])
+    P = O.body_force("C3D8" if nn == 8 else "C3D20", conn, X, edofs, nDof, b).reshape(-1, 3)
+    V = L[0] * L[1] * L[2]
+    node_of_local = np.asarray(conn[0])
+    rows = np.array([base[n] // 3 for n in node_of_local])
+    w = np.full(nn, V / 8) if nn == 8 else np.r_[np.full(8, -V / 8), np.full(12, V / 6)]
+    assert np.allclose(P[rows], w[:, None] * b[None, :], rtol=1e-13, atol=1e-14)
+    # a jittered 3x2x2 mesh: total load = b x volume (partition of unity; the Gauss rule integrates detJ exactly)
+    X2, conn2, _ = O.boxgen(3, 2, 2, *L, nnodes=nn)
+    rng = np.random.default_rng(3)
+    inner = np.all((X2 > 1e-9) & (X2 < np.array(L) - 1e-9), axis=1)
+    X2 = X2 + inner[:, None] * rng.uniform(-0.05, 0.05, X2.shape)
+    base2, nDof2, edofs2 = O.dof_numbering(len(X2), conn2)
+    P2 = O.body_force("C3D8" if nn == 8 else "C3D20", conn2, X2, edofs2, nDof2, b).reshape(-1, 3)
+    assert np.allclose(P2.sum(axis=0), b * V, rtol=1e-12)
+    # element subset: the loads of the two halves add up to the whole
+    half = np.arange(len(conn2)) % 2 == 0
+    Pa = O.body_force("C3D8" if nn == 8 else "C3D20", conn2, X2, edofs2, nDof2, b, np.nonzero(half)[0])
+    Pb = O.body_force("C3D8" if nn == 8 else "C3D20", conn2, X2, edofs2, nDof2, b, np.nonzero(~half)[0])
+    assert np.allclose(Pa + Pb, P2.ravel(), rtol=1e-13, atol=1e-15)
