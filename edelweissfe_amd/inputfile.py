@@ -1,0 +1,384 @@
+"""Input decks in the reference's Abaqus-like syntax, run on the GPU path.
+
+The subset of the reference's input language that the GPU path can run, with the reference's structure:
+
+  * ``parseInputFile`` (edelweissfe/utils/inputfileparser.py:255-363): keyword lines ``*keyword, opt=val`` split
+    at commas outside quotes, data lines appended to the last keyword, ``**`` comments, ``*include`` parsed in
+    place; keywords and option names are case-insensitive (stored lower case here);
+  * ``createModelFromInputFile`` (edelweissfe/helpers/inputfilehelpers.py:169-211 and
+    edelweissfe/generators/abqmodelconstructor.py:67-265): boxGen generators first, then ``*node`` (label order
+    of definition), ``*element``, ``*nSet`` / ``*elSet`` (labels, ``generate=True``, or names of other sets), the
+    implicit sets ``all``, materials and sections;
+  * ``createStepsFromInputFile`` (inputfilehelpers.py:214-244, edelweissfe/steps/stepmanager.py:133-200): step
+    actions live across steps, keyed by module and name; a redefinition in a later step updates the action;
+  * ``finiteElementSimulation`` (edelweissfe/drivers/inputfiledrivensimulation.py:56-217) without output
+    managers: every ``*step`` is solved by NISTGPU, whatever CPU solver (NIST, NISTParallel, ...) the deck names;
+  * ``runReferenceTest`` (edelweissfe/_cli/_run_tests_edelweissfe.py:88-109): the final nodal displacements
+    against ``U.ref``, 1e-6 absolute.
+
+Decks outside the subset (other element types or providers' features, several element blocks or materials,
+constraints, distributed loads, analytical fields, ...) raise NotImplementedError.
+"""
+
+from __future__ import annotations
+
+import ast
+import math
+import operator
+import os
+import shlex
+
+import numpy as np
+
+from .model import ElementBlock, FEModel, Material, boxGenArrays
+
+# keyword -> option names it accepts; the output keywords are read and not acted on
+_KEYWORDS = {
+    "*node": {"nset"},
+    "*element": {"elset", "type", "provider"},
+    "*elset": {"elset", "generate", "boolean", "newelset"},
+    "*nset": {"nset", "generate"},
+    "*surface": {"type", "name"},
+    "*section": {"name", "thickness", "material", "type"},
+    "*material": {"name", "id", "statevars", "provider"},
+    "*fieldoutput": set(),
+    "*output": {"name", "type", "jobname"},
+    "*job": {"name", "domain", "solver", "starttime"},
+    "*solver": {"name", "solver"},
+    "*step": {"steplength", "startinc", "maxinc", "mininc", "maxnuminc", "maxiter", "type", "solver", "criticaliter",
+              "cutbackfactor", "jobname", "maxgrowiter"},
+    "*updateconfiguration": {"configuration"},
+    "*modelgenerator": {"generator", "name", "executeaftermanualgeneration"},
+    "*include": {"input"},
+    "*analyticalfield": {"name", "type"},
+    "*constraint": {"name", "type"},
+    "*advancedmaterial": {"name", "id", "provider"},
+}
+
+# material names of the reference's providers -> materials of the GPU path
+_MATERIALS = {"linearelastic": "linearelastic", "vonmises": "vonmises", "neohookeanwa": "neohookeanwa",
+              "neohookeanpencegouformulationa": "neohookeanwa"}
+# solvers a deck may name; all run as NISTGPU here (edelweissfe/config/solvers.py:48-76)
+_SOLVERS = {"nist", "nistparallel", "nistparallelformarmotelements", "nistgpu"}
+
+
+def splitLineAtCommas(line: str) -> list:
+    """edelweissfe/utils/misc.py:45-65: split at commas outside quotes, strip the parts."""
+    lexer = shlex.shlex(line, posix=True)
+    lexer.whitespace_split = True
+    lexer.whitespace = ","
+    return [x.strip() for x in lexer]
+
+
+def _assignments(parts) -> dict:
+    """misc.py:68-91: 'a=b' -> {a: b}; a bare word maps to 'True'."""
+    out = {}
+    for entry in parts:
+        k, *v = [x.strip() for x in entry.split("=")]
+        out[k] = "=".join(v) if v else "True"
+    return out
+
+
+def parseInputFile(fileName: str, currentKeyword: str | None = None, existingFileDict: dict | None = None) -> dict:
+    """Parsed deck: lower-case keyword -> list of entries (dict: lower-case options, 'data' lines, 'inputFile')."""
+    fileDict = existingFileDict if existingFileDict is not None else {k: [] for k in _KEYWORDS}
+    keyword = currentKeyword
+    with open(fileName) as f:
+        for raw in f:
+            line = raw.strip()
+            if not line or line.startswith("**"):
+                continue
+            if line.startswith("*"):
+                parts = splitLineAtCommas(line)
+                lastKeyword, keyword = keyword, parts[0].lower()
+                if keyword not in _KEYWORDS:
+                    raise KeyError(f"keyword {parts[0]} is not part of the input language of the GPU path")
+                entry = {"data": [], "inputFile": fileName}
+                for ass in parts[1:]:
+                    if not ass:
+                        continue
+                    k, _, v = ass.partition("=")
+                    k = k.strip().lower()
+                    if k not in _KEYWORDS[keyword]:
+                        raise KeyError(f'option "{k}" not valid for {keyword}')
+                    entry[k] = v.strip()
+                if keyword == "*include":
+                    parseInputFile(os.path.join(os.path.dirname(fileName), entry["input"]), lastKeyword, fileDict)
+                    keyword = lastKeyword
+                else:
+                    fileDict[keyword].append(entry)
+            else:
+                if keyword is None:
+                    raise KeyError(f"data line before any keyword in {fileName}")
+                fileDict[keyword][-1]["data"].append(line)
+    return fileDict
+
+
+def _numbers(lines, dtype=float):
+    return np.array([dtype(t) for ln in lines for t in splitLineAtCommas(ln) if t], dtype=dtype)
+
+
+def _is_int(s):
+    try:
+        int(s)
+        return True
+    except ValueError:
+        return False
+
+
+def _truthy(v):
+    return str(v).strip().lower() in ("true", "yes", "1", "on")
+
+
+class _Deck:
+    """Geometry collected from a deck before it becomes an FEModel (node labels in definition order)."""
+
+    def __init__(self):
+        self.nodeLabels, self.coords = [], []
+        self.nodeIndex = {}
+        self.elType, self.elLabels, self.elNodes = None, [], []
+        self.nodeSets, self.elementSets = {}, {}
+
+    def addNodes(self, labels, coords):
+        for lab, x in zip(labels, coords):
+            if lab in self.nodeIndex:  # a redefinition keeps the node's place (dict.update semantics)
+                self.coords[self.nodeIndex[lab]] = x
+            else:
+                self.nodeIndex[lab] = len(self.nodeLabels)
+                self.nodeLabels.append(lab)
+                self.coords.append(x)
+
+    def addElements(self, elType, labels, nodeLists):
+        et = elType.upper()
+        if self.elType not in (None, et):
+            raise NotImplementedError("decks with several element types need several element blocks")
+        self.elType = et
+        self.elLabels += list(labels)
+        self.elNodes += list(nodeLists)
+
+
+def _boxgen(deck: _Deck, gen: dict):
+    """edelweissfe/generators/boxgen.py:91-448 through model.boxGenArrays; node labels continue after the
+    highest existing label, element labels likewise."""
+    opts = _assignments(gen["data"])  # 'nX = 4' lines
+    name = gen.get("name", "boxGen")
+    g = lambda k, d: opts.get(k, d)  # noqa: E731
+    elType = g("elType", None)
+    if elType is None:
+        raise KeyError("boxGen needs elType")
+    X, conn, sets, _ = boxGenArrays(int(g("nX", 1)), int(g("nY", 1)), int(g("nZ", 1)), float(g("lX", 1.0)),
+                                    float(g("lY", 1.0)), float(g("lZ", 1.0)), float(g("x0", 0.0)), float(g("y0", 0.0)),
+                                    float(g("z0", 0.0)), elType=elType, name=name)
+    n0 = max(deck.nodeLabels, default=0)
+    labels = list(range(n0 + 1, n0 + 1 + len(X)))
+    deck.addNodes(labels, list(X))
+    e0 = max(deck.elLabels, default=0)
+    first = len(deck.elLabels)
+    deck.addElements(elType, range(e0 + 1, e0 + 1 + len(conn)), [[labels[i] for i in row] for row in conn])
+    for k, idx in sets.items():
+        deck.nodeSets[k] = [labels[i] for i in idx]
+    deck.elementSets[f"{name}_all"] = [deck.elLabels[first + i] for i in range(len(conn))]
+
+
+def createModelFromInputFile(inputFile: dict) -> FEModel:
+    """FEModel of a parsed deck: nodes, one element block with its material, node and element sets."""
+    for job in inputFile["*job"]:
+        if job.get("domain", "3d").lower() != "3d":
+            raise NotImplementedError("the GPU path covers 3D solid elements only")
+    deck = _Deck()
+    gens = inputFile["*modelgenerator"]
+    for gen in gens:
+        if gen["generator"].lower() != "boxgen":
+            raise NotImplementedError(f"model generator {gen['generator']} is not part of the GPU path")
+    for gen in (g for g in gens if not _truthy(g.get("executeaftermanualgeneration", "false"))):
+        _boxgen(deck, gen)
+    for nd in inputFile["*node"]:
+        rows = [splitLineAtCommas(ln) for ln in nd["data"]]
+        labels = [int(r[0]) for r in rows]
+        deck.addNodes(labels, [np.array([float(t) for t in r[1:4]] + [0.0] * (4 - len(r))) for r in rows])
+        if "nset" in nd:
+            deck.nodeSets[nd["nset"]] = labels
+    for el in inputFile["*element"]:
+        rows = [[int(t) for t in splitLineAtCommas(ln) if t] for ln in el["data"]]
+        deck.addElements(el["type"], [r[0] for r in rows], [r[1:] for r in rows])
+        if "elset" in el:
+            deck.elementSets[el["elset"]] = [r[0] for r in rows]
+    for es in inputFile["*elset"]:
+        data = [t for ln in es["data"] for t in splitLineAtCommas(ln) if t]
+        if "boolean" in es:
+            raise NotImplementedError("boolean element set operations are not part of the GPU path")
+        if data and _is_int(data[0]):
+            nums = [int(t) for t in data]
+            deck.elementSets[es["elset"]] = (list(range(nums[0], nums[1] + 1, nums[2])) if _truthy(es.get("generate", "")) else nums)
+        else:
+            deck.elementSets[es["elset"]] = [lab for s in data for lab in deck.elementSets[s]]
+    for ns in inputFile["*nset"]:
+        data = [t for ln in ns["data"] for t in splitLineAtCommas(ln) if t]
+        if data and _is_int(data[0]):
+            nums = [int(t) for t in data]
+            deck.nodeSets[ns["nset"]] = (list(range(nums[0], nums[1] + 1, nums[2])) if _truthy(ns.get("generate", "")) else nums)
+        else:
+            deck.nodeSets[ns["nset"]] = [lab for s in data for lab in deck.nodeSets[s]]
+    for gen in (g for g in gens if _truthy(g.get("executeaftermanualgeneration", "false"))):
+        _boxgen(deck, gen)
+    if deck.elType is None:
+        raise ValueError("the deck defines no elements")
+
+    model = FEModel()
+    model.addNodes(np.array(deck.coords, dtype=np.float64).reshape(-1, 3), np.array(deck.nodeLabels))
+    elIndex = {lab: i for i, lab in enumerate(deck.elLabels)}
+    conn = np.array([[deck.nodeIndex[n] for n in nodes] for nodes in deck.elNodes], dtype=np.int32)
+    block = ElementBlock(deck.elType, conn, labels=np.array(deck.elLabels))
+    model.addElementBlock(block, "all")
+    for name, labs in deck.elementSets.items():
+        model.elementSets[name] = (0, np.array(sorted(elIndex[lab] for lab in labs), dtype=np.int64))
+    for name, labs in deck.nodeSets.items():
+        model.nodeSets[name] = np.array([deck.nodeIndex[lab] for lab in labs], dtype=np.int64)
+    model.nodeSets["all"] = np.arange(model.nNodes)
+
+    # materials (abqmodelconstructor.py:241-265) and sections (edelweissfe/sections/solid.py:45-64)
+    materials = {}
+    for m in inputFile["*material"]:
+        key = m["name"].lower()
+        if key not in _MATERIALS:
+            raise NotImplementedError(f"material {m['name']} is not available on the GPU path")
+        materials[m.get("id", m["name"])] = Material(_MATERIALS[key], _numbers(m["data"]))
+    assigned = np.zeros(len(block), dtype=bool)
+    chosen = None
+    for sec in inputFile["*section"]:
+        if sec.get("type", "solid").lower() != "solid":
+            raise NotImplementedError(f"section type {sec.get('type')} is not part of the GPU path")
+        mat = materials[sec["material"]]
+        for setName in (t for ln in sec["data"] for t in splitLineAtCommas(ln) if t):
+            _, idx = model.elementSets[setName]
+            if chosen is not None and mat is not chosen:
+                raise NotImplementedError("one material per model on the GPU path (one element block)")
+            chosen = mat
+            assigned[idx] = True
+    if chosen is None or not assigned.all():
+        raise ValueError("every element needs a section with a material")
+    block.setMaterial(chosen)
+    return model
+
+
+class _Amplitude:
+    """f(t) of a step action: an arithmetic expression in t (numbers, + - * / **, sqrt, exp, sin, cos, ...),
+    evaluated without executing anything from the deck (the reference lambdifies it with sympy)."""
+
+    _ops = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul, ast.Div: operator.truediv,
+            ast.Pow: operator.pow, ast.USub: operator.neg, ast.UAdd: operator.pos}
+    _funcs = {"sqrt": math.sqrt, "exp": math.exp, "sin": math.sin, "cos": math.cos, "tan": math.tan,
+              "log": math.log, "abs": abs, "min": min, "max": max}
+
+    def __init__(self, expr: str):
+        self.expr = expr
+        self.tree = ast.parse(expr, mode="eval").body
+        self(0.5)  # reject unsupported expressions early
+
+    def _eval(self, node, t):
+        if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)):
+            return node.value
+        if isinstance(node, ast.Name) and node.id == "t":
+            return t
+        if isinstance(node, ast.Name) and node.id == "pi":
+            return math.pi
+        if isinstance(node, ast.BinOp) and type(node.op) in self._ops:
+            return self._ops[type(node.op)](self._eval(node.left, t), self._eval(node.right, t))
+        if isinstance(node, ast.UnaryOp) and type(node.op) in self._ops:
+            return self._ops[type(node.op)](self._eval(node.operand, t))
+        if (isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in self._funcs
+                and not node.keywords):
+            return self._funcs[node.func.id](*(self._eval(a, t) for a in node.args))
+        raise ValueError(f"unsupported amplitude expression {self.expr!r}")
+
+    def __call__(self, t):
+        return float(self._eval(self.tree, float(t)))
+
+
+def _vector(s):
+    return [float(t) for t in splitLineAtCommas(s) if t]
+
+
+def _makeAction(module, name, kw, model):
+    from .stepping import BodyForce, Dirichlet, NodeForces, Options
+
+    amp = _Amplitude(kw["f(t)"]) if "f(t)" in kw else None
+    field = kw.get("field", "displacement")
+    if module == "dirichlet":
+        comps = {int(k) - 1: float(v) for k, v in kw.items() if _is_int(k)}
+        return Dirichlet(name, model, kw["nSet"], comps, field=field, amplitude=amp)
+    if module == "nodeforces":
+        return NodeForces(name, model, kw["nSet"], _vector(kw["components"]), field=field, amplitude=amp)
+    if module == "bodyforce":
+        return BodyForce(name, model, kw["elSet"], _vector(kw["forceVector"]), amplitude=amp)
+    if module == "options":
+        return Options(name, kw.pop("category", name), **kw)
+    raise NotImplementedError(f"step action {module} is not part of the GPU path")
+
+
+def createStepsFromInputFile(model: FEModel, inputFile: dict):
+    """Yields (AdaptiveStep, solver name) per *step; actions persist across steps and are updated when a later
+    step defines them again (stepmanager.py:133-200). Must be consumed step by step: a step's actions are
+    created against the model as it is when the step starts."""
+    from .stepping import AdaptiveStep
+
+    actions = {}
+    for number, st in enumerate(inputFile["*step"]):
+        opts = {k: v for k, v in st.items() if k not in ("data", "inputFile", "type", "solver", "jobname")}
+        if st.get("type", "AdaptiveStep").lower() != "adaptivestep":
+            raise NotImplementedError(f"step type {st['type']} is not part of the GPU path")
+        seen = set()
+        for line in st["data"]:
+            module, *definition = splitLineAtCommas(line)
+            module = module.lower()
+            kw = _assignments(definition)
+            name = kw.pop("name", None) or kw.get("category") or module
+            if name in seen:
+                raise Exception(f"StepAction {name} has multiple definitions in step {number}")
+            seen.add(name)
+            typ = actions.setdefault(module, {})
+            if name in typ:
+                act = typ[name]
+                if module == "bodyforce":
+                    act.updateStepAction(forceVector=_vector(kw["forceVector"]) if "forceVector" in kw else None,
+                                         delta=_vector(kw["delta"]) if "delta" in kw else None,
+                                         amplitude=_Amplitude(kw["f(t)"]) if "f(t)" in kw else None)
+                elif module == "options":
+                    act.options.update({k: v for k, v in kw.items() if k != "category"})
+                else:
+                    raise NotImplementedError(f"updating a {module} action in a later step is not part of the GPU path")
+            else:
+                typ[name] = _makeAction(module, name, dict(kw), model)
+        yield AdaptiveStep(number, model, actions, **opts), st.get("solver", "default")
+
+
+def finiteElementSimulation(inputFile: dict, verbose: bool = False, solverOptions: dict | None = None):
+    """Builds the model of a parsed deck and solves its steps with NISTGPU (inputfiledrivensimulation.py:56-217,
+    without output managers). Returns the model; model.U holds the final displacements (dof order)."""
+    from .solvers.nonlinearimplicitstaticgpu import NISTGPU, _Journal
+
+    model = createModelFromInputFile(inputFile)
+    solverDefs = {s.get("name", "default"): s for s in inputFile["*solver"]}
+    for s in solverDefs.values():
+        if s["solver"].lower() not in _SOLVERS:
+            raise NotImplementedError(f"solver {s['solver']} is not part of the GPU path")
+    solvers = {}
+    for step, solverName in createStepsFromInputFile(model, inputFile):
+        if solverName not in solvers:
+            s = solverDefs.get(solverName)
+            opts = dict(_assignments(t for ln in (s["data"] if s else []) for t in splitLineAtCommas(ln)))
+            opts.update(solverOptions or {})
+            solvers[solverName] = NISTGPU(journal=_Journal(verbose=verbose), **opts)
+        solvers[solverName].solveStep(step, model)
+    return model
+
+
+def runReferenceTest(directory: str, verbose: bool = False, solverOptions: dict | None = None):
+    """_run_tests_edelweissfe.py:88-109: runs <directory>/test.inp and compares the final displacements with
+    <directory>/U.ref. Returns (passed at 1e-6 absolute, max absolute difference)."""
+    inputFile = parseInputFile(os.path.join(directory, "test.inp"))
+    model = finiteElementSimulation(inputFile, verbose=verbose, solverOptions=solverOptions)
+    U = model.U.detach().cpu().numpy().ravel()
+    Uref = np.loadtxt(os.path.join(directory, "U.ref"))
+    err = float(np.abs(U - Uref).max())
+    return err < 1e-6, err

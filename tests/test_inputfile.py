@@ -1,0 +1,97 @@
+"""The input-deck reader (edelweissfe_amd.inputfile) on the reference's own regression decks.
+
+CPU: the models it builds equal those of the golden fixtures (parsed independently by
+tests/golden/make_fixtures.py), and the oracle's Newton loop run on them reproduces the decks' U.ref (1e-6
+absolute, the reference's bar, ew/_cli/_run_tests_edelweissfe.py:96-109). GPU: runReferenceTest (parse, build,
+NISTGPU, compare with U.ref) on every deck.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import decks
+from edelweissfe_amd import inputfile as IF
+from oracle import oracle as O
+
+DECK_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks")
+FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C3D8", "C3D20", "WallShearHexa8",
+                 "WallShearHexa20"]
+ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D"]
+
+
+def _deck(name):
+    return IF.parseInputFile(os.path.join(DECK_DIR, name, "test.inp"))
+
+
+def _oracle_step(model, step):
+    """The oracle's step description of a one-step deck built by the reader."""
+    a = step.actions
+    st = {k: getattr(step, f) for k, f in (("maxInc", "maxIncrementSize"), ("minInc", "minIncrementSize"),
+                                          ("maxNumInc", "maxNumberIncrements"), ("maxIter", "maxIter"),
+                                          ("stepLength", "length"), ("startInc", "startIncrementSize"))}
+    opts = a["options"]["NISTSolver"].options if "NISTSolver" in a["options"] else {}
+    st["extrapolation"] = opts.get("extrapolation", "linear")
+    st["dirichlet"] = [(d.nSet, {c: float(v) for c, v in zip(d.components, d.delta[0])}) for d in a["dirichlet"].values()]
+    st["nodeforces"] = [(f.nodeSet, list(f.nodeForcesDelta[0])) for f in a["nodeforces"].values()]
+    st["bodyforces"] = []
+    for b in a["bodyforce"].values():
+        assert b.amplitude(0.5) == 0.5  # the oracle scales body forces linearly with the step progress
+        assert len(b.elementSet) == len(model.elementBlocks[0])
+        st["bodyforces"].append(list(b.delta))
+    return st
+
+
+def test_parser_structure():
+    d = _deck("BodyForce3D")  # *include of mesh.inp in the middle of the deck
+    assert len(d["*node"]) == 1 and len(d["*element"]) == 1 and d["*element"][0]["type"] == "C3D8"
+    assert d["*step"][0]["maxinc"] == "1e-1" and len(d["*step"][0]["data"]) == 3
+    assert IF.splitLineAtCommas("bodyforce, name=b, forceVector='0.0, 10.0, 20.0', f(t)=t") == [
+        "bodyforce", "name=b", "forceVector=0.0, 10.0, 20.0", "f(t)=t"]
+    amp = IF._Amplitude("t**2 + 0.5*sin(pi*t)")
+    assert abs(amp(0.3) - (0.09 + 0.5 * np.sin(np.pi * 0.3))) < 1e-15
+    with pytest.raises(ValueError):
+        IF._Amplitude("__import__('os')")
+
+
+@pytest.mark.parametrize("name", FIXTURE_DECKS)
+def test_reader_builds_the_fixture_models(name):
+    model = IF.createModelFromInputFile(_deck(name))
+    fx = decks.load(name)
+    X, conn, sets = decks.mesh(fx, O.boxgen)
+    block = model.elementBlocks[0]
+    assert block.elType == fx["el_type"]
+    assert np.array_equal(model.nodeCoordinates, X) and np.array_equal(block.conn, conn)
+    assert block.material.name == fx["material"] and np.allclose(block.material.materialProperties[: len(fx["mp"])],
+                                                                 fx["mp"])
+    for act in fx["step"]["dirichlet"] + fx["step"]["nodeforces"]:
+        assert np.array_equal(np.sort(model.nodeSets[act["nSet"]]), np.sort(sets[act["nSet"]]))
+    (step, _), = list(IF.createStepsFromInputFile(model, _deck(name)))
+    st = _oracle_step(model, step)
+    ref = decks.step(fx, sets)
+    assert st["extrapolation"] == ref["extrapolation"] and st["maxInc"] == ref["maxInc"]
+    assert len(st["dirichlet"]) == len(ref["dirichlet"]) and len(st["nodeforces"]) == len(ref["nodeforces"])
+    assert st["bodyforces"] == ref["bodyforces"]
+
+
+@pytest.mark.parametrize("name", ALL_DECKS)
+def test_reader_and_oracle_reproduce_uref(name):
+    """Deck -> reader -> oracle Newton loop -> U.ref (CPU)."""
+    inp = _deck(name)
+    model = IF.createModelFromInputFile(inp)
+    (step, _), = list(IF.createStepsFromInputFile(model, inp))
+    b = model.elementBlocks[0]
+    r = O.solve_step(dict(el_kind=b.elType, mat_kind=b.material.name, mp=b.material.materialProperties,
+                          X=model.nodeCoordinates, conn=b.conn), _oracle_step(model, step))
+    Uref = np.loadtxt(os.path.join(DECK_DIR, name, "U.ref"))
+    assert np.abs(r["U"] - Uref).max() < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ALL_DECKS)
+def test_run_reference_decks_from_input_files(name):
+    """Deck -> reader -> NISTGPU -> U.ref, like the reference's test runner."""
+    print(f"deck {name}", flush=True)
+    passed, err = IF.runReferenceTest(os.path.join(DECK_DIR, name), solverOptions={"pcgTolerance": 1e-13})
+    print(f"deck {name}: max |U - U.ref| = {err:.3e}", flush=True)
+    assert passed
