@@ -141,6 +141,20 @@ class ElementAssembly:
         f = np.ascontiguousarray(np.asarray(force, dtype=np.float64).ravel()[:3])
         check(lib().ewfe_body_force(self._plan, ptr(self.X), ptr(mask), ptr(f), ptr(PExt), self._stream()))
 
+    def qpResults(self, result, U, which="converged"):
+        """[nEl, nInt, 6] device tensor (element order) of the quadrature-point "stress" or "strain" at the
+        displacement U, with the converged ("converged") or trial ("trial") state for J2 stresses."""
+        if result not in ("stress", "strain"):
+            raise KeyError(f"result {result} is not available on the GPU path (stress, strain)")
+        if U.numel() != self.nDof or U.dtype != self.torch.float64 or not U.is_contiguous():
+            raise ValueError("U must be a contiguous float64 tensor of length nDof")
+        b = self.block
+        state = b.stateVars if which == "converged" else b.stateVarsTemp
+        out = self.torch.empty((len(b), b.nInt, 6), dtype=self.torch.float64, device=self.device)
+        check(lib().ewfe_qp_results(self._plan, self.matKind, ptr(self.matParams), ptr(self.elParams), ptr(self.X),
+                                    ptr(U), ptr(state), 0 if result == "stress" else 1, ptr(out), self._stream()))
+        return out
+
     def dirichletRows(self, rows_dev, vals):
         check(lib().ewfe_dirichlet_rows(self._plan, ptr(rows_dev), rows_dev.numel(), ptr(vals),
                                         self._dtype_code(vals), self._stream()))

@@ -1007,6 +1007,80 @@ __global__ __launch_bounds__(256) void k_body_nodes(const int32_t *__restrict__ 
 }
 
 // ------------------------------------------------------------------------------------------------
+// Quadrature-point results (the per-point "stress" and "strain" arrays the reference's elements expose through
+// getResultArray, collected by ElementResultCollector, edelweissfe/utils/elementresultcollector.pyx:37-127),
+// at a given displacement U and state, one thread per (element, point), element order out[e][q][6]:
+//   small strain (displacementelement/element.py:222-235, 290-346): strain = eps(U), engineering Voigt
+//     [11, 22, 33, 12, 13, 23]; stress = C : eps (linear elastic) or C : (eps - eps_p) with the plastic strain of
+//     `state` (J2, the stress of a converged state);
+//   total Lagrangian (displacementtlelement/element.py:377-412): strain = Green-Lagrange E, stress = Kirchhoff
+//     tau (Neo-Hooke Wa), both Voigt [11, 22, 33, 12, 23, 13], engineering shear strains.
+// ------------------------------------------------------------------------------------------------
+template <int NN, int NQ, int FORM, int MAT>
+__global__ __launch_bounds__(256) void k_qp_results(EvalArgs A, const double *__restrict__ state, int what,
+                                                    double *__restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = A.n_el;
+    if (t >= n * NQ) return;
+    const int64_t p = t / NQ;
+    const int q = (int)(t % NQ);
+    const int64_t e = A.order[p];
+    const double(*dN)[3] = NN == 8 ? c_dN8[q] : c_dN20[q];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3], H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    int32_t dof[NN];
+    for (int a = 0; a < NN; ++a) dof[a] = A.edof[(int64_t)a * n + p];
+    for (int a = 0; a < NN; ++a)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[i][j] += dN[a][i] * A.X[dof[a] + j];
+    inv3(J, Ji);
+    for (int a = 0; a < NN; ++a) {
+        double g[3];
+        for (int j = 0; j < 3; ++j) g[j] = Ji[j][0] * dN[a][0] + Ji[j][1] * dN[a][1] + Ji[j][2] * dN[a][2];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) H[i][j] += A.U[dof[a] + i] * g[j];
+    }
+    double v[6];
+    const double Epar = A.el_param ? A.el_param[e] : A.mat.p[0];
+    if (FORM == FORM_SS) {
+        const double eps[6] = {H[0][0], H[1][1], H[2][2], H[0][1] + H[1][0], H[0][2] + H[2][0], H[1][2] + H[2][1]};
+        if (what == 1) {
+            for (int i = 0; i < 6; ++i) v[i] = eps[i];
+        } else {
+            double ee[6];
+            for (int i = 0; i < 6; ++i) ee[i] = eps[i];
+            if (MAT == M_J2)
+                for (int s = 0; s < 6; ++s) ee[s] -= state[((int64_t)s * NQ + q) * n + p];
+            const double nu = A.mat.p[1];
+            const double f = Epar / ((1 + nu) * (1 - 2 * nu));
+            const double lam = f * nu, mu = f * (1 - 2 * nu) / 2;
+            const double tr = ee[0] + ee[1] + ee[2];
+            for (int i = 0; i < 3; ++i) v[i] = lam * tr + 2 * mu * ee[i];
+            for (int i = 3; i < 6; ++i) v[i] = mu * ee[i];
+        }
+    } else {
+        double F[3][3], Fi[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
+        if (what == 1) {
+            double E[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    E[i][j] = 0.5 * (H[i][j] + H[j][i] + H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j]);
+            const double o[6] = {E[0][0], E[1][1], E[2][2], 2 * E[0][1], 2 * E[1][2], 2 * E[0][2]};
+            for (int i = 0; i < 6; ++i) v[i] = o[i];
+        } else {
+            const double Jd = inv3(F, Fi);
+            Recipe r;
+            nh_recipe(Epar, A.mat.p[1], F, Jd, r);  // r.S: [11, 22, 33, 12, 13, 23]
+            const double o[6] = {r.S[0], r.S[1], r.S[2], r.S[3], r.S[5], r.S[4]};
+            for (int i = 0; i < 6; ++i) v[i] = o[i];
+        }
+    }
+    for (int i = 0; i < 6; ++i) out[(e * NQ + q) * 6 + i] = v[i];
+}
+
+// ------------------------------------------------------------------------------------------------
 // Adjoint: grad[e] = lambda^T dR/dtheta_e for a per-element material parameter theta_e (the first material
 // property: mu of Neo-Hooke Wa on total-Lagrangian hexahedra, E of linear elasticity on small-strain
 // hexahedra). R = P + PExt with P = -f_int, so grad[e] = -sum_a lambda_a . df_a/dtheta_e. The element
@@ -1736,6 +1810,56 @@ extern "C" int ewfe_body_force(const ewfe_plan *p, const double *X_dev, const ui
     const unsigned nb = (unsigned)((p->n_rownodes + 255) / 256);
     hipLaunchKernelGGL(k_body_nodes, dim3(nb), dim3(256), 0, s, p->d_rownode_dof, p->d_ninc_ptr, p->d_ninc,
                        pm->d_bweights, p->npe, force_host[0], force_host[1], force_host[2], PExt_dev, p->n_rownodes);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ewfe_qp_results(const ewfe_plan *p, int mat_kind, const double *mat_params_host,
+                               const double *el_param_dev, const double *X_dev, const double *U_dev,
+                               const double *state_dev, int result, double *out_dev, void *stream)
+{
+    if (!p || !mat_params_host || !X_dev || !U_dev || !out_dev || (result != 0 && result != 1)) {
+        ewfe::set_error("ewfe_qp_results: invalid arguments (result: 0 = stress, 1 = strain)");
+        return EWFE_EINVAL;
+    }
+    if (!p->d_edof) {
+        ewfe::set_error("ewfe_qp_results: plan has no device data");
+        return EWFE_EINVAL;
+    }
+    const bool tl = is_tl(p->elem_kind);
+    const bool ok = (!tl && (mat_kind == EWFE_LINEAR_ELASTIC || mat_kind == EWFE_VON_MISES)) ||
+                    (tl && mat_kind == EWFE_NEO_HOOKE_WA);
+    if (!ok) {
+        ewfe::set_error("ewfe_qp_results: unsupported element/material combination");
+        return EWFE_EINVAL;
+    }
+    if (mat_kind == EWFE_VON_MISES && result == 0 && !state_dev) {
+        ewfe::set_error("ewfe_qp_results: the J2 stress needs the state");
+        return EWFE_EINVAL;
+    }
+    EvalArgs A{};
+    A.edof = p->d_edof;
+    A.order = p->d_order;
+    A.el_param = el_param_dev;
+    A.X = X_dev;
+    A.U = U_dev;
+    A.n_el = p->n_el;
+    for (int i = 0; i < 8; ++i) A.mat.p[i] = mat_params_host[i];
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int64_t nt = p->n_el * p->nqp;
+    const unsigned blocks = (unsigned)((nt + 255) / 256);
+#define EWFE_QPR(NN, NQ, FORM, MAT) \
+    hipLaunchKernelGGL((k_qp_results<NN, NQ, FORM, MAT>), dim3(blocks), dim3(256), 0, s, A, state_dev, result, out_dev)
+    if (p->npe == 8) {
+        if (tl) EWFE_QPR(8, 8, FORM_TL, M_NH);
+        else if (mat_kind == EWFE_VON_MISES) EWFE_QPR(8, 8, FORM_SS, M_J2);
+        else EWFE_QPR(8, 8, FORM_SS, M_LE);
+    } else {
+        if (tl) EWFE_QPR(20, 27, FORM_TL, M_NH);
+        else if (mat_kind == EWFE_VON_MISES) EWFE_QPR(20, 27, FORM_SS, M_J2);
+        else EWFE_QPR(20, 27, FORM_SS, M_LE);
+    }
+#undef EWFE_QPR
     HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -13,6 +13,7 @@ The subset of the reference's input language that the GPU path can run, with the
     actions live across steps, keyed by module and name; a redefinition in a later step updates the action;
   * ``finiteElementSimulation`` (edelweissfe/drivers/inputfiledrivensimulation.py:56-217) without output
     managers: every ``*step`` is solved by NISTGPU, whatever CPU solver (NIST, NISTParallel, ...) the deck names;
+    the solver of the last step stays reachable as ``model.solver`` (quadrature-point results);
   * ``runReferenceTest`` (edelweissfe/_cli/_run_tests_edelweissfe.py:88-109): the final nodal displacements
     against ``U.ref``, 1e-6 absolute.
 
@@ -370,6 +371,7 @@ def finiteElementSimulation(inputFile: dict, verbose: bool = False, solverOption
             opts.update(solverOptions or {})
             solvers[solverName] = NISTGPU(journal=_Journal(verbose=verbose), **opts)
         solvers[solverName].solveStep(step, model)
+        model.solver = solvers[solverName]
     return model
 
 

@@ -190,6 +190,16 @@ int ewfe_pcg_mg(ewfe_mg *mg, const void *csr_vals_dev, int vals_dtype, const dou
 int ewfe_body_force(const ewfe_plan *plan, const double *X_dev, const uint8_t *el_mask_dev, const double *force_host,
                     double *PExt_dev, void *stream);
 
+/* Quadrature-point results at displacement U_dev and state state_dev (plan order, as ewfe_eval_assemble;
+ * J2 stress only): out_dev [n_el][qp][6] in element order; result 0 = "stress", 1 = "strain", the arrays the
+ * reference's elements expose through getResultArray (displacementelement/element.py:222-235,
+ * displacementtlelement/element.py:377-412) and ElementResultCollector gathers
+ * (edelweissfe/utils/elementresultcollector.pyx:37-127). Small strain: Voigt [11,22,33,12,13,23], total
+ * Lagrangian (Kirchhoff stress, Green strain): [11,22,33,12,23,13]; engineering shear strains. */
+int ewfe_qp_results(const ewfe_plan *plan, int mat_kind, const double *mat_params_host, const double *el_param_dev,
+                    const double *X_dev, const double *U_dev, const double *state_dev, int result, double *out_dev,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

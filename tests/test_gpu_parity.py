@@ -268,6 +268,33 @@ def test_body_force_matches_oracle(el):
     assert rel(PExt.cpu().numpy(), O.body_force(el, conn, X, edofs, nDof, b, sub)) < 1e-13
 
 
+@pytest.mark.parametrize("el,mat,mp", [("C3D8", "linearelastic", LE), ("C3D8", "vonmises", J2),
+                                       ("C3D20", "vonmises", J2), ("C3D8TL", "neohookeanwa", NH),
+                                       ("C3D20TL", "neohookeanwa", NH)])
+def test_qp_results_match_oracle_state(el, mat, mp):
+    """Quadrature-point stress and strain of ewfe_qp_results against the per-point state the oracle keeps in
+    the reference's layout (stress 6 | strain 6 | ...), after one evaluation from the virgin state."""
+    import torch
+
+    print(f"case qp results {el} {mat}", flush=True)
+    n = (3, 2, 2) if el.startswith("C3D8") else (2, 1, 1)
+    X, conn, _ = jittered_box(n, el, seed=14)
+    rng = np.random.default_rng(15)
+    amp = 0.08 if "TL" in el else 4e-3
+    U = _uniaxial_field(X, amp, rng, amp / 20)
+    g = GpuEval(el, mat, mp, X, conn)
+    g(U, U)  # trial state of the J2 return mapping at U
+    ref = oracle_eval(el, mat, mp, X, conn, U, U)
+    nq = g.block.nInt
+    st = ref["state"].reshape(len(conn), nq, -1)
+    Ud = g.vec(U)
+    stress = g.asm.qpResults("stress", Ud, which="trial")
+    strain = g.asm.qpResults("strain", Ud, which="trial")
+    torch.cuda.synchronize()
+    assert rel(stress.cpu().numpy(), st[:, :, 0:6]) < 1e-12
+    assert rel(strain.cpu().numpy(), st[:, :, 6:12]) < 1e-12
+
+
 def test_fp32_tangent_newton_reaches_fp64_tolerance():
     fx, U32, s32 = _deck_run("C3D8", "fp32")
     _, U64, s64 = _deck_run("C3D8", "fp64")
