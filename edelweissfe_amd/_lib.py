@@ -42,6 +42,7 @@ SIGNATURES = {
     "ewfe_adjoint_param_grad": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ewfe_body_force": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "ewfe_qp_results": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
+    "ewfe_surface_pressure": (ctypes.c_int, [_vp, _vp, _vp, _i64, ctypes.c_int, _dbl, _vp, _vp]),
     "ewfe_dirichlet_rows": (ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.c_int, _vp]),
     "ewfe_spmv": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "ewfe_pcg": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _i64, _dbl, ctypes.c_int, ctypes.c_int, _vp,

@@ -141,6 +141,24 @@ class ElementAssembly:
         f = np.ascontiguousarray(np.asarray(force, dtype=np.float64).ravel()[:3])
         check(lib().ewfe_body_force(self._plan, ptr(self.X), ptr(mask), ptr(f), ptr(PExt), self._stream()))
 
+    def surfacePressure(self, PExt, elements, face, pressure):
+        """PExt += the pressure `pressure` (positive pushes into the element) on face `face` (Abaqus numbering 1..6)
+        of `elements` (element indices of this block), reference configuration; the distributed load of type
+        pressure of the Marmot solid elements (nonlinearimplicitstatic.py:460-501)."""
+        if PExt.numel() != self.nDof or PExt.dtype != self.torch.float64 or not PExt.is_contiguous():
+            raise ValueError("PExt must be a contiguous float64 tensor of length nDof")
+        if int(face) not in range(1, 7):
+            raise ValueError(f"face {face}: hexahedra have faces 1..6")
+        els = np.asarray(elements, dtype=np.int64).ravel()
+        if len(els) and (els.min() < 0 or els.max() >= len(self.block) or len(np.unique(els)) != len(els)):
+            raise ValueError("elements must be distinct element indices of this block")
+        cache = self.__dict__.setdefault("_faceSets", {})  # id -> (the caller's array, its device copy)
+        hit = cache.get(id(elements))
+        if hit is None or hit[0] is not elements:
+            hit = cache[id(elements)] = (elements, self.torch.as_tensor(els.astype(np.int32), device=self.device))
+        check(lib().ewfe_surface_pressure(self._plan, ptr(self.X), ptr(hit[1]), len(els), int(face),
+                                          float(pressure), ptr(PExt), self._stream()))
+
     def qpResults(self, result, U, which="converged"):
         """[nEl, nInt, 6] device tensor (element order) of the quadrature-point "stress" or "strain" at the
         displacement U, with the converged ("converged") or trial ("trial") state for J2 stresses."""

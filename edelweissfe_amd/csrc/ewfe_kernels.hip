@@ -102,6 +102,48 @@ static void host_dN(int nn, const double x[3], double dN[][3])
     }
 }
 
+// Shape functions and parametric derivatives at an arbitrary point (device), same node tables as below
+__constant__ double c_hexXi[20][3] = {
+    {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
+    {0, -1, -1},  {1, 0, -1},  {0, 1, -1}, {-1, 0, -1}, {0, -1, 1},  {1, 0, 1},  {0, 1, 1}, {-1, 0, 1},
+    {-1, -1, 0},  {1, -1, 0},  {1, 1, 0},  {-1, 1, 0}};
+template <int NN>
+__device__ void shape8_20(const double x[3], double N[NN], double dN[NN][3])
+{
+    for (int a = 0; a < NN; ++a) {
+        const double c[3] = {c_hexXi[a][0], c_hexXi[a][1], c_hexXi[a][2]};
+        if (NN == 8) {
+            const double f0 = 1 + c[0] * x[0], f1 = 1 + c[1] * x[1], f2 = 1 + c[2] * x[2];
+            N[a] = 0.125 * f0 * f1 * f2;
+            dN[a][0] = 0.125 * c[0] * f1 * f2;
+            dN[a][1] = 0.125 * f0 * c[1] * f2;
+            dN[a][2] = 0.125 * f0 * f1 * c[2];
+        } else if (a < 8) {
+            const double f0 = 1 + c[0] * x[0], f1 = 1 + c[1] * x[1], f2 = 1 + c[2] * x[2];
+            const double s = c[0] * x[0] + c[1] * x[1] + c[2] * x[2] - 2;
+            N[a] = 0.125 * f0 * f1 * f2 * s;
+            dN[a][0] = 0.125 * c[0] * f1 * f2 * (s + f0);
+            dN[a][1] = 0.125 * c[1] * f0 * f2 * (s + f1);
+            dN[a][2] = 0.125 * c[2] * f0 * f1 * (s + f2);
+        } else {
+            double f[3], df[3];
+            for (int d = 0; d < 3; ++d) {
+                if (c[d] == 0) {
+                    f[d] = 1 - x[d] * x[d];
+                    df[d] = -2 * x[d];
+                } else {
+                    f[d] = 1 + c[d] * x[d];
+                    df[d] = c[d];
+                }
+            }
+            N[a] = 0.25 * f[0] * f[1] * f[2];
+            dN[a][0] = 0.25 * df[0] * f[1] * f[2];
+            dN[a][1] = 0.25 * f[0] * df[1] * f[2];
+            dN[a][2] = 0.25 * f[0] * f[1] * df[2];
+        }
+    }
+}
+
 // Shape function values: trilinear (8 nodes) and serendipity (20 nodes, Abaqus node order)
 static void host_N(int nn, const double x[3], double N[])
 {
@@ -1007,6 +1049,85 @@ __global__ __launch_bounds__(256) void k_body_nodes(const int32_t *__restrict__ 
 }
 
 // ------------------------------------------------------------------------------------------------
+// Distributed load of type pressure (nonlinearimplicitstatic.py:460-501; the element side is the Marmot
+// element's computeDistributedLoad, element.pxd:97-144): for the listed elements, face `face` (Abaqus numbering:
+// S1 zeta = -1, S2 zeta = +1, S3 eta = -1, S4 xi = +1, S5 eta = +1, S6 xi = -1) carries the traction -p n on
+// the reference configuration: P_a = -p sum_g N_a n dA w_g over a 2 x 2 (8 nodes) or 3 x 3 (20 nodes) Gauss
+// rule on the face. Per element [npe][3] into scratch, then summed per node in ascending element order.
+// ------------------------------------------------------------------------------------------------
+template <int NN>
+__global__ __launch_bounds__(256) void k_face_pressure(const int32_t *__restrict__ edof,
+                                                       const int32_t *__restrict__ pos_of_el,
+                                                       const double *__restrict__ X, const int32_t *__restrict__ els,
+                                                       int64_t n_list, int face, double pressure,
+                                                       double *__restrict__ W, int64_t n_el)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_list) return;
+    const int64_t e = els[k];
+    if (e < 0 || e >= n_el) return;  // not an element of the plan
+    const int64_t p = pos_of_el[e];
+    double x[NN][3];
+    for (int a = 0; a < NN; ++a) {
+        const int32_t d = edof[(int64_t)a * n_el + p];
+        for (int c = 0; c < 3; ++c) x[a][c] = X[d + c];
+    }
+    // fixed parametric axis and its value; (u, v) = the cyclic successors of the fixed axis
+    constexpr int kAxis[7] = {0, 2, 2, 1, 0, 1, 0};
+    constexpr int kSign[7] = {0, -1, 1, -1, 1, 1, -1};
+    const int ax = kAxis[face], sg = kSign[face];
+    const int u = (ax + 1) % 3, v = (ax + 2) % 3;
+    const int ng = NN == 8 ? 2 : 3;
+    const double g2 = 0.5773502691896257645, g3 = 0.7745966692414833770;
+    const double pts2[2] = {-g2, g2}, pts3[3] = {-g3, 0.0, g3}, wts3[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    double f[NN][3];
+    for (int a = 0; a < NN; ++a) f[a][0] = f[a][1] = f[a][2] = 0.0;
+    for (int i = 0; i < ng; ++i)
+        for (int j = 0; j < ng; ++j) {
+            double xi[3];
+            xi[ax] = sg;
+            xi[u] = ng == 2 ? pts2[i] : pts3[i];
+            xi[v] = ng == 2 ? pts2[j] : pts3[j];
+            const double w = ng == 2 ? 1.0 : wts3[i] * wts3[j];
+            double N[NN], dN[NN][3];
+            shape8_20<NN>(xi, N, dN);
+            double tu[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
+            for (int a = 0; a < NN; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    tu[c] += dN[a][u] * x[a][c];
+                    tv[c] += dN[a][v] * x[a][c];
+                }
+            const double nA[3] = {sg * (tu[1] * tv[2] - tu[2] * tv[1]), sg * (tu[2] * tv[0] - tu[0] * tv[2]),
+                                  sg * (tu[0] * tv[1] - tu[1] * tv[0])};
+            for (int a = 0; a < NN; ++a)
+                for (int c = 0; c < 3; ++c) f[a][c] -= pressure * N[a] * nA[c] * w;
+        }
+    for (int a = 0; a < NN; ++a)
+        for (int c = 0; c < 3; ++c) W[(e * NN + a) * 3 + c] = f[a][c];  // a repeated element writes the same
+}
+
+__global__ __launch_bounds__(256) void k_node_vec_sum(const int32_t *__restrict__ rownode_dof,
+                                                      const int32_t *__restrict__ ninc_ptr,
+                                                      const int32_t *__restrict__ ninc, const double *__restrict__ W,
+                                                      int npe, double *__restrict__ PExt, int64_t n_rownodes)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rownodes) return;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    for (int32_t k = ninc_ptr[r]; k < ninc_ptr[r + 1]; ++k) {
+        const int32_t code = ninc[k];  // 32 * element + local node, ascending element
+        const double *w = W + ((int64_t)(code >> 5) * npe + (code & 31)) * 3;
+        v0 += w[0];
+        v1 += w[1];
+        v2 += w[2];
+    }
+    const int32_t d = rownode_dof[r];
+    PExt[d + 0] += v0;
+    PExt[d + 1] += v1;
+    PExt[d + 2] += v2;
+}
+
+// ------------------------------------------------------------------------------------------------
 // Quadrature-point results (the per-point "stress" and "strain" arrays the reference's elements expose through
 // getResultArray, collected by ElementResultCollector, edelweissfe/utils/elementresultcollector.pyx:37-127),
 // at a given displacement U and state, one thread per (element, point), element order out[e][q][6]:
@@ -1536,7 +1657,7 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
     void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
                     p->d_rownode_dof, p->d_gdesc, p->d_gblob, p->d_kbuf,
                     p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta,
-                    p->d_rorder,   p->d_ninc_ptr, p->d_ninc,   p->d_bweights};
+                    p->d_rorder,   p->d_ninc_ptr, p->d_ninc,   p->d_bweights, p->d_fweights, p->d_pos_of_el};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     for (void *ev : p->chunk_events)
@@ -1860,6 +1981,45 @@ extern "C" int ewfe_qp_results(const ewfe_plan *p, int mat_kind, const double *m
         else EWFE_QPR(20, 27, FORM_SS, M_LE);
     }
 #undef EWFE_QPR
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ewfe_surface_pressure(const ewfe_plan *p, const double *X_dev, const int32_t *elements_dev,
+                                     int64_t n_elements, int face, double pressure, double *PExt_dev, void *stream)
+{
+    if (!p || !X_dev || !PExt_dev || (n_elements > 0 && !elements_dev) || face < 1 || face > 6) {
+        ewfe::set_error("ewfe_surface_pressure: invalid arguments (face: 1..6)");
+        return EWFE_EINVAL;
+    }
+    if (!p->d_edof || !p->d_ninc) {
+        ewfe::set_error("ewfe_surface_pressure: plan has no device data");
+        return EWFE_EINVAL;
+    }
+    if (n_elements == 0) return 0;
+    ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // scratch is allocated lazily by the plan
+    const size_t wbytes = sizeof(double) * 3 * (size_t)p->npe * p->n_el;
+    if (!pm->d_fweights) HIP_CHECK(hipMalloc((void **)&pm->d_fweights, wbytes));
+    if (!pm->d_pos_of_el) {
+        std::vector<int32_t> inv(p->n_el);
+        for (int64_t i = 0; i < p->n_el; ++i) inv[p->order[i]] = (int32_t)i;
+        HIP_CHECK(hipMalloc((void **)&pm->d_pos_of_el, sizeof(int32_t) * p->n_el));
+        HIP_CHECK(hipMemcpy(pm->d_pos_of_el, inv.data(), sizeof(int32_t) * p->n_el, hipMemcpyHostToDevice));
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int rc;
+    if ((rc = fill<double>(pm->d_fweights, 3 * (int64_t)p->npe * p->n_el, 0.0, s))) return rc;
+    const unsigned lb = (unsigned)((n_elements + 255) / 256);
+    if (p->npe == 8)
+        hipLaunchKernelGGL(k_face_pressure<8>, dim3(lb), dim3(256), 0, s, p->d_edof, pm->d_pos_of_el, X_dev, elements_dev,
+                           n_elements, face, pressure, pm->d_fweights, p->n_el);
+    else
+        hipLaunchKernelGGL(k_face_pressure<20>, dim3(lb), dim3(256), 0, s, p->d_edof, pm->d_pos_of_el, X_dev,
+                           elements_dev, n_elements, face, pressure, pm->d_fweights, p->n_el);
+    HIP_CHECK(hipGetLastError());
+    const unsigned nb = (unsigned)((p->n_rownodes + 255) / 256);
+    hipLaunchKernelGGL(k_node_vec_sum, dim3(nb), dim3(256), 0, s, p->d_rownode_dof, p->d_ninc_ptr, p->d_ninc,
+                       pm->d_fweights, p->npe, PExt_dev, p->n_rownodes);
     HIP_CHECK(hipGetLastError());
     return 0;
 }

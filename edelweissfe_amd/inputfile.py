@@ -8,7 +8,7 @@ The subset of the reference's input language that the GPU path can run, with the
   * ``createModelFromInputFile`` (edelweissfe/helpers/inputfilehelpers.py:169-211 and
     edelweissfe/generators/abqmodelconstructor.py:67-265): boxGen generators first, then ``*node`` (label order
     of definition), ``*element``, ``*nSet`` / ``*elSet`` (labels, ``generate=True``, or names of other sets), the
-    implicit sets ``all``, materials and sections;
+    implicit sets ``all``, materials, sections and element-based ``*surface`` definitions;
   * ``createStepsFromInputFile`` (inputfilehelpers.py:214-244, edelweissfe/steps/stepmanager.py:133-200): step
     actions live across steps, keyed by module and name; a redefinition in a later step updates the action;
   * ``finiteElementSimulation`` (edelweissfe/drivers/inputfiledrivensimulation.py:56-217) without output
@@ -18,7 +18,7 @@ The subset of the reference's input language that the GPU path can run, with the
     against ``U.ref``, 1e-6 absolute.
 
 Decks outside the subset (other element types or providers' features, several element blocks or materials,
-constraints, distributed loads, analytical fields, ...) raise NotImplementedError.
+constraints, distributed loads other than pressure, analytical fields, ...) raise NotImplementedError.
 """
 
 from __future__ import annotations
@@ -259,6 +259,18 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
     if chosen is None or not assigned.all():
         raise ValueError("every element needs a section with a material")
     block.setMaterial(chosen)
+
+    # element-based surfaces: data lines 'elSet, face' with face 1..6 or S1..S6 (abqmodelconstructor.py:213-225)
+    for sf in inputFile["*surface"]:
+        if sf.get("type", "element").lower() != "element":
+            raise NotImplementedError(f"surface type {sf.get('type')} is not part of the GPU path")
+        surface = {}
+        for ln in sf["data"]:
+            elSet, face = splitLineAtCommas(ln)
+            if elSet not in model.elementSets:
+                raise KeyError(f"surface {sf['name']}: unknown element set {elSet}")
+            surface[int(face.upper().replace("S", ""))] = elSet
+        model.surfaces[sf["name"]] = surface
     return model
 
 
@@ -301,7 +313,7 @@ def _vector(s):
 
 
 def _makeAction(module, name, kw, model):
-    from .stepping import BodyForce, Dirichlet, NodeForces, Options
+    from .stepping import BodyForce, Dirichlet, DistributedLoad, NodeForces, Options
 
     amp = _Amplitude(kw["f(t)"]) if "f(t)" in kw else None
     field = kw.get("field", "displacement")
@@ -312,6 +324,8 @@ def _makeAction(module, name, kw, model):
         return NodeForces(name, model, kw["nSet"], _vector(kw["components"]), field=field, amplitude=amp)
     if module == "bodyforce":
         return BodyForce(name, model, kw["elSet"], _vector(kw["forceVector"]), amplitude=amp)
+    if module == "distributedload":
+        return DistributedLoad(name, model, kw["surface"], kw["type"], _vector(kw["magnitude"]), amplitude=amp)
     if module == "options":
         return Options(name, kw.pop("category", name), **kw)
     raise NotImplementedError(f"step action {module} is not part of the GPU path")
@@ -342,6 +356,10 @@ def createStepsFromInputFile(model: FEModel, inputFile: dict):
                 act = typ[name]
                 if module == "bodyforce":
                     act.updateStepAction(forceVector=_vector(kw["forceVector"]) if "forceVector" in kw else None,
+                                         delta=_vector(kw["delta"]) if "delta" in kw else None,
+                                         amplitude=_Amplitude(kw["f(t)"]) if "f(t)" in kw else None)
+                elif module == "distributedload":
+                    act.updateStepAction(magnitude=_vector(kw["magnitude"]) if "magnitude" in kw else None,
                                          delta=_vector(kw["delta"]) if "delta" in kw else None,
                                          amplitude=_Amplitude(kw["f(t)"]) if "f(t)" in kw else None)
                 elif module == "options":

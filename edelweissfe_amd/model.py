@@ -118,6 +118,8 @@ class FEModel:
         self.nodeSets: dict[str, np.ndarray] = {}
         self.elementBlocks: list[ElementBlock] = []
         self.elementSets: dict[str, tuple[int, np.ndarray]] = {}  # name -> (block index, element indices)
+        # name -> {face number (1..6, Abaqus numbering): element set name} (abqmodelconstructor.py:213-225)
+        self.surfaces: dict[str, dict[int, str]] = {}
         self.time = 0.0
         self.constraints = {}
         self.scalarVariables = {}

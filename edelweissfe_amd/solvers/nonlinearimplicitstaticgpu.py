@@ -201,8 +201,7 @@ class NISTGPU:
         dirichlets = list(stepActions["dirichlet"].values())
         nodeforces = list(stepActions["nodeforces"].values())
         bodyforces = list(stepActions.get("bodyforce", {}).values())
-        if stepActions.get("distributedload"):
-            raise NotImplementedError("distributed loads are not on the GPU path yet")
+        distributedloads = list(stepActions.get("distributedload", {}).values())
         self.applyStepActionsAtIncrementStart(model, timeStep, stepActions)
         dU, isExtrapolatedIncrement = self.extrapolateLastIncrement(extrapolation, timeStep, dU, dirichlets,
                                                                     prevTimeStep, model)
@@ -212,7 +211,7 @@ class NISTGPU:
             U_np += dU
             PExt.zero_()
             P, K, F = self.computeElements(elements, U_np, dU, P, K, F, timeStep)
-            PExt, K = self.assembleLoads(nodeforces, [], bodyforces, U_np, PExt, K, timeStep)
+            PExt, K = self.assembleLoads(nodeforces, distributedloads, bodyforces, U_np, PExt, K, timeStep)
             R.copy_(P)
             R += PExt
             if iterationCounter == 0 and not isExtrapolatedIncrement and dirichlets:
@@ -350,7 +349,27 @@ class NISTGPU:
             load = cLoad.getCurrentLoad(timeStep).flatten()
             PExt.index_add_(0, self.torch.as_tensor(idx, device=PExt.device),
                             self.torch.as_tensor(load, dtype=self.torch.float64, device=PExt.device))
+        PExt, K = self.computeDistributedLoads(distributedLoads, U_np, PExt, K, timeStep)
         PExt, K = self.computeBodyForces(bodyForces, U_np, PExt, K, timeStep)
+        return PExt, K
+
+    def computeDistributedLoads(self, distributedLoads, U_np, PExt, K, timeStep):
+        """nonlinearimplicitstatic.py:460-501 on the device: every distributed load adds, face by face of its
+        surface, the load on the faces of the surface's elements to PExt. The GPU path carries the load type
+        pressure, which acts on the reference configuration and adds nothing to K."""
+        if not distributedLoads:
+            return PExt, K
+        tic = _time.perf_counter()
+        for dLoad in distributedLoads:
+            if str(dLoad.loadType).lower() != "pressure":
+                raise NotImplementedError(f"distributed load type {dLoad.loadType} is not on the GPU path (pressure)")
+            load = np.asarray(dLoad.getCurrentLoad(timeStep), dtype=np.float64).ravel()
+            for faceID, (blockIndex, elements) in dLoad.surface.items():
+                if blockIndex != 0:
+                    raise NotImplementedError("distributed loads act on the element block of the GPU path")
+                self.assembly.surfacePressure(PExt, elements, faceID, float(load[0]))
+        self.torch.cuda.synchronize(self.assembly.device)
+        self.computationTimes["distributed loads"] += _time.perf_counter() - tic
         return PExt, K
 
     def computeBodyForces(self, bodyForces, U_np, PExt, K, timeStep):

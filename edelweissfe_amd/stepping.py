@@ -285,6 +285,60 @@ class BodyForce:
         return self.elements
 
 
+class DistributedLoad:
+    """``distributedload, surface=.., type=pressure, magnitude=p`` (edelweissfe/stepactions/distributedload.py:
+    52-120): a load on the faces of an element-based surface, held at its step-end value afterwards; ``amplitude``
+    is f(stepProgress). A later step changes it with ``updateStepAction(magnitude=...)`` (total) or
+    ``(delta=...)``; the GPU path carries the load type pressure."""
+
+    def __init__(self, name, model, surface: str, loadType: str, magnitude, amplitude=None):
+        self.name = name
+        if surface not in model.surfaces:
+            raise KeyError(f"DistributedLoad {name}: unknown surface {surface}")
+        self.surfaceName = surface
+        self._surface = {}
+        for face, elSet in model.surfaces[surface].items():
+            self._surface[int(face)] = model.elementSets[elSet] if isinstance(elSet, str) else elSet
+        self._loadType = loadType
+        self.magnitudeAtStepStart = 0.0
+        self.delta = np.asarray(magnitude, dtype=np.float64).ravel()
+        self.amplitude = amplitude or _linear
+        self._idle = False
+
+    @property
+    def surface(self):
+        """{face number: (block index, element indices)}."""
+        return self._surface
+
+    @property
+    def loadType(self):
+        return self._loadType
+
+    def applyAtStepStart(self, model):
+        pass
+
+    def applyAtIncrementStart(self, model, timeStep):
+        pass
+
+    def applyAtStepEnd(self, model):
+        if not self._idle:
+            self.magnitudeAtStepStart = self.magnitudeAtStepStart + self.delta * self.amplitude(1.0)
+            self.delta = 0
+            self._idle = True
+
+    def updateStepAction(self, magnitude=None, delta=None, amplitude=None):
+        if magnitude is not None:
+            self.delta = np.asarray(magnitude, dtype=np.float64).ravel() - self.magnitudeAtStepStart
+        elif delta is not None:
+            self.delta = np.asarray(delta, dtype=np.float64).ravel()
+        self.amplitude = amplitude or _linear
+        self._idle = False
+
+    def getCurrentLoad(self, timeStep: TimeStep):
+        t = 1.0 if self._idle else timeStep.stepProgress
+        return self.magnitudeAtStepStart + self.delta * self.amplitude(t)
+
+
 class Options:
     """``options, category=NISTSolver, key=value`` (edelweissfe/stepactions/options.py:38-61)."""
 

@@ -200,6 +200,14 @@ int ewfe_qp_results(const ewfe_plan *plan, int mat_kind, const double *mat_param
                     const double *X_dev, const double *U_dev, const double *state_dev, int result, double *out_dev,
                     void *stream);
 
+/* Distributed load of type pressure (edelweissfe/solvers/nonlinearimplicitstatic.py:460-501 with the
+ * stepaction edelweissfe/stepactions/distributedload.py; the element side is the Marmot element's
+ * computeDistributedLoad, element.pxd): PExt += -p * integral over face `face` (Abaqus numbering 1..6) of
+ * N_a n dA, reference configuration, for the elements elements_dev[0..n) (element indices; an element listed
+ * twice is loaded once, indices outside [0, n_el) are skipped); summed per node in ascending element order. */
+int ewfe_surface_pressure(const ewfe_plan *plan, const double *X_dev, const int32_t *elements_dev, int64_t n_elements,
+                          int face, double pressure, double *PExt_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -515,3 +515,45 @@ void oracle_body_force(int ek, int64_t n_el, const int32_t *conn, const double *
         for (int k = 0; k < nd; ++k) PExt[edofs[e * nd + k]] += Pe[k];
     }
 }
+
+/* Pressure on one face of the masked elements (the Marmot element's computeDistributedLoad for type pressure,
+ * called by nonlinearimplicitstatic.py:460-501): Pe_a = -p * sum_g N_a (dX/du x dX/dv) s w_g on the face with
+ * Abaqus number `face` (1: zeta=-1, 2: zeta=+1, 3: eta=-1, 4: xi=+1, 5: eta=+1, 6: xi=-1), (u, v) the parametric
+ * axes after the fixed one in cyclic order and s the sign of the fixed coordinate, so that the normal points
+ * out of the element; Gauss rule 2x2 (8 nodes) or 3x3 (20 nodes); reference configuration. */
+void oracle_surface_pressure(int ek, int64_t n_el, const int32_t *conn, const double *X, const int64_t *edofs,
+                             const uint8_t *mask, int face, double pressure, double *PExt)
+{
+    static const int axis[7] = {0, 2, 2, 1, 0, 1, 0}, sign[7] = {0, -1, 1, -1, 1, 1, -1};
+    const int nn = el_nnodes(ek), nd = 3 * nn, ng = nn == 8 ? 2 : 3;
+    const int ax = axis[face], u = (ax + 1) % 3, v = (ax + 2) % 3;
+    const double s = sign[face];
+    const double g2[2] = {-1.0 / sqrt(3.0), 1.0 / sqrt(3.0)}, w2[2] = {1.0, 1.0};
+    const double g3[3] = {-sqrt(0.6), 0.0, sqrt(0.6)}, w3[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    for (int64_t e = 0; e < n_el; ++e) {
+        if (mask && !mask[e]) continue;
+        double Xe[3 * MAXN], Pe[MAXD];
+        for (int a = 0; a < nn; ++a)
+            for (int c = 0; c < 3; ++c) Xe[3 * a + c] = X[3 * (int64_t)conn[e * nn + a] + c];
+        for (int k = 0; k < nd; ++k) Pe[k] = 0.0;
+        for (int i = 0; i < ng; ++i)
+            for (int j = 0; j < ng; ++j) {
+                double p[3], dN[MAXN][3], N[MAXN], tu[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
+                p[ax] = s;
+                p[u] = ng == 2 ? g2[i] : g3[i];
+                p[v] = ng == 2 ? g2[j] : g3[j];
+                const double w = ng == 2 ? w2[i] * w2[j] : w3[i] * w3[j];
+                shape_deriv(nn, p, dN, N);
+                for (int a = 0; a < nn; ++a)
+                    for (int c = 0; c < 3; ++c) {
+                        tu[c] += dN[a][u] * Xe[3 * a + c];
+                        tv[c] += dN[a][v] * Xe[3 * a + c];
+                    }
+                const double n[3] = {s * (tu[1] * tv[2] - tu[2] * tv[1]), s * (tu[2] * tv[0] - tu[0] * tv[2]),
+                                     s * (tu[0] * tv[1] - tu[1] * tv[0])};
+                for (int a = 0; a < nn; ++a)
+                    for (int c = 0; c < 3; ++c) Pe[3 * a + c] -= pressure * N[a] * n[c] * w;
+            }
+        for (int k = 0; k < nd; ++k) PExt[edofs[e * nd + k]] += Pe[k];
+    }
+}

@@ -73,6 +73,9 @@ def lib():
         L.oracle_body_force.restype = None
         L.oracle_body_force.argtypes = [ctypes.c_int, ctypes.c_int64, i32, d, i64, ctypes.POINTER(ctypes.c_uint8), d,
                                         d]
+        L.oracle_surface_pressure.restype = None
+        L.oracle_surface_pressure.argtypes = [ctypes.c_int, ctypes.c_int64, i32, d, i64, ctypes.POINTER(ctypes.c_uint8),
+                                              ctypes.c_int, ctypes.c_double, d]
         _lib = L
     return _lib
 
@@ -248,6 +251,25 @@ def body_force(el_kind, conn, X, edofs, nDof, b, elements=None):
     return PExt
 
 
+def surface_pressure(el_kind, conn, X, edofs, nDof, face, pressure, elements=None):
+    """Distributed load of type pressure (nonlinearimplicitstatic.py:460-501 calling the Marmot element's
+    computeDistributedLoad): the external load vector of the pressure `pressure` on face `face` (Abaqus numbering
+    1..6) of `elements` (all when None), reference configuration. Pinned by flat-face closed forms and the
+    LinearElasticIsotropic deck (tests/test_oracle.py)."""
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    mask = None
+    if elements is not None:
+        mask = np.zeros(len(conn), dtype=np.uint8)
+        mask[np.asarray(elements, dtype=np.int64)] = 1
+    PExt = np.zeros(nDof)
+    lib().oracle_surface_pressure(EL_KINDS[el_kind], len(conn), _p(conn, ctypes.c_int32),
+                                  _p(np.ascontiguousarray(X, dtype=np.float64)),
+                                  _p(np.ascontiguousarray(edofs, dtype=np.int64), ctypes.c_int64),
+                                  _p(mask, ctypes.c_uint8) if mask is not None else None, int(face), float(pressure),
+                                  _p(PExt))
+    return PExt
+
+
 # ----------------------------------------------------------------------------------------------
 # Newton loop (NIST).  Pinned by the reference's U.ref decks (tests/golden, 1e-6 abs).
 # ----------------------------------------------------------------------------------------------
@@ -324,7 +346,8 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
     model: dict(el_kind, mat_kind, mp, X, conn, el_mp?)
     step:  dict(maxInc, minInc, maxNumInc, maxIter, criticalIter=5, maxGrowIter=10, cutbackFactor=0.25,
                 startInc=1.0, stepLength=1.0, extrapolation='linear',
-                dirichlet=[(node_idx, {comp: value})], nodeforces=[(node_idx, vec3)])
+                dirichlet=[(node_idx, {comp: value})], nodeforces=[(node_idx, vec3)], bodyforces=[vec3],
+                pressures=[(element idx, face 1..6, magnitude, f(stepProgress) or None)])
     Returns dict(U, P, state, history=[per increment: list of (|R|inf, |ddU|inf)], iterations=[...]).
     """
     X, conn = model["X"], model["conn"]
@@ -384,6 +407,10 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
                 PExt = np.zeros(nDof)
                 for idx, load in nodeforces:
                     PExt[idx] += load * tstep["stepProgress"]
+                for els, face, mag, amp in step.get("pressures", []):
+                    PExt += surface_pressure(model["el_kind"], conn, X, edofs, nDof, face,
+                                             mag * (amp(tstep["stepProgress"]) if amp else tstep["stepProgress"]),
+                                             els)
                 for bvec in step.get("bodyforces", []):
                     PExt += body_force(model["el_kind"], conn, X, edofs, nDof,
                                        np.asarray(bvec, float) * tstep["stepProgress"])

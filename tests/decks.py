@@ -43,7 +43,22 @@ def step(fx, sets):
         if d["elSet"] != "all":
             raise NotImplementedError("deck body forces on element sets other than 'all'")
     st["bodyforces"] = [d["vector"] for d in st.get("bodyforces", [])]
+    if st.get("pressures"):
+        if "elements" not in fx:
+            raise NotImplementedError("deck pressures on generated meshes")
+        pos = {e[0]: i for i, e in enumerate(fx["elements"])}
+        st["pressures"] = [(np.array([pos[lab] for lab in d["elements"]], dtype=np.int64), d["face"], d["magnitude"],
+                            amplitude(d["f(t)"])) for d in st["pressures"]]
     return st
+
+
+def amplitude(expr):
+    """f(t) of a fixture's step action (an arithmetic expression in t written by make_fixtures.py from the
+    deck; linear when absent)."""
+    if not expr:
+        return lambda t: t
+    code = compile(expr, "<f(t)>", "eval")
+    return lambda t: float(eval(code, {"__builtins__": {}}, {"t": float(t)}))
 
 
 def used_node_order(X, conn):

@@ -16,7 +16,7 @@ import os
 import sys
 
 DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C3D8", "C3D20", "WallShearHexa8",
-         "WallShearHexa20"]
+         "WallShearHexa20", "LinearElasticIsotropic"]
 
 
 def read_keywords(path):
@@ -73,7 +73,9 @@ def numbers(lines):
 def convert(deck_dir, name):
     blocks = read_keywords(os.path.join(deck_dir, "test.inp"))
     fx = {"name": name}
-    nodes, elements, nsets = [], [], {}
+    nodes, elements, nsets, elsets, surfaces = [], [], {}, {}, {}
+    actions = {}  # step actions persist across steps, keyed by (kind, name) (stepmanager.py:133-200)
+    n_steps = 0
     for b in blocks:
         kw, o = b["kw"], b["opts"]
         if kw == "material":
@@ -94,6 +96,14 @@ def convert(deck_dir, name):
             if o.get("generate", "").lower() == "true":
                 vals = list(range(vals[0], vals[1] + 1, vals[2]))
             nsets[o["nset"]] = vals
+        elif kw == "elset":
+            vals = [int(v) for v in numbers(b["data"])]
+            if o.get("generate", "").lower() == "true":
+                vals = list(range(vals[0], vals[1] + 1, vals[2]))
+            elsets[o["elset"]] = vals
+        elif kw == "surface":
+            surfaces[o["name"]] = [(ln.split(",")[0].strip(), int(ln.split(",")[1].strip().upper().replace("S", "")))
+                                   for ln in b["data"]]
         elif kw == "modelgenerator":
             g = {}
             for ln in b["data"]:
@@ -106,20 +116,36 @@ def convert(deck_dir, name):
                   if k.lower() in o}
             st = {k: (int(float(v)) if k in ("maxNumInc", "maxIter") else float(v)) for k, v in st.items()}
             st["extrapolation"] = "linear"
-            st["dirichlet"], st["nodeforces"], st["bodyforces"] = [], [], []
+            loaded = any(k[0] != "dirichlet" or any(v["components"].values()) for k, v in actions.items())
+            if n_steps and loaded:
+                raise NotImplementedError(f"{name}: only the last step of a deck may carry loads")
+            n_steps += 1
             for ln in b["data"]:
                 kind, lo = parse_line_opts(ln)
+                key = (kind, lo.get("name", kind))
                 if kind == "options" and "extrapolation" in lo:
                     st["extrapolation"] = lo["extrapolation"]
                 elif kind == "dirichlet":
                     comps = {str(int(k) - 1): float(v) for k, v in lo.items() if k in ("1", "2", "3")}
-                    st["dirichlet"].append({"nSet": lo["nSet"], "components": comps})
+                    actions[key] = {"nSet": lo["nSet"], "components": comps}
                 elif kind == "nodeforces":
                     vec = [float(t) for t in lo["components"].split(",")]
-                    st["nodeforces"].append({"nSet": lo["nSet"], "vector": vec})
+                    actions[key] = {"nSet": lo["nSet"], "vector": vec}
                 elif kind == "bodyforce":
                     vec = [float(t) for t in lo["forceVector"].split(",")]
-                    st["bodyforces"].append({"elSet": lo["elSet"], "vector": vec})
+                    actions[key] = {"elSet": lo["elSet"], "vector": vec}
+                elif kind == "distributedload":
+                    if lo["type"].lower() != "pressure":
+                        raise NotImplementedError(f"{name}: distributed load type {lo['type']}")
+                    for elSet, face in surfaces[lo["surface"]]:
+                        labs = [e[0] for e in elements] if elSet == "all" else elsets[elSet]
+                        actions[key + (face,)] = {"surface": lo["surface"], "elements": labs, "face": face,
+                                                  "magnitude": float(lo["magnitude"]), "f(t)": lo.get("f(t)")}
+            st["dirichlet"] = [v for k, v in actions.items() if k[0] == "dirichlet"]
+            st["nodeforces"] = [v for k, v in actions.items() if k[0] == "nodeforces"]
+            st["bodyforces"] = [v for k, v in actions.items() if k[0] == "bodyforce"]
+            if any(k[0] == "distributedload" for k in actions):
+                st["pressures"] = [v for k, v in actions.items() if k[0] == "distributedload"]
             fx["step"] = st
     if nodes:
         fx["nodes"] = nodes

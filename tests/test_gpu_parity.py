@@ -202,7 +202,7 @@ def test_pcg_solves_row_eliminated_system():
 
 def _deck_run(name, tangent="fp64", solver_options=None):
     from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
-    from edelweissfe_amd.stepping import AdaptiveStep, BodyForce, Dirichlet, NodeForces, Options
+    from edelweissfe_amd.stepping import AdaptiveStep, BodyForce, Dirichlet, DistributedLoad, NodeForces, Options
 
     fx = decks.load(name)
     X, conn, sets = decks.mesh(fx, O.boxgen)
@@ -215,6 +215,11 @@ def _deck_run(name, tangent="fp64", solver_options=None):
         acts["nodeforces"][f"f{i}"] = NodeForces(f"f{i}", model, d["nSet"], d["vector"])
     for i, d in enumerate(st.get("bodyforces", [])):
         acts["bodyforce"][f"b{i}"] = BodyForce(f"b{i}", model, d["elSet"], d["vector"])
+    acts["distributedload"] = {}
+    for i, (els, face, mag, amp) in enumerate(decks.step(fx, sets).get("pressures", [])):
+        model.elementSets[f"pset{i}"] = (0, els)
+        model.surfaces[f"s{i}"] = {face: f"pset{i}"}
+        acts["distributedload"][f"p{i}"] = DistributedLoad(f"p{i}", model, f"s{i}", "pressure", [mag], amplitude=amp)
     acts["options"]["NISTSolver"] = Options("o", "NISTSolver", extrapolation=st["extrapolation"])
     kw = {k: st[k] for k in ("maxInc", "minInc", "maxNumInc", "maxIter", "stepLength", "startInc") if k in st}
     step = AdaptiveStep(1, model, acts, **kw)
@@ -224,7 +229,7 @@ def _deck_run(name, tangent="fp64", solver_options=None):
 
 
 @pytest.mark.parametrize("name", ["TensionBarHexa8", "CantileverBeamHexa8", "C3D20", "C3D8", "SimpleBeamHexa8",
-                                  "WallShearHexa8", "WallShearHexa20"])
+                                  "WallShearHexa8", "WallShearHexa20", "LinearElasticIsotropic"])
 def test_reference_decks_on_gpu(name):
     print(f"deck {name}", flush=True)
     fx, U, solver = _deck_run(name)
@@ -266,6 +271,31 @@ def test_body_force_matches_oracle(el):
     g.asm.bodyForce(PExt, b, sub)
     torch.cuda.synchronize()
     assert rel(PExt.cpu().numpy(), O.body_force(el, conn, X, edofs, nDof, b, sub)) < 1e-13
+
+
+@pytest.mark.parametrize("el", ["C3D8", "C3D20"])
+def test_surface_pressure_matches_oracle(el):
+    """ewfe_surface_pressure on a jittered mesh, every face on all elements and face 5 on every second element,
+    against the oracle's restatement of the pressure load; PExt is accumulated, not overwritten."""
+    import torch
+
+    print(f"case surface pressure {el}", flush=True)
+    X, conn, _ = jittered_box((3, 2, 4) if el == "C3D8" else (2, 2, 1), el, seed=16)
+    g = GpuEval(el, "linearelastic", LE, X, conn)
+    base, nDof, edofs = O.dof_numbering(len(X), conn)
+    allEl = np.arange(len(conn))
+    PExt = g.dm.constructDofVector()
+    for face in range(1, 7):
+        PExt.fill_(0.5)
+        g.asm.surfacePressure(PExt, allEl, face, 1.3 * face)
+        torch.cuda.synchronize()
+        ref = O.surface_pressure(el, conn, X, edofs, nDof, face, 1.3 * face) + 0.5
+        assert rel(PExt.cpu().numpy(), ref) < 1e-13, face
+    sub = np.arange(1, len(conn), 2)
+    PExt.zero_()
+    g.asm.surfacePressure(PExt, sub, 5, -0.8)
+    torch.cuda.synchronize()
+    assert rel(PExt.cpu().numpy(), O.surface_pressure(el, conn, X, edofs, nDof, 5, -0.8, sub)) < 1e-13
 
 
 @pytest.mark.parametrize("el,mat,mp", [("C3D8", "linearelastic", LE), ("C3D8", "vonmises", J2),

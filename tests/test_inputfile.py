@@ -16,7 +16,7 @@ from oracle import oracle as O
 
 DECK_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks")
 FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C3D8", "C3D20", "WallShearHexa8",
-                 "WallShearHexa20"]
+                 "WallShearHexa20", "LinearElasticIsotropic"]
 ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D"]
 
 
@@ -25,21 +25,37 @@ def _deck(name):
 
 
 def _oracle_step(model, step):
-    """The oracle's step description of a one-step deck built by the reader."""
+    """The oracle's step description of a step built by the reader (loads of earlier steps are not carried:
+    decks whose earlier steps only set boundary conditions)."""
     a = step.actions
     st = {k: getattr(step, f) for k, f in (("maxInc", "maxIncrementSize"), ("minInc", "minIncrementSize"),
                                           ("maxNumInc", "maxNumberIncrements"), ("maxIter", "maxIter"),
                                           ("stepLength", "length"), ("startInc", "startIncrementSize"))}
     opts = a["options"]["NISTSolver"].options if "NISTSolver" in a["options"] else {}
     st["extrapolation"] = opts.get("extrapolation", "linear")
-    st["dirichlet"] = [(d.nSet, {c: float(v) for c, v in zip(d.components, d.delta[0])}) for d in a["dirichlet"].values()]
+    st["dirichlet"] = [(d.nSet, {c: float(v) * d.active for c, v in zip(d.components, d.delta[0])})
+                       for d in a["dirichlet"].values()]
     st["nodeforces"] = [(f.nodeSet, list(f.nodeForcesDelta[0])) for f in a["nodeforces"].values()]
     st["bodyforces"] = []
     for b in a["bodyforce"].values():
         assert b.amplitude(0.5) == 0.5  # the oracle scales body forces linearly with the step progress
         assert len(b.elementSet) == len(model.elementBlocks[0])
         st["bodyforces"].append(list(b.delta))
+    st["pressures"] = []
+    for d in a.get("distributedload", {}).values():
+        assert d.loadType == "pressure" and np.all(d.magnitudeAtStepStart == 0)
+        for face, (_, els) in d.surface.items():
+            st["pressures"].append((els, face, float(d.delta[0]), d.amplitude))
     return st
+
+
+def _steps(model, inp):
+    """Every step of the deck, with the step-end actions applied between steps as the solver does."""
+    for step, solverName in IF.createStepsFromInputFile(model, inp):
+        yield step, solverName
+        for typ in step.actions.values():
+            for action in typ.values():
+                action.applyAtStepEnd(model)
 
 
 def test_parser_structure():
@@ -66,12 +82,15 @@ def test_reader_builds_the_fixture_models(name):
                                                                  fx["mp"])
     for act in fx["step"]["dirichlet"] + fx["step"]["nodeforces"]:
         assert np.array_equal(np.sort(model.nodeSets[act["nSet"]]), np.sort(sets[act["nSet"]]))
-    (step, _), = list(IF.createStepsFromInputFile(model, _deck(name)))
-    st = _oracle_step(model, step)
+    for step, _ in _steps(model, _deck(name)):
+        st = _oracle_step(model, step)  # the last step's description, taken before its step-end actions
     ref = decks.step(fx, sets)
     assert st["extrapolation"] == ref["extrapolation"] and st["maxInc"] == ref["maxInc"]
     assert len(st["dirichlet"]) == len(ref["dirichlet"]) and len(st["nodeforces"]) == len(ref["nodeforces"])
     assert st["bodyforces"] == ref["bodyforces"]
+    assert len(st["pressures"]) == len(ref.get("pressures", []))
+    for (e1, f1, m1, a1), (e2, f2, m2, a2) in zip(st["pressures"], ref.get("pressures", [])):
+        assert np.array_equal(np.sort(e1), np.sort(e2)) and f1 == f2 and m1 == m2 and a1(0.3) == a2(0.3)
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)
@@ -79,12 +98,16 @@ def test_reader_and_oracle_reproduce_uref(name):
     """Deck -> reader -> oracle Newton loop -> U.ref (CPU)."""
     inp = _deck(name)
     model = IF.createModelFromInputFile(inp)
-    (step, _), = list(IF.createStepsFromInputFile(model, inp))
     b = model.elementBlocks[0]
-    r = O.solve_step(dict(el_kind=b.elType, mat_kind=b.material.name, mp=b.material.materialProperties,
-                          X=model.nodeCoordinates, conn=b.conn), _oracle_step(model, step))
+    om = dict(el_kind=b.elType, mat_kind=b.material.name, mp=b.material.materialProperties,
+              X=model.nodeCoordinates, conn=b.conn)
+    U = state = None
+    for step, _ in _steps(model, inp):
+        r = O.solve_step(om, _oracle_step(model, step), U0=U, state_n=state)
+        U, state = r["U"], r["state"]
     Uref = np.loadtxt(os.path.join(DECK_DIR, name, "U.ref"))
-    assert np.abs(r["U"] - Uref).max() < 1e-6
+    assert np.abs(U - Uref).max() < 1e-6
+    assert np.abs(U - Uref).max() < 1e-7 * max(1.0, np.abs(Uref).max())
 
 
 @pytest.mark.gpu

@@ -21,7 +21,7 @@ def _build():
 
 
 @pytest.mark.parametrize("name", ["TensionBarHexa8", "CantileverBeamHexa8", "C3D20", "C3D8", "SimpleBeamHexa8",
-                                  "WallShearHexa8", "WallShearHexa20"])
+                                  "WallShearHexa8", "WallShearHexa20", "LinearElasticIsotropic"])
 def test_reference_decks(name):
     fx = decks.load(name)
     X, conn, sets = decks.mesh(fx, O.boxgen)
@@ -182,3 +182,77 @@ def test_body_force_brick_closed_form(nn):
     Pa = O.body_force("C3D8" if nn == 8 else "C3D20", conn2, X2, edofs2, nDof2, b, np.nonzero(half)[0])
     Pb = O.body_force("C3D8" if nn == 8 else "C3D20", conn2, X2, edofs2, nDof2, b, np.nonzero(~half)[0])
     assert np.allclose(Pa + Pb, P2.ravel(), rtol=1e-13, atol=1e-15)
+
+
+def test_pressure_deck_pins_face_numbering():
+    """LinearElasticIsotropic: one C3D8 (Marmot), point supports, pressure 0.15 * t**2 on face S4 in the second
+    step. The linear solution is reproduced to rounding (U.ref holds 19 digits, displacements ~1e-5), which fixes
+    the face's nodes, its normal's sign and the face quadrature of the restatement."""
+    fx = decks.load("LinearElasticIsotropic")
+    X, conn, sets = decks.mesh(fx, O.boxgen)
+    r = O.solve_step(dict(el_kind=fx["el_type"], mat_kind=fx["material"], mp=fx["mp"], X=X, conn=conn),
+                     decks.step(fx, sets))
+    Uref = np.array(fx["U_ref"])
+    assert np.abs(r["U"] - Uref).max() < 1e-12 * np.abs(Uref).max()
+
+
+# outward unit normal and the local nodes (Abaqus order, 1-based in the comments) of faces 1..6 of a hexahedron
+_FACES = {1: ((0, 0, -1), [0, 1, 2, 3], [8, 9, 10, 11]), 2: ((0, 0, 1), [4, 5, 6, 7], [12, 13, 14, 15]),
+          3: ((0, -1, 0), [0, 1, 4, 5], [8, 12, 16, 17]), 4: ((1, 0, 0), [1, 2, 5, 6], [9, 13, 17, 18]),
+          5: ((0, 1, 0), [2, 3, 6, 7], [10, 14, 18, 19]), 6: ((-1, 0, 0), [0, 3, 4, 7], [11, 15, 16, 19])}
+
+
+@pytest.mark.parametrize("nn", [8, 20])
+def test_surface_pressure_brick_closed_form(nn):
+    """Consistent nodal loads of a pressure p on each face of a rectangular brick: -p A n / 4 at the four face
+    nodes of a C3D8; +p A n / 12 at the face corners and -p A n / 3 at the face mid-side nodes of a C3D20
+    (S1 = nodes 1-2-3-4 at zeta=-1, ..., S4 = 2-3-7-6 at xi=+1, ...); nothing on the other nodes."""
+    L = (2.0, 3.0, 0.5)
+    ek = "C3D8" if nn == 8 else "C3D20"
+    xi = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1],
+                   [0, -1, -1], [1, 0, -1], [0, 1, -1], [-1, 0, -1], [0, -1, 1], [1, 0, 1], [0, 1, 1], [-1, 0, 1],
+                   [-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], dtype=float)[:nn]  # Abaqus node order
+    X = (xi + 1) / 2 * np.array(L) + np.array([0.4, -1.0, 2.0])
+    conn = np.arange(nn, dtype=np.int32)[None, :]
+    base, nDof, edofs = O.dof_numbering(len(X), conn)
+    rows = np.array([base[n] // 3 for n in conn[0]])
+    p = 1.7
+    for face, (n, corners, mids) in _FACES.items():
+        A = np.prod([L[i] for i in range(3) if n[i] == 0])
+        P = O.surface_pressure(ek, conn, X, edofs, nDof, face, p).reshape(-1, 3)[rows]
+        want = np.zeros((nn, 3))
+        if nn == 8:
+            want[corners] = -p * A * np.array(n) / 4
+        else:
+            want[corners] = p * A * np.array(n) / 12
+            want[mids] = -p * A * np.array(n) / 3
+        assert np.allclose(P, want, rtol=1e-13, atol=1e-13), face
+
+
+@pytest.mark.parametrize("nn", [8, 20])
+def test_surface_pressure_closed_surface_invariants(nn):
+    """On a distorted element, a uniform pressure over all six faces is self-equilibrated: the face loads sum to
+    zero force (the closed surface integral of n dA vanishes) and zero moment about the origin (sum_a X_a x P_a =
+    integral of x x n dA = 0), which fails for any face with a flipped normal or mismatched node order; a subset
+    of elements adds up with its complement."""
+    ek = "C3D8" if nn == 8 else "C3D20"
+    X, conn, _ = O.boxgen(2, 1, 2, 1.0, 1.5, 0.8, nnodes=nn)
+    rng = np.random.default_rng(11)
+    X = X + rng.uniform(-0.05, 0.05, X.shape)
+    base, nDof, edofs = O.dof_numbering(len(X), conn)
+    node_of_row = np.empty(nDof // 3, dtype=np.int64)
+    node_of_row[base // 3] = np.arange(len(X))
+    tot = np.zeros(nDof)
+    for face in range(1, 7):
+        tot += O.surface_pressure(ek, conn, X, edofs, nDof, face, 2.5)
+    F = tot.reshape(-1, 3)
+    assert np.abs(F.sum(axis=0)).max() < 1e-12
+    M = np.cross(X[node_of_row], F).sum(axis=0)
+    assert np.abs(M).max() < 1e-12
+    one = O.surface_pressure(ek, conn, X, edofs, nDof, 4, 2.5).reshape(-1, 3)
+    assert np.abs(one.sum(axis=0)).max() > 0.1  # a single face is not equilibrated
+    sub = np.array([0, 3])
+    rest = np.setdiff1d(np.arange(len(conn)), sub)
+    a = O.surface_pressure(ek, conn, X, edofs, nDof, 2, 2.5, sub)
+    b = O.surface_pressure(ek, conn, X, edofs, nDof, 2, 2.5, rest)
+    assert np.allclose(a + b, O.surface_pressure(ek, conn, X, edofs, nDof, 2, 2.5), rtol=1e-14, atol=1e-15)
