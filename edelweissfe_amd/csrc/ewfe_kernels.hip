@@ -388,14 +388,20 @@ __device__ __forceinline__ void add_to(VT *vals, int64_t idx, double v)
 template <int NN, int FORM, int MAT, typename ND, typename PG, typename GG, typename PP>
 __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd, PG put_g, GG get_g, PP put_p,
                                          const EvalArgs &A, double Epar, int64_t pos, int q, int NQ, Recipe &r,
-                                         double &wd)
+                                         double &wd, const double *state_loaded = nullptr)
 {
-    // converged state first: its loads do not depend on anything below
+    // converged state first: its loads do not depend on anything below (or the caller has loaded it:
+    // state_loaded = plastic strain 6, kappa)
     double ep[6] = {0, 0, 0, 0, 0, 0}, kn = 0.0;
     if (FORM == FORM_SS && MAT == M_J2) {
         const int64_t n = A.n_el;
-        for (int s = 0; s < 6; ++s) ep[s] = A.state_n[((int64_t)s * NQ + q) * n + pos];
-        kn = A.state_n[((int64_t)6 * NQ + q) * n + pos];
+        if (state_loaded) {
+            for (int s = 0; s < 6; ++s) ep[s] = state_loaded[s];
+            kn = state_loaded[6];
+        } else {
+            for (int s = 0; s < 6; ++s) ep[s] = A.state_n[((int64_t)s * NQ + q) * n + pos];
+            kn = A.state_n[((int64_t)6 * NQ + q) * n + pos];
+        }
     }
     double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3];
     for (int a = 0; a < NN; ++a)
@@ -547,32 +553,74 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     const int64_t n = A.n_el;
     const int64_t el = active ? (int64_t)A.order[pos] : 0;  // records and nodal forces are stored by element
 
-    // phase 1: lane (le, q=t) evaluates one quadrature point; the 8 lanes of an element read its nodal
-    // X, U (, dU) straight from global memory (same addresses: one fetch per cache line), all loads issued
-    // before any arithmetic
+    // phase 1: lane (le, t) first stages node t of its element (X, U (, dU)) in LDS, in the fields that
+    // phase 1 writes last (the tangent coefficients and the stress); then lane (le, q = t) evaluates one
+    // quadrature point reading the element's nodes from there. One node per lane keeps the nodal data out
+    // of the registers.
+    constexpr int NVN = (MAT == M_J2) ? 9 : 6;  // staged values per node
+    static_assert(EPB * NN * NVN <= (L::NF - L::AL) * 64, "nodal staging must fit in the last fields");
+    double *s_nd = s_qp + L::AL * 64;
     double Epar = A.mat.p[0];
     if (active && A.el_param) Epar = A.el_param[A.order[pos]];
+    double st[7] = {0, 0, 0, 0, 0, 0, 0};  // converged J2 state of point (le, q = t), loaded with the nodes
+    if (active && FORM == FORM_SS && MAT == M_J2)
+        for (int s = 0; s < 7; ++s) st[s] = A.state_n[((int64_t)s * NQ + t) * n + pos];
+    if (active) {
+        const int32_t da = A.edof[(int64_t)t * n + pos];
+        double *dst = s_nd + (le * NN + t) * NVN;
+        const double x0 = A.X[da], x1 = A.X[da + 1], x2 = A.X[da + 2];
+        const double u0 = A.U[da], u1 = A.U[da + 1], u2 = A.U[da + 2];
+        double d0 = 0, d1 = 0, d2 = 0;
+        if (MAT == M_J2) {
+            d0 = A.dU[da];
+            d1 = A.dU[da + 1];
+            d2 = A.dU[da + 2];
+        }
+        dst[0] = x0;
+        dst[1] = x1;
+        dst[2] = x2;
+        dst[3] = u0;
+        dst[4] = u1;
+        dst[5] = u2;
+        if (MAT == M_J2) {
+            dst[6] = d0;
+            dst[7] = d1;
+            dst[8] = d2;
+        }
+    }
+    __syncthreads();
+    Recipe r;
+    double wd = 0.0;
     if (active) {
         const int q = t;
-        double xn[NN][3], un[NN][3], dun[NN][3];
-#pragma unroll
-        for (int a = 0; a < NN; ++a) {
-            const int32_t da = A.edof[(int64_t)a * n + pos];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                xn[a][c] = A.X[da + c];
-                un[a][c] = A.U[da + c];
-                dun[a][c] = (MAT == M_J2) ? A.dU[da + c] : 0.0;
-            }
-        }
-        auto nd = [&](int f, int a) { return f < 3 ? xn[a][f] : (f < 6 ? un[a][f - 3] : dun[a][f - 6]); };
+        const double *src = s_nd + le * NN * NVN;
+        auto nd = [&](int f, int a) { return (MAT != M_J2 && f >= 6) ? 0.0 : src[a * NVN + f]; };
         auto put_g = [&](int a, int j, double v) { s_qp[slot_node(L::G, a, j, q, le)] = v; };
         auto get_g = [&](int a, int j) { return s_qp[slot_node(L::G, a, j, q, le)]; };
         auto put_p = [&](int a, int j, double v) { s_qp[slot_node(L::P, a, j, q, le)] = v; };
-        Recipe r;
-        double wd;
-        const bool ok = qp_phase<NN, FORM, MAT>(c_dN8[q], c_w8[q], nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd);
+        // parametric derivatives of the trilinear shape functions at Gauss point q (point q has coordinates
+        // (+-g, +-g, +-g) by the bits of q; the same arithmetic as the host tables, so the same values)
+        double dNq[NN][3];
+        {
+            const double g = 1.0 / sqrt(3.0);
+            const double x0 = (q & 1) ? g : -g, x1 = (q & 2) ? g : -g, x2 = (q & 4) ? g : -g;
+#pragma unroll
+            for (int b = 0; b < NN; ++b) {
+                const double c0 = ((b & 3) == 1 || (b & 3) == 2) ? 1.0 : -1.0;
+                const double c1 = ((b & 3) >= 2) ? 1.0 : -1.0;
+                const double c2 = (b >= 4) ? 1.0 : -1.0;
+                const double f0 = 1 + c0 * x0, f1 = 1 + c1 * x1, f2 = 1 + c2 * x2;
+                dNq[b][0] = 0.125 * c0 * f1 * f2;
+                dNq[b][1] = 0.125 * f0 * c1 * f2;
+                dNq[b][2] = 0.125 * f0 * f1 * c2;
+            }
+        }
+        const bool ok = qp_phase<NN, FORM, MAT>(dNq, 1.0, nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd, st);
         if (!ok) atomic_min_pos(A.pnewdt, 0.5);
+    }
+    __syncthreads();  // every lane has read the staged nodes
+    if (active) {
+        const int q = t;
         auto put = [&](int f, double v) { s_qp[slot_scalar(f, q, le)] = v; };
         put(L::AL, wd * r.alpha);
         put(L::BE, wd * r.beta);
