@@ -532,6 +532,17 @@ __device__ __forceinline__ int slot_node(int base, int b, int i, int q, int le)
     return (base + 3 * b + i) * 64 + 16 * (le >> 1) + ((2 * q + (le & 1)) ^ (2 * b));
 }
 
+template <typename VT>
+struct Vec16;  // 16-byte vector of VT
+template <>
+struct Vec16<double> {
+    using type = double2;
+};
+template <>
+struct Vec16<float> {
+    using type = float4;
+};
+
 template <int FORM, int MAT, typename VT>
 __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 {
@@ -540,7 +551,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     constexpr int QP_WORDS = L::NF * 64;
     constexpr int HALF_STAGE_WORDS = ((EPB / 2) * kRec8 * (int)sizeof(VT) + 7) / 8;
     static_assert(HALF_STAGE_WORDS <= QP_WORDS, "record staging must fit in the quadrature-point area");
-    __shared__ double s_qp[QP_WORDS];
+    __shared__ __align__(16) double s_qp[QP_WORDS];
     __shared__ int32_t s_el[EPB];
     const int lane = threadIdx.x;
     const int le = lane >> 3, t = lane & 7;
@@ -785,10 +796,16 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
         }
         __syncthreads();
         const int nv = valid - 4 * half;  // elements of this half
-        const int nw = (nv < 4 ? nv : 4) * kRec8;
+        // 16-byte copies: records start at multiples of 16 bytes and hold a multiple of 16 bytes
+        using V16 = typename Vec16<VT>::type;
+        constexpr int VW = 16 / (int)sizeof(VT);
+        static_assert((kRec8 * (int)sizeof(VT)) % 16 == 0, "records must be whole 16-byte units");
+        const int nw = (nv < 4 ? nv : 4) * (kRec8 / VW);
         for (int w = lane; w < nw; w += 64) {
-            const int l = w / kRec8;
-            reinterpret_cast<VT *>(R.kbuf)[(int64_t)s_el[4 * half + l] * kRec8 + (w - l * kRec8)] = stage[w];
+            const int l = w / (kRec8 / VW);
+            const V16 val = reinterpret_cast<const V16 *>(stage)[w];
+            *reinterpret_cast<V16 *>(reinterpret_cast<VT *>(R.kbuf) + (int64_t)s_el[4 * half + l] * kRec8 +
+                                     (w - l * (kRec8 / VW)) * VW) = val;
         }
     }
 }
