@@ -43,7 +43,9 @@ struct ewfe_plan {
     int32_t *d_ninc_ptr = nullptr;     // [n_rownodes + 1] incident elements of every row node (all kinds)
     int32_t *d_ninc = nullptr;         // 32 * element + local node, ascending element
     double *d_bweights = nullptr;      // body-force scratch: nodal volume weights per element (lazy)
-    double *d_fweights = nullptr;      // surface-load scratch: nodal force vectors per element (lazy)
+    double *d_fweights = nullptr;      // surface-load scratch: nodal force vectors per listed element (lazy)
+    int64_t fweights_cap = 0;          // listed elements d_fweights holds
+    int32_t *d_fslot = nullptr;        // surface-load scratch: element -> list slot, -1 if not listed (lazy)
     int32_t *d_pos_of_el = nullptr;    // element -> plan position (lazy)
     int32_t *d_gdesc = nullptr;        // [n_rownodes][4] gather descriptors in gather order
     int32_t *d_gblob = nullptr;        // gather blobs

@@ -1125,12 +1125,14 @@ __global__ __launch_bounds__(256) void k_face_pressure(const int32_t *__restrict
                                                        const int32_t *__restrict__ pos_of_el,
                                                        const double *__restrict__ X, const int32_t *__restrict__ els,
                                                        int64_t n_list, int face, double pressure,
-                                                       double *__restrict__ W, int64_t n_el)
+                                                       double *__restrict__ W, int32_t *__restrict__ slot_of_el,
+                                                       int64_t n_el)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_list) return;
     const int64_t e = els[k];
     if (e < 0 || e >= n_el) return;  // not an element of the plan
+    slot_of_el[e] = (int32_t)k;  // an element listed twice keeps one of its (equal) slots
     const int64_t p = pos_of_el[e];
     double x[NN][3];
     for (int a = 0; a < NN; ++a) {
@@ -1168,20 +1170,23 @@ __global__ __launch_bounds__(256) void k_face_pressure(const int32_t *__restrict
                 for (int c = 0; c < 3; ++c) f[a][c] -= pressure * N[a] * nA[c] * w;
         }
     for (int a = 0; a < NN; ++a)
-        for (int c = 0; c < 3; ++c) W[(e * NN + a) * 3 + c] = f[a][c];  // a repeated element writes the same
+        for (int c = 0; c < 3; ++c) W[(k * NN + a) * 3 + c] = f[a][c];
 }
 
 __global__ __launch_bounds__(256) void k_node_vec_sum(const int32_t *__restrict__ rownode_dof,
                                                       const int32_t *__restrict__ ninc_ptr,
                                                       const int32_t *__restrict__ ninc, const double *__restrict__ W,
-                                                      int npe, double *__restrict__ PExt, int64_t n_rownodes)
+                                                      const int32_t *__restrict__ slot_of_el, int npe,
+                                                      double *__restrict__ PExt, int64_t n_rownodes)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rownodes) return;
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
     for (int32_t k = ninc_ptr[r]; k < ninc_ptr[r + 1]; ++k) {
         const int32_t code = ninc[k];  // 32 * element + local node, ascending element
-        const double *w = W + ((int64_t)(code >> 5) * npe + (code & 31)) * 3;
+        const int32_t slot = slot_of_el[code >> 5];
+        if (slot < 0) continue;  // element not loaded
+        const double *w = W + ((int64_t)slot * npe + (code & 31)) * 3;
         v0 += w[0];
         v1 += w[1];
         v2 += w[2];
@@ -1722,7 +1727,8 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
     void *bufs[] = {p->d_indptr,   p->d_indices, p->d_edof,    p->d_colpos,    p->d_order,
                     p->d_rownode_dof, p->d_gdesc, p->d_gblob, p->d_kbuf,
                     p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta,
-                    p->d_rorder,   p->d_ninc_ptr, p->d_ninc,   p->d_bweights, p->d_fweights, p->d_pos_of_el};
+                    p->d_rorder,   p->d_ninc_ptr, p->d_ninc,   p->d_bweights, p->d_fweights, p->d_pos_of_el,
+                    p->d_fslot};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     for (void *ev : p->chunk_events)
@@ -1971,7 +1977,7 @@ extern "C" int ewfe_dirichlet_rows(const ewfe_plan *p, const int32_t *rows_dev, 
     return 0;
 }
 
-extern "C" int ewfe_body_force(const ewfe_plan *p, const double *X_dev, const uint8_t *el_mask_dev,
+extern "C" int ewfe_body_force(ewfe_plan *p, const double *X_dev, const uint8_t *el_mask_dev,
                                const double *force_host, double *PExt_dev, void *stream)
 {
     if (!p || !X_dev || !force_host || !PExt_dev) {
@@ -1982,7 +1988,7 @@ extern "C" int ewfe_body_force(const ewfe_plan *p, const double *X_dev, const ui
         ewfe::set_error("ewfe_body_force: plan has no device data");
         return EWFE_EINVAL;
     }
-    ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // scratch weights are allocated lazily by the plan
+    ewfe_plan *pm = p;  // the scratch weights are allocated by the plan on first use
     if (!pm->d_bweights) HIP_CHECK(hipMalloc((void **)&pm->d_bweights, sizeof(double) * (size_t)p->npe * p->n_el));
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const unsigned eb = (unsigned)((p->n_el + 255) / 256);
@@ -2050,10 +2056,10 @@ extern "C" int ewfe_qp_results(const ewfe_plan *p, int mat_kind, const double *m
     return 0;
 }
 
-extern "C" int ewfe_surface_pressure(const ewfe_plan *p, const double *X_dev, const int32_t *elements_dev,
+extern "C" int ewfe_surface_pressure(ewfe_plan *p, const double *X_dev, const int32_t *elements_dev,
                                      int64_t n_elements, int face, double pressure, double *PExt_dev, void *stream)
 {
-    if (!p || !X_dev || !PExt_dev || (n_elements > 0 && !elements_dev) || face < 1 || face > 6) {
+    if (!p || !X_dev || !PExt_dev || n_elements < 0 || (n_elements > 0 && !elements_dev) || face < 1 || face > 6) {
         ewfe::set_error("ewfe_surface_pressure: invalid arguments (face: 1..6)");
         return EWFE_EINVAL;
     }
@@ -2062,29 +2068,36 @@ extern "C" int ewfe_surface_pressure(const ewfe_plan *p, const double *X_dev, co
         return EWFE_EINVAL;
     }
     if (n_elements == 0) return 0;
-    ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // scratch is allocated lazily by the plan
-    const size_t wbytes = sizeof(double) * 3 * (size_t)p->npe * p->n_el;
-    if (!pm->d_fweights) HIP_CHECK(hipMalloc((void **)&pm->d_fweights, wbytes));
-    if (!pm->d_pos_of_el) {
+    // scratch, allocated by the plan on first use: element -> plan position, element -> list slot, and the
+    // nodal forces of the listed elements (grown to the largest list seen)
+    if (!p->d_pos_of_el) {
         std::vector<int32_t> inv(p->n_el);
         for (int64_t i = 0; i < p->n_el; ++i) inv[p->order[i]] = (int32_t)i;
-        HIP_CHECK(hipMalloc((void **)&pm->d_pos_of_el, sizeof(int32_t) * p->n_el));
-        HIP_CHECK(hipMemcpy(pm->d_pos_of_el, inv.data(), sizeof(int32_t) * p->n_el, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMalloc((void **)&p->d_pos_of_el, sizeof(int32_t) * p->n_el));
+        HIP_CHECK(hipMemcpy(p->d_pos_of_el, inv.data(), sizeof(int32_t) * p->n_el, hipMemcpyHostToDevice));
+    }
+    if (!p->d_fslot) HIP_CHECK(hipMalloc((void **)&p->d_fslot, sizeof(int32_t) * p->n_el));
+    if (n_elements > p->fweights_cap) {
+        if (p->d_fweights) HIP_CHECK(hipFree(p->d_fweights));
+        p->d_fweights = nullptr;
+        p->fweights_cap = 0;
+        HIP_CHECK(hipMalloc((void **)&p->d_fweights, sizeof(double) * 3 * (size_t)p->npe * n_elements));
+        p->fweights_cap = n_elements;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     int rc;
-    if ((rc = fill<double>(pm->d_fweights, 3 * (int64_t)p->npe * p->n_el, 0.0, s))) return rc;
+    if ((rc = fill<int32_t>(p->d_fslot, p->n_el, -1, s))) return rc;
     const unsigned lb = (unsigned)((n_elements + 255) / 256);
     if (p->npe == 8)
-        hipLaunchKernelGGL(k_face_pressure<8>, dim3(lb), dim3(256), 0, s, p->d_edof, pm->d_pos_of_el, X_dev, elements_dev,
-                           n_elements, face, pressure, pm->d_fweights, p->n_el);
+        hipLaunchKernelGGL(k_face_pressure<8>, dim3(lb), dim3(256), 0, s, p->d_edof, p->d_pos_of_el, X_dev,
+                           elements_dev, n_elements, face, pressure, p->d_fweights, p->d_fslot, p->n_el);
     else
-        hipLaunchKernelGGL(k_face_pressure<20>, dim3(lb), dim3(256), 0, s, p->d_edof, pm->d_pos_of_el, X_dev,
-                           elements_dev, n_elements, face, pressure, pm->d_fweights, p->n_el);
+        hipLaunchKernelGGL(k_face_pressure<20>, dim3(lb), dim3(256), 0, s, p->d_edof, p->d_pos_of_el, X_dev,
+                           elements_dev, n_elements, face, pressure, p->d_fweights, p->d_fslot, p->n_el);
     HIP_CHECK(hipGetLastError());
     const unsigned nb = (unsigned)((p->n_rownodes + 255) / 256);
     hipLaunchKernelGGL(k_node_vec_sum, dim3(nb), dim3(256), 0, s, p->d_rownode_dof, p->d_ninc_ptr, p->d_ninc,
-                       pm->d_fweights, p->npe, PExt_dev, p->n_rownodes);
+                       p->d_fweights, p->d_fslot, p->npe, PExt_dev, p->n_rownodes);
     HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -186,8 +186,8 @@ int ewfe_pcg_mg(ewfe_mg *mg, const void *csr_vals_dev, int vals_dtype, const dou
  * edelweissfe/solvers/nonlinearimplicitstatic.py:903-945): PExt += sum over the elements with
  * el_mask_dev[element] != 0 (all elements when NULL; mask indexed by element) of
  * P_e,a = sum_q N_a(xi_q) b detJ_q w_q, b = force_host[0..2], reference configuration, summed per node in
- * ascending element order. */
-int ewfe_body_force(const ewfe_plan *plan, const double *X_dev, const uint8_t *el_mask_dev, const double *force_host,
+ * ascending element order. Allocates its scratch in the plan on first use (8 npe bytes per element). */
+int ewfe_body_force(ewfe_plan *plan, const double *X_dev, const uint8_t *el_mask_dev, const double *force_host,
                     double *PExt_dev, void *stream);
 
 /* Quadrature-point results at displacement U_dev and state state_dev (plan order, as ewfe_eval_assemble;
@@ -204,8 +204,9 @@ int ewfe_qp_results(const ewfe_plan *plan, int mat_kind, const double *mat_param
  * stepaction edelweissfe/stepactions/distributedload.py; the element side is the Marmot element's
  * computeDistributedLoad, element.pxd): PExt += -p * integral over face `face` (Abaqus numbering 1..6) of
  * N_a n dA, reference configuration, for the elements elements_dev[0..n) (element indices; an element listed
- * twice is loaded once, indices outside [0, n_el) are skipped); summed per node in ascending element order. */
-int ewfe_surface_pressure(const ewfe_plan *plan, const double *X_dev, const int32_t *elements_dev, int64_t n_elements,
+ * twice is loaded once, indices outside [0, n_el) are skipped); summed per node in ascending element order.
+ * Allocates its scratch in the plan on first use (8 bytes per element, 24 npe bytes per listed element). */
+int ewfe_surface_pressure(ewfe_plan *plan, const double *X_dev, const int32_t *elements_dev, int64_t n_elements,
                           int face, double pressure, double *PExt_dev, void *stream);
 
 #ifdef __cplusplus
