@@ -1402,11 +1402,14 @@ __global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
     }
     __syncthreads();
     if (lane >= 3 * NN) return;
-    // lane = (a, i): row i of node a
+    // lane = (a, i): row i of node a, for the blocks K_ab with b = a + j (mod 20), j < nb: every unordered node
+    // pair once (j = 10 only from a < 10); the lane also writes the transposed entries K_ba = K_ab^T
     const int a = lane / 3, i = lane % 3;
-    double K[NN][3];
-    for (int b = 0; b < NN; ++b)
-        for (int k = 0; k < 3; ++k) K[b][k] = 0.0;
+    constexpr int kJ = NN / 2 + 1;  // 11
+    const int nb = (a < NN / 2) ? kJ : kJ - 1;
+    double K[kJ][3];
+    for (int j = 0; j < kJ; ++j)
+        for (int k = 0; k < 3; ++k) K[j][k] = 0.0;
     double fi = 0.0;
     auto ld = [&](int f, int q) { return s_qp[f * NQ + q]; };
     for (int q = 0; q < NQ; ++q) {
@@ -1435,24 +1438,28 @@ __global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
         const double Aai = al * pa[i];
         double Ba[3];
         for (int m = 0; m < 3; ++m) Ba[m] = be * pa[m];
-        for (int b = 0; b < NN; ++b) {
-            double pb[3], rb[3];
-            for (int m = 0; m < 3; ++m) {
-                rb[m] = ld(L::G + 3 * b + m, q);
-                pb[m] = ld(L::P + 3 * b + m, q);
-            }
-            const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
-            double vb[3] = {0, 0, 0};
-            if (MAT == M_J2) {
-                vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
-                vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
-                vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
-            }
-            for (int k = 0; k < 3; ++k) {
-                double v = Aai * pb[k] + Ba[k] * pb[i];
-                if (MAT == M_J2) v -= vai * vb[k];
-                if (i == k) v += d;
-                K[b][k] += v;
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) {
+            if (j < nb) {
+                const int b = (a + j) % NN;
+                double pb[3], rb[3];
+                for (int m = 0; m < 3; ++m) {
+                    rb[m] = ld(L::G + 3 * b + m, q);
+                    pb[m] = ld(L::P + 3 * b + m, q);
+                }
+                const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                double vb[3] = {0, 0, 0};
+                if (MAT == M_J2) {
+                    vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
+                    vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
+                    vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
+                }
+                for (int k = 0; k < 3; ++k) {
+                    double v = Aai * pb[k] + Ba[k] * pb[i];
+                    if (MAT == M_J2) v -= vai * vb[k];
+                    if (i == k) v += d;
+                    K[j][k] += v;
+                }
             }
         }
     }
@@ -1461,9 +1468,17 @@ __global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
     A.F[row] += fabs(fi);
     VT *vals = reinterpret_cast<VT *>(A.vals);
     const int64_t rs = A.indptr[row];
-    for (int b = 0; b < NN; ++b) {
-        const int32_t cab = A.colpos[(int64_t)(a * NN + b) * n + pos];
-        for (int k = 0; k < 3; ++k) add_to(vals, rs + cab + k, K[b][k]);
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+        if (j < nb) {
+            const int b = (a + j) % NN;
+            const int32_t cab = A.colpos[(int64_t)(a * NN + b) * n + pos];
+            for (int k = 0; k < 3; ++k) add_to(vals, rs + cab + k, K[j][k]);
+            if (j > 0) {  // K_ba[k][i] = K_ab[i][k] into row k of node b, column i of node a
+                const int32_t cba = A.colpos[(int64_t)(b * NN + a) * n + pos];
+                for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)A.indptr[s_dof[b] + k] + cba + i, K[j][k]);
+            }
+        }
     }
 }
 
