@@ -31,7 +31,7 @@ import shlex
 
 import numpy as np
 
-from .model import ElementBlock, FEModel, Material, boxGenArrays
+from .model import BOXGEN_SURFACES, ElementBlock, FEModel, Material, boxGenArrays, boxGenElementSets
 
 # keyword -> option names it accepts; the output keywords are read and not acted on
 _KEYWORDS = {
@@ -138,7 +138,7 @@ class _Deck:
         self.nodeLabels, self.coords = [], []
         self.nodeIndex = {}
         self.elType, self.elLabels, self.elNodes = None, [], []
-        self.nodeSets, self.elementSets = {}, {}
+        self.nodeSets, self.elementSets, self.surfaces = {}, {}, {}
 
     def addNodes(self, labels, coords):
         for lab, x in zip(labels, coords):
@@ -178,7 +178,10 @@ def _boxgen(deck: _Deck, gen: dict):
     deck.addElements(elType, range(e0 + 1, e0 + 1 + len(conn)), [[labels[i] for i in row] for row in conn])
     for k, idx in sets.items():
         deck.nodeSets[k] = [labels[i] for i in idx]
-    deck.elementSets[f"{name}_all"] = [deck.elLabels[first + i] for i in range(len(conn))]
+    for k, idx in boxGenElementSets(int(g("nX", 1)), int(g("nY", 1)), int(g("nZ", 1)), name).items():
+        deck.elementSets[k] = [deck.elLabels[first + i] for i in idx]
+    for suffix, face in BOXGEN_SURFACES.items():
+        deck.surfaces[f"{name}_{suffix}"] = {face: f"{name}_{suffix}"}
 
 
 def createModelFromInputFile(inputFile: dict) -> FEModel:
@@ -260,7 +263,9 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
         raise ValueError("every element needs a section with a material")
     block.setMaterial(chosen)
 
-    # element-based surfaces: data lines 'elSet, face' with face 1..6 or S1..S6 (abqmodelconstructor.py:213-225)
+    # element-based surfaces: boxGen's, then data lines 'elSet, face' with face 1..6 or S1..S6
+    # (abqmodelconstructor.py:213-225)
+    model.surfaces.update(deck.surfaces)
     for sf in inputFile["*surface"]:
         if sf.get("type", "element").lower() != "element":
             raise NotImplementedError(f"surface type {sf.get('type')} is not part of the GPU path")

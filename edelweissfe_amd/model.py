@@ -207,6 +207,30 @@ def boxGenArrays(nX, nY, nZ, lX=1.0, lY=1.0, lZ=1.0, x0=0.0, y0=0.0, z0=0.0, elT
     return coords, conn, sets, grid
 
 
+# boxGen's element-based surfaces: set name suffix -> face number (boxgen.py:438-446); with boxGen's element node
+# order (parametric axes xi, eta, zeta along z, x, y) face 1 (zeta = -1) is y = y0, face 4 (xi = +1) is z = z0 + lZ
+BOXGEN_SURFACES = {"bottom": 1, "top": 2, "left": 3, "front": 4, "right": 5, "back": 6}
+
+
+def boxGenElementSets(nX, nY, nZ, name="gen"):
+    """boxGen's element sets (boxgen.py:376-416) as element indices: element (ix, iy, iz) has index
+    (ix nY + iy) nZ + iz."""
+    g = np.arange(nX * nY * nZ).reshape(nX, nY, nZ)
+    sets = {f"{name}_all": g.ravel(), f"{name}_bottom": g[:, 0, :].ravel(), f"{name}_top": g[:, -1, :].ravel(),
+            f"{name}_right": g[-1, :, :].ravel(), f"{name}_left": g[0, :, :].ravel(),
+            f"{name}_front": g[:, :, -1].ravel(), f"{name}_back": g[:, :, 0].ravel(),
+            f"{name}_centralFrontToBack": g[int(nX / 2), int(nY / 2), 0:nZ].ravel(),
+            f"{name}_centerSliceX": g[int(nX / 2), :, :].ravel(), f"{name}_centerSliceY": g[:, int(nY / 2), :].ravel(),
+            f"{name}_centerSliceZ": g[:, :, int(nZ / 2)].ravel()}
+    nBand = min(nX, nY)
+    if nBand > 3:
+        band = np.concatenate([g[int(nX / 2 + i - nBand / 2), int(nY / 2 + i - nBand / 2), 0:nZ].ravel()
+                               for i in range(nBand)])
+        sets[f"{name}_shearBandFrontToBack"] = band
+        sets[f"{name}_shearBandCenterFrontToBack"] = band[(int(nBand / 2) - 1) * nZ: (int(nBand / 2) + 2) * nZ]
+    return sets
+
+
 def jitterNodes(coords, grid, gridShape, h, amplitude, seed):
     """Moves every node by U(-amplitude*h, +amplitude*h) per component, except along the normal of a
     boundary face it lies on, so loaded and fixed faces stay planar. Seeded numpy.random.default_rng."""
@@ -232,6 +256,11 @@ def generateModelData(generatorDefinition: dict, model: FEModel, journal=None) -
     model.addNodes(coords)
     block = ElementBlock(o["elType"], conn + first)
     model.addElementBlock(block, f"{name}_all")
+    blockIndex = len(model.elementBlocks) - 1
+    for k, v in boxGenElementSets(int(o.get("nX", 1)), int(o.get("nY", 1)), int(o.get("nZ", 1)), name).items():
+        model.elementSets[k] = (blockIndex, v)
+    for suffix, face in BOXGEN_SURFACES.items():
+        model.surfaces[f"{name}_{suffix}"] = {face: f"{name}_{suffix}"}
     for k, v in sets.items():
         model.nodeSets[k] = v + first
     model.nodeSets.setdefault("all", np.arange(model.nNodes))

@@ -136,3 +136,56 @@ def test_element_result_collector_on_tension_bar():
     assert np.abs(sig[:, :, 2] - 4.0).max() < 1e-3
     one = ElementResultCollector(model.solver, model, "all", slice(0, 1), "strain").getCurrentResults()
     assert one.shape == (len(block), 6)
+
+
+@pytest.mark.parametrize("elType", ["C3D8", "C3D20"])
+def test_boxgen_surfaces_carry_pressure_on_their_faces(tmp_path, elType):
+    """boxGen's element sets and surfaces (boxgen.py:376-446) read from a deck: a pressure p on each of the six
+    surfaces of a 3 x 4 x 2 box (5 x 6 x 2.5) sums to -p A n (outward normal n of that face of the box) and loads
+    only the nodes on that face; the surface's element set is the layer of elements along it."""
+    deck = tmp_path / "box.inp"
+    deck.write_text(f"""*material, name=LinearElastic, id=mat
+30000.0, 0.15
+*job, name=job, domain=3d
+*solver, solver=NIST, name=theSolver
+*modelGenerator, generator=boxGen, name=gen
+nX=3
+nY=4
+nZ=2
+lX=5
+lY=6
+lZ=2.5
+elType={elType}
+*section, name=section1, material=mat, type=solid
+all
+*step, solver=theSolver, maxInc=1, minInc=1, maxNumInc=1, maxIter=10
+dirichlet, name=fix, nSet=gen_back, field=displacement, 1=0, 2=0, 3=0
+distributedload, name=p, surface=gen_top, type=pressure, magnitude=2.0
+""")
+    inp = IF.parseInputFile(str(deck))
+    model = IF.createModelFromInputFile(inp)
+    b = model.elementBlocks[0]
+    X = model.nodeCoordinates
+    base, nDof, edofs = O.dof_numbering(len(X), b.conn)
+    L = np.array([5.0, 6.0, 2.5])
+    normals = {"left": (0, -1), "right": (0, 1), "bottom": (1, -1), "top": (1, 1), "back": (2, -1), "front": (2, 1)}
+    centroids = X[b.conn].mean(axis=1)
+    for side, (axis, sign) in normals.items():
+        face = list(model.surfaces[f"gen_{side}"].keys())[0]
+        els = model.elementSets[model.surfaces[f"gen_{side}"][face]][1]
+        h = L[axis] / (3, 4, 2)[axis]
+        layer = np.nonzero((centroids[:, axis] > L[axis] - h) if sign > 0 else (centroids[:, axis] < h))[0]
+        assert np.array_equal(np.sort(els), layer)
+        P = O.surface_pressure(elType, b.conn, X, edofs, nDof, face, 2.0, els).reshape(-1, 3)
+        area = np.prod(np.delete(L, axis))
+        want = np.zeros(3)
+        want[axis] = -2.0 * area * sign
+        assert np.allclose(P.sum(axis=0), want, rtol=1e-12, atol=1e-12), side
+        on_face = np.isclose(X[:, axis], L[axis] if sign > 0 else 0.0)
+        rows_off_face = np.asarray(base[~on_face] // 3, dtype=np.int64)  # P rows of the nodes off the face
+        assert np.abs(P[rows_off_face]).max() == 0.0, side
+    (step, _), = list(IF.createStepsFromInputFile(model, inp))
+    act = step.actions["distributedload"]["p"]
+    (face, (blockIndex, els)), = act.surface.items()
+    assert face == 2 and blockIndex == 0 and np.array_equal(els, model.elementSets["gen_top"][1])
+    assert act.loadType == "pressure"
