@@ -8,7 +8,8 @@ The subset of the reference's input language that the GPU path can run, with the
   * ``createModelFromInputFile`` (edelweissfe/helpers/inputfilehelpers.py:169-211 and
     edelweissfe/generators/abqmodelconstructor.py:67-265): boxGen generators first, then ``*node`` (label order
     of definition), ``*element``, ``*nSet`` / ``*elSet`` (labels, ``generate=True``, or names of other sets), the
-    implicit sets ``all``, materials, sections and element-based ``*surface`` definitions;
+    implicit sets ``all``, materials, sections (with ``materialParameterFromField`` over scalarExpression
+    ``*analyticalField`` definitions for the material's first property) and element-based ``*surface`` definitions;
   * ``createStepsFromInputFile`` (inputfilehelpers.py:214-244, edelweissfe/steps/stepmanager.py:133-200): step
     actions live across steps, keyed by module and name; a redefinition in a later step updates the action;
   * ``finiteElementSimulation`` (edelweissfe/drivers/inputfiledrivensimulation.py:56-217) without output
@@ -18,7 +19,8 @@ The subset of the reference's input language that the GPU path can run, with the
     against ``U.ref``, 1e-6 absolute.
 
 Decks outside the subset (other element types or providers' features, several element blocks or materials,
-constraints, distributed loads other than pressure, analytical fields, ...) raise NotImplementedError.
+constraints, distributed loads other than pressure, analytical fields other than scalarExpression, ...) raise
+NotImplementedError.
 """
 
 from __future__ import annotations
@@ -247,21 +249,36 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
         if key not in _MATERIALS:
             raise NotImplementedError(f"material {m['name']} is not available on the GPU path")
         materials[m.get("id", m["name"])] = Material(_MATERIALS[key], _numbers(m["data"]))
+    fields = _analyticalFields(inputFile)
     assigned = np.zeros(len(block), dtype=bool)
     chosen = None
+    elParams = None
     for sec in inputFile["*section"]:
         if sec.get("type", "solid").lower() != "solid":
             raise NotImplementedError(f"section type {sec.get('type')} is not part of the GPU path")
         mat = materials[sec["material"]]
-        for setName in (t for ln in sec["data"] for t in splitLineAtCommas(ln) if t):
+        defs, setNames = _sectionLines(sec, fields)
+        for setName in setNames:
             _, idx = model.elementSets[setName]
             if chosen is not None and mat is not chosen:
                 raise NotImplementedError("one material per model on the GPU path (one element block)")
             chosen = mat
             assigned[idx] = True
+            if defs:  # sectionbase.py:86-128: the parameter of each element from the fields at its centre
+                if elParams is None:
+                    elParams = np.full(len(block), float(mat.materialProperties[0]))
+                centres = model.nodeCoordinates[block.conn[idx]].mean(axis=1)  # getCoordinatesAtCenter
+                for e, c in zip(idx, centres):
+                    value = float(mat.materialProperties[0])
+                    for typ, field, expr in defs:
+                        f = field(*c)
+                        value = expr(value, f) if typ == "settovalue" else value * expr(value, f)
+                    elParams[e] = value
     if chosen is None or not assigned.all():
         raise ValueError("every element needs a section with a material")
     block.setMaterial(chosen)
+    if elParams is not None:
+        block.elParams = np.ascontiguousarray(elParams)
 
     # element-based surfaces: boxGen's, then data lines 'elSet, face' with face 1..6 or S1..S6
     # (abqmodelconstructor.py:213-225)
@@ -279,38 +296,95 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
     return model
 
 
-class _Amplitude:
-    """f(t) of a step action: an arithmetic expression in t (numbers, + - * / **, sqrt, exp, sin, cos, ...),
-    evaluated without executing anything from the deck (the reference lambdifies it with sympy)."""
+def _analyticalFields(inputFile):
+    """*analyticalField definitions (edelweissfe/analyticalfields/scalarexpression.py): name -> f(x, y, z).
+    The GPU path reads the scalarExpression type; randomScalar needs gstools, which it does not carry."""
+    fields = {}
+    for af in inputFile["*analyticalfield"]:
+        typ = af.get("type", "").lower()
+        if typ != "scalarexpression":
+            raise NotImplementedError(f"analytical field type {af.get('type')} is not part of the GPU path")
+        opts = _assignments([t for ln in af["data"] for t in splitLineAtCommas(ln) if t])
+        fields[af["name"]] = _Expression(opts["f(x,y,z)"], ("x", "y", "z"))
+    return fields
+
+
+def _sectionLines(sec, fields):
+    """A *section's data lines (sectionbase.py:31-75): materialParameterFromField definitions and the element
+    set names. Returns ([(type, field, f(p, f))], set names)."""
+    defs, setNames = [], []
+    for ln in sec["data"]:
+        parts = [t for t in splitLineAtCommas(ln) if t]
+        if not parts:
+            continue
+        key = parts[0].lower()
+        if key == "materialparameterfromfield":
+            d = _assignments(parts[1:])
+            typ = d.get("type", "").lower()
+            if typ not in ("settovalue", "scale"):
+                raise KeyError(f"{sec.get('name')}: materialParameterFromField type must be 'setToValue' or 'scale'")
+            if int(d["index"]) != 0:
+                raise NotImplementedError("per-element parameters cover the material's first property on the GPU path")
+            defs.append((typ, fields[d["field"]], _Expression(d.get("f(p,f)", "f"), ("p", "f"))))
+        elif key == "writematerialpropertiestofile":
+            raise NotImplementedError("writeMaterialPropertiesToFile (an output) is not part of the GPU path")
+        else:
+            setNames += parts
+    return defs, setNames
+
+
+class _Expression:
+    """An arithmetic expression in the given variables (numbers, + - * / **, pi, sqrt, exp, sin, cos, tan, log,
+    abs, min, max, also as np.<name> or math.<name>), evaluated without executing anything from the deck (the
+    reference evaluates such strings with Python's eval or lambdifies them with sympy)."""
 
     _ops = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul, ast.Div: operator.truediv,
             ast.Pow: operator.pow, ast.USub: operator.neg, ast.UAdd: operator.pos}
     _funcs = {"sqrt": math.sqrt, "exp": math.exp, "sin": math.sin, "cos": math.cos, "tan": math.tan,
               "log": math.log, "abs": abs, "min": min, "max": max}
+    _modules = ("np", "numpy", "math")
 
-    def __init__(self, expr: str):
+    def __init__(self, expr: str, names=("t",)):
         self.expr = expr
+        self.names = tuple(names)
         self.tree = ast.parse(expr, mode="eval").body
-        self(0.5)  # reject unsupported expressions early
+        self(*([0.5] * len(self.names)))  # reject unsupported expressions early
 
-    def _eval(self, node, t):
+    def _func(self, f):
+        if isinstance(f, ast.Name) and f.id in self._funcs:
+            return self._funcs[f.id]
+        if (isinstance(f, ast.Attribute) and isinstance(f.value, ast.Name) and f.value.id in self._modules
+                and f.attr in self._funcs):
+            return self._funcs[f.attr]
+        return None
+
+    def _eval(self, node, env):
         if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)):
             return node.value
-        if isinstance(node, ast.Name) and node.id == "t":
-            return t
+        if isinstance(node, ast.Name) and node.id in env:
+            return env[node.id]
         if isinstance(node, ast.Name) and node.id == "pi":
             return math.pi
+        if (isinstance(node, ast.Attribute) and isinstance(node.value, ast.Name) and node.value.id in self._modules
+                and node.attr == "pi"):
+            return math.pi
         if isinstance(node, ast.BinOp) and type(node.op) in self._ops:
-            return self._ops[type(node.op)](self._eval(node.left, t), self._eval(node.right, t))
+            return self._ops[type(node.op)](self._eval(node.left, env), self._eval(node.right, env))
         if isinstance(node, ast.UnaryOp) and type(node.op) in self._ops:
-            return self._ops[type(node.op)](self._eval(node.operand, t))
-        if (isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in self._funcs
-                and not node.keywords):
-            return self._funcs[node.func.id](*(self._eval(a, t) for a in node.args))
-        raise ValueError(f"unsupported amplitude expression {self.expr!r}")
+            return self._ops[type(node.op)](self._eval(node.operand, env))
+        if isinstance(node, ast.Call) and not node.keywords and self._func(node.func) is not None:
+            return self._func(node.func)(*(self._eval(a, env) for a in node.args))
+        raise ValueError(f"unsupported expression {self.expr!r}")
 
-    def __call__(self, t):
-        return float(self._eval(self.tree, float(t)))
+    def __call__(self, *values):
+        return float(self._eval(self.tree, {k: float(v) for k, v in zip(self.names, values)}))
+
+
+class _Amplitude(_Expression):
+    """f(t) of a step action (an expression in the step progress t)."""
+
+    def __init__(self, expr: str):
+        super().__init__(expr, ("t",))
 
 
 def _vector(s):

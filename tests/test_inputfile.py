@@ -189,3 +189,81 @@ distributedload, name=p, surface=gen_top, type=pressure, magnitude=2.0
     (face, (blockIndex, els)), = act.surface.items()
     assert face == 2 and blockIndex == 0 and np.array_equal(els, model.elementSets["gen_top"][1])
     assert act.loadType == "pressure"
+
+
+_FIELD_DECK = """*material, name=LinearElastic, id=mat
+30000.0, 0.15
+*job, name=job, domain=3d
+*solver, solver=NISTParallel, name=theSolver
+*modelGenerator, generator=boxGen, name=gen
+nX=10
+nY=10
+nZ=1
+lX=50
+lY=50
+lZ=10
+elType=C3D8
+*section, name=section1, material=mat, type=solid
+materialParameterFromField, index=0, field=F1, type=settovalue
+materialParameterFromField, index=0, field=F2, type=scale, "f(p,f)=1 + f + 0*p"
+all
+*analyticalField, name=F1, type=scalarExpression
+"f(x,y,z)" = "30000*(1 + 0.5*x/50)"
+*analyticalField, name=F2, type=scalarExpression
+"f(x,y,z)" = "0.1*np.sin(np.pi*y/100)"
+*step, solver=theSolver, maxInc=1, minInc=1, maxNumInc=10, maxIter=25
+dirichlet, name=x0u, nSet=gen_left, field=displacement, 1=0.
+dirichlet, name=y0u, nSet=gen_bottom, field=displacement, 2=0.
+dirichlet, name=z0u, nSet=gen_back, field=displacement, 3=0.
+*step, solver=theSolver, maxInc=1, minInc=1e-2, maxNumInc=100, maxIter=25
+distributedload, name=p, surface=gen_top, type=pressure, magnitude=100
+"""
+
+
+def _field_deck(tmp_path):
+    deck = tmp_path / "fields.inp"
+    deck.write_text(_FIELD_DECK)
+    return IF.parseInputFile(str(deck))
+
+
+def test_material_parameter_from_scalar_fields(tmp_path):
+    """sectionbase.py:86-128: per-element Young's modulus from two scalarExpression fields at the element
+    centres (setToValue, then scale by f(p, f)); on the regular 10 x 10 x 1 grid the centre of element
+    (ix, iy, iz) is ((ix + 1/2) 5, (iy + 1/2) 5, 5). The deck then runs through the oracle with those
+    parameters (two steps, pressure on boxGen's gen_top)."""
+    inp = _field_deck(tmp_path)
+    model = IF.createModelFromInputFile(inp)
+    b = model.elementBlocks[0]
+    ix, iy = np.meshgrid(np.arange(10), np.arange(10), indexing="ij")
+    xc, yc = ((ix + 0.5) * 5).ravel(), ((iy + 0.5) * 5).ravel()
+    E = 30000 * (1 + 0.5 * xc / 50) * (1 + 0.1 * np.sin(np.pi * yc / 100))
+    assert np.allclose(b.elParams, E, rtol=1e-14)
+    bad = IF.parseInputFile(str(tmp_path / "fields.inp"))
+    bad["*section"][0]["data"][0] = bad["*section"][0]["data"][0].replace("index=0", "index=1")
+    with pytest.raises(NotImplementedError):
+        IF.createModelFromInputFile(bad)
+    U = _oracle_run(model, inp)
+    assert np.isfinite(U).all() and np.abs(U).max() > 0
+
+
+def _oracle_run(model, inp):
+    b = model.elementBlocks[0]
+    om = dict(el_kind=b.elType, mat_kind=b.material.name, mp=b.material.materialProperties,
+              X=model.nodeCoordinates, conn=b.conn, el_mp=b.elParams)
+    U = state = None
+    for step, _ in _steps(model, inp):
+        r = O.solve_step(om, _oracle_step(model, step), U0=U, state_n=state)
+        U, state = r["U"], r["state"]
+    return U
+
+
+@pytest.mark.gpu
+def test_material_parameter_fields_deck_on_gpu(tmp_path):
+    """The field deck above through NISTGPU against the oracle's Newton with the same per-element parameters."""
+    inp = _field_deck(tmp_path)
+    model = IF.finiteElementSimulation(inp, solverOptions={"pcgTolerance": 1e-13})
+    Ug = model.U.cpu().numpy()
+    Uo = _oracle_run(IF.createModelFromInputFile(inp), inp)
+    err = np.abs(Ug - Uo).max() / np.abs(Uo).max()
+    print(f"field deck: GPU vs oracle {err:.2e}", flush=True)
+    assert err < 1e-9
