@@ -250,6 +250,7 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
             raise NotImplementedError(f"material {m['name']} is not available on the GPU path")
         materials[m.get("id", m["name"])] = Material(_MATERIALS[key], _numbers(m["data"]))
     fields = _analyticalFields(inputFile)
+    model.analyticalFields = fields
     assigned = np.zeros(len(block), dtype=bool)
     chosen = None
     elParams = None
@@ -391,16 +392,47 @@ def _vector(s):
     return [float(t) for t in splitLineAtCommas(s) if t]
 
 
+def _componentList(s, unset):
+    """'[x, 0.5, 0]' -> [unset, 0.5, 0.0] ('x' marks a component that is not set)."""
+    out = []
+    for t in s.strip().strip("[]()").split(","):
+        t = t.strip()
+        if t:
+            out.append(unset if t.lower() == "x" else float(t))
+    return out
+
+
+def _dirichletValues(kw, model):
+    """dirichlet.py:80-103: prescribed values from '1=..', '2=..', '3=..' or 'components=[..]' (x: free), scaled
+    node by node by the analytical field at the node when 'analyticalField' is given."""
+    comps = {}
+    if "components" in kw:
+        comps.update({i: v for i, v in enumerate(_componentList(kw["components"], None)) if v is not None})
+    comps.update({int(k) - 1: float(v) for k, v in kw.items() if _is_int(k)})
+    scale = None
+    if "analyticalField" in kw:
+        f = model.analyticalFields[kw["analyticalField"]]
+        scale = [f(*model.nodeCoordinates[n]) for n in np.asarray(model.nodeSets[kw["nSet"]])]
+    return comps, scale
+
+
+def _nodeLoad(kw):
+    """nodeforces.py:85-108: 'components=[..]' (x: 0) or '1=..', '2=..', '3=..'."""
+    if "components" in kw:
+        return (_componentList(kw["components"], 0.0) + [0.0, 0.0, 0.0])[:3]
+    return [float(kw.get(str(i + 1), 0.0)) for i in range(3)]
+
+
 def _makeAction(module, name, kw, model):
     from .stepping import BodyForce, Dirichlet, DistributedLoad, NodeForces, Options
 
     amp = _Amplitude(kw["f(t)"]) if "f(t)" in kw else None
     field = kw.get("field", "displacement")
     if module == "dirichlet":
-        comps = {int(k) - 1: float(v) for k, v in kw.items() if _is_int(k)}
-        return Dirichlet(name, model, kw["nSet"], comps, field=field, amplitude=amp)
+        comps, scale = _dirichletValues(kw, model)
+        return Dirichlet(name, model, kw["nSet"], comps, field=field, amplitude=amp, scale=scale)
     if module == "nodeforces":
-        return NodeForces(name, model, kw["nSet"], _vector(kw["components"]), field=field, amplitude=amp)
+        return NodeForces(name, model, kw["nSet"], _nodeLoad(kw), field=field, amplitude=amp)
     if module == "bodyforce":
         return BodyForce(name, model, kw["elSet"], _vector(kw["forceVector"]), amplitude=amp)
     if module == "distributedload":
@@ -443,6 +475,12 @@ def createStepsFromInputFile(model: FEModel, inputFile: dict):
                                          amplitude=_Amplitude(kw["f(t)"]) if "f(t)" in kw else None)
                 elif module == "options":
                     act.options.update({k: v for k, v in kw.items() if k != "category"})
+                elif module == "dirichlet":
+                    comps, scale = _dirichletValues(kw, model)
+                    act.updateStepAction(comps, amplitude=_Amplitude(kw["f(t)"]) if "f(t)" in kw else None,
+                                         scale=scale)
+                elif module == "nodeforces":
+                    act.updateStepAction(_nodeLoad(kw), amplitude=_Amplitude(kw["f(t)"]) if "f(t)" in kw else None)
                 else:
                     raise NotImplementedError(f"updating a {module} action in a later step is not part of the GPU path")
             else:

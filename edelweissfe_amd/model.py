@@ -123,6 +123,7 @@ class FEModel:
         self.time = 0.0
         self.constraints = {}
         self.scalarVariables = {}
+        self.analyticalFields = {}  # name -> f(x, y, z)
 
     def addNodes(self, coords, labels=None):
         coords = np.asarray(coords, dtype=np.float64).reshape(-1, 3)

@@ -168,18 +168,25 @@ def _linear(x):
 
 class Dirichlet:
     """``dirichlet, nSet=.., field=displacement, 1=.., 2=.., 3=..`` (dirichlet.py:53-158). ``components``
-    maps 0-based component -> prescribed value at the end of the step; ``amplitude`` is f(stepProgress)."""
+    maps 0-based component -> prescribed increment over the step; ``amplitude`` is f(stepProgress); ``scale``
+    (one value per node of the set, the ``analyticalField`` option) multiplies the values node by node. A later
+    step redefines it with ``updateStepAction``: the new values are the next step's increments."""
 
-    def __init__(self, name, model, nSet: str, components: dict, field="displacement", amplitude=None):
+    def __init__(self, name, model, nSet: str, components: dict, field="displacement", amplitude=None, scale=None):
         if field != "displacement":
             raise NotImplementedError("the GPU path carries the displacement field only")
         self.name = name
         self.field = field
         self.nSetName = nSet
         self.nSet = np.asarray(model.nodeSets[nSet], dtype=np.int64)
+        self.updateStepAction(components, amplitude, scale)
+
+    def updateStepAction(self, components: dict, amplitude=None, scale=None):
         comps = sorted(int(c) for c in components)
         self._components = comps
         self.delta = np.tile([float(components[c]) for c in comps], (len(self.nSet), 1))
+        if scale is not None:
+            self.delta = self.delta * np.asarray(scale, dtype=np.float64).reshape(-1, 1)
         self.amplitude = amplitude or _linear
         self.active = True
 
@@ -230,6 +237,12 @@ class NodeForces:
             self.nodeForcesStepStart += self.nodeForcesDelta * self.amplitude(1.0)
             self.nodeForcesDelta[:] = 0
             self._idle = True
+
+    def updateStepAction(self, vector, amplitude=None):
+        """nodeforces.py:85-108: the new vector is added over the next step, on top of the loads so far."""
+        self.nodeForcesDelta = np.tile(np.asarray(vector, dtype=np.float64), (len(self.nodeSet), 1))
+        self.amplitude = amplitude or _linear
+        self._idle = False
 
     def getCurrentLoad(self, timeStep: TimeStep):
         if self._idle:

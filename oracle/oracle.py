@@ -370,7 +370,8 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
     for nodes, comps in step.get("dirichlet", []):
         cs = sorted(comps)
         idx = (base[nodes][:, None] + np.array(cs)[None, :]).ravel()
-        delta = np.tile([comps[c] for c in cs], (len(nodes), 1)).ravel()
+        # a value is one number for all nodes or one per node (dirichlet.py:96-99, analyticalField)
+        delta = np.column_stack([np.broadcast_to(np.asarray(comps[c], dtype=float), (len(nodes),)) for c in cs]).ravel()
         dirichlets.append((idx, delta))
     nodeforces = []
     for nodes, vec in step.get("nodeforces", []):

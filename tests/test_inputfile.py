@@ -33,7 +33,7 @@ def _oracle_step(model, step):
                                           ("stepLength", "length"), ("startInc", "startIncrementSize"))}
     opts = a["options"]["NISTSolver"].options if "NISTSolver" in a["options"] else {}
     st["extrapolation"] = opts.get("extrapolation", "linear")
-    st["dirichlet"] = [(d.nSet, {c: float(v) * d.active for c, v in zip(d.components, d.delta[0])})
+    st["dirichlet"] = [(d.nSet, {c: d.delta[:, j] * d.active for j, c in enumerate(d.components)})
                        for d in a["dirichlet"].values()]
     st["nodeforces"] = [(f.nodeSet, list(f.nodeForcesDelta[0])) for f in a["nodeforces"].values()]
     st["bodyforces"] = []
@@ -86,6 +86,9 @@ def test_reader_builds_the_fixture_models(name):
         st = _oracle_step(model, step)  # the last step's description, taken before its step-end actions
     ref = decks.step(fx, sets)
     assert st["extrapolation"] == ref["extrapolation"] and st["maxInc"] == ref["maxInc"]
+    for (n1, c1), (n2, c2) in zip(st["dirichlet"], ref["dirichlet"]):
+        assert np.array_equal(np.sort(n1), np.sort(n2)) and sorted(c1) == sorted(c2)
+        assert all(np.all(np.asarray(c1[k]) == c2[k]) for k in c1)
     assert len(st["dirichlet"]) == len(ref["dirichlet"]) and len(st["nodeforces"]) == len(ref["nodeforces"])
     assert st["bodyforces"] == ref["bodyforces"]
     assert len(st["pressures"]) == len(ref.get("pressures", []))
@@ -266,4 +269,65 @@ def test_material_parameter_fields_deck_on_gpu(tmp_path):
     Uo = _oracle_run(IF.createModelFromInputFile(inp), inp)
     err = np.abs(Ug - Uo).max() / np.abs(Uo).max()
     print(f"field deck: GPU vs oracle {err:.2e}", flush=True)
+    assert err < 1e-9
+
+
+_ACTIONS_DECK = """*material, name=LinearElastic, id=mat
+1000.0, 0.3
+*job, name=job, domain=3d
+*solver, solver=NIST, name=theSolver
+*modelGenerator, generator=boxGen, name=gen
+nX=2
+nY=2
+nZ=4
+lX=1
+lY=1
+lZ=4
+elType=C3D8
+*section, name=section1, material=mat, type=solid
+all
+*analyticalField, name=g, type=scalarExpression
+"f(x,y,z)" = "1 + 0.5*x"
+*step, solver=theSolver, maxInc=0.5, minInc=0.5, maxNumInc=10, maxIter=25
+dirichlet, name=fix, nSet=gen_back, field=displacement, 1=0, 2=0, 3=0
+dirichlet, name=pull, nSet=gen_front, field=displacement, 3=0.01, analyticalField=g
+*step, solver=theSolver, maxInc=0.5, minInc=0.5, maxNumInc=10, maxIter=25
+dirichlet, name=pull, nSet=gen_front, field=displacement, components='[x, x, 0.01]'
+nodeforces, name=side, nSet=gen_top, field=displacement, components='[0, 0.5, x]'
+"""
+
+
+def _actions_deck(tmp_path):
+    deck = tmp_path / "actions.inp"
+    deck.write_text(_ACTIONS_DECK)
+    return IF.parseInputFile(str(deck))
+
+
+def test_dirichlet_fields_components_and_redefinition(tmp_path):
+    """dirichlet.py:80-112 and nodeforces.py:85-108: a prescribed value scaled node by node by a scalarExpression
+    field (analyticalField), a redefinition in the next step (components=[x, x, v]: the next step's increment),
+    and nodal forces from components=[..]; the oracle's Newton over both steps ends with uz on the loaded face
+    equal to 0.01 (1 + x / 2) + 0.01."""
+    inp = _actions_deck(tmp_path)
+    model = IF.createModelFromInputFile(inp)
+    front = np.asarray(model.nodeSets["gen_front"])
+    U = _oracle_run(model, inp)
+    uz = U.reshape(-1, 3)[front, 2]
+    # to the direct solve's rounding on the prescribed rows (~3e-15)
+    assert np.allclose(uz, 0.01 * (1 + 0.5 * model.nodeCoordinates[front, 0]) + 0.01, rtol=0, atol=1e-13)
+    model2 = IF.createModelFromInputFile(inp)
+    steps = list(IF.createStepsFromInputFile(model2, inp))
+    side = steps[1][0].actions["nodeforces"]["side"]
+    assert np.array_equal(side.nodeForcesDelta[0], [0.0, 0.5, 0.0])
+
+
+@pytest.mark.gpu
+def test_dirichlet_fields_deck_on_gpu(tmp_path):
+    """The same deck through NISTGPU against the oracle's Newton."""
+    inp = _actions_deck(tmp_path)
+    model = IF.finiteElementSimulation(inp, solverOptions={"pcgTolerance": 1e-13})
+    Ug = model.U.cpu().numpy()
+    Uo = _oracle_run(IF.createModelFromInputFile(inp), inp)
+    err = np.abs(Ug - Uo).max() / np.abs(Uo).max()
+    print(f"actions deck: GPU vs oracle {err:.2e}", flush=True)
     assert err < 1e-9
