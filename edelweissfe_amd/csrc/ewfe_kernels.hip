@@ -69,6 +69,11 @@ static const double kHexXi[20][3] = {
     {0, -1, -1},  {1, 0, -1},  {0, 1, -1}, {-1, 0, -1}, {0, -1, 1},  {1, 0, 1},  {0, 1, 1}, {-1, 0, 1},
     {-1, -1, 0},  {1, -1, 0},  {1, 1, 0},  {-1, 1, 0}};
 
+// signs of eta and zeta of the 2x2x2 points, by q mod 4 (the kernels use the same signs as bit arithmetic:
+// eta = bit 1 of q, zeta = bit 0 xor bit 1)
+static const double kEta8[4] = {-1.0, -1.0, 1.0, 1.0};
+static const double kZeta8[4] = {-1.0, 1.0, 1.0, -1.0};
+
 static void host_dN(int nn, const double x[3], double dN[][3])
 {
     for (int a = 0; a < nn; ++a) {
@@ -171,15 +176,17 @@ static int upload_tables(int device)
     if (done[device]) return 0;
     double dN8[8][8][3], w8[8], dN20[27][20][3], w27[27], N8[8][8], N20[27][20];
     const double g2 = 1.0 / std::sqrt(3.0), g3 = std::sqrt(0.6);
+    // quadrature points in the reference's order (library.py:212-227, 260-275): 2x2x2: xi = -g for q < 4,
+    // +g after, (eta, zeta) = (-,-), (-,+), (+,+), (+,-) by q mod 4; 3x3x3: q = 9 i + 3 j + k over (xi, eta, zeta)
     for (int q = 0; q < 8; ++q) {
-        const double x[3] = {(q & 1) ? g2 : -g2, (q & 2) ? g2 : -g2, (q & 4) ? g2 : -g2};
+        const double x[3] = {q < 4 ? -g2 : g2, kEta8[q & 3] * g2, kZeta8[q & 3] * g2};
         host_dN(8, x, dN8[q]);
         host_N(8, x, N8[q]);
         w8[q] = 1.0;
     }
     const double pts[3] = {-g3, 0.0, g3}, wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
     for (int q = 0; q < 27; ++q) {
-        const int i = q % 3, j = (q / 3) % 3, k = q / 9;
+        const int i = q / 9, j = (q / 3) % 3, k = q % 3;
         const double x[3] = {pts[i], pts[j], pts[k]};
         host_dN(20, x, dN20[q]);
         host_N(20, x, N20[q]);
@@ -609,12 +616,12 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
         auto put_g = [&](int a, int j, double v) { s_qp[slot_node(L::G, a, j, q, le)] = v; };
         auto get_g = [&](int a, int j) { return s_qp[slot_node(L::G, a, j, q, le)]; };
         auto put_p = [&](int a, int j, double v) { s_qp[slot_node(L::P, a, j, q, le)] = v; };
-        // parametric derivatives of the trilinear shape functions at Gauss point q (point q has coordinates
-        // (+-g, +-g, +-g) by the bits of q; the same arithmetic as the host tables, so the same values)
+        // parametric derivatives of the trilinear shape functions at Gauss point q (the points in the order of
+        // the host tables; the same arithmetic, so the same values)
         double dNq[NN][3];
         {
             const double g = 1.0 / sqrt(3.0);
-            const double x0 = (q & 1) ? g : -g, x1 = (q & 2) ? g : -g, x2 = (q & 4) ? g : -g;
+            const double x0 = q < 4 ? -g : g, x1 = (q & 2) ? g : -g, x2 = ((q ^ (q >> 1)) & 1) ? g : -g;
 #pragma unroll
             for (int b = 0; b < NN; ++b) {
                 const double c0 = ((b & 3) == 1 || (b & 3) == 2) ? 1.0 : -1.0;

@@ -26,7 +26,7 @@
  * functions in Abaqus node order, which is the order boxGen emits
  * (ew/generators/boxgen.py:171-299). Gauss rules: 2x2x2 (w=1) and 3x3x3
  * (points 0, +-sqrt(0.6), weights 5/9, 8/9), ew/elements/library.py:33-47,212-275.
- * Quadrature points are ordered q = i + n*j + n*n*k over (xi_i, eta_j, zeta_k).
+ * Quadrature points are in the reference's order (see gauss_point).
  *
  * Ke is written into the VIJ slice exactly as the reference element writes it:
  * reshape(K_, (nDof, nDof)) row-major, i.e. V[off + a*n + b] = Ke[a][b]
@@ -99,19 +99,23 @@ static void shape_deriv(int nn, const double p[3], double dN[][3], double *N)
     }
 }
 
+/* Quadrature point q in the reference's order (library.py:212-227, 260-275): 2x2x2: xi = -g for q < 4, +g
+ * after, (eta, zeta) = g (-1,-1), (-1,+1), (+1,+1), (+1,-1) by q mod 4; 3x3x3: q = 9 i + 3 j + k over
+ * (xi_i, eta_j, zeta_k). */
 static void gauss_point(int nqp, int q, double p[3], double *w)
 {
     if (nqp == 8) {
         const double g = 1.0 / sqrt(3.0);
-        p[0] = (q & 1) ? g : -g;
-        p[1] = (q & 2) ? g : -g;
-        p[2] = (q & 4) ? g : -g;
+        static const int eta[4] = {-1, -1, 1, 1}, zeta[4] = {-1, 1, 1, -1};
+        p[0] = q < 4 ? -g : g;
+        p[1] = eta[q & 3] * g;
+        p[2] = zeta[q & 3] * g;
         *w = 1.0;
     } else {
         const double g = sqrt(0.6);
         const double pts[3] = {-g, 0.0, g};
         const double wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
-        const int i = q % 3, j = (q / 3) % 3, k = q / 9;
+        const int i = q / 9, j = (q / 3) % 3, k = q % 3;
         p[0] = pts[i];
         p[1] = pts[j];
         p[2] = pts[k];
