@@ -539,17 +539,6 @@ __device__ __forceinline__ int slot_node(int base, int b, int i, int q, int le)
     return (base + 3 * b + i) * 64 + 16 * (le >> 1) + ((2 * q + (le & 1)) ^ (2 * b));
 }
 
-template <typename VT>
-struct Vec16;  // 16-byte vector of VT
-template <>
-struct Vec16<double> {
-    using type = double2;
-};
-template <>
-struct Vec16<float> {
-    using type = float4;
-};
-
 template <int FORM, int MAT, typename VT>
 __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 {
@@ -804,15 +793,18 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
         __syncthreads();
         const int nv = valid - 4 * half;  // elements of this half
         // 16-byte copies: records start at multiples of 16 bytes and hold a multiple of 16 bytes
-        using V16 = typename Vec16<VT>::type;
         constexpr int VW = 16 / (int)sizeof(VT);
+        typedef VT nvec __attribute__((ext_vector_type(VW)));
         static_assert((kRec8 * (int)sizeof(VT)) % 16 == 0, "records must be whole 16-byte units");
         const int nw = (nv < 4 ? nv : 4) * (kRec8 / VW);
         for (int w = lane; w < nw; w += 64) {
             const int l = w / (kRec8 / VW);
-            const V16 val = reinterpret_cast<const V16 *>(stage)[w];
-            *reinterpret_cast<V16 *>(reinterpret_cast<VT *>(R.kbuf) + (int64_t)s_el[4 * half + l] * kRec8 +
-                                     (w - l * (kRec8 / VW)) * VW) = val;
+            // non-temporal: the records are read back by the gather only after the whole pass, so caching
+            // them would only evict the pass's inputs
+            const nvec val = reinterpret_cast<const nvec *>(stage)[w];
+            __builtin_nontemporal_store(val, reinterpret_cast<nvec *>(reinterpret_cast<VT *>(R.kbuf) +
+                                                                 (int64_t)s_el[4 * half + l] * kRec8 +
+                                                                 (w - l * (kRec8 / VW)) * VW));
         }
     }
 }
@@ -1036,7 +1028,8 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) vals[base + i * L + 3 * m + kk] = (VT)acc[3 * i + kk];
+            for (int kk = 0; kk < 3; ++kk)  // non-temporal: the CSR values are not read again in this call
+                __builtin_nontemporal_store((VT)acc[3 * i + kk], vals + base + i * L + 3 * m + kk);
     }
     if (m >= 29) {  // P and F of component m - 29, in ascending element order
         const int c = m - 29;
