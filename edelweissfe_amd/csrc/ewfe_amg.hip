@@ -487,7 +487,8 @@ __global__ __launch_bounds__(64) void k_mg_ap(FineOp<VT> F, BsrOp B, const int32
         for (int t = 0; t < 9; ++t) wval[9 * (int64_t)(w0 + u) + t] = acc[t];
     }
 }
-constexpr int kFineMaxRow = 64, kFineMaxStage = 256;      // level 0: LDS ~7 KB per workgroup
+constexpr int kFineMaxRow = 64, kFineMaxStage = 256;      // level 0, 8-node meshes: LDS ~7 KB per workgroup
+constexpr int kWideMaxRow = 128, kWideMaxStage = 512;      // level 0, 20-node meshes (up to 81 blocks): ~15 KB
 constexpr int kCoarseMaxRow = 128, kCoarseMaxStage = 512;  // levels >= 1: LDS ~51 KB
 
 // A_c = P^T W, aggregate I (one 64-lane wave): lane v owns output block v (column J) and sums P_tI^T W_iJ over
@@ -859,10 +860,16 @@ int mg_setup(ewfe_mg *mg, const void *vals, const int32_t *rows, int64_t n_rows,
                                (const double *)L.dinv, (const int32_t *)L.pptr, (const uint8_t *)L.pslot,
                                (VT *)L.pval, (const double *)(mg->lam + l));
             const int64_t chunk = (L.n + 7) / 8;
-            hipLaunchKernelGGL((k_mg_ap<true, VT, VT, kFineMaxRow, kFineMaxStage, false>),
-                               dim3((unsigned)(8 * chunk)), dim3(64), 0, s, F, B, (const int32_t *)L.pptr,
-                               (const int32_t *)L.pcol, (const VT *)L.pval, (const int32_t *)L.wptr,
-                               (const int32_t *)L.wcol, L.wval, L.n, chunk);
+            if (mg->plan->max_deg <= kFineMaxRow)
+                hipLaunchKernelGGL((k_mg_ap<true, VT, VT, kFineMaxRow, kFineMaxStage, false>),
+                                   dim3((unsigned)(8 * chunk)), dim3(64), 0, s, F, B, (const int32_t *)L.pptr,
+                                   (const int32_t *)L.pcol, (const VT *)L.pval, (const int32_t *)L.wptr,
+                                   (const int32_t *)L.wcol, L.wval, L.n, chunk);
+            else
+                hipLaunchKernelGGL((k_mg_ap<true, VT, VT, kWideMaxRow, kWideMaxStage, false>),
+                                   dim3((unsigned)(8 * chunk)), dim3(64), 0, s, F, B, (const int32_t *)L.pptr,
+                                   (const int32_t *)L.pcol, (const VT *)L.pval, (const int32_t *)L.wptr,
+                                   (const int32_t *)L.wcol, L.wval, L.n, chunk);
         } else {
             hipLaunchKernelGGL((k_mg_prolongator<false, double, VT>), dim3((unsigned)L.n), dim3(64), 0, s,
                                FineOp<double>{}, B, (const double *)L.dinv, (const int32_t *)L.pptr,
@@ -1060,7 +1067,7 @@ extern "C" int ewfe_mg_create(const ewfe_plan *p, int degree, double ratio, ewfe
                 (rc = upload(mg, L.ri, h.ri)) || (rc = upload(mg, L.rt, h.rt)))
                 return fail(rc);
             // k_mg_ap keeps a row of A_l in LDS
-            const int64_t max_row = l == 0 ? kFineMaxRow : kCoarseMaxRow;
+            const int64_t max_row = l > 0 ? kCoarseMaxRow : (p->max_deg <= kFineMaxRow ? kFineMaxRow : kWideMaxRow);
             for (int64_t i = 0; i < h.n; ++i)
                 if (h.aptr[i + 1] - h.aptr[i] > max_row) {
                     ewfe::set_error("ewfe_mg_create: level " + std::to_string(l) + " has a row with more than " +

@@ -22,7 +22,8 @@ Differences that are part of the design (not of the semantics):
     "CSR generation" phase is fused into "elements" and its timer stays 0;
   * the linear solve is device PCG on the row-eliminated matrix, preconditioned by node-block Jacobi
     (option ``linsolver="pcg"``) or by a smoothed-aggregation multigrid V-cycle (``linsolver="amg"``, the
-    device counterpart of the reference's pyamg-preconditioned "gmres" solver), converged to
+    device counterpart of the reference's pyamg-preconditioned "gmres" solver; the default ``"auto"`` takes
+    the multigrid from 20,000 dofs on when the mesh allows it, block Jacobi otherwise), converged to
     ``pcgTolerance`` relative residual; the Newton loop's own flux/correction tolerances decide convergence
     exactly as in the reference;
   * ``tangentPrecision="fp32"`` builds and stores the tangent in FP32 (residual, return mapping and update
@@ -40,6 +41,7 @@ import time as _time
 
 import numpy as np
 
+from .._lib import EwfeError
 from ..assembly import ElementAssembly
 from ..dofmanager import DofManager
 from ..stepping import (ConditionalStop, CutbackRequest, DivergingSolution, ReachedMaxIncrements,
@@ -81,7 +83,7 @@ class NISTGPU:
         "defaultCriticalIter": 5,
         "defaultMaxGrowingIter": 10,
         "extrapolation": "linear",
-        "linsolver": "pcg",
+        "linsolver": "auto",
         "linsolverConfigFile": "",
         "tangentPrecision": "fp64",
         "pcgTolerance": 1e-10,
@@ -90,6 +92,8 @@ class NISTGPU:
         "fluxResidualFloor": 1e-7,
         "device": "cuda",
     }
+
+    AUTO_AMG_MIN_DOFS = 20000  # linsolver="auto": the multigrid from this many dofs on
 
     def __init__(self, jobInfo=None, journal=None, **kwargs):
         jobInfo = jobInfo or DEFAULT_JOB_INFO
@@ -126,7 +130,7 @@ class NISTGPU:
             with open(self.options["linsolverConfigFile"]) as f:
                 cfg = json.load(f)
             self._updateOptions({k: v for k, v in cfg.items() if k in self.NISTOptions}, self.journal)
-        if self.options["linsolver"].lower() not in ("pcg", "amg"):
+        if self.options["linsolver"].lower() not in ("pcg", "amg", "auto"):
             raise AttributeError("invalid linear solver {:} requested".format(self.options["linsolver"]))
         if len(model.elementBlocks) != 1:
             raise NotImplementedError("NISTGPU assembles one element block (one element type and section) per model")
@@ -140,7 +144,18 @@ class NISTGPU:
         if id(block) not in self.assemblies or self.assemblies[id(block)].nDof != self.theDofManager.nDof:
             self.assemblies[id(block)] = ElementAssembly(block, self.theDofManager, model, device=device)
         self.assembly = self.assemblies[id(block)]
-        if self.options["linsolver"].lower() == "amg":
+        self._linsolver = self.options["linsolver"].lower()
+        if self._linsolver == "auto":
+            # the multigrid from AUTO_AMG_MIN_DOFS dofs on, where its setup pays; block Jacobi below and on meshes
+            # the multigrid does not take (dofs that are not 3 per node)
+            self._linsolver = "pcg"
+            if self.theDofManager.nDof >= self.AUTO_AMG_MIN_DOFS:
+                try:
+                    self.assembly.multigrid()
+                    self._linsolver = "amg"
+                except EwfeError as e:
+                    self.journal.message(f"multigrid not available ({e}); node-block Jacobi PCG", self.identification)
+        if self._linsolver == "amg":
             self.assembly.multigrid()  # aggregation and sparsity patterns: once per mesh
         self.computationTimes = dict.fromkeys(TIMING_KEYS + ["multigrid setup", "setup"], 0.0)
         # dof numbering, CSR pattern, element plan, multigrid patterns: once per step (mesh), not per increment
@@ -325,7 +340,7 @@ class NISTGPU:
         # an FP32 tangent (entries rounded to ~6e-8) makes Newton contract linearly, by ~1e-4..1e-3 per
         # iteration at these condition numbers: its corrections are solved only to pcgToleranceFP32
         rtol = self.options["pcgToleranceFP32"] if A.dtype == self.torch.float32 else self.options["pcgTolerance"]
-        amg = self.options["linsolver"].lower() == "amg"
+        amg = self._linsolver == "amg"
         if amg:
             self.assembly.multigridSetup(A, rows)
             self.torch.cuda.synchronize(self.assembly.device)
