@@ -152,9 +152,13 @@ class ElementAssembly:
         els = np.asarray(elements, dtype=np.int64).ravel()
         if len(els) and (els.min() < 0 or els.max() >= len(self.block) or len(np.unique(els)) != len(els)):
             raise ValueError("elements must be distinct element indices of this block")
-        cache = self.__dict__.setdefault("_faceSets", {})  # id -> (the caller's array, its device copy)
+        # device copies of the element lists of the surfaces in use: id -> (the caller's array, its device copy),
+        # at most 16 (a surface load's lists are the same objects in every iteration)
+        cache = self.__dict__.setdefault("_faceSets", {})
         hit = cache.get(id(elements))
         if hit is None or hit[0] is not elements:
+            if len(cache) >= 16:
+                cache.pop(next(iter(cache)))
             hit = cache[id(elements)] = (elements, self.torch.as_tensor(els.astype(np.int32), device=self.device))
         check(lib().ewfe_surface_pressure(self._plan, ptr(self.X), ptr(hit[1]), len(els), int(face),
                                           float(pressure), ptr(PExt), self._stream()))
