@@ -144,7 +144,8 @@ class ElementAssembly:
     def surfacePressure(self, PExt, elements, face, pressure):
         """PExt += the pressure `pressure` (positive pushes into the element) on face `face` (Abaqus numbering 1..6)
         of `elements` (element indices of this block), reference configuration; the distributed load of type
-        pressure of the Marmot solid elements (nonlinearimplicitstatic.py:460-501)."""
+        pressure of the Marmot solid elements (nonlinearimplicitstatic.py:460-501). The device copy of `elements`
+        is kept per array object: pass a new array rather than changing one in place."""
         if PExt.numel() != self.nDof or PExt.dtype != self.torch.float64 or not PExt.is_contiguous():
             raise ValueError("PExt must be a contiguous float64 tensor of length nDof")
         if int(face) not in range(1, 7):
