@@ -63,6 +63,26 @@ __constant__ double c_dN20[27][20][3];
 __constant__ double c_w27[27];
 __constant__ double c_N8[8][8];    // shape function values at the quadrature points
 __constant__ double c_N20[27][20];
+__constant__ double c_dN20r[8][20][3];  // 20-node functions at the 2x2x2 points (C3D20R)
+__constant__ double c_N20r[8][20];
+
+// the element's tables by node count NN and point count NQ
+typedef const double (*DNRow)[3];
+template <int NN, int NQ>
+__device__ __forceinline__ DNRow qp_dN(int q)
+{
+    return NN == 8 ? c_dN8[q] : (NQ == 8 ? c_dN20r[q] : c_dN20[q]);
+}
+template <int NN, int NQ>
+__device__ __forceinline__ const double *qp_N(int q)
+{
+    return NN == 8 ? c_N8[q] : (NQ == 8 ? c_N20r[q] : c_N20[q]);
+}
+template <int NN, int NQ>
+__device__ __forceinline__ double qp_w(int q)
+{
+    return NQ == 8 ? c_w8[q] : c_w27[q];
+}
 
 static const double kHexXi[20][3] = {
     {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
@@ -174,7 +194,7 @@ static int upload_tables(int device)
     std::lock_guard<std::mutex> lock(mu);
     if (device < 0 || device >= 256) return EWFE_EINVAL;
     if (done[device]) return 0;
-    double dN8[8][8][3], w8[8], dN20[27][20][3], w27[27], N8[8][8], N20[27][20];
+    double dN8[8][8][3], w8[8], dN20[27][20][3], w27[27], N8[8][8], N20[27][20], dN20r[8][20][3], N20r[8][20];
     const double g2 = 1.0 / std::sqrt(3.0), g3 = std::sqrt(0.6);
     // quadrature points in the reference's order (library.py:212-227, 260-275): 2x2x2: xi = -g for q < 4,
     // +g after, (eta, zeta) = (-,-), (-,+), (+,+), (+,-) by q mod 4; 3x3x3: q = 9 i + 3 j + k over (xi, eta, zeta)
@@ -182,6 +202,8 @@ static int upload_tables(int device)
         const double x[3] = {q < 4 ? -g2 : g2, kEta8[q & 3] * g2, kZeta8[q & 3] * g2};
         host_dN(8, x, dN8[q]);
         host_N(8, x, N8[q]);
+        host_dN(20, x, dN20r[q]);
+        host_N(20, x, N20r[q]);
         w8[q] = 1.0;
     }
     const double pts[3] = {-g3, 0.0, g3}, wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
@@ -198,6 +220,8 @@ static int upload_tables(int device)
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_w8), w8, sizeof(w8)));
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN20), dN20, sizeof(dN20)));
     HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_w27), w27, sizeof(w27)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN20r), dN20r, sizeof(dN20r)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_N20r), N20r, sizeof(N20r)));
     done[device] = true;
     return 0;
 }
@@ -1075,9 +1099,9 @@ __global__ __launch_bounds__(256) void k_body_weights(const int32_t *__restrict_
             for (int c = 0; c < 3; ++c) x[a][c] = X[d + c];
         }
         for (int q = 0; q < NQ; ++q) {
-            const double(*dN)[3] = NN == 8 ? c_dN8[q] : c_dN20[q];
-            const double *Nq = NN == 8 ? c_N8[q] : c_N20[q];
-            const double wq = NN == 8 ? c_w8[q] : c_w27[q];
+            const DNRow dN = qp_dN<NN, NQ>(q);
+            const double *Nq = qp_N<NN, NQ>(q);
+            const double wq = qp_w<NN, NQ>(q);
             double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
             for (int a = 0; a < NN; ++a)
@@ -1217,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_qp_results(EvalArgs A, const double *__
     const int64_t p = t / NQ;
     const int q = (int)(t % NQ);
     const int64_t e = A.order[p];
-    const double(*dN)[3] = NN == 8 ? c_dN8[q] : c_dN20[q];
+    const DNRow dN = qp_dN<NN, NQ>(q);
     double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3], H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     int32_t dof[NN];
     for (int a = 0; a < NN; ++a) dof[a] = A.edof[(int64_t)a * n + p];
@@ -1355,12 +1379,12 @@ __global__ __launch_bounds__(64) void k_adjoint8(EvalArgs A, const double *__res
 }
 
 // ------------------------------------------------------------------------------------------------
-// C3D20 / C3D20TL: one element per 64-lane workgroup
+// C3D20 / C3D20TL (27 points) and C3D20R / C3D20RTL (8 points): one element per 64-lane workgroup
 // ------------------------------------------------------------------------------------------------
-template <int FORM, int MAT, typename VT>
+template <int FORM, int MAT, typename VT, int NQ>
 __global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
 {
-    constexpr int NN = 20, NQ = 27;
+    constexpr int NN = 20;
     using L = QPLayout<NN, FORM>;
     __shared__ double s_nd[9 * NN];      // [field][node]
     __shared__ double s_qp[L::NF * NQ];  // [field][q]
@@ -1390,7 +1414,8 @@ __global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
         Recipe r;
         double wd;
         const bool ok =
-            qp_phase<NN, FORM, MAT>(c_dN20[q], c_w27[q], nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd);
+            qp_phase<NN, FORM, MAT>(qp_dN<NN, NQ>(q), qp_w<NN, NQ>(q), nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r,
+                                    wd);
         if (!ok) atomic_min_pos(A.pnewdt, 0.5);
         auto put = [&](int f, double v) { s_qp[f * NQ + q] = v; };
         put(L::COEF + 0, wd * r.alpha);
@@ -1519,8 +1544,8 @@ static int fill(T *p, int64_t n, T v, hipStream_t s)
 // C ABI
 // ------------------------------------------------------------------------------------------------
 static int npe_of(int k) { return (k == EWFE_C3D8 || k == EWFE_C3D8TL) ? 8 : 20; }
-static int nqp_of(int k) { return (k == EWFE_C3D8 || k == EWFE_C3D8TL) ? 8 : 27; }
-static bool is_tl(int k) { return k == EWFE_C3D8TL || k == EWFE_C3D20TL; }
+static int nqp_of(int k) { return (k == EWFE_C3D20 || k == EWFE_C3D20TL) ? 27 : 8; }
+static bool is_tl(int k) { return k == EWFE_C3D8TL || k == EWFE_C3D20TL || k == EWFE_C3D20RTL; }
 
 extern "C" int ewfe_state_size(int mat_kind)
 {
@@ -1539,7 +1564,7 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
         ewfe::set_error("ewfe_plan_create: invalid arguments");
         return EWFE_EINVAL;
     }
-    if (elem_kind < EWFE_C3D8 || elem_kind > EWFE_C3D20TL) {
+    if (elem_kind < EWFE_C3D8 || elem_kind > EWFE_C3D20RTL) {
         ewfe::set_error("ewfe_plan_create: unknown element kind");
         return EWFE_EINVAL;
     }
@@ -1843,7 +1868,10 @@ static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s)
         A.p_end = p->color_begin[c + 1];
         const int64_t cnt = A.p_end - A.p_begin;
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL((k_hex20<FORM, MAT, VT>), dim3((unsigned)cnt), dim3(64), 0, s, A);
+        if (p->nqp == 8)
+            hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 8>), dim3((unsigned)cnt), dim3(64), 0, s, A);
+        else
+            hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 27>), dim3((unsigned)cnt), dim3(64), 0, s, A);
         HIP_CHECK(hipGetLastError());
     }
     return 0;
@@ -2010,6 +2038,9 @@ extern "C" int ewfe_body_force(ewfe_plan *p, const double *X_dev, const uint8_t 
     if (p->npe == 8)
         hipLaunchKernelGGL((k_body_weights<8, 8>), dim3(eb), dim3(256), 0, s, p->d_edof, p->d_order, X_dev, el_mask_dev,
                            pm->d_bweights, p->n_el);
+    else if (p->nqp == 8)
+        hipLaunchKernelGGL((k_body_weights<20, 8>), dim3(eb), dim3(256), 0, s, p->d_edof, p->d_order, X_dev,
+                           el_mask_dev, pm->d_bweights, p->n_el);
     else
         hipLaunchKernelGGL((k_body_weights<20, 27>), dim3(eb), dim3(256), 0, s, p->d_edof, p->d_order, X_dev,
                            el_mask_dev, pm->d_bweights, p->n_el);
@@ -2061,6 +2092,10 @@ extern "C" int ewfe_qp_results(const ewfe_plan *p, int mat_kind, const double *m
         if (tl) EWFE_QPR(8, 8, FORM_TL, M_NH);
         else if (mat_kind == EWFE_VON_MISES) EWFE_QPR(8, 8, FORM_SS, M_J2);
         else EWFE_QPR(8, 8, FORM_SS, M_LE);
+    } else if (p->nqp == 8) {
+        if (tl) EWFE_QPR(20, 8, FORM_TL, M_NH);
+        else if (mat_kind == EWFE_VON_MISES) EWFE_QPR(20, 8, FORM_SS, M_J2);
+        else EWFE_QPR(20, 8, FORM_SS, M_LE);
     } else {
         if (tl) EWFE_QPR(20, 27, FORM_TL, M_NH);
         else if (mat_kind == EWFE_VON_MISES) EWFE_QPR(20, 27, FORM_SS, M_J2);

@@ -186,6 +186,19 @@ def _boxgen(deck: _Deck, gen: dict):
         deck.surfaces[f"{name}_{suffix}"] = {face: f"{name}_{suffix}"}
 
 
+def _findClosestNode(deck: _Deck, gen: dict):
+    """edelweissfe/generators/findclosestnode.py:51-73: the node nearest to 'location' (the first of equally near
+    ones, in node order) becomes the node set 'storeIn'."""
+    opts = _assignments([t for ln in gen["data"] for t in splitLineAtCommas(ln) if t])
+    loc = np.array([float(t) for t in splitLineAtCommas(opts["location"]) if t])
+    if len(loc) != 3:
+        raise ValueError("findClosestNode: the location needs 3 coordinates")
+    if opts["storeIn"] in deck.nodeSets:
+        raise Exception(f"Nodeset {opts['storeIn']} already exists")
+    coords = np.asarray(deck.coords, dtype=np.float64).reshape(-1, 3)
+    deck.nodeSets[opts["storeIn"]] = [deck.nodeLabels[int(np.linalg.norm(coords - loc, axis=1).argmin())]]
+
+
 def createModelFromInputFile(inputFile: dict) -> FEModel:
     """FEModel of a parsed deck: nodes, one element block with its material, node and element sets."""
     for job in inputFile["*job"]:
@@ -193,11 +206,12 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
             raise NotImplementedError("the GPU path covers 3D solid elements only")
     deck = _Deck()
     gens = inputFile["*modelgenerator"]
+    generators = {"boxgen": _boxgen, "findclosestnode": _findClosestNode}
     for gen in gens:
-        if gen["generator"].lower() != "boxgen":
+        if gen["generator"].lower() not in generators:
             raise NotImplementedError(f"model generator {gen['generator']} is not part of the GPU path")
     for gen in (g for g in gens if not _truthy(g.get("executeaftermanualgeneration", "false"))):
-        _boxgen(deck, gen)
+        generators[gen["generator"].lower()](deck, gen)
     for nd in inputFile["*node"]:
         rows = [splitLineAtCommas(ln) for ln in nd["data"]]
         labels = [int(r[0]) for r in rows]
@@ -226,7 +240,7 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
         else:
             deck.nodeSets[ns["nset"]] = [lab for s in data for lab in deck.nodeSets[s]]
     for gen in (g for g in gens if _truthy(g.get("executeaftermanualgeneration", "false"))):
-        _boxgen(deck, gen)
+        generators[gen["generator"].lower()](deck, gen)
     if deck.elType is None:
         raise ValueError("the deck defines no elements")
 

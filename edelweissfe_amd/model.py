@@ -26,6 +26,8 @@ ELEMENT_LIBRARY = {
     "C3D20": dict(nNodes=20, nInt=27, tl=False),
     "C3D8TL": dict(nNodes=8, nInt=8, tl=True),
     "C3D20TL": dict(nNodes=20, nInt=27, tl=True),
+    "C3D20R": dict(nNodes=20, nInt=8, tl=False),  # library.py:276-290, 2x2x2 points
+    "C3D20RTL": dict(nNodes=20, nInt=8, tl=True),  # library.py:517-531
 }
 
 MATERIAL_PARAMETERS = {

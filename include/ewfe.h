@@ -40,7 +40,14 @@ extern "C" {
 typedef struct ewfe_plan ewfe_plan;
 typedef struct ewfe_mg ewfe_mg;
 
-enum ewfe_elem_kind { EWFE_C3D8 = 0, EWFE_C3D20 = 1, EWFE_C3D8TL = 2, EWFE_C3D20TL = 3 };
+enum ewfe_elem_kind {
+    EWFE_C3D8 = 0,
+    EWFE_C3D20 = 1,
+    EWFE_C3D8TL = 2,
+    EWFE_C3D20TL = 3,
+    EWFE_C3D20R = 4,   /* 20 nodes, 2x2x2 points (edelweissfe/elements/library.py:276-290) */
+    EWFE_C3D20RTL = 5  /* the same, total Lagrangian (library.py:517-531) */
+};
 enum ewfe_mat_kind { EWFE_LINEAR_ELASTIC = 0, EWFE_VON_MISES = 1, EWFE_NEO_HOOKE_WA = 2 };
 enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1 };
 

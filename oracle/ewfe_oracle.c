@@ -42,7 +42,7 @@
 #endif
 #include <time.h>
 
-enum { EL_C3D8 = 0, EL_C3D20 = 1, EL_C3D8TL = 2, EL_C3D20TL = 3 };
+enum { EL_C3D8 = 0, EL_C3D20 = 1, EL_C3D8TL = 2, EL_C3D20TL = 3, EL_C3D20R = 4, EL_C3D20RTL = 5 };
 enum { MAT_LE = 0, MAT_J2 = 1, MAT_NH = 2 };
 
 #define MAXN 20
@@ -50,8 +50,8 @@ enum { MAT_LE = 0, MAT_J2 = 1, MAT_NH = 2 };
 #define MAXD 60
 
 static int el_nnodes(int k) { return (k == EL_C3D8 || k == EL_C3D8TL) ? 8 : 20; }
-static int el_nqp(int k) { return (k == EL_C3D8 || k == EL_C3D8TL) ? 8 : 27; }
-static int el_is_tl(int k) { return k == EL_C3D8TL || k == EL_C3D20TL; }
+static int el_nqp(int k) { return (k == EL_C3D20 || k == EL_C3D20TL) ? 27 : 8; } /* C3D20R(TL): 2x2x2 */
+static int el_is_tl(int k) { return k == EL_C3D8TL || k == EL_C3D20TL || k == EL_C3D20RTL; }
 
 /* Abaqus parametric corner/midside coordinates */
 static const double HEX20_XI[20][3] = {

@@ -40,7 +40,8 @@ def _uniaxial_field(X, strain, rng, noise):
 
 
 @pytest.mark.parametrize("el,mat,mp,n", [("C3D8", "linearelastic", LE, 4), ("C3D20", "linearelastic", LE, 2),
-                                         ("C3D8TL", "neohookeanwa", NH, 4), ("C3D20TL", "neohookeanwa", NH, 2)])
+                                         ("C3D8TL", "neohookeanwa", NH, 4), ("C3D20TL", "neohookeanwa", NH, 2),
+                                         ("C3D20R", "linearelastic", LE, 2), ("C3D20RTL", "neohookeanwa", NH, 2)])
 def test_eval_assemble_matches_oracle(el, mat, mp, n):
     print(f"case {el} {mat} n={n}", flush=True)
     X, conn, _ = jittered_box(n, el, seed=1)
@@ -105,7 +106,7 @@ def test_plain_method_10cube():
     assert hist < 1e-10 * np.abs(r["P"]).max()
 
 
-@pytest.mark.parametrize("el,n,mp", [("C3D8", 4, J2), ("C3D8", 3, J2NL), ("C3D20", 2, J2)])
+@pytest.mark.parametrize("el,n,mp", [("C3D8", 4, J2), ("C3D8", 3, J2NL), ("C3D20", 2, J2), ("C3D20R", 2, J2)])
 def test_j2_two_increments_match_oracle(el, n, mp):
     """Increment 1 from the virgin state into the plastic range, then a second increment from the
     converged state: P, F, CSR values, plastic strain and kappa against the oracle."""
@@ -306,7 +307,7 @@ def test_reference_decks_on_gpu(name):
     assert max(worst) < 1e-8
 
 
-@pytest.mark.parametrize("el", ["C3D8", "C3D20"])
+@pytest.mark.parametrize("el", ["C3D8", "C3D20", "C3D20R"])
 def test_body_force_matches_oracle(el):
     """ewfe_body_force on a jittered mesh (all elements, then every third element) against the oracle's
     restatement of computeBodyForce + computeBodyForces; PExt is accumulated, not overwritten."""
@@ -357,7 +358,8 @@ def test_surface_pressure_matches_oracle(el):
 
 @pytest.mark.parametrize("el,mat,mp", [("C3D8", "linearelastic", LE), ("C3D8", "vonmises", J2),
                                        ("C3D20", "vonmises", J2), ("C3D8TL", "neohookeanwa", NH),
-                                       ("C3D20TL", "neohookeanwa", NH)])
+                                       ("C3D20TL", "neohookeanwa", NH), ("C3D20R", "vonmises", J2),
+                                       ("C3D20RTL", "neohookeanwa", NH)])
 def test_qp_results_match_oracle_state(el, mat, mp):
     """Quadrature-point stress and strain of ewfe_qp_results against the per-point state the oracle keeps in
     the reference's layout (stress 6 | strain 6 | ...), after one evaluation from the virgin state."""

@@ -17,7 +17,9 @@ from oracle import oracle as O
 DECK_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks")
 FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C3D8", "C3D20", "WallShearHexa8",
                  "WallShearHexa20", "LinearElasticIsotropic"]
-ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D"]
+# decks read only through the reader: BodyForce3D (*include of the mesh), BoxGen (C3D20R, findClosestNode),
+# AnalyticalFieldsDirichlet (C3D20R torsion bar, Dirichlet values scaled by scalarExpression fields)
+ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet"]
 
 
 def _deck(name):

@@ -39,18 +39,25 @@ def _jittered_hex(nn, seed=0, amp=0.08):
     return Xe
 
 
-@pytest.mark.parametrize("ek,nn", [("C3D8", 8), ("C3D20", 20), ("C3D8TL", 8), ("C3D20TL", 20)])
+@pytest.mark.parametrize("ek,nn", [("C3D8", 8), ("C3D20", 20), ("C3D8TL", 8), ("C3D20TL", 20), ("C3D20R", 20),
+                                   ("C3D20RTL", 20)])
 def test_symmetry_rigid_body_patch(ek, nn):
     Xe = _jittered_hex(nn)
     nd = 3 * nn
     mk, mp = ("neohookeanwa", [80.0, 200.0]) if "TL" in ek else ("linearelastic", [210000.0, 0.3])
-    nq = 8 if nn == 8 else 27
+    nq = O.NQP[O.EL_KINDS[ek]]
     st = np.zeros(nq * O.NSTATE[O.MAT_KINDS[mk]])
     K, P, _, _ = O.element(ek, mk, mp, Xe.ravel(), np.zeros(nd), np.zeros(nd), st)
     scale = np.abs(K).max()
     assert np.abs(K - K.T).max() < 1e-12 * scale
     w = np.linalg.eigvalsh(0.5 * (K + K.T))
-    assert np.all(np.abs(w[:6]) < 1e-10 * scale) and w[6] > 1e-3 * scale
+    # six rigid-body modes; the 2x2x2 rule leaves a single 20-node element six more zero-energy modes
+    zeros = 12 if ek.startswith("C3D20R") else 6
+    assert np.all(np.abs(w[:zeros]) < 1e-10 * scale) and w[zeros] > 1e-3 * scale
+    c = Xe.mean(axis=0)
+    for R in (np.eye(3)[k] for k in range(3)):  # translations and rotations are in the null space
+        assert np.abs(K @ np.tile(R, nn)).max() < 1e-10 * scale
+        assert np.abs(K @ np.cross(R, Xe - c).ravel()).max() < 1e-10 * scale
     assert np.abs(P).max() < 1e-12 * scale
     if "TL" not in ek:  # patch test: linear field -> constant strain, K u = -P exactly
         G = np.array([[1e-3, 2e-4, 0], [0, -3e-4, 1e-4], [5e-4, 0, 2e-4]])
@@ -88,12 +95,12 @@ def test_j2_tangent_vs_fd_plastic():
     assert np.abs(K - K.T).max() < 1e-10 * np.abs(K).max()  # associative J2: symmetric tangent
 
 
-@pytest.mark.parametrize("ek,nn", [("C3D8TL", 8), ("C3D20TL", 20)])
+@pytest.mark.parametrize("ek,nn", [("C3D8TL", 8), ("C3D20TL", 20), ("C3D20RTL", 20)])
 def test_neohooke_tangent_vs_fd(ek, nn):
     Xe = _jittered_hex(nn, seed=5)
     rng = np.random.default_rng(2)
     u = rng.uniform(-0.08, 0.08, 3 * nn)
-    nq = 8 if nn == 8 else 27
+    nq = O.NQP[O.EL_KINDS[ek]]
     st = np.zeros(nq * 13)
     K, P, _, _ = O.element(ek, "neohookeanwa", [80.0, 200.0], Xe.ravel(), u, u, st)
     Kfd = _fd_tangent(ek, "neohookeanwa", [80.0, 200.0], Xe.ravel(), u, u, st, h=1e-6)
