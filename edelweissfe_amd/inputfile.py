@@ -517,6 +517,9 @@ def finiteElementSimulation(inputFile: dict, verbose: bool = False, solverOption
         if solverName not in solvers:
             s = solverDefs.get(solverName)
             opts = dict(_assignments(t for ln in (s["data"] if s else []) for t in splitLineAtCommas(ln)))
+            cfg = opts.get("linsolverConfigFile")
+            if cfg and not os.path.isabs(cfg):  # relative to the deck that names it
+                opts["linsolverConfigFile"] = os.path.join(os.path.dirname(s["inputFile"]), cfg)
             opts.update(solverOptions or {})
             solvers[solverName] = NISTGPU(journal=_Journal(verbose=verbose), **opts)
         solvers[solverName].solveStep(step, model)

@@ -29,6 +29,14 @@ ELEMENT_LIBRARY = {
     "C3D20R": dict(nNodes=20, nInt=8, tl=False),  # library.py:276-290, 2x2x2 points
     "C3D20RTL": dict(nNodes=20, nInt=8, tl=True),  # library.py:517-531
 }
+# other names of the same elements (library.py:539-550)
+ELEMENT_ALIASES = {"C3D8N": "C3D8", "C3D20N": "C3D20", "C3D8NTL": "C3D8TL", "C3D20NTL": "C3D20TL"}
+
+
+def elementType(name: str) -> str:
+    """The element library's name of an element type (aliases resolved, upper case)."""
+    et = name.upper()
+    return ELEMENT_ALIASES.get(et, et)
 
 MATERIAL_PARAMETERS = {
     # name -> names of the material properties (the reference material classes' constructors)
@@ -67,7 +75,7 @@ class ElementBlock:
     """
 
     def __init__(self, elType: str, conn, labels=None, material: Material | None = None, elParams=None):
-        et = elType.upper()
+        et = elementType(elType)
         if et not in ELEMENT_LIBRARY:
             raise AttributeError(f"element type {elType} is not available on the GPU path")
         self.elType = et
@@ -160,7 +168,7 @@ class FEModel:
 def boxGenArrays(nX, nY, nZ, lX=1.0, lY=1.0, lZ=1.0, x0=0.0, y0=0.0, z0=0.0, elType="C3D8", name="gen"):
     """Vectorised boxGen (boxgen.py:125-371). Returns (coords [nNodes,3], conn [nEl,npe] int32, nodeSets,
     gridIndex [nNodes,3] int (ix, iy, iz) on the generator's node grid)."""
-    npe = ELEMENT_LIBRARY[elType.upper()]["nNodes"]
+    npe = ELEMENT_LIBRARY[elementType(elType)]["nNodes"]
     if npe == 8:
         nx, ny, nz = nX + 1, nY + 1, nZ + 1
     else:

@@ -94,6 +94,10 @@ class NISTGPU:
     }
 
     AUTO_AMG_MIN_DOFS = 20000  # linsolver="auto": the multigrid from this many dofs on
+    # the reference's linear solver names (edelweissfe/config/linsolve.py:46-86): its direct solvers map to the
+    # default choice, its algebraic-multigrid Krylov solvers to the multigrid PCG
+    REFERENCE_LINSOLVERS = {"superlu": "auto", "umfpack": "auto", "pardiso": "auto", "panuapardiso": "auto",
+                            "klu": "auto", "petsclu": "auto", "mumps": "auto", "gmres": "amg", "amgcl": "amg"}
 
     def __init__(self, jobInfo=None, journal=None, **kwargs):
         jobInfo = jobInfo or DEFAULT_JOB_INFO
@@ -130,7 +134,7 @@ class NISTGPU:
             with open(self.options["linsolverConfigFile"]) as f:
                 cfg = json.load(f)
             self._updateOptions({k: v for k, v in cfg.items() if k in self.NISTOptions}, self.journal)
-        if self.options["linsolver"].lower() not in ("pcg", "amg", "auto"):
+        if self.options["linsolver"].lower() not in ("pcg", "amg", "auto") + tuple(self.REFERENCE_LINSOLVERS):
             raise AttributeError("invalid linear solver {:} requested".format(self.options["linsolver"]))
         if len(model.elementBlocks) != 1:
             raise NotImplementedError("NISTGPU assembles one element block (one element type and section) per model")
@@ -144,7 +148,8 @@ class NISTGPU:
         if id(block) not in self.assemblies or self.assemblies[id(block)].nDof != self.theDofManager.nDof:
             self.assemblies[id(block)] = ElementAssembly(block, self.theDofManager, model, device=device)
         self.assembly = self.assemblies[id(block)]
-        self._linsolver = self.options["linsolver"].lower()
+        self._linsolver = self.REFERENCE_LINSOLVERS.get(self.options["linsolver"].lower(),
+                                                        self.options["linsolver"].lower())
         if self._linsolver == "auto":
             # the multigrid from AUTO_AMG_MIN_DOFS dofs on, where its setup pays; block Jacobi below and on meshes
             # the multigrid does not take (dofs that are not 3 per node)

@@ -19,7 +19,8 @@ FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C
                  "WallShearHexa20", "LinearElasticIsotropic"]
 # decks read only through the reader: BodyForce3D (*include of the mesh), BoxGen (C3D20R, findClosestNode),
 # AnalyticalFieldsDirichlet (C3D20R torsion bar, Dirichlet values scaled by scalarExpression fields)
-ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet"]
+# WallShearHexa8GMRES (C3D8N, the reference's pyamg-GMRES solver: the multigrid PCG here)
+ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet", "WallShearHexa8GMRES"]
 
 
 def _deck(name):
