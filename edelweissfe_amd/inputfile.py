@@ -265,6 +265,7 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
         materials[m.get("id", m["name"])] = Material(_MATERIALS[key], _numbers(m["data"]))
     fields = _analyticalFields(inputFile)
     model.analyticalFields = fields
+    model.fieldOutputs = _fieldOutputs(inputFile)
     assigned = np.zeros(len(block), dtype=bool)
     chosen = None
     elParams = None
@@ -437,8 +438,27 @@ def _nodeLoad(kw):
     return [float(kw.get(str(i + 1), 0.0)) for i in range(3)]
 
 
+def _fieldOutputs(inputFile):
+    """*fieldOutput 'create=perElement' definitions (edelweissfe/outputmanagers/ and utils/fieldoutput.py): name ->
+    (result, element set, quadrature points); the GPU path uses them for the setfield action only."""
+    out = {}
+    for fo in inputFile["*fieldoutput"]:
+        for ln in fo["data"]:
+            d = _assignments([t for t in splitLineAtCommas(ln) if t])
+            if d.get("create", "").lower() != "perelement":
+                continue
+            qp = d.get("quadraturePoint", "0")
+            if ":" in qp:
+                a, b = (int(t) if t else None for t in qp.split(":"))
+                qps = slice(a, b)
+            else:
+                qps = np.array([int(t) for t in qp.split(",")])
+            out[d["name"]] = (d["result"], d.get("elSet", "all"), qps)
+    return out
+
+
 def _makeAction(module, name, kw, model):
-    from .stepping import BodyForce, Dirichlet, DistributedLoad, NodeForces, Options
+    from .stepping import BodyForce, Dirichlet, DistributedLoad, NodeForces, Options, SetField
 
     amp = _Amplitude(kw["f(t)"]) if "f(t)" in kw else None
     field = kw.get("field", "displacement")
@@ -451,6 +471,11 @@ def _makeAction(module, name, kw, model):
         return BodyForce(name, model, kw["elSet"], _vector(kw["forceVector"]), amplitude=amp)
     if module == "distributedload":
         return DistributedLoad(name, model, kw["surface"], kw["type"], _vector(kw["magnitude"]), amplitude=amp)
+    if module == "setfield":
+        result, elSet, qps = model.fieldOutputs[kw["fieldOutput"]]
+        kind = kw["type"]
+        value = model.analyticalFields[kw["value"]] if kind == "analyticalField" else _vector(kw["value"])
+        return SetField(name, model, result, elSet, qps, kind, value)
     if module == "options":
         return Options(name, kw.pop("category", name), **kw)
     raise NotImplementedError(f"step action {module} is not part of the GPU path")
@@ -495,6 +520,8 @@ def createStepsFromInputFile(model: FEModel, inputFile: dict):
                                          scale=scale)
                 elif module == "nodeforces":
                     act.updateStepAction(_nodeLoad(kw), amplitude=_Amplitude(kw["f(t)"]) if "f(t)" in kw else None)
+                elif module == "setfield":
+                    act.updateStepAction()
                 else:
                     raise NotImplementedError(f"updating a {module} action in a later step is not part of the GPU path")
             else:

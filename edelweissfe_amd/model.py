@@ -134,6 +134,7 @@ class FEModel:
         self.constraints = {}
         self.scalarVariables = {}
         self.analyticalFields = {}  # name -> f(x, y, z)
+        self.fieldOutputs = {}  # name -> (result, element set, quadrature points)
 
     def addNodes(self, coords, labels=None):
         coords = np.asarray(coords, dtype=np.float64).reshape(-1, 3)
@@ -160,6 +161,48 @@ class FEModel:
         self.time = time
         for b in self.elementBlocks:
             b.acceptLastState()
+
+
+# --------------------------------------------------------------------------------------------------
+# quadrature points
+# --------------------------------------------------------------------------------------------------
+_HEX_NODES = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1],
+                       [-1, 1, 1], [0, -1, -1], [1, 0, -1], [0, 1, -1], [-1, 0, -1], [0, -1, 1], [1, 0, 1],
+                       [0, 1, 1], [-1, 0, 1], [-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], dtype=float)
+
+
+def quadraturePoints(nInt: int) -> np.ndarray:
+    """Parametric coordinates [nInt, 3] in the reference's order (library.py:212-227, 260-275): 2x2x2: xi = -g
+    for the first four, (eta, zeta) = (-,-), (-,+), (+,+), (+,-); 3x3x3: q = 9 i + 3 j + k over (xi, eta, zeta)."""
+    if nInt == 8:
+        g = 1.0 / np.sqrt(3.0)
+        eta, zeta = np.array([-1, -1, 1, 1]), np.array([-1, 1, 1, -1])
+        return g * np.stack([np.repeat([-1.0, 1.0], 4), np.tile(eta, 2), np.tile(zeta, 2)], axis=1)
+    g = np.sqrt(0.6)
+    pts = np.array([-g, 0.0, g])
+    i, j, k = np.meshgrid(np.arange(3), np.arange(3), np.arange(3), indexing="ij")
+    return np.stack([pts[i.ravel()], pts[j.ravel()], pts[k.ravel()]], axis=1)
+
+
+def shapeFunctions(nNodes: int, xi: np.ndarray) -> np.ndarray:
+    """N [nPoints, nNodes] of the trilinear (8) or serendipity (20) hexahedron, Abaqus node order."""
+    c = _HEX_NODES[:nNodes]
+    x = np.atleast_2d(xi)[:, None, :]  # [p, 1, 3]
+    f = 1 + c[None] * x  # [p, a, 3]
+    if nNodes == 8:
+        return 0.125 * f.prod(axis=2)
+    N = np.empty((x.shape[0], 20))
+    N[:, :8] = 0.125 * f[:, :8].prod(axis=2) * ((c[None, :8] * x).sum(axis=2) - 2)
+    mid = np.where(c[None, 8:] == 0, 1 - x * x, f[:, 8:])
+    N[:, 8:] = 0.25 * mid.prod(axis=2)
+    return N
+
+
+def quadraturePointCoordinates(block, coords) -> np.ndarray:
+    """[nEl, nInt, 3] coordinates of the quadrature points of every element of the block (reference
+    configuration; getCoordinatesAtQuadraturePoints of the reference's elements)."""
+    N = shapeFunctions(block.nNodes, quadraturePoints(block.nInt))  # [q, a]
+    return np.einsum("qa,eac->eqc", N, np.asarray(coords)[block.conn])
 
 
 # --------------------------------------------------------------------------------------------------

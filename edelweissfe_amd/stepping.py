@@ -352,6 +352,66 @@ class DistributedLoad:
         return self.magnitudeAtStepStart + self.delta * self.amplitude(t)
 
 
+class SetField:
+    """``setfield, fieldOutput=.., type=uniform|analyticalField, value=..`` (edelweissfe/stepactions/setfield.py):
+    at the start of the step, a per-element quadrature-point result of the field output (its element set and
+    points) is set to a value, or to an analytical field at the points. On the GPU path the result is a
+    quadrature-point state variable: ``kappa`` of the J2 material (state index 6, after the plastic strain)."""
+
+    STATE_INDEX = {("vonmises", "kappa"): 6}
+
+    def __init__(self, name, model, result: str, elSet: str, quadraturePoints, kind: str, value):
+        self.name = name
+        if kind not in ("uniform", "analyticalField"):
+            raise Exception(f"Invalid type: {kind}")
+        self.result = result
+        self.blockIndex, self.elements = model.elementSets[elSet]
+        self.quadraturePoints = quadraturePoints  # slice or index array
+        self.kind = kind
+        self.value = value  # list of floats (uniform) or f(x, y, z)
+        self.active = True
+
+    def applyAtStepEnd(self, model):
+        self.active = False
+
+    def updateStepAction(self):
+        self.active = True
+
+    def applyAtIncrementStart(self, model, timeStep):
+        pass
+
+    def qpValues(self, model):
+        """(element indices, quadrature point indices, values [nElements, nPoints]) to be set."""
+        from .model import quadraturePointCoordinates
+
+        block = model.elementBlocks[self.blockIndex]
+        qps = np.arange(block.nInt)[self.quadraturePoints]
+        if self.kind == "uniform":
+            if len(self.value) != 1:
+                raise ValueError(f"SetField {self.name}: {self.result} takes one value per point")
+            return self.elements, qps, np.full((len(self.elements), len(qps)), float(self.value[0]))
+        xq = quadraturePointCoordinates(block, model.nodeCoordinates)[self.elements][:, qps]
+        vals = np.array([[self.value(*x) for x in row] for row in xq], dtype=np.float64).reshape(len(self.elements),
+                                                                                                 len(qps))
+        return self.elements, qps, vals
+
+    def applyAtStepStart(self, model):
+        if not self.active:
+            return
+        block = model.elementBlocks[self.blockIndex]
+        key = (block.material.name if block.material else None, self.result)
+        if key not in self.STATE_INDEX or block.stateVars is None or block.planOrder is None:
+            raise NotImplementedError(f"setfield of {self.result} for {key[0]} is not part of the GPU path")
+        els, qps, vals = self.qpValues(model)
+        nEl, nInt = len(block), block.nInt
+        st = block.stateVars.detach().cpu().numpy().reshape(-1, nInt, nEl)  # [state][point][plan position]
+        pos = np.empty(nEl, dtype=np.int64)
+        pos[np.asarray(block.planOrder)] = np.arange(nEl)
+        for j, q in enumerate(qps):
+            st[self.STATE_INDEX[key], q, pos[np.asarray(els)]] = vals[:, j]
+        block.stateVars.copy_(block.stateVars.new_tensor(st.ravel()))
+
+
 class Options:
     """``options, category=NISTSolver, key=value`` (edelweissfe/stepactions/options.py:38-61)."""
 

@@ -20,7 +20,9 @@ FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C
 # decks read only through the reader: BodyForce3D (*include of the mesh), BoxGen (C3D20R, findClosestNode),
 # AnalyticalFieldsDirichlet (C3D20R torsion bar, Dirichlet values scaled by scalarExpression fields)
 # WallShearHexa8GMRES (C3D8N, the reference's pyamg-GMRES solver: the multigrid PCG here)
-ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet", "WallShearHexa8GMRES"]
+# AnalyticalFieldsSetField (J2 with kappa set from a scalarExpression field at the points)
+ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet", "WallShearHexa8GMRES",
+                             "AnalyticalFieldsSetField"]
 
 
 def _deck(name):
@@ -104,13 +106,7 @@ def test_reader_and_oracle_reproduce_uref(name):
     """Deck -> reader -> oracle Newton loop -> U.ref (CPU)."""
     inp = _deck(name)
     model = IF.createModelFromInputFile(inp)
-    b = model.elementBlocks[0]
-    om = dict(el_kind=b.elType, mat_kind=b.material.name, mp=b.material.materialProperties,
-              X=model.nodeCoordinates, conn=b.conn)
-    U = state = None
-    for step, _ in _steps(model, inp):
-        r = O.solve_step(om, _oracle_step(model, step), U0=U, state_n=state)
-        U, state = r["U"], r["state"]
+    U = _oracle_run(model, inp)
     Uref = np.loadtxt(os.path.join(DECK_DIR, name, "U.ref"))
     assert np.abs(U - Uref).max() < 1e-6
     assert np.abs(U - Uref).max() < 1e-7 * max(1.0, np.abs(Uref).max())
@@ -252,12 +248,21 @@ def test_material_parameter_from_scalar_fields(tmp_path):
     assert np.isfinite(U).all() and np.abs(U).max() > 0
 
 
+_ORACLE_STATE_INDEX = {"kappa": 12}  # the oracle's J2 state per point: stress 6, strain 6, kappa
+
+
 def _oracle_run(model, inp):
     b = model.elementBlocks[0]
     om = dict(el_kind=b.elType, mat_kind=b.material.name, mp=b.material.materialProperties,
               X=model.nodeCoordinates, conn=b.conn, el_mp=b.elParams)
-    U = state = None
+    nq, ns = O.NQP[O.EL_KINDS[b.elType]], O.NSTATE[O.MAT_KINDS[b.material.name]]
+    U, state = None, np.zeros(len(b) * nq * ns)
     for step, _ in _steps(model, inp):
+        for act in step.actions.get("setfield", {}).values():  # setfield.py: at step start
+            if act.active:
+                els, qps, vals = act.qpValues(model)
+                state.reshape(len(b), nq, ns)[np.asarray(els)[:, None], qps[None, :],
+                                              _ORACLE_STATE_INDEX[act.result]] = vals
         r = O.solve_step(om, _oracle_step(model, step), U0=U, state_n=state)
         U, state = r["U"], r["state"]
     return U
