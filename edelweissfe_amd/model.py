@@ -198,6 +198,39 @@ def shapeFunctions(nNodes: int, xi: np.ndarray) -> np.ndarray:
     return N
 
 
+def shapeFunctionDerivatives(nNodes: int, xi: np.ndarray) -> np.ndarray:
+    """dN/dxi [nPoints, nNodes, 3] of the trilinear (8) or serendipity (20) hexahedron, Abaqus node order."""
+    c = _HEX_NODES[:nNodes]
+    x = np.atleast_2d(xi)[:, None, :]  # [p, 1, 3]
+    f = 1 + c[None] * x  # [p, a, 3]
+    dN = np.empty((x.shape[0], nNodes, 3))
+    for d in range(3):
+        o = [k for k in range(3) if k != d]
+        if nNodes == 8:
+            dN[:, :, d] = 0.125 * c[None, :, d] * f[:, :, o[0]] * f[:, :, o[1]]
+            continue
+        s = (c[None, :8] * x).sum(axis=2) - 2
+        dN[:, :8, d] = 0.125 * c[None, :8, d] * f[:, :8, o[0]] * f[:, :8, o[1]] * (s + f[:, :8, d])
+        cm = c[None, 8:]
+        fm = np.where(cm == 0, 1 - x * x, f[:, 8:])
+        dfm = np.where(cm == 0, -2 * x, cm)
+        dN[:, 8:, d] = 0.25 * dfm[:, :, d] * fm[:, :, o[0]] * fm[:, :, o[1]]
+    return dN
+
+
+def quadraturePointStrains(block, coords, U) -> np.ndarray:
+    """[nEl, nInt, 6] small strains at the quadrature points (engineering Voigt [11,22,33,12,13,23]) of the
+    nodal displacements U ([nNodes, 3] in model node order)."""
+    dN = shapeFunctionDerivatives(block.nNodes, quadraturePoints(block.nInt))  # [q, a, i]
+    Xe = np.asarray(coords)[block.conn]  # [e, a, j]
+    Ue = np.asarray(U)[block.conn]
+    J = np.einsum("qai,eaj->eqij", dN, Xe)
+    g = np.einsum("eqik,qak->eqai", np.linalg.inv(J), dN)  # g_a = J^-1 dN_a
+    H = np.einsum("eak,eqai->eqki", Ue, g)  # du_k / dx_i
+    return np.stack([H[..., 0, 0], H[..., 1, 1], H[..., 2, 2], H[..., 0, 1] + H[..., 1, 0],
+                     H[..., 0, 2] + H[..., 2, 0], H[..., 1, 2] + H[..., 2, 1]], axis=-1)
+
+
 def quadraturePointCoordinates(block, coords) -> np.ndarray:
     """[nEl, nInt, 3] coordinates of the quadrature points of every element of the block (reference
     configuration; getCoordinatesAtQuadraturePoints of the reference's elements)."""

@@ -355,10 +355,13 @@ class DistributedLoad:
 class SetField:
     """``setfield, fieldOutput=.., type=uniform|analyticalField, value=..`` (edelweissfe/stepactions/setfield.py):
     at the start of the step, a per-element quadrature-point result of the field output (its element set and
-    points) is set to a value, or to an analytical field at the points. On the GPU path the result is a
-    quadrature-point state variable: ``kappa`` of the J2 material (state index 6, after the plastic strain)."""
+    points) is set to a value, or to an analytical field at the points. On the GPU path the results are those of
+    the small-strain J2 material: ``kappa`` (state index 6) and ``stress``. The reference integrates the J2 stress
+    incrementally from its stored value; the GPU path stores the plastic strain instead, so a prescribed stress
+    s is stored as the plastic strain eps - C^-1 s at the current strain eps, which gives the same stress and the
+    same trial stresses afterwards."""
 
-    STATE_INDEX = {("vonmises", "kappa"): 6}
+    RESULTS = {("vonmises", "kappa"): 1, ("vonmises", "stress"): 6}  # values per point
 
     def __init__(self, name, model, result: str, elSet: str, quadraturePoints, kind: str, value):
         self.name = name
@@ -381,34 +384,57 @@ class SetField:
         pass
 
     def qpValues(self, model):
-        """(element indices, quadrature point indices, values [nElements, nPoints]) to be set."""
+        """(element indices, quadrature point indices, values [nElements, nPoints, nComponents]) to be set."""
         from .model import quadraturePointCoordinates
 
         block = model.elementBlocks[self.blockIndex]
         qps = np.arange(block.nInt)[self.quadraturePoints]
         if self.kind == "uniform":
-            if len(self.value) != 1:
-                raise ValueError(f"SetField {self.name}: {self.result} takes one value per point")
-            return self.elements, qps, np.full((len(self.elements), len(qps)), float(self.value[0]))
+            v = np.asarray(self.value, dtype=np.float64)
+            return self.elements, qps, np.broadcast_to(v, (len(self.elements), len(qps), len(v))).copy()
         xq = quadraturePointCoordinates(block, model.nodeCoordinates)[self.elements][:, qps]
-        vals = np.array([[self.value(*x) for x in row] for row in xq], dtype=np.float64).reshape(len(self.elements),
-                                                                                                 len(qps))
-        return self.elements, qps, vals
+        vals = np.array([[self.value(*x) for x in row] for row in xq], dtype=np.float64)
+        return self.elements, qps, vals.reshape(len(self.elements), len(qps), 1)
 
     def applyAtStepStart(self, model):
         if not self.active:
             return
+        from .model import ELEMENT_LIBRARY, quadraturePointStrains
+
         block = model.elementBlocks[self.blockIndex]
         key = (block.material.name if block.material else None, self.result)
-        if key not in self.STATE_INDEX or block.stateVars is None or block.planOrder is None:
+        if (key not in self.RESULTS or block.stateVars is None or block.planOrder is None
+                or ELEMENT_LIBRARY[block.elType]["tl"]):
             raise NotImplementedError(f"setfield of {self.result} for {key[0]} is not part of the GPU path")
         els, qps, vals = self.qpValues(model)
+        if vals.shape[2] != self.RESULTS[key]:
+            raise ValueError(f"SetField {self.name}: {self.result} has {self.RESULTS[key]} components per point")
+        els = np.asarray(els)
         nEl, nInt = len(block), block.nInt
         st = block.stateVars.detach().cpu().numpy().reshape(-1, nInt, nEl)  # [state][point][plan position]
         pos = np.empty(nEl, dtype=np.int64)
         pos[np.asarray(block.planOrder)] = np.arange(nEl)
-        for j, q in enumerate(qps):
-            st[self.STATE_INDEX[key], q, pos[np.asarray(els)]] = vals[:, j]
+        if self.result == "kappa":
+            for j, q in enumerate(qps):
+                st[6, q, pos[els]] = vals[:, j, 0]
+        else:  # stress s -> plastic strain eps - C^-1 s (engineering shears)
+            used = np.unique(np.concatenate([b.conn.ravel() for b in model.elementBlocks]))
+            Un = np.zeros((model.nNodes, 3))
+            U = getattr(model, "U", None)
+            if U is not None:
+                Un[used] = U.detach().cpu().numpy().reshape(-1, 3)
+            eps = quadraturePointStrains(block, model.nodeCoordinates, Un)[els][:, qps]  # [e, q, 6]
+            mp = block.material.materialProperties
+            E = np.full(len(els), float(mp[0])) if block.elParams is None else block.elParams[els]
+            nu = float(mp[1])
+            s = vals
+            e_el = np.empty_like(s)
+            for i in range(3):
+                e_el[..., i] = (s[..., i] - nu * (s[..., (i + 1) % 3] + s[..., (i + 2) % 3])) / E[:, None]
+            e_el[..., 3:] = s[..., 3:] * (2 * (1 + nu) / E)[:, None, None]
+            ep = eps - e_el
+            for j, q in enumerate(qps):
+                st[0:6, q, pos[els]] = ep[:, j, :].T
         block.stateVars.copy_(block.stateVars.new_tensor(st.ravel()))
 
 

@@ -368,6 +368,8 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
 
     dirichlets = []
     for nodes, comps in step.get("dirichlet", []):
+        if not comps:  # a Dirichlet action redefined without components constrains nothing (dirichlet.py:80-93)
+            continue
         cs = sorted(comps)
         idx = (base[nodes][:, None] + np.array(cs)[None, :]).ravel()
         # a value is one number for all nodes or one per node (dirichlet.py:96-99, analyticalField)

@@ -21,8 +21,9 @@ FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C
 # AnalyticalFieldsDirichlet (C3D20R torsion bar, Dirichlet values scaled by scalarExpression fields)
 # WallShearHexa8GMRES (C3D8N, the reference's pyamg-GMRES solver: the multigrid PCG here)
 # AnalyticalFieldsSetField (J2 with kappa set from a scalarExpression field at the points)
+# and SetField (kappa set uniformly, then the stress set at every point after a support is released)
 ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet", "WallShearHexa8GMRES",
-                             "AnalyticalFieldsSetField"]
+                             "AnalyticalFieldsSetField", "SetField"]
 
 
 def _deck(name):
@@ -248,7 +249,7 @@ def test_material_parameter_from_scalar_fields(tmp_path):
     assert np.isfinite(U).all() and np.abs(U).max() > 0
 
 
-_ORACLE_STATE_INDEX = {"kappa": 12}  # the oracle's J2 state per point: stress 6, strain 6, kappa
+_ORACLE_STATE_INDEX = {"kappa": [12], "stress": list(range(6))}  # the oracle's J2 state: stress 6, strain 6, kappa
 
 
 def _oracle_run(model, inp):
@@ -261,8 +262,9 @@ def _oracle_run(model, inp):
         for act in step.actions.get("setfield", {}).values():  # setfield.py: at step start
             if act.active:
                 els, qps, vals = act.qpValues(model)
-                state.reshape(len(b), nq, ns)[np.asarray(els)[:, None], qps[None, :],
-                                              _ORACLE_STATE_INDEX[act.result]] = vals
+                st = state.reshape(len(b), nq, ns)
+                for c, k in enumerate(_ORACLE_STATE_INDEX[act.result]):
+                    st[np.asarray(els)[:, None], qps[None, :], k] = vals[:, :, c]
         r = O.solve_step(om, _oracle_step(model, step), U0=U, state_n=state)
         U, state = r["U"], r["state"]
     return U
