@@ -419,7 +419,8 @@ __device__ __forceinline__ void add_to(VT *vals, int64_t idx, double v)
 template <int NN, int FORM, int MAT, typename ND, typename PG, typename GG, typename PP>
 __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd, PG put_g, GG get_g, PP put_p,
                                          const EvalArgs &A, double Epar, int64_t pos, int q, int NQ, Recipe &r,
-                                         double &wd, const double *state_loaded = nullptr)
+                                         double &wd, const double *state_loaded = nullptr,
+                                         double *state_held = nullptr)
 {
     // converged state first: its loads do not depend on anything below (or the caller has loaded it:
     // state_loaded = plastic strain 6, kappa)
@@ -462,8 +463,13 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
             double ep_t[6], kt;
             const int64_t n = A.n_el;
             ok = j2_recipe(A.mat, Epar, H, dH, ep, kn, ep_t, kt, r);
-            for (int s = 0; s < 6; ++s) A.state_t[((int64_t)s * NQ + q) * n + pos] = ep_t[s];
-            A.state_t[((int64_t)6 * NQ + q) * n + pos] = kt;
+            if (state_held) {  // the caller stores the trial state later
+                for (int s = 0; s < 6; ++s) state_held[s] = ep_t[s];
+                state_held[6] = kt;
+            } else {
+                for (int s = 0; s < 6; ++s) A.state_t[((int64_t)s * NQ + q) * n + pos] = ep_t[s];
+                A.state_t[((int64_t)6 * NQ + q) * n + pos] = kt;
+            }
         }
     } else {
         double F[3][3], Fi[3][3];
@@ -594,6 +600,9 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     double Epar = A.mat.p[0];
     if (active && A.el_param) Epar = A.el_param[A.order[pos]];
     double st[7] = {0, 0, 0, 0, 0, 0, 0};  // converged J2 state of point (le, q = t), loaded with the nodes
+    // its trial state, stored after phase 2: stored at once, the next wait for a load (which on this GPU also
+    // waits for every earlier store) would hold phase 2 until the stores are done
+    double st_t[7] = {0, 0, 0, 0, 0, 0, 0};
     if (active && FORM == FORM_SS && MAT == M_J2)
         for (int s = 0; s < 7; ++s) st[s] = A.state_n[((int64_t)s * NQ + t) * n + pos];
     if (active) {
@@ -646,7 +655,8 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                 dNq[b][2] = 0.125 * f0 * f1 * c2;
             }
         }
-        const bool ok = qp_phase<NN, FORM, MAT>(dNq, 1.0, nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd, st);
+        const bool ok =
+            qp_phase<NN, FORM, MAT>(dNq, 1.0, nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd, st, st_t);
         if (!ok) atomic_min_pos(A.pnewdt, 0.5);
     }
     __syncthreads();  // every lane has read the staged nodes
@@ -785,6 +795,8 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     if (t == 0) s_el[le] = (int32_t)el;
     if (active)
         for (int i = 0; i < 3; ++i) R.fe[el * 24 + 3 * a + i] = fa[i];
+    if (active && FORM == FORM_SS && MAT == M_J2)
+        for (int s = 0; s < 7; ++s) A.state_t[((int64_t)s * NQ + t) * n + pos] = st_t[s];
 
     // records: staged through the (now free) quadrature-point area, 4 elements at a time, then written
     // contiguously per element (consecutive plan positions are mostly consecutive elements)
