@@ -23,7 +23,7 @@ struct ewfe_plan {
     int32_t *d_indptr = nullptr;
     int32_t *d_indices = nullptr;
     int32_t *d_edof = nullptr;    // [npe][n_el]   first dof of element node a, plan order
-    int32_t *d_colpos = nullptr;  // [npe*npe][n_el] column offset of node b's block inside node a's rows
+    int32_t *d_colpos = nullptr;  // [n_el][npe*npe] column offset of node b's block inside node a's rows
     int32_t *d_order = nullptr;   // [n_el] plan position -> element index
 
     // row nodes (nodes with dofs and elements, ascending first dof) and their neighbour blocks: node-block
@@ -71,7 +71,8 @@ struct ewfe_plan {
 namespace ewfe {
 void set_error(const std::string &msg);
 // Builds pattern, colouring and slot map on the host; fills everything but the device pointers.
-// Returns 0 or a negative error code. edof_host / colpos_host receive the SoA device images.
+// Returns 0 or a negative error code. edof_host ([npe][n_el]) and colpos_host ([n_el][npe*npe]) receive the
+// device images.
 struct GatherHost {
     std::vector<int32_t> rownode_dof, inc_ptr, inc, nb_ptr, nbcol, diag_rank, rn_meta;
     std::vector<uint8_t> inc_rank, cl;

@@ -385,7 +385,7 @@ __device__ __forceinline__ void nh_recipe(double mu, double Kb, const double F[3
 // ------------------------------------------------------------------------------------------------
 struct EvalArgs {
     const int32_t *edof;    // [npe][n_el]
-    const int32_t *colpos;  // [npe*npe][n_el]
+    const int32_t *colpos;  // [n_el][npe*npe] (plan position major: one element's offsets are contiguous)
     const int32_t *order;   // [n_el]
     const int32_t *indptr;  // [n_dof+1]
     const double *el_param;  // [n_el] original element order, or null
@@ -1391,15 +1391,19 @@ __global__ __launch_bounds__(64) void k_adjoint8(EvalArgs A, const double *__res
 }
 
 // ------------------------------------------------------------------------------------------------
-// C3D20 / C3D20TL (27 points) and C3D20R / C3D20RTL (8 points): one element per 64-lane workgroup
+// C3D20 / C3D20TL (27 points) and C3D20R / C3D20RTL (8 points): one element per 64-lane workgroup. The
+// 27-point rule is evaluated in two chunks of points (14 + 13) so that the per-point data of a chunk fits
+// 17 KB of LDS, which keeps 8 workgroups per CU (the register limit) instead of 5; the sums over the points
+// keep their order.
 // ------------------------------------------------------------------------------------------------
 template <int FORM, int MAT, typename VT, int NQ>
-__global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_hex20(EvalArgs A)
 {
     constexpr int NN = 20;
+    constexpr int QC = NQ > 16 ? 14 : NQ;  // points per chunk
     using L = QPLayout<NN, FORM>;
     __shared__ double s_nd[9 * NN];      // [field][node]
-    __shared__ double s_qp[L::NF * NQ];  // [field][q]
+    __shared__ double s_qp[L::NF * QC];  // [field][point of the chunk]
     __shared__ int32_t s_dof[NN];
     const int lane = threadIdx.x;
     const int64_t pos = A.p_begin + blockIdx.x;
@@ -1414,92 +1418,97 @@ __global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
             s_nd[(6 + c) * NN + lane] = A.dU[d + c];
         }
     }
-    __syncthreads();
     double Epar = A.mat.p[0];
     if (A.el_param) Epar = A.el_param[A.order[pos]];
-    if (lane < NQ) {
-        const int q = lane;
-        auto nd = [&](int f, int a) { return s_nd[f * NN + a]; };
-        auto put_g = [&](int a, int j, double v) { s_qp[(L::G + 3 * a + j) * NQ + q] = v; };
-        auto get_g = [&](int a, int j) { return s_qp[(L::G + 3 * a + j) * NQ + q]; };
-        auto put_p = [&](int a, int j, double v) { s_qp[(L::P + 3 * a + j) * NQ + q] = v; };
-        Recipe r;
-        double wd;
-        const bool ok =
-            qp_phase<NN, FORM, MAT>(qp_dN<NN, NQ>(q), qp_w<NN, NQ>(q), nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r,
-                                    wd);
-        if (!ok) atomic_min_pos(A.pnewdt, 0.5);
-        auto put = [&](int f, double v) { s_qp[f * NQ + q] = v; };
-        put(L::COEF + 0, wd * r.alpha);
-        put(L::COEF + 1, wd * r.beta);
-        put(L::COEF + 2, wd * r.gam);
-        put(L::COEF + 3, wd * r.c);
-        for (int i = 0; i < 6; ++i) put(L::NV + i, (MAT == M_J2) ? r.N[i] : 0.0);
-        for (int i = 0; i < 6; ++i) put(L::S + i, wd * r.S[i]);
-    }
-    __syncthreads();
-    if (lane >= 3 * NN) return;
     // lane = (a, i): row i of node a, for the blocks K_ab with b = a + j (mod 20), j < nb: every unordered node
     // pair once (j = 10 only from a < 10); the lane also writes the transposed entries K_ba = K_ab^T
-    const int a = lane / 3, i = lane % 3;
+    const bool row_lane = lane < 3 * NN;
+    const int a = row_lane ? lane / 3 : 0, i = lane % 3;
     constexpr int kJ = NN / 2 + 1;  // 11
     const int nb = (a < NN / 2) ? kJ : kJ - 1;
     double K[kJ][3];
     for (int j = 0; j < kJ; ++j)
         for (int k = 0; k < 3; ++k) K[j][k] = 0.0;
     double fi = 0.0;
-    auto ld = [&](int f, int q) { return s_qp[f * NQ + q]; };
-    for (int q = 0; q < NQ; ++q) {
-        const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q), cc = ld(L::COEF + 3, q);
-        double pa[3], ra[3], Nq[6], vai = 0.0;
-        for (int m = 0; m < 3; ++m) {
-            ra[m] = ld(L::G + 3 * a + m, q);
-            pa[m] = ld(L::P + 3 * a + m, q);
+    for (int q0 = 0; q0 < NQ; q0 += QC) {
+        const int nq = NQ - q0 < QC ? NQ - q0 : QC;
+        __syncthreads();  // the nodes are staged, or the previous chunk's points are summed
+        if (lane < nq) {
+            const int ql = lane, q = q0 + lane;
+            auto nd = [&](int f, int b) { return s_nd[f * NN + b]; };
+            auto put_g = [&](int b, int j, double v) { s_qp[(L::G + 3 * b + j) * QC + ql] = v; };
+            auto get_g = [&](int b, int j) { return s_qp[(L::G + 3 * b + j) * QC + ql]; };
+            auto put_p = [&](int b, int j, double v) { s_qp[(L::P + 3 * b + j) * QC + ql] = v; };
+            Recipe r;
+            double wd;
+            const bool ok = qp_phase<NN, FORM, MAT>(qp_dN<NN, NQ>(q), qp_w<NN, NQ>(q), nd, put_g, get_g, put_p, A,
+                                                    Epar, pos, q, NQ, r, wd);
+            if (!ok) atomic_min_pos(A.pnewdt, 0.5);
+            auto put = [&](int f, double v) { s_qp[f * QC + ql] = v; };
+            put(L::COEF + 0, wd * r.alpha);
+            put(L::COEF + 1, wd * r.beta);
+            put(L::COEF + 2, wd * r.gam);
+            put(L::COEF + 3, wd * r.c);
+            for (int m = 0; m < 6; ++m) put(L::NV + m, (MAT == M_J2) ? r.N[m] : 0.0);
+            for (int m = 0; m < 6; ++m) put(L::S + m, wd * r.S[m]);
         }
-        {
-            double Srow[3];
-            const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
-            const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
-            Srow[0] = (i == 0) ? S0 : (i == 1 ? S3 : S4);
-            Srow[1] = (i == 0) ? S3 : (i == 1 ? S1 : S5);
-            Srow[2] = (i == 0) ? S4 : (i == 1 ? S5 : S2);
-            fi += Srow[0] * pa[0] + Srow[1] * pa[1] + Srow[2] * pa[2];
-        }
-        if (MAT == M_J2) {
-            for (int m = 0; m < 6; ++m) Nq[m] = ld(L::NV + m, q);
-            const double Nr0 = (i == 0) ? Nq[0] : (i == 1 ? Nq[3] : Nq[4]);
-            const double Nr1 = (i == 0) ? Nq[3] : (i == 1 ? Nq[1] : Nq[5]);
-            const double Nr2 = (i == 0) ? Nq[4] : (i == 1 ? Nq[5] : Nq[2]);
-            vai = cc * (Nr0 * pa[0] + Nr1 * pa[1] + Nr2 * pa[2]);
-        }
-        const double Aai = al * pa[i];
-        double Ba[3];
-        for (int m = 0; m < 3; ++m) Ba[m] = be * pa[m];
+        __syncthreads();
+        if (!row_lane) continue;
+        auto ld = [&](int f, int q) { return s_qp[f * QC + q]; };
+        for (int q = 0; q < nq; ++q) {
+            const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q),
+                         cc = ld(L::COEF + 3, q);
+            double pa[3], ra[3], Nq[6], vai = 0.0;
+            for (int m = 0; m < 3; ++m) {
+                ra[m] = ld(L::G + 3 * a + m, q);
+                pa[m] = ld(L::P + 3 * a + m, q);
+            }
+            {
+                double Srow[3];
+                const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+                const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+                Srow[0] = (i == 0) ? S0 : (i == 1 ? S3 : S4);
+                Srow[1] = (i == 0) ? S3 : (i == 1 ? S1 : S5);
+                Srow[2] = (i == 0) ? S4 : (i == 1 ? S5 : S2);
+                fi += Srow[0] * pa[0] + Srow[1] * pa[1] + Srow[2] * pa[2];
+            }
+            if (MAT == M_J2) {
+                for (int m = 0; m < 6; ++m) Nq[m] = ld(L::NV + m, q);
+                const double Nr0 = (i == 0) ? Nq[0] : (i == 1 ? Nq[3] : Nq[4]);
+                const double Nr1 = (i == 0) ? Nq[3] : (i == 1 ? Nq[1] : Nq[5]);
+                const double Nr2 = (i == 0) ? Nq[4] : (i == 1 ? Nq[5] : Nq[2]);
+                vai = cc * (Nr0 * pa[0] + Nr1 * pa[1] + Nr2 * pa[2]);
+            }
+            const double Aai = al * pa[i];
+            double Ba[3];
+            for (int m = 0; m < 3; ++m) Ba[m] = be * pa[m];
 #pragma unroll
-        for (int j = 0; j < kJ; ++j) {
-            if (j < nb) {
-                const int b = (a + j) % NN;
-                double pb[3], rb[3];
-                for (int m = 0; m < 3; ++m) {
-                    rb[m] = ld(L::G + 3 * b + m, q);
-                    pb[m] = ld(L::P + 3 * b + m, q);
-                }
-                const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
-                double vb[3] = {0, 0, 0};
-                if (MAT == M_J2) {
-                    vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
-                    vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
-                    vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
-                }
-                for (int k = 0; k < 3; ++k) {
-                    double v = Aai * pb[k] + Ba[k] * pb[i];
-                    if (MAT == M_J2) v -= vai * vb[k];
-                    if (i == k) v += d;
-                    K[j][k] += v;
+            for (int j = 0; j < kJ; ++j) {
+                if (j < nb) {
+                    const int b = (a + j) % NN;
+                    double pb[3], rb[3];
+                    for (int m = 0; m < 3; ++m) {
+                        rb[m] = ld(L::G + 3 * b + m, q);
+                        pb[m] = ld(L::P + 3 * b + m, q);
+                    }
+                    const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                    double vb[3] = {0, 0, 0};
+                    if (MAT == M_J2) {
+                        vb[0] = Nq[0] * pb[0] + Nq[3] * pb[1] + Nq[4] * pb[2];
+                        vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
+                        vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        double v = Aai * pb[k] + Ba[k] * pb[i];
+                        if (MAT == M_J2) v -= vai * vb[k];
+                        if (i == k) v += d;
+                        K[j][k] += v;
+                    }
                 }
             }
         }
     }
+    if (!row_lane) return;
     const int32_t row = s_dof[a] + i;
     A.P[row] -= fi;
     A.F[row] += fabs(fi);
@@ -1509,10 +1518,10 @@ __global__ __launch_bounds__(64) void k_hex20(EvalArgs A)
     for (int j = 0; j < kJ; ++j) {
         if (j < nb) {
             const int b = (a + j) % NN;
-            const int32_t cab = A.colpos[(int64_t)(a * NN + b) * n + pos];
+            const int32_t cab = A.colpos[pos * (NN * NN) + a * NN + b];
             for (int k = 0; k < 3; ++k) add_to(vals, rs + cab + k, K[j][k]);
             if (j > 0) {  // K_ba[k][i] = K_ab[i][k] into row k of node b, column i of node a
-                const int32_t cba = A.colpos[(int64_t)(b * NN + a) * n + pos];
+                const int32_t cba = A.colpos[pos * (NN * NN) + b * NN + a];
                 for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)A.indptr[s_dof[b] + k] + cba + i, K[j][k]);
             }
         }

@@ -345,7 +345,7 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
     }
 
 
-    // --- SoA device images: element dof bases and the node-pair slot map -------------------------
+    // --- device images: element dof bases (SoA) and the node-pair slot map (element-major) -------
     edof_host.resize((size_t)npe * n_el);
     const bool need_colpos = npe != 8;  // the 8-node path uses the row-node gather lists instead
     colpos_host.resize(need_colpos ? (size_t)npe * npe * n_el : 0);
@@ -362,7 +362,7 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
                     const int32_t db = (int32_t)node_dof[conn[el * npe + bb]];
                     const int32_t *it = std::lower_bound(lo, hi, db);
                     if (it == hi || *it != db) bad = 1;
-                    colpos_host[((size_t)a * npe + bb) * n_el + p] = (int32_t)(3 * (it - lo));
+                    colpos_host[((size_t)p * npe + a) * npe + bb] = (int32_t)(3 * (it - lo));
                 }
             }
         }
