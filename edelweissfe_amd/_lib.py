@@ -38,6 +38,7 @@ SIGNATURES = {
     "ewfe_state_size": (ctypes.c_int, [ctypes.c_int]),
     "ewfe_plan_info": (ctypes.c_int, [_vp, _P(_i64), _P(_i32), _P(_i32), _P(_i64)]),
     "ewfe_plan_destroy": (ctypes.c_int, [_vp]),
+    "ewfe_plan_set_materials": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
     "ewfe_eval_assemble": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                           ctypes.c_int, _vp, _vp]),
     "ewfe_adjoint_param_grad": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

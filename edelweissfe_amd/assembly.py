@@ -61,6 +61,10 @@ class ElementAssembly:
         self.matParams = mp
         self.elParams = (None if block.elParams is None else
                          torch.as_tensor(block.elParams, dtype=torch.float64, device=self.device))
+        if block.materialIndex is not None:  # several materials: the plan's per-element parameter sets
+            table = np.ascontiguousarray(block.materialParameterTable())
+            check(L.ewfe_plan_set_materials(plan, len(table), ptr(table),
+                                            ptr(np.ascontiguousarray(block.materialIndex, dtype=np.int32))))
         self.nState = L.ewfe_state_size(self.matKind)
         nInt = block.nInt
         if self.nState > 0:

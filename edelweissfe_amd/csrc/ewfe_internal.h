@@ -47,6 +47,9 @@ struct ewfe_plan {
     int64_t fweights_cap = 0;          // listed elements d_fweights holds
     int32_t *d_fslot = nullptr;        // surface-load scratch: element -> list slot, -1 if not listed (lazy)
     int32_t *d_pos_of_el = nullptr;    // element -> plan position (lazy)
+    double *d_mtab = nullptr;          // per-element material sets [n_mat][8], or null
+    int32_t *d_mid = nullptr;          // [n_el] set of the element at each plan position
+    int32_t n_mat = 0;
     int32_t *d_gdesc = nullptr;        // [n_rownodes][4] gather descriptors in gather order
     int32_t *d_gblob = nullptr;        // gather blobs
     // element records (allocated on first evaluation): 36 unique 3x3 blocks per element, and nodal forces

@@ -389,6 +389,8 @@ struct EvalArgs {
     const int32_t *order;   // [n_el]
     const int32_t *indptr;  // [n_dof+1]
     const double *el_param;  // [n_el] original element order, or null
+    const double *mtab;      // per-element material sets [n_mat][8], or null (then mat holds the parameters)
+    const int32_t *mid;      // [n_el] set of the element at each plan position (with mtab)
     const double *X, *U, *dU;
     const double *state_n;
     double *state_t;
@@ -399,6 +401,17 @@ struct EvalArgs {
     int64_t p_begin, p_end;
     MatParams mat;
 };
+
+// parameters of the element at plan position pos: its material's set when the plan holds a table
+__device__ __forceinline__ MatParams mat_at(const EvalArgs &A, int64_t pos)
+{
+    MatParams m = A.mat;
+    if (A.mtab) {
+        const double *t = A.mtab + 8 * (int64_t)A.mid[pos];
+        for (int i = 0; i < 8; ++i) m.p[i] = t[i];
+    }
+    return m;
+}
 
 __device__ __forceinline__ void atomic_min_pos(double *addr, double v)
 {
@@ -418,7 +431,8 @@ __device__ __forceinline__ void add_to(VT *vals, int64_t idx, double v)
 // p = g and put_p is not used.
 template <int NN, int FORM, int MAT, typename ND, typename PG, typename GG, typename PP>
 __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd, PG put_g, GG get_g, PP put_p,
-                                         const EvalArgs &A, double Epar, int64_t pos, int q, int NQ, Recipe &r,
+                                         const EvalArgs &A, const MatParams &M, double Epar, int64_t pos, int q,
+                                         int NQ, Recipe &r,
                                          double &wd, const double *state_loaded = nullptr,
                                          double *state_held = nullptr)
 {
@@ -458,11 +472,11 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
     bool ok = true;
     if (FORM == FORM_SS) {
         if (MAT == M_LE) {
-            le_recipe(A.mat, Epar, H, r);
+            le_recipe(M, Epar, H, r);
         } else {
             double ep_t[6], kt;
             const int64_t n = A.n_el;
-            ok = j2_recipe(A.mat, Epar, H, dH, ep, kn, ep_t, kt, r);
+            ok = j2_recipe(M, Epar, H, dH, ep, kn, ep_t, kt, r);
             if (state_held) {  // the caller stores the trial state later
                 for (int s = 0; s < 6; ++s) state_held[s] = ep_t[s];
                 state_held[6] = kt;
@@ -476,7 +490,7 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
         const double Jd = inv3(F, Fi);
-        nh_recipe(Epar, A.mat.p[1], F, Jd, r);
+        nh_recipe(Epar, M.p[1], F, Jd, r);
         // spatial gradients NA_a = F^-T g_a
         for (int a = 0; a < NN; ++a) {
             const double g0 = get_g(a, 0), g1 = get_g(a, 1), g2 = get_g(a, 2);
@@ -597,7 +611,8 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     constexpr int NVN = (MAT == M_J2) ? 9 : 6;  // staged values per node
     static_assert(EPB * NN * NVN <= (L::NF - L::AL) * 64, "nodal staging must fit in the last fields");
     double *s_nd = s_qp + L::AL * 64;
-    double Epar = A.mat.p[0];
+    const MatParams M = mat_at(A, active ? pos : q0);
+    double Epar = M.p[0];
     if (active && A.el_param) Epar = A.el_param[A.order[pos]];
     double st[7] = {0, 0, 0, 0, 0, 0, 0};  // converged J2 state of point (le, q = t), loaded with the nodes
     // its trial state, stored after phase 2: stored at once, the next wait for a load (which on this GPU also
@@ -656,7 +671,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             }
         }
         const bool ok =
-            qp_phase<NN, FORM, MAT>(dNq, 1.0, nd, put_g, get_g, put_p, A, Epar, pos, q, NQ, r, wd, st, st_t);
+            qp_phase<NN, FORM, MAT>(dNq, 1.0, nd, put_g, get_g, put_p, A, M, Epar, pos, q, NQ, r, wd, st, st_t);
         if (!ok) atomic_min_pos(A.pnewdt, 0.5);
     }
     __syncthreads();  // every lane has read the staged nodes
@@ -712,7 +727,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                 }
             }
         }
-        const double nu = A.mat.p[1];
+        const double nu = M.p[1];
         const double ratio = 2.0 * nu / (1.0 - 2.0 * nu);  // lambda / mu
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
@@ -1268,7 +1283,8 @@ __global__ __launch_bounds__(256) void k_qp_results(EvalArgs A, const double *__
             for (int j = 0; j < 3; ++j) H[i][j] += A.U[dof[a] + i] * g[j];
     }
     double v[6];
-    const double Epar = A.el_param ? A.el_param[e] : A.mat.p[0];
+    const MatParams M = mat_at(A, p);
+    const double Epar = A.el_param ? A.el_param[e] : M.p[0];
     if (FORM == FORM_SS) {
         const double eps[6] = {H[0][0], H[1][1], H[2][2], H[0][1] + H[1][0], H[0][2] + H[2][0], H[1][2] + H[2][1]};
         if (what == 1) {
@@ -1278,7 +1294,7 @@ __global__ __launch_bounds__(256) void k_qp_results(EvalArgs A, const double *__
             for (int i = 0; i < 6; ++i) ee[i] = eps[i];
             if (MAT == M_J2)
                 for (int s = 0; s < 6; ++s) ee[s] -= state[((int64_t)s * NQ + q) * n + p];
-            const double nu = A.mat.p[1];
+            const double nu = M.p[1];
             const double f = Epar / ((1 + nu) * (1 - 2 * nu));
             const double lam = f * nu, mu = f * (1 - 2 * nu) / 2;
             const double tr = ee[0] + ee[1] + ee[2];
@@ -1299,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_qp_results(EvalArgs A, const double *__
         } else {
             const double Jd = inv3(F, Fi);
             Recipe r;
-            nh_recipe(Epar, A.mat.p[1], F, Jd, r);  // r.S: [11, 22, 33, 12, 13, 23]
+            nh_recipe(Epar, M.p[1], F, Jd, r);  // r.S: [11, 22, 33, 12, 13, 23]
             const double o[6] = {r.S[0], r.S[1], r.S[2], r.S[3], r.S[5], r.S[4]};
             for (int i = 0; i < 6; ++i) v[i] = o[i];
         }
@@ -1355,7 +1371,8 @@ __global__ __launch_bounds__(64) void k_adjoint8(EvalArgs A, const double *__res
                 for (int j = 0; j < 3; ++j) H[i][j] += nd(3 + i, a) * g[a][j];
         }
         double D[3][3];  // d(stress)/d(theta), symmetric
-        double theta = A.mat.p[0];
+        const MatParams M = mat_at(A, pos);
+        double theta = M.p[0];
         if (A.el_param) theta = A.el_param[A.order[pos]];
         if (FORM == FORM_TL) {
             double F[3][3], Fi[3][3];
@@ -1374,7 +1391,7 @@ __global__ __launch_bounds__(64) void k_adjoint8(EvalArgs A, const double *__res
             }
         } else {
             Recipe r;
-            le_recipe(A.mat, theta, H, r);
+            le_recipe(M, theta, H, r);
             const double S[3][3] = {{r.S[0], r.S[3], r.S[4]}, {r.S[3], r.S[1], r.S[5]}, {r.S[4], r.S[5], r.S[2]}};
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j) D[i][j] = S[i][j] / theta;
@@ -1418,7 +1435,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             s_nd[(6 + c) * NN + lane] = A.dU[d + c];
         }
     }
-    double Epar = A.mat.p[0];
+    const MatParams M = mat_at(A, pos);
+    double Epar = M.p[0];
     if (A.el_param) Epar = A.el_param[A.order[pos]];
     // lane = (a, i): row i of node a, for the blocks K_ab with b = a + j (mod 20), j < nb: every unordered node
     // pair once (j = 10 only from a < 10); the lane also writes the transposed entries K_ba = K_ab^T
@@ -1442,7 +1460,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             Recipe r;
             double wd;
             const bool ok = qp_phase<NN, FORM, MAT>(qp_dN<NN, NQ>(q), qp_w<NN, NQ>(q), nd, put_g, get_g, put_p, A,
-                                                    Epar, pos, q, NQ, r, wd);
+                                                    M, Epar, pos, q, NQ, r, wd);
             if (!ok) atomic_min_pos(A.pnewdt, 0.5);
             auto put = [&](int f, double v) { s_qp[f * QC + ql] = v; };
             put(L::COEF + 0, wd * r.alpha);
@@ -1782,6 +1800,36 @@ extern "C" int ewfe_plan_info(const ewfe_plan *p, int64_t *n_el, int32_t *npe, i
     return 0;
 }
 
+extern "C" int ewfe_plan_set_materials(ewfe_plan *p, int32_t n_mat, const double *params_host,
+                                       const int32_t *material_of_element_host)
+{
+    if (!p || n_mat < 0 || (n_mat > 0 && (!params_host || !material_of_element_host))) {
+        ewfe::set_error("ewfe_plan_set_materials: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    if (p->d_mtab) HIP_CHECK(hipFree(p->d_mtab));
+    if (p->d_mid) HIP_CHECK(hipFree(p->d_mid));
+    p->d_mtab = nullptr;
+    p->d_mid = nullptr;
+    p->n_mat = 0;
+    if (n_mat == 0) return 0;
+    std::vector<int32_t> mid(p->n_el);  // by plan position
+    for (int64_t i = 0; i < p->n_el; ++i) {
+        const int32_t m = material_of_element_host[p->order[i]];
+        if (m < 0 || m >= n_mat) {
+            ewfe::set_error("ewfe_plan_set_materials: material index outside [0, n_mat)");
+            return EWFE_EINVAL;
+        }
+        mid[i] = m;
+    }
+    HIP_CHECK(hipMalloc((void **)&p->d_mtab, sizeof(double) * 8 * (size_t)n_mat));
+    HIP_CHECK(hipMalloc((void **)&p->d_mid, sizeof(int32_t) * (size_t)std::max<int64_t>(p->n_el, 1)));
+    HIP_CHECK(hipMemcpy(p->d_mtab, params_host, sizeof(double) * 8 * (size_t)n_mat, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(p->d_mid, mid.data(), sizeof(int32_t) * (size_t)p->n_el, hipMemcpyHostToDevice));
+    p->n_mat = n_mat;
+    return 0;
+}
+
 extern "C" int ewfe_plan_destroy(ewfe_plan *p)
 {
     if (!p) return 0;
@@ -1789,7 +1837,7 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
                     p->d_rownode_dof, p->d_gdesc, p->d_gblob, p->d_kbuf,
                     p->d_fe,       p->d_nb_ptr,  p->d_nbcol,   p->d_diag_rank, p->d_rn_meta,
                     p->d_rorder,   p->d_ninc_ptr, p->d_ninc,   p->d_bweights, p->d_fweights, p->d_pos_of_el,
-                    p->d_fslot};
+                    p->d_fslot,    p->d_mtab,     p->d_mid};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
     for (void *ev : p->chunk_events)
@@ -1961,6 +2009,8 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
     A.order = p->d_order;
     A.indptr = p->d_indptr;
     A.el_param = el_param_dev;
+    A.mtab = p->n_mat ? p->d_mtab : nullptr;
+    A.mid = p->d_mid;
     A.X = X_dev;
     A.U = U_np_dev;
     A.dU = dU_dev;
@@ -2003,6 +2053,8 @@ extern "C" int ewfe_adjoint_param_grad(const ewfe_plan *p, int mat_kind, const d
     A.edof = p->d_edof;
     A.order = p->d_order;
     A.el_param = el_param_dev;
+    A.mtab = p->n_mat ? p->d_mtab : nullptr;
+    A.mid = p->d_mid;
     A.X = X_dev;
     A.U = U_dev;
     A.n_el = p->n_el;
@@ -2100,6 +2152,8 @@ extern "C" int ewfe_qp_results(const ewfe_plan *p, int mat_kind, const double *m
     A.edof = p->d_edof;
     A.order = p->d_order;
     A.el_param = el_param_dev;
+    A.mtab = p->n_mat ? p->d_mtab : nullptr;
+    A.mid = p->d_mid;
     A.X = X_dev;
     A.U = U_dev;
     A.n_el = p->n_el;

@@ -266,23 +266,26 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
     fields = _analyticalFields(inputFile)
     model.analyticalFields = fields
     model.fieldOutputs = _fieldOutputs(inputFile)
+    # every section assigns its material to the elements of its sets (sectionbase.py:84-112); the block keeps
+    # the distinct materials (of one kind) and every element's index
     assigned = np.zeros(len(block), dtype=bool)
-    chosen = None
+    used, matIndex = [], np.zeros(len(block), dtype=np.int32)
     elParams = None
     for sec in inputFile["*section"]:
         if sec.get("type", "solid").lower() != "solid":
             raise NotImplementedError(f"section type {sec.get('type')} is not part of the GPU path")
         mat = materials[sec["material"]]
+        if not any(mat is m for m in used):
+            used.append(mat)
+        mi = next(i for i, m in enumerate(used) if m is mat)
         defs, setNames = _sectionLines(sec, fields)
         for setName in setNames:
             _, idx = model.elementSets[setName]
-            if chosen is not None and mat is not chosen:
-                raise NotImplementedError("one material per model on the GPU path (one element block)")
-            chosen = mat
             assigned[idx] = True
+            matIndex[idx] = mi
             if defs:  # sectionbase.py:86-128: the parameter of each element from the fields at its centre
                 if elParams is None:
-                    elParams = np.full(len(block), float(mat.materialProperties[0]))
+                    elParams = np.full(len(block), np.nan)
                 centres = model.nodeCoordinates[block.conn[idx]].mean(axis=1)  # getCoordinatesAtCenter
                 for e, c in zip(idx, centres):
                     value = float(mat.materialProperties[0])
@@ -290,10 +293,12 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
                         f = field(*c)
                         value = expr(value, f) if typ == "settovalue" else value * expr(value, f)
                     elParams[e] = value
-    if chosen is None or not assigned.all():
+    if not used or not assigned.all():
         raise ValueError("every element needs a section with a material")
-    block.setMaterial(chosen)
+    block.setMaterials(used, matIndex if len(used) > 1 else None)
     if elParams is not None:
+        unset = np.isnan(elParams)  # elements of sections without fields keep their material's value
+        elParams[unset] = block.materialParameterTable()[matIndex[unset], 0]
         block.elParams = np.ascontiguousarray(elParams)
 
     # element-based surfaces: boxGen's, then data lines 'elSet, face' with face 1..6 or S1..S6

@@ -66,12 +66,16 @@ class Material:
 
 
 class ElementBlock:
-    """All elements of one type sharing one material (a section applied to an element set).
+    """All elements of one type with materials of one kind (sections applied to element sets).
 
-    conn      int32 [nEl, nNodes]   model node indices, element-local (Abaqus) node order
-    labels    int64 [nEl]           element labels
-    elParams  float64 [nEl] or None per-element value of the material's first property
-              (section ``materialParameterFromField``, sectionbase.py:86-128)
+    conn           int32 [nEl, nNodes]   model node indices, element-local (Abaqus) node order
+    labels         int64 [nEl]           element labels
+    elParams       float64 [nEl] or None per-element value of the material's first property
+                   (section ``materialParameterFromField``, sectionbase.py:86-128)
+    materials      list of Material      the block's materials (one per section), all of one kind;
+                   ``material`` is the first
+    materialIndex  int32 [nEl] or None   the material of every element (None: all take ``material``;
+                   every section assigns its material to its elements, sectionbase.py:84-112)
     """
 
     def __init__(self, elType: str, conn, labels=None, material: Material | None = None, elParams=None):
@@ -89,6 +93,8 @@ class ElementBlock:
         self.nInt = lib["nInt"]
         self.labels = np.arange(1, len(self.conn) + 1) if labels is None else np.asarray(labels)
         self.material = material
+        self.materials = [] if material is None else [material]
+        self.materialIndex = None
         self.elParams = None if elParams is None else np.ascontiguousarray(elParams, dtype=np.float64)
         self.fields = [["displacement"]] * self.nNodes
         # quadrature-point state (device tensors, plan order), owned here like the reference's elements own
@@ -105,10 +111,43 @@ class ElementBlock:
         return self.material is not None
 
     def setMaterial(self, material: Material):
-        if self.elType.endswith("TL") != (material.name == "neohookeanwa"):
-            raise AttributeError(f"{self.elType} cannot take material {material.name}: small-strain elements take "
-                                 "linearelastic / vonmises, total-Lagrangian elements take neohookeanwa")
-        self.material = material
+        self.setMaterials([material])
+
+    def setMaterials(self, materials, materialIndex=None):
+        """Materials of one kind and, with several, the material index of every element."""
+        materials = list(materials)
+        if not materials:
+            raise ValueError("a block needs at least one material")
+        for material in materials:
+            if self.elType.endswith("TL") != (material.name == "neohookeanwa"):
+                raise AttributeError(f"{self.elType} cannot take material {material.name}: small-strain elements "
+                                     "take linearelastic / vonmises, total-Lagrangian elements take neohookeanwa")
+        if len({m.name for m in materials}) != 1:
+            raise NotImplementedError("one element block takes materials of one kind (one kernel per evaluation)")
+        if len(materials) > 1:
+            idx = np.ascontiguousarray(materialIndex, dtype=np.int32)
+            if idx.shape != (len(self),) or idx.min() < 0 or idx.max() >= len(materials):
+                raise ValueError(f"materialIndex must hold one index in [0, {len(materials)}) per element")
+            self.materialIndex = idx
+        else:
+            self.materialIndex = None
+        self.materials = materials
+        self.material = materials[0]
+
+    def materialParameterTable(self):
+        """float64 [nMaterials, 8]: the parameter sets of the block's materials (zero padded)."""
+        t = np.zeros((len(self.materials), 8))
+        for i, m in enumerate(self.materials):
+            t[i, : min(8, len(m.materialProperties))] = m.materialProperties[:8]
+        return t
+
+    def elementMaterialProperties(self):
+        """float64 [nEl, 8]: every element's parameters, the per-element first property included."""
+        t = self.materialParameterTable()
+        out = t[self.materialIndex] if self.materialIndex is not None else np.repeat(t[:1], len(self), axis=0)
+        if self.elParams is not None:
+            out[:, 0] = self.elParams
+        return out
 
     def acceptLastState(self):
         """state_n <- state_trial (marmotelement/element.pyx:278-281)."""

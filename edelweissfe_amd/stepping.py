@@ -424,14 +424,13 @@ class SetField:
             if U is not None:
                 Un[used] = U.detach().cpu().numpy().reshape(-1, 3)
             eps = quadraturePointStrains(block, model.nodeCoordinates, Un)[els][:, qps]  # [e, q, 6]
-            mp = block.material.materialProperties
-            E = np.full(len(els), float(mp[0])) if block.elParams is None else block.elParams[els]
-            nu = float(mp[1])
+            mp = block.elementMaterialProperties()[els]
+            E, nu = mp[:, 0, None], mp[:, 1, None]
             s = vals
             e_el = np.empty_like(s)
             for i in range(3):
-                e_el[..., i] = (s[..., i] - nu * (s[..., (i + 1) % 3] + s[..., (i + 2) % 3])) / E[:, None]
-            e_el[..., 3:] = s[..., 3:] * (2 * (1 + nu) / E)[:, None, None]
+                e_el[..., i] = (s[..., i] - nu * (s[..., (i + 1) % 3] + s[..., (i + 2) % 3])) / E
+            e_el[..., 3:] = s[..., 3:] * (2 * (1 + nu) / E)[:, :, None]
             ep = eps - e_el
             for j, q in enumerate(qps):
                 st[0:6, q, pos[els]] = ep[:, j, :].T

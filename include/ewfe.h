@@ -114,6 +114,16 @@ int ewfe_plan_info(const ewfe_plan *plan, int64_t *n_el, int32_t *nodes_per_el, 
                    int64_t *n_dof);
 int ewfe_plan_destroy(ewfe_plan *plan);
 
+/* Per-element material parameter sets (several sections with materials of one kind, e.g. two linear
+ * elastic materials on two element sets: every section assigns its material to the elements of its sets,
+ * ew/sections/base/sectionbase.py:84-112, ew/generators/abqmodelconstructor.py:326-370). params_host holds
+ * n_mat sets of 8 doubles (the layout of mat_params_host below); material_of_element_host[e] (original element
+ * order) selects element e's set. Once set, every evaluation, quadrature-point result and adjoint call of the
+ * plan takes each element's parameters from its set and ignores mat_params_host; el_param_dev still overrides
+ * the first parameter. n_mat = 0 clears the table. Fails with EWFE_EINVAL on an index outside [0, n_mat). */
+int ewfe_plan_set_materials(ewfe_plan *plan, int32_t n_mat, const double *params_host,
+                            const int32_t *material_of_element_host);
+
 /* One Newton iteration's element evaluation + assembly (the reference zeroes P, F and the CSR values
  * first, nonlinearimplicitstatic.py:419, csrgenerator.pyx:111; so does this call).
  *   mat_params_host  f64[8]  linear elastic [E, nu]; von Mises [E, nu, fy0, H, dfy, delta];

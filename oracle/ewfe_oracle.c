@@ -423,6 +423,8 @@ static double now_s(void)
  *  conn      int32 [n_el * nn] node indices (element-local order)
  *  X         f64 [n_nodes * 3]
  *  edofs     int64 [n_el * nd] element dof index list (dofmanager.py:445-471)
+ *  el_mp8    optional f64 [n_el * 8] per-element parameter sets (several materials, each element its
+ *            section's material, sectionbase.py:84-112), replacing mp; or NULL
  *  el_mp     optional per-element override of mp[0] (mu_e / E_e), or NULL
  *  V         f64 [n_el * nd * nd] VIJ values
  *  x_slot    int32 [n_el * nd * nd] VIJ position -> CSR slot (csrgenerator.pyx:78-97), or NULL
@@ -430,7 +432,7 @@ static double now_s(void)
  *  times[2]  out: seconds in "elements" (loop + scatter) and "CSR generation"
  * Returns 0, or the (1-based) index of the first element that requested a cutback.
  */
-int64_t oracle_eval_assemble(int ek, int mk, const double *mp, const double *el_mp, int64_t n_el,
+int64_t oracle_eval_assemble(int ek, int mk, const double *mp, const double *el_mp8, const double *el_mp, int64_t n_el,
                              const int32_t *conn, const double *X, const int64_t *edofs, const double *U_np,
                              const double *dU, const double *state_n, double *state_t, int nstate, double *P,
                              double *F, double *V, const int32_t *x_slot, int64_t nnz, double *csr_data,
@@ -452,7 +454,7 @@ int64_t oracle_eval_assemble(int ek, int mk, const double *mp, const double *el_
             Ue[k] = U_np[edofs[e * nd + k]];
             dUe[k] = dU[edofs[e * nd + k]];
         }
-        for (int i = 0; i < 8; ++i) mpe[i] = mp[i];
+        for (int i = 0; i < 8; ++i) mpe[i] = el_mp8 ? el_mp8[e * 8 + i] : mp[i];
         if (el_mp) mpe[0] = el_mp[e];
         double pdt = 1e36;
         oracle_element(ek, mk, mpe, Xe, Ue, dUe, state_n + e * nq * nstate, state_t + e * nq * nstate, nstate,

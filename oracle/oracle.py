@@ -67,7 +67,7 @@ def lib():
         L.oracle_element.argtypes = [ctypes.c_int, ctypes.c_int, d, d, d, d, d, d, ctypes.c_int, d, d, d]
         L.oracle_eval_assemble.restype = ctypes.c_int64
         L.oracle_eval_assemble.argtypes = [
-            ctypes.c_int, ctypes.c_int, d, d, ctypes.c_int64, i32, d, i64, d, d, d, d, ctypes.c_int, d, d, d, i32,
+            ctypes.c_int, ctypes.c_int, d, d, d, ctypes.c_int64, i32, d, i64, d, d, d, d, ctypes.c_int, d, d, d, i32,
             ctypes.c_int64, d, ctypes.c_int, d, d,
         ]
         L.oracle_body_force.restype = None
@@ -203,9 +203,10 @@ def element(el_kind, mat_kind, mp, Xe, Ue, dUe, state_n):
 
 
 def eval_assemble(el_kind, mat_kind, mp, conn, X, edofs, U_np, dU, state_n, x_slot=None, nnz=0, el_mp=None,
-                  n_threads=1, return_vij=False):
+                  n_threads=1, return_vij=False, el_mp8=None):
     """NISTParallel.computeElements + CSRGenerator.updateCSR. Returns dict(P, F, V?, data, state, pNewDT,
-    failed, t_elements, t_csr)."""
+    failed, t_elements, t_csr). el_mp8 [nEl, 8]: per-element parameter sets replacing mp (several materials);
+    el_mp [nEl]: per-element first parameter."""
     ek, mk = EL_KINDS[el_kind], MAT_KINDS[mat_kind]
     nEl, nn = conn.shape
     nd = 3 * nn
@@ -221,8 +222,13 @@ def eval_assemble(el_kind, mat_kind, mp, conn, X, edofs, U_np, dU, state_n, x_sl
     conn = np.ascontiguousarray(conn, dtype=np.int32)
     edofs = np.ascontiguousarray(edofs, dtype=np.int64)
     el_mp = None if el_mp is None else np.ascontiguousarray(el_mp, dtype=np.float64)
+    if el_mp8 is not None:
+        full = np.zeros((nEl, 8))
+        el_mp8 = np.asarray(el_mp8, dtype=np.float64)
+        full[:, : el_mp8.shape[1]] = el_mp8[:, :8]
+        el_mp8 = np.ascontiguousarray(full)
     failed = lib().oracle_eval_assemble(
-        ek, mk, _p(mp8), _p(el_mp), nEl, _p(conn, ctypes.c_int32), _p(np.ascontiguousarray(X, dtype=np.float64)),
+        ek, mk, _p(mp8), _p(el_mp8), _p(el_mp), nEl, _p(conn, ctypes.c_int32), _p(np.ascontiguousarray(X, dtype=np.float64)),
         _p(edofs, ctypes.c_int64), _p(np.ascontiguousarray(U_np)), _p(np.ascontiguousarray(dU)),
         _p(np.ascontiguousarray(state_n)), _p(state_t), nstate, _p(P), _p(F), _p(V),
         _p(x_slot, ctypes.c_int32) if x_slot is not None else None, nnz, _p(data), n_threads, _p(pdt), _p(times))
@@ -343,7 +349,7 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
                state_n=None, U0=None):
     """NIST.solveStep for one element block with Dirichlet + nodal-force actions.
 
-    model: dict(el_kind, mat_kind, mp, X, conn, el_mp?)
+    model: dict(el_kind, mat_kind, mp, X, conn, el_mp?, el_mp8?)
     step:  dict(maxInc, minInc, maxNumInc, maxIter, criticalIter=5, maxGrowIter=10, cutbackFactor=0.25,
                 startInc=1.0, stepLength=1.0, extrapolation='linear',
                 dirichlet=[(node_idx, {comp: value})], nodeforces=[(node_idx, vec3)], bodyforces=[vec3],
@@ -403,7 +409,7 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
             while True:
                 U_np = U + dU
                 out = eval_assemble(model["el_kind"], model["mat_kind"], model["mp"], conn, X, edofs, U_np, dU,
-                                    state_n, x, nnz, el_mp=model.get("el_mp"))
+                                    state_n, x, nnz, el_mp=model.get("el_mp"), el_mp8=model.get("el_mp8"))
                 if out["failed"]:
                     raise CutbackRequest("element requested cutback", out["pNewDT"])
                 Pv, Fv = out["P"], out["F"]

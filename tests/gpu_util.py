@@ -6,11 +6,15 @@ from edelweissfe_amd.model import ElementBlock, FEModel, Material
 from oracle import oracle as O
 
 
-def one_block_model(el, mat, mp, X, conn, elParams=None, sets=None):
+def one_block_model(el, mat, mp, X, conn, elParams=None, sets=None, materials=None):
+    """materials = (list of parameter lists, material index per element): several materials of kind mat."""
     m = FEModel()
     m.addNodes(X)
     b = ElementBlock(el, conn, elParams=elParams)
-    b.setMaterial(Material(mat, mp))
+    if materials is None:
+        b.setMaterial(Material(mat, mp))
+    else:
+        b.setMaterials([Material(mat, q) for q in materials[0]], materials[1])
     m.addElementBlock(b, "all")
     if sets:
         m.nodeSets.update({k: np.asarray(v) for k, v in sets.items()})
@@ -18,14 +22,14 @@ def one_block_model(el, mat, mp, X, conn, elParams=None, sets=None):
 
 
 class GpuEval:
-    def __init__(self, el, mat, mp, X, conn, elParams=None, pipelineChunks=0):
+    def __init__(self, el, mat, mp, X, conn, elParams=None, pipelineChunks=0, materials=None):
         import torch
 
         from edelweissfe_amd.assembly import ElementAssembly
         from edelweissfe_amd.dofmanager import DofManager
 
         self.torch = torch
-        self.model, self.block = one_block_model(el, mat, mp, X, conn, elParams)
+        self.model, self.block = one_block_model(el, mat, mp, X, conn, elParams, materials=materials)
         self.dm = DofManager(self.model)
         self.asm = ElementAssembly(self.block, self.dm, self.model, pipelineChunks=pipelineChunks)
 
@@ -41,14 +45,15 @@ class GpuEval:
                     state=self.asm.stateInElementOrder("trial"), pNewDT=float(pdt.item()))
 
 
-def oracle_eval(el, mat, mp, X, conn, U_np, dU, state_n=None, el_mp=None):
+def oracle_eval(el, mat, mp, X, conn, U_np, dU, state_n=None, el_mp=None, el_mp8=None):
     base, nDof, edofs = O.dof_numbering(len(X), conn)
     I, J = O.vij_pattern(edofs)
     indptr, indices, x = O.csr_pattern(I, J, nDof)
     nq = O.NQP[O.EL_KINDS[el]]
     if state_n is None:
         state_n = np.zeros(len(conn) * nq * O.NSTATE[O.MAT_KINDS[mat]])
-    out = O.eval_assemble(el, mat, mp, conn, X, edofs, U_np, dU, state_n, x, len(indices), el_mp=el_mp)
+    out = O.eval_assemble(el, mat, mp, conn, X, edofs, U_np, dU, state_n, x, len(indices), el_mp=el_mp,
+                          el_mp8=el_mp8)
     out["indptr"], out["indices"] = indptr, indices
     return out
 

@@ -384,6 +384,41 @@ def test_qp_results_match_oracle_state(el, mat, mp):
     assert rel(strain.cpu().numpy(), st[:, :, 6:12]) < 1e-12
 
 
+MATERIAL_SETS = {"vonmises": [J2, [150000.0, 0.25, 300.0, 2000.0, 0.0, 0.0], J2NL],
+                 "linearelastic": [LE, [30000.0, 0.15], [5000.0, 0.49]],
+                 "neohookeanwa": [NH, [40.0, 500.0], [120.0, 150.0]]}
+
+
+@pytest.mark.parametrize("el,mat,n", [("C3D8", "vonmises", 4), ("C3D8", "linearelastic", 4),
+                                      ("C3D8TL", "neohookeanwa", 4), ("C3D20", "linearelastic", 2),
+                                      ("C3D20R", "vonmises", 2), ("C3D20RTL", "neohookeanwa", 2)])
+def test_several_materials_match_oracle(el, mat, n):
+    """Three materials of one kind assigned element by element (the plan's material table, as sections on
+    several element sets assign them): P, F, CSR values and the quadrature-point stress against the oracle
+    with the same per-element parameter sets; the single-material result differs."""
+    import torch
+
+    print(f"case materials {el} {mat} n={n}", flush=True)
+    X, conn, _ = jittered_box(n, el, seed=21)
+    rng = np.random.default_rng(22)
+    sets = MATERIAL_SETS[mat]
+    idx = rng.integers(0, 3, len(conn))
+    amp = 0.08 if "TL" in el else 4e-3
+    U = _uniaxial_field(X, amp, rng, amp / 10)
+    dU = U.copy() if "TL" not in el else 0.5 * U
+    g = GpuEval(el, mat, sets[0], X, conn, materials=(sets, idx))
+    out = g(U, dU)
+    ref = oracle_eval(el, mat, sets[0], X, conn, U, dU, el_mp8=g.block.elementMaterialProperties())
+    assert rel(out["data"], ref["data"]) < 1e-12
+    assert rel(out["P"], ref["P"]) < 1e-12 and rel(out["F"], ref["F"]) < 1e-12
+    single = oracle_eval(el, mat, sets[0], X, conn, U, dU)
+    assert rel(single["P"], ref["P"]) > 1e-3
+    stress = g.asm.qpResults("stress", g.vec(U), which="trial")
+    torch.cuda.synchronize()
+    st = ref["state"].reshape(len(conn), g.block.nInt, -1)
+    assert rel(stress.cpu().numpy(), st[:, :, 0:6]) < 1e-12
+
+
 def test_fp32_tangent_newton_reaches_fp64_tolerance():
     fx, U32, s32 = _deck_run("C3D8", "fp32")
     _, U64, s64 = _deck_run("C3D8", "fp64")

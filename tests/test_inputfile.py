@@ -23,7 +23,7 @@ FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C
 # AnalyticalFieldsSetField (J2 with kappa set from a scalarExpression field at the points)
 # and SetField (kappa set uniformly, then the stress set at every point after a support is released)
 ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet", "WallShearHexa8GMRES",
-                             "AnalyticalFieldsSetField", "SetField"]
+                             "AnalyticalFieldsSetField", "SetField", "SetDef", "StatusFile", "ComputeTimeMonitor"]
 
 
 def _deck(name):
@@ -255,7 +255,8 @@ _ORACLE_STATE_INDEX = {"kappa": [12], "stress": list(range(6))}  # the oracle's 
 def _oracle_run(model, inp):
     b = model.elementBlocks[0]
     om = dict(el_kind=b.elType, mat_kind=b.material.name, mp=b.material.materialProperties,
-              X=model.nodeCoordinates, conn=b.conn, el_mp=b.elParams)
+              X=model.nodeCoordinates, conn=b.conn, el_mp=b.elParams,
+              el_mp8=b.elementMaterialProperties() if b.materialIndex is not None else None)
     nq, ns = O.NQP[O.EL_KINDS[b.elType]], O.NSTATE[O.MAT_KINDS[b.material.name]]
     U, state = None, np.zeros(len(b) * nq * ns)
     for step, _ in _steps(model, inp):
