@@ -583,7 +583,9 @@ __device__ __forceinline__ int slot_node(int base, int b, int i, int q, int le)
     return (base + 3 * b + i) * 64 + 16 * (le >> 1) + ((2 * q + (le & 1)) ^ (2 * b));
 }
 
-template <int FORM, int MAT, typename VT>
+// MT: the plan holds a material table (a separate instantiation, so that the single-material path carries no
+// per-element parameter selects)
+template <int FORM, int MAT, typename VT, bool MT>
 __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 {
     constexpr int NN = 8, NQ = 8, EPB = 8;
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     constexpr int NVN = (MAT == M_J2) ? 9 : 6;  // staged values per node
     static_assert(EPB * NN * NVN <= (L::NF - L::AL) * 64, "nodal staging must fit in the last fields");
     double *s_nd = s_qp + L::AL * 64;
-    const MatParams M = mat_at(A, active ? pos : q0);
+    const MatParams M = MT ? mat_at(A, active ? pos : q0) : A.mat;
     double Epar = M.p[0];
     if (active && A.el_param) Epar = A.el_param[A.order[pos]];
     double st[7] = {0, 0, 0, 0, 0, 0, 0};  // converged J2 state of point (le, q = t), loaded with the nodes
@@ -1895,8 +1897,12 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s)
         const int64_t batches = (R.q_end - R.q_begin + 7) / 8;
         R.xcd_batches = (batches + 7) / 8;
         if (batches > 0) {
-            hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT>), dim3((unsigned)(8 * R.xcd_batches)), dim3(64), 0, s, A,
-                               R);
+            if (A.mtab)
+                hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, true>), dim3((unsigned)(8 * R.xcd_batches)), dim3(64),
+                                   0, s, A, R);
+            else
+                hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, false>), dim3((unsigned)(8 * R.xcd_batches)),
+                                   dim3(64), 0, s, A, R);
             HIP_CHECK(hipGetLastError());
         }
         if (K > 1) {
