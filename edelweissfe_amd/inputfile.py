@@ -33,7 +33,8 @@ import shlex
 
 import numpy as np
 
-from .model import BOXGEN_SURFACES, ElementBlock, FEModel, Material, boxGenArrays, boxGenElementSets
+from .model import (BOXGEN_SURFACES, ELEMENT_LIBRARY, ElementBlock, FEModel, Material, boxGenArrays,
+                    boxGenElementSets, elementType, pipeGenArrays)
 
 # keyword -> option names it accepts; the output keywords are read and not acted on
 _KEYWORDS = {
@@ -186,6 +187,52 @@ def _boxgen(deck: _Deck, gen: dict):
         deck.surfaces[f"{name}_{suffix}"] = {face: f"{name}_{suffix}"}
 
 
+def _pipegen(deck: _Deck, gen: dict):
+    """edelweissfe/generators/pipegen.py:91-357 through model.pipeGenArrays; node and element labels continue
+    after the highest existing ones."""
+    opts = _assignments(gen["data"])
+    name = gen.get("name", "cylBoxGen")
+    g = lambda k, d: opts.get(k, d)  # noqa: E731
+    elType = g("elType", None)
+    if elType is None:
+        raise KeyError("pipeGen needs elType")
+    Ro, Ri = str(g("Ro(y)", "1.0")), str(g("Ri(y)", "1.0"))
+    if Ri == "1.0":
+        expr, outer = Ro, True
+    elif Ro == "1.0":
+        expr, outer = Ri, False
+    else:
+        raise Exception("Only the outer or the inner radius can be defined as a function. Please use 'lT' option "
+                        "instead!")
+    phi, lT, lY, y0 = float(g("phi", 360)), float(g("lT", 0.5)), float(g("lY", 1.0)), float(g("y0", 0.0))
+    if abs(phi) > 360:
+        raise Exception("The angle can't be higher than 360° or lower than -360°.")
+    nT, nY, nC = int(g("nT", 1)), int(g("nY", 4)), int(g("nC", 4))
+    f = _Expression(expr, ("y",))
+    npe = ELEMENT_LIBRARY[elementType(elType)]["nNodes"]
+    heights = np.linspace(y0, y0 + lY, (nY if npe == 8 else 2 * nY) + 1) - y0
+    R = np.array([f(h) for h in heights])
+    if (np.any(R - lT < 1e-5) and outer) or (np.any(R < 1e-5) and not outer):
+        raise Exception("The radius must be > 1e-5 at every point.")
+    # the option exG is used as given: a deck's 'exG = False' is a non-empty text and, as in the generator,
+    # counts as set (nodes on the exact geometry)
+    exG = g("exG", True)
+    X, conn, sets, esets, surfaces = pipeGenArrays(
+        nT, nY, nC, lambda h: f(h), lT=lT, lY=lY, phi=phi, x0=float(g("x0", 0.0)), y0=y0, z0=float(g("z0", 0.0)),
+        outer=outer, reduceMidRadius=not exG, elType=elType, name=name)
+    n0 = max(deck.nodeLabels, default=0)
+    labels = list(range(n0 + 1, n0 + 1 + len(X)))
+    deck.addNodes(labels, list(X))
+    e0 = max(deck.elLabels, default=0)
+    first = len(deck.elLabels)
+    deck.addElements(elType, range(e0 + 1, e0 + 1 + len(conn)), [[labels[i] for i in row] for row in conn])
+    for k, idx in sets.items():
+        deck.nodeSets[k] = [labels[i] for i in idx]
+    for k, idx in esets.items():
+        deck.elementSets[k] = [deck.elLabels[first + i] for i in idx]
+    deck.surfaces.update(surfaces)
+
+
 def _findClosestNode(deck: _Deck, gen: dict):
     """edelweissfe/generators/findclosestnode.py:51-73: the node nearest to 'location' (the first of equally near
     ones, in node order) becomes the node set 'storeIn'."""
@@ -206,7 +253,7 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
             raise NotImplementedError("the GPU path covers 3D solid elements only")
     deck = _Deck()
     gens = inputFile["*modelgenerator"]
-    generators = {"boxgen": _boxgen, "findclosestnode": _findClosestNode}
+    generators = {"boxgen": _boxgen, "pipegen": _pipegen, "findclosestnode": _findClosestNode}
     for gen in gens:
         if gen["generator"].lower() not in generators:
             raise NotImplementedError(f"model generator {gen['generator']} is not part of the GPU path")

@@ -333,6 +333,82 @@ def boxGenArrays(nX, nY, nZ, lX=1.0, lY=1.0, lZ=1.0, x0=0.0, y0=0.0, z0=0.0, elT
     return coords, conn, sets, grid
 
 
+def pipeGenArrays(nT, nY, nC, radius, lT=0.5, lY=1.0, phi=360.0, x0=0.0, y0=0.0, z0=0.0, outer=True,
+                  reduceMidRadius=False, elType="C3D8", name="gen"):
+    """pipeGen (pipegen.py:91-357): a pipe segment about the y axis, nT elements through the thickness lT, nY
+    along the height lY, nC around the angle phi (degrees; 360 closes the ring). radius(h) is the outer
+    (outer=True) or inner radius at height h = y - y0. Node grid (iy, it, ic): x = x0 + r sin(c),
+    z = z0 + r cos(c), with r on nNodesT points from R - lT to R (outer) or R to R + lT; 20-node meshes skip
+    nodes with two or more odd grid indices and, with reduceMidRadius, place the circumferentially mid-side nodes
+    at r cos(dc). Elements in the order it, iy, ic. Returns (coords, conn, nodeSets, elementSets, surfaces)
+    with node and element indices; surfaces map a set name to {face: element set name}."""
+    npe = ELEMENT_LIBRARY[elementType(elType)]["nNodes"]
+    m = 1 if npe == 8 else 2
+    nNT, nNY, nNC = m * nT + 1, m * nY + 1, m * nC + 1
+    closed = phi == 360
+    nCp = nNC - (1 if closed else 0)  # node columns around the ring (the closing column is the first one)
+    cLayers = np.linspace(0.0, np.pi * phi / 180.0, nNC)
+    yLayers = np.linspace(y0, y0 + lY, nNY)
+    iy, it, ic = np.meshgrid(np.arange(nNY), np.arange(nNT), np.arange(nCp), indexing="ij")
+    present = np.ones(iy.shape, dtype=bool) if npe == 8 else ((iy % 2) + (it % 2) + (ic % 2)) < 2
+    R = np.array([radius(y - y0) for y in yLayers], dtype=np.float64)
+    # radial positions [iy, it]: the generator's linspace per layer, between R - lT and R (outer) or R and R + lT
+    t = np.stack([np.linspace(Ry - lT, Ry, nNT) if outer else np.linspace(Ry, Ry + lT, nNT) for Ry in R])
+    r = np.broadcast_to(t[:, :, None], iy.shape).copy()
+    if npe == 20 and reduceMidRadius:
+        r[:, :, 1::2] *= np.cos((np.pi * phi / 180.0) / nCp)
+    c = cLayers[ic]
+    X = np.stack([x0 + r * np.sin(c), yLayers[iy], z0 + r * np.cos(c)], axis=-1)
+    g2m = np.full(iy.shape, -1, dtype=np.int64)
+    g2m[present] = np.arange(int(present.sum()))
+    coords = X[present]
+
+    if npe == 8:
+        offs = [(0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0), (0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0)]
+    else:
+        offs = [(0, 0, 0), (0, 0, 2), (2, 0, 2), (2, 0, 0), (0, 2, 0), (0, 2, 2), (2, 2, 2), (2, 2, 0),
+                (0, 0, 1), (1, 0, 2), (2, 0, 1), (1, 0, 0), (0, 2, 1), (1, 2, 2), (2, 2, 1), (1, 2, 0),
+                (0, 1, 0), (0, 1, 2), (2, 1, 2), (2, 1, 0)]
+    et, ey, ec = (a.ravel() for a in np.meshgrid(np.arange(nT), np.arange(nY), np.arange(nC), indexing="ij"))
+    conn = np.stack([g2m[m * ey + dy, m * et + dt, (m * ec + dc) % nCp] for dy, dt, dc in offs], axis=1)
+    conn = conn.astype(np.int32)
+
+    def filt(sl):
+        sub = g2m[sl]
+        return sub[sub >= 0].ravel()
+
+    sets = {}
+    for nm, sl in [("outer", np.s_[:, -1, :]), ("inner", np.s_[:, 0, :]), ("top", np.s_[-1, :, :]),
+                   ("bottom", np.s_[0, :, :]), ("end", np.s_[:, :, -1]), ("begin", np.s_[:, :, 0]),
+                   ("beginTop", np.s_[-1, :, 0]), ("endTop", np.s_[-1, :, -1]), ("beginBottom", np.s_[0, :, 0]),
+                   ("endBottom", np.s_[0, :, -1]), ("innerEnd", np.s_[:, 0, -1]), ("innerBegin", np.s_[:, 0, 0]),
+                   ("innerTop", np.s_[-1, 0, :]), ("innerBottom", np.s_[0, 0, :]), ("outerEnd", np.s_[:, -1, -1]),
+                   ("outerBegin", np.s_[:, -1, 0]), ("outerTop", np.s_[-1, -1, :]),
+                   ("outerBottom", np.s_[0, -1, :]), ("centerY", np.s_[int(nNY / 2), :, :]),
+                   ("centerT", np.s_[:, int(nNT / 2), :]), ("centerC", np.s_[:, :, int(nCp / 2)])]:
+        sets[f"{name}_{nm}"] = filt(sl)
+    for nm, (i, j, k) in [("topEndInner", (-1, 0, -1)), ("topBeginInner", (-1, 0, 0)),
+                          ("bottomEndInner", (0, 0, -1)), ("bottomBeginInner", (0, 0, 0)),
+                          ("topEndOuter", (-1, -1, -1)), ("topBeginOuter", (-1, -1, 0)),
+                          ("bottomEndOuter", (0, -1, -1)), ("bottomBeginOuter", (0, -1, 0))]:
+        sets[f"{name}_{nm}"] = np.array([g2m[i, j, k]])
+
+    eg = np.arange(nT * nY * nC).reshape(nT, nY, nC)
+    esets = {f"{name}_all": eg.ravel(), f"{name}_bottom": eg[:, 0, :].ravel(), f"{name}_top": eg[:, -1, :].ravel(),
+             f"{name}_outer": eg[-1, :, :].ravel(), f"{name}_inner": eg[0, :, :].ravel(),
+             f"{name}_end": eg[:, :, -1].ravel(), f"{name}_begin": eg[:, :, 0].ravel(),
+             f"{name}_centerSliceY": eg[int(nT / 2), :, :].ravel(), f"{name}_centerSliceT": eg[:, int(nY / 2), :].ravel(),
+             f"{name}_centerSliceC": eg[:, :, int(nC / 2)].ravel()}
+    nBand = min(nT, nY)
+    if nBand > 3:
+        band = np.concatenate([eg[int(nT / 2 + i - nBand / 2), int(nY / 2 + i - nBand / 2), 0:nC] for i in range(nBand)])
+        esets[f"{name}_shearBandInnerToOuter"] = band
+        esets[f"{name}_shearBandCenterInnerToOuter"] = band[(int(nBand / 2) - 1) * nC:(int(nBand / 2) + 2) * nC]
+    surfaces = {f"{name}_{k}": {face: f"{name}_{k}"} for k, face in
+                (("bottom", 1), ("top", 2), ("outer", 5), ("inner", 3), ("end", 4), ("begin", 6))}
+    return coords, conn, sets, esets, surfaces
+
+
 # boxGen's element-based surfaces: set name suffix -> face number (boxgen.py:438-446); with boxGen's element node
 # order (parametric axes xi, eta, zeta along z, x, y) face 1 (zeta = -1) is y = y0, face 4 (xi = +1) is z = z0 + lZ
 BOXGEN_SURFACES = {"bottom": 1, "top": 2, "left": 3, "front": 4, "right": 5, "back": 6}

@@ -23,7 +23,8 @@ FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C
 # AnalyticalFieldsSetField (J2 with kappa set from a scalarExpression field at the points)
 # and SetField (kappa set uniformly, then the stress set at every point after a support is released)
 ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet", "WallShearHexa8GMRES",
-                             "AnalyticalFieldsSetField", "SetField", "SetDef", "StatusFile", "ComputeTimeMonitor"]
+                             "AnalyticalFieldsSetField", "SetField", "SetDef", "StatusFile", "ComputeTimeMonitor",
+                             "PipeGen"]
 
 
 def _deck(name):
@@ -221,6 +222,30 @@ dirichlet, name=z0u, nSet=gen_back, field=displacement, 3=0.
 *step, solver=theSolver, maxInc=1, minInc=1e-2, maxNumInc=100, maxIter=25
 distributedload, name=p, surface=gen_top, type=pressure, magnitude=100
 """
+
+
+def test_pipegen_closed_ring_and_radii():
+    """pipeGen beyond the PipeGen deck (a 90 degree C3D20 segment): a closed 360 degree C3D8 ring wraps its last
+    element onto the first node column, every node lies on its layer's radius, every element has positive
+    volume, and the ring's inner and outer node sets are the radii R - lT and R."""
+    from edelweissfe_amd.model import pipeGenArrays, quadraturePoints, shapeFunctionDerivatives
+
+    R = lambda h: 3.0 + 0.5 * h  # noqa: E731
+    nT, nY, nC, lT = 2, 3, 12, 0.6
+    X, conn, sets, esets, surf = pipeGenArrays(nT, nY, nC, R, lT=lT, lY=2.0, phi=360.0, x0=1.0, y0=-1.0, z0=2.0,
+                                               elType="C3D8")
+    assert len(X) == (nY + 1) * (nT + 1) * nC and len(conn) == nT * nY * nC
+    last, first = conn[nC - 1], conn[0]  # element (it=0, iy=0, ic=nC-1) closes onto column 0
+    assert last[1] == first[0] and last[2] == first[3] and last[5] == first[4] and last[6] == first[7]
+    r = np.hypot(X[:, 0] - 1.0, X[:, 2] - 2.0)
+    h = X[:, 1] + 1.0
+    assert np.allclose(r[sets["gen_outer"]], R(h[sets["gen_outer"]]), atol=1e-13)
+    assert np.allclose(r[sets["gen_inner"]], R(h[sets["gen_inner"]]) - lT, atol=1e-13)
+    dN = shapeFunctionDerivatives(8, quadraturePoints(8))  # [q, a, k]
+    J = np.einsum("qak,eai->eqik", dN, X[conn])
+    assert np.linalg.det(J).min() > 0
+    assert set(esets) >= {"gen_all", "gen_inner", "gen_outer", "gen_top", "gen_bottom"}
+    assert surf["gen_inner"] == {3: "gen_inner"} and surf["gen_bottom"] == {1: "gen_bottom"}
 
 
 def _field_deck(tmp_path):
