@@ -233,32 +233,8 @@ def _pipegen(deck: _Deck, gen: dict):
     deck.surfaces.update(surfaces)
 
 
-def _findClosestNode(deck: _Deck, gen: dict):
-    """edelweissfe/generators/findclosestnode.py:51-73: the node nearest to 'location' (the first of equally near
-    ones, in node order) becomes the node set 'storeIn'."""
-    opts = _assignments([t for ln in gen["data"] for t in splitLineAtCommas(ln) if t])
-    loc = np.array([float(t) for t in splitLineAtCommas(opts["location"]) if t])
-    if len(loc) != 3:
-        raise ValueError("findClosestNode: the location needs 3 coordinates")
-    if opts["storeIn"] in deck.nodeSets:
-        raise Exception(f"Nodeset {opts['storeIn']} already exists")
-    coords = np.asarray(deck.coords, dtype=np.float64).reshape(-1, 3)
-    deck.nodeSets[opts["storeIn"]] = [deck.nodeLabels[int(np.linalg.norm(coords - loc, axis=1).argmin())]]
-
-
-def createModelFromInputFile(inputFile: dict) -> FEModel:
-    """FEModel of a parsed deck: nodes, one element block with its material, node and element sets."""
-    for job in inputFile["*job"]:
-        if job.get("domain", "3d").lower() != "3d":
-            raise NotImplementedError("the GPU path covers 3D solid elements only")
-    deck = _Deck()
-    gens = inputFile["*modelgenerator"]
-    generators = {"boxgen": _boxgen, "pipegen": _pipegen, "findclosestnode": _findClosestNode}
-    for gen in gens:
-        if gen["generator"].lower() not in generators:
-            raise NotImplementedError(f"model generator {gen['generator']} is not part of the GPU path")
-    for gen in (g for g in gens if not _truthy(g.get("executeaftermanualgeneration", "false"))):
-        generators[gen["generator"].lower()](deck, gen)
+def _geometry(deck: _Deck, inputFile: dict):
+    """Nodes, elements and sets of a parsed deck or mesh file (abqmodelconstructor.py:67-212)."""
     for nd in inputFile["*node"]:
         rows = [splitLineAtCommas(ln) for ln in nd["data"]]
         labels = [int(r[0]) for r in rows]
@@ -286,6 +262,56 @@ def createModelFromInputFile(inputFile: dict) -> FEModel:
             deck.nodeSets[ns["nset"]] = (list(range(nums[0], nums[1] + 1, nums[2])) if _truthy(ns.get("generate", "")) else nums)
         else:
             deck.nodeSets[ns["nset"]] = [lab for s in data for lab in deck.nodeSets[s]]
+
+
+def _cubit(deck: _Deck, gen: dict):
+    """edelweissfe/generators/cubit.py:26-100: the mesh Cubit exported (outFile, default mesh.inc, beside the deck),
+    element types replaced by elType or per element set by elTypePerBlock, added like a deck's own geometry. Cubit
+    itself is not part of this build: the exported file must exist (the reference re-exports it when overwrite is
+    set; its content is the same mesh)."""
+    opts = _assignments(gen["data"])
+    outFile = os.path.join(os.path.dirname(gen.get("inputFile", "")), opts.get("outFile", "mesh.inc").strip("'\""))
+    if not os.path.exists(outFile):
+        raise NotImplementedError(f"running Cubit is not part of the GPU path and {outFile} does not exist")
+    mesh = parseInputFile(outFile)
+    elType = opts.get("elType")
+    perBlock = opts.get("elTypePerBlock")
+    for el in mesh["*element"]:
+        if elType:
+            el["type"] = elType.strip("'\"")
+        if perBlock:
+            pairs = dict(t.split("=", 1) for t in perBlock.strip("'\"").replace(" ", "").split(",") if t)
+            el["type"] = pairs[el["elset"]]
+    _geometry(deck, mesh)
+
+
+def _findClosestNode(deck: _Deck, gen: dict):
+    """edelweissfe/generators/findclosestnode.py:51-73: the node nearest to 'location' (the first of equally near
+    ones, in node order) becomes the node set 'storeIn'."""
+    opts = _assignments([t for ln in gen["data"] for t in splitLineAtCommas(ln) if t])
+    loc = np.array([float(t) for t in splitLineAtCommas(opts["location"]) if t])
+    if len(loc) != 3:
+        raise ValueError("findClosestNode: the location needs 3 coordinates")
+    if opts["storeIn"] in deck.nodeSets:
+        raise Exception(f"Nodeset {opts['storeIn']} already exists")
+    coords = np.asarray(deck.coords, dtype=np.float64).reshape(-1, 3)
+    deck.nodeSets[opts["storeIn"]] = [deck.nodeLabels[int(np.linalg.norm(coords - loc, axis=1).argmin())]]
+
+
+def createModelFromInputFile(inputFile: dict) -> FEModel:
+    """FEModel of a parsed deck: nodes, one element block with its material, node and element sets."""
+    for job in inputFile["*job"]:
+        if job.get("domain", "3d").lower() != "3d":
+            raise NotImplementedError("the GPU path covers 3D solid elements only")
+    deck = _Deck()
+    gens = inputFile["*modelgenerator"]
+    generators = {"boxgen": _boxgen, "pipegen": _pipegen, "cubit": _cubit, "findclosestnode": _findClosestNode}
+    for gen in gens:
+        if gen["generator"].lower() not in generators:
+            raise NotImplementedError(f"model generator {gen['generator']} is not part of the GPU path")
+    for gen in (g for g in gens if not _truthy(g.get("executeaftermanualgeneration", "false"))):
+        generators[gen["generator"].lower()](deck, gen)
+    _geometry(deck, inputFile)
     for gen in (g for g in gens if _truthy(g.get("executeaftermanualgeneration", "false"))):
         generators[gen["generator"].lower()](deck, gen)
     if deck.elType is None:
