@@ -60,8 +60,10 @@ _KEYWORDS = {
 }
 
 # material names of the reference's providers -> materials of the GPU path
-_MATERIALS = {"linearelastic": "linearelastic", "vonmises": "vonmises", "neohookeanwa": "neohookeanwa",
-              "neohookeanpencegouformulationa": "neohookeanwa"}
+# material names a deck may use -> the GPU path's laws; ADLinearElastic is Marmot's isotropic linear elasticity
+# with an automatically differentiated tangent (the same law: [E, nu])
+_MATERIALS = {"linearelastic": "linearelastic", "adlinearelastic": "linearelastic", "vonmises": "vonmises",
+              "neohookeanwa": "neohookeanwa", "neohookeanpencegouformulationa": "neohookeanwa"}
 # solvers a deck may name; all run as NISTGPU here (edelweissfe/config/solvers.py:48-76)
 _SOLVERS = {"nist", "nistparallel", "nistparallelformarmotelements", "nistgpu"}
 

@@ -24,7 +24,7 @@ FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C
 # and SetField (kappa set uniformly, then the stress set at every point after a support is released)
 ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet", "WallShearHexa8GMRES",
                              "AnalyticalFieldsSetField", "SetField", "SetDef", "StatusFile", "ComputeTimeMonitor",
-                             "PipeGen", "CubitGen"]
+                             "PipeGen", "CubitGen", "ADLinearElastic"]
 
 
 def _deck(name):
