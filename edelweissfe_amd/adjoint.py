@@ -36,7 +36,10 @@ def parameter_gradient(solver, model, dJdU, rtol=1e-12, maxit=100000):
     if rows is not None:
         b[rows.long()] = 0.0
     lam = dm.constructDofVector()
-    it, rel = asm.pcg(K, b, lam, rows, rtol=rtol, maxit=maxit)
+    # the forward solve's linear solver: the multigrid-preconditioned CG where the solver chose it
+    amg = getattr(solver, "_linsolver", "pcg") == "amg"
+    it, rel = asm.pcg(K, b, lam, rows, rtol=rtol, maxit=maxit, check_every=4 if amg else 32,
+                      precond="amg" if amg else "jacobi")
     grad = torch.empty(len(asm.block), dtype=torch.float64, device=asm.device)
     mp = asm.matParams
     check(lib().ewfe_adjoint_param_grad(asm._plan, asm.matKind, ptr(mp), ptr(asm.elParams), ptr(asm.X), ptr(U),
