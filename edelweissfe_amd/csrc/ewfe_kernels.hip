@@ -1424,6 +1424,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __shared__ double s_nd[9 * NN];      // [field][node]
     __shared__ double s_qp[L::NF * QC];  // [field][point of the chunk]
     __shared__ int32_t s_dof[NN];
+    __shared__ int32_t s_rs[3 * NN];     // CSR row starts of the element's 60 dof rows
     const int lane = threadIdx.x;
     const int64_t pos = A.p_begin + blockIdx.x;
     if (pos >= A.p_end) return;  // uniform over the workgroup
@@ -1432,6 +1433,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int32_t d = A.edof[(int64_t)lane * n + pos];
         s_dof[lane] = d;
         for (int c = 0; c < 3; ++c) {
+            s_rs[3 * lane + c] = A.indptr[d + c];
             s_nd[(0 + c) * NN + lane] = A.X[d + c];
             s_nd[(3 + c) * NN + lane] = A.U[d + c];
             s_nd[(6 + c) * NN + lane] = A.dU[d + c];
@@ -1533,7 +1535,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     A.P[row] -= fi;
     A.F[row] += fabs(fi);
     VT *vals = reinterpret_cast<VT *>(A.vals);
-    const int64_t rs = A.indptr[row];
+    const int64_t rs = s_rs[3 * a + i];
 #pragma unroll
     for (int j = 0; j < kJ; ++j) {
         if (j < nb) {
@@ -1542,7 +1544,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             for (int k = 0; k < 3; ++k) add_to(vals, rs + cab + k, K[j][k]);
             if (j > 0) {  // K_ba[k][i] = K_ab[i][k] into row k of node b, column i of node a
                 const int32_t cba = A.colpos[pos * (NN * NN) + b * NN + a];
-                for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)A.indptr[s_dof[b] + k] + cba + i, K[j][k]);
+                for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)s_rs[3 * b + k] + cba + i, K[j][k]);
             }
         }
     }
