@@ -14,6 +14,8 @@ from edelweissfe_amd.model import FEModel, boxGenArrays, jitterNodes
 from edelweissfe_amd.stepping import AdaptiveTimeStepper, ReachedMinIncrementSize
 from oracle import oracle as O
 
+EWFE_EINVAL = -1  # include/ewfe.h
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -118,6 +120,52 @@ def test_plan_rejects_bad_input():
     assert rc == -1 and b"outside" in L.ewfe_last_error()
     with pytest.raises(_lib.EwfeError):
         _lib.check(rc)
+
+
+def test_material_table_rejects_bad_indices():
+    """ewfe_plan_set_materials checks its arguments before any device work: an index outside [0, n_mat) or a
+    missing table fails with EWFE_EINVAL and a message; n_mat = 0 (no table) succeeds."""
+    L = _lib.lib()
+    X, conn, _, _ = boxGenArrays(2, 2, 2, elType="C3D8")
+    base, nDof, _ = O.dof_numbering(len(X), conn)
+    plan = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    node_dof = np.ascontiguousarray(base, dtype=np.int64)
+    _lib.check(L.ewfe_plan_create_host(ctypes.byref(plan), _lib.EWFE_C3D8, len(conn), _lib.ptr(conn), len(node_dof),
+                                       _lib.ptr(node_dof), nDof, ctypes.byref(nnz), 0))
+    try:
+        table = np.zeros((2, 8))
+        idx = np.zeros(len(conn), dtype=np.int32)
+        idx[3] = 2  # outside [0, 2)
+        rc = L.ewfe_plan_set_materials(plan, 2, _lib.ptr(table), _lib.ptr(idx))
+        assert rc == EWFE_EINVAL and b"outside" in L.ewfe_last_error()
+        idx[3] = -1
+        assert L.ewfe_plan_set_materials(plan, 2, _lib.ptr(table), _lib.ptr(idx)) == EWFE_EINVAL
+        assert L.ewfe_plan_set_materials(plan, 2, None, _lib.ptr(idx)) == EWFE_EINVAL
+        assert L.ewfe_plan_set_materials(plan, -1, _lib.ptr(table), _lib.ptr(idx)) == EWFE_EINVAL
+        assert L.ewfe_plan_set_materials(plan, 0, None, None) == 0
+    finally:
+        L.ewfe_plan_destroy(plan)
+
+
+def test_block_materials_of_one_kind():
+    """ElementBlock.setMaterials: several materials of one kind with one index per element; the per-element
+    parameter sets follow the index, the per-element first parameter overrides them; mixed kinds and bad
+    indices are rejected."""
+    from edelweissfe_amd.model import ElementBlock, Material
+
+    X, conn, _, _ = boxGenArrays(2, 1, 1, elType="C3D8")
+    b = ElementBlock("C3D8", conn)
+    b.setMaterials([Material("linearelastic", [30000.0, 0.2]), Material("linearelastic", [5000.0, 0.49])], [1, 0])
+    mp = b.elementMaterialProperties()
+    assert mp.shape == (2, 8) and np.array_equal(mp[:, :2], [[5000.0, 0.49], [30000.0, 0.2]])
+    b.elParams = np.array([7.0, 8.0])
+    assert np.array_equal(b.elementMaterialProperties()[:, 0], [7.0, 8.0])
+    with pytest.raises(NotImplementedError):
+        b.setMaterials([Material("linearelastic", [1.0, 0.2]), Material("vonmises", [1.0, 0.3, 1.0, 0.0])], [0, 1])
+    with pytest.raises(ValueError):
+        b.setMaterials([Material("linearelastic", [1.0, 0.2]), Material("linearelastic", [2.0, 0.2])], [0, 2])
 
 
 @pytest.mark.parametrize("el,n", [("C3D8", (3, 2, 4)), ("C3D20", (2, 3, 1))])
