@@ -250,8 +250,21 @@ def _geometry(deck: _Deck, inputFile: dict):
             deck.elementSets[el["elset"]] = [r[0] for r in rows]
     for es in inputFile["*elset"]:
         data = [t for ln in es["data"] for t in splitLineAtCommas(ln) if t]
-        if "boolean" in es:
-            raise NotImplementedError("boolean element set operations are not part of the GPU path")
+        if "boolean" in es:  # abqmodelconstructor.py:153-171: the set combined with the listed labels
+            op = es["boolean"].lower()
+            nums = [int(t) for t in data]
+            base = deck.elementSets[es["elset"]]
+            if op == "difference":
+                labs = [n for n in base if n not in set(nums)]
+            elif op == "union":
+                labs = list(base) + nums
+            elif op == "intersection":
+                labs = [n for n in nums if n in set(base)]
+            else:
+                raise Exception("Undefined boolean operation!")
+            # stored as newElSet (the set itself is replaced when newElSet names it or is not given)
+            deck.elementSets[es.get("newelset", es["elset"])] = sorted(set(labs))
+            continue
         if data and _is_int(data[0]):
             nums = [int(t) for t in data]
             deck.elementSets[es["elset"]] = (list(range(nums[0], nums[1] + 1, nums[2])) if _truthy(es.get("generate", "")) else nums)

@@ -224,6 +224,34 @@ distributedload, name=p, surface=gen_top, type=pressure, magnitude=100
 """
 
 
+def test_boolean_element_sets(tmp_path):
+    """*elSet with boolean = difference / union / intersection (abqmodelconstructor.py:153-171) on a boxGen
+    mesh, into new sets and in place."""
+    deck = tmp_path / "b.inp"
+    deck.write_text("""*modelGenerator, generator=boxGen, name=gen
+nX=4
+nY=1
+nZ=1
+elType=C3D8
+*elSet, elSet=gen_all, boolean=difference, newElSet=inner
+1, 4
+*elSet, elSet=inner, boolean=union, newElSet=grown
+1
+*elSet, elSet=gen_all, boolean=intersection, newElSet=ends
+1, 4, 9
+*elSet, elSet=grown, boolean=difference
+2
+*material, name=linearelastic, id=m
+100., 0.3
+*section, name=s, material=m, type=solid
+all
+""")
+    model = IF.createModelFromInputFile(IF.parseInputFile(str(deck)))
+    labels = model.elementBlocks[0].labels
+    got = {k: sorted(int(labels[i]) for i in model.elementSets[k][1]) for k in ("inner", "grown", "ends")}
+    assert got == {"inner": [2, 3], "grown": [1, 3], "ends": [1, 4]}
+
+
 def test_pipegen_closed_ring_and_radii():
     """pipeGen beyond the PipeGen deck (a 90 degree C3D20 segment): a closed 360 degree C3D8 ring wraps its last
     element onto the first node column, every node lies on its layer's radius, every element has positive
