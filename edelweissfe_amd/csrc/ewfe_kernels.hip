@@ -1018,8 +1018,10 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
                                                  double *__restrict__ F, int64_t q_begin, int64_t count,
                                                  int64_t xcd_chunk)
 {
-    __shared__ double s_blk[2][64 * kBlk];  // per half: block (a, b) of the incident element at corner o, slot 8 o + b
-    __shared__ double s_fe[2][8 * 3];
+    // per half: block (a, b) of the incident element at corner o in slot 8 o + b (9 of its kBlk = 10 doubles); the
+    // 10th double of slot 8 o + c holds component c of the element's nodal force f_a (LDS 10 KB: 16 workgroups
+    // per CU)
+    __shared__ double s_blk[2][64 * kBlk];
     const int64_t k = blockIdx.x >> 3;
     const int64_t pair = (int64_t)(blockIdx.x & 7) * xcd_chunk + k;
     if (k >= xcd_chunk || 2 * pair >= count) return;  // uniform over the workgroup
@@ -1049,7 +1051,7 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
             double *dst = s_blk[h] + blk * kBlk;
 #pragma unroll
             for (int t = 0; t < 9; ++t) dst[t] = v[t];
-            if (b < 3) s_fe[h][o * 3 + b] = fe[e * 24 + 3 * a + b];
+            if (b < 3) dst[9] = fe[e * 24 + 3 * a + b];
         }
     }
     __syncthreads();
@@ -1091,7 +1093,7 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
         for (int o = 0; o < 8; ++o) {
             int64_t e;
             if (elem(o, e)) {
-                const double x = s_fe[h][o * 3 + c];
+                const double x = s_blk[h][(8 * o + c) * kBlk + 9];
                 f -= x;
                 fa += fabs(x);
             }
