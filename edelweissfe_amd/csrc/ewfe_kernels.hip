@@ -1018,10 +1018,14 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
                                                  double *__restrict__ F, int64_t q_begin, int64_t count,
                                                  int64_t xcd_chunk)
 {
-    // per half: block (a, b) of the incident element at corner o in slot 8 o + b (9 of its kBlk = 10 doubles); the
-    // 10th double of slot 8 o + c holds component c of the element's nodal force f_a (LDS 10 KB: 16 workgroups
-    // per CU)
-    __shared__ double s_blk[2][64 * kBlk];
+    // per half: block (a, b) of the incident element at corner o in slot 8 o + b, in the records' precision (9
+    // values of a slot of SB = 10 doubles or 12 floats, 16-byte multiples); the spare double at value FO of slot
+    // 8 o + c holds component c of the element's nodal force f_a. LDS 10 KB (FP64: 16 workgroups per CU) or 6 KB
+    // (FP32 records, exact in float: the sums below are FP64 either way)
+    constexpr int SB = sizeof(VT) == 8 ? kBlk : 12;
+    constexpr int FO = sizeof(VT) == 8 ? 9 : 10;
+    __shared__ __align__(16) VT s_blk[2][64 * SB];
+    auto fe_slot = [&](int hh, int slot) { return reinterpret_cast<double *>(s_blk[hh] + slot * SB + FO); };
     const int64_t k = blockIdx.x >> 3;
     const int64_t pair = (int64_t)(blockIdx.x & 7) * xcd_chunk + k;
     if (k >= xcd_chunk || 2 * pair >= count) return;  // uniform over the workgroup
@@ -1048,10 +1052,10 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
             const int a = local_of_offset(1 - (o >> 2), 1 - ((o >> 1) & 1), 1 - (o & 1));
             double v[9];
             load_block8(kbuf + e * kRec8, a, b, v);
-            double *dst = s_blk[h] + blk * kBlk;
+            VT *dst = s_blk[h] + blk * SB;
 #pragma unroll
-            for (int t = 0; t < 9; ++t) dst[t] = v[t];
-            if (b < 3) dst[9] = fe[e * 24 + 3 * a + b];
+            for (int t = 0; t < 9; ++t) dst[t] = (VT)v[t];  // v holds record values: exact
+            if (b < 3) *fe_slot(h, blk) = fe[e * 24 + 3 * a + b];
         }
     }
     __syncthreads();
@@ -1075,9 +1079,9 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
             if ((d0 == 0 || d0 == 2 * o0 - 1) && (d1 == 0 || d1 == 2 * o1 - 1) && (d2 == 0 || d2 == 2 * o2 - 1) &&
                 elem(o, e)) {
                 const int b = local_of_offset(1 - o0 + d0, 1 - o1 + d1, 1 - o2 + d2);
-                const double *src = s_blk[h] + (8 * o + b) * kBlk;
+                const VT *src = s_blk[h] + (8 * o + b) * SB;
 #pragma unroll
-                for (int t = 0; t < 9; ++t) acc[t] += src[t];
+                for (int t = 0; t < 9; ++t) acc[t] += (double)src[t];
             }
         }
 #pragma unroll
@@ -1093,7 +1097,7 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
         for (int o = 0; o < 8; ++o) {
             int64_t e;
             if (elem(o, e)) {
-                const double x = s_blk[h][(8 * o + c) * kBlk + 9];
+                const double x = *fe_slot(h, 8 * o + c);
                 f -= x;
                 fa += fabs(x);
             }
