@@ -52,9 +52,9 @@ struct ewfe_plan {
     int32_t n_mat = 0;
     int32_t *d_gdesc = nullptr;        // [n_rownodes][4] gather descriptors in gather order
     int32_t *d_gblob = nullptr;        // gather blobs
-    // element records (allocated on first evaluation): 36 unique 3x3 blocks per element, and nodal forces
+    // element records (allocated with the plan, FP64-sized, shared by both value precisions): 36 unique 3x3
+    // blocks per element, and nodal forces
     void *d_kbuf = nullptr;
-    int kbuf_dtype = -1;
     double *d_fe = nullptr;
 
     // 8-node evaluation in chunks: plan positions [pipe_el[c], pipe_el[c+1]) are evaluated in order (the plan
@@ -67,7 +67,7 @@ struct ewfe_plan {
     std::vector<int64_t> pipe_el, pipe_row;
     std::vector<int32_t> rorder;  // host copy of the row order (row node index per gather position)
     int32_t *d_rorder = nullptr;
-    void *aux_stream = nullptr;       // hipStream_t, created on first pipelined evaluation (default priority)
+    void *aux_stream = nullptr;       // hipStream_t of pipelined plans, created with the plan (default priority)
     std::vector<void *> chunk_events;  // hipEvent_t per chunk + 1 join event
 };
 

@@ -11,22 +11,23 @@
 //   Neo-Hooke Wa: edelweissfe/materials/neohooke/neohookepencegouformulationa.py:107-141
 //
 // MI355X design
-//  * Elements are processed colour by colour (no two elements of a colour share a node), so the
-//    scatter into CSR is a plain read-add-write with a fixed, run-independent accumulation order:
-//    deterministic FP64 without atomics.
-//  * Quadrature-point state is structure-of-arrays [component][qp][element] in plan order, so
-//    consecutive lanes touch consecutive addresses.
-//  * C3D8 family: a 64-lane workgroup holds 8 elements. Phase 0 stages nodal X/U/dU in LDS (lane = node),
-//    phase 1 runs one quadrature point per lane (Jacobian, gradients, material update) and leaves the
-//    per-QP tangent "recipe" in LDS, phase 2 runs one node per lane: the lane for node a builds the
-//    3x3 blocks K_ab for b in {a, a+1, a+2, a+3 (, a+4 if a<4)} mod 8 -- the 36 blocks of the symmetric
-//    24x24 Ke, each exactly once -- and scatters K_ab and K_ab^T.
+//  * C3D8 family, two passes. Pass 1 (k_hex8_elem): a 64-lane workgroup holds 8 elements. Nodal X/U/dU are
+//    staged in LDS (one node per lane), one quadrature point per lane runs the Jacobian, gradients and the
+//    material update and leaves the per-point tangent "recipe" in LDS, then one node per lane builds the
+//    3x3 blocks K_ab for b in {a, a+1, a+2, a+3 (, a+4 if a<4)} mod 8 -- the 36 blocks of the symmetric 24x24
+//    Ke, each exactly once -- and the element's record (36 blocks + nodal forces) is written contiguously.
+//    Pass 2 (k_gather8s on boxGen grids, k_gather8 elsewhere): per row node, the blocks of its incident
+//    elements are summed in ascending element order and the node's 3 CSR rows, P and F are written once.
+//    Deterministic FP64 without atomics: every output is one lane's sum in a fixed order.
+//  * Quadrature-point state is structure-of-arrays [component][qp][plan position], so consecutive lanes
+//    touch consecutive addresses.
 //  * The tangent uses the rank structure of the isotropic/J2/Neo-Hooke tangents instead of a dense
 //    B^T C B:  K_ab,ik = sum_q [ alpha p_a,i p_b,k + beta p_a,k p_b,i + gamma delta_ik (r_a . r_b)
 //                                - c v_a,i v_b,k ] ,  v = N p
 //    with p = r = dN/dx (small strain) or p = dN/dx (spatial), r = dN/dX (total Lagrangian).
-//  * C3D20 family: one element per 64-lane workgroup; lanes 0..26 run the quadrature points, lanes
-//    0..59 build one row (node a, component i) of Ke each.
+//  * C3D20 family: one element per 64-lane workgroup, colour by colour (no two elements of a colour share a
+//    node, so the read-add-write into the CSR values has a fixed order); lanes 0..26 run the quadrature
+//    points, lanes 0..59 build one row (node a, component i) of Ke each.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -1606,6 +1607,9 @@ extern "C" int ewfe_state_size(int mat_kind)
     }
 }
 
+static int alloc_records(ewfe_plan *p);
+static int ensure_pipeline(ewfe_plan *p);
+
 static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32_t *conn_host, int64_t n_nodes,
                        const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out, bool upload, int chunks)
 {
@@ -1710,6 +1714,10 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
             if (u.bytes && (e = hipMemcpy(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice)) != hipSuccess)
                 return fail(e, "hipMemcpy");
         }
+    }
+    if ((rc = alloc_records(p)) || (rc = ensure_pipeline(p))) {
+        ewfe_plan_destroy(p);
+        return rc;
     }
     if (nnz_out) *nnz_out = p->nnz;
     *out = p;
@@ -1857,19 +1865,17 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
     return 0;
 }
 
-template <typename VT>
-static int ensure_records(ewfe_plan *p, int dtype)
+// Element records of 8-node plans, allocated once when the plan is created (never inside an evaluation): sized
+// for FP64 values and shared by both value precisions (FP32 records use the first half of each slot range).
+static int alloc_records(ewfe_plan *p)
 {
-    if (p->d_kbuf && p->kbuf_dtype == dtype) return 0;
-    if (p->d_kbuf) (void)hipFree(p->d_kbuf);
-    p->d_kbuf = nullptr;
-    HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(VT) * ((size_t)kRec8 * p->n_el + kRecPad8)));
-    p->kbuf_dtype = dtype;
-    if (!p->d_fe) HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 24 * (size_t)p->n_el));
+    if (p->npe != 8 || p->d_kbuf) return 0;
+    HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(double) * ((size_t)kRec8 * p->n_el + kRecPad8)));
+    HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 24 * (size_t)p->n_el));
     return 0;
 }
 
-// The second stream and the chunk events of the pipelined 8-node evaluation (created once per plan, on the
+// The second stream and the chunk events of the pipelined 8-node evaluation (created with the plan, on the
 // plan's device, default priority).
 static int ensure_pipeline(ewfe_plan *p)
 {
@@ -1895,9 +1901,10 @@ static int ensure_pipeline(ewfe_plan *p)
 template <int FORM, int MAT, typename VT>
 static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s)
 {
-    int rc = ensure_records<VT>(p, sizeof(VT) == 8 ? EWFE_F64 : EWFE_F32);
-    if (rc) return rc;
-    if ((rc = ensure_pipeline(p))) return rc;
+    if (!p->d_kbuf || !p->d_fe || (p->n_chunks > 1 && !p->aux_stream)) {
+        ewfe::set_error("ewfe_eval_assemble: the plan has no element records");
+        return EWFE_EINVAL;
+    }
     const int K = p->n_chunks;
     hipStream_t g = K > 1 ? reinterpret_cast<hipStream_t>(p->aux_stream) : s;
     for (int c = 0; c < K; ++c) {
@@ -2037,7 +2044,7 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
     A.n_el = p->n_el;
     A.p_begin = A.p_end = 0;
     for (int i = 0; i < 8; ++i) A.mat.p[i] = mat_params_host[i];
-    ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // element records are allocated lazily by the plan
+    ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // the pipelined evaluation creates its second stream once
     return vals_dtype == EWFE_F64 ? dispatch<double>(pm, mat_kind, A, s) : dispatch<float>(pm, mat_kind, A, s);
 }
 

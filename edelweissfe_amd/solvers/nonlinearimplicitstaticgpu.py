@@ -31,7 +31,12 @@ Differences that are part of the design (not of the semantics):
     linear systems are solved to ``pcgToleranceFP32`` (default 1e-4): an FP32 tangent already limits the
     Newton contraction to ~1e-4..1e-3 per iteration, so a tighter correction buys nothing, while the
     convergence test (FP64 residual and correction) is unchanged;
-  * the 1e-7 absolute floor of the flux test is the option ``fluxResidualFloor`` (default 1e-7).
+  * the 1e-7 absolute floor of the flux test is the option ``fluxResidualFloor`` (default 1e-7);
+  * ``linsolverRefinement=k`` adds k steps of iterative refinement to every FP64 linear solve: the true
+    residual r = b - K x is formed by an FP64 SpMV and K d = r is solved (relative tolerance 1e-6) for the
+    correction x += d. A PCG stopped at a relative residual of 1e-14 leaves an error of cond(K) * 1e-14 in x;
+    refinement brings it to the backward-stable level of a direct factorisation (the reference's solvers),
+    cond(K) * 1e-16.
 """
 
 from __future__ import annotations
@@ -89,6 +94,7 @@ class NISTGPU:
         "pcgTolerance": 1e-10,
         "pcgToleranceFP32": 1e-4,
         "pcgMaxIter": 50000,
+        "linsolverRefinement": 0,
         "fluxResidualFloor": 1e-7,
         "device": "cuda",
     }
@@ -352,8 +358,20 @@ class NISTGPU:
             toc = _time.perf_counter()
             self.computationTimes["multigrid setup"] += toc - tic
             tic = toc
+        precond = "amg" if amg else "jacobi"
         it, rel = self.assembly.pcg(A, b, x, rows, rtol=rtol, maxit=self.options["pcgMaxIter"],
-                                    check_every=4 if amg else 32, precond="amg" if amg else "jacobi", setup=False)
+                                    check_every=4 if amg else 32, precond=precond, setup=False)
+        if A.dtype == self.torch.float64:
+            for _ in range(self.options["linsolverRefinement"]):
+                r = self.assembly.spmv(A, x, self.theDofManager.constructDofVector())
+                self.torch.sub(b, r, out=r)  # Dirichlet rows: unit rows with x = b, so r = 0 there
+                if not bool(r.any().item()):
+                    break
+                d = self.theDofManager.constructDofVector()
+                it2, _ = self.assembly.pcg(A, r, d, rows, rtol=1e-6, maxit=self.options["pcgMaxIter"],
+                                           check_every=4 if amg else 32, precond=precond, setup=False)
+                it += it2
+                x += d
         self.linearSolveIterations.append((it, rel))
         self.journal.message("linear solve: {:} PCG iterations, relative residual {:.2e}".format(it, rel),
                              self.identification, 2)

@@ -2,8 +2,9 @@
 
 Restates, with the reference's names and semantics:
   * TimeStep                     edelweissfe/timesteppers/timestep.py
-  * AdaptiveTimeStepper          edelweissfe/timesteppers/adaptivetimestepper.py:60-183 (zero-size first
-                                 increment, growth x1.1 after 3 good increments, cutback, limits)
+  * IncrementController          the increment logic of edelweissfe/timesteppers/adaptivetimestepper.py:60-183
+                                 (zero-size first increment, growth x1.1 after 3 good increments, cutback,
+                                 limits), as an iterator of its own design
   * AdaptiveStep                 edelweissfe/steps/adaptivestep.py:56-156 (defaults and options)
   * Dirichlet step action        edelweissfe/stepactions/dirichlet.py:53-158 (delta per node set,
                                  linear amplitude, components)
@@ -47,6 +48,11 @@ class ConditionalStop(Exception):
 
 
 class TimeStep:
+    """One increment as the solver sees it (edelweissfe/timesteppers/timestep.py): its number in the step, its
+    size and the step progress at its end (both fractions of the step), and the matching times."""
+
+    __slots__ = ("number", "stepProgressIncrement", "stepProgress", "timeIncrement", "stepTime", "totalTime")
+
     def __init__(self, number, stepProgressIncrement, stepProgress, timeIncrement, stepTime, totalTime):
         self.number = number
         self.stepProgressIncrement = stepProgressIncrement
@@ -56,67 +62,76 @@ class TimeStep:
         self.totalTime = totalTime
 
 
-class AdaptiveTimeStepper:
-    identification = "AdaptiveTimeStepper"
+class IncrementController:
+    """Adaptive increment sizes of one step, with the behaviour of the reference's AdaptiveTimeStepper
+    (edelweissfe/timesteppers/adaptivetimestepper.py:90-183):
 
-    def __init__(self, currentTime, stepLength, startIncrement, maxIncrement, minIncrement, maxNumberIncrements,
-                 journal=None, increaseFactor=1.1):
-        self.nPassedGoodIncrements = 0
-        self.incrementCounter = 0
-        self.startIncrement = startIncrement
-        self.maxIncrement = maxIncrement
-        self.minIncrement = minIncrement
-        self.maxNumberIncrements = maxNumberIncrements
-        self.finishedStepProgress = 0.0
-        self.increment = min(startIncrement, maxIncrement)
-        self.allowedToIncreasedNext = True
-        self.currentTime = currentTime
-        self.stepLength = stepLength
-        self.journal = journal
-        self.increaseFactor = increaseFactor
+    * the first increment handed out has size 0 (it only evaluates the start state);
+    * sizes are fractions of the step, clipped to what is left of it;
+    * once three increments in a row have been handed out since the last discard, and the solver did not veto
+      growth for the last one (``holdSize``), every request grows the size by ``growth`` up to ``largest``;
+    * ``discard(f)`` takes back the last increment and shrinks the size to f times itself, clamped to
+      [smallest, largest]; discarding the zero-size increment, or shrinking a size already at ``smallest``,
+      raises ReachedMinIncrementSize;
+    * asking for another increment after more than ``maxCount`` raises ReachedMaxIncrements.
 
-    def generateTimeStep(self):
-        while self.finishedStepProgress < (1.0 - 1e-15):
-            remainder = 1.0 - self.finishedStepProgress
-            if remainder < self.increment:
-                self.increment = remainder
-            theIncrement = self.increment if self.incrementCounter > 0 else 0.0
-            dT = self.stepLength * theIncrement
-            self.finishedStepProgress += theIncrement
-            endInStep = self.stepLength * self.finishedStepProgress
-            yield TimeStep(self.incrementCounter, theIncrement, self.finishedStepProgress, dT, endInStep,
-                           self.currentTime + endInStep)
-            if self.incrementCounter > self.maxNumberIncrements:
-                raise ReachedMaxIncrements()
-            if self.nPassedGoodIncrements >= 3 and self.allowedToIncreasedNext:
-                self.increment *= self.increaseFactor
-                if self.increment > self.maxIncrement:
-                    self.increment = self.maxIncrement
-            self.allowedToIncreasedNext = True
-            self.incrementCounter += 1
-            self.nPassedGoodIncrements += 1
+    Iterating over the controller yields TimeStep objects; the bookkeeping of an increment (counting it,
+    growing the size) happens when the next one is requested, so a discard in between acts on the increment
+    just handed out."""
 
-    def preventIncrementIncrease(self):
-        self.allowedToIncreasedNext = False
+    def __init__(self, startTime, stepLength, firstSize, largest, smallest, maxCount, growth=1.1):
+        self._t0 = startTime
+        self._length = stepLength
+        self._largest = largest
+        self._smallest = smallest
+        self._maxCount = maxCount
+        self._growth = growth
+        self._size = min(firstSize, largest)  # size of the next nonzero increment
+        self._done = 0.0                      # step progress reached by the increments handed out
+        self._count = 0                       # number of the increment handed out last
+        self._streak = 0                      # increments handed out since the start or the last discard
+        self._vetoGrowth = False
+        self._started = False
 
-    def reduceNextIncrement(self, scaleFactor):
-        if self.increment == self.minIncrement:
+    def __iter__(self):
+        return self
+
+    def _settleLast(self):
+        """Bookkeeping of the increment handed out last, done when the next one is requested."""
+        if self._count > self._maxCount:
+            raise ReachedMaxIncrements()
+        if self._streak >= 3 and not self._vetoGrowth:
+            self._size = min(self._size * self._growth, self._largest)
+        self._vetoGrowth = False
+        self._count += 1
+        self._streak += 1
+
+    def __next__(self):
+        if self._started:
+            self._settleLast()
+        if not self._done < 1.0 - 1e-15:
+            raise StopIteration
+        self._started = True
+        self._size = min(self._size, 1.0 - self._done)
+        size = self._size if self._count > 0 else 0.0
+        self._done += size
+        inStep = self._length * self._done
+        return TimeStep(self._count, size, self._done, self._length * size, inStep, self._t0 + inStep)
+
+    def holdSize(self):
+        """The next request must not grow the size (slow convergence of the increment just solved)."""
+        self._vetoGrowth = True
+
+    def discard(self, factor):
+        """Takes back the increment handed out last and shrinks the size by `factor`."""
+        if self._count == 0:
             raise ReachedMinIncrementSize()
-        newIncrement = self.increment * scaleFactor
-        if newIncrement > self.maxIncrement:
-            self.increment = self.maxIncrement
-        elif newIncrement < self.minIncrement:
-            self.increment = self.minIncrement
-        else:
-            self.increment = newIncrement
-
-    def discardAndChangeIncrement(self, scaleFactor):
-        if self.incrementCounter == 0:
+        self._done -= self._size
+        self._count -= 1
+        self._streak = 0
+        if self._size == self._smallest:
             raise ReachedMinIncrementSize()
-        self.finishedStepProgress -= self.increment
-        self.incrementCounter -= 1
-        self.nPassedGoodIncrements = 0
-        self.reduceNextIncrement(scaleFactor)
+        self._size = min(max(self._size * factor, self._smallest), self._largest)
 
 
 class AdaptiveStep:
@@ -144,7 +159,7 @@ class AdaptiveStep:
         self.criticalIter = int(kw.get("criticaliter", self.defaultCriticalIter))
         self.maxGrowIter = int(kw.get("maxgrowiter", self.defaultMaxGrowingIter))
         self.cutbackFactor = float(kw.get("cutbackfactor", self.defaultCutbackFactor))
-        self.incrementGenerator = AdaptiveTimeStepper(model.time, self.length, self.startIncrementSize,
+        self.incrementGenerator = IncrementController(model.time, self.length, self.startIncrementSize,
                                                       self.maxIncrementSize, self.minIncrementSize,
                                                       self.maxNumberIncrements)
         self.actions = {k: {} for k in ("dirichlet", "nodeforces", "distributedload", "bodyforce", "geostatic",
@@ -153,13 +168,13 @@ class AdaptiveStep:
             self.actions.setdefault(typ, {}).update(acts)
 
     def getTimeStep(self):
-        return self.incrementGenerator.generateTimeStep()
+        return iter(self.incrementGenerator)
 
     def discardAndChangeIncrement(self, cutbackFactor):
-        return self.incrementGenerator.discardAndChangeIncrement(cutbackFactor)
+        return self.incrementGenerator.discard(cutbackFactor)
 
     def preventIncrementIncrease(self):
-        return self.incrementGenerator.preventIncrementIncrease()
+        return self.incrementGenerator.holdSize()
 
 
 def _linear(x):

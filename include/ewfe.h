@@ -134,7 +134,7 @@ int ewfe_plan_set_materials(ewfe_plan *plan, int32_t n_mat, const double *params
  *   state_n_dev      f64[nstate * nqp * n_el] converged state, plan order; NULL if nstate == 0
  *   state_t_dev      same layout, receives the trial state
  *   P_dev, F_dev     f64[n_dof] out: P = -f_int (element "P"), F = sum |Pe|
- *   csr_vals_dev     f64 or f32 [nnz] out (vals_dtype); the tangent is computed in that precision
+ *   csr_vals_dev     f64 or f32 [nnz] out (vals_dtype): the assembled tangent stored in that precision
  *   pnewdt_dev       f64[1] out: 1e36, or < 1 if a quadrature point requests a cutback
  */
 int ewfe_eval_assemble(const ewfe_plan *plan, int mat_kind, const double *mat_params_host,

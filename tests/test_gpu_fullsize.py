@@ -2,7 +2,8 @@
 
 The oracle cannot run these sizes, so each run is checked against what the mechanics fixes:
 
-* 64^3 C3D8 J2 (262,144 elements), uniaxial stress to 1 % strain in 10 increments on a jittered cube with planar
+* 64^3 C3D8 J2 (262,144 elements; FP64 and FP32 tangent) and 128^3 (2,097,152 elements, FP32 tangent), uniaxial
+  stress to 1 % strain in 10 increments on a jittered cube with planar
   faces (rollers on x=0, y=0, z=0, prescribed uz on z=1): isoparametric elements reproduce the homogeneous
   solution exactly, so every quadrature point must carry the closed-form uniaxial J2 state
   sigma_zz = (fy0 + H eps) / (1 + H / E), eps_xx = eps_yy = -nu sigma / E - kappa / 2, all other stresses zero;
@@ -32,16 +33,23 @@ def _rollers(model, top_uz):
                           "z1": Dirichlet("z1", model, "gen_front", {2: top_uz})}}
 
 
-def test_j2_uniaxial_64cube_closed_form():
+# The FP32 tangent converges linearly (contraction ~1e-3 per iteration): the iterate that passes the FP64
+# convergence test (|ddU|inf < 1e-8) is off the exact solution by about the contraction times that correction,
+# which at these meshes is ~1e-7 of the stress scale; FP64 Newton converges quadratically to rounding.
+CLOSED_FORM_BAR = {"fp64": 1e-10, "fp32": 1e-6}
+
+
+@pytest.mark.parametrize("tangent,n", [("fp64", 64), ("fp32", 64), ("fp32", 128)])
+def test_j2_uniaxial_cube_closed_form(tangent, n):
     from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
     from edelweissfe_amd.stepping import AdaptiveStep
 
     E, nu, fy0, H = 210000.0, 0.3, 250.0, 1000.0
-    print("case J2 64^3 uniaxial, 10 increments", flush=True)
-    X, conn, sets = jittered_box(64, "C3D8", amp=0.05, seed=0)
+    print(f"case J2 {n}^3 uniaxial, 10 increments, {tangent} tangent", flush=True)
+    X, conn, sets = jittered_box(n, "C3D8", amp=0.05, seed=0)
     model, block = one_block_model("C3D8", "vonmises", [E, nu, fy0, H, 0.0, 0.0], X, conn, sets=sets)
     step = AdaptiveStep(1, model, _rollers(model, 0.01), maxInc=0.1, startInc=0.1, maxIter=25)
-    solver = NISTGPU()
+    solver = NISTGPU(tangentPrecision=tangent)
     t0 = time.perf_counter()
     solver.solveStep(step, model)
     its = [len(h) for h in solver.history]
@@ -61,9 +69,10 @@ def test_j2_uniaxial_64cube_closed_form():
     e_eshear = np.abs(Ep[:, 3:]).max() / eps
     print(f"sigma_zz {sig:.6f}: max rel err {e_szz:.2e}; other stresses {e_soff:.2e}; eps_zz {e_ezz:.2e}; "
           f"lateral {e_elat:.2e}; shear strain {e_eshear:.2e}", flush=True)
-    # measured ~5e-13: the Newton loop converges the homogeneous state to rounding
-    assert e_szz < 1e-10 and e_soff < 1e-10
-    assert e_ezz < 1e-10 and e_elat < 1e-10 and e_eshear < 1e-10
+    # measured ~5e-13 (FP64): the Newton loop converges the homogeneous state to rounding
+    bar = CLOSED_FORM_BAR[tangent]
+    assert e_szz < bar and e_soff < bar
+    assert e_ezz < bar and e_elat < bar and e_eshear < bar
 
 
 def _lateral_stretch(lam, mu, K):
@@ -81,16 +90,17 @@ def _lateral_stretch(lam, mu, K):
     return lat
 
 
-def test_c3d20tl_neohooke_40cube_stretch_closed_form():
+@pytest.mark.parametrize("tangent", ["fp64", "fp32"])
+def test_c3d20tl_neohooke_40cube_stretch_closed_form(tangent):
     from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
     from edelweissfe_amd.stepping import AdaptiveStep
 
     mu, K = 80.0, 200.0
-    print("case C3D20TL Neo-Hooke 40^3 stretch 1.5, 20 increments", flush=True)
+    print(f"case C3D20TL Neo-Hooke 40^3 stretch 1.5, 20 increments, {tangent} tangent", flush=True)
     X, conn, sets = jittered_box(40, "C3D20TL", amp=0.03, seed=0)
     model, block = one_block_model("C3D20TL", "neohookeanwa", [mu, K], X, conn, sets=sets)
     step = AdaptiveStep(1, model, _rollers(model, 0.5), maxInc=0.05, startInc=0.05, maxIter=25)
-    solver = NISTGPU()
+    solver = NISTGPU(tangentPrecision=tangent)
     t0 = time.perf_counter()
     solver.solveStep(step, model)
     its = [len(h) for h in solver.history]
@@ -110,7 +120,8 @@ def test_c3d20tl_neohooke_40cube_stretch_closed_form():
     e_gs = np.abs(Eg[:, 3:]).max() / G[2]
     print(f"lateral stretch {lat:.12f}, tau_zz {tau_zz:.6f}: max rel err {e_tzz:.2e}; other stresses {e_toff:.2e}; "
           f"Green strain {e_g:.2e}; shear {e_gs:.2e}", flush=True)
-    assert e_tzz < 1e-10 and e_toff < 1e-10 and e_g < 1e-10 and e_gs < 1e-10  # measured ~1.5e-13
+    bar = CLOSED_FORM_BAR[tangent]
+    assert e_tzz < bar and e_toff < bar and e_g < bar and e_gs < bar  # measured ~1.5e-13 (FP64)
 
 
 def test_adjoint_48cube_directional_derivative():
@@ -164,3 +175,66 @@ def test_adjoint_48cube_directional_derivative():
     print(f"directional derivative: adjoint {ad:.10e} central difference {fd:.10e} "
           f"(rel {abs(fd - ad) / abs(ad):.2e}); total {time.perf_counter() - t0:.1f} s", flush=True)
     assert abs(fd - ad) <= 1e-6 * abs(ad)  # measured 6e-9 (h = 1e-6: truncation ~h^2, rounding ~1e-16 / h)
+
+
+def test_adjoint_48cube_fd_32_elements():
+    """The adjoint gradient at 48^3 (110,592 mu_e) against central differences of J on 32 random elements,
+    h = 1e-6 mu_e. Every perturbed solve starts from the converged displacement with the full load held
+    (Newton then needs a few iterations), and is converged to |R| <= 1e-12 of the averaged flux with one
+    refinement step per linear solve, so that J carries ~1e-15 of relative noise against the ~1e-11 relative
+    change that one element's perturbation makes."""
+    import torch
+
+    from edelweissfe_amd.adjoint import parameter_gradient, update_element_parameters
+    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
+    from edelweissfe_amd.stepping import AdaptiveStep, Dirichlet, NodeForces
+
+    print("case adjoint 48^3 C3D8TL Neo-Hooke, central differences on 32 elements", flush=True)
+    X, conn, sets = jittered_box(48, "C3D8TL", amp=0.05, seed=0)
+    theta = 80.0 * np.exp(0.2 * np.random.default_rng(1).standard_normal(len(conn)))
+    top = np.asarray(sets["gen_front"])
+    fz = 8.0 / len(top)
+    tight = {"fieldCorrectionTolerance": {"displacement": 1e-14},
+             "fluxResidualTolerance": {"displacement": 1e-12},
+             "fluxResidualToleranceAlternative": {"displacement": 1e-12}}
+    model, block = one_block_model("C3D8TL", "neohookeanwa", [80.0, 200.0], X, conn, elParams=theta, sets=sets)
+    fix = Dirichlet("fix", model, "gen_back", {0: 0.0, 1: 0.0, 2: 0.0})
+    acts = {"dirichlet": {"fix": fix}, "nodeforces": {"pull": NodeForces("pull", model, "gen_front", [0, 0, fz])}}
+    solver = NISTGPU(tight, pcgTolerance=1e-14, fluxResidualFloor=0.0, pcgMaxIter=100000, linsolverRefinement=1)
+    t0 = time.perf_counter()
+    solver.solveStep(AdaptiveStep(1, model, acts, maxInc=0.5, startInc=0.5, maxIter=30), model)
+    U0 = model.U.clone()
+    dofs = torch.as_tensor((3 * top[:, None] + np.arange(3)).ravel(), device="cuda")
+    rng = np.random.default_rng(2)
+    target = U0[dofs] * torch.as_tensor(1 + 0.1 * rng.standard_normal(len(dofs)), device="cuda")
+    dJdU = torch.zeros_like(U0)
+    dJdU[dofs] = U0[dofs] - target
+    g, _, (it, rr) = parameter_gradient(solver, model, dJdU, rtol=1e-14)
+    g = g.cpu().numpy()
+    print(f"forward + adjoint {time.perf_counter() - t0:.1f} s (adjoint PCG {it} iterations, {rr:.1e})", flush=True)
+
+    def J(th):
+        update_element_parameters(solver.assembly, th)
+        model.U = U0.clone()
+        hold = NodeForces("hold", model, "gen_front", [0, 0, fz])
+        hold.applyAtStepEnd(model)  # the full load, held constant over the step
+        step = AdaptiveStep(2, model, {"dirichlet": {"fix": fix}, "nodeforces": {"hold": hold}}, maxInc=1.0,
+                            startInc=1.0, maxIter=30)
+        solver.solveStep(step, model)
+        return 0.5 * float(((model.U[dofs] - target) ** 2).sum().item())
+
+    els = rng.choice(len(conn), 32, replace=False)
+    gmax = np.abs(g).max()
+    worst = 0.0
+    for e in els:
+        h = 1e-6 * theta[e]
+        tp, tm = theta.copy(), theta.copy()
+        tp[e] += h
+        tm[e] -= h
+        fd = (J(tp) - J(tm)) / (2 * h)
+        err = abs(fd - g[e])
+        worst = max(worst, err / (abs(g[e]) + 1e-3 * gmax))
+        print(f"element {e}: adjoint {g[e]:.10e} central difference {fd:.10e} (|diff| {err:.2e})", flush=True)
+        assert err <= 1e-2 * abs(g[e]) + 1e-4 * gmax
+    print(f"32 elements in {time.perf_counter() - t0:.1f} s; worst |fd - adjoint| / (|g_e| + 1e-3 max|g|) = "
+          f"{worst:.2e}", flush=True)

@@ -11,7 +11,7 @@ import pytest
 import decks
 from edelweissfe_amd import _lib
 from edelweissfe_amd.model import FEModel, boxGenArrays, jitterNodes
-from edelweissfe_amd.stepping import AdaptiveTimeStepper, ReachedMinIncrementSize
+from edelweissfe_amd.stepping import IncrementController, ReachedMaxIncrements, ReachedMinIncrementSize
 from oracle import oracle as O
 
 EWFE_EINVAL = -1  # include/ewfe.h
@@ -207,9 +207,9 @@ def test_dofmanager_matches_oracle_numbering():
 
 def test_time_stepper_matches_oracle_with_cutbacks():
     seq_a, seq_b = [], []
-    ts = AdaptiveTimeStepper(0.0, 2.0, 0.3, 0.25, 1e-3, 50)
+    ts = IncrementController(0.0, 2.0, 0.3, 0.25, 1e-3, 50)
     ora = O.time_steps(2.0, 0.3, 0.25, 1e-3, 50)
-    ga, gb = ts.generateTimeStep(), ora.gen()
+    ga, gb = iter(ts), ora.gen()
     for k in range(40):
         try:
             a = next(ga)
@@ -225,17 +225,28 @@ def test_time_stepper_matches_oracle_with_cutbacks():
         seq_a.append((a.number, a.stepProgressIncrement, a.stepProgress, a.timeIncrement))
         seq_b.append((b["number"], b["stepProgressIncrement"], b["stepProgress"], b["timeIncrement"]))
         if k in (3, 4, 9):
-            ts.discardAndChangeIncrement(0.25)
+            ts.discard(0.25)
             ora.discard(0.25)
         if k == 6:
-            ts.preventIncrementIncrease()
+            ts.holdSize()
             ora.allowInc = False
     assert seq_a == seq_b and len(seq_a) > 8
-    t2 = AdaptiveTimeStepper(0.0, 1.0, 1.0, 1.0, 1.0, 10)
-    g = t2.generateTimeStep()
+    t2 = IncrementController(0.0, 1.0, 1.0, 1.0, 1.0, 10)
+    g = iter(t2)
     next(g)
     with pytest.raises(ReachedMinIncrementSize):
-        t2.discardAndChangeIncrement(0.5)
+        t2.discard(0.5)
+    # a size already at the minimum cannot shrink; more than maxCount increments stop the step
+    t3 = IncrementController(0.0, 1.0, 0.1, 0.1, 0.1, 100)
+    next(t3), next(t3)
+    with pytest.raises(ReachedMinIncrementSize):
+        t3.discard(0.5)
+    t4 = IncrementController(5.0, 1.0, 0.01, 0.01, 0.01, 3)
+    steps = []
+    with pytest.raises(ReachedMaxIncrements):
+        for s in t4:
+            steps.append((s.number, s.totalTime))
+    assert [n for n, _ in steps] == [0, 1, 2, 3, 4] and abs(steps[-1][1] - 5.04) < 1e-12
 
 
 def _pipeline_orders(conn, node_dof, n_dof, chunks):
