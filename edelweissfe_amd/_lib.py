@@ -50,6 +50,8 @@ SIGNATURES = {
     "ewfe_pcg": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _i64, _dbl, ctypes.c_int, ctypes.c_int, _vp,
                                 _P(ctypes.c_int), _P(_dbl), _vp]),
     "ewfe_pcg_workspace_size": (_i64, [_vp]),
+    "ewfe_pcg_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _i64, _dbl, ctypes.c_int, ctypes.c_int,
+                                        _P(ctypes.c_int), _P(_dbl), _vp]),
     "ewfe_mg_create": (ctypes.c_int, [_vp, ctypes.c_int, _dbl, _P(_vp), _P(ctypes.c_int)]),
     "ewfe_mg_destroy": (None, [_vp]),
     "ewfe_mg_level_info": (ctypes.c_int, [_vp, ctypes.c_int, _P(_i64), _P(_i64)]),

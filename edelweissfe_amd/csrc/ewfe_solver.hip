@@ -16,6 +16,7 @@
 // not all node dofs fall back to a row-wise CSR SpMV with point Jacobi.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "ewfe.h"
@@ -622,6 +623,84 @@ extern "C" int ewfe_pcg(const ewfe_plan *p, const void *vals, int vals_dtype, co
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     double *ws = reinterpret_cast<double *>(workspace);
+    if (vals_dtype == EWFE_F64)
+        return pcg<double>(p, (const double *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
+                           relres_out, s);
+    return pcg<float>(p, (const float *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
+                      relres_out, s);
+}
+
+// ------------------------------------------------------------------------------------------------
+// ewfe_pcg_noalloc: ewfe_pcg with its workspace in a file-scope device array instead of a caller's buffer, for
+// callers that cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+// Every GPU holds its own copy of the array from the module load to the end of the process. In host code the
+// array's name is only a stand-in for the symbol: its device address is looked up once per device with
+// hipGetSymbolAddress (with that device current) and only that address is handed to kernels. Calls on one device
+// share the array, so calls that overlap on one GPU race; the scalars a solve reads before writing are cleared by
+// a fill kernel at the start of every call.
+// ------------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[EWFE_PCG_NOALLOC_MAX_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {nullptr};
+
+// device address of this device's scratch array (looked up on the first call on the device)
+int scratch_address(void **out)
+{
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        ewfe::set_error("ewfe_pcg_noalloc: device index beyond the supported range");
+        return EWFE_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *ptr = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) {
+            ewfe::set_error("ewfe_pcg_noalloc: hipGetSymbolAddress returned no address");
+            return EWFE_EHIP;
+        }
+        g_scratch_addr[dev] = ptr;
+    }
+    *out = g_scratch_addr[dev];
+    return 0;
+}
+
+__global__ void k_clear_scalars(double *scal)
+{
+    if (threadIdx.x < S_COUNT) scal[threadIdx.x] = 0.0;
+}
+}  // namespace
+
+extern "C" int ewfe_pcg_noalloc(const ewfe_plan *p, const void *vals, int vals_dtype, const double *b, double *x,
+                                const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every,
+                                int *iters_out, double *relres_out, void *stream)
+{
+    if (!p || !p->d_indptr || !vals || !b || !x || maxit < 0 || check_every <= 0 ||
+        (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) || !(rtol >= 0.0) || (n_rows > 0 && !rows_dev)) {
+        ewfe::set_error("ewfe_pcg_noalloc: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    const int64_t need = ewfe_pcg_workspace_size(p);
+    if (need > (int64_t)EWFE_PCG_NOALLOC_MAX_BYTES) {
+        ewfe::set_error("ewfe_pcg_noalloc: the system needs " + std::to_string(need) +
+                        " bytes of workspace, beyond the fixed scratch of " +
+                        std::to_string((int64_t)EWFE_PCG_NOALLOC_MAX_BYTES) + " bytes (about " +
+                        std::to_string(((int64_t)EWFE_PCG_NOALLOC_MAX_BYTES - 3 * 8 * kBlocks - 8 * S_COUNT) / 56) +
+                        " dofs); use ewfe_pcg with a workspace of ewfe_pcg_workspace_size bytes");
+        return EWFE_ERANGE;
+    }
+    void *addr = nullptr;
+    int rc = scratch_address(&addr);
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    double *ws = reinterpret_cast<double *>(addr);
+    // the scalars sit after the vectors and the partial sums (the layout of pcg())
+    hipLaunchKernelGGL(k_clear_scalars, dim3(1), dim3(64), 0, s, ws + 7 * p->n_dof + 3 * kBlocks);
+    HIP_CHECK(hipGetLastError());
     if (vals_dtype == EWFE_F64)
         return pcg<double>(p, (const double *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
                            relres_out, s);

@@ -56,7 +56,10 @@ enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1 };
 #define EWFE_EINVAL -1   /* bad argument */
 #define EWFE_EHIP -2     /* HIP runtime error */
 #define EWFE_ENOMEM -3   /* allocation failed */
-#define EWFE_ERANGE -4   /* size beyond int32 CSR indexing */
+#define EWFE_ERANGE -4   /* size beyond int32 CSR indexing, or beyond a fixed scratch size */
+
+/* Size of the fixed device scratch of ewfe_pcg_noalloc (64 MiB: systems of up to about 1.19 million dofs). */
+#define EWFE_PCG_NOALLOC_MAX_BYTES (64ull << 20)
 
 const char *ewfe_last_error(void);
 int ewfe_version(void);
@@ -174,6 +177,17 @@ int64_t ewfe_pcg_workspace_size(const ewfe_plan *plan);
 int ewfe_pcg(const ewfe_plan *plan, const void *csr_vals_dev, int vals_dtype, const double *b_dev, double *x_dev,
              const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every, void *workspace_dev,
              int *iters_out, double *relres_out, void *stream);
+
+/* ewfe_pcg without a workspace argument: the workspace is a fixed device array of EWFE_PCG_NOALLOC_MAX_BYTES inside
+ * the library (one per GPU, held from the library's load to the end of the process). No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); it calls no allocating routine. Same arguments, stopping
+ * rule, results and errors as ewfe_pcg (bitwise the same iterates); fails with EWFE_ERANGE when
+ * ewfe_pcg_workspace_size(plan) exceeds EWFE_PCG_NOALLOC_MAX_BYTES. Every call on one GPU shares its array:
+ * calls that overlap on one GPU (several host threads or streams) race. */
+int ewfe_pcg_noalloc(const ewfe_plan *plan, const void *csr_vals_dev, int vals_dtype, const double *b_dev,
+                     double *x_dev, const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every,
+                     int *iters_out, double *relres_out, void *stream);
 
 /* Smoothed-aggregation multigrid preconditioner for the same systems (the reference's iterative solver is
  * GMRES with a pyamg smoothed-aggregation preconditioner, edelweissfe/linsolve/gmres/gmres.py:70-88, selected
