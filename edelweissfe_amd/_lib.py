@@ -50,6 +50,7 @@ SIGNATURES = {
     "ewfe_pcg": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _i64, _dbl, ctypes.c_int, ctypes.c_int, _vp,
                                 _P(ctypes.c_int), _P(_dbl), _vp]),
     "ewfe_pcg_workspace_size": (_i64, [_vp]),
+    "ewfe_residual": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "ewfe_pcg_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _i64, _dbl, ctypes.c_int, ctypes.c_int,
                                         _P(ctypes.c_int), _P(_dbl), _vp]),
     "ewfe_mg_create": (ctypes.c_int, [_vp, ctypes.c_int, _dbl, _P(_vp), _P(ctypes.c_int)]),

@@ -190,6 +190,12 @@ class ElementAssembly:
         check(lib().ewfe_spmv(self._plan, ptr(vals), self._dtype_code(vals), ptr(x), ptr(y), self._stream()))
         return y
 
+    def residual(self, vals, b, x, r):
+        """r = b - K x with row sums accurate to twice the working precision (refinement residuals)."""
+        check(lib().ewfe_residual(self._plan, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(r),
+                                  self._stream()))
+        return r
+
     def multigrid(self, degree=2, ratio=20.0):
         """The smoothed-aggregation hierarchy of this plan (built on first use; patterns only)."""
         if self._mg is None:

@@ -16,6 +16,7 @@
 // not all node dofs fall back to a row-wise CSR SpMV with point Jacobi.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 
@@ -608,6 +609,57 @@ extern "C" int ewfe_spmv(const ewfe_plan *p, const void *vals, int vals_dtype, c
         hipLaunchKernelGGL(k_spmv<double>, dim3(kBlocks), dim3(kThreads), 0, s, M, (const double *)vals, x, y);
     else
         hipLaunchKernelGGL(k_spmv<float>, dim3(kBlocks), dim3(kThreads), 0, s, M, (const float *)vals, x, y);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// r = b - K x with every row's sum carried in twice the working precision: the compensated dot product Dot2 of
+// Ogita, Rump and Oishi ("Accurate sum and dot product", SIAM J. Sci. Comput. 26, 2005): each product is split
+// exactly into p + pe by an FMA, each running sum exactly into t + se by Knuth's TwoSum, and the error terms are
+// summed beside it. The result is as accurate as a sum in 106-bit arithmetic rounded once, which is what iterative
+// refinement needs to reach a working-precision solution however ill-conditioned K is. One lane per row: the call
+// serves refinement steps, not the Krylov iterations.
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void k_residual_dot2(const int32_t *__restrict__ indptr,
+                                                          const int32_t *__restrict__ indices,
+                                                          const VT *__restrict__ vals, const double *__restrict__ b,
+                                                          const double *__restrict__ x, double *__restrict__ r,
+                                                          int64_t n)
+{
+#pragma clang fp contract(off)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double s = b[i], c = 0.0;
+        for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+            const double a = (double)vals[k], y = -x[indices[k]];
+            const double p = a * y;
+            const double pe = fma(a, y, -p);  // a y = p + pe exactly
+            const double t = s + p;
+            const double z = t - s;
+            const double se = (s - (t - z)) + (p - z);  // s + p = t + se exactly
+            s = t;
+            c += se + pe;
+        }
+        r[i] = s + c;
+    }
+}
+
+extern "C" int ewfe_residual(const ewfe_plan *p, const void *vals, int vals_dtype, const double *b, const double *x,
+                             double *r, void *stream)
+{
+    if (!p || !p->d_indptr || !vals || !b || !x || !r || (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32)) {
+        ewfe::set_error("ewfe_residual: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int64_t n = p->n_dof;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 65536);
+    if (vals_dtype == EWFE_F64)
+        hipLaunchKernelGGL(k_residual_dot2<double>, dim3(blocks), dim3(kThreads), 0, s, p->d_indptr, p->d_indices,
+                           (const double *)vals, b, x, r, n);
+    else
+        hipLaunchKernelGGL(k_residual_dot2<float>, dim3(blocks), dim3(kThreads), 0, s, p->d_indptr, p->d_indices,
+                           (const float *)vals, b, x, r, n);
     HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -32,11 +32,12 @@ Differences that are part of the design (not of the semantics):
     Newton contraction to ~1e-4..1e-3 per iteration, so a tighter correction buys nothing, while the
     convergence test (FP64 residual and correction) is unchanged;
   * the 1e-7 absolute floor of the flux test is the option ``fluxResidualFloor`` (default 1e-7);
-  * ``linsolverRefinement=k`` adds k steps of iterative refinement to every FP64 linear solve: the true
-    residual r = b - K x is formed by an FP64 SpMV and K d = r is solved (relative tolerance 1e-6) for the
-    correction x += d. A PCG stopped at a relative residual of 1e-14 leaves an error of cond(K) * 1e-14 in x;
-    refinement brings it to the backward-stable level of a direct factorisation (the reference's solvers),
-    cond(K) * 1e-16.
+  * ``linsolverRefinement=k`` adds k steps of iterative refinement to every FP64 linear solve: the residual
+    r = b - K x is formed with row sums in twice the working precision (ewfe_residual) and K d = r is solved to
+    the same PCG tolerance for the correction x += d. A PCG stopped at a relative residual of 1e-14 leaves an
+    error of up to cond(K) * 1e-14 in x, a direct factorisation with refinement in working precision (the
+    reference's solvers) up to cond(K) * 1e-16; with the extended residuals the refined x is accurate to
+    working precision whenever cond(K) * 1e-16 < 1.
 """
 
 from __future__ import annotations
@@ -363,12 +364,12 @@ class NISTGPU:
                                     check_every=4 if amg else 32, precond=precond, setup=False)
         if A.dtype == self.torch.float64:
             for _ in range(self.options["linsolverRefinement"]):
-                r = self.assembly.spmv(A, x, self.theDofManager.constructDofVector())
-                self.torch.sub(b, r, out=r)  # Dirichlet rows: unit rows with x = b, so r = 0 there
+                # Dirichlet rows are unit rows with x = b, so r = 0 there
+                r = self.assembly.residual(A, b, x, self.theDofManager.constructDofVector())
                 if not bool(r.any().item()):
                     break
                 d = self.theDofManager.constructDofVector()
-                it2, _ = self.assembly.pcg(A, r, d, rows, rtol=1e-6, maxit=self.options["pcgMaxIter"],
+                it2, _ = self.assembly.pcg(A, r, d, rows, rtol=rtol, maxit=self.options["pcgMaxIter"],
                                            check_every=4 if amg else 32, precond=precond, setup=False)
                 it += it2
                 x += d

@@ -165,6 +165,14 @@ int ewfe_adjoint_param_grad(const ewfe_plan *plan, int mat_kind, const double *m
 int ewfe_spmv(const ewfe_plan *plan, const void *csr_vals_dev, int vals_dtype, const double *x_dev, double *y_dev,
               void *stream);
 
+/* r = b - K x (K in f64 or f32, b, x, r f64) with every row's sum accurate to twice the working precision
+ * (compensated dot product). The residual of iterative refinement for NIST.linearSolve
+ * (nonlinearimplicitstatic.py:727-751): the reference's direct solvers refine their solutions (PARDISO, the
+ * default of edelweissfe/config/linsolve.py:46-86, runs iterative-refinement steps by default); with residuals in
+ * twice the precision the refined solution is accurate to working precision however ill-conditioned K is. */
+int ewfe_residual(const ewfe_plan *plan, const void *csr_vals_dev, int vals_dtype, const double *b_dev,
+                  const double *x_dev, double *r_dev, void *stream);
+
 /* Bytes of device workspace ewfe_pcg needs for this plan. */
 int64_t ewfe_pcg_workspace_size(const ewfe_plan *plan);
 

@@ -263,3 +263,22 @@ def test_surface_pressure_closed_surface_invariants(nn):
     a = O.surface_pressure(ek, conn, X, edofs, nDof, 2, 2.5, sub)
     b = O.surface_pressure(ek, conn, X, edofs, nDof, 2, 2.5, rest)
     assert np.allclose(a + b, O.surface_pressure(ek, conn, X, edofs, nDof, 2, 2.5), rtol=1e-14, atol=1e-15)
+
+
+def test_exact_residual_is_correctly_rounded():
+    """The oracle's refinement residual b - K x equals the exactly computed residual rounded once (rational
+    arithmetic), also where the terms cancel to far below their size."""
+    from fractions import Fraction
+
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(3)
+    n = 40
+    K = (sp.random(n, n, density=0.3, random_state=4, format="csr") * 1e4 + sp.eye(n)).tocsr()
+    x = rng.standard_normal(n)
+    b = K @ x + 1e-9 * rng.standard_normal(n)  # residuals ~1e-9 from terms ~1e4: heavy cancellation
+    r = O.exact_residual(K, x, b)
+    Kd = K.toarray()
+    exact = [float(Fraction(b[i]) - sum(Fraction(Kd[i, j]) * Fraction(x[j]) for j in range(n))) for i in range(n)]
+    assert np.array_equal(r, np.array(exact))
+    assert not np.array_equal(r, b - K @ x)  # the plain residual is not that accurate here
