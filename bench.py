@@ -367,15 +367,15 @@ def main():
 
     stream = torch.cuda.current_stream(dev)
 
-    def timed(K, steps, warmup):
+    def timed(K, steps, warmup, mfma=False):
         for _ in range(warmup):
-            asm.evaluate(U_np_t, dU_t, P, F, K)
+            asm.evaluate(U_np_t, dU_t, P, F, K, mfma=mfma)
         torch.cuda.synchronize()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t_wall = time.perf_counter()
         ev0.record(stream)
         for _ in range(steps):
-            asm.evaluate(U_np_t, dU_t, P, F, K)
+            asm.evaluate(U_np_t, dU_t, P, F, K, mfma=mfma)
         ev1.record(stream)
         torch.cuda.synchronize()
         t_wall = time.perf_counter() - t_wall
@@ -384,8 +384,10 @@ def main():
     ms64, wall64 = timed(K64, args.steps, args.warmup)
     pdt = float(asm.pnewdt.item())
     ms32, wall32 = timed(K32, args.steps, args.warmup)
+    ms32m, wall32m = timed(K32, args.steps, args.warmup, mfma=True)
     print(f"fp64 step {ms64:.3f} ms (wall {wall64:.3f}), fp32-tangent step {ms32:.3f} ms (wall {wall32:.3f}), "
-          f"plastic QPs {plastic_frac:.3f}, pNewDT {pdt:g}", file=sys.stderr, flush=True)
+          f"fp32-mfma tangent step {ms32m:.3f} ms (wall {wall32m:.3f}), plastic QPs {plastic_frac:.3f}, "
+          f"pNewDT {pdt:g}", file=sys.stderr, flush=True)
 
     n_el = len(conn)
     B64 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 8)
@@ -476,7 +478,11 @@ def main():
         "cpu_baseline": cpu,
         "fp32_tangent": {"value": n_el / (ms32 * 1e-3), "ms_per_step": ms32, "speedup_vs_fp64": ms64 / ms32,
                          "roofline_achieved_GBs": ach32, "roofline_frac": ach32 / HBM_PEAK_GBS,
-                         "algorithmic_bytes_per_element": B32},
+                         "algorithmic_bytes_per_element": B32,
+                         "tangent": "FP64 FMAs, rounded to FP32 when the element records are stored"},
+        "fp32_tangent_mfma": {"value": n_el / (ms32m * 1e-3), "ms_per_step": ms32m, "speedup_vs_fp64": ms64 / ms32m,
+                              "roofline_frac": B32 * n_el / (ms32m * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              "tangent": "FP32 on the matrix cores (v_mfma_f32_16x16x4_f32)"},
         "setup_s": t_setup,
         "newton_increment": newton,
     }

@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import EWFE_F32, EWFE_F64, check, lib, ptr
+from ._lib import EWFE_F32, EWFE_F32_MFMA, EWFE_F64, check, lib, ptr
 
 
 class ElementAssembly:
@@ -114,9 +114,10 @@ class ElementAssembly:
     def _stream(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
 
-    def evaluate(self, U_np, dU, P, F, vals):
+    def evaluate(self, U_np, dU, P, F, vals, mfma=False):
         """Fills P (= sum Pe), F (= sum |Pe|), the CSR values and the trial state. Returns pNewDT (device
-        tensor, 1e36 or < 1 when a quadrature point asks for a cutback)."""
+        tensor, 1e36 or < 1 when a quadrature point asks for a cutback). FP32 values: ``mfma=True`` forms the
+        tangents of 8-node elements in FP32 on the matrix cores, otherwise with FP64 FMAs rounded when stored."""
         for t in (U_np, dU, P, F):
             if t.numel() != self.nDof or t.dtype != self.torch.float64 or not t.is_contiguous():
                 raise ValueError("dof vectors must be contiguous float64 tensors of length nDof")
@@ -125,7 +126,9 @@ class ElementAssembly:
         b = self.block
         check(lib().ewfe_eval_assemble(self._plan, self.matKind, ptr(self.matParams), ptr(self.elParams),
                                        ptr(self.X), ptr(U_np), ptr(dU), ptr(b.stateVars), ptr(b.stateVarsTemp),
-                                       ptr(P), ptr(F), ptr(vals), self._dtype_code(vals), ptr(self.pnewdt),
+                                       ptr(P), ptr(F), ptr(vals),
+                                       EWFE_F32_MFMA if (mfma and vals.dtype == self.torch.float32) else
+                                       self._dtype_code(vals), ptr(self.pnewdt),
                                        self._stream()))
         return self.pnewdt
 

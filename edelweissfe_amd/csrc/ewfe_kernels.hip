@@ -584,9 +584,172 @@ __device__ __forceinline__ int slot_node(int base, int b, int i, int q, int le)
     return (base + 3 * b + i) * 64 + 16 * (le >> 1) + ((2 * q + (le & 1)) ^ (2 * b));
 }
 
+// Phase 2 of k_hex8_elem for FP32 records computed on the matrix cores (tangent mode EWFE_F32_MFMA).
+// With the per-point fields of phase 1 in LDS, K_e = T + B' + G, where over the quadrature points q
+//   T_mn  = sum_q (alpha_q p_m p_n - c_q v_m v_n)      (m = 3a + i, n = 3b + k: the 24 element dofs)
+//   B'_mn = sum_q beta_q p_(3a+k) p_(3b+i)               (the 3x3 blocks of Tb = sum_q beta_q p p^T, transposed)
+//   G_mn  = delta_ik S_ab,  S_ab = sum_q gamma_q (r_a . r_b)
+// T and Tb are symmetric rank-8 (rank-16 for J2) products, formed by v_mfma_f32_16x16x4_f32 on the three 16x16
+// tiles (0,0), (0,1), (1,1) that cover the upper triangle of the 24x24 matrix (rows and columns 24..31 are
+// zero), lane quarter g supplying points q = g and g + 4; S is one more 16x16 tile (rows 8..15 zero) with the 24
+// products (q, j) as its inner dimension: 24 MFMAs for J2, 18 otherwise, per element. The nodal forces stay on
+// the vector units in FP64, exactly as in the FP64 variant (P and F are bitwise the same). T + G and Tb are
+// staged in LDS in the record's slot order, each slot written by one lane, and summed when the record is
+// written out (deterministic). One element at a time; the stress fields of LDS, no longer needed after the
+// forces, hold the staging.
+template <int FORM, int MAT, typename VT>
+__device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const RecordArgs &R, double *s_qp,
+                                                  int32_t *s_el, int le, int t, bool active, int64_t q0, int64_t el,
+                                                  const double (&st_t)[7])
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int NQ = 8, EPB = 8;
+    using L = Hex8Layout<FORM, MAT>;
+    constexpr int PB = L::TL ? L::P : L::G;  // the gradient the tangent's p terms use
+    const int lane = threadIdx.x;
+    const int a = t;
+    const int64_t n_el = A.n_el;
+    const int64_t pos = q0 + le;
+    auto ld = [&](int f, int q, int e) { return s_qp[slot_scalar(f, q, e)]; };
+    auto ldg = [&](int base, int b, int i, int q, int e) { return s_qp[slot_node(base, b, i, q, e)]; };
+
+    // (A) lane (le, a): f_a in FP64, the same arithmetic as the FP64 variant
+    double fa[3] = {0, 0, 0};
+    if (active) {
+        for (int q = 0; q < NQ; ++q) {
+            double pa[3];
+            for (int i = 0; i < 3; ++i) pa[i] = ldg(PB, a, i, q, le);
+            const double S0 = ld(L::S + 0, q, le), S1 = ld(L::S + 1, q, le), S2 = ld(L::S + 2, q, le);
+            const double S3 = ld(L::S + 3, q, le), S4 = ld(L::S + 4, q, le), S5 = ld(L::S + 5, q, le);
+            fa[0] += S0 * pa[0] + S3 * pa[1] + S4 * pa[2];
+            fa[1] += S3 * pa[0] + S1 * pa[1] + S5 * pa[2];
+            fa[2] += S4 * pa[0] + S5 * pa[1] + S2 * pa[2];
+        }
+    }
+    if (t == 0) s_el[le] = (int32_t)el;
+    if (active)
+        for (int i = 0; i < 3; ++i) R.fe[el * 24 + 3 * a + i] = fa[i];
+    if (active && FORM == FORM_SS && MAT == M_J2)
+        for (int s = 0; s < 7; ++s) A.state_t[((int64_t)s * NQ + t) * n_el + pos] = st_t[s];
+    __syncthreads();  // every lane is done with the stress fields
+    float *s_ta = reinterpret_cast<float *>(s_qp + L::S * 64);  // T + G in record slot order (300 values)
+    float *s_tb = s_ta + kRec8;                                   // Tb in record slot order
+    float *s_sg = s_tb + kRec8;                                   // S_ab of the pairs a <= b (36 values)
+    static_assert(2 * kRec8 + kPairs8 <= 6 * 64 * 2, "the staging must fit in the stress fields");
+    static_assert((kRec8 * 4) % 16 == 0, "staged records must be 16-byte aligned");
+
+    // (B) one element at a time: lane (g, c) supplies rows c and 16 + c for points g and g + 4
+    const int g = lane >> 4, c = lane & 15;
+    const int m0 = c, m1 = 16 + c;
+    const bool has1 = m1 < 24;
+    const int a0 = m0 / 3, i0 = m0 % 3, a1 = has1 ? m1 / 3 : 0, i1 = has1 ? m1 % 3 : 0;
+    const int nvalid = (R.q_end - q0) < EPB ? (int)(R.q_end - q0) : EPB;
+    struct Ops {
+        float x0[2], x1[2], al[2], be[2], y0[2], y1[2], mc[2];
+    };
+    auto operands = [&](int e, Ops &o) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = g + 4 * h;
+            o.x0[h] = (float)ldg(PB, a0, i0, q, e);
+            o.x1[h] = has1 ? (float)ldg(PB, a1, i1, q, e) : 0.f;
+            o.al[h] = (float)ld(L::AL, q, e);
+            o.be[h] = (float)ld(L::BE, q, e);
+            o.y0[h] = o.y1[h] = o.mc[h] = 0.f;
+            if (L::HAS_N) {
+                // v = N p with the flow direction N (tensor components [11,22,33,12,13,23])
+                float Nv[6];
+                for (int k = 0; k < 6; ++k) Nv[k] = (float)ld(L::NV + k, q, e);
+                auto vrow = [&](int aa, int ii) {
+                    const int j0 = ii == 0 ? 0 : (ii == 1 ? 3 : 4), j1 = ii == 0 ? 3 : (ii == 1 ? 1 : 5),
+                              j2 = ii == 0 ? 4 : (ii == 1 ? 5 : 2);
+                    return Nv[j0] * (float)ldg(PB, aa, 0, q, e) + Nv[j1] * (float)ldg(PB, aa, 1, q, e) +
+                           Nv[j2] * (float)ldg(PB, aa, 2, q, e);
+                };
+                o.y0[h] = vrow(a0, i0);
+                o.y1[h] = has1 ? vrow(a1, i1) : 0.f;
+                o.mc[h] = -(float)ld(L::CC, q, e);
+            }
+        }
+    };
+    Ops cur;
+    if (nvalid > 0) operands(0, cur);
+    for (int e = 0; e < nvalid; ++e) {
+        f4 acc[7];
+#pragma unroll
+        for (int u = 0; u < 7; ++u) acc[u] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float x0 = cur.x0[h], x1 = cur.x1[h], al = cur.al[h], be = cur.be[h];
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(al * x0, x0, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(al * x0, x1, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(al * x1, x1, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(be * x0, x0, acc[3], 0, 0, 0);
+            acc[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(be * x0, x1, acc[4], 0, 0, 0);
+            acc[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(be * x1, x1, acc[5], 0, 0, 0);
+            if (L::HAS_N) {
+                const float y0 = cur.y0[h], y1 = cur.y1[h], mc = cur.mc[h];
+                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(mc * y0, y0, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(mc * y0, y1, acc[1], 0, 0, 0);
+                acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(mc * y1, y1, acc[2], 0, 0, 0);
+            }
+        }
+        // S: rows a = c (< 8), inner index k = 4 s + g over (q, j) = (k / 3, k % 3)
+#pragma unroll
+        for (int st = 0; st < 6; ++st) {
+            const int k = 4 * st + g, q = k / 3, j = k % 3;
+            const float r = c < 8 ? (float)ldg(L::G, c, j, q, e) : 0.f;
+            const float gq = (float)ld(L::GA, q, e);
+            acc[6] = __builtin_amdgcn_mfma_f32_16x16x4f32(gq * r, r, acc[6], 0, 0, 0);
+        }
+        // the next element's operands, loaded while this element's products drain
+        Ops nxt = cur;
+        if (e + 1 < nvalid) operands(e + 1, nxt);
+        // S_ab: lane (g, c) holds rows a = 4 g + v, column b = c
+        if (g < 2 && c < 8)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ra = 4 * g + v;
+                if (ra <= c) s_sg[pair8(ra, c)] = acc[6][v];
+            }
+        __syncthreads();
+        // accumulator u % 3 = tile (0,0), (0,1), (1,1); lane (g, c) holds rows 16 R + 4 g + v, column 16 C + c.
+        // Values with m <= n < 24 are the record's: block (a, b) = (m / 3, n / 3), a <= b. T + G goes to the slot of
+        // K_ab[i][k] (i = m % 3, k = n % 3); the Tb value (m, n) = ((a, x), (b, y)) belongs to K_ab[y][x].
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int n = (u == 0 ? 0 : 16) + c;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = (u == 2 ? 16 : 0) + 4 * g + v;
+                if (m <= n && n < 24) {
+                    const int am = m / 3, i = m % 3, bn = n / 3, k = n % 3;
+                    float val = acc[u][v];
+                    if (i == k) val += s_sg[pair8(am, bn)];
+                    const bool offd = am < bn;
+                    s_ta[offd ? block_off8(am, bn) + 3 * i + k : block_off8(am, am) + kSym6[i][k]] = val;
+                    s_tb[offd ? block_off8(am, bn) + 3 * k + i : block_off8(am, am) + kSym6[i][k]] = acc[3 + u][v];
+                }
+            }
+        }
+        __syncthreads();
+        // the record = staged T + G plus staged Tb: 75 16-byte units, non-temporal (read back by the gather pass)
+        constexpr int NW = kRec8 / 4;
+        static_assert(kRec8 % 4 == 0, "records are whole 16-byte units");
+        VT *dst = reinterpret_cast<VT *>(R.kbuf) + (int64_t)s_el[e] * kRec8;
+        for (int w = lane; w < NW; w += 64) {
+            const f4 sum = reinterpret_cast<const f4 *>(s_ta)[w] + reinterpret_cast<const f4 *>(s_tb)[w];
+            __builtin_nontemporal_store(sum, reinterpret_cast<f4 *>(dst) + w);
+        }
+        __syncthreads();  // the staging is free for the next element
+        cur = nxt;
+    }
+}
+
 // MT: the plan holds a material table (a separate instantiation, so that the single-material path carries no
 // per-element parameter selects)
-template <int FORM, int MAT, typename VT, bool MT>
+// MF: FP32 records with the tangent formed on the matrix cores (only with VT = float)
+template <int FORM, int MAT, typename VT, bool MT, bool MF>
 __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 {
     constexpr int NN = 8, NQ = 8, EPB = 8;
@@ -692,7 +855,12 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     }
     __syncthreads();
 
-    // phase 2: lane (le, a=t) builds blocks K_ab, b = a + j (mod 8), j < nb, and f_a
+    if constexpr (MF) {
+        static_assert(sizeof(VT) == 4, "the matrix-core tangent writes FP32 records");
+        hex8_tangent_mfma<FORM, MAT, VT>(A, R, s_qp, s_el, le, t, active, q0, el, st_t);
+        return;
+    }
+    // phase 2 (FP64 tangent): lane (le, a=t) builds blocks K_ab, b = a + j (mod 8), j < nb, and f_a
     const int a = t;
     const int nb = (a < 4) ? 5 : 4;
     double K[5][3][3];
@@ -1899,7 +2067,7 @@ static int ensure_pipeline(ewfe_plan *p)
 // element pass of chunk c + 1 (arithmetic bound) overlaps the gather of chunk c (memory bound). The caller's
 // stream waits for the last gather: seen from the caller, the call is ordered on its stream as before.
 template <int FORM, int MAT, typename VT>
-static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s)
+static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
 {
     if (!p->d_kbuf || !p->d_fe || (p->n_chunks > 1 && !p->aux_stream)) {
         ewfe::set_error("ewfe_eval_assemble: the plan has no element records");
@@ -1912,12 +2080,18 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s)
         const int64_t batches = (R.q_end - R.q_begin + 7) / 8;
         R.xcd_batches = (batches + 7) / 8;
         if (batches > 0) {
-            if (A.mtab)
-                hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, true>), dim3((unsigned)(8 * R.xcd_batches)), dim3(64),
-                                   0, s, A, R);
-            else
-                hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, false>), dim3((unsigned)(8 * R.xcd_batches)),
-                                   dim3(64), 0, s, A, R);
+            const dim3 grid((unsigned)(8 * R.xcd_batches));
+            if (sizeof(VT) == 4 && mfma) {
+                if (A.mtab)
+                    hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, true, sizeof(VT) == 4>), grid, dim3(64), 0, s, A, R);
+                else
+                    hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, false, sizeof(VT) == 4>), grid, dim3(64), 0, s, A,
+                                       R);
+            } else if (A.mtab) {
+                hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, true, false>), grid, dim3(64), 0, s, A, R);
+            } else {
+                hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, false, false>), grid, dim3(64), 0, s, A, R);
+            }
             HIP_CHECK(hipGetLastError());
         }
         if (K > 1) {
@@ -1950,9 +2124,9 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s)
 }
 
 template <int FORM, int MAT, typename VT>
-static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s)
+static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
 {
-    if (p->npe == 8) return launch_hex8<FORM, MAT, VT>(p, A, s);
+    if (p->npe == 8) return launch_hex8<FORM, MAT, VT>(p, A, s, mfma);
     for (int c = 0; c < p->n_colors; ++c) {
         A.p_begin = p->color_begin[c];
         A.p_end = p->color_begin[c + 1];
@@ -1968,12 +2142,12 @@ static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s)
 }
 
 template <typename VT>
-static int dispatch(ewfe_plan *p, int mat_kind, EvalArgs A, hipStream_t s)
+static int dispatch(ewfe_plan *p, int mat_kind, EvalArgs A, hipStream_t s, bool mfma)
 {
     const bool tl = is_tl(p->elem_kind);
-    if (!tl && mat_kind == EWFE_LINEAR_ELASTIC) return launch_family<FORM_SS, M_LE, VT>(p, A, s);
-    if (!tl && mat_kind == EWFE_VON_MISES) return launch_family<FORM_SS, M_J2, VT>(p, A, s);
-    if (tl && mat_kind == EWFE_NEO_HOOKE_WA) return launch_family<FORM_TL, M_NH, VT>(p, A, s);
+    if (!tl && mat_kind == EWFE_LINEAR_ELASTIC) return launch_family<FORM_SS, M_LE, VT>(p, A, s, mfma);
+    if (!tl && mat_kind == EWFE_VON_MISES) return launch_family<FORM_SS, M_J2, VT>(p, A, s, mfma);
+    if (tl && mat_kind == EWFE_NEO_HOOKE_WA) return launch_family<FORM_TL, M_NH, VT>(p, A, s, mfma);
     ewfe::set_error("unsupported element/material combination (small-strain elements take linear elastic or "
                     "von Mises, total-Lagrangian elements take Neo-Hooke)");
     return EWFE_EINVAL;
@@ -1994,8 +2168,8 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
         ewfe::set_error("ewfe_eval_assemble: material needs state_n/state_t buffers");
         return EWFE_EINVAL;
     }
-    if (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) {
-        ewfe::set_error("ewfe_eval_assemble: vals_dtype must be EWFE_F64 or EWFE_F32");
+    if (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32 && vals_dtype != EWFE_F32_MFMA) {
+        ewfe::set_error("ewfe_eval_assemble: vals_dtype must be EWFE_F64, EWFE_F32 or EWFE_F32_MFMA");
         return EWFE_EINVAL;
     }
     if (!p->d_edof) {
@@ -2045,7 +2219,8 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
     A.p_begin = A.p_end = 0;
     for (int i = 0; i < 8; ++i) A.mat.p[i] = mat_params_host[i];
     ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // the pipelined evaluation creates its second stream once
-    return vals_dtype == EWFE_F64 ? dispatch<double>(pm, mat_kind, A, s) : dispatch<float>(pm, mat_kind, A, s);
+    if (vals_dtype == EWFE_F64) return dispatch<double>(pm, mat_kind, A, s, false);
+    return dispatch<float>(pm, mat_kind, A, s, vals_dtype == EWFE_F32_MFMA);
 }
 
 extern "C" int ewfe_adjoint_param_grad(const ewfe_plan *p, int mat_kind, const double *mat_params_host,

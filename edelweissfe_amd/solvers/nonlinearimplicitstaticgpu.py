@@ -26,8 +26,10 @@ Differences that are part of the design (not of the semantics):
     the multigrid from 20,000 dofs on when the mesh allows it, block Jacobi otherwise), converged to
     ``pcgTolerance`` relative residual; the Newton loop's own flux/correction tolerances decide convergence
     exactly as in the reference;
-  * ``tangentPrecision="fp32"`` builds and stores the tangent in FP32 (residual, return mapping and update
-    stay FP64): Newton then converges to the same FP64 residual tolerance, possibly in more iterations; its
+  * ``tangentPrecision="fp32"`` stores the tangent in FP32 (element tangents from FP64 FMAs, rounded when
+    stored), ``"fp32-mfma"`` also forms the element tangents of 8-node elements in FP32 on the matrix cores;
+    residual, return mapping and update stay FP64 in both. Newton then converges to the same FP64 residual
+    tolerance, possibly in more iterations; its
     linear systems are solved to ``pcgToleranceFP32`` (default 1e-4): an FP32 tangent already limits the
     Newton contraction to ~1e-4..1e-3 per iteration, so a tighter correction buys nothing, while the
     convergence test (FP64 residual and correction) is unchanged;
@@ -172,7 +174,11 @@ class NISTGPU:
         self.computationTimes = dict.fromkeys(TIMING_KEYS + ["multigrid setup", "setup"], 0.0)
         # dof numbering, CSR pattern, element plan, multigrid patterns: once per step (mesh), not per increment
         self.computationTimes["setup"] = _time.perf_counter() - tic_setup
-        K = self.assembly.newValues("fp32" if self.options["tangentPrecision"] == "fp32" else "fp64")
+        tp = self.options["tangentPrecision"].lower()
+        if tp not in ("fp64", "fp32", "fp32-mfma"):
+            raise AttributeError("invalid tangentPrecision {:} (fp64, fp32, fp32-mfma)".format(tp))
+        self._mfmaTangent = tp == "fp32-mfma"
+        K = self.assembly.newValues("fp64" if tp == "fp64" else "fp32")
 
         extrapolation = self.options["extrapolation"]
         maxIter, criticalIter = step.maxIter, step.criticalIter
@@ -263,7 +269,7 @@ class NISTGPU:
     # ------------------------------------------------------------------------------------------------
     def computeElements(self, elements, U_np, dU, P, K, F, timeStep):
         tic = _time.perf_counter()
-        pnewdt = self.assembly.evaluate(U_np, dU, P, F, K)
+        pnewdt = self.assembly.evaluate(U_np, dU, P, F, K, mfma=getattr(self, "_mfmaTangent", False))
         pdt = float(pnewdt.item())  # synchronises
         self.computationTimes["elements"] += _time.perf_counter() - tic
         if pdt < 1.0:

@@ -49,7 +49,11 @@ enum ewfe_elem_kind {
     EWFE_C3D20RTL = 5  /* the same, total Lagrangian (library.py:517-531) */
 };
 enum ewfe_mat_kind { EWFE_LINEAR_ELASTIC = 0, EWFE_VON_MISES = 1, EWFE_NEO_HOOKE_WA = 2 };
-enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1 };
+/* Precision of the assembled tangent. EWFE_F32 and EWFE_F32_MFMA both store FP32 CSR values; EWFE_F32 forms the
+ * element tangents with FP64 FMAs and rounds them when they are stored, EWFE_F32_MFMA forms the tangents of 8-node
+ * elements in FP32 on the matrix cores (v_mfma_f32_16x16x4_f32; 20-node elements as EWFE_F32). Only
+ * ewfe_eval_assemble takes EWFE_F32_MFMA; every other call takes EWFE_F32 for FP32 values. */
+enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1, EWFE_F32_MFMA = 2 };
 
 /* Error codes */
 #define EWFE_OK 0
@@ -137,7 +141,7 @@ int ewfe_plan_set_materials(ewfe_plan *plan, int32_t n_mat, const double *params
  *   state_n_dev      f64[nstate * nqp * n_el] converged state, plan order; NULL if nstate == 0
  *   state_t_dev      same layout, receives the trial state
  *   P_dev, F_dev     f64[n_dof] out: P = -f_int (element "P"), F = sum |Pe|
- *   csr_vals_dev     f64 or f32 [nnz] out (vals_dtype): the assembled tangent stored in that precision
+ *   csr_vals_dev     f64 or f32 [nnz] out: the assembled tangent, vals_dtype EWFE_F64, EWFE_F32 or EWFE_F32_MFMA
  *   pnewdt_dev       f64[1] out: 1e36, or < 1 if a quadrature point requests a cutback
  */
 int ewfe_eval_assemble(const ewfe_plan *plan, int mat_kind, const double *mat_params_host,

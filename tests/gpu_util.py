@@ -37,9 +37,11 @@ class GpuEval:
         return self.torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device="cuda")
 
     def __call__(self, U_np, dU, dtype="fp64"):
+        """dtype: "fp64", "fp32" (FP32 values, tangent from FP64 FMAs) or "fp32-mfma" (tangent on the matrix
+        cores)."""
         P, F = self.dm.constructDofVector(), self.dm.constructDofVector()
-        vals = self.asm.newValues(dtype)
-        pdt = self.asm.evaluate(self.vec(U_np), self.vec(dU), P, F, vals)
+        vals = self.asm.newValues("fp64" if dtype == "fp64" else "fp32")
+        pdt = self.asm.evaluate(self.vec(U_np), self.vec(dU), P, F, vals, mfma=dtype == "fp32-mfma")
         self.torch.cuda.synchronize()
         return dict(P=P.cpu().numpy(), F=F.cpu().numpy(), data=vals.double().cpu().numpy(),
                     state=self.asm.stateInElementOrder("trial"), pNewDT=float(pdt.item()))

@@ -155,7 +155,7 @@ def test_pipelined_chunks_match_single_pass(mesh, chunks):
     many = GpuEval("C3D8", "vonmises", J2, X, conn, pipelineChunks=chunks)
     assert one.asm.pipelineChunks == 1 and many.asm.pipelineChunks == chunks
     assert many.asm.structuredMesh == (mesh == "box")
-    for dtype in ("fp64", "fp32"):
+    for dtype in ("fp64", "fp32", "fp32-mfma"):
         a, b = one(U, U, dtype), many(U, U, dtype)
         for k in ("data", "P", "F", "state"):
             assert np.array_equal(a[k], b[k]), (dtype, k)
@@ -176,38 +176,41 @@ def test_structured_gather_matches_descriptor_gather():
     lists = GpuEval("C3D8", "vonmises", J2, X2, conn)
     assert grid.asm.structuredMesh and not lists.asm.structuredMesh
     assert lists.dm.nDof == grid.dm.nDof  # the extra node has no element, hence no dofs
-    for dtype in ("fp64", "fp32"):
+    for dtype in ("fp64", "fp32", "fp32-mfma"):
         a, b = grid(U, U, dtype), lists(U, U, dtype)
         for k in ("data", "P", "F", "state"):
             assert np.array_equal(a[k], b[k]), (dtype, k)
 
 
-@pytest.mark.parametrize("el,mat,mp,n", [("C3D8", "vonmises", J2, 4), ("C3D8", "vonmises", J2NL, 3),
-                                         ("C3D8", "linearelastic", LE, 4), ("C3D8TL", "neohookeanwa", NH, 4),
-                                         ("C3D20", "vonmises", J2, 2), ("C3D20", "linearelastic", LE, 2),
-                                         ("C3D20TL", "neohookeanwa", NH, 2), ("C3D20R", "vonmises", J2, 2),
-                                         ("C3D20RTL", "neohookeanwa", NH, 2)])
-def test_fp32_tangent_matches_oracle(el, mat, mp, n):
-    """The FP32-tangent variant against the FP64 oracle: CSR values within 2e-6 normwise (FP32 storage and, for
-    20-node elements, FP32 accumulation of the element blocks into the CSR values), while the residual P, F and
-    the trial state stay FP64: bitwise equal to the FP64 variant's and within 1e-12 of the oracle."""
-    print(f"case fp32 tangent {el} {mat} n={n}", flush=True)
+FP32_CASES = [("C3D8", "vonmises", J2, 4), ("C3D8", "vonmises", J2NL, 3), ("C3D8", "linearelastic", LE, 4),
+              ("C3D8TL", "neohookeanwa", NH, 4), ("C3D20", "vonmises", J2, 2), ("C3D20", "linearelastic", LE, 2),
+              ("C3D20TL", "neohookeanwa", NH, 2), ("C3D20R", "vonmises", J2, 2), ("C3D20RTL", "neohookeanwa", NH, 2)]
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp32-mfma"])
+@pytest.mark.parametrize("el,mat,mp,n", FP32_CASES)
+def test_fp32_tangent_matches_oracle(el, mat, mp, n, mode):
+    """The FP32-tangent variants against the FP64 oracle: CSR values within 2e-6 normwise (FP32 storage; FP32
+    arithmetic of the matrix-core tangent of 8-node elements; FP32 accumulation of the element blocks into the CSR
+    values for 20-node elements), while the residual P, F and the trial state stay FP64: bitwise equal to the FP64
+    variant's and within 1e-12 of the oracle."""
+    print(f"case {mode} tangent {el} {mat} n={n}", flush=True)
     X, conn, _ = jittered_box(n, el, seed=5)
     rng = np.random.default_rng(6)
     amp = 0.08 if "TL" in el else 4e-3
     U = _uniaxial_field(X, amp, rng, amp / 20)
     dU = U.copy() if "TL" not in el else 0.5 * U
     g = GpuEval(el, mat, mp, X, conn)
-    o32 = g(U, dU, "fp32")
+    o32 = g(U, dU, mode)
     o64 = g(U, dU, "fp64")
     ref = oracle_eval(el, mat, mp, X, conn, U, dU)
     e32 = rel(o32["data"], ref["data"])
-    print(f"fp32 CSR values vs oracle {e32:.2e}", flush=True)
+    print(f"{mode} CSR values vs oracle {e32:.2e}", flush=True)
     assert e32 < 2e-6
     assert rel(o32["P"], ref["P"]) < 1e-12 and rel(o32["F"], ref["F"]) < 1e-12
     for k in ("P", "F", "state"):  # the residual, fluxes and return mapping stay FP64
         assert np.array_equal(o32[k], o64[k]), k
-    again = g(U, dU, "fp32")
+    again = g(U, dU, mode)
     assert np.array_equal(again["data"], o32["data"])  # deterministic
 
 
@@ -521,8 +524,9 @@ def test_several_materials_match_oracle(el, mat, n):
     assert rel(stress.cpu().numpy(), st[:, :, 0:6]) < 1e-12
 
 
-def test_fp32_tangent_newton_reaches_fp64_tolerance():
-    fx, U32, s32 = _deck_run("C3D8", "fp32")
+@pytest.mark.parametrize("mode", ["fp32", "fp32-mfma"])
+def test_fp32_tangent_newton_reaches_fp64_tolerance(mode):
+    fx, U32, s32 = _deck_run("C3D8", mode)
     _, U64, s64 = _deck_run("C3D8", "fp64")
     assert np.abs(U32 - np.array(fx["U_ref"])).max() < 1e-6
     assert rel(U32, U64) < 1e-8
