@@ -39,6 +39,8 @@ SIGNATURES = {
     "ewfe_plan_info": (ctypes.c_int, [_vp, _P(_i64), _P(_i32), _P(_i32), _P(_i64)]),
     "ewfe_plan_destroy": (ctypes.c_int, [_vp]),
     "ewfe_plan_set_materials": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
+    "ewfe_plan_create_union": (ctypes.c_int, [_P(_vp), _vp, _i32, _i64, _vp, _P(_i64)]),
+    "ewfe_csr_accumulate": (ctypes.c_int, [_vp, _i32, _vp, _vp, ctypes.c_int, _vp]),
     "ewfe_eval_assemble": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                           ctypes.c_int, _vp, _vp]),
     "ewfe_adjoint_param_grad": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -21,7 +21,112 @@ from . import _lib
 from ._lib import EWFE_F32, EWFE_F32_MFMA, EWFE_F64, check, lib, ptr
 
 
-class ElementAssembly:
+class PlanLinearAlgebra:
+    """The linear-algebra calls on a plan's CSR pattern (Dirichlet rows, SpMV, residual, PCG, multigrid), shared
+    by the element plans of ElementAssembly and the system plan of several blocks (SystemAssembly)."""
+
+    def _init_solver_state(self):
+        self._pcg_ws = None
+        self._mg = None
+        self.mgLevels = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_mg", None):
+                lib().ewfe_mg_destroy(self._mg)
+                self._mg = None
+            if getattr(self, "_plan", None):
+                lib().ewfe_plan_destroy(self._plan)
+                self._plan = None
+        except Exception:
+            pass
+
+    def pattern(self):
+        """Host copy of the CSR pattern (indptr, indices), int32."""
+        indptr = np.zeros(self.nDof + 1, dtype=np.int32)
+        indices = np.zeros(self.nnz, dtype=np.int32)
+        check(lib().ewfe_plan_pattern(self._plan, ptr(indptr), ptr(indices)))
+        return indptr, indices
+
+    def newValues(self, dtype="fp64"):
+        tdt = self.torch.float64 if dtype == "fp64" else self.torch.float32
+        return self.torch.zeros(self.nnz, dtype=tdt, device=self.device)
+
+    @staticmethod
+    def _dtype_code(vals):
+        import torch
+
+        if vals.dtype == torch.float64:
+            return EWFE_F64
+        if vals.dtype == torch.float32:
+            return EWFE_F32
+        raise TypeError("CSR values must be float64 or float32")
+
+    def _stream(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def dirichletRows(self, rows_dev, vals):
+        check(lib().ewfe_dirichlet_rows(self._plan, ptr(rows_dev), rows_dev.numel(), ptr(vals),
+                                        self._dtype_code(vals), self._stream()))
+
+    def spmv(self, vals, x, y):
+        check(lib().ewfe_spmv(self._plan, ptr(vals), self._dtype_code(vals), ptr(x), ptr(y), self._stream()))
+        return y
+
+    def residual(self, vals, b, x, r):
+        """r = b - K x with row sums accurate to twice the working precision (refinement residuals)."""
+        check(lib().ewfe_residual(self._plan, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(r),
+                                  self._stream()))
+        return r
+
+    def multigrid(self, degree=2, ratio=20.0):
+        """The smoothed-aggregation hierarchy of this plan (built on first use; patterns only)."""
+        if self._mg is None:
+            mg = ctypes.c_void_p()
+            nl = ctypes.c_int()
+            check(lib().ewfe_mg_create(self._plan, int(degree), float(ratio), ctypes.byref(mg), ctypes.byref(nl)))
+            self._mg = mg
+            sizes = []
+            for lvl in range(nl.value):
+                nn, nb = ctypes.c_int64(), ctypes.c_int64()
+                check(lib().ewfe_mg_level_info(mg, lvl, ctypes.byref(nn), ctypes.byref(nb)))
+                sizes.append((nn.value, nb.value))
+            self.mgLevels = sizes
+        return self._mg
+
+    def multigridSetup(self, vals, rows_dev):
+        """Numeric multigrid setup for the row-eliminated matrix `vals` (Dirichlet rows `rows_dev`)."""
+        n_rows = 0 if rows_dev is None else rows_dev.numel()
+        check(lib().ewfe_mg_setup(self.multigrid(), ptr(vals), self._dtype_code(vals), ptr(rows_dev), n_rows,
+                                  self._stream()))
+
+    def pcg(self, vals, b, x, rows_dev, rtol=1e-10, maxit=20000, check_every=32, precond="jacobi", setup=True):
+        """PCG for the row-eliminated system, preconditioned by node-block Jacobi (``precond="jacobi"``) or by
+        the smoothed-aggregation V-cycle (``precond="amg"``, set up here for `vals` unless ``setup=False``);
+        returns (iterations, relative residual)."""
+        if self._pcg_ws is None:
+            nbytes = lib().ewfe_pcg_workspace_size(self._plan)
+            self._pcg_ws = self.torch.empty((nbytes + 7) // 8, dtype=self.torch.float64, device=self.device)
+        it = ctypes.c_int()
+        rr = ctypes.c_double()
+        n_rows = 0 if rows_dev is None else rows_dev.numel()
+        if precond == "amg":
+            if setup:
+                self.multigridSetup(vals, rows_dev)
+            mg = self.multigrid()
+            check(lib().ewfe_pcg_mg(mg, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev), n_rows,
+                                    float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws), ctypes.byref(it),
+                                    ctypes.byref(rr), self._stream()))
+        elif precond == "jacobi":
+            check(lib().ewfe_pcg(self._plan, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev),
+                                 n_rows, float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws),
+                                 ctypes.byref(it), ctypes.byref(rr), self._stream()))
+        else:
+            raise ValueError(f"unknown preconditioner {precond!r}")
+        return it.value, rr.value
+
+
+class ElementAssembly(PlanLinearAlgebra):
     def __init__(self, block, dofManager, model, device="cuda", pipelineChunks=0):
         import torch
 
@@ -74,46 +179,9 @@ class ElementAssembly:
             block.stateVars = block.stateVarsTemp = None
         block.planOrder = order
         self.pnewdt = torch.empty(1, dtype=torch.float64, device=self.device)
-        self._pcg_ws = None
-        self._mg = None
-        self.mgLevels = None
-
-    def __del__(self):
-        try:
-            if getattr(self, "_mg", None):
-                lib().ewfe_mg_destroy(self._mg)
-                self._mg = None
-            if getattr(self, "_plan", None):
-                lib().ewfe_plan_destroy(self._plan)
-                self._plan = None
-        except Exception:
-            pass
+        self._init_solver_state()
 
     # ------------------------------------------------------------------------------------------------
-    def pattern(self):
-        """Host copy of the CSR pattern (indptr, indices), int32."""
-        indptr = np.zeros(self.nDof + 1, dtype=np.int32)
-        indices = np.zeros(self.nnz, dtype=np.int32)
-        check(lib().ewfe_plan_pattern(self._plan, ptr(indptr), ptr(indices)))
-        return indptr, indices
-
-    def newValues(self, dtype="fp64"):
-        tdt = self.torch.float64 if dtype == "fp64" else self.torch.float32
-        return self.torch.zeros(self.nnz, dtype=tdt, device=self.device)
-
-    @staticmethod
-    def _dtype_code(vals):
-        import torch
-
-        if vals.dtype == torch.float64:
-            return EWFE_F64
-        if vals.dtype == torch.float32:
-            return EWFE_F32
-        raise TypeError("CSR values must be float64 or float32")
-
-    def _stream(self):
-        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
-
     def evaluate(self, U_np, dU, P, F, vals, mfma=False):
         """Fills P (= sum Pe), F (= sum |Pe|), the CSR values and the trial state. Returns pNewDT (device
         tensor, 1e36 or < 1 when a quadrature point asks for a cutback). FP32 values: ``mfma=True`` forms the
@@ -185,66 +253,6 @@ class ElementAssembly:
                                     ptr(U), ptr(state), 0 if result == "stress" else 1, ptr(out), self._stream()))
         return out
 
-    def dirichletRows(self, rows_dev, vals):
-        check(lib().ewfe_dirichlet_rows(self._plan, ptr(rows_dev), rows_dev.numel(), ptr(vals),
-                                        self._dtype_code(vals), self._stream()))
-
-    def spmv(self, vals, x, y):
-        check(lib().ewfe_spmv(self._plan, ptr(vals), self._dtype_code(vals), ptr(x), ptr(y), self._stream()))
-        return y
-
-    def residual(self, vals, b, x, r):
-        """r = b - K x with row sums accurate to twice the working precision (refinement residuals)."""
-        check(lib().ewfe_residual(self._plan, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(r),
-                                  self._stream()))
-        return r
-
-    def multigrid(self, degree=2, ratio=20.0):
-        """The smoothed-aggregation hierarchy of this plan (built on first use; patterns only)."""
-        if self._mg is None:
-            mg = ctypes.c_void_p()
-            nl = ctypes.c_int()
-            check(lib().ewfe_mg_create(self._plan, int(degree), float(ratio), ctypes.byref(mg), ctypes.byref(nl)))
-            self._mg = mg
-            sizes = []
-            for lvl in range(nl.value):
-                nn, nb = ctypes.c_int64(), ctypes.c_int64()
-                check(lib().ewfe_mg_level_info(mg, lvl, ctypes.byref(nn), ctypes.byref(nb)))
-                sizes.append((nn.value, nb.value))
-            self.mgLevels = sizes
-        return self._mg
-
-    def multigridSetup(self, vals, rows_dev):
-        """Numeric multigrid setup for the row-eliminated matrix `vals` (Dirichlet rows `rows_dev`)."""
-        n_rows = 0 if rows_dev is None else rows_dev.numel()
-        check(lib().ewfe_mg_setup(self.multigrid(), ptr(vals), self._dtype_code(vals), ptr(rows_dev), n_rows,
-                                  self._stream()))
-
-    def pcg(self, vals, b, x, rows_dev, rtol=1e-10, maxit=20000, check_every=32, precond="jacobi", setup=True):
-        """PCG for the row-eliminated system, preconditioned by node-block Jacobi (``precond="jacobi"``) or by
-        the smoothed-aggregation V-cycle (``precond="amg"``, set up here for `vals` unless ``setup=False``);
-        returns (iterations, relative residual)."""
-        if self._pcg_ws is None:
-            nbytes = lib().ewfe_pcg_workspace_size(self._plan)
-            self._pcg_ws = self.torch.empty((nbytes + 7) // 8, dtype=self.torch.float64, device=self.device)
-        it = ctypes.c_int()
-        rr = ctypes.c_double()
-        n_rows = 0 if rows_dev is None else rows_dev.numel()
-        if precond == "amg":
-            if setup:
-                self.multigridSetup(vals, rows_dev)
-            mg = self.multigrid()
-            check(lib().ewfe_pcg_mg(mg, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev), n_rows,
-                                    float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws), ctypes.byref(it),
-                                    ctypes.byref(rr), self._stream()))
-        elif precond == "jacobi":
-            check(lib().ewfe_pcg(self._plan, ptr(vals), self._dtype_code(vals), ptr(b), ptr(x), ptr(rows_dev),
-                                 n_rows, float(rtol), int(maxit), int(check_every), ptr(self._pcg_ws),
-                                 ctypes.byref(it), ctypes.byref(rr), self._stream()))
-        else:
-            raise ValueError(f"unknown preconditioner {precond!r}")
-        return it.value, rr.value
-
     def stateInElementOrder(self, which="trial"):
         """[nEl, nInt, nState] host array of the state in original element order."""
         t = self.block.stateVarsTemp if which == "trial" else self.block.stateVars
@@ -262,6 +270,35 @@ class ElementAssembly:
         s = np.asarray(state, dtype=np.float64)[self.elementOrder]  # [p][q][s]
         t = self.torch.as_tensor(np.ascontiguousarray(np.transpose(s, (2, 1, 0))).ravel(), device=self.device)
         self.block.stateVars.copy_(t)
+
+
+class SystemAssembly(PlanLinearAlgebra):
+    """The linear system of a model with several element blocks: the system plan of ewfe_plan_create_union over the
+    blocks' plans (the row-wise union of their patterns, the reference's single VIJ pattern over all elements,
+    dofmanager.py:522-557, csrgenerator.pyx:47-98), on which the Dirichlet rows, SpMV, residual, PCG and multigrid
+    run; ``accumulate`` adds a block's CSR values into the system's."""
+
+    def __init__(self, assemblies, dofManager, device="cuda"):
+        import torch
+
+        self.torch = torch
+        self.device = assemblies[0].device
+        self.nDof = dofManager.nDof
+        plans = (ctypes.c_void_p * len(assemblies))(*[a._plan.value for a in assemblies])
+        node_dof = np.ascontiguousarray(dofManager.nodeDofBase, dtype=np.int64)
+        plan, nnz = ctypes.c_void_p(), ctypes.c_int64()
+        check(lib().ewfe_plan_create_union(ctypes.byref(plan), plans, len(assemblies), len(node_dof), ptr(node_dof),
+                                           ctypes.byref(nnz)))
+        self._plan = plan
+        self.nnz = nnz.value
+        self._init_solver_state()
+
+    def accumulate(self, block, blockValues, vals):
+        """vals += the CSR values of block `block` (its plan's pattern), mapped into the system pattern."""
+        if blockValues.dtype != vals.dtype:
+            raise TypeError("block and system values must have one precision")
+        check(lib().ewfe_csr_accumulate(self._plan, int(block), ptr(blockValues), ptr(vals), self._dtype_code(vals),
+                                        self._stream()))
 
 
 _ = _lib  # re-exported for callers that need the enum values

@@ -67,6 +67,10 @@ struct ewfe_plan {
     std::vector<int64_t> pipe_el, pipe_row;
     std::vector<int32_t> rorder;  // host copy of the row order (row node index per gather position)
     int32_t *d_rorder = nullptr;
+    // system plans of several blocks (ewfe_plan_create_union): no elements of their own; per block the position
+    // in this pattern of each entry of the block's pattern
+    std::vector<int32_t *> d_umap;
+    std::vector<int64_t> umap_len;
     void *aux_stream = nullptr;       // hipStream_t of pipelined plans, created with the plan (default priority)
     std::vector<void *> chunk_events;  // hipEvent_t per chunk + 1 join event
 };
@@ -87,6 +91,10 @@ struct GatherHost {
     std::vector<int32_t> gdesc, gblob;
     std::vector<int32_t> ninc_ptr, ninc;  // row node -> incident elements (32 * element + local node)
 };
+// System plan of several block plans: the row-wise union of their patterns, its row-node graph, and per block
+// the position of each of its pattern entries in the union.
+int build_union_host(ewfe_plan &u, const ewfe_plan *const *plans, int n_plans, const int64_t *node_dof,
+                     int64_t n_nodes, GatherHost &gather, std::vector<std::vector<int32_t>> &maps);
 int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_dof, std::vector<int32_t> &edof_host,
                     std::vector<int32_t> &colpos_host, GatherHost &gather);
 }  // namespace ewfe

@@ -30,6 +30,7 @@
 //    points, lanes 0..59 build one row (node a, component i) of Ke each.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -1914,6 +1915,77 @@ extern "C" int ewfe_plan_create_host(ewfe_plan **out, int elem_kind, int64_t n_e
                        pipeline_chunks);
 }
 
+extern "C" int ewfe_plan_create_union(ewfe_plan **out, const ewfe_plan *const *plans, int32_t n_plans,
+                                      int64_t n_nodes, const int64_t *node_dof_host, int64_t *nnz_out)
+{
+    if (!out || !plans || n_plans < 1 || n_nodes <= 0 || !node_dof_host) {
+        ewfe::set_error("ewfe_plan_create_union: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    *out = nullptr;
+    for (int k = 0; k < n_plans; ++k)
+        if (!plans[k] || !plans[k]->d_indptr || plans[k]->n_dof != plans[0]->n_dof ||
+            plans[k]->device != plans[0]->device || (int64_t)plans[k]->indptr.size() != plans[0]->n_dof + 1) {
+            ewfe::set_error("ewfe_plan_create_union: every plan must be a device plan over the same dofs and device");
+            return EWFE_EINVAL;
+        }
+    int dev;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev != plans[0]->device) {
+        ewfe::set_error("ewfe_plan_create_union: current device differs from the plans' device");
+        return EWFE_EINVAL;
+    }
+    ewfe_plan *u = new (std::nothrow) ewfe_plan();
+    if (!u) {
+        ewfe::set_error("out of host memory");
+        return EWFE_ENOMEM;
+    }
+    u->elem_kind = -1;
+    u->n_nodes = n_nodes;
+    u->n_dof = plans[0]->n_dof;
+    u->device = dev;
+    ewfe::GatherHost gh;
+    std::vector<std::vector<int32_t>> maps;
+    int rc;
+    try {
+        rc = ewfe::build_union_host(*u, plans, n_plans, node_dof_host, n_nodes, gh, maps);
+    } catch (const std::bad_alloc &) {
+        rc = EWFE_ENOMEM;
+        ewfe::set_error("out of host memory while building the system plan");
+    }
+    if (rc) {
+        delete u;
+        return rc;
+    }
+    auto up = [&](int32_t **dst, const void *src, size_t bytes) -> int {
+        HIP_CHECK(hipMalloc((void **)dst, bytes ? bytes : 4));
+        if (bytes) HIP_CHECK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    if ((rc = up(&u->d_indptr, u->indptr.data(), sizeof(int32_t) * u->indptr.size())) ||
+        (rc = up(&u->d_indices, u->indices.data(), sizeof(int32_t) * u->indices.size())) ||
+        (rc = up(&u->d_rownode_dof, gh.rownode_dof.data(), sizeof(int32_t) * gh.rownode_dof.size())) ||
+        (rc = up(&u->d_nb_ptr, gh.nb_ptr.data(), sizeof(int32_t) * gh.nb_ptr.size())) ||
+        (rc = up(&u->d_nbcol, gh.nbcol.data(), sizeof(int32_t) * gh.nbcol.size())) ||
+        (rc = up(&u->d_diag_rank, gh.diag_rank.data(), sizeof(int32_t) * gh.diag_rank.size())) ||
+        (rc = up(&u->d_rn_meta, gh.rn_meta.data(), sizeof(int32_t) * gh.rn_meta.size()))) {
+        ewfe_plan_destroy(u);
+        return rc;
+    }
+    u->d_umap.assign(n_plans, nullptr);
+    u->umap_len.assign(n_plans, 0);
+    for (int k = 0; k < n_plans; ++k) {
+        if ((rc = up(&u->d_umap[k], maps[k].data(), sizeof(int32_t) * maps[k].size()))) {
+            ewfe_plan_destroy(u);
+            return rc;
+        }
+        u->umap_len[k] = (int64_t)maps[k].size();
+    }
+    if (nnz_out) *nnz_out = u->nnz;
+    *out = u;
+    return 0;
+}
+
 extern "C" int ewfe_plan_pipeline_orders(const ewfe_plan *p, int64_t *pipe_el_host, int64_t *pipe_row_host,
                                          int32_t *eorder_host, int32_t *rorder_host)
 {
@@ -2026,6 +2098,8 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
                     p->d_fslot,    p->d_mtab,     p->d_mid};
     for (auto b : bufs)
         if (b) (void)hipFree(b);
+    for (auto m : p->d_umap)
+        if (m) (void)hipFree(m);
     for (void *ev : p->chunk_events)
         if (ev) (void)hipEventDestroy(reinterpret_cast<hipEvent_t>(ev));
     if (p->aux_stream) (void)hipStreamDestroy(reinterpret_cast<hipStream_t>(p->aux_stream));
@@ -2173,7 +2247,8 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
         return EWFE_EINVAL;
     }
     if (!p->d_edof) {
-        ewfe::set_error("ewfe_eval_assemble: plan was built without device upload (ewfe_plan_create_host)");
+        ewfe::set_error("ewfe_eval_assemble: plan has no elements on the device (ewfe_plan_create_host, or a system "
+                        "plan of ewfe_plan_create_union)");
         return EWFE_EINVAL;
     }
     int dev;
@@ -2261,6 +2336,37 @@ extern "C" int ewfe_adjoint_param_grad(const ewfe_plan *p, int mat_kind, const d
         hipLaunchKernelGGL((k_adjoint8<FORM_TL, M_NH>), dim3((unsigned)blocks), dim3(64), 0, s, A, lambda_dev, grad_dev);
     else
         hipLaunchKernelGGL((k_adjoint8<FORM_SS, M_LE>), dim3((unsigned)blocks), dim3(64), 0, s, A, lambda_dev, grad_dev);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename VT>
+__global__ void k_csr_accumulate(const int32_t *__restrict__ map, int64_t n, const VT *__restrict__ src,
+                                 VT *__restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[map[i]] += src[i];
+}
+
+extern "C" int ewfe_csr_accumulate(const ewfe_plan *u, int32_t block, const void *src_vals_dev, void *dst_vals_dev,
+                                   int vals_dtype, void *stream)
+{
+    if (!u || block < 0 || block >= (int32_t)u->d_umap.size() || !src_vals_dev || !dst_vals_dev ||
+        (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32)) {
+        ewfe::set_error("ewfe_csr_accumulate: invalid arguments (a system plan of ewfe_plan_create_union and one of "
+                        "its block indices)");
+        return EWFE_EINVAL;
+    }
+    const int64_t n = u->umap_len[block];
+    if (n == 0) return 0;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+    if (vals_dtype == EWFE_F64)
+        hipLaunchKernelGGL(k_csr_accumulate<double>, dim3(blocks), dim3(256), 0, s, u->d_umap[block], n,
+                           (const double *)src_vals_dev, (double *)dst_vals_dev);
+    else
+        hipLaunchKernelGGL(k_csr_accumulate<float>, dim3(blocks), dim3(256), 0, s, u->d_umap[block], n,
+                           (const float *)src_vals_dev, (float *)dst_vals_dev);
     HIP_CHECK(hipGetLastError());
     return 0;
 }

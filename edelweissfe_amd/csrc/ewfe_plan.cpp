@@ -532,4 +532,114 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
     return 0;
 }
 
+// --- system plan of several element blocks ------------------------------------------------------------
+// The blocks of a model (ew/models/femodel.py keeps every element in one dict; the reference's DofManager and
+// CSRGenerator build one VIJ pattern over all of them, dofmanager.py:522-557, csrgenerator.pyx:47-98) each have
+// their own plan and pattern over the common dof numbering. The system pattern is the row-wise union of the
+// blocks' patterns (sorted unique columns per row: the canonical CSR of the union of their VIJ entries); its
+// row-node graph is what the linear solvers read, and maps[b][k] is the position in the system pattern of entry k
+// of block b's pattern.
+int build_union_host(ewfe_plan &u, const ewfe_plan *const *plans, int n_plans, const int64_t *node_dof,
+                     int64_t n_nodes, GatherHost &gather, std::vector<std::vector<int32_t>> &maps)
+{
+    const int64_t n_dof = u.n_dof;
+    u.indptr.assign(n_dof + 1, 0);
+    // row lengths of the union
+    std::vector<int64_t> len(n_dof, 0);
+    parallel_for(n_dof, [&](int64_t b, int64_t e) {
+        std::vector<int32_t> tmp;
+        for (int64_t r = b; r < e; ++r) {
+            tmp.clear();
+            for (int k = 0; k < n_plans; ++k) {
+                const ewfe_plan &p = *plans[k];
+                tmp.insert(tmp.end(), p.indices.begin() + p.indptr[r], p.indices.begin() + p.indptr[r + 1]);
+            }
+            std::sort(tmp.begin(), tmp.end());
+            len[r] = std::unique(tmp.begin(), tmp.end()) - tmp.begin();
+        }
+    });
+    int64_t acc = 0;
+    for (int64_t r = 0; r < n_dof; ++r) {
+        u.indptr[r] = (int32_t)acc;
+        acc += len[r];
+        if (acc >= (int64_t)INT32_MAX) {
+            set_error("nnz beyond int32 CSR indexing");
+            return -4;
+        }
+    }
+    u.indptr[n_dof] = (int32_t)acc;
+    u.nnz = acc;
+    u.indices.resize(acc);
+    maps.assign(n_plans, {});
+    for (int k = 0; k < n_plans; ++k) maps[k].resize(plans[k]->nnz);
+    parallel_for(n_dof, [&](int64_t b, int64_t e) {
+        std::vector<int32_t> tmp;
+        for (int64_t r = b; r < e; ++r) {
+            tmp.clear();
+            for (int k = 0; k < n_plans; ++k) {
+                const ewfe_plan &p = *plans[k];
+                tmp.insert(tmp.end(), p.indices.begin() + p.indptr[r], p.indices.begin() + p.indptr[r + 1]);
+            }
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            std::copy(tmp.begin(), tmp.end(), u.indices.begin() + u.indptr[r]);
+            for (int k = 0; k < n_plans; ++k) {
+                const ewfe_plan &p = *plans[k];
+                for (int32_t j = p.indptr[r]; j < p.indptr[r + 1]; ++j)
+                    maps[k][j] = u.indptr[r] + (int32_t)(std::lower_bound(tmp.begin(), tmp.end(), p.indices[j]) -
+                                                         tmp.begin());
+            }
+        }
+    });
+    // row nodes: nodes whose first dof has a nonempty row; neighbour blocks = the node's first row in steps of 3
+    std::vector<int32_t> rownode_dof;
+    for (int64_t n = 0; n < n_nodes; ++n) {
+        const int64_t d = node_dof[n];
+        if (d < 0) continue;
+        if (d + 3 > n_dof) {
+            set_error("node dof range beyond n_dof");
+            return -1;
+        }
+        if (u.indptr[d + 1] > u.indptr[d]) rownode_dof.push_back((int32_t)d);
+    }
+    std::sort(rownode_dof.begin(), rownode_dof.end());
+    const int64_t nr = (int64_t)rownode_dof.size();
+    u.n_rownodes = nr;
+    u.rows_cover_all_dofs = (3 * nr == n_dof);
+    gather.rownode_dof = rownode_dof;
+    gather.nb_ptr.assign(nr + 1, 0);
+    gather.diag_rank.assign(nr, 0);
+    for (int64_t r = 0; r < nr; ++r) {
+        const int32_t d = rownode_dof[r];
+        const int32_t rl = u.indptr[d + 1] - u.indptr[d];
+        for (int c = 1; c < 3; ++c)
+            if (u.indptr[d + c + 1] - u.indptr[d + c] != rl) {
+                set_error("the rows of a node's dofs have different patterns");
+                return -1;
+            }
+        if (rl % 3) {
+            set_error("a row's columns are not whole node blocks");
+            return -1;
+        }
+        gather.nb_ptr[r + 1] = gather.nb_ptr[r] + rl / 3;
+        u.max_deg = std::max<int32_t>(u.max_deg, rl / 3);
+    }
+    u.n_nbblocks = gather.nb_ptr[nr];
+    gather.nbcol.resize(u.n_nbblocks);
+    gather.rn_meta.resize((size_t)4 * nr);
+    for (int64_t r = 0; r < nr; ++r) {
+        const int32_t d = rownode_dof[r];
+        for (int32_t m = 0; m < gather.nb_ptr[r + 1] - gather.nb_ptr[r]; ++m) {
+            const int32_t col = u.indices[u.indptr[d] + 3 * m];
+            gather.nbcol[gather.nb_ptr[r] + m] = col;
+            if (col == d) gather.diag_rank[r] = m;
+        }
+        gather.rn_meta[4 * r + 0] = d;
+        gather.rn_meta[4 * r + 1] = u.indptr[d];
+        gather.rn_meta[4 * r + 2] = gather.nb_ptr[r];
+        gather.rn_meta[4 * r + 3] = gather.nb_ptr[r + 1] - gather.nb_ptr[r];
+    }
+    return 0;
+}
+
 }  // namespace ewfe

@@ -50,8 +50,10 @@ import time as _time
 import numpy as np
 
 from .._lib import EwfeError
-from ..assembly import ElementAssembly
+from ..adapter import ReferenceModelAdapter
+from ..assembly import ElementAssembly, SystemAssembly
 from ..dofmanager import DofManager
+from ..model import FEModel
 from ..stepping import (ConditionalStop, CutbackRequest, DivergingSolution, ReachedMaxIncrements,
                         ReachedMaxIterations, ReachedMinIncrementSize, StepFailed)
 
@@ -63,6 +65,13 @@ DEFAULT_JOB_INFO = {  # edelweissfe/config/phenomena.py:59-93, displacement fiel
     "fluxResidualTolerance": {"displacement": 1e-8},
     "fluxResidualToleranceAlternative": {"displacement": 5e-3},
 }
+
+
+def _setName(action, attribute):
+    """The node set of a step action: its name (the GPU path's actions) or the NodeSet object of a reference
+    action (dirichlet.py: ``nSet``, nodeforces.py: ``nodeSet``), whose name keys the DofManager's index lists."""
+    name = getattr(action, "nSetName", None)
+    return name if name is not None else getattr(action, attribute).name
 
 
 class _Journal:
@@ -117,6 +126,9 @@ class NISTGPU:
         self.options = self.NISTOptions.copy()
         self._updateOptions(kwargs, self.journal)
         self.assemblies = {}
+        self._adapters = {}
+        self._systemKey = None
+        self.system = None
         self.history = []  # per converged increment: list of (|R|inf, |ddU|inf) per iteration
         self.linearSolveIterations = []
 
@@ -145,18 +157,39 @@ class NISTGPU:
             self._updateOptions({k: v for k, v in cfg.items() if k in self.NISTOptions}, self.journal)
         if self.options["linsolver"].lower() not in ("pcg", "amg", "auto") + tuple(self.REFERENCE_LINSOLVERS):
             raise AttributeError("invalid linear solver {:} requested".format(self.options["linsolver"]))
-        if len(model.elementBlocks) != 1:
-            raise NotImplementedError("NISTGPU assembles one element block (one element type and section) per model")
+        # a model in the reference's object shape (model.nodes / model.elements / model.sections ...) runs through
+        # its GPU-path image; step actions and results stay with the given model
+        refModel = model
+        if isinstance(model, FEModel):
+            self._adapter = None
+        else:
+            self._adapter = self._adapters.get(id(model))
+            if self._adapter is None or self._adapter.refModel is not model:
+                self._adapter = self._adapters[id(model)] = ReferenceModelAdapter(model)
+            model = self._adapter.model
+        self.gpuModel = model
         device = self.options["device"]
         tic_setup = _time.perf_counter()
         self.theDofManager = DofManager(model, device=device)
         self.journal.message("total size of eq. system: {:}".format(self.theDofManager.nDof), self.identification)
-        block = model.elementBlocks[0]
-        if not block.hasMaterial:
-            raise Exception("No material was assigned to the element block")
-        if id(block) not in self.assemblies or self.assemblies[id(block)].nDof != self.theDofManager.nDof:
-            self.assemblies[id(block)] = ElementAssembly(block, self.theDofManager, model, device=device)
-        self.assembly = self.assemblies[id(block)]
+        for block in model.elementBlocks:
+            if not block.hasMaterial:
+                raise Exception("No material was assigned to the element block")
+        self.blockAssemblies = []
+        for block in model.elementBlocks:
+            asm = self.assemblies.get(id(block))
+            if asm is None or asm.block is not block or asm.nDof != self.theDofManager.nDof:
+                asm = self.assemblies[id(block)] = ElementAssembly(block, self.theDofManager, model, device=device)
+            self.blockAssemblies.append(asm)
+        self.assembly = self.blockAssemblies[0]
+        if len(self.blockAssemblies) == 1:
+            self.system = self.assembly
+        else:  # one linear system over every block's elements (the system plan of the blocks' plans)
+            key = tuple(id(a) for a in self.blockAssemblies)
+            if self._systemKey != key:
+                self.system = SystemAssembly(self.blockAssemblies, self.theDofManager, device=device)
+                self._systemKey = key
+        self._blockValues = None
         self._linsolver = self.REFERENCE_LINSOLVERS.get(self.options["linsolver"].lower(),
                                                         self.options["linsolver"].lower())
         if self._linsolver == "auto":
@@ -165,12 +198,12 @@ class NISTGPU:
             self._linsolver = "pcg"
             if self.theDofManager.nDof >= self.AUTO_AMG_MIN_DOFS:
                 try:
-                    self.assembly.multigrid()
+                    self.system.multigrid()
                     self._linsolver = "amg"
                 except EwfeError as e:
                     self.journal.message(f"multigrid not available ({e}); node-block Jacobi PCG", self.identification)
         if self._linsolver == "amg":
-            self.assembly.multigrid()  # aggregation and sparsity patterns: once per mesh
+            self.system.multigrid()  # aggregation and sparsity patterns: once per mesh
         self.computationTimes = dict.fromkeys(TIMING_KEYS + ["multigrid setup", "setup"], 0.0)
         # dof numbering, CSR pattern, element plan, multigrid patterns: once per step (mesh), not per increment
         self.computationTimes["setup"] = _time.perf_counter() - tic_setup
@@ -178,7 +211,7 @@ class NISTGPU:
         if tp not in ("fp64", "fp32", "fp32-mfma"):
             raise AttributeError("invalid tangentPrecision {:} (fp64, fp32, fp32-mfma)".format(tp))
         self._mfmaTangent = tp == "fp32-mfma"
-        K = self.assembly.newValues("fp64" if tp == "fp64" else "fp32")
+        K = self.system.newValues("fp64" if tp == "fp64" else "fp32")
 
         extrapolation = self.options["extrapolation"]
         maxIter, criticalIter = step.maxIter, step.criticalIter
@@ -186,16 +219,20 @@ class NISTGPU:
 
         U = getattr(model, "U", None)
         U = self.theDofManager.constructDofVector() if U is None else U.clone()
+        if self._adapter is not None:  # nonlinearimplicitstatic.py:194-195: U from the NodeField
+            U0 = self._adapter.readDisplacement(self.theDofManager)
+            if U0 is not None:
+                U.copy_(torch.as_tensor(U0, dtype=torch.float64, device=U.device))
         P = self.theDofManager.constructDofVector()
         dU = self.theDofManager.constructDofVector()
         self._dirichletRowsCache = {}
         prevTimeStep = None
-        self.applyStepActionsAtStepStart(model, step.actions)
+        self.applyStepActionsAtStepStart(refModel, step.actions)
         try:
             for timeStep in step.getTimeStep():
                 try:
                     U, dU, P, iterationCounter, incrementResidualHistory = self.solveIncrement(
-                        U, dU, P, K, step.actions, model, timeStep, prevTimeStep, extrapolation, maxIter,
+                        U, dU, P, K, step.actions, refModel, timeStep, prevTimeStep, extrapolation, maxIter,
                         maxGrowingIter)
                 except CutbackRequest as e:
                     self.journal.message(str(e), self.identification, 1)
@@ -211,6 +248,8 @@ class NISTGPU:
                         step.preventIncrementIncrease()
                     model.U, model.P, model.dU = U, P, dU
                     model.advanceToTime(timeStep.totalTime)
+                    if self._adapter is not None:
+                        self._adapter.writeBack(self.theDofManager, U, P, dU, timeStep.totalTime)
                     self.history.append(list(self._currentHistory))
                     self.journal.message("Converged in {:} iteration(s)".format(iterationCounter),
                                          self.identification, 1)
@@ -218,10 +257,10 @@ class NISTGPU:
             self.journal.errorMessage("Incrementation failed", self.identification)
             raise StepFailed()
         except ConditionalStop:
-            self.applyStepActionsAtStepEnd(model, step.actions)
+            self.applyStepActionsAtStepEnd(refModel, step.actions)
         else:
-            self.applyStepActionsAtStepEnd(model, step.actions)
-        return True, model
+            self.applyStepActionsAtStepEnd(refModel, step.actions)
+        return True, refModel
 
     def solveIncrement(self, U_n, dU, P, K, stepActions, model, timeStep, prevTimeStep, extrapolation, maxIter,
                        maxGrowingIter):
@@ -238,7 +277,7 @@ class NISTGPU:
         self.applyStepActionsAtIncrementStart(model, timeStep, stepActions)
         dU, isExtrapolatedIncrement = self.extrapolateLastIncrement(extrapolation, timeStep, dU, dirichlets,
                                                                     prevTimeStep, model)
-        elements = model.elementBlocks
+        elements = self.gpuModel.elementBlocks
         while True:
             U_np.copy_(U_n)
             U_np += dU
@@ -269,7 +308,26 @@ class NISTGPU:
     # ------------------------------------------------------------------------------------------------
     def computeElements(self, elements, U_np, dU, P, K, F, timeStep):
         tic = _time.perf_counter()
-        pnewdt = self.assembly.evaluate(U_np, dU, P, F, K, mfma=getattr(self, "_mfmaTangent", False))
+        mfma = getattr(self, "_mfmaTangent", False)
+        if len(self.blockAssemblies) == 1:
+            pnewdt = self.assembly.evaluate(U_np, dU, P, F, K, mfma=mfma)
+        else:
+            # block by block (blocks hold consecutive runs of the model's elements): P, F and the system's CSR
+            # values are the sums of the blocks' in block order
+            if self._blockValues is None or self._blockValues[0][0].dtype != K.dtype:
+                dm = self.theDofManager
+                self._blockValues = [(a.newValues("fp64" if K.dtype == self.torch.float64 else "fp32"),
+                                      dm.constructDofVector(), dm.constructDofVector()) for a in self.blockAssemblies]
+            P.zero_()
+            F.zero_()
+            K.zero_()
+            pnewdt = None
+            for b, (asm, (Kb, Pb, Fb)) in enumerate(zip(self.blockAssemblies, self._blockValues)):
+                pb = asm.evaluate(U_np, dU, Pb, Fb, Kb, mfma=mfma)
+                P += Pb
+                F += Fb
+                self.system.accumulate(b, Kb, K)
+                pnewdt = pb.clone() if pnewdt is None else self.torch.minimum(pnewdt, pb)
         pdt = float(pnewdt.item())  # synchronises
         self.computationTimes["elements"] += _time.perf_counter() - tic
         if pdt < 1.0:
@@ -285,7 +343,7 @@ class NISTGPU:
         if key not in self._dirichletRowsCache:
             idx = self.findDirichletIndices(dirichlet)
             self._dirichletRowsCache[key] = self.torch.as_tensor(idx, dtype=self.torch.int64,
-                                                                 device=self.assembly.device)
+                                                                 device=self.system.device)
         return self._dirichletRowsCache[key]
 
     def _allRows32(self, dirichlets):
@@ -293,13 +351,13 @@ class NISTGPU:
         if key not in self._dirichletRowsCache:
             rows = np.concatenate([self.findDirichletIndices(d) for d in dirichlets]) if dirichlets else \
                 np.zeros(0, dtype=np.int64)
-            self._dirichletRowsCache[key] = self.torch.as_tensor(rows.astype(np.int32), device=self.assembly.device)
+            self._dirichletRowsCache[key] = self.torch.as_tensor(rows.astype(np.int32), device=self.system.device)
         return self._dirichletRowsCache[key]
 
     def applyDirichletK(self, K, dirichlets):
         if dirichlets:
             tic = _time.perf_counter()
-            self.assembly.dirichletRows(self._allRows32(dirichlets), K)
+            self.system.dirichletRows(self._allRows32(dirichlets), K)
             self.computationTimes["dirichlet K"] += _time.perf_counter() - tic
         return K
 
@@ -360,22 +418,22 @@ class NISTGPU:
         rtol = self.options["pcgToleranceFP32"] if A.dtype == self.torch.float32 else self.options["pcgTolerance"]
         amg = self._linsolver == "amg"
         if amg:
-            self.assembly.multigridSetup(A, rows)
-            self.torch.cuda.synchronize(self.assembly.device)
+            self.system.multigridSetup(A, rows)
+            self.torch.cuda.synchronize(self.system.device)
             toc = _time.perf_counter()
             self.computationTimes["multigrid setup"] += toc - tic
             tic = toc
         precond = "amg" if amg else "jacobi"
-        it, rel = self.assembly.pcg(A, b, x, rows, rtol=rtol, maxit=self.options["pcgMaxIter"],
+        it, rel = self.system.pcg(A, b, x, rows, rtol=rtol, maxit=self.options["pcgMaxIter"],
                                     check_every=4 if amg else 32, precond=precond, setup=False)
         if A.dtype == self.torch.float64:
             for _ in range(self.options["linsolverRefinement"]):
                 # Dirichlet rows are unit rows with x = b, so r = 0 there
-                r = self.assembly.residual(A, b, x, self.theDofManager.constructDofVector())
+                r = self.system.residual(A, b, x, self.theDofManager.constructDofVector())
                 if not bool(r.any().item()):
                     break
                 d = self.theDofManager.constructDofVector()
-                it2, _ = self.assembly.pcg(A, r, d, rows, rtol=rtol, maxit=self.options["pcgMaxIter"],
+                it2, _ = self.system.pcg(A, r, d, rows, rtol=rtol, maxit=self.options["pcgMaxIter"],
                                            check_every=4 if amg else 32, precond=precond, setup=False)
                 it += it2
                 x += d
@@ -390,7 +448,7 @@ class NISTGPU:
     def assembleLoads(self, nodeForces, distributedLoads, bodyForces, U_np, PExt, K, timeStep):
         """nonlinearimplicitstatic.py:903-945: nodal forces, then distributed loads, then body forces."""
         for cLoad in nodeForces:
-            idx = self.theDofManager.idcsOfFieldsOnNodeSetsInDofVector[cLoad.field][cLoad.nSetName]
+            idx = self.theDofManager.idcsOfFieldsOnNodeSetsInDofVector[cLoad.field][_setName(cLoad, "nodeSet")]
             load = cLoad.getCurrentLoad(timeStep).flatten()
             PExt.index_add_(0, self.torch.as_tensor(idx, device=PExt.device),
                             self.torch.as_tensor(load, dtype=self.torch.float64, device=PExt.device))
@@ -409,11 +467,10 @@ class NISTGPU:
             if str(dLoad.loadType).lower() != "pressure":
                 raise NotImplementedError(f"distributed load type {dLoad.loadType} is not on the GPU path (pressure)")
             load = np.asarray(dLoad.getCurrentLoad(timeStep), dtype=np.float64).ravel()
-            for faceID, (blockIndex, elements) in dLoad.surface.items():
-                if blockIndex != 0:
-                    raise NotImplementedError("distributed loads act on the element block of the GPU path")
-                self.assembly.surfacePressure(PExt, elements, faceID, float(load[0]))
-        self.torch.cuda.synchronize(self.assembly.device)
+            for faceID, elSet in dLoad.surface.items():
+                for blockIndex, elements in self._elementSetParts(elSet):
+                    self.blockAssemblies[blockIndex].surfacePressure(PExt, elements, int(faceID), float(load[0]))
+        self.torch.cuda.synchronize(self.system.device)
         self.computationTimes["distributed loads"] += _time.perf_counter() - tic
         return PExt, K
 
@@ -424,10 +481,11 @@ class NISTGPU:
             return PExt, K
         tic = _time.perf_counter()
         for bForce in bodyForces:
-            if bForce.blockIndex != 0:
-                raise NotImplementedError("body forces act on the element block of the GPU path")
-            self.assembly.bodyForce(PExt, bForce.getCurrentLoad(timeStep), bForce.elementSet)
-        self.torch.cuda.synchronize(self.assembly.device)
+            parts = ([(bForce.blockIndex, bForce.elementSet)] if hasattr(bForce, "blockIndex") else
+                     self._elementSetParts(bForce.elementSet))
+            for blockIndex, elements in parts:
+                self.blockAssemblies[blockIndex].bodyForce(PExt, bForce.getCurrentLoad(timeStep), elements)
+        self.torch.cuda.synchronize(self.system.device)
         self.computationTimes["body forces"] += _time.perf_counter() - tic
         return PExt, K
 
@@ -447,8 +505,20 @@ class NISTGPU:
                 return True
         return False
 
+    def _elementSetParts(self, elSet):
+        """[(block index, element indices)] of an element set given as (block, indices), as a set of reference
+        element objects (models in the reference's shape) or by name."""
+        if isinstance(elSet, tuple) and len(elSet) == 2 and not hasattr(elSet, "name"):
+            return [elSet]
+        if isinstance(elSet, str):
+            if elSet in self.gpuModel.elementSets:
+                return [self.gpuModel.elementSets[elSet]]
+            return self._adapter.multiBlockElementSets[elSet]
+        return self._adapter.elementSetParts(elSet)
+
     def findDirichletIndices(self, dirichlet):
-        fieldIndices = self.theDofManager.idcsOfFieldsOnNodeSetsInDofVector[dirichlet.field][dirichlet.nSetName]
+        fieldIndices = self.theDofManager.idcsOfFieldsOnNodeSetsInDofVector[dirichlet.field][_setName(dirichlet,
+                                                                                                       "nSet")]
         return fieldIndices.reshape((len(dirichlet.nSet), -1))[:, dirichlet.components].flatten()
 
     def applyStepActionsAtStepStart(self, model, stepActions):

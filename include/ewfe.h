@@ -121,6 +121,23 @@ int ewfe_plan_info(const ewfe_plan *plan, int64_t *n_el, int32_t *nodes_per_el, 
                    int64_t *n_dof);
 int ewfe_plan_destroy(ewfe_plan *plan);
 
+/* System plan of a model with several element blocks (element types or material kinds): the reference keeps all
+ * elements of a model in one VIJ pattern (DofManager over model.elements, dofmanager.py:522-557) and one
+ * CSRGenerator (csrgenerator.pyx:47-98). Here every block has its own plan over the common dofs (node_dof and n_dof
+ * of every plan the same); the system plan holds the row-wise union of their patterns (sorted unique columns: the
+ * canonical CSR of the union of the blocks' VIJ entries) and its row-node graph, and takes ewfe_dirichlet_rows,
+ * ewfe_spmv, ewfe_residual, ewfe_pcg(_noalloc), ewfe_pcg_workspace_size and ewfe_mg_create like an element plan;
+ * ewfe_eval_assemble and the load / result calls stay with the block plans. plans[0 .. n_plans) stay owned by the
+ * caller and may be destroyed before the system plan. */
+int ewfe_plan_create_union(ewfe_plan **plan, const ewfe_plan *const *plans, int32_t n_plans, int64_t n_nodes,
+                           const int64_t *node_dof_host, int64_t *nnz_out);
+
+/* dst[map(k)] += src[k] over the pattern of block `block` of a system plan: the block's CSR values (its plan's
+ * pattern) added into the system's CSR values. Blocks added in block order give every system entry the sum of its
+ * block contributions in that order (csrgenerator.pyx:100-115 sums them in element order). */
+int ewfe_csr_accumulate(const ewfe_plan *system_plan, int32_t block, const void *src_vals_dev, void *dst_vals_dev,
+                        int vals_dtype, void *stream);
+
 /* Per-element material parameter sets (several sections with materials of one kind, e.g. two linear
  * elastic materials on two element sets: every section assigns its material to the elements of its sets,
  * ew/sections/base/sectionbase.py:84-112, ew/generators/abqmodelconstructor.py:326-370). params_host holds

@@ -386,30 +386,64 @@ def _amp_delta(tstep):
 
 def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flux_floor=1e-7, log=None,
                state_n=None, U0=None):
-    """NIST.solveStep for one element block with Dirichlet + nodal-force actions.
+    """NIST.solveStep with Dirichlet + nodal-force actions (and, for one block, element loads).
 
-    model: dict(el_kind, mat_kind, mp, X, conn, el_mp?, el_mp8?)
+    model: dict(el_kind, mat_kind, mp, X, conn, el_mp?, el_mp8?), or dict(X, blocks=[dict(el_kind, mat_kind, mp,
+           conn, el_mp?, el_mp8?), ...]) for several element blocks (element types / material kinds) over X
     step:  dict(maxInc, minInc, maxNumInc, maxIter, criticalIter=5, maxGrowIter=10, cutbackFactor=0.25,
                 startInc=1.0, stepLength=1.0, extrapolation='linear',
                 dirichlet=[(node_idx, {comp: value})], nodeforces=[(node_idx, vec3)], bodyforces=[vec3],
                 pressures=[(element idx, face 1..6, magnitude, f(stepProgress) or None)])
     Returns dict(U, P, state, history=[per increment: list of (|R|inf, |ddU|inf)], iterations=[...]).
     """
-    X, conn = model["X"], model["conn"]
-    base, nDof, edofs = dof_numbering(len(X), conn)
-    I, J = vij_pattern(edofs)
-    indptr, indices, x = csr_pattern(I, J, nDof)
+    X = model["X"]
+    blocks = model.get("blocks")
+    single = blocks is None
+    if single:
+        blocks = [dict(el_kind=model["el_kind"], mat_kind=model["mat_kind"], mp=model["mp"], conn=model["conn"],
+                       el_mp=model.get("el_mp"), el_mp8=model.get("el_mp8"))]
+    elif step.get("pressures") or step.get("bodyforces"):
+        raise NotImplementedError("element loads of models with several blocks")
+    # one DofManager and one VIJ pattern over the elements of every block, in block order (femodel.py keeps
+    # them in one dict; dofmanager.py:259-300, 522-557)
+    used = np.zeros(len(X), dtype=bool)
+    for b in blocks:
+        used[np.asarray(b["conn"]).ravel()] = True
+    base = -np.ones(len(X), dtype=np.int64)
+    base[used] = 3 * np.arange(used.sum())
+    nDof = int(3 * used.sum())
+    bedofs = [(base[np.asarray(b["conn"])][:, :, None] + np.arange(3)[None, None, :]).reshape(len(b["conn"]), -1)
+              for b in blocks]
+    vij = [vij_pattern(e) for e in bedofs]
+    I, J = np.concatenate([v[0] for v in vij]), np.concatenate([v[1] for v in vij])
+    indptr, indices, xall = csr_pattern(I, J, nDof)
+    bx = np.split(xall, np.cumsum([len(v[0]) for v in vij])[:-1])
     nnz = len(indices)
-    mk = MAT_KINDS[model["mat_kind"]]
-    nq = NQP[EL_KINDS[model["el_kind"]]]
-    nEl = len(conn)
-    nstate = NSTATE[mk]
+    conn, edofs = blocks[0]["conn"], bedofs[0]  # element loads: single-block models only
     if state_n is None:
-        state_n = np.zeros(nEl * nq * nstate)
+        bstate = [np.zeros(len(b["conn"]) * NQP[EL_KINDS[b["el_kind"]]] * NSTATE[MAT_KINDS[b["mat_kind"]]])
+                  for b in blocks]
+    else:
+        bstate = [state_n] if single else list(state_n)
     U = np.zeros(nDof) if U0 is None else U0.copy()
     dU = np.zeros(nDof)
     P = np.zeros(nDof)
-    nAcc = 3 * conn.size  # dofmanager.py:354-379: sum over elements of nodes*dim
+    nAcc = sum(3 * np.asarray(b["conn"]).size for b in blocks)  # dofmanager.py:354-379: elements x nodes x dim
+
+    def evaluate(U_np, dU):
+        """computeElements over every block in block order: P, F and the CSR values summed block by block."""
+        outs = [eval_assemble(b["el_kind"], b["mat_kind"], b["mp"], b["conn"], X, e, U_np, dU, st, xb, nnz,
+                              el_mp=b.get("el_mp"), el_mp8=b.get("el_mp8"))
+                for b, e, st, xb in zip(blocks, bedofs, bstate, bx)]
+        out = dict(outs[0])
+        for o in outs[1:]:
+            out["P"] = out["P"] + o["P"]
+            out["F"] = out["F"] + o["F"]
+            out["data"] = out["data"] + o["data"]
+            out["failed"] = out["failed"] or o["failed"]
+            out["pNewDT"] = min(out["pNewDT"], o["pNewDT"])
+        out["states"] = [o["state"] for o in outs]
+        return out
 
     dirichlets = []
     for nodes, comps in step.get("dirichlet", []):
@@ -447,8 +481,7 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
             resHist = (0.0, 0)
             while True:
                 U_np = U + dU
-                out = eval_assemble(model["el_kind"], model["mat_kind"], model["mp"], conn, X, edofs, U_np, dU,
-                                    state_n, x, nnz, el_mp=model.get("el_mp"), el_mp8=model.get("el_mp8"))
+                out = evaluate(U_np, dU)
                 if out["failed"]:
                     raise CutbackRequest("element requested cutback", out["pNewDT"])
                 Pv, Fv = out["P"], out["F"]
@@ -456,11 +489,11 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
                 for idx, load in nodeforces:
                     PExt[idx] += load * tstep["stepProgress"]
                 for els, face, mag, amp in step.get("pressures", []):
-                    PExt += surface_pressure(model["el_kind"], conn, X, edofs, nDof, face,
+                    PExt += surface_pressure(blocks[0]["el_kind"], conn, X, edofs, nDof, face,
                                              mag * (amp(tstep["stepProgress"]) if amp else tstep["stepProgress"]),
                                              els)
                 for bvec in step.get("bodyforces", []):
-                    PExt += body_force(model["el_kind"], conn, X, edofs, nDof,
+                    PExt += body_force(blocks[0]["el_kind"], conn, X, edofs, nDof,
                                        np.asarray(bvec, float) * tstep["stepProgress"])
                 R = Pv + PExt
                 if it == 0 and not isExtrap and dirichlets:
@@ -512,8 +545,8 @@ def solve_step(model, step, tol_flux=1e-8, tol_corr=1e-8, tol_flux_alt=5e-3, flu
             ts.allowInc = False
         U = U_np
         P = Pv
-        state_n = out["state"]
+        bstate = out["states"]
         history.append(hist)
         iterations.append(it)
-    return dict(U=U, P=P, state=state_n, history=history, iterations=iterations, nDof=nDof, edofs=edofs,
-                indptr=indptr, indices=indices)
+    return dict(U=U, P=P, state=bstate[0] if single else bstate, history=history, iterations=iterations, nDof=nDof,
+                edofs=edofs if single else bedofs, indptr=indptr, indices=indices)

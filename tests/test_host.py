@@ -367,3 +367,45 @@ def test_structured_gather_index_math_matches_pattern(E):
             # contributions: every element holding both nodes, ascending element, with their local indices
             ref = [(e, a, list(conn[e]).index(nb)) for e, a in inc[r] if nb in conn[e]]
             assert contrib == sorted(ref)
+
+
+def test_reference_shaped_model_adapter():
+    """A model in the reference's object shape (nodes, elements, sections, sets, NodeField) becomes one element
+    block per element type and material kind, with the elements' order kept, materials of one kind in a block's
+    table, sets mapped to indices, and results written back into the NodeField's rows."""
+    import torch
+
+    import refshape as R
+    from edelweissfe_amd.adapter import ReferenceModelAdapter
+    from edelweissfe_amd.dofmanager import DofManager
+
+    X, conn, sets, _ = boxGenArrays(4, 2, 3, elType="C3D8")
+    le1 = {"name": "LinearElastic", "properties": np.array([210000.0, 0.3])}
+    j2 = {"name": "VonMises", "properties": np.array([210000.0, 0.3, 250.0, 1000.0, 0.0, 0.0])}
+    le2 = {"name": "LinearElastic", "properties": np.array([30000.0, 0.2])}
+    ref = R.model_from_arrays(X, [("C3D8", conn[:8], le1), ("C3D8", conn[8:16], j2), ("C3D8", conn[16:], le2)],
+                              {"bottom": sets["gen_back"], "top": sets["gen_front"]})
+    nf = ref.nodeFields["displacement"]
+    nf.nodes.reverse()  # rows of the NodeField in another order than the dofs
+    nf._indicesOfNodesInArray = {n: i for i, n in enumerate(nf.nodes)}
+    ad = ReferenceModelAdapter(ref)
+    gm = ad.model
+    assert np.allclose(gm.nodeCoordinates, X)
+    assert [(b.elType, b.material.name) for b in gm.elementBlocks] == [("C3D8", "linearelastic"), ("C3D8", "vonmises")]
+    b0, b1 = gm.elementBlocks
+    assert np.array_equal(b0.conn, conn[np.r_[0:8, 16:len(conn)]]) and np.array_equal(b1.conn, conn[8:16])
+    assert np.array_equal(b0.labels, np.r_[1:9, 17:len(conn) + 1])
+    assert np.array_equal(b0.materialIndex, np.repeat([0, 1], [8, len(conn) - 16]))
+    assert np.array_equal(b0.materialParameterTable()[1, :2], [30000.0, 0.2])
+    assert np.array_equal(gm.nodeSets["top"], sets["gen_front"])
+    assert gm.elementSets["block1"][0] == 1 and np.array_equal(gm.elementSets["block1"][1], np.arange(8))
+    assert gm.elementSets["block2"][0] == 0 and np.array_equal(gm.elementSets["block2"][1], np.arange(8, 16))
+    dm = DofManager(gm, device="cpu")
+    U = torch.arange(dm.nDof, dtype=torch.float64)
+    ad.writeBack(dm, U, 2 * U, 3 * U, 1.5)
+    for r, n in enumerate(dm.nodesWithDofs):
+        row = nf._indicesOfNodesInArray[list(ref.nodes.values())[n]]
+        assert np.array_equal(nf["U"][row], [3 * r, 3 * r + 1, 3 * r + 2])
+        assert np.array_equal(nf["dU"][row], 3 * nf["U"][row])
+    assert ref.time == 1.5
+    assert np.array_equal(ad.readDisplacement(dm), U.numpy())

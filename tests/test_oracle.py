@@ -282,3 +282,29 @@ def test_exact_residual_is_correctly_rounded():
     exact = [float(Fraction(b[i]) - sum(Fraction(Kd[i, j]) * Fraction(x[j]) for j in range(n))) for i in range(n)]
     assert np.array_equal(r, np.array(exact))
     assert not np.array_equal(r, b - K @ x)  # the plain residual is not that accurate here
+
+
+def test_oracle_newton_with_several_blocks():
+    """The oracle's Newton over several element blocks: a box split into two blocks, one linear elastic and one
+    von Mises whose yield stress is never reached (the same elastic law), converges to the single-block
+    linear-elastic solution; a J2 block that does yield changes it."""
+    X, conn, sets = O.boxgen(4, 2, 3)
+    step = dict(maxInc=0.5, minInc=1e-4, maxNumInc=100, maxIter=25,
+                dirichlet=[(np.asarray(sets["gen_back"]), {0: 0.0, 1: 0.0, 2: 0.0}),
+                           (np.asarray(sets["gen_front"]), {2: 4e-3})])
+    le = [210000.0, 0.3]
+    one = O.solve_step(dict(el_kind="C3D8", mat_kind="linearelastic", mp=le, X=X, conn=conn), step)
+    half = len(conn) // 2
+
+    def two(fy):
+        blocks = [dict(el_kind="C3D8", mat_kind="linearelastic", mp=le, conn=conn[:half]),
+                  dict(el_kind="C3D8", mat_kind="vonmises", mp=[210000.0, 0.3, fy, 1000.0, 0.0, 0.0],
+                       conn=conn[half:])]
+        return O.solve_step(dict(X=X, blocks=blocks), step)
+
+    el = two(1e12)
+    assert np.abs(el["U"] - one["U"]).max() < 1e-12 * np.abs(one["U"]).max()
+    assert len(el["state"]) == 2 and np.all(el["state"][1].reshape(-1, 13)[:, 12] == 0.0)  # kappa stays 0
+    pl = two(250.0)
+    assert np.abs(pl["U"] - one["U"]).max() > 1e-3 * np.abs(one["U"]).max()
+    assert np.all(pl["state"][1].reshape(-1, 13)[:, 12] > 0.0)
