@@ -56,14 +56,17 @@ def algorithmic_bytes_per_element(n_el, n_nodes, nnz, n_state, n_qp, val_bytes, 
             + val_bytes * nnz / n_el)
 
 
-def newton_increment(n, tangent, strain=2e-3, pcg_tol=1e-10, seed=0, linsolver="amg", pcg_tol_fp32=1e-6):
-    """One implicit static increment of the same J2 cube from the virgin state to `strain` (into the plastic
-    range), solved by NISTGPU: uz = 0 on z = 0, uz = strain on z = 1, ux = uy = 0 at (0,0,0), uy = 0 at
-    (1,0,0). Newton to the reference's criteria |R|inf <= 1e-8 * averaged flux (without the 1e-7 absolute
-    floor) and |ddU|inf < 1e-8; returns timings per phase, iterations and PCG iterations."""
+def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, linsolver="amg"):
+    """The J2 cube's load case as configured: from the virgin state to `strain` uniaxial strain in `increments`
+    equal increments (after the reference's zero-size first increment, adaptivetimestepper.py:90-134; NIST's default
+    linear extrapolation), solved by NISTGPU: uz = 0 on z = 0, uz = strain on z = 1, ux = uy = 0 at (0,0,0), uy = 0
+    at (1,0,0). Newton to the reference's criteria |R|inf <= 1e-8 * averaged flux (without the 1e-7 absolute
+    floor) and |ddU|inf < 1e-8, linear solves to the relative PCG tolerance `pcg_tol`. Returns timings per phase,
+    iterations per increment and PCG iterations; plan and multigrid-pattern construction (once per mesh) are
+    reported apart."""
     import torch
 
-    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
+    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU, _Journal
     from edelweissfe_amd.stepping import AdaptiveStep, Dirichlet
 
     m, b, X, conn = build_workload(n, seed)
@@ -73,61 +76,93 @@ def newton_increment(n, tangent, strain=2e-3, pcg_tol=1e-10, seed=0, linsolver="
         "origin": Dirichlet("origin", m, "gen_bottomLeftBack", {0: 0.0, 1: 0.0}),
         "xaxis": Dirichlet("xaxis", m, "gen_bottomRightBack", {1: 0.0}),
     }}
-    step = AdaptiveStep(1, m, acts, maxInc=1.0, startInc=1.0, minInc=1e-3, maxIter=20)
+    inc = 1.0 / increments
+    step = AdaptiveStep(1, m, acts, maxInc=inc, startInc=inc, minInc=1e-4, maxIter=25)
     jobInfo = {"fieldCorrectionTolerance": {"displacement": 1e-8},
                "fluxResidualTolerance": {"displacement": 1e-8},
                "fluxResidualToleranceAlternative": {"displacement": 1e-8}}
-    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import _Journal
-
-    solver = NISTGPU(jobInfo, _Journal(verbose=True, stream=sys.stderr), tangentPrecision=tangent,
-                     pcgTolerance=pcg_tol, pcgToleranceFP32=pcg_tol_fp32, fluxResidualFloor=0.0,
-                     extrapolation="off", linsolver=linsolver)
+    solver = NISTGPU(jobInfo, _Journal(verbose=False, stream=sys.stderr), tangentPrecision=tangent,
+                     pcgTolerance=pcg_tol, pcgToleranceFP32=pcg_tol, fluxResidualFloor=0.0, linsolver=linsolver)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solver.solveStep(step, m)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     its = [len(h) for h in solver.history]
-    newton_iters = sum(its)  # linear solves = convergence checks of the increments that converged
-    pcg_its = [it for it, _ in solver.linearSolveIterations]
+    newton_iters = sum(its)
+    solves = [it for it, _ in solver.linearSolveIterations]
     U = m.U.detach().cpu().numpy()
-    levels = solver.assembly.mgLevels
     t_setup = solver.computationTimes["setup"]
-    return dict(seconds=t - t_setup, setup_seconds=t_setup, linsolver=linsolver, multigrid_levels=levels,
-                pcg_tolerance=pcg_tol_fp32 if tangent == "fp32" else pcg_tol,
-                newton_iterations=newton_iters,
-                iterations_per_increment=its, pcg_iterations=pcg_its,
+    return dict(seconds=t - t_setup, setup_seconds=t_setup, pcg_tolerance=pcg_tol, increments=len(its) - 1,
+                newton_iterations=newton_iters, iterations_per_increment=its, linear_solves=len(solves),
+                pcg_iterations_total=int(sum(solves)),
                 ms_per_newton_iteration=1e3 * (t - t_setup) / max(1, newton_iters),
-                phases_s={k: v for k, v in solver.computationTimes.items() if v > 0},
-                residual_history=[h for h in solver.history], U=U)
+                phases_s={k: round(v, 4) for k, v in solver.computationTimes.items() if v > 0},
+                multigrid_levels=solver.system.mgLevels, U=U)
 
 
-def cpu_baseline(X, conn, U_n, dU, layers, n_per_layer, target_s=12.0):
-    """The oracle (C + OpenMP restatement of NISTParallel's element loop, serial P/F scatter and serial
-    VIJ->CSR scatter) on the first `layers` x-layers of the same mesh and inputs, repeated until about
-    `target_s` seconds of evaluation time have been measured (median reported); plus one single-thread run."""
-    from oracle import oracle as O
+def host_cores():
+    """The host cores this process may use and how that was determined: the CPU affinity mask, capped by a cgroup
+    CPU quota when one is set; plus what nproc, OMP_NUM_THREADS and the CPU model say."""
+    info = {"os_cpu_count": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+        info["affinity"] = cores
+    except (AttributeError, OSError):
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            q = max(1, int(float(quota) / float(period)))
+            info["cgroup_quota_cores"] = q
+            cores = min(cores, q)
+    except (OSError, ValueError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            info["cpu_model"] = next(ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        info["cpu_model"] = None
+    info["cores_used"] = cores
+    return cores, info
 
-    O.build()
-    ns = layers * n_per_layer
+
+def _slab(X, conn, ns):
+    """The first ns elements of a mesh with its nodes renumbered compactly (the oracle's inputs)."""
     c = conn[:ns]
     used = np.unique(c)
     remap = -np.ones(len(X), dtype=np.int64)
     remap[used] = np.arange(len(used))
-    cs = remap[c].astype(np.int32)
-    Xs = X[used]
+    return remap[c].astype(np.int32), X[used], used
+
+
+def cpu_baseline(el, mat, mp, X, conn, U_np, dU, ns, desc, target_s=12.0, state_from=None):
+    """The oracle (C + OpenMP restatement of NISTParallel's element loop, serial P/F scatter and serial VIJ->CSR
+    scatter, nonlinearimplicitstaticparallelmk2.pyx:111-198, csrgenerator.pyx:100-115) on the first `ns` elements
+    of the mesh (`desc` names them), on every host core this process may use, repeated until about `target_s`
+    seconds of evaluation time were measured (median reported); plus one single-thread run. state_from: the
+    displacement from which a converged state is first computed (J2)."""
+    from oracle import oracle as O
+
+    O.build()
+    cs, Xs, used = _slab(X, conn, ns)
     base, nDof, edofs = O.dof_numbering(len(Xs), cs)
     I, J = O.vij_pattern(edofs)
     indptr, indices, x = O.csr_pattern(I, J, nDof)
     del I, J
-    Un = U_n.reshape(-1, 3)[used].ravel()
+    Un = U_np.reshape(-1, 3)[used].ravel()
     dUs = dU.reshape(-1, 3)[used].ravel()
-    threads = min(os.cpu_count() or 1, 16)
-    st0 = np.zeros(ns * 8 * 13)
-    r0 = O.eval_assemble("C3D8", "vonmises", J2, cs, Xs, edofs, Un, Un, st0, x, len(indices), n_threads=threads)
+    threads, info = host_cores()
+    nq = O.NQP[O.EL_KINDS[el]]
+    st = np.zeros(ns * nq * O.NSTATE[O.MAT_KINDS[mat]])
+    if state_from is not None:
+        S = state_from.reshape(-1, 3)[used].ravel()
+        st = O.eval_assemble(el, mat, mp, cs, Xs, edofs, S, S, st, x, len(indices), n_threads=threads)["state"]
+
     def one(nt):
-        r = O.eval_assemble("C3D8", "vonmises", J2, cs, Xs, edofs, Un + dUs, dUs, r0["state"], x, len(indices),
-                            n_threads=nt)
+        r = O.eval_assemble(el, mat, mp, cs, Xs, edofs, Un, dUs, st, x, len(indices), n_threads=nt)
         return r["t_elements"] + r["t_csr"], r["t_elements"], r["t_csr"]
 
     ts = [one(threads)]
@@ -136,11 +171,56 @@ def cpu_baseline(X, conn, U_n, dU, layers, n_per_layer, target_s=12.0):
     t, te, tc = (float(np.median([a[k] for a in ts])) for k in range(3))
     t1 = one(1)
     return dict(value=ns / t, unit="elements/s", cores=threads, kind="port",
-                sample=f"first {layers} x-layers ({ns} elements) of the same jittered 128^3 J2 mesh and state, "
-                       f"{len(ts)} timed evaluations ({sum(a[0] for a in ts):.1f} s), median "
+                sample=f"{desc}: {ns} elements, {len(ts)} timed evaluations ({sum(a[0] for a in ts):.1f} s), median "
                        f"'elements' {te:.4f} s + 'CSR generation' {tc:.4f} s",
-                single_thread_value=ns / t1[0],
+                host=info, single_thread_value=ns / t1[0],
                 single_thread_split_s={"elements": t1[1], "CSR generation": t1[2]})
+
+
+def cfg1_newton():
+    """The plain CPU-runnable configuration in full: a 10^3 C3D8 unit cube, linear elastic E = 210000, nu = 0.3,
+    bottom face fixed, top face uz = +1e-3 in one increment, Newton to a 1e-10 flux tolerance: the oracle's Newton
+    (element loop + serial scatters in C, scipy LU with refinement; one host thread) against NISTGPU."""
+    import torch
+
+    from edelweissfe_amd.model import ElementBlock, FEModel, Material, boxGenArrays
+    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import DEFAULT_JOB_INFO, NISTGPU
+    from edelweissfe_amd.stepping import AdaptiveStep, Dirichlet
+    from oracle import oracle as O
+
+    LE = [210000.0, 0.3]
+    X, conn, sets, _ = boxGenArrays(10, 10, 10, elType="C3D8")
+    jobInfo = dict(DEFAULT_JOB_INFO, fluxResidualTolerance={"displacement": 1e-10})
+
+    def gpu_run():
+        m = FEModel()
+        m.addNodes(X)
+        b = ElementBlock("C3D8", conn)
+        b.setMaterial(Material("linearelastic", LE))
+        m.addElementBlock(b, "all")
+        m.nodeSets.update(sets)
+        acts = {"dirichlet": {"bottom": Dirichlet("bottom", m, "gen_back", {0: 0.0, 1: 0.0, 2: 0.0}),
+                              "top": Dirichlet("top", m, "gen_front", {2: 1e-3})}}
+        solver = NISTGPU(jobInfo, pcgTolerance=1e-13, linsolverRefinement=2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.solveStep(AdaptiveStep(1, m, acts, maxInc=1.0, startInc=1.0, maxIter=25), m)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, m.U.cpu().numpy(), [len(h) for h in solver.history]
+
+    gpu_run()  # first-call costs (plan code paths, torch kernels) stay out of the timing
+    t_gpu, Ug, its = gpu_run()
+    step = dict(maxInc=1.0, minInc=1e-4, maxNumInc=1000, maxIter=25,
+                dirichlet=[(np.asarray(sets["gen_back"]), {0: 0.0, 1: 0.0, 2: 0.0}),
+                           (np.asarray(sets["gen_front"]), {2: 1e-3})])
+    t0 = time.perf_counter()
+    r = O.solve_step(dict(el_kind="C3D8", mat_kind="linearelastic", mp=LE, X=X, conn=conn), step, tol_flux=1e-10)
+    t_cpu = time.perf_counter() - t0
+    return {"workload": "C3D8 linear elastic 10^3 unit cube (1,331 nodes), bottom fixed, top uz = 1e-3, one "
+                        "increment, Newton to a 1e-10 flux tolerance",
+            "gpu_seconds": t_gpu, "cpu_seconds": t_cpu, "cpu_kind": "port (oracle Newton, one host thread)",
+            "iterations_per_increment": its,
+            "max_rel_displacement_difference_gpu_vs_oracle": float(np.abs(Ug - r["U"]).max() / np.abs(r["U"]).max())}
 
 
 def copy_calibration(dev, nbytes=1 << 30):
@@ -174,18 +254,17 @@ def _timed_evals(asm, U, dU, P, F, K, steps, warmup):
     return ev0.elapsed_time(ev1) / steps
 
 
-def workload_c3d20_tl(args):
-    """Finite-strain total-Lagrangian compressible Neo-Hooke (mu=80, K=200) on 40^3 C3D20 hexahedra (64,000
-    elements, 27 quadrature points, 60 dofs each), nodes jittered by U(-3%, 3%) of h (seed 0): one evaluation +
-    CSR assembly at the state of a 1.25 homogeneous stretch along z (lambda = 1.5 is reached in 20
-    increments; this is increment 10), FP64 and FP32 values."""
+def c3d20_tl_measure(n, steps, warmup, cpu=True):
+    """Finite-strain total-Lagrangian compressible Neo-Hooke (mu=80, K=200) on n^3 C3D20 hexahedra (27 quadrature
+    points, 60 dofs each), nodes jittered by U(-3%, 3%) of h (seed 0): one evaluation + CSR assembly at the state
+    of a 1.25 homogeneous stretch along z (lambda = 1.5 is reached in 20 increments; this is increment 10), FP64 and
+    FP32 values; the oracle on the first 2 x-layers as the CPU baseline."""
     import torch
 
     from edelweissfe_amd.assembly import ElementAssembly
     from edelweissfe_amd.dofmanager import DofManager
     from edelweissfe_amd.model import ElementBlock, FEModel, Material, boxGenArrays, jitterNodes
 
-    n = args.n if args.n != 128 else 40
     X, conn, sets, grid = boxGenArrays(n, n, n, elType="C3D20TL")
     X = jitterNodes(X, grid, (2 * n + 1,) * 3, np.full(3, 1.0 / n), 0.03, 0)
     m = FEModel()
@@ -202,30 +281,47 @@ def workload_c3d20_tl(args):
     U = torch.as_tensor(u.ravel(), device="cuda")
     P, F = dm.constructDofVector(), dm.constructDofVector()
     K64, K32 = asm.newValues("fp64"), asm.newValues("fp32")
-    ms64 = _timed_evals(asm, U, U, P, F, K64, args.steps, args.warmup)
-    ms32 = _timed_evals(asm, U, U, P, F, K32, args.steps, args.warmup)
+    ms64 = _timed_evals(asm, U, U, P, F, K64, steps, warmup)
+    ms32 = _timed_evals(asm, U, U, P, F, K32, steps, warmup)
     n_el = len(conn)
     B64 = 4 * 20 + 120.0 * m.nNodes / n_el + 8.0 * asm.nnz / n_el  # no quadrature-point state
     B32 = 4 * 20 + 120.0 * m.nNodes / n_el + 4.0 * asm.nnz / n_el
-    ach = B64 * n_el / (ms64 * 1e-3) / 1e9
-    # FLOP bound: the dense B^T C B count per element, 2 (6 6 n + n n 6) per point with n = 60 dofs, 27 points
-    # (the kernel itself executes the rank-structured form of the same tangent, about a quarter of that)
-    flop_el = 27 * 2 * (6 * 6 * 60 + 60 * 60 * 6)
-    tflops = flop_el * n_el / (ms64 * 1e-3) / 1e12
+    # executed FLOPs of the rank-structured tangent and the forces: per point and node pair (a <= b) ~27 FMAs per
+    # 3x3 block, 210 pairs, plus ~3 NN^2 operations for the gradients and forces; the dense B^T C B count
+    # 2 (6 6 n + n n 6) per point, n = 60, is what a dense formulation would execute
+    flop_dense = 27 * 2 * (6 * 6 * 60 + 60 * 60 * 6)
+    flop_exec = 27 * (210 * 27 * 2 + 20 * 20 * 3 * 2)
+    out = dict(n_el=n_el, dofs=dm.nDof, nnz=asm.nnz, ms64=ms64, ms32=ms32, B64=B64, B32=B32, flop_dense=flop_dense,
+               flop_exec=flop_exec, cpu=None)
+    if cpu:
+        out["cpu"] = cpu_baseline("C3D20TL", "neohookeanwa", [80.0, 200.0], X, conn, u.ravel(), u.ravel(),
+                                  2 * n * n, f"first 2 x-layers of the jittered {n}^3 C3D20TL mesh at the same state",
+                                  target_s=8.0)
+    return out
+
+
+def workload_c3d20_tl(args):
+    """The C3D20TL Neo-Hooke 40^3 evaluation + assembly as its own bench line (c3d20_tl_measure)."""
+    n = args.n if args.n != 128 else 40
+    r = c3d20_tl_measure(n, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
+    ms64, n_el = r["ms64"], r["n_el"]
+    ach = r["B64"] * n_el / (ms64 * 1e-3) / 1e9
+    tflops = r["flop_exec"] * n_el / (ms64 * 1e-3) / 1e12
     return {
         "metric": "C3D20-TL Neo-Hooke elem evals+assembly/s", "value": n_el / (ms64 * 1e-3), "unit": "elements/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"C3D20TL Neo-Hooke Wa {n}^3 jittered cube, eval + CSR assembly", "elements": n_el,
-                   "dofs": dm.nDof, "nnz": asm.nnz},
-        "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None, "algorithmic_flop_per_element": flop_el,
-                     "scope": "fill + colour launches of k_hex20; FP64 vector and matrix peaks are the same",
-                     "hbm_view": {"achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                                  "algorithmic_bytes_per_element": B64}},
-        "cpu_baseline": None,
-        "fp32_tangent": {"value": n_el / (ms32 * 1e-3), "ms_per_step": ms32,
-                         "algorithmic_bytes_per_element": B32},
+                   "dofs": r["dofs"], "nnz": r["nnz"]},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_element": r["B64"],
+                     "scope": "fill + colour launches of k_hex20",
+                     "flop_view": {"executed_flop_per_element": r["flop_exec"], "achieved": tflops,
+                                   "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
+                                   "dense_btcb_flop_per_element": r["flop_dense"]}},
+        "cpu_baseline": r["cpu"],
+        "fp32_tangent": {"value": n_el / (r["ms32"] * 1e-3), "ms_per_step": r["ms32"],
+                         "algorithmic_bytes_per_element": r["B32"]},
     }
 
 
@@ -299,7 +395,10 @@ def workload_adjoint(args):
                    "grad_norm": float(grad.norm().item())},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_element": B, "scope": "k_adjoint8"},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(
+            "C3D8TL", "neohookeanwa", [80.0, 200.0], X, conn, m.U.cpu().numpy(), m.U.cpu().numpy(), 4 * n * n,
+            f"forward evaluation + assembly of the first 4 x-layers of the {n}^3 C3D8TL mesh at the converged state "
+            "(mu = 80 for all elements)", target_s=8.0),
     }
 
 
@@ -312,13 +411,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-layers", type=int, default=4)
     ap.add_argument("--no-newton", action="store_true")
-    ap.add_argument("--newton-n", type=int, default=None, help="mesh of the Newton increment (default --n)")
+    ap.add_argument("--newton-n", type=int, default=None, help="mesh of the Newton schedule (default --n)")
     ap.add_argument("--linsolver", default="amg", choices=["amg", "pcg"],
-                    help="Newton increment's linear solver: multigrid- or node-block-Jacobi-preconditioned CG")
+                    help="Newton schedule's linear solver: multigrid- or node-block-Jacobi-preconditioned CG")
     ap.add_argument("--pcg-tol-fp32", type=float, default=1e-4,
-                    help="relative PCG tolerance of the FP32-tangent increment (NISTGPU's default)")
+                    help="relative PCG tolerance of the FP32-tangent schedule (NISTGPU's default)")
     ap.add_argument("--no-newton-controls", action="store_true",
-                    help="skip the control increments (FP32 tangent at 1e-6, FP64 at the FP32 tolerance)")
+                    help="skip the matrix-core FP32 tangent schedule")
+    ap.add_argument("--newton-increments", type=int, default=10, help="increments of the 1%% uniaxial schedule")
+    ap.add_argument("--no-legs", action="store_true", help="skip the 64^3, C3D20TL 40^3 and 10^3 Newton legs")
     ap.add_argument("--workload", default="c3d8-j2", choices=["c3d8-j2", "c3d20-nh-tl", "c3d8-nh-adjoint"])
     ap.add_argument("--chunks", type=int, default=0,
                     help="pipeline chunks of the 8-node evaluation (0 = the library's automatic choice)")
@@ -392,7 +493,6 @@ def main():
     n_el = len(conn)
     B64 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 8)
     B32 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 4)
-    nnz = asm.nnz
     ach64 = B64 * n_el / (ms64 * 1e-3) / 1e9
     ach32 = B32 * n_el / (ms32 * 1e-3) / 1e9
 
@@ -408,49 +508,93 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(X, conn, U_n, dU, args.cpu_layers, n * n)
-        print(f"cpu baseline {cpu['value']:.4g} el/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline("C3D8", "vonmises", J2, X, conn, U_n + dU, dU, args.cpu_layers * n * n,
+                           f"first {args.cpu_layers} x-layers of the same jittered {n}^3 J2 mesh and state",
+                           state_from=U_n)
+        print(f"cpu baseline {cpu['value']:.4g} el/s on {cpu['cores']} threads ({cpu['host']})", file=sys.stderr,
+              flush=True)
 
     nnz, ncolors, n_chunks = asm.nnz, asm.nColors, asm.pipelineChunks
+    del asm, K64, K32
+    torch.cuda.empty_cache()
+
+    # the other configurations' legs: the 64^3 J2 evaluation (cfg2), the 40^3 C3D20TL evaluation (cfg4) and the
+    # plain 10^3 configuration's full Newton (cfg1), each with its CPU baseline
+    legs = None
+    if not args.no_legs:
+        legs = {}
+        m2, b2, X2, conn2 = build_workload(64)
+        dm2 = DofManager(m2, device="cuda")
+        asm2 = ElementAssembly(b2, dm2, m2, device="cuda")
+        U2n, dU2 = uniaxial(X2, 5e-3), uniaxial(X2, 1e-3)
+        P2, F2 = dm2.constructDofVector(), dm2.constructDofVector()
+        K2 = asm2.newValues("fp64")
+        t2n = torch.as_tensor(U2n, device=dev)
+        asm2.evaluate(t2n, t2n, P2, F2, K2)
+        b2.acceptLastState()
+        ms2 = _timed_evals(asm2, torch.as_tensor(U2n + dU2, device=dev), torch.as_tensor(dU2, device=dev), P2, F2, K2,
+                           args.steps, args.warmup)
+        B2 = algorithmic_bytes_per_element(len(conn2), m2.nNodes, asm2.nnz, 7, 8, 8)
+        legs["cfg2_c3d8_j2_64cube"] = {
+            "elements": len(conn2), "ms_per_evaluation_fp64": ms2, "value": len(conn2) / (ms2 * 1e-3),
+            "unit": "elements/s", "roofline_frac": B2 * len(conn2) / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(
+                "C3D8", "vonmises", J2, X2, conn2, U2n + dU2, dU2, 4 * 64 * 64,
+                "first 4 x-layers of the jittered 64^3 J2 mesh and state", target_s=8.0, state_from=U2n)}
+        del asm2, K2
+        torch.cuda.empty_cache()
+        r4 = c3d20_tl_measure(40, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
+        legs["cfg4_c3d20tl_neohooke_40cube"] = {
+            "elements": r4["n_el"], "ms_per_evaluation_fp64": r4["ms64"], "ms_per_evaluation_fp32": r4["ms32"],
+            "value": r4["n_el"] / (r4["ms64"] * 1e-3), "unit": "elements/s",
+            "roofline_frac": r4["B64"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "executed_flop_frac": r4["flop_exec"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "cpu_baseline": r4["cpu"]}
+        torch.cuda.empty_cache()
+        legs["cfg1_plain_10cube_newton"] = cfg1_newton()
+        print(f"legs: {json.dumps({k: {kk: vv for kk, vv in v.items() if kk != 'cpu_baseline'} for k, v in legs.items()})}",
+              file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
     newton = None
     if not args.no_newton:
-        del asm, K64, K32
-        torch.cuda.empty_cache()
         nn_ = args.newton_n or n
-        res = {}
-        # untimed warm-up of both variants on a small mesh: first-call costs (library loading, torch kernels)
-        # stay out of the timed increments
-        for tan in ("fp64", "fp32"):
-            newton_increment(8, tan, linsolver=args.linsolver)
+        # untimed warm-up of every variant on a small mesh: first-call costs stay out of the timed schedules
+        for tan in ("fp64", "fp32", "fp32-mfma"):
+            newton_schedule(8, tan, 1e-4, increments=2, linsolver=args.linsolver)
         torch.cuda.empty_cache()
-        # headline: plain FP64 (PCG 1e-10, the reference's exact-solve semantics) vs FP32 tangent at its
-        # precision-matched PCG tolerance; controls separate the precision effect from the tolerance effect
-        runs = [("fp64", 1e-10, "fp64"), ("fp32", args.pcg_tol_fp32, "fp32")]
+        # FP64 at several PCG tolerances (the fastest one that meets the Newton tolerance is the baseline), the FP32
+        # tangent at its precision-matched tolerance, and the matrix-core FP32 tangent
+        runs = [("fp64", 1e-10), ("fp64", 1e-6), ("fp64", args.pcg_tol_fp32), ("fp32", args.pcg_tol_fp32)]
         if not args.no_newton_controls:
-            runs += [("fp32", 1e-6, "fp32_tol1e-6"), ("fp64", args.pcg_tol_fp32, "fp64_inexact")]
-        controls = {}
-        for tan, tol, key in runs:
-            kw = {"pcg_tol_fp32": tol} if tan == "fp32" else {"pcg_tol": tol}
-            r = newton_increment(nn_, tan, linsolver=args.linsolver, **kw)
-            print(f"newton {key} (pcg tol {r['pcg_tolerance']:g}): {r['seconds']:.3f}s, {r['newton_iterations']} "
-                  f"iterations, pcg {r['pcg_iterations']}, phases {r['phases_s']}", file=sys.stderr, flush=True)
-            if key in ("fp64", "fp32"):
-                res[tan] = r
-            else:
-                Uc = r.pop("U")
-                controls[key] = {k: r[k] for k in ("pcg_tolerance", "seconds", "newton_iterations", "pcg_iterations")}
-                controls[key]["Uc"] = Uc
+            runs.append(("fp32-mfma", args.pcg_tol_fp32))
+        res = {}
+        for tan, tol in runs:
+            r = newton_schedule(nn_, tan, tol, increments=args.newton_increments, linsolver=args.linsolver)
+            print(f"newton {tan} (pcg tol {tol:g}): {r['seconds']:.3f}s, iterations per increment "
+                  f"{r['iterations_per_increment']}, {r['linear_solves']} solves, {r['pcg_iterations_total']} PCG "
+                  f"iterations, phases {r['phases_s']}", file=sys.stderr, flush=True)
+            res[f"{tan}_pcg{tol:g}"] = r
             torch.cuda.empty_cache()
-        U64, U32 = res["fp64"].pop("U"), res["fp32"].pop("U")
-        for c in controls.values():
-            c["max_rel_displacement_difference_vs_fp64"] = float(np.abs(c.pop("Uc") - U64).max() / np.abs(U64).max())
-            c["speedup_vs_fp64"] = res["fp64"]["seconds"] / c["seconds"]
-        newton = {"mesh": f"{nn_}^3", "load": "one increment from the virgin state to 0.2% uniaxial strain",
-                  "fp64": res["fp64"], "fp32_tangent": res["fp32"],
-                  "increment_speedup_fp32_vs_fp64": res["fp64"]["seconds"] / res["fp32"]["seconds"],
-                  "controls": controls,
-                  "max_rel_displacement_difference_fp32_vs_fp64":
-                      float(np.abs(U32 - U64).max() / np.abs(U64).max())}
+        ref64 = res["fp64_pcg1e-10"]
+        U64 = ref64["U"]
+        for r in res.values():
+            r["max_rel_displacement_difference_vs_fp64_pcg1e-10"] = float(np.abs(r["U"] - U64).max() /
+                                                                          np.abs(U64).max())
+        for r in res.values():
+            r.pop("U")
+        fp64_keys = [k for k in res if k.startswith("fp64")]
+        best64 = min(fp64_keys, key=lambda k: res[k]["seconds"])
+        fp32_key = f"fp32_pcg{args.pcg_tol_fp32:g}"
+        newton = {"mesh": f"{nn_}^3", "load": f"virgin state to 1% uniaxial strain in {args.newton_increments} "
+                                              "increments (after the zero-size first increment), linear extrapolation",
+                  "runs": res, "fastest_fp64": best64,
+                  "speedup_fp32_vs_fastest_fp64": res[best64]["seconds"] / res[fp32_key]["seconds"],
+                  "speedup_fp32_vs_fp64_pcg1e-10": ref64["seconds"] / res[fp32_key]["seconds"],
+                  "newton_step_ms": {k: v["ms_per_newton_iteration"] for k, v in res.items()}}
+        if not args.no_newton_controls:
+            newton["speedup_fp32mfma_vs_fastest_fp64"] = (res[best64]["seconds"] /
+                                                          res[f"fp32-mfma_pcg{args.pcg_tol_fp32:g}"]["seconds"])
 
     line = {
         "metric": "C3D8 J2 elem evals+assembly/s",
@@ -484,7 +628,8 @@ def main():
                               "roofline_frac": B32 * n_el / (ms32m * 1e-3) / 1e9 / HBM_PEAK_GBS,
                               "tangent": "FP32 on the matrix cores (v_mfma_f32_16x16x4_f32)"},
         "setup_s": t_setup,
-        "newton_increment": newton,
+        "legs": legs,
+        "newton": newton,
     }
     print(json.dumps(line), flush=True)
 
