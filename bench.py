@@ -281,8 +281,15 @@ def c3d20_tl_measure(n, steps, warmup, cpu=True):
     U = torch.as_tensor(u.ravel(), device="cuda")
     P, F = dm.constructDofVector(), dm.constructDofVector()
     K64, K32 = asm.newValues("fp64"), asm.newValues("fp32")
+    from edelweissfe_amd._lib import check, lib
+
     ms64 = _timed_evals(asm, U, U, P, F, K64, steps, warmup)
     ms32 = _timed_evals(asm, U, U, P, F, K32, steps, warmup)
+    # the colour-ordered read-modify-write path of the same plan, for comparison
+    check(lib().ewfe_plan_set_row_gather(asm._plan, 0))
+    ms64c = _timed_evals(asm, U, U, P, F, K64, steps, warmup)
+    ms32c = _timed_evals(asm, U, U, P, F, K32, steps, warmup)
+    check(lib().ewfe_plan_set_row_gather(asm._plan, 1))
     n_el = len(conn)
     B64 = 4 * 20 + 120.0 * m.nNodes / n_el + 8.0 * asm.nnz / n_el  # no quadrature-point state
     B32 = 4 * 20 + 120.0 * m.nNodes / n_el + 4.0 * asm.nnz / n_el
@@ -291,7 +298,8 @@ def c3d20_tl_measure(n, steps, warmup, cpu=True):
     # 2 (6 6 n + n n 6) per point, n = 60, is what a dense formulation would execute
     flop_dense = 27 * 2 * (6 * 6 * 60 + 60 * 60 * 6)
     flop_exec = 27 * (210 * 27 * 2 + 20 * 20 * 3 * 2)
-    out = dict(n_el=n_el, dofs=dm.nDof, nnz=asm.nnz, ms64=ms64, ms32=ms32, B64=B64, B32=B32, flop_dense=flop_dense,
+    out = dict(n_el=n_el, dofs=dm.nDof, nnz=asm.nnz, ms64=ms64, ms32=ms32, ms64_colour=ms64c, ms32_colour=ms32c,
+               B64=B64, B32=B32, flop_dense=flop_dense,
                flop_exec=flop_exec, cpu=None)
     if cpu:
         out["cpu"] = cpu_baseline("C3D20TL", "neohookeanwa", [80.0, 200.0], X, conn, u.ravel(), u.ravel(),
@@ -315,13 +323,14 @@ def workload_c3d20_tl(args):
                    "dofs": r["dofs"], "nnz": r["nnz"]},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_element": r["B64"],
-                     "scope": "fill + colour launches of k_hex20",
+                     "scope": "element pass (records) + row-node gather, k_hex20 + k_gather20",
                      "flop_view": {"executed_flop_per_element": r["flop_exec"], "achieved": tflops,
                                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
                                    "dense_btcb_flop_per_element": r["flop_dense"]}},
         "cpu_baseline": r["cpu"],
         "fp32_tangent": {"value": n_el / (r["ms32"] * 1e-3), "ms_per_step": r["ms32"],
                          "algorithmic_bytes_per_element": r["B32"]},
+        "colour_path": {"ms_per_step_fp64": r["ms64_colour"], "ms_per_step_fp32": r["ms32_colour"]},
     }
 
 
@@ -436,6 +445,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.workload != "c3d8-j2":
+        copy_calibration(torch.device("cuda:0"))
         fn = workload_c3d20_tl if args.workload == "c3d20-nh-tl" else workload_adjoint
         print(json.dumps(fn(args)), flush=True)
         return

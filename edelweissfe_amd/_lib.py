@@ -29,6 +29,7 @@ SIGNATURES = {
     "ewfe_plan_create_chunked": (ctypes.c_int, [_P(_vp), ctypes.c_int, _i64, _vp, _i64, _vp, _i64, _P(_i64),
                                                 ctypes.c_int]),
     "ewfe_plan_pipeline_info": (ctypes.c_int, [_vp, _P(_i32), _P(_i32)]),
+    "ewfe_plan_set_row_gather": (ctypes.c_int, [_vp, _i32]),
     "ewfe_plan_create_host": (ctypes.c_int, [_P(_vp), ctypes.c_int, _i64, _vp, _i64, _vp, _i64, _P(_i64),
                                              ctypes.c_int]),
     "ewfe_plan_pipeline_orders": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),

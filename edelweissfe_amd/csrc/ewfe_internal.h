@@ -63,6 +63,10 @@ struct ewfe_plan {
     // gathered after them; with more than one chunk on a second stream, beside the next chunk's element pass.
     int n_chunks = 0;  // requested at creation (0 = automatic), the number used after it
     bool structured = false;  // boxGen-ordered structured hexahedral mesh (tile-streaming orders)
+    // 20-node elements: records + row-node gather available (every node in at most 8 elements, at most 128
+    // neighbours), and the path in use (false: colour-ordered read-modify-write of the CSR values)
+    bool gather20 = false;
+    bool use_gather20 = false;
     int64_t grid_el[3] = {0, 0, 0};  // elements per axis (slowest first) when structured
     std::vector<int64_t> pipe_el, pipe_row;
     std::vector<int32_t> rorder;  // host copy of the row order (row node index per gather position)

@@ -529,6 +529,13 @@ __host__ __device__ constexpr int pair8(int lo, int hi) { return lo * 8 - lo * (
 __host__ __device__ constexpr int block_off8(int lo, int hi) { return 9 * pair8(lo, hi) - 3 * (lo + (hi > lo ? 1 : 0)); }
 constexpr int kSym6[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};  // (i, k) -> upper-triangle slot
 
+// Records of 20-node elements, the same layout: the 210 unique blocks K_ab (a <= b) in pair order
+// pair20(a, b) = a 20 - a (a + 1) / 2 + b, 190 off-diagonal blocks of 9 values and 20 diagonal upper triangles of
+// 6: 1830 values, padded to 1832 so that FP32 records are whole 16-byte units too.
+constexpr int kRec20 = 190 * 9 + 20 * 6 + 2;  // 1832
+__host__ __device__ constexpr int pair20(int lo, int hi) { return lo * 20 - lo * (lo + 1) / 2 + hi; }
+__host__ __device__ constexpr int block_off20(int lo, int hi) { return 9 * pair20(lo, hi) - 3 * (lo + (hi > lo ? 1 : 0)); }
+
 // Reads block K_ab of a record oriented for row node a (transposed when the record holds K_ba). Every lane
 // loads 9 values, whatever the block kind, so the loads of a wave do not diverge; for a diagonal block the
 // last 3 belong to the next block (or to the padding after the last record) and are not used.
@@ -542,6 +549,23 @@ __device__ __forceinline__ void load_block8(const VT *rec, int a, int b, double 
 #pragma unroll
     for (int t = 0; t < 9; ++t) u[t] = (double)src[t];
     const bool diag = a == b, tr = a > b;  // tr: the stored block is K_ba, its transpose is K_ab
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[3 * i + k] = diag ? u[kSym6[i][k]] : (tr ? u[3 * k + i] : u[3 * i + k]);
+}
+
+// block K_ab of a 20-node record oriented for row node a (load_block8's convention; the last 3 values loaded for a
+// diagonal block belong to the next block or to the padding and are not used)
+template <typename VT>
+__device__ __forceinline__ void load_block20(const VT *rec, int a, int b, double v[9])
+{
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const VT *src = rec + block_off20(lo, hi);
+    double u[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) u[t] = (double)src[t];
+    const bool diag = a == b, tr = a > b;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -1591,8 +1615,10 @@ __global__ __launch_bounds__(64) void k_adjoint8(EvalArgs A, const double *__res
 // 17 KB of LDS, which keeps 8 workgroups per CU (the register limit) instead of 5; the sums over the points
 // keep their order.
 // ------------------------------------------------------------------------------------------------
-template <int FORM, int MAT, typename VT, int NQ>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_hex20(EvalArgs A)
+// REC: write the element's record (unique blocks + nodal forces) for the row-node gather instead of adding into
+// the CSR values (then the launch needs no colouring)
+template <int FORM, int MAT, typename VT, int NQ, bool REC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_hex20(EvalArgs A, RecordArgs R)
 {
     constexpr int NN = 20;
     constexpr int QC = NQ > 16 ? 14 : NQ;  // points per chunk
@@ -1707,6 +1733,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         }
     }
     if (!row_lane) return;
+    if (REC) {
+        // record of element el: row i of each block K_ab this lane built, stored as the record holds it (K_ab
+        // row-major for a < b, K_ba = K_ab^T otherwise, the upper triangle of K_aa); the nodal force f_(a,i)
+        const int64_t el = A.order[pos];
+        VT *rec = reinterpret_cast<VT *>(R.kbuf) + el * kRec20;
+        R.fe[el * 60 + 3 * a + i] = fi;
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) {
+            if (j < nb) {
+                const int b = (a + j) % NN;
+                if (j == 0) {
+                    for (int k = i; k < 3; ++k) rec[block_off20(a, a) + kSym6[i][k]] = (VT)K[0][k];
+                } else if (a < b) {
+                    for (int k = 0; k < 3; ++k) rec[block_off20(a, b) + 3 * i + k] = (VT)K[j][k];
+                } else {
+                    for (int k = 0; k < 3; ++k) rec[block_off20(b, a) + 3 * k + i] = (VT)K[j][k];
+                }
+            }
+        }
+        return;
+    }
     const int32_t row = s_dof[a] + i;
     A.P[row] -= fi;
     A.F[row] += fabs(fi);
@@ -1723,6 +1770,88 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)s_rs[3 * b + k] + cba + i, K[j][k]);
             }
         }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// C3D20 family, pass 2: one row node per 64-lane workgroup. The node's incident elements (at most 8, ascending
+// element order) and per neighbour rank m the (slot, element node) pairs contributing to block m come from the
+// descriptor and blob built with the plan. Lane t first fetches blocks t, t + 64, t + 128 of the 20 nk blocks
+// (slot, b) into LDS (oriented for the row node), then lane m sums neighbour block m (and m + 64) over its
+// contributions in ascending element order and writes the 3 x 3 values into the node's 3 CSR rows; lanes 0..2 sum
+// P and F. Every CSR value, P and F entry is written once: no colouring, no read-modify-write.
+// ------------------------------------------------------------------------------------------------
+constexpr int kMaxInc20 = 8;
+constexpr int kMaxDeg20 = 128;
+constexpr int kMaxBlobWords20 = kMaxInc20 + (kMaxDeg20 + 1 + 20 * kMaxInc20 + 3) / 4;
+
+template <typename VT>
+__global__ __launch_bounds__(64) void k_gather20(const int4 *__restrict__ gdesc, const int32_t *__restrict__ gblob,
+                                                 const VT *__restrict__ kbuf, const double *__restrict__ fe,
+                                                 VT *__restrict__ vals, double *__restrict__ P,
+                                                 double *__restrict__ F, int64_t r_begin, int64_t count,
+                                                 int64_t xcd_chunk)
+{
+    __shared__ __align__(16) double s_blk[20 * kMaxInc20 * kBlk];
+    __shared__ double s_fe[kMaxInc20 * 3];
+    __shared__ int32_t s_blob[kMaxBlobWords20 + 1];
+    // workgroups are dealt round-robin to the 8 XCDs: XCD x takes the row nodes [x * xcd_chunk, (x + 1) *
+    // xcd_chunk) of the range, so neighbouring row nodes (which read the same element records) share one L2
+    const int64_t idx = blockIdx.x >> 3, xr = (int64_t)(blockIdx.x & 7) * xcd_chunk + idx;
+    if (idx >= xcd_chunk || xr >= count) return;  // uniform over the workgroup
+    const int64_t r = r_begin + xr;
+    const int lane = threadIdx.x;
+    const int4 ds = gdesc[r];
+    const int32_t d = ds.x, base = ds.y, boff = ds.z;
+    const int deg = ds.w & 255, nk = ds.w >> 8;
+    const int nwords = nk + (deg + 1 + 20 * nk + 3) / 4;
+    for (int w = lane; w < nwords; w += 64) s_blob[w] = gblob[boff + w];
+    __syncthreads();
+    const int32_t *codes = s_blob;
+    const uint8_t *cs = reinterpret_cast<const uint8_t *>(s_blob + nk);
+    const uint8_t *ent = cs + deg + 1;
+    for (int t = lane; t < 20 * nk; t += 64) {
+        const int sl = t / 20, b = t % 20;
+        const int64_t e = codes[sl] >> 5;
+        const int a = codes[sl] & 31;
+        double v[9];
+        load_block20(kbuf + e * kRec20, a, b, v);
+        double *dst = s_blk + t * kBlk;
+#pragma unroll
+        for (int u = 0; u < 9; ++u) dst[u] = v[u];
+        if (b < 3) s_fe[sl * 3 + b] = fe[e * 60 + 3 * a + b];
+    }
+    __syncthreads();
+    const int L = 3 * deg;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int m = lane + 64 * u;
+        if (m < deg) {
+            double acc[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+            for (int c = cs[m]; c < cs[m + 1]; ++c) {
+                const int code = ent[c];
+                const double *src = s_blk + ((code >> 5) * 20 + (code & 31)) * kBlk;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) acc[t] += src[t];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    __builtin_nontemporal_store((VT)acc[3 * i + k], vals + (int64_t)base + i * L + 3 * m + k);
+        }
+    }
+    if (lane < 3) {  // P and F of component lane, in ascending element order
+        double f = 0.0, fa = 0.0;
+        for (int sl = 0; sl < nk; ++sl) {
+            const double x = s_fe[sl * 3 + lane];
+            f -= x;
+            fa += fabs(x);
+        }
+        P[d + lane] = f;
+        F[d + lane] = fa;
     }
 }
 
@@ -1871,7 +2000,7 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
                 return fail(e, "hipMemcpy");
         }
     }
-    if (p->npe == 8) {
+    if (p->npe == 8 || p->gather20) {
         Up gups[] = {
             {&p->d_gdesc, gh.gdesc.data(), sizeof(int32_t) * gh.gdesc.size()},
             {&p->d_gblob, gh.gblob.data(), sizeof(int32_t) * gh.gblob.size()},
@@ -1888,8 +2017,29 @@ static int plan_create(ewfe_plan **out, int elem_kind, int64_t n_el, const int32
         ewfe_plan_destroy(p);
         return rc;
     }
+    p->use_gather20 = p->gather20;
     if (nnz_out) *nnz_out = p->nnz;
     *out = p;
+    return 0;
+}
+
+extern "C" int ewfe_plan_set_row_gather(ewfe_plan *p, int32_t use_gather)
+{
+    if (!p) {
+        ewfe::set_error("ewfe_plan_set_row_gather: null plan");
+        return EWFE_EINVAL;
+    }
+    if (p->npe == 8) {
+        if (use_gather) return 0;
+        ewfe::set_error("ewfe_plan_set_row_gather: 8-node plans always assemble through the row-node gather");
+        return EWFE_EINVAL;
+    }
+    if (use_gather && (!p->gather20 || !p->d_kbuf)) {
+        ewfe::set_error("ewfe_plan_set_row_gather: the plan has no row-node gather (a node in more than 8 elements "
+                        "or coupled with more than 128 nodes)");
+        return EWFE_EINVAL;
+    }
+    p->use_gather20 = use_gather != 0;
     return 0;
 }
 
@@ -2111,9 +2261,14 @@ extern "C" int ewfe_plan_destroy(ewfe_plan *p)
 // for FP64 values and shared by both value precisions (FP32 records use the first half of each slot range).
 static int alloc_records(ewfe_plan *p)
 {
-    if (p->npe != 8 || p->d_kbuf) return 0;
-    HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(double) * ((size_t)kRec8 * p->n_el + kRecPad8)));
-    HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 24 * (size_t)p->n_el));
+    if (p->d_kbuf) return 0;
+    if (p->npe == 8) {
+        HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(double) * ((size_t)kRec8 * p->n_el + kRecPad8)));
+        HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 24 * (size_t)p->n_el));
+    } else if (p->gather20) {
+        HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(double) * ((size_t)kRec20 * p->n_el)));
+        HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 60 * (size_t)p->n_el));
+    }
     return 0;
 }
 
@@ -2197,22 +2352,60 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
     return 0;
 }
 
+// 20-node elements: with the row-node gather, one element pass over all positions writes the records and the
+// gather sums them into every CSR value, P and F entry once; otherwise one launch per colour adds into the CSR
+// values (no two elements of a colour share a node).
+template <int FORM, int MAT, typename VT>
+static int launch_hex20(ewfe_plan *p, EvalArgs A, hipStream_t s)
+{
+    constexpr int64_t kMaxGrid = (int64_t)1 << 24;  // workgroups per launch (64 lanes each)
+    if (p->use_gather20) {
+        if (!p->d_kbuf || !p->d_fe || !p->d_gdesc) {
+            ewfe::set_error("ewfe_eval_assemble: the plan has no element records");
+            return EWFE_EINVAL;
+        }
+        const RecordArgs R{p->d_kbuf, p->d_fe, 0, p->n_el, 0};
+        for (int64_t b = 0; b < p->n_el; b += kMaxGrid) {
+            A.p_begin = b;
+            A.p_end = std::min<int64_t>(p->n_el, b + kMaxGrid);
+            const dim3 grid((unsigned)(A.p_end - A.p_begin));
+            if (p->nqp == 8)
+                hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 8, true>), grid, dim3(64), 0, s, A, R);
+            else
+                hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 27, true>), grid, dim3(64), 0, s, A, R);
+            HIP_CHECK(hipGetLastError());
+        }
+        for (int64_t b = 0; b < p->n_rownodes; b += kMaxGrid) {
+            const int64_t count = std::min<int64_t>(p->n_rownodes - b, kMaxGrid);
+            const int64_t chunk = (count + 7) / 8;
+            hipLaunchKernelGGL((k_gather20<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, s,
+                               reinterpret_cast<const int4 *>(p->d_gdesc), p->d_gblob, (const VT *)p->d_kbuf,
+                               (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, b, count, chunk);
+            HIP_CHECK(hipGetLastError());
+        }
+        return 0;
+    }
+    const RecordArgs R{nullptr, nullptr, 0, 0, 0};
+    for (int c = 0; c < p->n_colors; ++c) {
+        for (int64_t b = p->color_begin[c]; b < p->color_begin[c + 1]; b += kMaxGrid) {
+            A.p_begin = b;
+            A.p_end = std::min<int64_t>(p->color_begin[c + 1], b + kMaxGrid);
+            const dim3 grid((unsigned)(A.p_end - A.p_begin));
+            if (p->nqp == 8)
+                hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 8, false>), grid, dim3(64), 0, s, A, R);
+            else
+                hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 27, false>), grid, dim3(64), 0, s, A, R);
+            HIP_CHECK(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
 template <int FORM, int MAT, typename VT>
 static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
 {
     if (p->npe == 8) return launch_hex8<FORM, MAT, VT>(p, A, s, mfma);
-    for (int c = 0; c < p->n_colors; ++c) {
-        A.p_begin = p->color_begin[c];
-        A.p_end = p->color_begin[c + 1];
-        const int64_t cnt = A.p_end - A.p_begin;
-        if (cnt <= 0) continue;
-        if (p->nqp == 8)
-            hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 8>), dim3((unsigned)cnt), dim3(64), 0, s, A);
-        else
-            hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 27>), dim3((unsigned)cnt), dim3(64), 0, s, A);
-        HIP_CHECK(hipGetLastError());
-    }
-    return 0;
+    return launch_hex20<FORM, MAT, VT>(p, A, s);
 }
 
 template <typename VT>
@@ -2259,8 +2452,8 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     int rc;
-    // the 8-node gather writes P, F and every CSR value of the row nodes once; other dofs must read zero
-    const bool gather = p->npe == 8;
+    // the row-node gather writes P, F and every CSR value of the row nodes once; other dofs must read zero
+    const bool gather = p->npe == 8 || p->use_gather20;
     if (!gather || 3 * p->n_rownodes != p->n_dof) {
         if ((rc = fill<double>(P_dev, p->n_dof, 0.0, s))) return rc;
         if ((rc = fill<double>(F_dev, p->n_dof, 0.0, s))) return rc;

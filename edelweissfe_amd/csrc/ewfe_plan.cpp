@@ -431,6 +431,71 @@ int build_plan_host(ewfe_plan &plan, const int32_t *conn, const int64_t *node_do
         }
     });
 
+    // --- row-node gather lists (20-node elements): every row node's incident elements (ascending element order,
+    //     code 32 element + local node) and, per neighbour rank m, the (slot, b) pairs of the elements whose node b is
+    //     that neighbour (bytes: slot << 5 | b). Taken when every node belongs to at most 8 elements and couples with
+    //     at most 128 nodes (hexahedral meshes of the generators); otherwise the plan keeps the colour path only.
+    plan.gather20 = false;
+    if (npe == 20) {
+        bool ok = plan.max_deg <= 128 && n_el < ((int64_t)1 << 26);
+        for (int64_t r = 0; ok && r < nr; ++r)
+            if (inc_ptr[rownodes[r] + 1] - inc_ptr[rownodes[r]] > 8) ok = false;
+        if (ok) {
+            std::vector<int64_t> woff(nr + 1, 0);
+            for (int64_t r = 0; r < nr; ++r) {
+                const int64_t nk = inc_ptr[rownodes[r] + 1] - inc_ptr[rownodes[r]];
+                const int64_t dg = gather.nb_ptr[r + 1] - gather.nb_ptr[r];
+                woff[r + 1] = woff[r] + nk + (dg + 1 + 20 * nk + 3) / 4;
+            }
+            if (woff[nr] >= (int64_t)INT32_MAX) {
+                set_error("gather lists beyond int32 indexing");
+                return -4;
+            }
+            gather.gdesc.resize((size_t)4 * nr);
+            gather.gblob.assign(woff[nr], 0);
+            plan.rorder.resize(nr);
+            parallel_for(nr, [&](int64_t b0, int64_t e0) {
+                std::vector<int> cnt;
+                std::vector<uint8_t> rank;
+                for (int64_t r = b0; r < e0; ++r) {
+                    plan.rorder[r] = (int32_t)r;
+                    const int32_t n = rownodes[r];
+                    const int32_t nk = (int32_t)(inc_ptr[n + 1] - inc_ptr[n]);
+                    const int32_t dg = gather.nb_ptr[r + 1] - gather.nb_ptr[r];
+                    const int32_t *lo = nb_dof.data() + nb_ptr[n], *hi = nb_dof.data() + nb_ptr[n + 1];
+                    int32_t *g = gather.gdesc.data() + 4 * r;
+                    g[0] = gather.rownode_dof[r];
+                    g[1] = plan.indptr[gather.rownode_dof[r]];
+                    g[2] = (int32_t)woff[r];
+                    g[3] = dg | (nk << 8);
+                    int32_t *w = gather.gblob.data() + woff[r];
+                    uint8_t *cs = reinterpret_cast<uint8_t *>(w + nk), *ent = cs + dg + 1;
+                    rank.assign((size_t)20 * nk, 0);
+                    cnt.assign(dg + 1, 0);
+                    for (int32_t sl = 0; sl < nk; ++sl) {
+                        const int64_t el = inc[inc_ptr[n] + sl];  // ascending element order
+                        int a = 0;
+                        while (a < 20 && conn[el * 20 + a] != n) ++a;
+                        w[sl] = (int32_t)(el * 32 + a);
+                        for (int bb = 0; bb < 20; ++bb) {
+                            const int32_t db = (int32_t)node_dof[conn[el * 20 + bb]];
+                            rank[(size_t)20 * sl + bb] = (uint8_t)(std::lower_bound(lo, hi, db) - lo);
+                            cnt[rank[(size_t)20 * sl + bb] + 1]++;
+                        }
+                    }
+                    for (int32_t m = 0; m < dg; ++m) cnt[m + 1] += cnt[m];
+                    for (int32_t m = 0; m <= dg; ++m) cs[m] = (uint8_t)cnt[m];
+                    for (int32_t sl = 0; sl < nk; ++sl)
+                        for (int bb = 0; bb < 20; ++bb) {
+                            const int m = rank[(size_t)20 * sl + bb];
+                            ent[cnt[m]++] = (uint8_t)((sl << 5) | bb);
+                        }
+                }
+            });
+            plan.gather20 = true;
+        }
+    }
+
     // --- row-node gather lists (8-node elements) ---------------------------------------------------
     if (npe == 8) {
         if (plan.max_deg > 64) {

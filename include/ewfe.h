@@ -85,6 +85,13 @@ int ewfe_plan_create_chunked(ewfe_plan **plan, int elem_kind, int64_t n_el, cons
                              int64_t n_nodes, const int64_t *node_dof_host, int64_t n_dof, int64_t *nnz_out,
                              int pipeline_chunks);
 
+/* Assembly path of a 20-node plan (no counterpart in the reference; tools and tests): use_gather 1 = element records
+ * summed by the row-node gather (the default where the plan has it: every node in at most 8 elements and coupled
+ * with at most 128 nodes), 0 = colour-ordered additions into the CSR values. Both give the same sums up to the
+ * order of the floating-point additions. EWFE_EINVAL for 1 on a plan without the gather lists; 8-node plans always
+ * gather and take 1 only. */
+int ewfe_plan_set_row_gather(ewfe_plan *plan, int32_t use_gather);
+
 /* Pipeline chunks in use (0 for 20-node elements) and whether the mesh was recognised as a boxGen
  * structured hexahedral grid (boxgen.py:125-185), which selects the tile-streaming work orders. */
 int ewfe_plan_pipeline_info(const ewfe_plan *plan, int32_t *n_chunks, int32_t *structured);

@@ -65,6 +65,39 @@ def test_eval_assemble_matches_oracle(el, mat, mp, n):
     assert np.array_equal(out2["data"], out["data"]) and np.array_equal(out2["P"], out["P"])
 
 
+@pytest.mark.parametrize("dtype", ["fp64", "fp32"])
+@pytest.mark.parametrize("el,mat,mp", [("C3D20", "vonmises", J2), ("C3D20R", "linearelastic", LE),
+                                       ("C3D20TL", "neohookeanwa", NH), ("C3D20RTL", "neohookeanwa", NH)])
+def test_c3d20_gather_and_colour_paths(el, mat, mp, dtype):
+    """20-node assembly through element records summed by the row-node gather (the plan's default) and through
+    colour-ordered additions into the CSR values: both against the oracle (1e-12 FP64, 2e-6 FP32 values), the
+    gather bitwise deterministic, the state bitwise the same on both paths."""
+    from edelweissfe_amd._lib import lib
+
+    print(f"case {el} {mat} {dtype}: gather and colour paths", flush=True)
+    X, conn, _ = jittered_box((3, 2, 4), el, seed=5)
+    rng = np.random.default_rng(6)
+    amp = 0.08 if "TL" in el else 2e-3
+    U = _uniaxial_field(X, amp, rng, amp / 10)
+    dU = U.copy() if "TL" not in el else 0.5 * U
+    g = GpuEval(el, mat, mp, X, conn)
+    ref = oracle_eval(el, mat, mp, X, conn, U, dU)
+    bar = 1e-12 if dtype == "fp64" else 2e-6
+    out = {}
+    for path in (1, 0):
+        assert lib().ewfe_plan_set_row_gather(g.asm._plan, path) == 0
+        out[path] = g(U, dU, dtype)
+        errs = [rel(out[path][k], ref[k]) for k in ("data", "P", "F")]
+        print(f"  {'gather' if path else 'colour'}: CSR values {errs[0]:.2e}, P {errs[1]:.2e}, F {errs[2]:.2e}",
+              flush=True)
+        assert max(errs) < bar
+    assert np.array_equal(out[0]["state"], out[1]["state"])
+    assert rel(out[1]["data"], out[0]["data"]) < (1e-14 if dtype == "fp64" else 1e-6)
+    assert lib().ewfe_plan_set_row_gather(g.asm._plan, 1) == 0
+    again = g(U, dU, dtype)
+    assert np.array_equal(again["data"], out[1]["data"]) and np.array_equal(again["P"], out[1]["P"])
+
+
 def test_plain_method_10cube():
     """The CPU-runnable configuration at its own size: a 10^3 unit cube of C3D8 (1,331 nodes), linear elastic
     E = 210000, nu = 0.3, bottom face (z = 0) fixed, top face uz = +1e-3 in one increment. One evaluation + CSR
