@@ -1624,7 +1624,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     constexpr int QC = NQ > 16 ? 14 : NQ;  // points per chunk
     using L = QPLayout<NN, FORM>;
     __shared__ double s_nd[9 * NN];      // [field][node]
-    __shared__ double s_qp[L::NF * QC];  // [field][point of the chunk]
+    // [field][point of the chunk]; with REC afterwards the staged record (written out in whole 16-byte units)
+    constexpr int kQP = L::NF * QC, kStage = (REC ? (kRec20 * (int)sizeof(VT) + 7) / 8 : 0);
+    __shared__ __align__(16) double s_qp[kQP > kStage ? kQP : kStage];
     __shared__ int32_t s_dof[NN];
     __shared__ int32_t s_rs[3 * NN];     // CSR row starts of the element's 60 dof rows
     const int lane = threadIdx.x;
@@ -1644,16 +1646,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const MatParams M = mat_at(A, pos);
     double Epar = M.p[0];
     if (A.el_param) Epar = A.el_param[A.order[pos]];
-    // lane = (a, i): row i of node a, for the blocks K_ab with b = a + j (mod 20), j < nb: every unordered node
-    // pair once (j = 10 only from a < 10); the lane also writes the transposed entries K_ba = K_ab^T
+    // lane = (a, g): the force row (a, i = g) and all three rows of the blocks K_ab, b = a + j (mod 20), for the
+    // offsets j of group g: {0..3}, {4..7}, {8..10}, with j < nb (j = 10 only from a < 10: every unordered node pair
+    // once). The lanes of one node share its point data; per point a lane reads node a's and its 4 partner nodes'
+    // vectors from LDS once for 36 block entries.
     const bool row_lane = lane < 3 * NN;
-    const int a = row_lane ? lane / 3 : 0, i = lane % 3;
-    constexpr int kJ = NN / 2 + 1;  // 11
+    const int a = row_lane ? lane / 3 : 0, g = lane % 3, i = g;
+    constexpr int kJ = NN / 2 + 1;  // 11 offsets
+    constexpr int kG = 4;           // offsets per lane
+    const int j0 = kG * g;
     const int nb = (a < NN / 2) ? kJ : kJ - 1;
-    double K[kJ][3];
-    for (int j = 0; j < kJ; ++j)
-        for (int k = 0; k < 3; ++k) K[j][k] = 0.0;
+    const int nj = nb - j0 < kG ? nb - j0 : kG;  // 4, 4, 3 or 2
+    double K[kG][9];
+    for (int j = 0; j < kG; ++j)
+        for (int t = 0; t < 9; ++t) K[j][t] = 0.0;
     double fi = 0.0;
+#pragma unroll 1
     for (int q0 = 0; q0 < NQ; q0 += QC) {
         const int nq = NQ - q0 < QC ? NQ - q0 : QC;
         __syncthreads();  // the nodes are staged, or the previous chunk's points are summed
@@ -1679,10 +1687,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         __syncthreads();
         if (!row_lane) continue;
         auto ld = [&](int f, int q) { return s_qp[f * QC + q]; };
+#pragma unroll 1
         for (int q = 0; q < nq; ++q) {
             const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q),
                          cc = ld(L::COEF + 3, q);
-            double pa[3], ra[3], Nq[6], vai = 0.0;
+            double pa[3], ra[3], Nq[6], va[3] = {0.0, 0.0, 0.0};
             for (int m = 0; m < 3; ++m) {
                 ra[m] = ld(L::G + 3 * a + m, q);
                 pa[m] = ld(L::P + 3 * a + m, q);
@@ -1698,18 +1707,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             }
             if (MAT == M_J2) {
                 for (int m = 0; m < 6; ++m) Nq[m] = ld(L::NV + m, q);
-                const double Nr0 = (i == 0) ? Nq[0] : (i == 1 ? Nq[3] : Nq[4]);
-                const double Nr1 = (i == 0) ? Nq[3] : (i == 1 ? Nq[1] : Nq[5]);
-                const double Nr2 = (i == 0) ? Nq[4] : (i == 1 ? Nq[5] : Nq[2]);
-                vai = cc * (Nr0 * pa[0] + Nr1 * pa[1] + Nr2 * pa[2]);
+                va[0] = cc * (Nq[0] * pa[0] + Nq[3] * pa[1] + Nq[4] * pa[2]);
+                va[1] = cc * (Nq[3] * pa[0] + Nq[1] * pa[1] + Nq[5] * pa[2]);
+                va[2] = cc * (Nq[4] * pa[0] + Nq[5] * pa[1] + Nq[2] * pa[2]);
             }
-            const double Aai = al * pa[i];
-            double Ba[3];
-            for (int m = 0; m < 3; ++m) Ba[m] = be * pa[m];
+            double Aa[3], Ba[3];
+            for (int m = 0; m < 3; ++m) {
+                Aa[m] = al * pa[m];
+                Ba[m] = be * pa[m];
+            }
 #pragma unroll
-            for (int j = 0; j < kJ; ++j) {
-                if (j < nb) {
-                    const int b = (a + j) % NN;
+            for (int jj = 0; jj < kG; ++jj) {
+                if (jj < nj) {
+                    const int b = (a + j0 + jj) % NN;
                     double pb[3], rb[3];
                     for (int m = 0; m < 3; ++m) {
                         rb[m] = ld(L::G + 3 * b + m, q);
@@ -1722,52 +1732,79 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                         vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
                         vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
                     }
-                    for (int k = 0; k < 3; ++k) {
-                        double v = Aai * pb[k] + Ba[k] * pb[i];
-                        if (MAT == M_J2) v -= vai * vb[k];
-                        if (i == k) v += d;
-                        K[j][k] += v;
+#pragma unroll
+                    for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            double v = Aa[ii] * pb[k] + Ba[k] * pb[ii];
+                            if (MAT == M_J2) v -= va[ii] * vb[k];
+                            if (ii == k) v += d;
+                            K[jj][3 * ii + k] += v;
+                        }
+                }
+            }
+        }
+    }
+    if (REC) {
+        // record of element el staged in LDS (the point data are no longer read once every lane passed the
+        // barrier): the blocks as the record holds them (K_ab row-major for a < b, K_ba = K_ab^T otherwise, the
+        // upper triangle of K_aa), then written out in whole 16-byte units; the nodal forces f_(a,i), lane 3a + i
+        __syncthreads();
+        VT *s_rec = reinterpret_cast<VT *>(s_qp);
+        const int64_t el = A.order[pos];
+        if (row_lane) {
+            R.fe[el * 60 + lane] = fi;
+#pragma unroll
+            for (int jj = 0; jj < kG; ++jj) {
+                if (jj < nj) {
+                    const int j = j0 + jj, b = (a + j) % NN;
+                    if (j == 0) {
+#pragma unroll
+                        for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                            for (int k = ii; k < 3; ++k) s_rec[block_off20(a, a) + kSym6[ii][k]] = (VT)K[0][3 * ii + k];
+                    } else if (a < b) {
+#pragma unroll
+                        for (int t = 0; t < 9; ++t) s_rec[block_off20(a, b) + t] = (VT)K[jj][t];
+                    } else {
+#pragma unroll
+                        for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) s_rec[block_off20(b, a) + 3 * k + ii] = (VT)K[jj][3 * ii + k];
                     }
                 }
             }
         }
-    }
-    if (!row_lane) return;
-    if (REC) {
-        // record of element el: row i of each block K_ab this lane built, stored as the record holds it (K_ab
-        // row-major for a < b, K_ba = K_ab^T otherwise, the upper triangle of K_aa); the nodal force f_(a,i)
-        const int64_t el = A.order[pos];
-        VT *rec = reinterpret_cast<VT *>(R.kbuf) + el * kRec20;
-        R.fe[el * 60 + 3 * a + i] = fi;
-#pragma unroll
-        for (int j = 0; j < kJ; ++j) {
-            if (j < nb) {
-                const int b = (a + j) % NN;
-                if (j == 0) {
-                    for (int k = i; k < 3; ++k) rec[block_off20(a, a) + kSym6[i][k]] = (VT)K[0][k];
-                } else if (a < b) {
-                    for (int k = 0; k < 3; ++k) rec[block_off20(a, b) + 3 * i + k] = (VT)K[j][k];
-                } else {
-                    for (int k = 0; k < 3; ++k) rec[block_off20(b, a) + 3 * k + i] = (VT)K[j][k];
-                }
-            }
-        }
+        __syncthreads();
+        static_assert((kRec20 * (int)sizeof(VT)) % 16 == 0, "records are whole 16-byte units");
+        constexpr int NW = kRec20 * (int)sizeof(VT) / 16;
+        typedef float u4 __attribute__((ext_vector_type(4)));  // 16 bytes moved as they are
+        const u4 *src = reinterpret_cast<const u4 *>(s_rec);
+        u4 *dst = reinterpret_cast<u4 *>(reinterpret_cast<VT *>(R.kbuf) + el * kRec20);
+        for (int w = lane; w < NW; w += 64) __builtin_nontemporal_store(src[w], dst + w);
         return;
     }
+    if (!row_lane) return;
     const int32_t row = s_dof[a] + i;
     A.P[row] -= fi;
     A.F[row] += fabs(fi);
     VT *vals = reinterpret_cast<VT *>(A.vals);
-    const int64_t rs = s_rs[3 * a + i];
 #pragma unroll
-    for (int j = 0; j < kJ; ++j) {
-        if (j < nb) {
-            const int b = (a + j) % NN;
+    for (int jj = 0; jj < kG; ++jj) {
+        if (jj < nj) {
+            const int j = j0 + jj, b = (a + j) % NN;
             const int32_t cab = A.colpos[pos * (NN * NN) + a * NN + b];
-            for (int k = 0; k < 3; ++k) add_to(vals, rs + cab + k, K[j][k]);
-            if (j > 0) {  // K_ba[k][i] = K_ab[i][k] into row k of node b, column i of node a
+#pragma unroll
+            for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)s_rs[3 * a + ii] + cab + k, K[jj][3 * ii + k]);
+            if (j > 0) {  // K_ba[k][ii] = K_ab[ii][k] into row k of node b, column ii of node a
                 const int32_t cba = A.colpos[pos * (NN * NN) + b * NN + a];
-                for (int k = 0; k < 3; ++k) add_to(vals, (int64_t)s_rs[3 * b + k] + cba + i, K[j][k]);
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        add_to(vals, (int64_t)s_rs[3 * b + k] + cba + ii, K[jj][3 * ii + k]);
             }
         }
     }
