@@ -315,6 +315,14 @@ def workload_c3d20_tl(args):
     ms64, n_el = r["ms64"], r["n_el"]
     ach = r["B64"] * n_el / (ms64 * 1e-3) / 1e9
     tflops = r["flop_exec"] * n_el / (ms64 * 1e-3) / 1e12
+    # HBM bytes per evaluation from the committed rocprofv3 --pmc summary of this workload (copy-calibrated)
+    traffic, traffic_src = None, None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_c3d20tl_neohooke_40cube.json")
+    if n == 40 and os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+        traffic = tj["bytes_per_evaluation"]["fp64"]["total"]
+        traffic_src = tj["source"] + " (k_hex20 + k_gather20, bytes per evaluation)"
     return {
         "metric": "C3D20-TL Neo-Hooke elem evals+assembly/s", "value": n_el / (ms64 * 1e-3), "unit": "elements/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True,
@@ -322,7 +330,7 @@ def workload_c3d20_tl(args):
         "config": {"workload": f"C3D20TL Neo-Hooke Wa {n}^3 jittered cube, eval + CSR assembly", "elements": n_el,
                    "dofs": r["dofs"], "nnz": r["nnz"]},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_element": r["B64"],
+                     "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes_per_element": r["B64"],
                      "scope": "element pass (records) + row-node gather, k_hex20 + k_gather20",
                      "flop_view": {"executed_flop_per_element": r["flop_exec"], "achieved": tflops,
                                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
@@ -556,6 +564,7 @@ def main():
         r4 = c3d20_tl_measure(40, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
         legs["cfg4_c3d20tl_neohooke_40cube"] = {
             "elements": r4["n_el"], "ms_per_evaluation_fp64": r4["ms64"], "ms_per_evaluation_fp32": r4["ms32"],
+            "ms_per_evaluation_fp64_colour_path": r4["ms64_colour"],
             "value": r4["n_el"] / (r4["ms64"] * 1e-3), "unit": "elements/s",
             "roofline_frac": r4["B64"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "executed_flop_frac": r4["flop_exec"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
