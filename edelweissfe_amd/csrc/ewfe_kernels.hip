@@ -533,6 +533,9 @@ constexpr int kSym6[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};  // (i, k) -> upp
 // pair20(a, b) = a 20 - a (a + 1) / 2 + b, 190 off-diagonal blocks of 9 values and 20 diagonal upper triangles of
 // 6: 1830 values, padded to 1832 so that FP32 records are whole 16-byte units too.
 constexpr int kRec20 = 190 * 9 + 20 * 6 + 2;  // 1832
+// values after the last 20-node record of the buffer: the 9 values loaded for the last diagonal block (19, 19) run
+// one value past its record
+constexpr int kRecPad20 = 2;
 __host__ __device__ constexpr int pair20(int lo, int hi) { return lo * 20 - lo * (lo + 1) / 2 + hi; }
 __host__ __device__ constexpr int block_off20(int lo, int hi) { return 9 * pair20(lo, hi) - 3 * (lo + (hi > lo ? 1 : 0)); }
 
@@ -2303,7 +2306,7 @@ static int alloc_records(ewfe_plan *p)
         HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(double) * ((size_t)kRec8 * p->n_el + kRecPad8)));
         HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 24 * (size_t)p->n_el));
     } else if (p->gather20) {
-        HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(double) * ((size_t)kRec20 * p->n_el)));
+        HIP_CHECK(hipMalloc(&p->d_kbuf, sizeof(double) * ((size_t)kRec20 * p->n_el + kRecPad20)));
         HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * 60 * (size_t)p->n_el));
     }
     return 0;
