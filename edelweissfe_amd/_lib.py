@@ -10,13 +10,13 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libewfe.so")
 
 EWFE_C3D8, EWFE_C3D20, EWFE_C3D8TL, EWFE_C3D20TL, EWFE_C3D20R, EWFE_C3D20RTL = 0, 1, 2, 3, 4, 5
-EWFE_LINEAR_ELASTIC, EWFE_VON_MISES, EWFE_NEO_HOOKE_WA = 0, 1, 2
+EWFE_LINEAR_ELASTIC, EWFE_VON_MISES, EWFE_NEO_HOOKE_WA, EWFE_ST_VENANT_KIRCHHOFF = 0, 1, 2, 3
 EWFE_F64, EWFE_F32, EWFE_F32_MFMA = 0, 1, 2
 
 ELEMENT_KINDS = {"C3D8": EWFE_C3D8, "C3D20": EWFE_C3D20, "C3D8TL": EWFE_C3D8TL, "C3D20TL": EWFE_C3D20TL,
                  "C3D20R": EWFE_C3D20R, "C3D20RTL": EWFE_C3D20RTL}
 MATERIAL_KINDS = {"linearelastic": EWFE_LINEAR_ELASTIC, "vonmises": EWFE_VON_MISES,
-                  "neohookeanwa": EWFE_NEO_HOOKE_WA}
+                  "neohookeanwa": EWFE_NEO_HOOKE_WA, "stvenantkirchhoff": EWFE_ST_VENANT_KIRCHHOFF}
 
 # Every symbol include/ewfe.h declares, with (restype, argtypes).
 _vp = ctypes.c_void_p

@@ -30,7 +30,8 @@ from .model import ElementBlock, FEModel, Material, elementType
 _MATERIAL_NAMES = {"linearelastic": "linearelastic", "adlinearelastic": "linearelastic", "vonmises": "vonmises",
                    "neohookeanwa": "neohookeanwa", "neohookeanpencegouformulationa": "neohookeanwa",
                    "linearelasticmaterial": "linearelastic", "vonmisesmaterial": "vonmises",
-                   "neohookeanwamaterial": "neohookeanwa"}
+                   "neohookeanwamaterial": "neohookeanwa", "stvenantkirchhoffisotropic": "stvenantkirchhoff",
+                   "stvenantkirchhoff": "stvenantkirchhoff"}
 
 
 def _material(mat):

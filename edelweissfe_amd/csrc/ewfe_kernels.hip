@@ -236,7 +236,7 @@ struct MatParams {
 };
 
 enum { FORM_SS = 0, FORM_TL = 1 };
-enum { M_LE = 0, M_J2 = 1, M_NH = 2 };
+enum { M_LE = 0, M_J2 = 1, M_NH = 2, M_SVK = 3 };
 
 struct Recipe {
     double S[6];  // stress, tensor components [11,22,33,12,13,23]
@@ -382,6 +382,43 @@ __device__ __forceinline__ void nh_recipe(double mu, double Kb, const double F[3
     r.c = 0;
 }
 
+// St. Venant-Kirchhoff: S = lambda tr(E) I + 2 mu E (E Green strain), tau = F S F^T. With p = dN/dx, r = dN/dX
+// and q = B p (= F r) the total-Lagrangian tangent of element.py:405-427 (dtau/dF of this law) reduces to
+//   K_ab,ik = lambda q_a,i q_b,k + mu q_a,k q_b,i + mu (r_a . r_b) B_ik + delta_ik (p_a . tau p_b),
+// so the recipe carries alpha = lambda, beta = mu, B in the flow-direction slot and tau as the stress.
+__device__ __forceinline__ void svk_recipe(double E, double nu, const double F[3][3], Recipe &r)
+{
+    const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+    double Eg[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            Eg[i][j] = 0.5 * (F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j] - (i == j ? 1.0 : 0.0));
+    const double tr = Eg[0][0] + Eg[1][1] + Eg[2][2];
+    double S2[3][3], FS[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) S2[i][j] = 2 * mu * Eg[i][j] + (i == j ? lam * tr : 0.0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) FS[i][j] = F[i][0] * S2[0][j] + F[i][1] * S2[1][j] + F[i][2] * S2[2][j];
+    auto tau = [&](int i, int j) { return FS[i][0] * F[j][0] + FS[i][1] * F[j][1] + FS[i][2] * F[j][2]; };
+    auto bc = [&](int i, int j) { return F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2]; };
+    r.S[0] = tau(0, 0);
+    r.S[1] = tau(1, 1);
+    r.S[2] = tau(2, 2);
+    r.S[3] = tau(0, 1);
+    r.S[4] = tau(0, 2);
+    r.S[5] = tau(1, 2);
+    r.N[0] = bc(0, 0);
+    r.N[1] = bc(1, 1);
+    r.N[2] = bc(2, 2);
+    r.N[3] = bc(0, 1);
+    r.N[4] = bc(0, 2);
+    r.N[5] = bc(1, 2);
+    r.alpha = lam;
+    r.beta = mu;
+    r.gam = 0;
+    r.c = 0;
+}
+
 // ------------------------------------------------------------------------------------------------
 // kernel arguments
 // ------------------------------------------------------------------------------------------------
@@ -492,7 +529,10 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
         const double Jd = inv3(F, Fi);
-        nh_recipe(Epar, M.p[1], F, Jd, r);
+        if (MAT == M_SVK)
+            svk_recipe(Epar, M.p[1], F, r);
+        else
+            nh_recipe(Epar, M.p[1], F, Jd, r);
         // spatial gradients NA_a = F^-T g_a
         for (int a = 0; a < NN; ++a) {
             const double g0 = get_g(a, 0), g1 = get_g(a, 1), g2 = get_g(a, 2);
@@ -589,7 +629,8 @@ struct RecordArgs {
 // strain] [, c wd and the flow direction N for J2] and the stress S wd.
 template <int FORM, int MAT>
 struct Hex8Layout {
-    static constexpr bool TL = FORM == FORM_TL, HAS_N = MAT == M_J2;
+    // HAS_N: the per-point 6-vector slot NV (the J2 flow direction, or B = F F^T for St. Venant-Kirchhoff)
+    static constexpr bool TL = FORM == FORM_TL, HAS_N = MAT == M_J2 || MAT == M_SVK;
     static constexpr int G = 0;
     static constexpr int P = TL ? 24 : 0;  // aliases g for small strain
     static constexpr int AL = TL ? 48 : 24;
@@ -937,6 +978,67 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             for (int i = 0; i < 3; ++i)
                 for (int k = 0; k < 3; ++k) K[j][i][k] = ratio * T[i][k] + T[k][i] + (i == k ? tr : 0.0);
         }
+    } else if (active && MAT == M_SVK) {
+        // St. Venant-Kirchhoff (total Lagrangian):
+        // K_ab,ik = alpha q_a,i q_b,k + beta q_a,k q_b,i + beta (r_a . r_b) B_ik + delta_ik (p_a . S p_b),
+        // q = B p, with alpha = lambda wd, beta = mu wd, S = wd tau, B unscaled
+        for (int q = 0; q < NQ; ++q) {
+            const double al = ld(L::AL, q), be = ld(L::BE, q);
+            double pa[3], ra[3], Bm[3][3], Sm[3][3];
+            for (int i = 0; i < 3; ++i) {
+                ra[i] = ldg(L::G, a, i, q);
+                pa[i] = ldg(L::P, a, i, q);
+            }
+            {
+                const double b0 = ld(L::NV + 0, q), b1 = ld(L::NV + 1, q), b2 = ld(L::NV + 2, q);
+                const double b3 = ld(L::NV + 3, q), b4 = ld(L::NV + 4, q), b5 = ld(L::NV + 5, q);
+                Bm[0][0] = b0, Bm[1][1] = b1, Bm[2][2] = b2;
+                Bm[0][1] = Bm[1][0] = b3, Bm[0][2] = Bm[2][0] = b4, Bm[1][2] = Bm[2][1] = b5;
+                const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+                const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+                Sm[0][0] = S0, Sm[1][1] = S1, Sm[2][2] = S2;
+                Sm[0][1] = Sm[1][0] = S3, Sm[0][2] = Sm[2][0] = S4, Sm[1][2] = Sm[2][1] = S5;
+            }
+            double qa[3], tpa[3];
+            for (int i = 0; i < 3; ++i) {
+                qa[i] = Bm[i][0] * pa[0] + Bm[i][1] * pa[1] + Bm[i][2] * pa[2];
+                tpa[i] = Sm[i][0] * pa[0] + Sm[i][1] * pa[1] + Sm[i][2] * pa[2];
+                fa[i] += tpa[i];  // internal force f_a = S p_a
+            }
+            {  // diagonal block: symmetric, upper triangle accumulated and mirrored after the loop
+                const double rr = be * (ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
+                const double d1 = pa[0] * tpa[0] + pa[1] * tpa[1] + pa[2] * tpa[2];
+                for (int i = 0; i < 3; ++i)
+                    for (int k = i; k < 3; ++k) {
+                        K[0][i][k] = fma((al + be) * qa[i], qa[k], K[0][i][k]);
+                        K[0][i][k] = fma(rr, Bm[i][k], K[0][i][k]);
+                    }
+                for (int i = 0; i < 3; ++i) K[0][i][i] += d1;
+            }
+#pragma unroll
+            for (int j = 1; j < 5; ++j) {
+                if (j < nb) {
+                    const int b = (a + j) & 7;
+                    double pb[3], rb[3], qb[3];
+                    for (int i = 0; i < 3; ++i) {
+                        rb[i] = ldg(L::G, b, i, q);
+                        pb[i] = ldg(L::P, b, i, q);
+                    }
+                    for (int i = 0; i < 3; ++i) qb[i] = Bm[i][0] * pb[0] + Bm[i][1] * pb[1] + Bm[i][2] * pb[2];
+                    const double rr = be * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                    const double d1 = tpa[0] * pb[0] + tpa[1] * pb[1] + tpa[2] * pb[2];
+                    for (int i = 0; i < 3; ++i)
+                        for (int k = 0; k < 3; ++k) {
+                            K[j][i][k] = fma(al * qa[i], qb[k], K[j][i][k]);
+                            K[j][i][k] = fma(be * qa[k], qb[i], K[j][i][k]);
+                            K[j][i][k] = fma(rr, Bm[i][k], K[j][i][k]);
+                        }
+                    for (int i = 0; i < 3; ++i) K[j][i][i] += d1;
+                }
+            }
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < i; ++k) K[0][i][k] = K[0][k][i];
     } else if (active) {
         for (int q = 0; q < NQ; ++q) {
             const double al = ld(L::AL, q), be = ld(L::BE, q), ga = ld(L::GA, q);
@@ -1520,7 +1622,10 @@ __global__ __launch_bounds__(256) void k_qp_results(EvalArgs A, const double *__
         } else {
             const double Jd = inv3(F, Fi);
             Recipe r;
-            nh_recipe(Epar, M.p[1], F, Jd, r);  // r.S: [11, 22, 33, 12, 13, 23]
+            if (MAT == M_SVK)
+                svk_recipe(Epar, M.p[1], F, r);
+            else
+                nh_recipe(Epar, M.p[1], F, Jd, r);  // r.S: [11, 22, 33, 12, 13, 23]
             const double o[6] = {r.S[0], r.S[1], r.S[2], r.S[3], r.S[5], r.S[4]};
             for (int i = 0; i < 6; ++i) v[i] = o[i];
         }
@@ -1684,7 +1789,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             put(L::COEF + 1, wd * r.beta);
             put(L::COEF + 2, wd * r.gam);
             put(L::COEF + 3, wd * r.c);
-            for (int m = 0; m < 6; ++m) put(L::NV + m, (MAT == M_J2) ? r.N[m] : 0.0);
+            for (int m = 0; m < 6; ++m) put(L::NV + m, (MAT == M_J2 || MAT == M_SVK) ? r.N[m] : 0.0);
             for (int m = 0; m < 6; ++m) put(L::S + m, wd * r.S[m]);
         }
         __syncthreads();
@@ -1707,6 +1812,49 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 Srow[1] = (i == 0) ? S3 : (i == 1 ? S1 : S5);
                 Srow[2] = (i == 0) ? S4 : (i == 1 ? S5 : S2);
                 fi += Srow[0] * pa[0] + Srow[1] * pa[1] + Srow[2] * pa[2];
+            }
+            if constexpr (MAT == M_SVK) {
+                // St. Venant-Kirchhoff: K_ab,ik = alpha q_a,i q_b,k + beta q_a,k q_b,i + beta (r_a . r_b) B_ik
+                // + delta_ik (p_a . S p_b), q = B p (alpha = lambda wd, beta = mu wd, S = wd tau)
+                double Bm[3][3], Sm[3][3];
+                {
+                    const double b0 = ld(L::NV + 0, q), b1 = ld(L::NV + 1, q), b2 = ld(L::NV + 2, q);
+                    const double b3 = ld(L::NV + 3, q), b4 = ld(L::NV + 4, q), b5 = ld(L::NV + 5, q);
+                    Bm[0][0] = b0, Bm[1][1] = b1, Bm[2][2] = b2;
+                    Bm[0][1] = Bm[1][0] = b3, Bm[0][2] = Bm[2][0] = b4, Bm[1][2] = Bm[2][1] = b5;
+                    const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+                    const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+                    Sm[0][0] = S0, Sm[1][1] = S1, Sm[2][2] = S2;
+                    Sm[0][1] = Sm[1][0] = S3, Sm[0][2] = Sm[2][0] = S4, Sm[1][2] = Sm[2][1] = S5;
+                }
+                double qa[3], tpa[3];
+                for (int m = 0; m < 3; ++m) {
+                    qa[m] = Bm[m][0] * pa[0] + Bm[m][1] * pa[1] + Bm[m][2] * pa[2];
+                    tpa[m] = Sm[m][0] * pa[0] + Sm[m][1] * pa[1] + Sm[m][2] * pa[2];
+                }
+#pragma unroll
+                for (int jj = 0; jj < kG; ++jj) {
+                    if (jj < nj) {
+                        const int b = (a + j0 + jj) % NN;
+                        double pb[3], rb[3], qb[3];
+                        for (int m = 0; m < 3; ++m) {
+                            rb[m] = ld(L::G + 3 * b + m, q);
+                            pb[m] = ld(L::P + 3 * b + m, q);
+                        }
+                        for (int m = 0; m < 3; ++m) qb[m] = Bm[m][0] * pb[0] + Bm[m][1] * pb[1] + Bm[m][2] * pb[2];
+                        const double rr = be * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                        const double d1 = tpa[0] * pb[0] + tpa[1] * pb[1] + tpa[2] * pb[2];
+#pragma unroll
+                        for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                double v = al * qa[ii] * qb[k] + be * qa[k] * qb[ii] + rr * Bm[ii][k];
+                                if (ii == k) v += d1;
+                                K[jj][3 * ii + k] += v;
+                            }
+                    }
+                }
+                continue;
             }
             if (MAT == M_J2) {
                 for (int m = 0; m < 6; ++m) Nq[m] = ld(L::NV + m, q);
@@ -1941,6 +2089,7 @@ extern "C" int ewfe_state_size(int mat_kind)
     case EWFE_LINEAR_ELASTIC: return 0;
     case EWFE_VON_MISES: return 7;
     case EWFE_NEO_HOOKE_WA: return 0;
+    case EWFE_ST_VENANT_KIRCHHOFF: return 0;
     default: ewfe::set_error("unknown material kind"); return EWFE_EINVAL;
     }
 }
@@ -2350,12 +2499,13 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
         R.xcd_batches = (batches + 7) / 8;
         if (batches > 0) {
             const dim3 grid((unsigned)(8 * R.xcd_batches));
-            if (sizeof(VT) == 4 && mfma) {
+            // the matrix-core tangent exists for the materials of the rank form without the B term
+            constexpr bool kHasMF = sizeof(VT) == 4 && MAT != M_SVK;
+            if (kHasMF && mfma) {
                 if (A.mtab)
-                    hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, true, sizeof(VT) == 4>), grid, dim3(64), 0, s, A, R);
+                    hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, true, kHasMF>), grid, dim3(64), 0, s, A, R);
                 else
-                    hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, false, sizeof(VT) == 4>), grid, dim3(64), 0, s, A,
-                                       R);
+                    hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, false, kHasMF>), grid, dim3(64), 0, s, A, R);
             } else if (A.mtab) {
                 hipLaunchKernelGGL((k_hex8_elem<FORM, MAT, VT, true, false>), grid, dim3(64), 0, s, A, R);
             } else {
@@ -2455,8 +2605,16 @@ static int dispatch(ewfe_plan *p, int mat_kind, EvalArgs A, hipStream_t s, bool 
     if (!tl && mat_kind == EWFE_LINEAR_ELASTIC) return launch_family<FORM_SS, M_LE, VT>(p, A, s, mfma);
     if (!tl && mat_kind == EWFE_VON_MISES) return launch_family<FORM_SS, M_J2, VT>(p, A, s, mfma);
     if (tl && mat_kind == EWFE_NEO_HOOKE_WA) return launch_family<FORM_TL, M_NH, VT>(p, A, s, mfma);
+    if (tl && mat_kind == EWFE_ST_VENANT_KIRCHHOFF) {
+        if (mfma) {
+            ewfe::set_error("ewfe_eval_assemble: the matrix-core FP32 tangent covers linear elastic, von Mises and "
+                            "Neo-Hooke; St. Venant-Kirchhoff takes EWFE_F32 or EWFE_F64");
+            return EWFE_EINVAL;
+        }
+        return launch_family<FORM_TL, M_SVK, VT>(p, A, s, false);
+    }
     ewfe::set_error("unsupported element/material combination (small-strain elements take linear elastic or "
-                    "von Mises, total-Lagrangian elements take Neo-Hooke)");
+                    "von Mises, total-Lagrangian elements take Neo-Hooke or St. Venant-Kirchhoff)");
     return EWFE_EINVAL;
 }
 
@@ -2674,7 +2832,7 @@ extern "C" int ewfe_qp_results(const ewfe_plan *p, int mat_kind, const double *m
     }
     const bool tl = is_tl(p->elem_kind);
     const bool ok = (!tl && (mat_kind == EWFE_LINEAR_ELASTIC || mat_kind == EWFE_VON_MISES)) ||
-                    (tl && mat_kind == EWFE_NEO_HOOKE_WA);
+                    (tl && (mat_kind == EWFE_NEO_HOOKE_WA || mat_kind == EWFE_ST_VENANT_KIRCHHOFF));
     if (!ok) {
         ewfe::set_error("ewfe_qp_results: unsupported element/material combination");
         return EWFE_EINVAL;
@@ -2698,16 +2856,20 @@ extern "C" int ewfe_qp_results(const ewfe_plan *p, int mat_kind, const double *m
     const unsigned blocks = (unsigned)((nt + 255) / 256);
 #define EWFE_QPR(NN, NQ, FORM, MAT) \
     hipLaunchKernelGGL((k_qp_results<NN, NQ, FORM, MAT>), dim3(blocks), dim3(256), 0, s, A, state_dev, result, out_dev)
+    const bool svk = mat_kind == EWFE_ST_VENANT_KIRCHHOFF;
     if (p->npe == 8) {
-        if (tl) EWFE_QPR(8, 8, FORM_TL, M_NH);
+        if (tl && svk) EWFE_QPR(8, 8, FORM_TL, M_SVK);
+        else if (tl) EWFE_QPR(8, 8, FORM_TL, M_NH);
         else if (mat_kind == EWFE_VON_MISES) EWFE_QPR(8, 8, FORM_SS, M_J2);
         else EWFE_QPR(8, 8, FORM_SS, M_LE);
     } else if (p->nqp == 8) {
-        if (tl) EWFE_QPR(20, 8, FORM_TL, M_NH);
+        if (tl && svk) EWFE_QPR(20, 8, FORM_TL, M_SVK);
+        else if (tl) EWFE_QPR(20, 8, FORM_TL, M_NH);
         else if (mat_kind == EWFE_VON_MISES) EWFE_QPR(20, 8, FORM_SS, M_J2);
         else EWFE_QPR(20, 8, FORM_SS, M_LE);
     } else {
-        if (tl) EWFE_QPR(20, 27, FORM_TL, M_NH);
+        if (tl && svk) EWFE_QPR(20, 27, FORM_TL, M_SVK);
+        else if (tl) EWFE_QPR(20, 27, FORM_TL, M_NH);
         else if (mat_kind == EWFE_VON_MISES) EWFE_QPR(20, 27, FORM_SS, M_J2);
         else EWFE_QPR(20, 27, FORM_SS, M_LE);
     }

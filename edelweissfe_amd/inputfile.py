@@ -63,7 +63,8 @@ _KEYWORDS = {
 # material names a deck may use -> the GPU path's laws; ADLinearElastic is Marmot's isotropic linear elasticity
 # with an automatically differentiated tangent (the same law: [E, nu])
 _MATERIALS = {"linearelastic": "linearelastic", "adlinearelastic": "linearelastic", "vonmises": "vonmises",
-              "neohookeanwa": "neohookeanwa", "neohookeanpencegouformulationa": "neohookeanwa"}
+              "neohookeanwa": "neohookeanwa", "neohookeanpencegouformulationa": "neohookeanwa",
+              "stvenantkirchhoffisotropic": "stvenantkirchhoff", "stvenantkirchhoff": "stvenantkirchhoff"}
 # solvers a deck may name; all run as NISTGPU here (edelweissfe/config/solvers.py:48-76)
 _SOLVERS = {"nist", "nistparallel", "nistparallelformarmotelements", "nistgpu"}
 

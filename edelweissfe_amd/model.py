@@ -43,7 +43,11 @@ MATERIAL_PARAMETERS = {
     "linearelastic": ("E", "nu"),  # linearelastic.py
     "vonmises": ("E", "nu", "fy0", "HLin", "dfy", "delta"),  # vonmises.py:105-124
     "neohookeanwa": ("mu", "K"),  # neohookepencegouformulationa.py:72-78
+    # St. Venant-Kirchhoff (the Marmot material StVenantKirchhoffIsotropic of the C3D8TL regression deck)
+    "stvenantkirchhoff": ("E", "nu"),
 }
+# materials of the total-Lagrangian elements (the small-strain elements take the others)
+FINITE_STRAIN_MATERIALS = ("neohookeanwa", "stvenantkirchhoff")
 
 
 class Material:
@@ -119,9 +123,10 @@ class ElementBlock:
         if not materials:
             raise ValueError("a block needs at least one material")
         for material in materials:
-            if self.elType.endswith("TL") != (material.name == "neohookeanwa"):
+            if self.elType.endswith("TL") != (material.name in FINITE_STRAIN_MATERIALS):
                 raise AttributeError(f"{self.elType} cannot take material {material.name}: small-strain elements "
-                                     "take linearelastic / vonmises, total-Lagrangian elements take neohookeanwa")
+                                     "take linearelastic / vonmises, total-Lagrangian elements take neohookeanwa / "
+                                     "stvenantkirchhoff")
         if len({m.name for m in materials}) != 1:
             raise NotImplementedError("one element block takes materials of one kind (one kernel per evaluation)")
         if len(materials) > 1:

@@ -48,7 +48,16 @@ enum ewfe_elem_kind {
     EWFE_C3D20R = 4,   /* 20 nodes, 2x2x2 points (edelweissfe/elements/library.py:276-290) */
     EWFE_C3D20RTL = 5  /* the same, total Lagrangian (library.py:517-531) */
 };
-enum ewfe_mat_kind { EWFE_LINEAR_ELASTIC = 0, EWFE_VON_MISES = 1, EWFE_NEO_HOOKE_WA = 2 };
+/* Materials: small-strain elements take EWFE_LINEAR_ELASTIC or EWFE_VON_MISES, total-Lagrangian elements
+ * EWFE_NEO_HOOKE_WA or EWFE_ST_VENANT_KIRCHHOFF (Marmot's StVenantKirchhoffIsotropic of the reference's C3D8TL
+ * regression deck, testfiles/C3D8TL/test.inp: S = lambda tr(E) I + 2 mu E with E the Green strain; no
+ * matrix-core tangent). */
+enum ewfe_mat_kind {
+    EWFE_LINEAR_ELASTIC = 0,
+    EWFE_VON_MISES = 1,
+    EWFE_NEO_HOOKE_WA = 2,
+    EWFE_ST_VENANT_KIRCHHOFF = 3
+};
 /* Precision of the assembled tangent. EWFE_F32 and EWFE_F32_MFMA both store FP32 CSR values; EWFE_F32 forms the
  * element tangents with FP64 FMAs and rounds them when they are stored, EWFE_F32_MFMA forms the tangents of 8-node
  * elements in FP32 on the matrix cores (v_mfma_f32_16x16x4_f32; 20-node elements as EWFE_F32). Only
@@ -158,7 +167,7 @@ int ewfe_plan_set_materials(ewfe_plan *plan, int32_t n_mat, const double *params
 /* One Newton iteration's element evaluation + assembly (the reference zeroes P, F and the CSR values
  * first, nonlinearimplicitstatic.py:419, csrgenerator.pyx:111; so does this call).
  *   mat_params_host  f64[8]  linear elastic [E, nu]; von Mises [E, nu, fy0, H, dfy, delta];
- *                            Neo-Hooke Wa [mu, K]
+ *                            Neo-Hooke Wa [mu, K]; St. Venant-Kirchhoff [E, nu]
  *   el_param_dev     f64[n_el] or NULL: per-element override of mat_params[0] (original element order)
  *   X_dev            f64[n_dof] reference coordinates laid out like a dof vector (X[dof(node)+c])
  *   U_np_dev, dU_dev f64[n_dof] current total displacement and increment displacement

@@ -18,6 +18,11 @@
  *     _elementcomputationmatrices.py:70-107,247-306;
  *   - compressible Neo-Hooke (Pence-Gou formulation "a"):
  *     ew/materials/neohooke/neohookepencegouformulationa.py:107-141;
+ *   - St. Venant-Kirchhoff (the Marmot material "StVenantKirchhoffIsotropic" of the
+ *     C3D8TL regression deck, testfiles/C3D8TL/test.inp; Marmot is not in the
+ *     reference tree, so the law is restated from its definition: S = lambda
+ *     tr(E) I + 2 mu E with lambda, mu from E, nu; tau = F S F^T; its
+ *     derivative dtau/dF in the element's formula below);
  *   - the parallel element loop (private Ke/Pe slots, then a serial P/F scatter
  *     in element order): ew/solvers/nonlinearimplicitstaticparallelmk2.pyx:111-198;
  *   - the VIJ -> CSR scatter-add in VIJ order: ew/numerics/csrgenerator.pyx:100-115.
@@ -43,7 +48,7 @@
 #include <time.h>
 
 enum { EL_C3D8 = 0, EL_C3D20 = 1, EL_C3D8TL = 2, EL_C3D20TL = 3, EL_C3D20R = 4, EL_C3D20RTL = 5 };
-enum { MAT_LE = 0, MAT_J2 = 1, MAT_NH = 2 };
+enum { MAT_LE = 0, MAT_J2 = 1, MAT_NH = 2, MAT_SVK = 3 };
 
 #define MAXN 20
 #define MAXQ 27
@@ -307,14 +312,16 @@ static int small_strain_element(int ek, int mk, const double *mp, const double *
     return 0;
 }
 
-/* Total-Lagrangian hexahedron with Neo-Hooke Wa (DisplacementTLElement,
- * hyperelastic branch). U is the total displacement. mp = [mu, K]. state per
- * QP: [Kirchhoff stress 6 (11,22,33,12,23,13) | Green strain 6 | energy]. */
-static int tl_element(int ek, const double *mp, const double *X, const double *U, double *state_t, int nstate,
-                      double *Ke, double *Pe)
+/* Total-Lagrangian hexahedron with a hyperelastic material (DisplacementTLElement,
+ * hyperelastic branch). U is the total displacement. mk = MAT_NH: mp = [mu, K];
+ * mk = MAT_SVK: mp = [E, nu]. state per QP: [Kirchhoff stress 6 (11,22,33,12,23,13) |
+ * Green strain 6 | energy]. */
+static int tl_element(int ek, int mk, const double *mp, const double *X, const double *U, double *state_t,
+                      int nstate, double *Ke, double *Pe)
 {
     const int nn = el_nnodes(ek), nq = el_nqp(ek), nd = 3 * nn;
     const double mu = mp[0], Kb = mp[1];
+    const double lamS = mp[0] * mp[1] / ((1 + mp[1]) * (1 - 2 * mp[1])), muS = mp[0] / (2 * (1 + mp[1]));
     memset(Ke, 0, sizeof(double) * nd * nd);
     memset(Pe, 0, sizeof(double) * nd);
     for (int q = 0; q < nq; ++q) {
@@ -333,8 +340,23 @@ static int tl_element(int ek, const double *mp, const double *X, const double *U
         const double lamBar = (Kb - 2.0 / 3.0 * mu) * (J * J - J) - mu;
         const double muBar = (Kb - 2.0 / 3.0 * mu) * (2 * J * J - J);
         double T[3][3];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) T[i][j] = mu * Bc[i][j] + (i == j ? lamBar : 0.0);
+        double S2[3][3], Egr[3][3], trE = 0;  /* SVK: second Piola-Kirchhoff stress, Green strain */
+        if (mk == MAT_SVK) {
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    Egr[i][j] = 0.5 * (F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j] - (i == j));
+            trE = Egr[0][0] + Egr[1][1] + Egr[2][2];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) S2[i][j] = 2 * muS * Egr[i][j] + (i == j ? lamS * trE : 0.0);
+            double FS[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) FS[i][j] = F[i][0] * S2[0][j] + F[i][1] * S2[1][j] + F[i][2] * S2[2][j];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) T[i][j] = FS[i][0] * F[j][0] + FS[i][1] * F[j][1] + FS[i][2] * F[j][2];
+        } else {
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) T[i][j] = mu * Bc[i][j] + (i == j ? lamBar : 0.0);
+        }
         double *st = state_t + q * nstate;
         st[0] = T[0][0];
         st[1] = T[1][1];
@@ -354,14 +376,38 @@ static int tl_element(int ek, const double *mp, const double *X, const double *U
         st[9] = 2 * Eg[0][1];
         st[10] = 2 * Eg[1][2];
         st[11] = 2 * Eg[0][2];
-        if (nstate > 12) st[12] = mu / 2 * (I1 - 3) + (Kb / 2 - mu / 3) * (J - 1) * (J - 1) - mu * log(J);
+        if (nstate > 12) {
+            if (mk == MAT_SVK) {
+                double EE = 0;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) EE += Egr[i][j] * Egr[i][j];
+                st[12] = 0.5 * lamS * trE * trE + muS * EE;
+            } else {
+                st[12] = mu / 2 * (I1 - 3) + (Kb / 2 - mu / 3) * (J - 1) * (J - 1) - mu * log(J);
+            }
+        }
         /* A_ijkl = dtau_ij/dF_kl */
         double A[3][3][3][3];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
-                for (int k = 0; k < 3; ++k)
-                    for (int l = 0; l < 3; ++l)
-                        A[i][j][k][l] = mu * ((i == k) * F[j][l] + F[i][l] * (j == k)) + muBar * (i == j) * Fi[l][k];
+        if (mk == MAT_SVK) {
+            /* tau = F S F^T, dS_mn/dF_kl = lambda delta_mn F_kl + mu (delta_ml F_kn + F_km delta_nl):
+             * A_ijkl = delta_ik (F S)_jl + (F S)_il delta_jk + lambda B_ij F_kl + mu (F_il B_kj + B_ik F_jl) */
+            double FS[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) FS[i][j] = F[i][0] * S2[0][j] + F[i][1] * S2[1][j] + F[i][2] * S2[2][j];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k)
+                        for (int l = 0; l < 3; ++l)
+                            A[i][j][k][l] = (i == k) * FS[j][l] + FS[i][l] * (j == k) + lamS * Bc[i][j] * F[k][l] +
+                                            muS * (F[i][l] * Bc[k][j] + Bc[i][k] * F[j][l]);
+        } else {
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k)
+                        for (int l = 0; l < 3; ++l)
+                            A[i][j][k][l] =
+                                mu * ((i == k) * F[j][l] + F[i][l] * (j == k)) + muBar * (i == j) * Fi[l][k];
+        }
         /* spatial gradients NAi[a][m] = sum_I g[a][I] Fi[I][m] */
         double NA[MAXN][3];
         for (int a = 0; a < nn; ++a)
@@ -403,7 +449,7 @@ int oracle_element(int ek, int mk, const double *mp, const double *X, const doub
     int rc;
     if (el_is_tl(ek)) {
         memcpy(state_t, state_n, sizeof(double) * el_nqp(ek) * nstate);
-        rc = tl_element(ek, mp, X, U, state_t, nstate, Ke, Pe);
+        rc = tl_element(ek, mk, mp, X, U, state_t, nstate, Ke, Pe);
     } else {
         rc = small_strain_element(ek, mk, mp, X, dU, state_n, state_t, nstate, Ke, Pe);
     }

@@ -33,11 +33,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libewfe_oracle.so")
 
 EL_KINDS = {"C3D8": 0, "C3D20": 1, "C3D8TL": 2, "C3D20TL": 3, "C3D20R": 4, "C3D20RTL": 5}
-MAT_KINDS = {"linearelastic": 0, "vonmises": 1, "neohookeanwa": 2}
+MAT_KINDS = {"linearelastic": 0, "vonmises": 1, "neohookeanwa": 2, "stvenantkirchhoff": 3}
 NNODES = {0: 8, 1: 20, 2: 8, 3: 20, 4: 20, 5: 20}
 NQP = {0: 8, 1: 27, 2: 8, 3: 27, 4: 8, 5: 8}
 # per-QP state sizes of the reference elements: stress 6 + strain 6 + material state
-NSTATE = {0: 12, 1: 13, 2: 13}
+NSTATE = {0: 12, 1: 13, 2: 13, 3: 13}
 
 _lib = None
 

@@ -21,6 +21,7 @@ J2 = [210000.0, 0.3, 250.0, 1000.0, 0.0, 0.0]
 J2NL = [210000.0, 0.3, 550.0, 1000.0, 200.0, 1400.0]
 LE = [210000.0, 0.3]
 NH = [80.0, 200.0]
+SVK = [30000.0, 0.15]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -245,6 +246,33 @@ def test_fp32_tangent_matches_oracle(el, mat, mp, n, mode):
         assert np.array_equal(o32[k], o64[k]), k
     again = g(U, dU, mode)
     assert np.array_equal(again["data"], o32["data"])  # deterministic
+
+
+@pytest.mark.parametrize("el,n", [("C3D8TL", 4), ("C3D20TL", 2), ("C3D20RTL", 2)])
+def test_st_venant_kirchhoff_matches_oracle(el, n):
+    """St. Venant-Kirchhoff on the total-Lagrangian elements (the material of the reference's C3D8TL deck): one
+    evaluation at a large deformation against the oracle — FP64 CSR values, P and F within 1e-12, FP32 values within
+    2e-6 with P and F bitwise the FP64 variant's; the matrix-core tangent refuses the material."""
+    from edelweissfe_amd._lib import EwfeError
+
+    print(f"case St. Venant-Kirchhoff {el} n={n}", flush=True)
+    X, conn, _ = jittered_box(n, el, seed=11)
+    rng = np.random.default_rng(12)
+    U = _uniaxial_field(X, 0.1, rng, 0.005)
+    g = GpuEval(el, "stvenantkirchhoff", SVK, X, conn)
+    o64 = g(U, U)
+    ref = oracle_eval(el, "stvenantkirchhoff", SVK, X, conn, U, U)
+    errs = [rel(o64[k], ref[k]) for k in ("data", "P", "F")]
+    print(f"FP64 CSR values {errs[0]:.2e}, P {errs[1]:.2e}, F {errs[2]:.2e}", flush=True)
+    assert max(errs) < 1e-12
+    o32 = g(U, U, "fp32")
+    e32 = rel(o32["data"], ref["data"])
+    print(f"FP32 CSR values {e32:.2e}", flush=True)
+    assert e32 < 2e-6 and np.array_equal(o32["P"], o64["P"]) and np.array_equal(o32["F"], o64["F"])
+    with pytest.raises(EwfeError):
+        g(U, U, "fp32-mfma")
+    again = g(U, U)
+    assert np.array_equal(again["data"], o64["data"]) and np.array_equal(again["P"], o64["P"])
 
 
 def test_per_element_parameter_field():
@@ -497,7 +525,8 @@ def test_surface_pressure_matches_oracle(el):
 @pytest.mark.parametrize("el,mat,mp", [("C3D8", "linearelastic", LE), ("C3D8", "vonmises", J2),
                                        ("C3D20", "vonmises", J2), ("C3D8TL", "neohookeanwa", NH),
                                        ("C3D20TL", "neohookeanwa", NH), ("C3D20R", "vonmises", J2),
-                                       ("C3D20RTL", "neohookeanwa", NH)])
+                                       ("C3D20RTL", "neohookeanwa", NH), ("C3D8TL", "stvenantkirchhoff", SVK),
+                                       ("C3D20TL", "stvenantkirchhoff", SVK)])
 def test_qp_results_match_oracle_state(el, mat, mp):
     """Quadrature-point stress and strain of ewfe_qp_results against the per-point state the oracle keeps in
     the reference's layout (stress 6 | strain 6 | ...), after one evaluation from the virgin state."""

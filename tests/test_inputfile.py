@@ -21,10 +21,11 @@ FIXTURE_DECKS = ["TensionBarHexa8", "SimpleBeamHexa8", "CantileverBeamHexa8", "C
 # AnalyticalFieldsDirichlet (C3D20R torsion bar, Dirichlet values scaled by scalarExpression fields)
 # WallShearHexa8GMRES (C3D8N, the reference's pyamg-GMRES solver: the multigrid PCG here)
 # AnalyticalFieldsSetField (J2 with kappa set from a scalarExpression field at the points)
-# and SetField (kappa set uniformly, then the stress set at every point after a support is released)
+# and SetField (kappa set uniformly, then the stress set at every point after a support is released);
+# C3D8TL (a C3D8TL beam of Marmot's StVenantKirchhoffIsotropic bent to a 20 mm tip deflection in 50 increments)
 ALL_DECKS = FIXTURE_DECKS + ["BodyForce3D", "BoxGen", "AnalyticalFieldsDirichlet", "WallShearHexa8GMRES",
                              "AnalyticalFieldsSetField", "SetField", "SetDef", "StatusFile", "ComputeTimeMonitor",
-                             "PipeGen", "CubitGen", "ADLinearElastic"]
+                             "PipeGen", "CubitGen", "ADLinearElastic", "C3D8TL"]
 
 
 def _deck(name):

@@ -107,6 +107,50 @@ def test_neohooke_tangent_vs_fd(ek, nn):
     assert np.abs(K - Kfd).max() < 1e-6 * np.abs(K).max()
 
 
+@pytest.mark.parametrize("ek,nn", [("C3D8TL", 8), ("C3D20TL", 20), ("C3D20RTL", 20)])
+def test_st_venant_kirchhoff_tangent_vs_fd(ek, nn):
+    """St. Venant-Kirchhoff on the TL elements: the tangent (dtau/dF in the element's formula) against central
+    differences of the forces, at a large deformation; symmetric (hyperelastic)."""
+    Xe = _jittered_hex(nn, seed=6)
+    rng = np.random.default_rng(4)
+    u = rng.uniform(-0.1, 0.1, 3 * nn)
+    nq = O.NQP[O.EL_KINDS[ek]]
+    st = np.zeros(nq * 13)
+    mp = [30000.0, 0.15]
+    K, P, _, _ = O.element(ek, "stvenantkirchhoff", mp, Xe.ravel(), u, u, st)
+    Kfd = _fd_tangent(ek, "stvenantkirchhoff", mp, Xe.ravel(), u, u, st, h=1e-6)
+    assert np.abs(K - Kfd).max() < 1e-6 * np.abs(K).max()
+    assert np.abs(K - K.T).max() < 1e-10 * np.abs(K).max()
+
+
+def test_st_venant_kirchhoff_homogeneous_closed_form():
+    """Homogeneous deformation gradient: every point holds tau = F S F^T with S = lambda tr(E) I + 2 mu E and
+    the Green strain E = (F^T F - I) / 2 (engineering shear), and the state's energy lambda/2 tr(E)^2 + mu E:E;
+    at rest the element is stress free and its tangent is the linear elastic one."""
+    E_, nu = 30000.0, 0.15
+    lam, mu = E_ * nu / ((1 + nu) * (1 - 2 * nu)), E_ / (2 * (1 + nu))
+    Xe = _jittered_hex(8, seed=9, amp=0.05)
+    Fh = np.array([[1.3, 0.1, 0.0], [0.05, 0.9, 0.02], [0.0, -0.1, 1.1]])
+    u = (Xe @ (Fh - np.eye(3)).T).ravel()
+    _, _, st1, _ = O.element("C3D8TL", "stvenantkirchhoff", [E_, nu], Xe.ravel(), u, u, np.zeros(8 * 13))
+    Eg = 0.5 * (Fh.T @ Fh - np.eye(3))
+    S = lam * np.trace(Eg) * np.eye(3) + 2 * mu * Eg
+    T = Fh @ S @ Fh.T
+    ref = np.array([T[0, 0], T[1, 1], T[2, 2], T[0, 1], T[1, 2], T[0, 2]])
+    W = 0.5 * lam * np.trace(Eg) ** 2 + mu * (Eg * Eg).sum()
+    for q in range(8):
+        sq = st1.reshape(8, 13)[q]
+        np.testing.assert_allclose(sq[:6], ref, rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(sq[6:12], [Eg[0, 0], Eg[1, 1], Eg[2, 2], 2 * Eg[0, 1], 2 * Eg[1, 2], 2 * Eg[0, 2]],
+                                   rtol=1e-12, atol=1e-15)
+        assert abs(sq[12] - W) < 1e-12 * W
+    K0, P0, _, _ = O.element("C3D8TL", "stvenantkirchhoff", [E_, nu], Xe.ravel(), np.zeros(24), np.zeros(24),
+                             np.zeros(8 * 13))
+    Kle, _, _, _ = O.element("C3D8", "linearelastic", [E_, nu], Xe.ravel(), np.zeros(24), np.zeros(24),
+                             np.zeros(8 * 12))
+    assert np.abs(P0).max() == 0.0 and np.abs(K0 - Kle).max() < 1e-10 * np.abs(Kle).max()
+
+
 def test_j2_linear_hardening_closed_form():
     """Homogeneous strain, one element: kappa follows the closed form and the returned stress sits on the
     yield surface (vonmises.py:121, 237-240)."""
