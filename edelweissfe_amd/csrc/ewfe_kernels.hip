@@ -542,6 +542,91 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
     return ok;
 }
 
+// Phase 1 of the 20-node kernel with four lanes per quadrature point: lane s of the point's group handles nodes s,
+// s + 4, ..., the node sums (J, grad u, grad du) are completed by a fixed butterfly over the group, so all four lanes
+// hold bitwise the same sums and evaluate the material identically; lane s = 0 alone writes the point's state and
+// reports a cutback. The group's lanes must be active together (the butterfly reads all four).
+template <int NN, int FORM, int MAT, typename ND, typename PG, typename PP>
+__device__ __forceinline__ bool qp_phase_coop(const double (*dN)[3], double w, int s, ND nd, PG put_g, PP put_p,
+                                              const EvalArgs &A, const MatParams &M, double Epar, int64_t pos, int q,
+                                              int NQ, Recipe &r, double &wd)
+{
+    constexpr int NS = NN / 4;  // nodes per lane
+    double ep[6] = {0, 0, 0, 0, 0, 0}, kn = 0.0;
+    if (FORM == FORM_SS && MAT == M_J2) {
+        const int64_t n = A.n_el;
+        for (int c = 0; c < 6; ++c) ep[c] = A.state_n[((int64_t)c * NQ + q) * n + pos];
+        kn = A.state_n[((int64_t)6 * NQ + q) * n + pos];
+    }
+    auto group_sum = [](double v) {
+        v += __shfl_xor(v, 1, 4);
+        v += __shfl_xor(v, 2, 4);
+        return v;
+    };
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        const int a = s + 4 * t;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[i][j] += dN[a][i] * nd(j, a);
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) J[i][j] = group_sum(J[i][j]);
+    const double detJ = inv3(J, Ji);
+    wd = w * detJ;
+    double g[NS][3];
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, dH[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        const int a = s + 4 * t;
+        for (int j = 0; j < 3; ++j) g[t][j] = Ji[j][0] * dN[a][0] + Ji[j][1] * dN[a][1] + Ji[j][2] * dN[a][2];
+        for (int j = 0; j < 3; ++j) put_g(a, j, g[t][j]);
+        for (int i = 0; i < 3; ++i) {
+            const double u = nd(3 + i, a);
+            const double du = (MAT == M_J2) ? nd(6 + i, a) : 0.0;
+            for (int j = 0; j < 3; ++j) {
+                H[i][j] += u * g[t][j];
+                if (MAT == M_J2) dH[i][j] += du * g[t][j];
+            }
+        }
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            H[i][j] = group_sum(H[i][j]);
+            if (MAT == M_J2) dH[i][j] = group_sum(dH[i][j]);
+        }
+    bool ok = true;
+    if (FORM == FORM_SS) {
+        if (MAT == M_LE) {
+            le_recipe(M, Epar, H, r);
+        } else {
+            double ep_t[6], kt;
+            const int64_t n = A.n_el;
+            ok = j2_recipe(M, Epar, H, dH, ep, kn, ep_t, kt, r);
+            if (s == 0) {
+                for (int c = 0; c < 6; ++c) A.state_t[((int64_t)c * NQ + q) * n + pos] = ep_t[c];
+                A.state_t[((int64_t)6 * NQ + q) * n + pos] = kt;
+            }
+        }
+    } else {
+        double F[3][3], Fi[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
+        const double Jd = inv3(F, Fi);
+        if (MAT == M_SVK)
+            svk_recipe(Epar, M.p[1], F, r);
+        else
+            nh_recipe(Epar, M.p[1], F, Jd, r);
+        // spatial gradients NA_a = F^-T g_a of the lane's nodes
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            const int a = s + 4 * t;
+            for (int m = 0; m < 3; ++m) put_p(a, m, g[t][0] * Fi[0][m] + g[t][1] * Fi[1][m] + g[t][2] * Fi[2][m]);
+        }
+    }
+    return ok;
+}
+
 // Number of LDS fields per quadrature point: g (3 NN) [+ p (3 NN) for TL] + alpha,beta,gam,c + N (6) + S (6)
 template <int NN, int FORM>
 struct QPLayout {
@@ -1773,24 +1858,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int q0 = 0; q0 < NQ; q0 += QC) {
         const int nq = NQ - q0 < QC ? NQ - q0 : QC;
         __syncthreads();  // the nodes are staged, or the previous chunk's points are summed
-        if (lane < nq) {
-            const int ql = lane, q = q0 + lane;
+        // four lanes per point: lane (ql, s) = 4 ql + s; the groups of points past nq stay idle as a whole
+        const int ql = lane >> 2, sq = lane & 3;
+        if (ql < nq) {
+            const int q = q0 + ql;
             auto nd = [&](int f, int b) { return s_nd[f * NN + b]; };
             auto put_g = [&](int b, int j, double v) { s_qp[(L::G + 3 * b + j) * QC + ql] = v; };
-            auto get_g = [&](int b, int j) { return s_qp[(L::G + 3 * b + j) * QC + ql]; };
             auto put_p = [&](int b, int j, double v) { s_qp[(L::P + 3 * b + j) * QC + ql] = v; };
             Recipe r;
             double wd;
-            const bool ok = qp_phase<NN, FORM, MAT>(qp_dN<NN, NQ>(q), qp_w<NN, NQ>(q), nd, put_g, get_g, put_p, A,
-                                                    M, Epar, pos, q, NQ, r, wd);
-            if (!ok) atomic_min_pos(A.pnewdt, 0.5);
-            auto put = [&](int f, double v) { s_qp[f * QC + ql] = v; };
-            put(L::COEF + 0, wd * r.alpha);
-            put(L::COEF + 1, wd * r.beta);
-            put(L::COEF + 2, wd * r.gam);
-            put(L::COEF + 3, wd * r.c);
-            for (int m = 0; m < 6; ++m) put(L::NV + m, (MAT == M_J2 || MAT == M_SVK) ? r.N[m] : 0.0);
-            for (int m = 0; m < 6; ++m) put(L::S + m, wd * r.S[m]);
+            const bool ok = qp_phase_coop<NN, FORM, MAT>(qp_dN<NN, NQ>(q), qp_w<NN, NQ>(q), sq, nd, put_g, put_p, A,
+                                                         M, Epar, pos, q, NQ, r, wd);
+            if (sq == 0) {  // every lane of the group holds the point's recipe; the first stores it
+                if (!ok) atomic_min_pos(A.pnewdt, 0.5);
+                auto put = [&](int f, double v) { s_qp[f * QC + ql] = v; };
+                put(L::COEF + 0, wd * r.alpha);
+                put(L::COEF + 1, wd * r.beta);
+                put(L::COEF + 2, wd * r.gam);
+                put(L::COEF + 3, wd * r.c);
+                for (int m = 0; m < 6; ++m) put(L::NV + m, (MAT == M_J2 || MAT == M_SVK) ? r.N[m] : 0.0);
+                for (int m = 0; m < 6; ++m) put(L::S + m, wd * r.S[m]);
+            }
         }
         __syncthreads();
         if (!row_lane) continue;
