@@ -99,6 +99,26 @@ def test_c3d20_gather_and_colour_paths(el, mat, mp, dtype):
     assert np.array_equal(again["data"], out[1]["data"]) and np.array_equal(again["P"], out[1]["P"])
 
 
+def test_c3d20_colour_path_when_a_node_has_more_than_eight_elements():
+    """A 20-node mesh whose nodes lie in more than 8 elements (here: nine coincident copies of every element of a
+    2 x 1 x 1 box, as a degenerate but evaluable mesh) has no row-node gather lists: the plan assembles through the
+    colour path, matches the oracle (which sums the copies in element order), and refuses to switch to a gather."""
+    from edelweissfe_amd._lib import lib
+
+    print("case C3D20: more than 8 elements per node", flush=True)
+    X, conn, _ = jittered_box((2, 1, 1), "C3D20", seed=21)
+    conn9 = np.repeat(conn, 9, axis=0)
+    rng = np.random.default_rng(22)
+    U = _uniaxial_field(X, 2e-3, rng, 2e-4)
+    g = GpuEval("C3D20", "linearelastic", LE, X, conn9)
+    assert lib().ewfe_plan_set_row_gather(g.asm._plan, 1) != 0  # no gather lists for this mesh
+    out = g(U, U)
+    ref = oracle_eval("C3D20", "linearelastic", LE, X, conn9, U, U)
+    errs = [rel(out[k], ref[k]) for k in ("data", "P", "F")]
+    print(f"colour path: CSR values {errs[0]:.2e}, P {errs[1]:.2e}, F {errs[2]:.2e}", flush=True)
+    assert max(errs) < 1e-12
+
+
 def test_plain_method_10cube():
     """The CPU-runnable configuration at its own size: a 10^3 unit cube of C3D8 (1,331 nodes), linear elastic
     E = 210000, nu = 0.3, bottom face (z = 0) fixed, top face uz = +1e-3 in one increment. One evaluation + CSR

@@ -122,6 +122,23 @@ def test_plan_rejects_bad_input():
         _lib.check(rc)
 
 
+def test_plan_rejects_empty_and_ragged_input():
+    """No elements, no nodes or no dofs fail with a message; a node numbered past n_nodes is refused."""
+    L = _lib.lib()
+    plan = ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    conn = np.zeros((0, 8), dtype=np.int32)
+    node_dof = np.arange(0, 24, 3, dtype=np.int64)
+    for n_el, n_nodes, n_dof in ((0, 8, 24), (1, 0, 24), (1, 8, 0)):
+        c = conn if n_el == 0 else np.arange(8, dtype=np.int32).reshape(1, 8)
+        rc = L.ewfe_plan_create_host(ctypes.byref(plan), 0, n_el, _lib.ptr(c), n_nodes, _lib.ptr(node_dof), n_dof,
+                                     ctypes.byref(nnz), 0)
+        assert rc != 0 and L.ewfe_last_error()
+    c = np.arange(8, dtype=np.int32).reshape(1, 8)
+    rc = L.ewfe_plan_create_host(ctypes.byref(plan), 0, 1, _lib.ptr(c), 7, _lib.ptr(node_dof), 24, ctypes.byref(nnz), 0)
+    assert rc != 0
+
+
 def test_material_table_rejects_bad_indices():
     """ewfe_plan_set_materials checks its arguments before any device work: an index outside [0, n_mat) or a
     missing table fails with EWFE_EINVAL and a message; n_mat = 0 (no table) succeeds."""
