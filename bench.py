@@ -238,17 +238,17 @@ def copy_calibration(dev, nbytes=1 << 30):
     torch.cuda.empty_cache()
 
 
-def _timed_evals(asm, U, dU, P, F, K, steps, warmup):
+def _timed_evals(asm, U, dU, P, F, K, steps, warmup, mfma=False):
     import torch
 
     for _ in range(warmup):
-        asm.evaluate(U, dU, P, F, K)
+        asm.evaluate(U, dU, P, F, K, mfma=mfma)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
     for _ in range(steps):
-        asm.evaluate(U, dU, P, F, K)
+        asm.evaluate(U, dU, P, F, K, mfma=mfma)
     ev1.record(stream)
     torch.cuda.synchronize()
     return ev0.elapsed_time(ev1) / steps
@@ -285,6 +285,7 @@ def c3d20_tl_measure(n, steps, warmup, cpu=True):
 
     ms64 = _timed_evals(asm, U, U, P, F, K64, steps, warmup)
     ms32 = _timed_evals(asm, U, U, P, F, K32, steps, warmup)
+    ms32m = _timed_evals(asm, U, U, P, F, K32, steps, warmup, mfma=True)
     # the colour-ordered read-modify-write path of the same plan, for comparison
     check(lib().ewfe_plan_set_row_gather(asm._plan, 0))
     ms64c = _timed_evals(asm, U, U, P, F, K64, steps, warmup)
@@ -298,7 +299,8 @@ def c3d20_tl_measure(n, steps, warmup, cpu=True):
     # 2 (6 6 n + n n 6) per point, n = 60, is what a dense formulation would execute
     flop_dense = 27 * 2 * (6 * 6 * 60 + 60 * 60 * 6)
     flop_exec = 27 * (210 * 27 * 2 + 20 * 20 * 3 * 2)
-    out = dict(n_el=n_el, dofs=dm.nDof, nnz=asm.nnz, ms64=ms64, ms32=ms32, ms64_colour=ms64c, ms32_colour=ms32c,
+    out = dict(n_el=n_el, dofs=dm.nDof, nnz=asm.nnz, ms64=ms64, ms32=ms32, ms32_mfma=ms32m, ms64_colour=ms64c,
+               ms32_colour=ms32c,
                B64=B64, B32=B32, flop_dense=flop_dense,
                flop_exec=flop_exec, cpu=None)
     if cpu:
@@ -338,6 +340,8 @@ def workload_c3d20_tl(args):
         "cpu_baseline": r["cpu"],
         "fp32_tangent": {"value": n_el / (r["ms32"] * 1e-3), "ms_per_step": r["ms32"],
                          "algorithmic_bytes_per_element": r["B32"]},
+        "fp32_tangent_mfma": {"value": n_el / (r["ms32_mfma"] * 1e-3), "ms_per_step": r["ms32_mfma"],
+                              "tangent": "FP32 on the matrix cores (v_mfma_f32_16x16x4_f32)"},
         "colour_path": {"ms_per_step_fp64": r["ms64_colour"], "ms_per_step_fp32": r["ms32_colour"]},
     }
 
@@ -564,7 +568,7 @@ def main():
         r4 = c3d20_tl_measure(40, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
         legs["cfg4_c3d20tl_neohooke_40cube"] = {
             "elements": r4["n_el"], "ms_per_evaluation_fp64": r4["ms64"], "ms_per_evaluation_fp32": r4["ms32"],
-            "ms_per_evaluation_fp64_colour_path": r4["ms64_colour"],
+            "ms_per_evaluation_fp32_mfma": r4["ms32_mfma"], "ms_per_evaluation_fp64_colour_path": r4["ms64_colour"],
             "value": r4["n_el"] / (r4["ms64"] * 1e-3), "unit": "elements/s",
             "roofline_frac": r4["B64"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "executed_flop_frac": r4["flop_exec"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,

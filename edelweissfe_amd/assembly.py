@@ -185,7 +185,8 @@ class ElementAssembly(PlanLinearAlgebra):
     def evaluate(self, U_np, dU, P, F, vals, mfma=False):
         """Fills P (= sum Pe), F (= sum |Pe|), the CSR values and the trial state. Returns pNewDT (device
         tensor, 1e36 or < 1 when a quadrature point asks for a cutback). FP32 values: ``mfma=True`` forms the
-        tangents of 8-node elements in FP32 on the matrix cores, otherwise with FP64 FMAs rounded when stored."""
+        tangents in FP32 on the matrix cores (8-node elements, and 20-node elements assembled through the row-node
+        gather), otherwise with FP64 FMAs rounded when stored."""
         for t in (U_np, dU, P, F):
             if t.numel() != self.nDof or t.dtype != self.torch.float64 or not t.is_contiguous():
                 raise ValueError("dof vectors must be contiguous float64 tensors of length nDof")

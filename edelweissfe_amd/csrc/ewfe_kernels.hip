@@ -1810,15 +1810,19 @@ __global__ __launch_bounds__(64) void k_adjoint8(EvalArgs A, const double *__res
 // ------------------------------------------------------------------------------------------------
 // REC: write the element's record (unique blocks + nodal forces) for the row-node gather instead of adding into
 // the CSR values (then the launch needs no colouring)
-template <int FORM, int MAT, typename VT, int NQ, bool REC>
+// MF (FP32 records only): the tangent formed on the matrix cores (v_mfma_f32_16x16x4_f32) instead of FP64 FMAs.
+template <int FORM, int MAT, typename VT, int NQ, bool REC, bool MF = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_hex20(EvalArgs A, RecordArgs R)
 {
+    static_assert(!MF || (REC && sizeof(VT) == 4 && MAT != M_SVK), "matrix-core tangent: FP32 records, rank form");
     constexpr int NN = 20;
     constexpr int QC = NQ > 16 ? 14 : NQ;  // points per chunk
     using L = QPLayout<NN, FORM>;
     __shared__ double s_nd[9 * NN];      // [field][node]
-    // [field][point of the chunk]; with REC afterwards the staged record (written out in whole 16-byte units)
-    constexpr int kQP = L::NF * QC, kStage = (REC ? (kRec20 * (int)sizeof(VT) + 7) / 8 : 0);
+    // [field][point of the chunk]; with REC afterwards the staged record (written out in whole 16-byte units);
+    // with MF the two staged halves of the record (T - C + S and Tb) and S_ab
+    constexpr int kQP = L::NF * QC,
+                  kStage = MF ? (2 * kRec20 + NN * NN) / 2 : (REC ? (kRec20 * (int)sizeof(VT) + 7) / 8 : 0);
     __shared__ __align__(16) double s_qp[kQP > kStage ? kQP : kStage];
     __shared__ int32_t s_dof[NN];
     __shared__ int32_t s_rs[3 * NN];     // CSR row starts of the element's 60 dof rows
@@ -1854,6 +1858,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int j = 0; j < kG; ++j)
         for (int t = 0; t < 9; ++t) K[j][t] = 0.0;
     double fi = 0.0;
+    // MF: the 60 x 60 element matrix padded to 64 x 64, its 10 upper 16 x 16 tiles (I <= J) accumulated over the
+    // points as T - C (alpha p p^T - c v v^T) and Tb (beta p p^T), and S_ab = sum_q gamma (r_a . r_b) over the
+    // 3 upper tiles of 32 x 32; lane (kq, c16) supplies row / column 16 I + c16 at inner index kq of each MFMA
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int kTiles = MF ? 10 : 1;
+    f4 accT[kTiles], accB[kTiles], accS[MF ? 3 : 1];
+    const int kq = lane >> 4, c16 = lane & 15;
+    if constexpr (MF) {
+        for (int t = 0; t < kTiles; ++t) accT[t] = accB[t] = (f4){0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < 3; ++t) accS[t] = (f4){0.f, 0.f, 0.f, 0.f};
+    }
 #pragma unroll 1
     for (int q0 = 0; q0 < NQ; q0 += QC) {
         const int nq = NQ - q0 < QC ? NQ - q0 : QC;
@@ -1881,8 +1896,72 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             }
         }
         __syncthreads();
-        if (!row_lane) continue;
         auto ld = [&](int f, int q) { return s_qp[f * QC + q]; };
+        if constexpr (MF) {
+            // forces in FP64 (the FP64 variant's arithmetic): row lane (a, i)
+            if (row_lane) {
+                for (int q = 0; q < nq; ++q) {
+                    const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+                    const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+                    const double Sr0 = (i == 0) ? S0 : (i == 1 ? S3 : S4);
+                    const double Sr1 = (i == 0) ? S3 : (i == 1 ? S1 : S5);
+                    const double Sr2 = (i == 0) ? S4 : (i == 1 ? S5 : S2);
+                    fi += Sr0 * ld(L::P + 3 * a + 0, q) + Sr1 * ld(L::P + 3 * a + 1, q) + Sr2 * ld(L::P + 3 * a + 2, q);
+                }
+            }
+            // T, Tb (and C): inner index = point kq of each group of 4 points of the chunk
+#pragma unroll 1
+            for (int k0 = 0; k0 < nq; k0 += 4) {
+                const int q = k0 + kq;
+                const bool qv = q < nq;
+                const float al = qv ? (float)ld(L::COEF + 0, q) : 0.f;
+                const float be = qv ? (float)ld(L::COEF + 1, q) : 0.f;
+                float x[4], y[4] = {0.f, 0.f, 0.f, 0.f};
+                float mc = 0.f;
+                float Nv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                if (MAT == M_J2) {
+                    mc = qv ? -(float)ld(L::COEF + 3, q) : 0.f;
+                    for (int m = 0; m < 6; ++m) Nv[m] = qv ? (float)ld(L::NV + m, q) : 0.f;
+                }
+#pragma unroll
+                for (int I = 0; I < 4; ++I) {
+                    const int m = 16 * I + c16;
+                    const bool mv = qv && m < 3 * NN;
+                    x[I] = mv ? (float)ld(L::P + m, q) : 0.f;
+                    if (MAT == M_J2 && mv) {  // v = N p, row m = (a, i) of the point's v
+                        const int am = m / 3, im = m % 3;
+                        const int j0 = im == 0 ? 0 : (im == 1 ? 3 : 4), j1 = im == 0 ? 3 : (im == 1 ? 1 : 5),
+                                  j2 = im == 0 ? 4 : (im == 1 ? 5 : 2);
+                        y[I] = Nv[j0] * (float)ld(L::P + 3 * am + 0, q) + Nv[j1] * (float)ld(L::P + 3 * am + 1, q) +
+                               Nv[j2] * (float)ld(L::P + 3 * am + 2, q);
+                    }
+                }
+                int t = 0;
+#pragma unroll
+                for (int I = 0; I < 4; ++I)
+#pragma unroll
+                    for (int J = I; J < 4; ++J, ++t) {
+                        accT[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(al * x[I], x[J], accT[t], 0, 0, 0);
+                        accB[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(be * x[I], x[J], accB[t], 0, 0, 0);
+                        if (MAT == M_J2)
+                            accT[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(mc * y[I], y[J], accT[t], 0, 0, 0);
+                    }
+            }
+            // S: inner index = (point, component) kappa = 3 q + j, rows a = c16 (tile row 0) and 16 + c16 (row 1)
+#pragma unroll 1
+            for (int k0 = 0; k0 < 3 * nq; k0 += 4) {
+                const int kap = k0 + kq, q = kap / 3, j = kap % 3;
+                const bool qv = q < nq;
+                const float gq = qv ? (float)ld(L::COEF + 2, q) : 0.f;
+                const float r0 = qv ? (float)ld(L::G + 3 * c16 + j, q) : 0.f;
+                const float r1 = (qv && 16 + c16 < NN) ? (float)ld(L::G + 3 * (16 + c16) + j, q) : 0.f;
+                accS[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(gq * r0, r0, accS[0], 0, 0, 0);
+                accS[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(gq * r0, r1, accS[1], 0, 0, 0);
+                accS[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(gq * r1, r1, accS[2], 0, 0, 0);
+            }
+            continue;
+        }
+        if (!row_lane) continue;
 #pragma unroll 1
         for (int q = 0; q < nq; ++q) {
             const double al = ld(L::COEF + 0, q), be = ld(L::COEF + 1, q), ga = ld(L::COEF + 2, q),
@@ -1983,6 +2062,52 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 }
             }
         }
+    }
+    if constexpr (MF) {
+        // record from the tiles: lane (kq, c16) holds rows 16 I + 4 kq + v (v = 0..3), column 16 J + c16 of tile
+        // (I, J). Values (m, n), m <= n < 60, are the record's: block (a, b) = (m / 3, n / 3), a <= b. T - C (+ S_ab on
+        // i == k) goes to the slot of K_ab[i][k] (i = m % 3, k = n % 3); Tb value (m, n) belongs to K_ab[k][i]. Each
+        // slot is written by one lane in each half, and the halves are summed when the record is written out.
+        __syncthreads();  // every lane is done with the point data
+        float *sT = reinterpret_cast<float *>(s_qp), *sB = sT + kRec20, *sS = sB + kRec20;
+        const int64_t el = A.order[pos];
+        if (row_lane) R.fe[el * 60 + lane] = fi;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int rb = (u == 2 ? 16 : 0), cb = (u == 0 ? 0 : 16);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ra = rb + 4 * kq + v, cn = cb + c16;
+                if (ra <= cn && cn < NN) sS[ra * NN + cn] = accS[u][v];
+            }
+        }
+        __syncthreads();
+        int t = 0;
+#pragma unroll
+        for (int I = 0; I < 4; ++I)
+#pragma unroll
+            for (int J = I; J < 4; ++J, ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = 16 * I + 4 * kq + v, n2 = 16 * J + c16;
+                    if (m <= n2 && n2 < 3 * NN) {
+                        const int am = m / 3, im = m % 3, bn = n2 / 3, kn = n2 % 3;
+                        float val = accT[t][v];
+                        if (im == kn) val += sS[am * NN + bn];
+                        const bool offd = am < bn;
+                        sT[offd ? block_off20(am, bn) + 3 * im + kn : block_off20(am, am) + kSym6[im][kn]] = val;
+                        sB[offd ? block_off20(am, bn) + 3 * kn + im : block_off20(am, am) + kSym6[im][kn]] = accB[t][v];
+                    }
+                }
+        __syncthreads();
+        constexpr int NW = kRec20 / 4;
+        static_assert(kRec20 % 4 == 0, "records are whole 16-byte units");
+        f4 *dst = reinterpret_cast<f4 *>(reinterpret_cast<float *>(R.kbuf) + el * kRec20);
+        for (int w = lane; w < NW; w += 64) {
+            const f4 sum = reinterpret_cast<const f4 *>(sT)[w] + reinterpret_cast<const f4 *>(sB)[w];
+            __builtin_nontemporal_store(sum, dst + w);
+        }
+        return;
     }
     if (REC) {
         // record of element el staged in LDS (the point data are no longer read once every lane passed the
@@ -2634,7 +2759,7 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
 // gather sums them into every CSR value, P and F entry once; otherwise one launch per colour adds into the CSR
 // values (no two elements of a colour share a node).
 template <int FORM, int MAT, typename VT>
-static int launch_hex20(ewfe_plan *p, EvalArgs A, hipStream_t s)
+static int launch_hex20(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
 {
     constexpr int64_t kMaxGrid = (int64_t)1 << 24;  // workgroups per launch (64 lanes each)
     if (p->use_gather20) {
@@ -2647,10 +2772,18 @@ static int launch_hex20(ewfe_plan *p, EvalArgs A, hipStream_t s)
             A.p_begin = b;
             A.p_end = std::min<int64_t>(p->n_el, b + kMaxGrid);
             const dim3 grid((unsigned)(A.p_end - A.p_begin));
-            if (p->nqp == 8)
+            // the matrix-core tangent: FP32 records of the rank-form materials
+            constexpr bool kHasMF = sizeof(VT) == 4 && MAT != M_SVK;
+            if (kHasMF && mfma) {
+                if (p->nqp == 8)
+                    hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 8, true, kHasMF>), grid, dim3(64), 0, s, A, R);
+                else
+                    hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 27, true, kHasMF>), grid, dim3(64), 0, s, A, R);
+            } else if (p->nqp == 8) {
                 hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 8, true>), grid, dim3(64), 0, s, A, R);
-            else
+            } else {
                 hipLaunchKernelGGL((k_hex20<FORM, MAT, VT, 27, true>), grid, dim3(64), 0, s, A, R);
+            }
             HIP_CHECK(hipGetLastError());
         }
         for (int64_t b = 0; b < p->n_rownodes; b += kMaxGrid) {
@@ -2683,7 +2816,7 @@ template <int FORM, int MAT, typename VT>
 static int launch_family(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
 {
     if (p->npe == 8) return launch_hex8<FORM, MAT, VT>(p, A, s, mfma);
-    return launch_hex20<FORM, MAT, VT>(p, A, s);
+    return launch_hex20<FORM, MAT, VT>(p, A, s, mfma);
 }
 
 template <typename VT>

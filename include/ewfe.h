@@ -59,9 +59,10 @@ enum ewfe_mat_kind {
     EWFE_ST_VENANT_KIRCHHOFF = 3
 };
 /* Precision of the assembled tangent. EWFE_F32 and EWFE_F32_MFMA both store FP32 CSR values; EWFE_F32 forms the
- * element tangents with FP64 FMAs and rounds them when they are stored, EWFE_F32_MFMA forms the tangents of 8-node
- * elements in FP32 on the matrix cores (v_mfma_f32_16x16x4_f32; 20-node elements as EWFE_F32). Only
- * ewfe_eval_assemble takes EWFE_F32_MFMA; every other call takes EWFE_F32 for FP32 values. */
+ * element tangents with FP64 FMAs and rounds them when they are stored, EWFE_F32_MFMA forms them in FP32 on the matrix
+ * cores (v_mfma_f32_16x16x4_f32) for 8-node elements and for 20-node elements assembled through the row-node gather
+ * (20-node plans on the colour path take it as EWFE_F32). Only ewfe_eval_assemble takes EWFE_F32_MFMA; every other
+ * call takes EWFE_F32 for FP32 values. */
 enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1, EWFE_F32_MFMA = 2 };
 
 /* Error codes */
