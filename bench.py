@@ -177,7 +177,7 @@ def cpu_baseline(el, mat, mp, X, conn, U_np, dU, ns, desc, target_s=12.0, state_
                 single_thread_split_s={"elements": t1[1], "CSR generation": t1[2]})
 
 
-def cfg1_newton():
+def plain_10cube_newton():
     """The plain CPU-runnable configuration in full: a 10^3 C3D8 unit cube, linear elastic E = 210000, nu = 0.3,
     bottom face fixed, top face uz = +1e-3 in one increment, Newton to a 1e-10 flux tolerance: the oracle's Newton
     (element loop + serial scatters in C, scipy LU with refinement; one host thread) against NISTGPU."""
@@ -540,8 +540,8 @@ def main():
     del asm, K64, K32
     torch.cuda.empty_cache()
 
-    # the other configurations' legs: the 64^3 J2 evaluation (cfg2), the 40^3 C3D20TL evaluation (cfg4) and the
-    # plain 10^3 configuration's full Newton (cfg1), each with its CPU baseline
+    # the other workloads' legs: the 64^3 J2 evaluation, the 40^3 C3D20TL evaluation and the plain 10^3
+    # linear elastic cube's full Newton, each with its CPU baseline
     legs = None
     if not args.no_legs:
         legs = {}
@@ -557,7 +557,7 @@ def main():
         ms2 = _timed_evals(asm2, torch.as_tensor(U2n + dU2, device=dev), torch.as_tensor(dU2, device=dev), P2, F2, K2,
                            args.steps, args.warmup)
         B2 = algorithmic_bytes_per_element(len(conn2), m2.nNodes, asm2.nnz, 7, 8, 8)
-        legs["cfg2_c3d8_j2_64cube"] = {
+        legs["c3d8_j2_64cube"] = {
             "elements": len(conn2), "ms_per_evaluation_fp64": ms2, "value": len(conn2) / (ms2 * 1e-3),
             "unit": "elements/s", "roofline_frac": B2 * len(conn2) / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(
@@ -566,7 +566,7 @@ def main():
         del asm2, K2
         torch.cuda.empty_cache()
         r4 = c3d20_tl_measure(40, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
-        legs["cfg4_c3d20tl_neohooke_40cube"] = {
+        legs["c3d20tl_neohooke_40cube"] = {
             "elements": r4["n_el"], "ms_per_evaluation_fp64": r4["ms64"], "ms_per_evaluation_fp32": r4["ms32"],
             "ms_per_evaluation_fp32_mfma": r4["ms32_mfma"], "ms_per_evaluation_fp64_colour_path": r4["ms64_colour"],
             "value": r4["n_el"] / (r4["ms64"] * 1e-3), "unit": "elements/s",
@@ -574,7 +574,7 @@ def main():
             "executed_flop_frac": r4["flop_exec"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
             "cpu_baseline": r4["cpu"]}
         torch.cuda.empty_cache()
-        legs["cfg1_plain_10cube_newton"] = cfg1_newton()
+        legs["plain_c3d8_le_10cube_newton"] = plain_10cube_newton()
         print(f"legs: {json.dumps({k: {kk: vv for kk, vv in v.items() if kk != 'cpu_baseline'} for k, v in legs.items()})}",
               file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
