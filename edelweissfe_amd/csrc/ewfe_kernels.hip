@@ -1254,6 +1254,32 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 // descriptor and one blob (incident elements + contribution lists), both read in gather order, so a node
 // costs three dependent global loads: descriptor, blob, element blocks.
 // ------------------------------------------------------------------------------------------------
+// A row node's 3 CSR rows are n = 3 L contiguous values. With FP64 values the gathers stage them in LDS shifted by
+// row_shift(row) values, so that a value's LDS index and its global index agree modulo the 16-byte vector width,
+// and write them in 16-byte units with the unaligned ends value by value (non-temporal: the CSR values are not read
+// again in the call); lane l of nl >= 8 lanes takes part. FP32 values are stored value by value: staging them was
+// measured slower (the rows are half the bytes, the two extra barriers cost more than the merged lines gain).
+template <typename VT>
+__device__ __forceinline__ int row_shift(const VT *row)
+{
+    return (int)((reinterpret_cast<uintptr_t>(row) / sizeof(VT)) % (16 / sizeof(VT)));
+}
+
+template <typename VT>
+__device__ __forceinline__ void write_rows(VT *row, const VT *st, int n, int l, int nl)
+{
+    constexpr int VW = 16 / (int)sizeof(VT);
+    typedef VT nvec __attribute__((ext_vector_type(VW)));
+    const int head = (VW - row_shift(row)) % VW;  // values before the first 16-byte boundary
+    const int nvecs = (n - head) / VW;            // whole vectors after it
+    const int tail0 = head + nvecs * VW;          // first value after the last whole vector
+    for (int w = l; w < nvecs; w += nl)
+        __builtin_nontemporal_store(*reinterpret_cast<const nvec *>(st + head + w * VW),
+                                    reinterpret_cast<nvec *>(row + head + w * VW));
+    if (l < head) __builtin_nontemporal_store(st[l], row + l);
+    if (l >= VW && l - VW < n - tail0) __builtin_nontemporal_store(st[tail0 + l - VW], row + tail0 + l - VW);
+}
+
 constexpr int kMaxDeg8 = 64;
 constexpr int kMaxInc8 = 31;
 constexpr int kMaxBlobWords8 = kMaxInc8 + (kMaxDeg8 + 1 + 8 * kMaxInc8 + 3) / 4;
@@ -1265,7 +1291,7 @@ __global__ __launch_bounds__(64) void k_gather8(const int4 *__restrict__ gdesc, 
                                                 VT *__restrict__ vals, double *__restrict__ P, double *__restrict__ F,
                                                 int64_t q_begin, int64_t count, int64_t xcd_chunk)
 {
-    __shared__ double s_blk[2][64 * kBlk];  // per half: blocks K_ab (oriented for the row node a) of 8 elements
+    __shared__ __align__(16) double s_blk[2][64 * kBlk];  // per half: blocks K_ab (oriented for node a) of 8 elements
     __shared__ double s_fe[2][8 * 3];
     __shared__ int32_t s_blob[2][kMaxBlobWords8 + 1];
     __shared__ int32_t s_nk[2];
@@ -1351,6 +1377,13 @@ __global__ __launch_bounds__(64) void k_gather8(const int4 *__restrict__ gdesc, 
         }
         __syncthreads();
     }
+    // the node's 3 CSR rows: FP64 staged in its half's block area (free after the last chunk's barrier) and
+    // written in 16-byte units
+    VT *const row = vals + base;
+    VT *const st = reinterpret_cast<VT *>(s_blk[h]) + row_shift(row);
+    static_assert(16 / sizeof(VT) - 1 + 9 * kMaxDeg8 <= 64 * kBlk * sizeof(double) / sizeof(VT),
+                  "a node's rows must fit its half's block area");
+    constexpr bool staged = sizeof(VT) == 8;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
         const int m = hl + 32 * u;
@@ -1358,8 +1391,17 @@ __global__ __launch_bounds__(64) void k_gather8(const int4 *__restrict__ gdesc, 
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int kk = 0; kk < 3; ++kk) vals[(int64_t)base + i * L + 3 * m + kk] = (VT)acc[u][3 * i + kk];
+                for (int kk = 0; kk < 3; ++kk) {
+                    if (staged)
+                        st[i * L + 3 * m + kk] = (VT)acc[u][3 * i + kk];
+                    else
+                        row[i * L + 3 * m + kk] = (VT)acc[u][3 * i + kk];
+                }
         }
+    }
+    if constexpr (staged) {
+        __syncthreads();
+        if (live) write_rows(row, st, 3 * L, hl, 32);
     }
     if (live && hl < 3) {
         P[d + hl] = f;
@@ -1443,19 +1485,17 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
         }
     }
     __syncthreads();
-    if (!live) return;  // no barrier below
     const int c0 = 1 + lo0 + (i0 < N0 - 1), c1 = 1 + lo1 + (i1 < N1 - 1), c2 = 1 + lo2 + (i2 < N2 - 1);
     const int deg = c0 * c1 * c2, L = 3 * deg;
     const int m = hl;
-    if (m < deg) {
+    const int64_t base = 9 * (axis_prefix(i0, N0) * (3 * N1 - 2) * (3 * N2 - 2) +
+                              (int64_t)c0 * axis_prefix(i1, N1) * (3 * N2 - 2) + (int64_t)c0 * c1 * axis_prefix(i2, N2));
+    double acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+    if (live && m < deg) {
         // 2) neighbour offset of rank m; its contributions in ascending element (= corner) order
         const int d0 = m / (c1 * c2) - lo0, d1 = (m / c2) % c1 - lo1, d2 = m % c2 - lo2;
-        const int64_t base = 9 * (axis_prefix(i0, N0) * (3 * N1 - 2) * (3 * N2 - 2) +
-                                  (int64_t)c0 * axis_prefix(i1, N1) * (3 * N2 - 2) +
-                                  (int64_t)c0 * c1 * axis_prefix(i2, N2));
-        double acc[9];
-#pragma unroll
-        for (int t = 0; t < 9; ++t) acc[t] = 0.0;
 #pragma unroll
         for (int o = 0; o < 8; ++o) {
             const int o0 = o >> 2, o1 = (o >> 1) & 1, o2 = o & 1;
@@ -1468,13 +1508,8 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
                 for (int t = 0; t < 9; ++t) acc[t] += (double)src[t];
             }
         }
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk)  // non-temporal: the CSR values are not read again in this call
-                __builtin_nontemporal_store((VT)acc[3 * i + kk], vals + base + i * L + 3 * m + kk);
     }
-    if (m >= 29) {  // P and F of component m - 29, in ascending element order
+    if (live && m >= 29) {  // P and F of component m - 29, in ascending element order
         const int c = m - 29;
         double f = 0.0, fa = 0.0;
 #pragma unroll
@@ -1488,6 +1523,25 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
         }
         P[3 * r + c] = f;
         F[3 * r + c] = fa;
+    }
+    // 3) the node's 3 CSR rows: FP64 staged in its half's block area and written in 16-byte units
+    VT *const row = vals + base;
+    if constexpr (sizeof(VT) == 8) {
+        static_assert(16 / sizeof(VT) - 1 + 3 * 3 * 27 <= 64 * SB, "a node's rows must fit its half's block area");
+        VT *const st = s_blk[h] + row_shift(row);
+        __syncthreads();  // every lane is done reading the blocks and forces
+        if (live && m < deg)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) st[i * L + 3 * m + kk] = (VT)acc[3 * i + kk];
+        __syncthreads();
+        if (live) write_rows(row, st, 3 * L, hl, 32);
+    } else if (live && m < deg) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) __builtin_nontemporal_store((VT)acc[3 * i + kk], row + i * L + 3 * m + kk);
     }
 }
 
@@ -2224,25 +2278,45 @@ __global__ __launch_bounds__(64) void k_gather20(const int4 *__restrict__ gdesc,
     }
     __syncthreads();
     const int L = 3 * deg;
+    double acc[2][9];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc[u][t] = 0.0;
         const int m = lane + 64 * u;
         if (m < deg) {
-            double acc[9];
-#pragma unroll
-            for (int t = 0; t < 9; ++t) acc[t] = 0.0;
             for (int c = cs[m]; c < cs[m + 1]; ++c) {
                 const int code = ent[c];
                 const double *src = s_blk + ((code >> 5) * 20 + (code & 31)) * kBlk;
 #pragma unroll
-                for (int t = 0; t < 9; ++t) acc[t] += src[t];
+                for (int t = 0; t < 9; ++t) acc[u][t] += src[t];
             }
+        }
+    }
+    // the node's 3 CSR rows: FP64 staged in the block area and written in 16-byte units
+    VT *const row = vals + base;
+    constexpr bool staged = sizeof(VT) == 8;
+    VT *const st = reinterpret_cast<VT *>(s_blk) + row_shift(row);
+    static_assert(16 / sizeof(VT) - 1 + 9 * kMaxDeg20 <= 20 * kMaxInc20 * kBlk * sizeof(double) / sizeof(VT),
+                  "a node's rows must fit the block area");
+    if constexpr (staged) __syncthreads();  // every lane is done reading the blocks
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int m = lane + 64 * u;
+        if (m < deg)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    __builtin_nontemporal_store((VT)acc[3 * i + k], vals + (int64_t)base + i * L + 3 * m + k);
-        }
+                for (int k = 0; k < 3; ++k) {
+                    if (staged)
+                        st[i * L + 3 * m + k] = (VT)acc[u][3 * i + k];
+                    else
+                        __builtin_nontemporal_store((VT)acc[u][3 * i + k], row + i * L + 3 * m + k);
+                }
+    }
+    if constexpr (staged) {
+        __syncthreads();
+        write_rows(row, st, 3 * L, lane, 64);
     }
     if (lane < 3) {  // P and F of component lane, in ascending element order
         double f = 0.0, fa = 0.0;
