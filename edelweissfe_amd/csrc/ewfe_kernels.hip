@@ -464,41 +464,29 @@ __device__ __forceinline__ void add_to(VT *vals, int64_t idx, double v)
     vals[idx] = vals[idx] + (VT)v;
 }
 
-// Phase 1 shared by both element families: one quadrature point. Streams over the nodes staged in LDS
-// (accessor nd(f, a) gives field f = X0..2, U0..2, dU0..2 of node a), writes per-node gradients through
-// put_g(a, j, v) / put_p(a, j, v) and returns wd = w detJ plus the recipe. For the small-strain form
-// p = g and put_p is not used.
-template <int NN, int FORM, int MAT, typename ND, typename PG, typename GG, typename PP>
-__device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd, PG put_g, GG get_g, PP put_p,
-                                         const EvalArgs &A, const MatParams &M, double Epar, int64_t pos, int q,
-                                         int NQ, Recipe &r,
-                                         double &wd, const double *state_loaded = nullptr,
-                                         double *state_held = nullptr)
+// Phase 1 of the 8-node kernel, one quadrature point: the kinematics from the nodes staged in LDS (accessor
+// nd(f, a) gives field f = X0..2, U0..2, dU0..2 of node a; dN(a, i) the parametric shape derivatives at the
+// point) and the material update. Returns the inverse Jacobian
+// Ji (the gradients are g_a = Ji dN_a), for the total-Lagrangian form also F^-1, wd = w detJ and the recipe.
+// Nothing is written to LDS here: the caller stores the gradients once every lane has read the staged nodes.
+template <int NN, int FORM, int MAT, typename DN, typename ND>
+__device__ __forceinline__ bool hex8_point(DN dN, ND nd, const MatParams &M, double Epar, const double *state_loaded, double *state_held,
+                                           double Ji[3][3], double Fi[3][3], Recipe &r, double &wd)
 {
-    // converged state first: its loads do not depend on anything below (or the caller has loaded it:
-    // state_loaded = plastic strain 6, kappa)
     double ep[6] = {0, 0, 0, 0, 0, 0}, kn = 0.0;
     if (FORM == FORM_SS && MAT == M_J2) {
-        const int64_t n = A.n_el;
-        if (state_loaded) {
-            for (int s = 0; s < 6; ++s) ep[s] = state_loaded[s];
-            kn = state_loaded[6];
-        } else {
-            for (int s = 0; s < 6; ++s) ep[s] = A.state_n[((int64_t)s * NQ + q) * n + pos];
-            kn = A.state_n[((int64_t)6 * NQ + q) * n + pos];
-        }
+        for (int s = 0; s < 6; ++s) ep[s] = state_loaded[s];
+        kn = state_loaded[6];
     }
-    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int a = 0; a < NN; ++a)
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) J[i][j] += dN[a][i] * nd(j, a);
-    const double detJ = inv3(J, Ji);
-    wd = w * detJ;
+            for (int j = 0; j < 3; ++j) J[i][j] += dN(a, i) * nd(j, a);
+    wd = inv3(J, Ji);  // Gauss weight 1
     double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, dH[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int a = 0; a < NN; ++a) {
         double g[3];
-        for (int j = 0; j < 3; ++j) g[j] = Ji[j][0] * dN[a][0] + Ji[j][1] * dN[a][1] + Ji[j][2] * dN[a][2];
-        for (int j = 0; j < 3; ++j) put_g(a, j, g[j]);
+        for (int j = 0; j < 3; ++j) g[j] = Ji[j][0] * dN(a, 0) + Ji[j][1] * dN(a, 1) + Ji[j][2] * dN(a, 2);
         for (int i = 0; i < 3; ++i) {
             const double u = nd(3 + i, a);
             const double du = nd(6 + i, a);
@@ -514,18 +502,12 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
             le_recipe(M, Epar, H, r);
         } else {
             double ep_t[6], kt;
-            const int64_t n = A.n_el;
             ok = j2_recipe(M, Epar, H, dH, ep, kn, ep_t, kt, r);
-            if (state_held) {  // the caller stores the trial state later
-                for (int s = 0; s < 6; ++s) state_held[s] = ep_t[s];
-                state_held[6] = kt;
-            } else {
-                for (int s = 0; s < 6; ++s) A.state_t[((int64_t)s * NQ + q) * n + pos] = ep_t[s];
-                A.state_t[((int64_t)6 * NQ + q) * n + pos] = kt;
-            }
+            for (int s = 0; s < 6; ++s) state_held[s] = ep_t[s];
+            state_held[6] = kt;
         }
     } else {
-        double F[3][3], Fi[3][3];
+        double F[3][3];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? 1.0 : 0.0);
         const double Jd = inv3(F, Fi);
@@ -533,11 +515,6 @@ __device__ __forceinline__ bool qp_phase(const double (*dN)[3], double w, ND nd,
             svk_recipe(Epar, M.p[1], F, r);
         else
             nh_recipe(Epar, M.p[1], F, Jd, r);
-        // spatial gradients NA_a = F^-T g_a
-        for (int a = 0; a < NN; ++a) {
-            const double g0 = get_g(a, 0), g1 = get_g(a, 1), g2 = get_g(a, 2);
-            for (int m = 0; m < 3; ++m) put_p(a, m, g0 * Fi[0][m] + g1 * Fi[1][m] + g2 * Fi[2][m]);
-        }
     }
     return ok;
 }
@@ -708,14 +685,17 @@ struct RecordArgs {
     int64_t q_begin, q_end, xcd_batches;
 };
 
-// LDS layout of the per-quadrature-point data of k_hex8_elem, per field f: 64 doubles, one per (element le,
-// quadrature point q). Fields: per-node gradients g (and p for the total-Lagrangian form), 3 per node, then
-// the tangent coefficients alpha wd, beta wd [, gamma wd for the total-Lagrangian form; gamma = beta for small
-// strain] [, c wd and the flow direction N for J2] and the stress S wd.
-template <int FORM, int MAT>
+// LDS layout of the per-quadrature-point data of k_hex8_elem, per field f: 64 values, one per (element le,
+// quadrature point q), doubles for FP64 records, floats for FP32 records (the precision of the tangent
+// arithmetic). Fields: per-node gradients g (and p for the total-Lagrangian form), 3 per node, then the tangent
+// coefficients alpha wd, beta wd [, gamma wd for the total-Lagrangian form; gamma = beta for small strain]
+// [, c wd and the flow direction N for J2] [, the stress S wd for St. Venant-Kirchhoff, whose tangent has a
+// stress term; KEEP_S keeps the six fields as staging space for the matrix-core tangent].
+template <int FORM, int MAT, bool KEEP_S = false>
 struct Hex8Layout {
     // HAS_N: the per-point 6-vector slot NV (the J2 flow direction, or B = F F^T for St. Venant-Kirchhoff)
     static constexpr bool TL = FORM == FORM_TL, HAS_N = MAT == M_J2 || MAT == M_SVK;
+    static constexpr bool HAS_S = KEEP_S || MAT == M_SVK;
     static constexpr int G = 0;
     static constexpr int P = TL ? 24 : 0;  // aliases g for small strain
     static constexpr int AL = TL ? 48 : 24;
@@ -723,15 +703,17 @@ struct Hex8Layout {
     static constexpr int GA = TL ? AL + 2 : BE;
     static constexpr int CC = TL ? AL + 3 : AL + 2;  // only with HAS_N
     static constexpr int NV = CC + 1;                // 6, only with HAS_N
-    static constexpr int S = HAS_N ? NV + 6 : CC;    // 6
-    static constexpr int NF = S + 6;
+    static constexpr int S = HAS_N ? NV + 6 : CC;    // 6, only with HAS_S
+    static constexpr int NF = HAS_S ? S + 6 : S;
 };
 
 // A 64-lane ds_*_b64 is served 16 lanes at a time by 32 four-byte banks, i.e. 16 double slots. Phase 1 writes
 // a field with lanes (le, q) (fixed field), phase 2 reads node fields with lanes (le, a) at a fixed q, where
 // lane a reads node b = (a + j) mod 8: a 16-lane group (le in {2g, 2g + 1}) then touches all 8 nodes b. Node
 // fields are XOR-swizzled by 2 b inside their 16-slot group so that both access patterns hit 16 different
-// slots; scalar fields are read as broadcasts (one address per element) and need no swizzle.
+// slots; scalar fields are read as broadcasts (one address per element) and need no swizzle. With float fields
+// (ds_*_b32, 32 lanes at a time over the 32 banks) the same indices are conflict-free too: lanes 0..31 are
+// (le, node) = (0..3, 0..7) and land on 16 (le >> 1) + 2 (q ^ b) + (le & 1), 32 different banks.
 __device__ __forceinline__ int slot_scalar(int f, int q, int le) { return f * 64 + 16 * (le >> 1) + 2 * q + (le & 1); }
 __device__ __forceinline__ int slot_node(int base, int b, int i, int q, int le)
 {
@@ -746,19 +728,19 @@ __device__ __forceinline__ int slot_node(int base, int b, int i, int q, int le)
 // T and Tb are symmetric rank-8 (rank-16 for J2) products, formed by v_mfma_f32_16x16x4_f32 on the three 16x16
 // tiles (0,0), (0,1), (1,1) that cover the upper triangle of the 24x24 matrix (rows and columns 24..31 are
 // zero), lane quarter g supplying points q = g and g + 4; S is one more 16x16 tile (rows 8..15 zero) with the 24
-// products (q, j) as its inner dimension: 24 MFMAs for J2, 18 otherwise, per element. The nodal forces stay on
-// the vector units in FP64, exactly as in the FP64 variant (P and F are bitwise the same). T + G and Tb are
-// staged in LDS in the record's slot order, each slot written by one lane, and summed when the record is
-// written out (deterministic). One element at a time; the stress fields of LDS, no longer needed after the
-// forces, hold the staging.
+// products (q, j) as its inner dimension: 24 MFMAs for J2, 18 otherwise, per element. The nodal forces fa come
+// from phase 1 in FP64, exactly as in the other variants (P and F are bitwise the same). T + G and Tb are staged
+// in LDS in the record's slot order, each slot written by one lane, and summed when the record is written out
+// (deterministic). One element at a time; the stress fields of the layout (not written by phase 1) hold the
+// staging.
 template <int FORM, int MAT, typename VT>
 __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const RecordArgs &R, double *s_qp,
                                                   int32_t *s_el, int le, int t, bool active, int64_t q0, int64_t el,
-                                                  const double (&st_t)[7])
+                                                  const double (&fa)[3], const double (&st_t)[7])
 {
     typedef float f4 __attribute__((ext_vector_type(4)));
     constexpr int NQ = 8, EPB = 8;
-    using L = Hex8Layout<FORM, MAT>;
+    using L = Hex8Layout<FORM, MAT, true>;
     constexpr int PB = L::TL ? L::P : L::G;  // the gradient the tangent's p terms use
     const int lane = threadIdx.x;
     const int a = t;
@@ -767,25 +749,12 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
     auto ld = [&](int f, int q, int e) { return s_qp[slot_scalar(f, q, e)]; };
     auto ldg = [&](int base, int b, int i, int q, int e) { return s_qp[slot_node(base, b, i, q, e)]; };
 
-    // (A) lane (le, a): f_a in FP64, the same arithmetic as the FP64 variant
-    double fa[3] = {0, 0, 0};
-    if (active) {
-        for (int q = 0; q < NQ; ++q) {
-            double pa[3];
-            for (int i = 0; i < 3; ++i) pa[i] = ldg(PB, a, i, q, le);
-            const double S0 = ld(L::S + 0, q, le), S1 = ld(L::S + 1, q, le), S2 = ld(L::S + 2, q, le);
-            const double S3 = ld(L::S + 3, q, le), S4 = ld(L::S + 4, q, le), S5 = ld(L::S + 5, q, le);
-            fa[0] += S0 * pa[0] + S3 * pa[1] + S4 * pa[2];
-            fa[1] += S3 * pa[0] + S1 * pa[1] + S5 * pa[2];
-            fa[2] += S4 * pa[0] + S5 * pa[1] + S2 * pa[2];
-        }
-    }
     if (t == 0) s_el[le] = (int32_t)el;
     if (active)
         for (int i = 0; i < 3; ++i) R.fe[el * 24 + 3 * a + i] = fa[i];
     if (active && FORM == FORM_SS && MAT == M_J2)
         for (int s = 0; s < 7; ++s) A.state_t[((int64_t)s * NQ + t) * n_el + pos] = st_t[s];
-    __syncthreads();  // every lane is done with the stress fields
+    __syncthreads();  // s_el is written
     float *s_ta = reinterpret_cast<float *>(s_qp + L::S * 64);  // T + G in record slot order (300 values)
     float *s_tb = s_ta + kRec8;                                   // Tb in record slot order
     float *s_sg = s_tb + kRec8;                                   // S_ab of the pairs a <= b (36 values)
@@ -903,15 +872,18 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
 // MT: the plan holds a material table (a separate instantiation, so that the single-material path carries no
 // per-element parameter selects)
 // MF: FP32 records with the tangent formed on the matrix cores (only with VT = float)
+// The per-point data in LDS and the tangent arithmetic of phase 2 have the precision of the records (PT = VT);
+// the matrix-core tangent keeps FP64 point data and rounds its operands when it packs them.
 template <int FORM, int MAT, typename VT, bool MT, bool MF>
 __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
 {
     constexpr int NN = 8, NQ = 8, EPB = 8;
-    using L = Hex8Layout<FORM, MAT>;
+    using PT = typename std::conditional<MF, double, VT>::type;
+    using L = Hex8Layout<FORM, MAT, MF>;
     constexpr int QP_WORDS = L::NF * 64;
-    constexpr int HALF_STAGE_WORDS = ((EPB / 2) * kRec8 * (int)sizeof(VT) + 7) / 8;
-    static_assert(HALF_STAGE_WORDS <= QP_WORDS, "record staging must fit in the quadrature-point area");
-    __shared__ __align__(16) double s_qp[QP_WORDS];
+    static_assert((EPB / 2) * kRec8 * sizeof(VT) <= QP_WORDS * sizeof(PT),
+                  "record staging must fit in the quadrature-point area");
+    __shared__ __align__(16) PT s_qp[QP_WORDS];
     __shared__ int32_t s_el[EPB];
     const int lane = threadIdx.x;
     const int le = lane >> 3, t = lane & 7;
@@ -924,13 +896,12 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
     const int64_t n = A.n_el;
     const int64_t el = active ? (int64_t)A.order[pos] : 0;  // records and nodal forces are stored by element
 
-    // phase 1: lane (le, t) first stages node t of its element (X, U (, dU)) in LDS, in the fields that
-    // phase 1 writes last (the tangent coefficients and the stress); then lane (le, q = t) evaluates one
-    // quadrature point reading the element's nodes from there. One node per lane keeps the nodal data out
-    // of the registers.
+    // phase 1a: lane (le, t) stages node t of its element (X, U (, dU)) in LDS, over the gradient fields (they are
+    // written only in phase 1b); then lane (le, q = t) evaluates one quadrature point reading the element's nodes
+    // from there. One node per lane keeps the nodal data out of the registers.
     constexpr int NVN = (MAT == M_J2) ? 9 : 6;  // staged values per node
-    static_assert(EPB * NN * NVN <= (L::NF - L::AL) * 64, "nodal staging must fit in the last fields");
-    double *s_nd = s_qp + L::AL * 64;
+    static_assert(EPB * NN * NVN * sizeof(double) <= 24 * 64 * sizeof(PT), "nodal staging must fit in the gradients");
+    double *s_nd = reinterpret_cast<double *>(s_qp);
     const MatParams M = MT ? mat_at(A, active ? pos : q0) : A.mat;
     double Epar = M.p[0];
     if (active && A.el_param) Epar = A.el_param[A.order[pos]];
@@ -964,40 +935,76 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
         }
     }
     __syncthreads();
-    Recipe r;
+    const int q = t;
+    // parametric derivatives of the trilinear shape functions at Gauss point q (the points in the order of the
+    // host tables; the same arithmetic, so the same values), recomputed where they are used
+    const double gp = 1.0 / sqrt(3.0);
+    const double xq0 = q < 4 ? -gp : gp, xq1 = (q & 2) ? gp : -gp, xq2 = ((q ^ (q >> 1)) & 1) ? gp : -gp;
+    auto dN = [&](int b, int i) {
+        const double c0 = ((b & 3) == 1 || (b & 3) == 2) ? 1.0 : -1.0;
+        const double c1 = ((b & 3) >= 2) ? 1.0 : -1.0;
+        const double c2 = (b >= 4) ? 1.0 : -1.0;
+        const double f0 = 1 + c0 * xq0, f1 = 1 + c1 * xq1, f2 = 1 + c2 * xq2;
+        return i == 0 ? 0.125 * c0 * f1 * f2 : (i == 1 ? 0.125 * f0 * c1 * f2 : 0.125 * f0 * f1 * c2);
+    };
+    Recipe r = {};
     double wd = 0.0;
+    double Ji[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Fi[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     if (active) {
-        const int q = t;
         const double *src = s_nd + le * NN * NVN;
         auto nd = [&](int f, int a) { return (MAT != M_J2 && f >= 6) ? 0.0 : src[a * NVN + f]; };
-        auto put_g = [&](int a, int j, double v) { s_qp[slot_node(L::G, a, j, q, le)] = v; };
-        auto get_g = [&](int a, int j) { return s_qp[slot_node(L::G, a, j, q, le)]; };
-        auto put_p = [&](int a, int j, double v) { s_qp[slot_node(L::P, a, j, q, le)] = v; };
-        // parametric derivatives of the trilinear shape functions at Gauss point q (the points in the order of
-        // the host tables; the same arithmetic, so the same values)
-        double dNq[NN][3];
-        {
-            const double g = 1.0 / sqrt(3.0);
-            const double x0 = q < 4 ? -g : g, x1 = (q & 2) ? g : -g, x2 = ((q ^ (q >> 1)) & 1) ? g : -g;
-#pragma unroll
-            for (int b = 0; b < NN; ++b) {
-                const double c0 = ((b & 3) == 1 || (b & 3) == 2) ? 1.0 : -1.0;
-                const double c1 = ((b & 3) >= 2) ? 1.0 : -1.0;
-                const double c2 = (b >= 4) ? 1.0 : -1.0;
-                const double f0 = 1 + c0 * x0, f1 = 1 + c1 * x1, f2 = 1 + c2 * x2;
-                dNq[b][0] = 0.125 * c0 * f1 * f2;
-                dNq[b][1] = 0.125 * f0 * c1 * f2;
-                dNq[b][2] = 0.125 * f0 * f1 * c2;
-            }
-        }
-        const bool ok =
-            qp_phase<NN, FORM, MAT>(dNq, 1.0, nd, put_g, get_g, put_p, A, M, Epar, pos, q, NQ, r, wd, st, st_t);
+        const bool ok = hex8_point<NN, FORM, MAT>(dN, nd, M, Epar, st, st_t, Ji, Fi, r, wd);
         if (!ok) atomic_min_pos(A.pnewdt, 0.5);
     }
     __syncthreads();  // every lane has read the staged nodes
-    if (active) {
-        const int q = t;
-        auto put = [&](int f, double v) { s_qp[slot_scalar(f, q, le)] = v; };
+
+    // phase 1b: lane (le, q) stores its point's gradients and tangent coefficients in LDS, in the tangent's
+    // precision, and forms the point's contributions wd S p_a to the nodal forces in FP64. The forces are summed
+    // over the element's 8 points by a fixed reduce-scatter over its 8 lanes (lane distance 4, 2, 1), after which
+    // lane (le, a) holds f_a; the order is fixed, so P and F are bitwise the same in every precision variant.
+    // Inactive elements run it on zeros (the exchanges stay inside an element's lanes).
+    double wS[6];
+    for (int i = 0; i < 6; ++i) wS[i] = wd * r.S[i];
+    auto grad = [&](int a, double c[3]) {
+        double g[3], p[3];
+        for (int j = 0; j < 3; ++j) g[j] = Ji[j][0] * dN(a, 0) + Ji[j][1] * dN(a, 1) + Ji[j][2] * dN(a, 2);
+        for (int j = 0; j < 3; ++j) s_qp[slot_node(L::G, a, j, q, le)] = (PT)g[j];
+        if (FORM == FORM_TL) {  // spatial gradients p_a = F^-T g_a
+            for (int m = 0; m < 3; ++m) p[m] = g[0] * Fi[0][m] + g[1] * Fi[1][m] + g[2] * Fi[2][m];
+            for (int m = 0; m < 3; ++m) s_qp[slot_node(L::P, a, m, q, le)] = (PT)p[m];
+        } else {
+            for (int m = 0; m < 3; ++m) p[m] = g[m];
+        }
+        c[0] = wS[0] * p[0] + wS[3] * p[1] + wS[4] * p[2];
+        c[1] = wS[3] * p[0] + wS[1] * p[1] + wS[5] * p[2];
+        c[2] = wS[4] * p[0] + wS[5] * p[1] + wS[2] * p[2];
+    };
+    const bool hi4 = (t & 4) != 0, hi2 = (t & 2) != 0, hi1 = (t & 1) != 0;
+    double h4[4][3];  // lane distance 4: sums of two points for the nodes (t & 4) | j
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        double c0[3], c1[3];
+        grad(j, c0);
+        grad(j + 4, c1);
+        for (int i = 0; i < 3; ++i) {
+            const double keep = hi4 ? c1[i] : c0[i], send = hi4 ? c0[i] : c1[i];
+            h4[j][i] = keep + __shfl_xor(send, 4, 8);
+        }
+    }
+    double h2[2][3];  // lane distance 2: sums of four points for the nodes (t & 6) | k
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < 3; ++i) {
+            const double keep = hi2 ? h4[2 + k][i] : h4[k][i], send = hi2 ? h4[k][i] : h4[2 + k][i];
+            h2[k][i] = keep + __shfl_xor(send, 2, 8);
+        }
+    double fa[3];  // lane distance 1: node t, all 8 points
+    for (int i = 0; i < 3; ++i) {
+        const double keep = hi1 ? h2[1][i] : h2[0][i], send = hi1 ? h2[0][i] : h2[1][i];
+        fa[i] = keep + __shfl_xor(send, 1, 8);
+    }
+    {
+        auto put = [&](int f, double v) { s_qp[slot_scalar(f, q, le)] = (PT)v; };
         put(L::AL, wd * r.alpha);
         put(L::BE, wd * r.beta);
         if (L::TL) put(L::GA, wd * r.gam);
@@ -1005,23 +1012,23 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             put(L::CC, wd * r.c);
             for (int i = 0; i < 6; ++i) put(L::NV + i, r.N[i]);
         }
-        for (int i = 0; i < 6; ++i) put(L::S + i, wd * r.S[i]);
+        if (MAT == M_SVK)
+            for (int i = 0; i < 6; ++i) put(L::S + i, wS[i]);
     }
     __syncthreads();
 
     if constexpr (MF) {
         static_assert(sizeof(VT) == 4, "the matrix-core tangent writes FP32 records");
-        hex8_tangent_mfma<FORM, MAT, VT>(A, R, s_qp, s_el, le, t, active, q0, el, st_t);
+        hex8_tangent_mfma<FORM, MAT, VT>(A, R, s_qp, s_el, le, t, active, q0, el, fa, st_t);
         return;
     }
-    // phase 2 (FP64 tangent): lane (le, a=t) builds blocks K_ab, b = a + j (mod 8), j < nb, and f_a
+    // phase 2: lane (le, a = t) builds the blocks K_ab, b = a + j (mod 8), j < nb, in the precision PT
     const int a = t;
     const int nb = (a < 4) ? 5 : 4;
-    double K[5][3][3];
+    PT K[5][3][3];
     for (int j = 0; j < 5; ++j)
         for (int i = 0; i < 3; ++i)
-            for (int k = 0; k < 3; ++k) K[j][i][k] = 0.0;
-    double fa[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k) K[j][i][k] = 0;
     auto ld = [&](int f, int q) { return s_qp[slot_scalar(f, q, le)]; };
     auto ldg = [&](int base, int b, int i, int q) { return s_qp[slot_node(base, b, i, q, le)]; };
     if (active && FORM == FORM_SS && MAT == M_LE) {
@@ -1029,21 +1036,14 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
         // single sum T_ab = sum_q mu wd g_a g_b^T:  K_ab = (lambda / mu) T + T^T + tr(T) I
 #pragma unroll 2
         for (int q = 0; q < NQ; ++q) {
-            const double muwd = ld(L::BE, q);
-            const double ga0 = ldg(L::G, a, 0, q), ga1 = ldg(L::G, a, 1, q), ga2 = ldg(L::G, a, 2, q);
-            const double wa[3] = {muwd * ga0, muwd * ga1, muwd * ga2};
-            {
-                const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
-                const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
-                fa[0] += S0 * ga0 + S3 * ga1 + S4 * ga2;
-                fa[1] += S3 * ga0 + S1 * ga1 + S5 * ga2;
-                fa[2] += S4 * ga0 + S5 * ga1 + S2 * ga2;
-            }
+            const PT muwd = ld(L::BE, q);
+            const PT ga0 = ldg(L::G, a, 0, q), ga1 = ldg(L::G, a, 1, q), ga2 = ldg(L::G, a, 2, q);
+            const PT wa[3] = {muwd * ga0, muwd * ga1, muwd * ga2};
 #pragma unroll
             for (int j = 0; j < 5; ++j) {
                 if (j < nb) {
                     const int b = (a + j) & 7;
-                    const double gb0 = ldg(L::G, b, 0, q), gb1 = ldg(L::G, b, 1, q), gb2 = ldg(L::G, b, 2, q);
+                    const PT gb0 = ldg(L::G, b, 0, q), gb1 = ldg(L::G, b, 1, q), gb2 = ldg(L::G, b, 2, q);
                     for (int i = 0; i < 3; ++i) {
                         K[j][i][0] += wa[i] * gb0;
                         K[j][i][1] += wa[i] * gb1;
@@ -1053,46 +1053,45 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             }
         }
         const double nu = M.p[1];
-        const double ratio = 2.0 * nu / (1.0 - 2.0 * nu);  // lambda / mu
+        const PT ratio = (PT)(2.0 * nu / (1.0 - 2.0 * nu));  // lambda / mu
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
-            double T[3][3];
+            PT T[3][3];
             for (int i = 0; i < 3; ++i)
                 for (int k = 0; k < 3; ++k) T[i][k] = K[j][i][k];
-            const double tr = T[0][0] + T[1][1] + T[2][2];
+            const PT tr = T[0][0] + T[1][1] + T[2][2];
             for (int i = 0; i < 3; ++i)
-                for (int k = 0; k < 3; ++k) K[j][i][k] = ratio * T[i][k] + T[k][i] + (i == k ? tr : 0.0);
+                for (int k = 0; k < 3; ++k) K[j][i][k] = ratio * T[i][k] + T[k][i] + (i == k ? tr : (PT)0);
         }
     } else if (active && MAT == M_SVK) {
         // St. Venant-Kirchhoff (total Lagrangian):
         // K_ab,ik = alpha q_a,i q_b,k + beta q_a,k q_b,i + beta (r_a . r_b) B_ik + delta_ik (p_a . S p_b),
         // q = B p, with alpha = lambda wd, beta = mu wd, S = wd tau, B unscaled
         for (int q = 0; q < NQ; ++q) {
-            const double al = ld(L::AL, q), be = ld(L::BE, q);
-            double pa[3], ra[3], Bm[3][3], Sm[3][3];
+            const PT al = ld(L::AL, q), be = ld(L::BE, q);
+            PT pa[3], ra[3], Bm[3][3], Sm[3][3];
             for (int i = 0; i < 3; ++i) {
                 ra[i] = ldg(L::G, a, i, q);
                 pa[i] = ldg(L::P, a, i, q);
             }
             {
-                const double b0 = ld(L::NV + 0, q), b1 = ld(L::NV + 1, q), b2 = ld(L::NV + 2, q);
-                const double b3 = ld(L::NV + 3, q), b4 = ld(L::NV + 4, q), b5 = ld(L::NV + 5, q);
+                const PT b0 = ld(L::NV + 0, q), b1 = ld(L::NV + 1, q), b2 = ld(L::NV + 2, q);
+                const PT b3 = ld(L::NV + 3, q), b4 = ld(L::NV + 4, q), b5 = ld(L::NV + 5, q);
                 Bm[0][0] = b0, Bm[1][1] = b1, Bm[2][2] = b2;
                 Bm[0][1] = Bm[1][0] = b3, Bm[0][2] = Bm[2][0] = b4, Bm[1][2] = Bm[2][1] = b5;
-                const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
-                const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
+                const PT S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
+                const PT S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
                 Sm[0][0] = S0, Sm[1][1] = S1, Sm[2][2] = S2;
                 Sm[0][1] = Sm[1][0] = S3, Sm[0][2] = Sm[2][0] = S4, Sm[1][2] = Sm[2][1] = S5;
             }
-            double qa[3], tpa[3];
+            PT qa[3], tpa[3];
             for (int i = 0; i < 3; ++i) {
                 qa[i] = Bm[i][0] * pa[0] + Bm[i][1] * pa[1] + Bm[i][2] * pa[2];
                 tpa[i] = Sm[i][0] * pa[0] + Sm[i][1] * pa[1] + Sm[i][2] * pa[2];
-                fa[i] += tpa[i];  // internal force f_a = S p_a
             }
             {  // diagonal block: symmetric, upper triangle accumulated and mirrored after the loop
-                const double rr = be * (ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
-                const double d1 = pa[0] * tpa[0] + pa[1] * tpa[1] + pa[2] * tpa[2];
+                const PT rr = be * (ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
+                const PT d1 = pa[0] * tpa[0] + pa[1] * tpa[1] + pa[2] * tpa[2];
                 for (int i = 0; i < 3; ++i)
                     for (int k = i; k < 3; ++k) {
                         K[0][i][k] = fma((al + be) * qa[i], qa[k], K[0][i][k]);
@@ -1104,14 +1103,14 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             for (int j = 1; j < 5; ++j) {
                 if (j < nb) {
                     const int b = (a + j) & 7;
-                    double pb[3], rb[3], qb[3];
+                    PT pb[3], rb[3], qb[3];
                     for (int i = 0; i < 3; ++i) {
                         rb[i] = ldg(L::G, b, i, q);
                         pb[i] = ldg(L::P, b, i, q);
                     }
                     for (int i = 0; i < 3; ++i) qb[i] = Bm[i][0] * pb[0] + Bm[i][1] * pb[1] + Bm[i][2] * pb[2];
-                    const double rr = be * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
-                    const double d1 = tpa[0] * pb[0] + tpa[1] * pb[1] + tpa[2] * pb[2];
+                    const PT rr = be * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                    const PT d1 = tpa[0] * pb[0] + tpa[1] * pb[1] + tpa[2] * pb[2];
                     for (int i = 0; i < 3; ++i)
                         for (int k = 0; k < 3; ++k) {
                             K[j][i][k] = fma(al * qa[i], qb[k], K[j][i][k]);
@@ -1126,9 +1125,9 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             for (int k = 0; k < i; ++k) K[0][i][k] = K[0][k][i];
     } else if (active) {
         for (int q = 0; q < NQ; ++q) {
-            const double al = ld(L::AL, q), be = ld(L::BE, q), ga = ld(L::GA, q);
-            const double cc = L::HAS_N ? ld(L::CC, q) : 0.0;
-            double pa[3], ra[3], va[3] = {0, 0, 0}, Nq[6];
+            const PT al = ld(L::AL, q), be = ld(L::BE, q), ga = ld(L::GA, q);
+            const PT cc = L::HAS_N ? ld(L::CC, q) : (PT)0;
+            PT pa[3], ra[3], va[3] = {0, 0, 0}, Nq[6];
             for (int i = 0; i < 3; ++i) {
                 ra[i] = ldg(L::G, a, i, q);
                 pa[i] = L::TL ? ldg(L::P, a, i, q) : ra[i];
@@ -1139,14 +1138,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                 va[1] = Nq[3] * pa[0] + Nq[1] * pa[1] + Nq[5] * pa[2];
                 va[2] = Nq[4] * pa[0] + Nq[5] * pa[1] + Nq[2] * pa[2];
             }
-            {  // internal force f_a = S p_a
-                const double S0 = ld(L::S + 0, q), S1 = ld(L::S + 1, q), S2 = ld(L::S + 2, q);
-                const double S3 = ld(L::S + 3, q), S4 = ld(L::S + 4, q), S5 = ld(L::S + 5, q);
-                fa[0] += S0 * pa[0] + S3 * pa[1] + S4 * pa[2];
-                fa[1] += S3 * pa[0] + S1 * pa[1] + S5 * pa[2];
-                fa[2] += S4 * pa[0] + S5 * pa[1] + S2 * pa[2];
-            }
-            double Aa[3], Ba[3], Va[3];
+            PT Aa[3], Ba[3], Va[3];
             for (int i = 0; i < 3; ++i) {
                 Aa[i] = al * pa[i];
                 Ba[i] = be * pa[i];
@@ -1154,8 +1146,8 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             }
             {  // diagonal block K_aa = (alpha + beta) p_a p_a^T + gamma |r_a|^2 I - c v_a v_a^T: symmetric, the
                // upper triangle is accumulated and mirrored after the loop
-                const double dd = ga * (ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
-                double sa[3];
+                const PT dd = ga * (ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
+                PT sa[3];
                 for (int i = 0; i < 3; ++i) sa[i] = Aa[i] + Ba[i];
                 for (int i = 0; i < 3; ++i)
                     for (int k = i; k < 3; ++k) {
@@ -1168,7 +1160,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
             for (int j = 1; j < 5; ++j) {
                 if (j < nb) {
                     const int b = (a + j) & 7;
-                    double pb[3], rb[3], vb[3] = {0, 0, 0};
+                    PT pb[3], rb[3], vb[3] = {0, 0, 0};
                     for (int i = 0; i < 3; ++i) {
                         rb[i] = ldg(L::G, b, i, q);
                         pb[i] = L::TL ? ldg(L::P, b, i, q) : rb[i];
@@ -1178,7 +1170,7 @@ __global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
                         vb[1] = Nq[3] * pb[0] + Nq[1] * pb[1] + Nq[5] * pb[2];
                         vb[2] = Nq[4] * pb[0] + Nq[5] * pb[1] + Nq[2] * pb[2];
                     }
-                    const double d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+                    const PT d = ga * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
                     for (int i = 0; i < 3; ++i)
                         for (int k = 0; k < 3; ++k) {
                             K[j][i][k] = fma(Aa[i], pb[k], K[j][i][k]);
