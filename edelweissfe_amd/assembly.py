@@ -186,7 +186,8 @@ class ElementAssembly(PlanLinearAlgebra):
         """Fills P (= sum Pe), F (= sum |Pe|), the CSR values and the trial state. Returns pNewDT (device
         tensor, 1e36 or < 1 when a quadrature point asks for a cutback). FP32 values: ``mfma=True`` forms the
         tangents in FP32 on the matrix cores (8-node elements, and 20-node elements assembled through the row-node
-        gather), otherwise with FP64 FMAs rounded when stored."""
+        gather), otherwise in FP32 arithmetic on the vector units (8-node elements, materials other than J2) or
+        with FP64 FMAs rounded when stored (J2, 20-node elements)."""
         for t in (U_np, dU, P, F):
             if t.numel() != self.nDof or t.dtype != self.torch.float64 or not t.is_contiguous():
                 raise ValueError("dof vectors must be contiguous float64 tensors of length nDof")

@@ -13,11 +13,12 @@
 // MI355X design
 //  * C3D8 family, two passes. Pass 1 (k_hex8_elem): a 64-lane workgroup holds 8 elements. Nodal X/U/dU are
 //    staged in LDS (one node per lane), one quadrature point per lane runs the Jacobian, gradients and the
-//    material update and leaves the per-point tangent "recipe" in LDS, then one node per lane builds the
-//    3x3 blocks K_ab for b in {a, a+1, a+2, a+3 (, a+4 if a<4)} mod 8 -- the 36 blocks of the symmetric 24x24
-//    Ke, each exactly once -- and the element's record (36 blocks + nodal forces) is written contiguously.
-//    Pass 2 (k_gather8s on boxGen grids, k_gather8 elsewhere): per row node, the blocks of its incident
-//    elements are summed in ascending element order and the node's 3 CSR rows, P and F are written once.
+//    material update, forms its contributions to the nodal forces (summed over the element's 8 lanes in a fixed
+//    order) and leaves the per-point tangent "recipe" in LDS, in the precision of the records; then one node per
+//    lane builds the 3x3 blocks K_ab for b in {a, a+1, a+2, a+3 (, a+4 if a<4)} mod 8 -- the 36 blocks of the
+//    symmetric 24x24 Ke, each exactly once -- and the element's record (36 blocks + nodal forces) is written
+//    contiguously. Pass 2 (k_gather8s on boxGen grids, k_gather8 elsewhere): per row node, the blocks of its
+//    incident elements are summed in ascending element order and the node's 3 CSR rows, P and F are written once.
 //    Deterministic FP64 without atomics: every output is one lane's sum in a fixed order.
 //  * Quadrature-point state is structure-of-arrays [component][qp][plan position], so consecutive lanes
 //    touch consecutive addresses.
@@ -25,9 +26,11 @@
 //    B^T C B:  K_ab,ik = sum_q [ alpha p_a,i p_b,k + beta p_a,k p_b,i + gamma delta_ik (r_a . r_b)
 //                                - c v_a,i v_b,k ] ,  v = N p
 //    with p = r = dN/dx (small strain) or p = dN/dx (spatial), r = dN/dX (total Lagrangian).
-//  * C3D20 family: one element per 64-lane workgroup, colour by colour (no two elements of a colour share a
-//    node, so the read-add-write into the CSR values has a fixed order); lanes 0..26 run the quadrature
-//    points, lanes 0..59 build one row (node a, component i) of Ke each.
+//  * C3D20 family (k_hex20): one element per 64-lane workgroup, four lanes per quadrature point, then lanes
+//    (node a, group) build the blocks of their node pairs; the element's record is written for the row-node
+//    gather (k_gather20, the same ascending-element sums as pass 2 above). Meshes beyond the gather's limits take
+//    the colour path instead: colour by colour (no two elements of a colour share a node), each element adds its
+//    blocks into the CSR values, which fixes the order of every sum.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -686,8 +689,8 @@ struct RecordArgs {
 };
 
 // LDS layout of the per-quadrature-point data of k_hex8_elem, per field f: 64 values, one per (element le,
-// quadrature point q), doubles for FP64 records, floats for FP32 records (the precision of the tangent
-// arithmetic). Fields: per-node gradients g (and p for the total-Lagrangian form), 3 per node, then the tangent
+// quadrature point q), in the precision of the tangent arithmetic (doubles, or floats for the FP32 tangent of the
+// materials other than J2). Fields: per-node gradients g (and p for the total-Lagrangian form), 3 per node, then the tangent
 // coefficients alpha wd, beta wd [, gamma wd for the total-Lagrangian form; gamma = beta for small strain]
 // [, c wd and the flow direction N for J2] [, the stress S wd for St. Venant-Kirchhoff, whose tangent has a
 // stress term; KEEP_S keeps the six fields as staging space for the matrix-core tangent].
@@ -869,16 +872,33 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
     }
 }
 
+// Precision of the per-point data and of the tangent arithmetic of k_hex8_elem: that of the records, except for
+// J2, whose FP32 records hold the FP64 tangent rounded once. Its consistent tangent has the deviatoric stiffness
+// 2G - a2 - a1 (about 2H/3, 1/240 of 2G for the benchmark's hardening) along the flow direction as a difference of
+// terms of size 2G; formed in FP32 it carries ~240 times the rounding of its terms in exactly the mode that governs
+// Newton's convergence (the 128^3 J2 schedule needed 12 instead of 10 solves). The matrix-core tangent keeps FP64
+// point data and rounds its operands when it packs them.
+template <int MAT, typename VT, bool MF>
+using hex8_point_t = typename std::conditional<MF || MAT == M_J2, double, VT>::type;
+
+// occupancy the compiler is asked for: float point data and accumulators fit 3 waves per SIMD (4 waves need
+// register spills)
+template <int MAT, typename VT, bool MF>
+constexpr int hex8_min_waves()
+{
+    return sizeof(hex8_point_t<MAT, VT, MF>) == 4 ? 3 : 1;
+}
+
 // MT: the plan holds a material table (a separate instantiation, so that the single-material path carries no
 // per-element parameter selects)
 // MF: FP32 records with the tangent formed on the matrix cores (only with VT = float)
-// The per-point data in LDS and the tangent arithmetic of phase 2 have the precision of the records (PT = VT);
-// the matrix-core tangent keeps FP64 point data and rounds its operands when it packs them.
+// PT: the precision of the per-point data in LDS and of the tangent arithmetic of phase 2 (hex8_point_t)
 template <int FORM, int MAT, typename VT, bool MT, bool MF>
-__global__ __launch_bounds__(64) void k_hex8_elem(EvalArgs A, RecordArgs R)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hex8_min_waves<MAT, VT, MF>()))) void k_hex8_elem(
+    EvalArgs A, RecordArgs R)
 {
     constexpr int NN = 8, NQ = 8, EPB = 8;
-    using PT = typename std::conditional<MF, double, VT>::type;
+    using PT = hex8_point_t<MAT, VT, MF>;
     using L = Hex8Layout<FORM, MAT, MF>;
     constexpr int QP_WORDS = L::NF * 64;
     static_assert((EPB / 2) * kRec8 * sizeof(VT) <= QP_WORDS * sizeof(PT),
@@ -1856,7 +1876,8 @@ __global__ __launch_bounds__(64) void k_adjoint8(EvalArgs A, const double *__res
 // ------------------------------------------------------------------------------------------------
 // REC: write the element's record (unique blocks + nodal forces) for the row-node gather instead of adding into
 // the CSR values (then the launch needs no colouring)
-// MF (FP32 records only): the tangent formed on the matrix cores (v_mfma_f32_16x16x4_f32) instead of FP64 FMAs.
+// MF (FP32 records only): the tangent formed on the matrix cores (v_mfma_f32_16x16x4_f32) instead of FP64 FMAs
+// (without MF, FP32 records hold the FP64 tangent rounded when it is stored).
 template <int FORM, int MAT, typename VT, int NQ, bool REC, bool MF = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_hex20(EvalArgs A, RecordArgs R)
 {

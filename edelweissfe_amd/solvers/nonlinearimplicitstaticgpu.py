@@ -26,8 +26,9 @@ Differences that are part of the design (not of the semantics):
     the multigrid from 20,000 dofs on when the mesh allows it, block Jacobi otherwise), converged to
     ``pcgTolerance`` relative residual; the Newton loop's own flux/correction tolerances decide convergence
     exactly as in the reference;
-  * ``tangentPrecision="fp32"`` stores the tangent in FP32 (element tangents from FP64 FMAs, rounded when
-    stored), ``"fp32-mfma"`` also forms the element tangents of 8-node elements in FP32 on the matrix cores;
+  * ``tangentPrecision="fp32"`` stores the tangent in FP32 (element tangents in FP32 arithmetic for 8-node
+    elements of the hyperelastic and linear elastic materials; from FP64 FMAs, rounded when stored, for J2 and for
+    20-node elements), ``"fp32-mfma"`` forms the element tangents in FP32 on the matrix cores;
     residual, return mapping and update stay FP64 in both. Newton then converges to the same FP64 residual
     tolerance, possibly in more iterations; its
     linear systems are solved to ``pcgToleranceFP32`` (default 1e-4): an FP32 tangent already limits the
