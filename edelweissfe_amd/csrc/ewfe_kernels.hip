@@ -422,6 +422,16 @@ __device__ __forceinline__ void svk_recipe(double E, double nu, const double F[3
     r.c = 0;
 }
 
+// Workgroup barrier that orders LDS accesses only. The element kernels share data between their lanes through LDS
+// alone; __syncthreads() also orders global memory, i.e. waits until every store issued before it has completed
+// (s_waitcnt vmcnt(0)), which would hold the record staging behind the previous stores.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ------------------------------------------------------------------------------------------------
 // kernel arguments
 // ------------------------------------------------------------------------------------------------
@@ -757,7 +767,7 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
         for (int i = 0; i < 3; ++i) R.fe[el * 24 + 3 * a + i] = fa[i];
     if (active && FORM == FORM_SS && MAT == M_J2)
         for (int s = 0; s < 7; ++s) A.state_t[((int64_t)s * NQ + t) * n_el + pos] = st_t[s];
-    __syncthreads();  // s_el is written
+    lds_barrier();  // s_el is written
     float *s_ta = reinterpret_cast<float *>(s_qp + L::S * 64);  // T + G in record slot order (300 values)
     float *s_tb = s_ta + kRec8;                                   // Tb in record slot order
     float *s_sg = s_tb + kRec8;                                   // S_ab of the pairs a <= b (36 values)
@@ -838,7 +848,7 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
                 const int ra = 4 * g + v;
                 if (ra <= c) s_sg[pair8(ra, c)] = acc[6][v];
             }
-        __syncthreads();
+        lds_barrier();
         // accumulator u % 3 = tile (0,0), (0,1), (1,1); lane (g, c) holds rows 16 R + 4 g + v, column 16 C + c.
         // Values with m <= n < 24 are the record's: block (a, b) = (m / 3, n / 3), a <= b. T + G goes to the slot of
         // K_ab[i][k] (i = m % 3, k = n % 3); the Tb value (m, n) = ((a, x), (b, y)) belongs to K_ab[y][x].
@@ -858,7 +868,7 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         // the record = staged T + G plus staged Tb: 75 16-byte units, non-temporal (read back by the gather pass)
         constexpr int NW = kRec8 / 4;
         static_assert(kRec8 % 4 == 0, "records are whole 16-byte units");
@@ -867,7 +877,7 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
             const f4 sum = reinterpret_cast<const f4 *>(s_ta)[w] + reinterpret_cast<const f4 *>(s_tb)[w];
             __builtin_nontemporal_store(sum, reinterpret_cast<f4 *>(dst) + w);
         }
-        __syncthreads();  // the staging is free for the next element
+        lds_barrier();  // the staging is free for the next element
         cur = nxt;
     }
 }
@@ -954,7 +964,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hex8_min_wav
             dst[8] = d2;
         }
     }
-    __syncthreads();
+    lds_barrier();
     const int q = t;
     // parametric derivatives of the trilinear shape functions at Gauss point q (the points in the order of the
     // host tables; the same arithmetic, so the same values), recomputed where they are used
@@ -976,7 +986,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hex8_min_wav
         const bool ok = hex8_point<NN, FORM, MAT>(dN, nd, M, Epar, st, st_t, Ji, Fi, r, wd);
         if (!ok) atomic_min_pos(A.pnewdt, 0.5);
     }
-    __syncthreads();  // every lane has read the staged nodes
+    lds_barrier();  // every lane has read the staged nodes
 
     // phase 1b: lane (le, q) stores its point's gradients and tangent coefficients in LDS, in the tangent's
     // precision, and forms the point's contributions wd S p_a to the nodal forces in FP64. The forces are summed
@@ -1035,7 +1045,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hex8_min_wav
         if (MAT == M_SVK)
             for (int i = 0; i < 6; ++i) put(L::S + i, wS[i]);
     }
-    __syncthreads();
+    lds_barrier();
 
     if constexpr (MF) {
         static_assert(sizeof(VT) == 4, "the matrix-core tangent writes FP32 records");
@@ -1216,7 +1226,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hex8_min_wav
     VT *stage = reinterpret_cast<VT *>(s_qp);
     const int valid = (R.q_end - q0) < EPB ? (int)(R.q_end - q0) : EPB;
     for (int half = 0; half < 2; ++half) {
-        __syncthreads();  // every lane is done with s_qp (phase 2) or with the previous half's copy-out
+        lds_barrier();  // every lane is done with s_qp (phase 2) or with the previous half's copy-out
         if (active && (le >> 2) == half) {
             const int ls = le & 3;
 #pragma unroll
@@ -1239,7 +1249,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hex8_min_wav
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         const int nv = valid - 4 * half;  // elements of this half
         // 16-byte copies: records start at multiples of 16 bytes and hold a multiple of 16 bytes
         constexpr int VW = 16 / (int)sizeof(VT);
@@ -1939,7 +1949,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 #pragma unroll 1
     for (int q0 = 0; q0 < NQ; q0 += QC) {
         const int nq = NQ - q0 < QC ? NQ - q0 : QC;
-        __syncthreads();  // the nodes are staged, or the previous chunk's points are summed
+        lds_barrier();  // the nodes are staged, or the previous chunk's points are summed
         // four lanes per point: lane (ql, s) = 4 ql + s; the groups of points past nq stay idle as a whole
         const int ql = lane >> 2, sq = lane & 3;
         if (ql < nq) {
@@ -1962,7 +1972,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 for (int m = 0; m < 6; ++m) put(L::S + m, wd * r.S[m]);
             }
         }
-        __syncthreads();
+        lds_barrier();
         auto ld = [&](int f, int q) { return s_qp[f * QC + q]; };
         if constexpr (MF) {
             // forces in FP64 (the FP64 variant's arithmetic): row lane (a, i)
@@ -2135,7 +2145,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         // (I, J). Values (m, n), m <= n < 60, are the record's: block (a, b) = (m / 3, n / 3), a <= b. T - C (+ S_ab on
         // i == k) goes to the slot of K_ab[i][k] (i = m % 3, k = n % 3); Tb value (m, n) belongs to K_ab[k][i]. Each
         // slot is written by one lane in each half, and the halves are summed when the record is written out.
-        __syncthreads();  // every lane is done with the point data
+        lds_barrier();  // every lane is done with the point data
         float *sT = reinterpret_cast<float *>(s_qp), *sB = sT + kRec20, *sS = sB + kRec20;
         const int64_t el = A.order[pos];
         if (row_lane) R.fe[el * 60 + lane] = fi;
@@ -2148,7 +2158,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 if (ra <= cn && cn < NN) sS[ra * NN + cn] = accS[u][v];
             }
         }
-        __syncthreads();
+        lds_barrier();
         int t = 0;
 #pragma unroll
         for (int I = 0; I < 4; ++I)
@@ -2166,7 +2176,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                         sB[offd ? block_off20(am, bn) + 3 * kn + im : block_off20(am, am) + kSym6[im][kn]] = accB[t][v];
                     }
                 }
-        __syncthreads();
+        lds_barrier();
         constexpr int NW = kRec20 / 4;
         static_assert(kRec20 % 4 == 0, "records are whole 16-byte units");
         f4 *dst = reinterpret_cast<f4 *>(reinterpret_cast<float *>(R.kbuf) + el * kRec20);
@@ -2180,7 +2190,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         // record of element el staged in LDS (the point data are no longer read once every lane passed the
         // barrier): the blocks as the record holds them (K_ab row-major for a < b, K_ba = K_ab^T otherwise, the
         // upper triangle of K_aa), then written out in whole 16-byte units; the nodal forces f_(a,i), lane 3a + i
-        __syncthreads();
+        lds_barrier();
         VT *s_rec = reinterpret_cast<VT *>(s_qp);
         const int64_t el = A.order[pos];
         if (row_lane) {
@@ -2206,7 +2216,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         static_assert((kRec20 * (int)sizeof(VT)) % 16 == 0, "records are whole 16-byte units");
         constexpr int NW = kRec20 * (int)sizeof(VT) / 16;
         typedef float u4 __attribute__((ext_vector_type(4)));  // 16 bytes moved as they are
