@@ -35,6 +35,9 @@ Differences that are part of the design (not of the semantics):
     Newton contraction to ~1e-4..1e-3 per iteration, so a tighter correction buys nothing, while the
     convergence test (FP64 residual and correction) is unchanged;
   * the 1e-7 absolute floor of the flux test is the option ``fluxResidualFloor`` (default 1e-7);
+  * ``multigridSetupPerIncrement=True`` sets the multigrid hierarchy up on the first solve of an increment only: the
+    later corrections of the increment are preconditioned by that hierarchy (its prolongator, coarse operators,
+    block-diagonal inverse and spectral bound), with the current matrix in the fine-level smoother and in CG;
   * ``linsolverRefinement=k`` adds k steps of iterative refinement to every FP64 linear solve: the residual
     r = b - K x is formed with row sums in twice the working precision (ewfe_residual) and K d = r is solved to
     the same PCG tolerance for the correction x += d. A PCG stopped at a relative residual of 1e-14 leaves an
@@ -108,6 +111,7 @@ class NISTGPU:
         "pcgToleranceFP32": 1e-4,
         "pcgMaxIter": 50000,
         "linsolverRefinement": 0,
+        "multigridSetupPerIncrement": False,
         "fluxResidualFloor": 1e-7,
         "device": "cuda",
     }
@@ -132,6 +136,8 @@ class NISTGPU:
         self.system = None
         self.history = []  # per converged increment: list of (|R|inf, |ddU|inf) per iteration
         self.linearSolveIterations = []
+        self.multigridSetups = 0  # numeric multigrid setups so far
+        self._mgSetUpThisIncrement = False
 
     def _updateOptions(self, updatedOptions, journal):
         for k, v in updatedOptions.items():
@@ -266,6 +272,7 @@ class NISTGPU:
     def solveIncrement(self, U_n, dU, P, K, stepActions, model, timeStep, prevTimeStep, extrapolation, maxIter,
                        maxGrowingIter):
         iterationCounter = 0
+        self._mgSetUpThisIncrement = False
         incrementResidualHistory = dict.fromkeys(self.theDofManager.idcsOfFieldsInDofVector, (0.0, 0))
         self._currentHistory = []
         dm = self.theDofManager
@@ -418,8 +425,10 @@ class NISTGPU:
         # iteration at these condition numbers: its corrections are solved only to pcgToleranceFP32
         rtol = self.options["pcgToleranceFP32"] if A.dtype == self.torch.float32 else self.options["pcgTolerance"]
         amg = self._linsolver == "amg"
-        if amg:
+        if amg and not (self.options["multigridSetupPerIncrement"] and self._mgSetUpThisIncrement):
             self.system.multigridSetup(A, rows)
+            self.multigridSetups += 1
+            self._mgSetUpThisIncrement = True
             self.torch.cuda.synchronize(self.system.device)
             toc = _time.perf_counter()
             self.computationTimes["multigrid setup"] += toc - tic

@@ -92,6 +92,26 @@ def test_reference_decks_with_amg(name):
     assert isinstance(solver, NISTGPU) and all(it > 0 for it, _ in solver.linearSolveIterations if it)
 
 
+@pytest.mark.parametrize("tangent", ["fp64", "fp32"])
+def test_multigrid_setup_per_increment(tangent):
+    """The multigrid hierarchy set up on the first solve of each increment and reused for the increment's later
+    corrections (multigridSetupPerIncrement): the plastic C3D8 cantilever deck (J2, several Newton iterations per
+    increment) still reproduces U.ref and the oracle's Newton solution, with fewer setups than solves."""
+    print(f"deck C3D8 amg, setup per increment, {tangent} tangent", flush=True)
+    fx, U, solver = _deck_run("C3D8", tangent, solver_options={"linsolver": "amg",
+                                                               "multigridSetupPerIncrement": True})
+    assert np.abs(U - np.array(fx["U_ref"])).max() < 1e-6
+    X, conn, sets = decks.mesh(fx, O.boxgen)
+    r = O.solve_step(dict(el_kind=fx["el_type"], mat_kind=fx["material"], mp=fx["mp"], X=X, conn=conn),
+                     decks.step(fx, sets))
+    assert rel(U, r["U"]) < (1e-9 if tangent == "fp64" else 1e-7)
+    solves = sum(1 for it, _ in solver.linearSolveIterations if it)
+    increments_with_solves = sum(1 for h in solver.history if any(d > 0 for _, d in h))
+    print(f"{solves} solves in {increments_with_solves} increments, {solver.multigridSetups} setups", flush=True)
+    # one setup per increment that solved (and per discarded attempt), fewer than the solves
+    assert increments_with_solves <= solver.multigridSetups < solves
+
+
 def test_amg_pcg_on_20_node_mesh():
     """The multigrid on a C3D20 mesh (corner nodes couple with 81 nodes: the wide fine-level A P kernel): the
     solution of a Neo-Hooke tangent system matches scipy's direct solve, in far fewer iterations than block
