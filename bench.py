@@ -56,15 +56,17 @@ def algorithmic_bytes_per_element(n_el, n_nodes, nnz, n_state, n_qp, val_bytes, 
             + val_bytes * nnz / n_el)
 
 
-def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, linsolver="amg", setup_per_increment=True):
+def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, linsolver="amg", setup_per_increment=True,
+                    fine_precision="matrix"):
     """The J2 cube's load case as configured: from the virgin state to `strain` uniaxial strain in `increments`
     equal increments (after the reference's zero-size first increment, adaptivetimestepper.py:90-134; NIST's default
     linear extrapolation), solved by NISTGPU: uz = 0 on z = 0, uz = strain on z = 1, ux = uy = 0 at (0,0,0), uy = 0
     at (1,0,0). Newton to the reference's criteria |R|inf <= 1e-8 * averaged flux (without the 1e-7 absolute
     floor) and |ddU|inf < 1e-8, linear solves to the relative PCG tolerance `pcg_tol`; with `setup_per_increment`
     the multigrid hierarchy is set up on the first solve of each increment and reused for its later corrections
-    (otherwise on every solve). Returns timings per phase, iterations per increment and PCG iterations; plan and
-    multigrid-pattern construction (once per mesh) are reported apart."""
+    (otherwise on every solve); `fine_precision="fp16"` runs the V-cycle's fine-level passes on a half-precision copy of
+    the matrix. Returns timings per phase, iterations per increment and PCG iterations; plan and multigrid-pattern
+    construction (once per mesh) are reported apart."""
     import torch
 
     from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU, _Journal
@@ -84,7 +86,7 @@ def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, lin
                "fluxResidualToleranceAlternative": {"displacement": 1e-8}}
     solver = NISTGPU(jobInfo, _Journal(verbose=False, stream=sys.stderr), tangentPrecision=tangent,
                      pcgTolerance=pcg_tol, pcgToleranceFP32=pcg_tol, fluxResidualFloor=0.0, linsolver=linsolver,
-                     multigridSetupPerIncrement=setup_per_increment)
+                     multigridSetupPerIncrement=setup_per_increment, multigridFinePrecision=fine_precision)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solver.solveStep(step, m)
@@ -96,7 +98,8 @@ def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, lin
     U = m.U.detach().cpu().numpy()
     t_setup = solver.computationTimes["setup"]
     return dict(seconds=t - t_setup, setup_seconds=t_setup, pcg_tolerance=pcg_tol,
-                multigrid_setup="per increment" if setup_per_increment else "per solve", increments=len(its) - 1,
+                multigrid_setup="per increment" if setup_per_increment else "per solve",
+                multigrid_fine_passes=fine_precision, increments=len(its) - 1,
                 newton_iterations=newton_iters, iterations_per_increment=its, linear_solves=len(solves),
                 pcg_iterations_total=int(sum(solves)),
                 ms_per_newton_iteration=1e3 * (t - t_setup) / max(1, newton_iters),
@@ -589,21 +592,26 @@ def main():
         for tan in ("fp64", "fp32", "fp32-mfma"):
             newton_schedule(8, tan, 1e-4, increments=2, linsolver=args.linsolver)
         torch.cuda.empty_cache()
-        # FP64 at several PCG tolerances (the fastest one that meets the Newton tolerance is the baseline), the FP32
-        # tangent at its precision-matched tolerance, and the matrix-core FP32 tangent, all with the multigrid set up
-        # once per increment; the FP64 and FP32 runs with a setup on every solve are the controls
-        runs = [("fp64", 1e-10, True), ("fp64", 1e-6, True), ("fp64", args.pcg_tol_fp32, True),
-                ("fp32", args.pcg_tol_fp32, True), ("fp64", 1e-6, False), ("fp32", args.pcg_tol_fp32, False)]
+        # plain FP64 at several PCG tolerances (the fastest one that meets the Newton tolerance is the baseline), the
+        # FP32 tangent at its precision-matched tolerance, and the matrix-core FP32 tangent, all with the multigrid set
+        # up once per increment. Controls: a setup on every solve; both precisions with a half-precision copy of the
+        # matrix in the V-cycle's fine-level passes (a mixed-precision preconditioner, so not the plain FP64 path)
+        runs = [("fp64", 1e-10, True, "matrix"), ("fp64", 1e-6, True, "matrix"),
+                ("fp64", args.pcg_tol_fp32, True, "matrix"), ("fp32", args.pcg_tol_fp32, True, "matrix"),
+                ("fp64", 1e-6, False, "matrix"), ("fp32", args.pcg_tol_fp32, False, "matrix")]
         if not args.no_newton_controls:
-            runs.append(("fp32-mfma", args.pcg_tol_fp32, True))
+            runs += [("fp32-mfma", args.pcg_tol_fp32, True, "matrix"), ("fp64", 1e-6, True, "fp16"),
+                     ("fp32", args.pcg_tol_fp32, True, "fp16")]
         res = {}
-        for tan, tol, per_inc in runs:
+        for tan, tol, per_inc, fine in runs:
             r = newton_schedule(nn_, tan, tol, increments=args.newton_increments, linsolver=args.linsolver,
-                                setup_per_increment=per_inc)
-            print(f"newton {tan} (pcg tol {tol:g}, multigrid setup {r['multigrid_setup']}): {r['seconds']:.3f}s, "
-                  f"iterations per increment {r['iterations_per_increment']}, {r['linear_solves']} solves, "
-                  f"{r['pcg_iterations_total']} PCG iterations, phases {r['phases_s']}", file=sys.stderr, flush=True)
-            res[f"{tan}_pcg{tol:g}" + ("" if per_inc else "_setup_per_solve")] = r
+                                setup_per_increment=per_inc, fine_precision=fine)
+            print(f"newton {tan} (pcg tol {tol:g}, multigrid setup {r['multigrid_setup']}, fine-level passes {fine}): "
+                  f"{r['seconds']:.3f}s, iterations per increment {r['iterations_per_increment']}, "
+                  f"{r['linear_solves']} solves, {r['pcg_iterations_total']} PCG iterations, phases {r['phases_s']}",
+                  file=sys.stderr, flush=True)
+            key = f"{tan}_pcg{tol:g}" + ("" if per_inc else "_setup_per_solve") + ("_fp16_fine" if fine == "fp16" else "")
+            res[key] = r
             torch.cuda.empty_cache()
         ref64 = res["fp64_pcg1e-10"]
         U64 = ref64["U"]
@@ -612,7 +620,7 @@ def main():
                                                                           np.abs(U64).max())
         for r in res.values():
             r.pop("U")
-        fp64_keys = [k for k in res if k.startswith("fp64")]
+        fp64_keys = [k for k in res if k.startswith("fp64") and not k.endswith("_fp16_fine")]  # plain FP64
         best64 = min(fp64_keys, key=lambda k: res[k]["seconds"])
         fp32_key = f"fp32_pcg{args.pcg_tol_fp32:g}"
         newton = {"mesh": f"{nn_}^3", "load": f"virgin state to 1% uniaxial strain in {args.newton_increments} "
@@ -626,6 +634,9 @@ def main():
         if not args.no_newton_controls:
             newton["speedup_fp32mfma_vs_fastest_fp64"] = (res[best64]["seconds"] /
                                                           res[f"fp32-mfma_pcg{args.pcg_tol_fp32:g}"]["seconds"])
+            # the same comparison when both precisions use the half-precision fine-level V-cycle passes
+            newton["speedup_fp32_vs_fp64_both_fp16_fine"] = (res["fp64_pcg1e-06_fp16_fine"]["seconds"] /
+                                                             res[f"fp32_pcg{args.pcg_tol_fp32:g}_fp16_fine"]["seconds"])
 
     line = {
         "metric": "C3D8 J2 elem evals+assembly/s",

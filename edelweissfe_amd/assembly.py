@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import EWFE_F32, EWFE_F32_MFMA, EWFE_F64, check, lib, ptr
+from ._lib import EWFE_F16, EWFE_F32, EWFE_F32_MFMA, EWFE_F64, check, lib, ptr
 
 
 class PlanLinearAlgebra:
@@ -94,8 +94,11 @@ class PlanLinearAlgebra:
             self.mgLevels = sizes
         return self._mg
 
-    def multigridSetup(self, vals, rows_dev):
-        """Numeric multigrid setup for the row-eliminated matrix `vals` (Dirichlet rows `rows_dev`)."""
+    def multigridSetup(self, vals, rows_dev, finePrecision=None):
+        """Numeric multigrid setup for the row-eliminated matrix `vals` (Dirichlet rows `rows_dev`). With
+        ``finePrecision="fp16"`` the V-cycle's fine-level passes read a half-precision copy of `vals` made here
+        (ewfe_mg_set_fine_precision); otherwise the values of each apply."""
+        check(lib().ewfe_mg_set_fine_precision(self.multigrid(), EWFE_F16 if finePrecision == "fp16" else EWFE_F64))
         n_rows = 0 if rows_dev is None else rows_dev.numel()
         check(lib().ewfe_mg_setup(self.multigrid(), ptr(vals), self._dtype_code(vals), ptr(rows_dev), n_rows,
                                   self._stream()))

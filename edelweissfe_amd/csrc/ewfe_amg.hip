@@ -89,6 +89,11 @@ struct ewfe_mg {
     double ratio = 20.0;
     bool ready = false;
     int setup_dtype = -1;  // precision of the matrix the hierarchy was set up for (= prolongator storage)
+    // fine-level passes of the V-cycle in half precision (ewfe_mg_set_fine_precision): the set-up matrix divided by
+    // a power of two (its largest magnitude then lies in [2^14, 2^15)) and rounded to FP16
+    bool fine16 = false;
+    _Float16 *vals16 = nullptr;
+    double *scale16 = nullptr;  // [2]: the largest magnitude, the scale
     std::vector<void *> allocs;
 };
 
@@ -133,6 +138,7 @@ struct FineOp {
     const VT *vals;
     const uint8_t *mask;
     int64_t n;
+    const double *scale = nullptr;  // half-precision values: [1] is the scale they were divided by
 };
 
 struct BsrOp {
@@ -159,6 +165,12 @@ __device__ __forceinline__ void row_apply(const FineOp<VT> &A, int64_t i, int la
     y0 = group_sum32(a0);
     y1 = group_sum32(a1);
     y2 = group_sum32(a2);
+    if constexpr (std::is_same<VT, _Float16>::value) {
+        const double s = A.scale[1];
+        y0 *= s;
+        y1 *= s;
+        y2 *= s;
+    }
 }
 
 __device__ __forceinline__ void row_apply(const BsrOp &A, int64_t i, int lane, const double *__restrict__ x,
@@ -220,6 +232,47 @@ __global__ void k_mg_mask(const int32_t *rows, int64_t n_rows, uint8_t *mask)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_rows; i += stride) mask[rows[i]] = 1;
+}
+
+// ---- half-precision copy of the fine-level matrix ---------------------------------------------------------------
+// scale16[0] = max |v| (positive doubles order like their bit patterns, so the maximum is an integer atomic max and
+// does not depend on the order of the updates)
+template <typename VT>
+__global__ __launch_bounds__(256) void k_mg_absmax(const VT *__restrict__ v, int64_t n, double *scale16)
+{
+    double m = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        m = fmax(m, fabs((double)v[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(reinterpret_cast<unsigned long long *>(scale16), (unsigned long long)__double_as_longlong(m));
+}
+
+// the scale: a power of two with max |v| / scale in [2^14, 2^15) (well inside FP16's range, 65504)
+__device__ __forceinline__ double half_scale(double vmax)
+{
+    if (!(vmax > 0.0)) return 1.0;
+    int e;
+    (void)frexp(vmax, &e);  // vmax = f 2^e, f in [0.5, 1)
+    return ldexp(1.0, e - 15);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(256) void k_mg_to_half(const VT *__restrict__ v, int64_t n, double *scale16,
+                                                    _Float16 *__restrict__ h)
+{
+    const double s = half_scale(scale16[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) scale16[1] = s;
+    const double inv = 1.0 / s;  // exact: s is a power of two
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        h[i] = (_Float16)((double)v[i] * inv);
+}
+
+__global__ void k_mg_zero2(double *p)
+{
+    if (threadIdx.x < 2) p[threadIdx.x] = 0.0;
 }
 
 template <typename VT>
@@ -887,6 +940,14 @@ int mg_setup(ewfe_mg *mg, const void *vals, const int32_t *rows, int64_t n_rows,
         HIP_CHECK(hipGetLastError());
     }
     RC(coarse_inverse(mg, s));
+    if (mg->fine16) {
+        const int64_t nnz = p->nnz;
+        hipLaunchKernelGGL(k_mg_zero2, dim3(1), dim3(64), 0, s, mg->scale16);
+        hipLaunchKernelGGL(k_mg_absmax<VT>, dim3(2048), dim3(256), 0, s, (const VT *)vals, nnz, mg->scale16);
+        hipLaunchKernelGGL(k_mg_to_half<VT>, dim3(4096), dim3(256), 0, s, (const VT *)vals, nnz, mg->scale16,
+                           mg->vals16);
+        HIP_CHECK(hipGetLastError());
+    }
     mg->ready = true;
     mg->setup_dtype = std::is_same<VT, float>::value ? EWFE_F32 : EWFE_F64;
     return 0;
@@ -939,7 +1000,13 @@ int vcycle(ewfe_mg *mg, int l, const void *vals, const double *done, hipStream_t
                            dc, L.x, lam, ratio, done);
         smooth_steps(dc, dn, op);
     };
-    if (l == 0) {
+    if (l == 0 && mg->fine16) {  // the fine-level passes on the half-precision copy of the set-up matrix
+        FineOp<_Float16> F = fine_op<_Float16>(mg, mg->vals16);
+        F.scale = mg->scale16;
+        level_body(F);
+        RC(vcycle<VT>(mg, l + 1, vals, done, s));
+        post(F);
+    } else if (l == 0) {
         const FineOp<VT> F = fine_op<VT>(mg, vals);
         level_body(F);
         RC(vcycle<VT>(mg, l + 1, vals, done, s));
@@ -1111,6 +1178,27 @@ extern "C" int ewfe_mg_setup(ewfe_mg *mg, const void *vals, int vals_dtype, cons
     mg->ready = false;
     if (vals_dtype == EWFE_F64) return mg_setup<double>(mg, vals, rows_dev, n_rows, s);
     return mg_setup<float>(mg, vals, rows_dev, n_rows, s);
+}
+
+extern "C" int ewfe_mg_set_fine_precision(ewfe_mg *mg, int dtype)
+{
+    if (!mg || (dtype != EWFE_F16 && dtype != EWFE_F32 && dtype != EWFE_F64)) {
+        ewfe::set_error("ewfe_mg_set_fine_precision: invalid arguments");
+        return EWFE_EINVAL;
+    }
+    const bool want = dtype == EWFE_F16;
+    if (want && !mg->vals16) {
+        int prev = 0;
+        HIP_CHECK(hipGetDevice(&prev));
+        HIP_CHECK(hipSetDevice(mg->device));
+        int rc = alloc(mg, mg->vals16, (size_t)mg->plan->nnz);
+        if (!rc) rc = alloc(mg, mg->scale16, (size_t)2);
+        (void)hipSetDevice(prev);
+        if (rc) return rc;
+    }
+    if (want != mg->fine16) mg->ready = false;  // the copy is made by the next ewfe_mg_setup
+    mg->fine16 = want;
+    return 0;
 }
 
 extern "C" int ewfe_mg_lambda(const ewfe_mg *mg, double *lam_out, int n)

@@ -38,6 +38,9 @@ Differences that are part of the design (not of the semantics):
   * ``multigridSetupPerIncrement=True`` sets the multigrid hierarchy up on the first solve of an increment only: the
     later corrections of the increment are preconditioned by that hierarchy (its prolongator, coarse operators,
     block-diagonal inverse and spectral bound), with the current matrix in the fine-level smoother and in CG;
+  * ``multigridFinePrecision="fp16"`` runs the V-cycle's fine-level smoothing and residual passes on a half-precision
+    copy of the matrix the hierarchy was set up for (scaled by a power of two; "matrix", the default, uses the
+    current matrix in its own precision); CG, its residuals and the Newton loop keep the assembled values;
   * ``linsolverRefinement=k`` adds k steps of iterative refinement to every FP64 linear solve: the residual
     r = b - K x is formed with row sums in twice the working precision (ewfe_residual) and K d = r is solved to
     the same PCG tolerance for the correction x += d. A PCG stopped at a relative residual of 1e-14 leaves an
@@ -112,6 +115,7 @@ class NISTGPU:
         "pcgMaxIter": 50000,
         "linsolverRefinement": 0,
         "multigridSetupPerIncrement": False,
+        "multigridFinePrecision": "matrix",
         "fluxResidualFloor": 1e-7,
         "device": "cuda",
     }
@@ -426,7 +430,7 @@ class NISTGPU:
         rtol = self.options["pcgToleranceFP32"] if A.dtype == self.torch.float32 else self.options["pcgTolerance"]
         amg = self._linsolver == "amg"
         if amg and not (self.options["multigridSetupPerIncrement"] and self._mgSetUpThisIncrement):
-            self.system.multigridSetup(A, rows)
+            self.system.multigridSetup(A, rows, finePrecision=self.options["multigridFinePrecision"])
             self.multigridSetups += 1
             self._mgSetUpThisIncrement = True
             self.torch.cuda.synchronize(self.system.device)

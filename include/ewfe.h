@@ -65,7 +65,7 @@ enum ewfe_mat_kind {
  * cores (v_mfma_f32_16x16x4_f32) for 8-node elements and for 20-node elements assembled through the row-node gather
  * (20-node plans on the colour path take it as EWFE_F32). The residual, nodal forces and state stay FP64. Only ewfe_eval_assemble takes EWFE_F32_MFMA; every other
  * call takes EWFE_F32 for FP32 values. */
-enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1, EWFE_F32_MFMA = 2 };
+enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1, EWFE_F32_MFMA = 2, EWFE_F16 = 3 /* ewfe_mg_set_fine_precision only */ };
 
 /* Error codes */
 #define EWFE_OK 0
@@ -254,6 +254,14 @@ int ewfe_mg_setup(ewfe_mg *mg, const void *csr_vals_dev, int vals_dtype, const i
                   void *stream);
 /* lambda_max(D^-1 A) of every level, after ewfe_mg_setup (host array of n >= levels). */
 int ewfe_mg_lambda(const ewfe_mg *mg, double *lambda_out, int n);
+/* Precision of the V-cycle's fine-level passes (smoothing steps and residuals). EWFE_F64 / EWFE_F32 (the default):
+ * the matrix values passed to each apply. EWFE_F16: every later ewfe_mg_setup also stores the set-up matrix divided
+ * by a power of two (its largest magnitude then in [2^14, 2^15)) and rounded to FP16, and the V-cycle's fine-level
+ * passes read that copy (half the bytes of FP32 values); the block-diagonal inverses, spectral bounds, prolongators
+ * and coarse operators still come from the set-up matrix in its own precision, and the CG iteration around the
+ * V-cycle keeps the caller's values. The copy (2 bytes per nonzero) is allocated here, not in the setup. No
+ * counterpart in the reference (its preconditioner is pyamg's, in FP64). */
+int ewfe_mg_set_fine_precision(ewfe_mg *mg, int dtype);
 
 /* ewfe_pcg with the multigrid V-cycle as preconditioner; the hierarchy must be set up for csr_vals_dev.
  * Same arguments, stopping rule, workspace (ewfe_pcg_workspace_size) and errors as ewfe_pcg. */

@@ -19,12 +19,16 @@ pytestmark = pytest.mark.gpu
 J2 = [210000.0, 0.3, 250.0, 1000.0, 0.0, 0.0]
 
 
-def _plastic_system(n, tangent):
-    """Fully plastic J2 tangent of an n^3 jittered cube under uniaxial strain with the bench's Dirichlet set."""
+def _plastic_system(n, tangent, elastic_modulus=None):
+    """Fully plastic J2 tangent of an n^3 jittered cube under uniaxial strain with the bench's Dirichlet set (or, with
+    `elastic_modulus`, the linear elastic tangent for that modulus)."""
     import torch
 
     X, conn, sets = jittered_box(n, "C3D8", seed=3)
-    g = GpuEval("C3D8", "vonmises", J2, X, conn)
+    if elastic_modulus is None:
+        g = GpuEval("C3D8", "vonmises", J2, X, conn)
+    else:
+        g = GpuEval("C3D8", "linearelastic", [elastic_modulus, 0.3], X, conn)
     nd = g.dm.nDof
     base = g.dm.nodeDofBase
     u = np.zeros((len(X), 3))
@@ -76,6 +80,61 @@ def test_amg_pcg_matches_direct_solve(tangent):
     x2 = g.dm.constructDofVector()
     it2, _ = g.asm.pcg(vals, g.vec(b), x2, rows_t, rtol=1e-12, maxit=2000, check_every=4, precond="amg")
     assert it2 == it and bool((x2 == x).all().item())
+
+
+@pytest.mark.parametrize("tangent", ["fp64", "fp32"])
+def test_amg_pcg_half_precision_fine_passes(tangent):
+    """ewfe_mg_set_fine_precision(EWFE_F16): the V-cycle's fine-level smoothing and residual passes read a scaled FP16
+    copy of the set-up matrix. The preconditioner changes, the solution does not: PCG to 1e-12 matches the sparse
+    direct solve, in at most twice the iterations of the default; the same problem in other units (moduli and yield
+    stress 1e6 times larger: entries far beyond FP16's range, 65504, unless the copy is scaled) is solved as well;
+    switching back reproduces the default bitwise."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    print(f"amg pcg, half-precision fine passes, {tangent}", flush=True)
+    g, vals, rows, rows_t, b = _plastic_system(20, tangent)
+    nd = g.dm.nDof
+    x0 = g.dm.constructDofVector()
+    it0, _ = g.asm.pcg(vals, g.vec(b), x0, rows_t, rtol=1e-12, maxit=2000, check_every=4, precond="amg")
+    g.asm.multigridSetup(vals, rows_t, finePrecision="fp16")
+    x = g.dm.constructDofVector()
+    it, rr = g.asm.pcg(vals, g.vec(b), x, rows_t, rtol=1e-12, maxit=2000, check_every=4, precond="amg", setup=False)
+    print(f"iterations: default {it0}, fp16 fine passes {it}", flush=True)
+    ip, ix = g.asm.pattern()
+    K = sp.csr_matrix((vals.cpu().numpy().astype(np.float64), ix, ip), shape=(nd, nd))
+    xr = spla.spsolve(K.tocsc(), b)
+    assert rr < 1e-12 and rel(x.cpu().numpy(), xr) < 1e-9
+    assert it0 <= it <= 2 * it0
+    # a linear elastic system in units with 1e6 times larger stresses (E = 2.1e11): modulus and the loads on the free
+    # rows scale, the prescribed displacements (unit rows) do not, so the solution is that of E = 2.1e5
+    ge, ke, _, rows_e, _ = _plastic_system(20, tangent, elastic_modulus=J2[0])
+    Ke = sp.csr_matrix((ke.cpu().numpy().astype(np.float64), ix, ip), shape=(nd, nd))
+    xe = spla.spsolve(Ke.tocsc(), b)
+    gb, big, _, rows_b, _ = _plastic_system(20, tangent, elastic_modulus=J2[0] * 1e6)
+    assert float(big.abs().max().item()) > 1e8
+    bb = b.copy()
+    free = np.ones(nd, dtype=bool)
+    free[rows] = False
+    bb[free] *= 1e6
+    gb.asm.multigridSetup(big, rows_b, finePrecision="fp16")
+    xb = gb.dm.constructDofVector()
+    itb, rrb = gb.asm.pcg(big, gb.vec(bb), xb, rows_b, rtol=1e-12, maxit=2000, check_every=4, precond="amg",
+                          setup=False)
+    gb.asm.multigridSetup(big, rows_b)
+    xd = gb.dm.constructDofVector()
+    itd, _ = gb.asm.pcg(big, gb.vec(bb), xd, rows_b, rtol=1e-12, maxit=2000, check_every=4, precond="amg",
+                        setup=False)
+    print(f"E x 1e6: {itb} iterations ({itd} with the default passes), relres {rrb:.2e}, vs the default passes "
+          f"{rel(xb.cpu().numpy(), xd.cpu().numpy()):.2e}, vs the direct solve at E {rel(xb.cpu().numpy(), xe):.2e}",
+          flush=True)
+    assert rrb < 1e-12 and rel(xb.cpu().numpy(), xd.cpu().numpy()) < 1e-9
+    if tangent == "fp64":  # FP32 values of 1e6 K are not 1e6 times the FP32 values of K
+        assert rel(xb.cpu().numpy(), xe) < 1e-9
+    g.asm.multigridSetup(vals, rows_t)
+    x2 = g.dm.constructDofVector()
+    it2, _ = g.asm.pcg(vals, g.vec(b), x2, rows_t, rtol=1e-12, maxit=2000, check_every=4, precond="amg", setup=False)
+    assert it2 == it0 and bool((x2 == x0).all().item())
 
 
 @pytest.mark.parametrize("name", ["TensionBarHexa8", "C3D8"])
