@@ -260,6 +260,42 @@ def _timed_evals(asm, U, dU, P, F, K, steps, warmup, mfma=False):
     return ev0.elapsed_time(ev1) / steps
 
 
+def c3d8tl_measure(n, steps, warmup):
+    """Finite-strain total-Lagrangian Neo-Hooke (mu=80, K=200) on the n^3 C3D8 mesh of the adjoint workload (jitter
+    +-5 % h, seed 0) at a homogeneous 5 % stretch along z: one evaluation + CSR assembly with FP64 values and with the
+    FP32 tangent (FP32 arithmetic for this material)."""
+    import torch
+
+    from edelweissfe_amd.assembly import ElementAssembly
+    from edelweissfe_amd.dofmanager import DofManager
+    from edelweissfe_amd.model import ElementBlock, FEModel, Material, boxGenArrays, jitterNodes
+
+    X, conn, _, grid = boxGenArrays(n, n, n, elType="C3D8TL")
+    X = jitterNodes(X, grid, (n + 1,) * 3, np.full(3, 1.0 / n), 0.05, 0)
+    m = FEModel()
+    m.addNodes(X)
+    b = ElementBlock("C3D8TL", conn)
+    b.setMaterial(Material("neohookeanwa", [80.0, 200.0]))
+    m.addElementBlock(b, "all")
+    dm = DofManager(m, device="cuda")
+    asm = ElementAssembly(b, dm, m, device="cuda")
+    U = torch.as_tensor(uniaxial(X, 0.05), device="cuda")
+    P, F = dm.constructDofVector(), dm.constructDofVector()
+    ms = {}
+    for prec in ("fp64", "fp32"):
+        K = asm.newValues(prec)
+        ms[prec] = _timed_evals(asm, U, U, P, F, K, steps, warmup)
+        del K
+    n_el = len(conn)
+    B64 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 0, 8, 8)
+    B32 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 0, 8, 4)
+    del asm
+    return {"elements": n_el, "ms_per_evaluation_fp64": ms["fp64"], "ms_per_evaluation_fp32": ms["fp32"],
+            "fp32_speedup": ms["fp64"] / ms["fp32"], "value": n_el / (ms["fp64"] * 1e-3), "unit": "elements/s",
+            "roofline_frac": B64 * n_el / (ms["fp64"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "roofline_frac_fp32": B32 * n_el / (ms["fp32"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def c3d20_tl_measure(n, steps, warmup, cpu=True):
     """Finite-strain total-Lagrangian compressible Neo-Hooke (mu=80, K=200) on n^3 C3D20 hexahedra (27 quadrature
     points, 60 dofs each), nodes jittered by U(-3%, 3%) of h (seed 0): one evaluation + CSR assembly at the state
@@ -579,6 +615,8 @@ def main():
             "roofline_frac": r4["B64"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "executed_flop_frac": r4["flop_exec"] * r4["n_el"] / (r4["ms64"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
             "cpu_baseline": r4["cpu"]}
+        torch.cuda.empty_cache()
+        legs["c3d8tl_neohooke_48cube"] = c3d8tl_measure(48, args.steps, args.warmup)
         torch.cuda.empty_cache()
         legs["plain_c3d8_le_10cube_newton"] = plain_10cube_newton()
         print(f"legs: {json.dumps({k: {kk: vv for kk, vv in v.items() if kk != 'cpu_baseline'} for k, v in legs.items()})}",
