@@ -57,7 +57,7 @@ def algorithmic_bytes_per_element(n_el, n_nodes, nnz, n_state, n_qp, val_bytes, 
 
 
 def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, linsolver="amg", setup_per_increment=True,
-                    fine_precision="matrix"):
+                    fine_precision="matrix", diagonal="rebuilt"):
     """The J2 cube's load case as configured: from the virgin state to `strain` uniaxial strain in `increments`
     equal increments (after the reference's zero-size first increment, adaptivetimestepper.py:90-134; NIST's default
     linear extrapolation), solved by NISTGPU: uz = 0 on z = 0, uz = strain on z = 1, ux = uy = 0 at (0,0,0), uy = 0
@@ -65,7 +65,9 @@ def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, lin
     floor) and |ddU|inf < 1e-8, linear solves to the relative PCG tolerance `pcg_tol`; with `setup_per_increment`
     the multigrid hierarchy is set up on the first solve of each increment and reused for its later corrections
     (otherwise on every solve); `fine_precision="fp16"` runs the V-cycle's fine-level passes on a half-precision copy of
-    the matrix. Returns timings per phase, iterations per increment and PCG iterations; plan and multigrid-pattern
+    the matrix; `diagonal="rebuilt"` (NISTGPU's default) gives the FP32 tangent FP64 corrections of its diagonal node
+    blocks that make every row sum to zero, "assembled" keeps the plain FP32 matrix (fp32TangentDiagonal). Returns
+    timings per phase, iterations per increment and PCG iterations; plan and multigrid-pattern
     construction (once per mesh) are reported apart."""
     import torch
 
@@ -86,7 +88,8 @@ def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, lin
                "fluxResidualToleranceAlternative": {"displacement": 1e-8}}
     solver = NISTGPU(jobInfo, _Journal(verbose=False, stream=sys.stderr), tangentPrecision=tangent,
                      pcgTolerance=pcg_tol, pcgToleranceFP32=pcg_tol, fluxResidualFloor=0.0, linsolver=linsolver,
-                     multigridSetupPerIncrement=setup_per_increment, multigridFinePrecision=fine_precision)
+                     multigridSetupPerIncrement=setup_per_increment, multigridFinePrecision=fine_precision,
+                     fp32TangentDiagonal=diagonal)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solver.solveStep(step, m)
@@ -99,12 +102,12 @@ def newton_schedule(n, tangent, pcg_tol, increments=10, strain=0.01, seed=0, lin
     t_setup = solver.computationTimes["setup"]
     return dict(seconds=t - t_setup, setup_seconds=t_setup, pcg_tolerance=pcg_tol,
                 multigrid_setup="per increment" if setup_per_increment else "per solve",
-                multigrid_fine_passes=fine_precision, increments=len(its) - 1,
+                multigrid_fine_passes=fine_precision, fp32_diagonal_blocks=diagonal, increments=len(its) - 1,
                 newton_iterations=newton_iters, iterations_per_increment=its, linear_solves=len(solves),
                 pcg_iterations_total=int(sum(solves)),
                 ms_per_newton_iteration=1e3 * (t - t_setup) / max(1, newton_iters),
                 phases_s={k: round(v, 4) for k, v in solver.computationTimes.items() if v > 0},
-                multigrid_levels=solver.system.mgLevels, U=U)
+                multigrid_levels=solver.system.mgLevels, U=U, history=solver.history)
 
 
 def host_cores():
@@ -477,8 +480,8 @@ def main():
     ap.add_argument("--newton-n", type=int, default=None, help="mesh of the Newton schedule (default --n)")
     ap.add_argument("--linsolver", default="amg", choices=["amg", "pcg"],
                     help="Newton schedule's linear solver: multigrid- or node-block-Jacobi-preconditioned CG")
-    ap.add_argument("--pcg-tol-fp32", type=float, default=1e-4,
-                    help="relative PCG tolerance of the FP32-tangent schedule (NISTGPU's default)")
+    ap.add_argument("--pcg-tol-fp32", type=float, default=2e-6,
+                    help="relative PCG tolerance of the FP32-tangent schedule and of its controls")
     ap.add_argument("--no-newton-controls", action="store_true",
                     help="skip the matrix-core FP32 tangent schedule")
     ap.add_argument("--newton-increments", type=int, default=10, help="increments of the 1%% uniaxial schedule")
@@ -522,7 +525,8 @@ def main():
     U_np_t = U_n_t + dU_t
     P, F = dm.constructDofVector(), dm.constructDofVector()
     K64 = asm.newValues("fp64")
-    K32 = asm.newValues("fp32")
+    K32 = asm.newValues("fp32-d64")  # the FP32 values and their FP64 diagonal corrections (NISTGPU's default)
+    K32p = asm.newValues("fp32")  # plain FP32 values
     # converged state at U_n (first loading from the virgin state), then accepted
     asm.evaluate(U_n_t, U_n_t, P, F, K64)
     b.acceptLastState()
@@ -549,14 +553,16 @@ def main():
     ms64, wall64 = timed(K64, args.steps, args.warmup)
     pdt = float(asm.pnewdt.item())
     ms32, wall32 = timed(K32, args.steps, args.warmup)
+    ms32p, _ = timed(K32p, args.steps, args.warmup)
     ms32m, wall32m = timed(K32, args.steps, args.warmup, mfma=True)
-    print(f"fp64 step {ms64:.3f} ms (wall {wall64:.3f}), fp32-tangent step {ms32:.3f} ms (wall {wall32:.3f}), "
+    print(f"fp64 step {ms64:.3f} ms (wall {wall64:.3f}), fp32-tangent step {ms32:.3f} ms (wall {wall32:.3f}; plain "
+          f"FP32 values {ms32p:.3f} ms), "
           f"fp32-mfma tangent step {ms32m:.3f} ms (wall {wall32m:.3f}), plastic QPs {plastic_frac:.3f}, "
           f"pNewDT {pdt:g}", file=sys.stderr, flush=True)
 
     n_el = len(conn)
     B64 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 8)
-    B32 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 4)
+    B32 = algorithmic_bytes_per_element(n_el, m.nNodes, asm.nnz, 7, 8, 4) + 72.0 * m.nNodes / n_el  # + corrections
     ach64 = B64 * n_el / (ms64 * 1e-3) / 1e9
     ach32 = B32 * n_el / (ms32 * 1e-3) / 1e9
 
@@ -579,7 +585,7 @@ def main():
               flush=True)
 
     nnz, ncolors, n_chunks = asm.nnz, asm.nColors, asm.pipelineChunks
-    del asm, K64, K32
+    del asm, K64, K32, K32p
     torch.cuda.empty_cache()
 
     # the other workloads' legs: the 64^3 J2 evaluation, the 40^3 C3D20TL evaluation and the plain 10^3
@@ -630,25 +636,36 @@ def main():
         for tan in ("fp64", "fp32", "fp32-mfma"):
             newton_schedule(8, tan, 1e-4, increments=2, linsolver=args.linsolver)
         torch.cuda.empty_cache()
-        # plain FP64 at several PCG tolerances (the fastest one that meets the Newton tolerance is the baseline), the
-        # FP32 tangent at its precision-matched tolerance, and the matrix-core FP32 tangent, all with the multigrid set
-        # up once per increment. Controls: a setup on every solve; both precisions with a half-precision copy of the
-        # matrix in the V-cycle's fine-level passes (a mixed-precision preconditioner, so not the plain FP64 path)
-        runs = [("fp64", 1e-10, True, "matrix"), ("fp64", 1e-6, True, "matrix"),
-                ("fp64", args.pcg_tol_fp32, True, "matrix"), ("fp32", args.pcg_tol_fp32, True, "matrix"),
-                ("fp64", 1e-6, False, "matrix"), ("fp32", args.pcg_tol_fp32, False, "matrix")]
+        # plain FP64 and the FP32 tangent (with its diagonal corrections, NISTGPU's default) at several PCG tolerances
+        # (the fastest run of each precision that meets the Newton tolerance is compared), the plain FP32 matrix
+        # (assembled diagonal blocks) at its own best tolerance, all with the multigrid set up once per increment.
+        # Controls: a setup on every solve; the matrix-core FP32 tangent; both precisions with a half-precision copy
+        # of the matrix in the V-cycle's fine-level passes (a mixed-precision preconditioner, so not the plain FP64
+        # path)
+        t32 = args.pcg_tol_fp32
+        runs = [("fp64", 1e-10, True, "matrix", "rebuilt"), ("fp64", 1e-6, True, "matrix", "rebuilt"),
+                ("fp64", 2e-6, True, "matrix", "rebuilt"), ("fp64", 1e-4, True, "matrix", "rebuilt"),
+                ("fp32", 1e-6, True, "matrix", "rebuilt"), ("fp32", 2e-6, True, "matrix", "rebuilt"),
+                ("fp32", 1e-4, True, "matrix", "assembled")]
+        if t32 not in (1e-6, 2e-6):
+            runs.append(("fp32", t32, True, "matrix", "rebuilt"))
         if not args.no_newton_controls:
-            runs += [("fp32-mfma", args.pcg_tol_fp32, True, "matrix"), ("fp64", 1e-6, True, "fp16"),
-                     ("fp32", args.pcg_tol_fp32, True, "fp16")]
+            runs += [("fp64", t32, False, "matrix", "rebuilt"), ("fp32", t32, False, "matrix", "rebuilt"),
+                     ("fp32-mfma", t32, True, "matrix", "rebuilt"), ("fp64", t32, True, "fp16", "rebuilt"),
+                     ("fp32", t32, True, "fp16", "rebuilt")]
         res = {}
-        for tan, tol, per_inc, fine in runs:
+        for tan, tol, per_inc, fine, dg in runs:
             r = newton_schedule(nn_, tan, tol, increments=args.newton_increments, linsolver=args.linsolver,
-                                setup_per_increment=per_inc, fine_precision=fine)
-            print(f"newton {tan} (pcg tol {tol:g}, multigrid setup {r['multigrid_setup']}, fine-level passes {fine}): "
-                  f"{r['seconds']:.3f}s, iterations per increment {r['iterations_per_increment']}, "
-                  f"{r['linear_solves']} solves, {r['pcg_iterations_total']} PCG iterations, phases {r['phases_s']}",
-                  file=sys.stderr, flush=True)
-            key = f"{tan}_pcg{tol:g}" + ("" if per_inc else "_setup_per_solve") + ("_fp16_fine" if fine == "fp16" else "")
+                                setup_per_increment=per_inc, fine_precision=fine, diagonal=dg)
+            print(f"newton {tan} (pcg tol {tol:g}, multigrid setup {r['multigrid_setup']}, fine-level passes {fine}, "
+                  f"fp32 diagonal {dg}): {r['seconds']:.3f}s, iterations per increment "
+                  f"{r['iterations_per_increment']}, {r['linear_solves']} solves, {r['pcg_iterations_total']} PCG "
+                  f"iterations, phases {r['phases_s']}", file=sys.stderr, flush=True)
+            key = (f"{tan}_pcg{tol:g}" + ("" if per_inc else "_setup_per_solve") +
+                   ("_fp16_fine" if fine == "fp16" else "") +
+                   ("_assembled_diagonal" if tan != "fp64" and dg == "assembled" else ""))
+            if tan == "fp64":
+                r.pop("fp32_diagonal_blocks")
             res[key] = r
             torch.cuda.empty_cache()
         ref64 = res["fp64_pcg1e-10"]
@@ -658,13 +675,16 @@ def main():
                                                                           np.abs(U64).max())
         for r in res.values():
             r.pop("U")
-        fp64_keys = [k for k in res if k.startswith("fp64") and not k.endswith("_fp16_fine")]  # plain FP64
-        best64 = min(fp64_keys, key=lambda k: res[k]["seconds"])
-        fp32_key = f"fp32_pcg{args.pcg_tol_fp32:g}"
+            r.pop("history")
+        plain = lambda k: not k.endswith(("_fp16_fine", "_setup_per_solve", "_assembled_diagonal"))
+        best64 = min((k for k in res if k.startswith("fp64_") and plain(k)), key=lambda k: res[k]["seconds"])
+        fp32_key = min((k for k in res if k.startswith("fp32_") and plain(k)), key=lambda k: res[k]["seconds"])
         newton = {"mesh": f"{nn_}^3", "load": f"virgin state to 1% uniaxial strain in {args.newton_increments} "
                                               "increments (after the zero-size first increment), linear extrapolation",
-                  "runs": res, "fastest_fp64": best64,
+                  "runs": res, "fastest_fp64": best64, "fastest_fp32": fp32_key,
                   "speedup_fp32_vs_fastest_fp64": res[best64]["seconds"] / res[fp32_key]["seconds"],
+                  "speedup_fp32_assembled_diagonal_vs_fastest_fp64": (res[best64]["seconds"] /
+                                                                      res["fp32_pcg0.0001_assembled_diagonal"]["seconds"]),
                   "speedup_fp32_vs_fp64_pcg1e-10": ref64["seconds"] / res[fp32_key]["seconds"],
                   "newton_step_speedup_fp32_vs_fastest_fp64": (res[best64]["ms_per_newton_iteration"] /
                                                                res[fp32_key]["ms_per_newton_iteration"]),
@@ -673,7 +693,7 @@ def main():
             newton["speedup_fp32mfma_vs_fastest_fp64"] = (res[best64]["seconds"] /
                                                           res[f"fp32-mfma_pcg{args.pcg_tol_fp32:g}"]["seconds"])
             # the same comparison when both precisions use the half-precision fine-level V-cycle passes
-            newton["speedup_fp32_vs_fp64_both_fp16_fine"] = (res["fp64_pcg1e-06_fp16_fine"]["seconds"] /
+            newton["speedup_fp32_vs_fp64_both_fp16_fine"] = (res[f"fp64_pcg{args.pcg_tol_fp32:g}_fp16_fine"]["seconds"] /
                                                              res[f"fp32_pcg{args.pcg_tol_fp32:g}_fp16_fine"]["seconds"])
 
     line = {
@@ -702,12 +722,13 @@ def main():
         "cpu_baseline": cpu,
         "fp32_tangent": {"value": n_el / (ms32 * 1e-3), "ms_per_step": ms32, "speedup_vs_fp64": ms64 / ms32,
                          "roofline_achieved_GBs": ach32, "roofline_frac": ach32 / HBM_PEAK_GBS,
-                         "algorithmic_bytes_per_element": B32,
-                         "tangent": "FP64 FMAs rounded to FP32 when the records are stored (the J2 plastic tangent "
-                                    "is a difference of large terms; FP32 arithmetic is used for the other materials)"},
+                         "algorithmic_bytes_per_element": B32, "ms_per_step_plain_fp32_values": ms32p,
+                         "tangent": "FP32 arithmetic on the vector units, FP32 CSR values with FP64 corrections of "
+                                    "the diagonal node blocks (every row sums to zero, as the FP64 tangent's)"},
         "fp32_tangent_mfma": {"value": n_el / (ms32m * 1e-3), "ms_per_step": ms32m, "speedup_vs_fp64": ms64 / ms32m,
                               "roofline_frac": B32 * n_el / (ms32m * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                              "tangent": "FP32 on the matrix cores (v_mfma_f32_16x16x4_f32)"},
+                              "tangent": "FP32 on the matrix cores (v_mfma_f32_16x16x4_f32), values and diagonal "
+                                         "corrections as fp32_tangent"},
         "setup_s": t_setup,
         "legs": legs,
         "newton": newton,

@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(PKG, "libewfe.so")
 
 EWFE_C3D8, EWFE_C3D20, EWFE_C3D8TL, EWFE_C3D20TL, EWFE_C3D20R, EWFE_C3D20RTL = 0, 1, 2, 3, 4, 5
 EWFE_LINEAR_ELASTIC, EWFE_VON_MISES, EWFE_NEO_HOOKE_WA, EWFE_ST_VENANT_KIRCHHOFF = 0, 1, 2, 3
-EWFE_F64, EWFE_F32, EWFE_F32_MFMA, EWFE_F16 = 0, 1, 2, 3
+EWFE_F64, EWFE_F32, EWFE_F32_MFMA, EWFE_F16, EWFE_F32_D64, EWFE_F32_MFMA_D64 = 0, 1, 2, 3, 4, 5
 
 ELEMENT_KINDS = {"C3D8": EWFE_C3D8, "C3D20": EWFE_C3D20, "C3D8TL": EWFE_C3D8TL, "C3D20TL": EWFE_C3D20TL,
                  "C3D20R": EWFE_C3D20R, "C3D20RTL": EWFE_C3D20RTL}
@@ -53,6 +53,7 @@ SIGNATURES = {
     "ewfe_pcg": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _i64, _dbl, ctypes.c_int, ctypes.c_int, _vp,
                                 _P(ctypes.c_int), _P(_dbl), _vp]),
     "ewfe_pcg_workspace_size": (_i64, [_vp]),
+    "ewfe_vals_bytes": (_i64, [_vp, ctypes.c_int]),
     "ewfe_residual": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "ewfe_pcg_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _i64, _dbl, ctypes.c_int, ctypes.c_int,
                                         _P(ctypes.c_int), _P(_dbl), _vp]),

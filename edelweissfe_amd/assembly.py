@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import EWFE_F16, EWFE_F32, EWFE_F32_MFMA, EWFE_F64, check, lib, ptr
+from ._lib import EWFE_F16, EWFE_F32, EWFE_F32_D64, EWFE_F32_MFMA, EWFE_F32_MFMA_D64, EWFE_F64, check, lib, ptr
 
 
 class PlanLinearAlgebra:
@@ -49,18 +49,39 @@ class PlanLinearAlgebra:
         return indptr, indices
 
     def newValues(self, dtype="fp64"):
-        tdt = self.torch.float64 if dtype == "fp64" else self.torch.float32
-        return self.torch.zeros(self.nnz, dtype=tdt, device=self.device)
+        """A zeroed CSR value array: "fp64" (float64 [nnz]), "fp32" (float32 [nnz]) or "fp32-d64" (float32 array of
+        ewfe_vals_bytes / 4 entries: the nnz FP32 values, then FP64 diagonal blocks, 9 per node, rebuilt from the
+        rounded off-diagonal values by every evaluation so that each row sums to zero as the FP64 matrix's does;
+        ``csrValues`` gives the nnz values)."""
+        if dtype == "fp64":
+            return self.torch.zeros(self.nnz, dtype=self.torch.float64, device=self.device)
+        if dtype == "fp32":
+            return self.torch.zeros(self.nnz, dtype=self.torch.float32, device=self.device)
+        if dtype == "fp32-d64":
+            return self.torch.zeros(self._d64Length(), dtype=self.torch.float32, device=self.device)
+        raise ValueError(f"unknown value precision {dtype!r} (fp64, fp32, fp32-d64)")
 
-    @staticmethod
-    def _dtype_code(vals):
-        import torch
+    def _d64Length(self):
+        n = getattr(self, "_d64len", None)
+        if n is None:
+            nbytes = lib().ewfe_vals_bytes(self._plan, EWFE_F32_D64)
+            check(int(min(nbytes, 0)))
+            n = self._d64len = nbytes // 4
+        return n
 
-        if vals.dtype == torch.float64:
+    def csrValues(self, vals):
+        """The nnz CSR values of a value array (a view)."""
+        return vals[: self.nnz]
+
+    def _dtype_code(self, vals):
+        if vals.dtype == self.torch.float64 and vals.numel() == self.nnz:
             return EWFE_F64
-        if vals.dtype == torch.float32:
-            return EWFE_F32
-        raise TypeError("CSR values must be float64 or float32")
+        if vals.dtype == self.torch.float32:
+            if vals.numel() == self.nnz:
+                return EWFE_F32
+            if vals.numel() == self._d64Length():
+                return EWFE_F32_D64
+        raise TypeError("CSR values must come from newValues of this plan (float64 or float32 [nnz], or fp32-d64)")
 
     def _stream(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
@@ -190,18 +211,18 @@ class ElementAssembly(PlanLinearAlgebra):
         tensor, 1e36 or < 1 when a quadrature point asks for a cutback). FP32 values: ``mfma=True`` forms the
         tangents in FP32 on the matrix cores (8-node elements, and 20-node elements assembled through the row-node
         gather), otherwise in FP32 arithmetic on the vector units (8-node elements, materials other than J2) or
-        with FP64 FMAs rounded when stored (J2, 20-node elements)."""
+        with FP64 FMAs rounded when stored (J2, 20-node elements); fp32-d64 values (``newValues``) then also
+        receive their FP64 diagonal blocks."""
         for t in (U_np, dU, P, F):
             if t.numel() != self.nDof or t.dtype != self.torch.float64 or not t.is_contiguous():
                 raise ValueError("dof vectors must be contiguous float64 tensors of length nDof")
-        if vals.numel() != self.nnz:
-            raise ValueError("CSR value array has the wrong length")
+        code = self._dtype_code(vals)
         b = self.block
         check(lib().ewfe_eval_assemble(self._plan, self.matKind, ptr(self.matParams), ptr(self.elParams),
                                        ptr(self.X), ptr(U_np), ptr(dU), ptr(b.stateVars), ptr(b.stateVarsTemp),
                                        ptr(P), ptr(F), ptr(vals),
-                                       EWFE_F32_MFMA if (mfma and vals.dtype == self.torch.float32) else
-                                       self._dtype_code(vals), ptr(self.pnewdt),
+                                       {EWFE_F32: EWFE_F32_MFMA, EWFE_F32_D64: EWFE_F32_MFMA_D64}.get(code, code)
+                                       if mfma else code, ptr(self.pnewdt),
                                        self._stream()))
         return self.pnewdt
 
@@ -299,7 +320,9 @@ class SystemAssembly(PlanLinearAlgebra):
         self._init_solver_state()
 
     def accumulate(self, block, blockValues, vals):
-        """vals += the CSR values of block `block` (its plan's pattern), mapped into the system pattern."""
+        """vals += the CSR values of block `block` (its plan's pattern), mapped into the system pattern. With fp32-d64
+        system values the system's FP64 diagonal blocks are rebuilt from its summed values after each block (the
+        block's own FP64 blocks are not read)."""
         if blockValues.dtype != vals.dtype:
             raise TypeError("block and system values must have one precision")
         check(lib().ewfe_csr_accumulate(self._plan, int(block), ptr(blockValues), ptr(vals), self._dtype_code(vals),

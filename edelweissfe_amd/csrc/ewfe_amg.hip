@@ -94,6 +94,9 @@ struct ewfe_mg {
     bool fine16 = false;
     _Float16 *vals16 = nullptr;
     double *scale16 = nullptr;  // [2]: the largest magnitude, the scale
+    // FP32 matrices: the fine level's block-Jacobi inverses rounded to FP32 for the V-cycle's smoothing steps
+    // (allocated by the first FP32 setup)
+    float *dinv32 = nullptr;
     std::vector<void *> allocs;
 };
 
@@ -147,8 +150,8 @@ struct BsrOp {
     int64_t n;
 };
 
-template <typename VT>
-__device__ __forceinline__ void row_apply(const FineOp<VT> &A, int64_t i, int lane, const double *__restrict__ x,
+template <typename VT, typename XT>
+__device__ __forceinline__ void row_apply(const FineOp<VT> &A, int64_t i, int lane, const XT *__restrict__ x,
                                           double &y0, double &y1, double &y2)
 {
     const int4 m4 = A.meta[i];
@@ -156,7 +159,7 @@ __device__ __forceinline__ void row_apply(const FineOp<VT> &A, int64_t i, int la
     double a0 = 0, a1 = 0, a2 = 0;
     for (int m = lane; m < deg; m += kGroup) {
         const int32_t c = A.nbcol[nb0 + m];
-        const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+        const double x0 = (double)x[c], x1 = (double)x[c + 1], x2 = (double)x[c + 2];
         const VT *v0 = A.vals + base + 3 * m;
         a0 += (double)v0[0] * x0 + (double)v0[1] * x1 + (double)v0[2] * x2;
         a1 += (double)v0[L] * x0 + (double)v0[L + 1] * x1 + (double)v0[L + 2] * x2;
@@ -173,13 +176,14 @@ __device__ __forceinline__ void row_apply(const FineOp<VT> &A, int64_t i, int la
     }
 }
 
-__device__ __forceinline__ void row_apply(const BsrOp &A, int64_t i, int lane, const double *__restrict__ x,
+template <typename XT>
+__device__ __forceinline__ void row_apply(const BsrOp &A, int64_t i, int lane, const XT *__restrict__ x,
                                           double &y0, double &y1, double &y2)
 {
     double a0 = 0, a1 = 0, a2 = 0;
     for (int32_t e = A.aptr[i] + lane; e < A.aptr[i + 1]; e += kGroup) {
         const int32_t c = 3 * A.acol[e];
-        const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+        const double x0 = (double)x[c], x1 = (double)x[c + 1], x2 = (double)x[c + 2];
         const double *v = A.aval + 9 * (int64_t)e;
         a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
         a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
@@ -190,12 +194,19 @@ __device__ __forceinline__ void row_apply(const BsrOp &A, int64_t i, int lane, c
     y2 = group_sum32(a2);
 }
 
-__device__ __forceinline__ void apply3(const double *__restrict__ Q, double r0, double r1, double r2, double &z0,
+template <typename DT>
+__device__ __forceinline__ void apply3(const DT *__restrict__ Q, double r0, double r1, double r2, double &z0,
                                        double &z1, double &z2)
 {
-    z0 = Q[0] * r0 + Q[1] * r1 + Q[2] * r2;
-    z1 = Q[3] * r0 + Q[4] * r1 + Q[5] * r2;
-    z2 = Q[6] * r0 + Q[7] * r1 + Q[8] * r2;
+    z0 = (double)Q[0] * r0 + (double)Q[1] * r1 + (double)Q[2] * r2;
+    z1 = (double)Q[3] * r0 + (double)Q[4] * r1 + (double)Q[5] * r2;
+    z2 = (double)Q[6] * r0 + (double)Q[7] * r1 + (double)Q[8] * r2;
+}
+
+__global__ void k_mg_narrow(int64_t n, const double *__restrict__ a, float *__restrict__ b)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = (float)a[i];
 }
 
 // inverse of a 3x3 diagonal block; a component whose row and column vanish (a coarse dof that only Dirichlet
@@ -604,8 +615,10 @@ __device__ __forceinline__ double cheb_rho(const Cheb &c, int k)
     return rho;
 }
 
-// first step from x = 0: r = b, d = D^-1 b / theta, x = d
-__global__ void k_mg_cheb0_zero(int64_t n, const double *dinv, const double *b, double *r, double *d, double *x,
+// first step from x = 0: r = b, d = D^-1 b / theta, x = d (vectors of type XT: double, or float on the fine level of
+// an FP32 matrix)
+template <typename DT, typename XT>
+__global__ void k_mg_cheb0_zero(int64_t n, const DT *dinv, const double *b, XT *r, XT *d, XT *x,
                                 const double *lam, double ratio, const double *done)
 {
     if (done && *done != 0.0) return;
@@ -616,14 +629,15 @@ __global__ void k_mg_cheb0_zero(int64_t n, const double *dinv, const double *b, 
         const double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
         double z0, z1, z2;
         apply3(dinv + 9 * i, b0, b1, b2, z0, z1, z2);
-        r[3 * i] = b0, r[3 * i + 1] = b1, r[3 * i + 2] = b2;
-        d[3 * i] = s * z0, d[3 * i + 1] = s * z1, d[3 * i + 2] = s * z2;
-        x[3 * i] = s * z0, x[3 * i + 1] = s * z1, x[3 * i + 2] = s * z2;
+        r[3 * i] = (XT)b0, r[3 * i + 1] = (XT)b1, r[3 * i + 2] = (XT)b2;
+        d[3 * i] = (XT)(s * z0), d[3 * i + 1] = (XT)(s * z1), d[3 * i + 2] = (XT)(s * z2);
+        x[3 * i] = (XT)(s * z0), x[3 * i + 1] = (XT)(s * z1), x[3 * i + 2] = (XT)(s * z2);
     }
 }
 
 // first step from a given x (r = b - A x already computed): d = D^-1 r / theta, x += d
-__global__ void k_mg_cheb0(int64_t n, const double *dinv, const double *r, double *d, double *x, const double *lam,
+template <typename DT, typename XT>
+__global__ void k_mg_cheb0(int64_t n, const DT *dinv, const XT *r, XT *d, XT *x, const double *lam,
                            double ratio, const double *done)
 {
     if (done && *done != 0.0) return;
@@ -632,15 +646,16 @@ __global__ void k_mg_cheb0(int64_t n, const double *dinv, const double *r, doubl
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         double z0, z1, z2;
-        apply3(dinv + 9 * i, r[3 * i], r[3 * i + 1], r[3 * i + 2], z0, z1, z2);
-        d[3 * i] = s * z0, d[3 * i + 1] = s * z1, d[3 * i + 2] = s * z2;
-        x[3 * i] += s * z0, x[3 * i + 1] += s * z1, x[3 * i + 2] += s * z2;
+        apply3(dinv + 9 * i, (double)r[3 * i], (double)r[3 * i + 1], (double)r[3 * i + 2], z0, z1, z2);
+        d[3 * i] = (XT)(s * z0), d[3 * i + 1] = (XT)(s * z1), d[3 * i + 2] = (XT)(s * z2);
+        x[3 * i] = (XT)((double)x[3 * i] + s * z0), x[3 * i + 1] = (XT)((double)x[3 * i + 1] + s * z1),
+        x[3 * i + 2] = (XT)((double)x[3 * i + 2] + s * z2);
     }
 }
 
 // r = b - A x (b may alias r: each node reads only its own entries of b)
-template <typename OP>
-__global__ __launch_bounds__(kThreads) void k_mg_resid(OP A, const double *b, const double *__restrict__ x, double *r,
+template <typename OP, typename BT, typename XT>
+__global__ __launch_bounds__(kThreads) void k_mg_resid(OP A, const BT *b, const XT *__restrict__ x, XT *r,
                                                        const double *done)
 {
     if (done && *done != 0.0) return;
@@ -649,15 +664,15 @@ __global__ __launch_bounds__(kThreads) void k_mg_resid(OP A, const double *b, co
          i += (int64_t)gridDim.x * kNodesPerBlock) {
         double y0, y1, y2;
         row_apply(A, i, lane, x, y0, y1, y2);
-        if (lane < 3) r[3 * i + lane] = b[3 * i + lane] - (lane == 0 ? y0 : (lane == 1 ? y1 : y2));
+        if (lane < 3) r[3 * i + lane] = (XT)((double)b[3 * i + lane] - (lane == 0 ? y0 : (lane == 1 ? y1 : y2)));
     }
 }
 
 // Chebyshev step k >= 1: r -= A d_old, d_new = rho_k rho_{k-1} d_old + (2 rho_k / delta) D^-1 r, x += d_new
-template <typename OP>
-__global__ __launch_bounds__(kThreads) void k_mg_cheb_step(OP A, const double *dinv, double *r,
-                                                           const double *__restrict__ d_old, double *d_new, double *x,
-                                                           const double *lam, double ratio, int k, const double *done)
+template <typename OP, typename DT, typename XT>
+__global__ __launch_bounds__(kThreads) void k_mg_cheb_step(OP A, const DT *dinv, XT *r, const XT *__restrict__ d_old,
+                                                           XT *d_new, XT *x, const double *lam, double ratio, int k,
+                                                           const double *done)
 {
     if (done && *done != 0.0) return;
     const Cheb c = cheb_params(*lam, ratio);
@@ -671,23 +686,24 @@ __global__ __launch_bounds__(kThreads) void k_mg_cheb_step(OP A, const double *d
         // lanes 0..2 each own one component; the 3x3 block-Jacobi product takes the three residual
         // components from those lanes by shuffles (the other lanes carry copies of component 2)
         const int c = lane < 3 ? lane : 2;
-        const double rc = r[3 * i + c] - (c == 0 ? y0 : (c == 1 ? y1 : y2));
+        const double rc = (double)r[3 * i + c] - (c == 0 ? y0 : (c == 1 ? y1 : y2));
         const double r0 = __shfl(rc, 0, kGroup), r1 = __shfl(rc, 1, kGroup), r2 = __shfl(rc, 2, kGroup);
         if (lane < 3) {
-            const double *Q = dinv + 9 * i + 3 * c;
-            const double e = ca * d_old[3 * i + c] + cb * (Q[0] * r0 + Q[1] * r1 + Q[2] * r2);
-            r[3 * i + c] = rc;
-            d_new[3 * i + c] = e;
-            x[3 * i + c] += e;
+            const DT *Q = dinv + 9 * i + 3 * c;
+            const double e =
+                ca * (double)d_old[3 * i + c] + cb * ((double)Q[0] * r0 + (double)Q[1] * r1 + (double)Q[2] * r2);
+            r[3 * i + c] = (XT)rc;
+            d_new[3 * i + c] = (XT)e;
+            x[3 * i + c] = (XT)((double)x[3 * i + c] + e);
         }
     }
 }
 
 // b_c[I] = sum over R(I) of P_t^T r[i] (one 64-lane wave per aggregate, fixed-order butterfly)
-template <typename PT>
+template <typename PT, typename XT>
 __global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t *__restrict__ rptr,
                                                     const int32_t *__restrict__ ri, const int32_t *__restrict__ rt,
-                                                    const PT *__restrict__ pval, const double *__restrict__ r,
+                                                    const PT *__restrict__ pval, const XT *__restrict__ r,
                                                     double *__restrict__ bc, const double *done)
 {
     if (done && *done != 0.0) return;
@@ -697,7 +713,7 @@ __global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t
     for (int32_t q = rptr[I] + lane; q < rptr[I + 1]; q += 64) {
         const int32_t i = ri[q];
         const PT *p = pval + 9 * (int64_t)rt[q];
-        const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+        const double r0 = (double)r[3 * i], r1 = (double)r[3 * i + 1], r2 = (double)r[3 * i + 2];
         a0 += (double)p[0] * r0 + (double)p[3] * r1 + (double)p[6] * r2;
         a1 += (double)p[1] * r0 + (double)p[4] * r1 + (double)p[7] * r2;
         a2 += (double)p[2] * r0 + (double)p[5] * r1 + (double)p[8] * r2;
@@ -709,9 +725,9 @@ __global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t
 }
 
 // x[i] += sum_t P_t x_c[J_t]
-template <typename PT>
+template <typename PT, typename XT>
 __global__ void k_mg_prolong(int64_t n, const int32_t *__restrict__ pptr, const int32_t *__restrict__ pcol,
-                             const PT *__restrict__ pval, const double *__restrict__ xc, double *__restrict__ x,
+                             const PT *__restrict__ pval, const double *__restrict__ xc, XT *__restrict__ x,
                              const double *done)
 {
     if (done && *done != 0.0) return;
@@ -726,8 +742,17 @@ __global__ void k_mg_prolong(int64_t n, const int32_t *__restrict__ pptr, const 
             a1 += (double)p[3] * c0 + (double)p[4] * c1 + (double)p[5] * c2;
             a2 += (double)p[6] * c0 + (double)p[7] * c1 + (double)p[8] * c2;
         }
-        x[3 * i] += a0, x[3 * i + 1] += a1, x[3 * i + 2] += a2;
+        x[3 * i] = (XT)((double)x[3 * i] + a0), x[3 * i + 1] = (XT)((double)x[3 * i + 1] + a1),
+        x[3 * i + 2] = (XT)((double)x[3 * i + 2] + a2);
     }
+}
+
+// z = x (the fine level's FP32 iterate handed back in FP64)
+__global__ void k_mg_widen(int64_t n3, const float *__restrict__ x, double *__restrict__ z, const double *done)
+{
+    if (done && *done != 0.0) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) z[i] = (double)x[i];
 }
 
 // x = A_c^-1 b (dense, one 64-lane wave per row)
@@ -901,6 +926,11 @@ int mg_setup(ewfe_mg *mg, const void *vals, const int32_t *rows, int64_t n_rows,
         const BsrOp B = bsr_op(L);
         if (l == 0) {
             hipLaunchKernelGGL((k_mg_dinv_fine<VT>), dim3(1024), dim3(256), 0, s, F, L.dinv);
+            if constexpr (std::is_same<VT, float>::value) {
+                if (!mg->dinv32) RC(alloc(mg, mg->dinv32, 9 * (size_t)L.n));
+                hipLaunchKernelGGL(k_mg_narrow, dim3(1024), dim3(256), 0, s, 9 * L.n, (const double *)L.dinv,
+                                   mg->dinv32);
+            }
             RC(power_iteration(mg, 0, F, L.dinv, mg->mask, s));
         } else {
             hipLaunchKernelGGL(k_mg_dinv_bsr, dim3(1024), dim3(256), 0, s, B, L.dinv);
@@ -969,53 +999,68 @@ int vcycle(ewfe_mg *mg, int l, const void *vals, const double *done, hipStream_t
     const double ratio = mg->ratio;
     const int nu = mg->degree;
     const unsigned ew = 1024;  // grid of the node-wise kernels
-    auto smooth_steps = [&](double *&dcur, double *&dnext, auto &&op) {
-        for (int k = 1; k < nu; ++k) {
-            hipLaunchKernelGGL((k_mg_cheb_step<std::decay_t<decltype(op)>>), dim3(kBlocks), dim3(kThreads), 0, s, op,
-                               (const double *)L.dinv, L.r, (const double *)dcur, dnext, L.x, lam, ratio, k, done);
-            std::swap(dcur, dnext);
-        }
-    };
-    auto level_body = [&](auto &&op) {
-        double *dc = L.d0, *dn = L.d1;
-        // pre-smoothing from x = 0
-        hipLaunchKernelGGL(k_mg_cheb0_zero, dim3(ew), dim3(256), 0, s, L.n, (const double *)L.dinv,
-                           (const double *)L.b, L.r, dc, L.x, lam, ratio, done);
-        smooth_steps(dc, dn, op);
+    // pre-smoothing from x = 0, residual, restriction, the next level, prolongation, post-smoothing; vectors r, x, d of
+    // type XT (b and the next level's vectors are FP64)
+    auto run_level = [&](auto &&op, const auto *dinv, auto *r, auto *x, auto *da, auto *db) -> int {
+        using OP = std::decay_t<decltype(op)>;
+        using DT = std::remove_const_t<std::remove_pointer_t<decltype(dinv)>>;
+        using XT = std::remove_pointer_t<decltype(r)>;
+        XT *dc = da, *dn = db;
+        auto smooth_steps = [&]() {
+            for (int k = 1; k < nu; ++k) {
+                hipLaunchKernelGGL((k_mg_cheb_step<OP, DT, XT>), dim3(kBlocks), dim3(kThreads), 0, s, op, dinv, r,
+                                   (const XT *)dc, dn, x, lam, ratio, k, done);
+                std::swap(dc, dn);
+            }
+        };
+        hipLaunchKernelGGL((k_mg_cheb0_zero<DT, XT>), dim3(ew), dim3(256), 0, s, L.n, dinv, (const double *)L.b, r,
+                           dc, x, lam, ratio, done);
+        smooth_steps();
         // residual of the smoothed iterate: r -= A d_last
-        hipLaunchKernelGGL((k_mg_resid<std::decay_t<decltype(op)>>), dim3(kBlocks), dim3(kThreads), 0, s, op,
-                           (const double *)L.r, (const double *)dc, L.r, done);
-        hipLaunchKernelGGL(k_mg_restrict<VT>, dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
-                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const VT *)L.pval,
-                           (const double *)L.r, N.b, done);
+        hipLaunchKernelGGL((k_mg_resid<OP, XT, XT>), dim3(kBlocks), dim3(kThreads), 0, s, op, (const XT *)r,
+                           (const XT *)dc, r, done);
+        hipLaunchKernelGGL((k_mg_restrict<VT, XT>), dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
+                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const VT *)L.pval, (const XT *)r, N.b,
+                           done);
+        RC(vcycle<VT>(mg, l + 1, vals, done, s));
+        hipLaunchKernelGGL((k_mg_prolong<VT, XT>), dim3(ew), dim3(256), 0, s, L.n, (const int32_t *)L.pptr,
+                           (const int32_t *)L.pcol, (const VT *)L.pval, (const double *)N.x, x, done);
+        dc = da, dn = db;
+        hipLaunchKernelGGL((k_mg_resid<OP, double, XT>), dim3(kBlocks), dim3(kThreads), 0, s, op,
+                           (const double *)L.b, (const XT *)x, r, done);
+        hipLaunchKernelGGL((k_mg_cheb0<DT, XT>), dim3(ew), dim3(256), 0, s, L.n, dinv, (const XT *)r, dc, x, lam,
+                           ratio, done);
+        smooth_steps();
         return 0;
     };
-    auto post = [&](auto &&op) {
-        hipLaunchKernelGGL(k_mg_prolong<VT>, dim3(ew), dim3(256), 0, s, L.n, (const int32_t *)L.pptr,
-                           (const int32_t *)L.pcol, (const VT *)L.pval, (const double *)N.x, L.x, done);
-        double *dc = L.d0, *dn = L.d1;
-        hipLaunchKernelGGL((k_mg_resid<std::decay_t<decltype(op)>>), dim3(kBlocks), dim3(kThreads), 0, s, op,
-                           (const double *)L.b, (const double *)L.x, L.r, done);
-        hipLaunchKernelGGL(k_mg_cheb0, dim3(ew), dim3(256), 0, s, L.n, (const double *)L.dinv, (const double *)L.r,
-                           dc, L.x, lam, ratio, done);
-        smooth_steps(dc, dn, op);
-    };
-    if (l == 0 && mg->fine16) {  // the fine-level passes on the half-precision copy of the set-up matrix
+    if constexpr (std::is_same<VT, float>::value) {
+        if (l == 0) {
+            // FP32 matrix: the fine level's vectors and block-Jacobi inverses in FP32 too (the vectors inside the
+            // level's own buffers, 3n doubles each: r and x share the first), the caller's b read and its result
+            // written in FP64
+            float *rf = reinterpret_cast<float *>(L.r), *xf = rf + 3 * L.n;
+            float *da = reinterpret_cast<float *>(L.d0), *db = reinterpret_cast<float *>(L.d1);
+            const float *dinv = mg->dinv32;
+            if (mg->fine16) {  // the fine-level passes on the half-precision copy of the set-up matrix
+                FineOp<_Float16> F = fine_op<_Float16>(mg, mg->vals16);
+                F.scale = mg->scale16;
+                RC(run_level(F, dinv, rf, xf, da, db));
+            } else {
+                RC(run_level(fine_op<VT>(mg, vals), dinv, rf, xf, da, db));
+            }
+            hipLaunchKernelGGL(k_mg_widen, dim3(ew), dim3(256), 0, s, 3 * L.n, (const float *)xf, L.x, done);
+            HIP_CHECK(hipGetLastError());
+            return 0;
+        }
+    }
+    if (l == 0 && mg->fine16) {
         FineOp<_Float16> F = fine_op<_Float16>(mg, mg->vals16);
         F.scale = mg->scale16;
-        level_body(F);
-        RC(vcycle<VT>(mg, l + 1, vals, done, s));
-        post(F);
+        RC(run_level(F, (const double *)L.dinv, L.r, L.x, L.d0, L.d1));
     } else if (l == 0) {
-        const FineOp<VT> F = fine_op<VT>(mg, vals);
-        level_body(F);
-        RC(vcycle<VT>(mg, l + 1, vals, done, s));
-        post(F);
+        RC(run_level(fine_op<VT>(mg, vals), (const double *)L.dinv, L.r, L.x, L.d0, L.d1));
     } else {
-        const BsrOp B = bsr_op(L);
-        level_body(B);
-        RC(vcycle<VT>(mg, l + 1, vals, done, s));
-        post(B);
+        RC(run_level(bsr_op(L), (const double *)L.dinv, L.r, L.x, L.d0, L.d1));
     }
     HIP_CHECK(hipGetLastError());
     return 0;
@@ -1170,13 +1215,15 @@ extern "C" int ewfe_mg_level_info(const ewfe_mg *mg, int level, int64_t *n_nodes
 extern "C" int ewfe_mg_setup(ewfe_mg *mg, const void *vals, int vals_dtype, const int32_t *rows_dev, int64_t n_rows,
                              void *stream)
 {
-    if (!mg || !vals || (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) || (n_rows > 0 && !rows_dev)) {
+    if (!mg || !vals || (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32 && vals_dtype != EWFE_F32_D64) ||
+        (n_rows > 0 && !rows_dev)) {
         ewfe::set_error("ewfe_mg_setup: invalid arguments");
         return EWFE_EINVAL;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     mg->ready = false;
     if (vals_dtype == EWFE_F64) return mg_setup<double>(mg, vals, rows_dev, n_rows, s);
+    // EWFE_F32_D64: the hierarchy is built from the FP32 values in front of the FP64 diagonal blocks
     return mg_setup<float>(mg, vals, rows_dev, n_rows, s);
 }
 

@@ -81,6 +81,20 @@ struct ewfe_plan {
 
 namespace ewfe {
 void set_error(const std::string &msg);
+// EWFE_F32_D64 values: nnz floats, then (from a 256-byte boundary) 9 doubles per row node
+inline int64_t d64_offset_bytes(int64_t nnz) { return (4 * nnz + 255) / 256 * 256; }
+inline double *d64_blocks(void *vals, int64_t nnz)
+{
+    return reinterpret_cast<double *>(reinterpret_cast<char *>(vals) + d64_offset_bytes(nnz));
+}
+inline bool d64_supported(const ewfe_plan *p)
+{
+    return p->rows_cover_all_dofs && p->d_rn_meta != nullptr && p->d_diag_rank != nullptr && p->d_nbcol != nullptr;
+}
+// EWFE_F32_D64: the FP64 diagonal blocks rebuilt from the FP32 off-diagonal values (stream-ordered)
+int rebuild_diag64(const ewfe_plan *p, void *vals, void *stream);
+// EWFE_F32_D64: the FP64 diagonal blocks' rows of Dirichlet dofs made unit rows (stream-ordered)
+int dirichlet_diag64(const ewfe_plan *p, const int32_t *rows, int64_t n_rows, void *vals, void *stream);
 // Builds pattern, colouring and slot map on the host; fills everything but the device pointers.
 // Returns 0 or a negative error code. edof_host ([npe][n_el]) and colpos_host ([n_el][npe*npe]) receive the
 // device images.

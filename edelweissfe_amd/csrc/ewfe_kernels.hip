@@ -448,6 +448,7 @@ struct EvalArgs {
     double *state_t;
     double *P, *F;
     void *vals;
+    double *d64;  // EWFE_F32_D64: FP64 corrections of the diagonal blocks (9 per row node), else null
     double *pnewdt;
     int64_t n_el;  // SoA stride
     int64_t p_begin, p_end;
@@ -882,14 +883,14 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
     }
 }
 
-// Precision of the per-point data and of the tangent arithmetic of k_hex8_elem: that of the records, except for
-// J2, whose FP32 records hold the FP64 tangent rounded once. Its consistent tangent has the deviatoric stiffness
-// 2G - a2 - a1 (about 2H/3, 1/240 of 2G for the benchmark's hardening) along the flow direction as a difference of
-// terms of size 2G; formed in FP32 it carries ~240 times the rounding of its terms in exactly the mode that governs
-// Newton's convergence (the 128^3 J2 schedule needed 12 instead of 10 solves). The matrix-core tangent keeps FP64
-// point data and rounds its operands when it packs them.
+// Precision of the per-point data and of the tangent arithmetic of k_hex8_elem: that of the records. (The J2
+// consistent tangent has the deviatoric stiffness 2G - a2 - a1, about 2H/3, along the flow direction as a difference
+// of terms of size 2G, so FP32 arithmetic puts ~240 times the rounding of its terms into that soft mode; with the
+// assembled FP32 diagonal blocks the 128^3 J2 schedule needed 12 instead of 7 solves, with the FP64 diagonal
+// corrections of EWFE_F32_D64, which make every row sum to zero again, it needs the FP64 tangent's 7.) The
+// matrix-core tangent keeps FP64 point data and rounds its operands when it packs them.
 template <int MAT, typename VT, bool MF>
-using hex8_point_t = typename std::conditional<MF || MAT == M_J2, double, VT>::type;
+using hex8_point_t = typename std::conditional<MF, double, VT>::type;
 
 // occupancy the compiler is asked for: float point data and accumulators fit 3 waves per SIMD (4 waves need
 // register spills)
@@ -1463,8 +1464,8 @@ template <typename VT>
 __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__restrict__ rorder,
                                                  const VT *__restrict__ kbuf, const double *__restrict__ fe,
                                                  VT *__restrict__ vals, double *__restrict__ P,
-                                                 double *__restrict__ F, int64_t q_begin, int64_t count,
-                                                 int64_t xcd_chunk)
+                                                 double *__restrict__ F, double *__restrict__ d64, int64_t q_begin,
+                                                 int64_t count, int64_t xcd_chunk)
 {
     // per half: block (a, b) of the incident element at corner o in slot 8 o + b, in the records' precision (9
     // values of a slot of SB = 10 doubles or 12 floats, 16-byte multiples); the spare double at value FO of slot
@@ -1545,6 +1546,23 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
         }
         P[3 * r + c] = f;
         F[3 * r + c] = fa;
+    }
+    // EWFE_F32_D64: E = -(sum of the rounded off-diagonal blocks) - (rounded diagonal block), summed over the half by
+    // a fixed butterfly (lane m holds block m: the lane assignment and order of the separate rebuild pass)
+    if (d64) {
+        const int dr = (lo0 * c1 + lo1) * c2 + lo2;  // rank of the node itself (offset 0, 0, 0)
+        double e[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const double v = (double)(VT)acc[t];
+            double x = (m == dr) ? 0.0 : v;
+#pragma unroll
+            for (int off = 16; off > 0; off >>= 1) x += __shfl_xor(x, off, 32);
+            e[t] = -x - v;
+        }
+        if (live && m == dr)
+#pragma unroll
+            for (int t = 0; t < 9; ++t) d64[9 * r + t] = e[t];
     }
     // 3) the node's 3 CSR rows: FP64 staged in its half's block area and written in 16-byte units
     VT *const row = vals + base;
@@ -2835,7 +2853,7 @@ static int launch_hex8(ewfe_plan *p, EvalArgs A, hipStream_t s, bool mfma)
         if (p->structured) {
             GridArgs G{p->grid_el[0], p->grid_el[1], p->grid_el[2]};
             hipLaunchKernelGGL((k_gather8s<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, g, G, p->d_rorder,
-                               (const VT *)p->d_kbuf, (const double *)p->d_fe, (VT *)A.vals, A.P, A.F,
+                               (const VT *)p->d_kbuf, (const double *)p->d_fe, (VT *)A.vals, A.P, A.F, A.d64,
                                p->pipe_row[c], count, chunk);
         } else {
             hipLaunchKernelGGL((k_gather8<VT>), dim3((unsigned)(8 * chunk)), dim3(64), 0, g,
@@ -2951,8 +2969,15 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
         ewfe::set_error("ewfe_eval_assemble: material needs state_n/state_t buffers");
         return EWFE_EINVAL;
     }
-    if (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32 && vals_dtype != EWFE_F32_MFMA) {
-        ewfe::set_error("ewfe_eval_assemble: vals_dtype must be EWFE_F64, EWFE_F32 or EWFE_F32_MFMA");
+    if (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32 && vals_dtype != EWFE_F32_MFMA &&
+        vals_dtype != EWFE_F32_D64 && vals_dtype != EWFE_F32_MFMA_D64) {
+        ewfe::set_error("ewfe_eval_assemble: vals_dtype must be EWFE_F64, EWFE_F32, EWFE_F32_MFMA, EWFE_F32_D64 or "
+                        "EWFE_F32_MFMA_D64");
+        return EWFE_EINVAL;
+    }
+    const bool d64 = vals_dtype == EWFE_F32_D64 || vals_dtype == EWFE_F32_MFMA_D64;
+    if (d64 && !ewfe::d64_supported(p)) {
+        ewfe::set_error("ewfe_eval_assemble: EWFE_F32_D64 needs a plan whose row nodes cover every dof");
         return EWFE_EINVAL;
     }
     if (!p->d_edof) {
@@ -2998,13 +3023,18 @@ extern "C" int ewfe_eval_assemble(const ewfe_plan *p, int mat_kind, const double
     A.P = P_dev;
     A.F = F_dev;
     A.vals = csr_vals_dev;
+    A.d64 = d64 ? ewfe::d64_blocks(csr_vals_dev, p->nnz) : nullptr;
     A.pnewdt = pnewdt_dev;
     A.n_el = p->n_el;
     A.p_begin = A.p_end = 0;
     for (int i = 0; i < 8; ++i) A.mat.p[i] = mat_params_host[i];
     ewfe_plan *pm = const_cast<ewfe_plan *>(p);  // the pipelined evaluation creates its second stream once
     if (vals_dtype == EWFE_F64) return dispatch<double>(pm, mat_kind, A, s, false);
-    return dispatch<float>(pm, mat_kind, A, s, vals_dtype == EWFE_F32_MFMA);
+    const bool mfma = vals_dtype == EWFE_F32_MFMA || vals_dtype == EWFE_F32_MFMA_D64;
+    if ((rc = dispatch<float>(pm, mat_kind, A, s, mfma))) return rc;
+    // the structured 8-node gather writes the diagonal corrections itself; other paths take a pass of their own
+    const bool fused = p->npe == 8 && p->structured;
+    return d64 && !fused ? ewfe::rebuild_diag64(p, csr_vals_dev, stream) : 0;
 }
 
 extern "C" int ewfe_adjoint_param_grad(const ewfe_plan *p, int mat_kind, const double *mat_params_host,
@@ -3061,7 +3091,8 @@ extern "C" int ewfe_csr_accumulate(const ewfe_plan *u, int32_t block, const void
                                    int vals_dtype, void *stream)
 {
     if (!u || block < 0 || block >= (int32_t)u->d_umap.size() || !src_vals_dev || !dst_vals_dev ||
-        (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32)) {
+        (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32 && vals_dtype != EWFE_F32_D64) ||
+        (vals_dtype == EWFE_F32_D64 && !ewfe::d64_supported(u))) {
         ewfe::set_error("ewfe_csr_accumulate: invalid arguments (a system plan of ewfe_plan_create_union and one of "
                         "its block indices)");
         return EWFE_EINVAL;
@@ -3077,7 +3108,9 @@ extern "C" int ewfe_csr_accumulate(const ewfe_plan *u, int32_t block, const void
         hipLaunchKernelGGL(k_csr_accumulate<float>, dim3(blocks), dim3(256), 0, s, u->d_umap[block], n,
                            (const float *)src_vals_dev, (float *)dst_vals_dev);
     HIP_CHECK(hipGetLastError());
-    return 0;
+    // EWFE_F32_D64: the system's diagonal blocks rebuilt from its off-diagonal values as they stand (the block's own
+    // FP64 blocks are not read; after the last block the system's are those of the summed matrix)
+    return vals_dtype == EWFE_F32_D64 ? ewfe::rebuild_diag64(u, dst_vals_dev, stream) : 0;
 }
 
 extern "C" int ewfe_dirichlet_rows(const ewfe_plan *p, const int32_t *rows_dev, int64_t n_rows, void *csr_vals_dev,
@@ -3092,6 +3125,12 @@ extern "C" int ewfe_dirichlet_rows(const ewfe_plan *p, const int32_t *rows_dev, 
         ewfe::set_error("ewfe_dirichlet_rows: plan has no device data");
         return EWFE_EINVAL;
     }
+    if ((vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32 && vals_dtype != EWFE_F32_D64) ||
+        (vals_dtype == EWFE_F32_D64 && !ewfe::d64_supported(p))) {
+        ewfe::set_error("ewfe_dirichlet_rows: vals_dtype must be EWFE_F64, EWFE_F32 or EWFE_F32_D64 (the latter on a "
+                        "plan whose row nodes cover every dof)");
+        return EWFE_EINVAL;
+    }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int64_t blocks = (n_rows + 3) / 4;
     if (vals_dtype == EWFE_F64)
@@ -3101,7 +3140,7 @@ extern "C" int ewfe_dirichlet_rows(const ewfe_plan *p, const int32_t *rows_dev, 
         hipLaunchKernelGGL(k_dirichlet_rows<float>, dim3((unsigned)blocks), dim3(256), 0, s, rows_dev, n_rows,
                            p->d_indptr, p->d_indices, reinterpret_cast<float *>(csr_vals_dev));
     HIP_CHECK(hipGetLastError());
-    return 0;
+    return vals_dtype == EWFE_F32_D64 ? ewfe::dirichlet_diag64(p, rows_dev, n_rows, csr_vals_dev, stream) : 0;
 }
 
 extern "C" int ewfe_body_force(ewfe_plan *p, const double *X_dev, const uint8_t *el_mask_dev,

@@ -48,6 +48,7 @@ struct Mat {
     const int32_t *indptr, *indices;                      // CSR
     const int32_t *rn_dof, *nb_ptr, *nbcol, *diag_rank;  // row nodes
     const int4 *rn_meta;                                  // {first dof, CSR offset, nb_ptr, degree} per row node
+    const double *d64;  // EWFE_F32_D64: FP64 corrections of the diagonal blocks (9 per row node); else null
     int64_t n, n_rownodes;
     bool blocked;
 };
@@ -103,6 +104,13 @@ __device__ __forceinline__ void node_dot(const Mat &M, const VT *__restrict__ va
     const int32_t base = meta.y, nb0 = meta.z, deg = meta.w;
     const int32_t L = 3 * deg;
     double a0 = 0, a1 = 0, a2 = 0;
+    if (M.d64 && lane32 == 0) {  // EWFE_F32_D64: E x_node starts lane 0's sums (loaded beside the blocks)
+        const double *E = M.d64 + 9 * rn;
+        const double x0 = x[meta.x], x1 = x[meta.x + 1], x2 = x[meta.x + 2];
+        a0 = E[0] * x0 + E[1] * x1 + E[2] * x2;
+        a1 = E[3] * x0 + E[4] * x1 + E[5] * x2;
+        a2 = E[6] * x0 + E[7] * x1 + E[8] * x2;
+    }
     for (int m = lane32; m < deg; m += kLanesPerNode) {
         const int32_t c = M.nbcol[nb0 + m];
         const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
@@ -159,7 +167,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_setup(Mat M, const VT *vals, d
             const int32_t off = base + 3 * M.diag_rank[rn];
             double B[3][3];
             for (int i = 0; i < 3; ++i)
-                for (int k = 0; k < 3; ++k) B[i][k] = (double)vals[off + i * L + k];
+                for (int k = 0; k < 3; ++k)
+                    B[i][k] = (double)vals[off + i * L + k] + (M.d64 ? M.d64[9 * rn + 3 * i + k] : 0.0);
             const double det = B[0][0] * (B[1][1] * B[2][2] - B[1][2] * B[2][1]) -
                                B[0][1] * (B[1][0] * B[2][2] - B[1][2] * B[2][0]) +
                                B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0]);
@@ -448,9 +457,10 @@ __global__ __launch_bounds__(kThreads) void k_pcg_dot(int64_t n, const double *r
     block_partials<1>(acc, partials, kBlocks);
 }
 
-Mat make_mat(const ewfe_plan *p)
+Mat make_mat(const ewfe_plan *p, const double *d64 = nullptr)
 {
     Mat M;
+    M.d64 = d64;
     M.indptr = p->d_indptr;
     M.indices = p->d_indices;
     M.rn_dof = p->d_rownode_dof;
@@ -465,10 +475,11 @@ Mat make_mat(const ewfe_plan *p)
 }
 
 template <typename VT>
-int pcg(const ewfe_plan *p, const VT *vals, const double *b, double *x, const int32_t *rows, int64_t n_rows,
-        double rtol, int maxit, int check_every, double *ws, int *iters_out, double *relres_out, hipStream_t s)
+int pcg(const ewfe_plan *p, const VT *vals, const double *d64, const double *b, double *x, const int32_t *rows,
+        int64_t n_rows, double rtol, int maxit, int check_every, double *ws, int *iters_out, double *relres_out,
+        hipStream_t s)
 {
-    const Mat M = make_mat(p);
+    const Mat M = make_mat(p, d64);
     const int64_t n = p->n_dof;
     double *r = ws, *z = ws + n, *pp = ws + 2 * n, *Ap = ws + 3 * n, *pre = ws + 4 * n;
     double *partials = pre + 3 * n, *scal = partials + 3 * kBlocks;
@@ -510,11 +521,11 @@ int pcg(const ewfe_plan *p, const VT *vals, const double *b, double *x, const in
 }
 
 template <typename VT>
-int pcg_mg(ewfe_mg *mg, const ewfe_plan *p, int vals_dtype, const VT *vals, const double *b, double *x,
-           const int32_t *rows, int64_t n_rows, double rtol, int maxit, int check_every, double *ws, int *iters_out,
-           double *relres_out, hipStream_t s)
+int pcg_mg(ewfe_mg *mg, const ewfe_plan *p, int vals_dtype, const VT *vals, const double *d64, const double *b,
+           double *x, const int32_t *rows, int64_t n_rows, double rtol, int maxit, int check_every, double *ws,
+           int *iters_out, double *relres_out, hipStream_t s)
 {
-    const Mat M = make_mat(p);
+    const Mat M = make_mat(p, d64);
     const int64_t n = p->n_dof;
     double *r = ws, *z = ws + n, *pp = ws + 2 * n, *Ap = ws + 3 * n;
     double *partials = ws + 7 * n, *scal = partials + 3 * kBlocks;
@@ -565,15 +576,114 @@ int pcg_mg(ewfe_mg *mg, const ewfe_plan *p, int vals_dtype, const VT *vals, cons
     return 0;
 }
 
+// Values of a vals_dtype: FP64 (f64 true), FP32, or FP32 with FP64 diagonal blocks (d64 set); false for another
+// dtype, or for EWFE_F32_D64 on a plan without row nodes covering every dof
+bool parse_vals(const ewfe_plan *p, const void *vals, int vals_dtype, bool &f64, const double *&d64)
+{
+    f64 = vals_dtype == EWFE_F64;
+    d64 = nullptr;
+    if (vals_dtype == EWFE_F64 || vals_dtype == EWFE_F32) return true;
+    if (vals_dtype != EWFE_F32_D64 || !p || !ewfe::d64_supported(p)) return false;
+    d64 = ewfe::d64_blocks(const_cast<void *>(vals), p->nnz);
+    return true;
+}
+
+// EWFE_F32_D64: E_r = -(sum of row node r's off-diagonal blocks) - (its diagonal block), the FP32 values summed in
+// FP64 (32 lanes per node, a fixed lane assignment and butterfly: bitwise reproducible). K_e of every element maps a
+// rigid translation to zero (the shape-function gradients of an element sum to zero), so the FP64 matrix has
+// diagonal blocks equal to minus the sums of the off-diagonal ones up to its own rounding; with E added to the
+// rounded diagonal blocks every row of the stored matrix sums to zero again.
+__global__ __launch_bounds__(kThreads) void k_rebuild_diag64(Mat M, const float *__restrict__ vals,
+                                                             double *__restrict__ d64)
+{
+    const int lane32 = threadIdx.x % kLanesPerNode;
+    for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
+         rn += (int64_t)gridDim.x * kNodesPerBlock) {
+        const int4 meta = M.rn_meta[rn];
+        const int32_t base = meta.y, deg = meta.w;
+        const int32_t L = 3 * deg;
+        const int32_t dr = M.diag_rank[rn];
+        double a[9], dg[9];
+        for (int t = 0; t < 9; ++t) a[t] = dg[t] = 0.0;
+        for (int m = lane32; m < deg; m += kLanesPerNode) {
+            const float *v = vals + base + 3 * m;
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    const double x = (double)v[i * L + k];
+                    if (m == dr)
+                        dg[3 * i + k] = x;
+                    else
+                        a[3 * i + k] += x;
+                }
+        }
+        double out = 0.0;
+        for (int t = 0; t < 9; ++t) {
+            // the butterfly of the structured gather's fused version: xor 16, 8, 4, 2, 1
+            double x = a[t];
+            for (int off = 16; off > 0; off >>= 1) x += __shfl_xor(x, off, 32);
+            const double d = group_sum32(dg[t]);  // the one lane holding the block: exact
+            if (lane32 == t) out = -x - d;
+        }
+        if (lane32 < 9) d64[9 * rn + lane32] = out;
+    }
+}
+
+// EWFE_F32_D64 Dirichlet rows: row c of the diagonal correction of the row's node becomes zero, leaving the unit row
+// of the FP32 values (row nodes cover every dof, so node r holds dofs 3 r .. 3 r + 2)
+__global__ void k_dirichlet_diag64(const int32_t *__restrict__ rows, int64_t n_rows, double *__restrict__ d64)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_rows) return;
+    const int32_t row = rows[i];
+    double *D = d64 + 9 * (int64_t)(row / 3) + 3 * (row % 3);
+    for (int k = 0; k < 3; ++k) D[k] = 0.0;
+}
+
 }  // namespace
+
+int ewfe::rebuild_diag64(const ewfe_plan *p, void *vals, void *stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (p->n_rownodes == 0) return 0;
+    hipLaunchKernelGGL(k_rebuild_diag64, dim3(kBlocks), dim3(kThreads), 0, s, make_mat(p), (const float *)vals,
+                       ewfe::d64_blocks(vals, p->nnz));
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int ewfe::dirichlet_diag64(const ewfe_plan *p, const int32_t *rows, int64_t n_rows, void *vals, void *stream)
+{
+    if (n_rows <= 0) return 0;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_dirichlet_diag64, dim3((unsigned)((n_rows + 255) / 256)), dim3(256), 0, s, rows, n_rows,
+                       ewfe::d64_blocks(vals, p->nnz));
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int64_t ewfe_vals_bytes(const ewfe_plan *p, int vals_dtype)
+{
+    if (!p) {
+        ewfe::set_error("ewfe_vals_bytes: null plan");
+        return EWFE_EINVAL;
+    }
+    if (vals_dtype == EWFE_F64) return 8 * p->nnz;
+    if (vals_dtype == EWFE_F32 || vals_dtype == EWFE_F32_MFMA) return 4 * p->nnz;
+    if (vals_dtype == EWFE_F32_D64 && ewfe::d64_supported(p)) return ewfe::d64_offset_bytes(p->nnz) + 72 * p->n_rownodes;
+    ewfe::set_error("ewfe_vals_bytes: dtype must be EWFE_F64, EWFE_F32, EWFE_F32_MFMA or EWFE_F32_D64 (the latter on a "
+                    "plan whose row nodes cover every dof)");
+    return EWFE_EINVAL;
+}
 
 extern "C" int ewfe_pcg_mg(ewfe_mg *mg, const void *vals, int vals_dtype, const double *b, double *x,
                            const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every,
                            void *workspace, int *iters_out, double *relres_out, void *stream)
 {
     const ewfe_plan *p = ewfe::mg_plan(mg);
+    bool f64;
+    const double *d64;
     if (!p || !vals || !b || !x || !workspace || maxit < 0 || check_every <= 0 ||
-        (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) || !(rtol >= 0.0) || (n_rows > 0 && !rows_dev)) {
+        !parse_vals(p, vals, vals_dtype, f64, d64) || !(rtol >= 0.0) || (n_rows > 0 && !rows_dev)) {
         ewfe::set_error("ewfe_pcg_mg: invalid arguments");
         return EWFE_EINVAL;
     }
@@ -583,11 +693,12 @@ extern "C" int ewfe_pcg_mg(ewfe_mg *mg, const void *vals, int vals_dtype, const 
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     double *ws = reinterpret_cast<double *>(workspace);
-    if (vals_dtype == EWFE_F64)
-        return pcg_mg<double>(mg, p, vals_dtype, (const double *)vals, b, x, rows_dev, n_rows, rtol, maxit,
+    if (f64)
+        return pcg_mg<double>(mg, p, EWFE_F64, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit,
                               check_every, ws, iters_out, relres_out, s);
-    return pcg_mg<float>(mg, p, vals_dtype, (const float *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every,
-                         ws, iters_out, relres_out, s);
+    // the V-cycle reads the FP32 values (the hierarchy is set up from them); the CG operator takes the FP64 blocks
+    return pcg_mg<float>(mg, p, EWFE_F32, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit,
+                         check_every, ws, iters_out, relres_out, s);
 }
 
 extern "C" int64_t ewfe_pcg_workspace_size(const ewfe_plan *p)
@@ -599,13 +710,15 @@ extern "C" int64_t ewfe_pcg_workspace_size(const ewfe_plan *p)
 extern "C" int ewfe_spmv(const ewfe_plan *p, const void *vals, int vals_dtype, const double *x, double *y,
                          void *stream)
 {
-    if (!p || !p->d_indptr || !vals || !x || !y || (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32)) {
+    bool f64;
+    const double *d64;
+    if (!p || !p->d_indptr || !vals || !x || !y || !parse_vals(p, vals, vals_dtype, f64, d64)) {
         ewfe::set_error("ewfe_spmv: invalid arguments");
         return EWFE_EINVAL;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const Mat M = make_mat(p);
-    if (vals_dtype == EWFE_F64)
+    const Mat M = make_mat(p, d64);
+    if (f64)
         hipLaunchKernelGGL(k_spmv<double>, dim3(kBlocks), dim3(kThreads), 0, s, M, (const double *)vals, x, y);
     else
         hipLaunchKernelGGL(k_spmv<float>, dim3(kBlocks), dim3(kThreads), 0, s, M, (const float *)vals, x, y);
@@ -622,7 +735,8 @@ extern "C" int ewfe_spmv(const ewfe_plan *p, const void *vals, int vals_dtype, c
 template <typename VT>
 __global__ __launch_bounds__(kThreads) void k_residual_dot2(const int32_t *__restrict__ indptr,
                                                           const int32_t *__restrict__ indices,
-                                                          const VT *__restrict__ vals, const double *__restrict__ b,
+                                                          const VT *__restrict__ vals, const double *__restrict__ d64,
+                                                          const double *__restrict__ b,
                                                           const double *__restrict__ x, double *__restrict__ r,
                                                           int64_t n)
 {
@@ -630,8 +744,12 @@ __global__ __launch_bounds__(kThreads) void k_residual_dot2(const int32_t *__res
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         double s = b[i], c = 0.0;
+        const int64_t d0 = i - i % 3;  // EWFE_F32_D64: first dof of the row's node (columns d0 .. d0 + 2: the block)
         for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
-            const double a = (double)vals[k], y = -x[indices[k]];
+            const int32_t col = indices[k];
+            // row i % 3 of node i / 3's correction: 9 (i / 3) + 3 (i % 3) = 3 i; value + correction is exact
+            const double a = (double)vals[k] + ((d64 && col >= d0 && col < d0 + 3) ? d64[3 * i + (col - d0)] : 0.0);
+            const double y = -x[col];
             const double p = a * y;
             const double pe = fma(a, y, -p);  // a y = p + pe exactly
             const double t = s + p;
@@ -647,19 +765,21 @@ __global__ __launch_bounds__(kThreads) void k_residual_dot2(const int32_t *__res
 extern "C" int ewfe_residual(const ewfe_plan *p, const void *vals, int vals_dtype, const double *b, const double *x,
                              double *r, void *stream)
 {
-    if (!p || !p->d_indptr || !vals || !b || !x || !r || (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32)) {
+    bool f64;
+    const double *d64;
+    if (!p || !p->d_indptr || !vals || !b || !x || !r || !parse_vals(p, vals, vals_dtype, f64, d64)) {
         ewfe::set_error("ewfe_residual: invalid arguments");
         return EWFE_EINVAL;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int64_t n = p->n_dof;
     const unsigned blocks = (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 65536);
-    if (vals_dtype == EWFE_F64)
+    if (f64)
         hipLaunchKernelGGL(k_residual_dot2<double>, dim3(blocks), dim3(kThreads), 0, s, p->d_indptr, p->d_indices,
-                           (const double *)vals, b, x, r, n);
+                           (const double *)vals, (const double *)nullptr, b, x, r, n);
     else
         hipLaunchKernelGGL(k_residual_dot2<float>, dim3(blocks), dim3(kThreads), 0, s, p->d_indptr, p->d_indices,
-                           (const float *)vals, b, x, r, n);
+                           (const float *)vals, d64, b, x, r, n);
     HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -668,17 +788,19 @@ extern "C" int ewfe_pcg(const ewfe_plan *p, const void *vals, int vals_dtype, co
                         const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every,
                         void *workspace, int *iters_out, double *relres_out, void *stream)
 {
+    bool f64;
+    const double *d64;
     if (!p || !p->d_indptr || !vals || !b || !x || !workspace || maxit < 0 || check_every <= 0 ||
-        (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) || !(rtol >= 0.0) || (n_rows > 0 && !rows_dev)) {
+        !parse_vals(p, vals, vals_dtype, f64, d64) || !(rtol >= 0.0) || (n_rows > 0 && !rows_dev)) {
         ewfe::set_error("ewfe_pcg: invalid arguments");
         return EWFE_EINVAL;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     double *ws = reinterpret_cast<double *>(workspace);
-    if (vals_dtype == EWFE_F64)
-        return pcg<double>(p, (const double *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
-                           relres_out, s);
-    return pcg<float>(p, (const float *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
+    if (f64)
+        return pcg<double>(p, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws,
+                           iters_out, relres_out, s);
+    return pcg<float>(p, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
                       relres_out, s);
 }
 
@@ -731,8 +853,10 @@ extern "C" int ewfe_pcg_noalloc(const ewfe_plan *p, const void *vals, int vals_d
                                 const int32_t *rows_dev, int64_t n_rows, double rtol, int maxit, int check_every,
                                 int *iters_out, double *relres_out, void *stream)
 {
+    bool f64;
+    const double *d64;
     if (!p || !p->d_indptr || !vals || !b || !x || maxit < 0 || check_every <= 0 ||
-        (vals_dtype != EWFE_F64 && vals_dtype != EWFE_F32) || !(rtol >= 0.0) || (n_rows > 0 && !rows_dev)) {
+        !parse_vals(p, vals, vals_dtype, f64, d64) || !(rtol >= 0.0) || (n_rows > 0 && !rows_dev)) {
         ewfe::set_error("ewfe_pcg_noalloc: invalid arguments");
         return EWFE_EINVAL;
     }
@@ -753,9 +877,9 @@ extern "C" int ewfe_pcg_noalloc(const ewfe_plan *p, const void *vals, int vals_d
     // the scalars sit after the vectors and the partial sums (the layout of pcg())
     hipLaunchKernelGGL(k_clear_scalars, dim3(1), dim3(64), 0, s, ws + 7 * p->n_dof + 3 * kBlocks);
     HIP_CHECK(hipGetLastError());
-    if (vals_dtype == EWFE_F64)
-        return pcg<double>(p, (const double *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
-                           relres_out, s);
-    return pcg<float>(p, (const float *)vals, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
+    if (f64)
+        return pcg<double>(p, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws,
+                           iters_out, relres_out, s);
+    return pcg<float>(p, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
                       relres_out, s);
 }

@@ -26,14 +26,17 @@ Differences that are part of the design (not of the semantics):
     the multigrid from 20,000 dofs on when the mesh allows it, block Jacobi otherwise), converged to
     ``pcgTolerance`` relative residual; the Newton loop's own flux/correction tolerances decide convergence
     exactly as in the reference;
-  * ``tangentPrecision="fp32"`` stores the tangent in FP32 (element tangents in FP32 arithmetic for 8-node
-    elements of the hyperelastic and linear elastic materials; from FP64 FMAs, rounded when stored, for J2 and for
-    20-node elements), ``"fp32-mfma"`` forms the element tangents in FP32 on the matrix cores;
-    residual, return mapping and update stay FP64 in both. Newton then converges to the same FP64 residual
-    tolerance, possibly in more iterations; its
-    linear systems are solved to ``pcgToleranceFP32`` (default 1e-4): an FP32 tangent already limits the
-    Newton contraction to ~1e-4..1e-3 per iteration, so a tighter correction buys nothing, while the
-    convergence test (FP64 residual and correction) is unchanged;
+  * ``tangentPrecision="fp32"`` stores the tangent in FP32 (element tangents of 8-node elements in FP32 arithmetic;
+    of 20-node elements from FP64 FMAs, rounded when stored), ``"fp32-mfma"`` forms the element tangents in FP32 on
+    the matrix cores; residual, return mapping and update stay FP64 in both. With ``fp32TangentDiagonal="rebuilt"``
+    (the default) the FP32 tangent carries FP64 corrections of its diagonal node blocks (``newValues("fp32-d64")``)
+    that make every row of the matrix sum to zero like the FP64 tangent's (rigid translations): rounding to FP32
+    otherwise limits Newton's contraction to ~1e-4..1e-3 per iteration at these condition numbers (the J2 cube at
+    128^3: 10-12 instead of 7 solves), with the corrections it converges in the FP64 tangent's iterations.
+    ``"assembled"`` keeps the plain FP32 matrix. Newton converges to the same FP64 residual tolerance either way,
+    extra iterations counted; the FP32 tangent's linear systems are solved to ``pcgToleranceFP32`` (default: the
+    FP64 ``pcgTolerance`` with rebuilt diagonals, 1e-4 with assembled ones, where a tighter solve buys nothing),
+    the convergence test (FP64 residual and correction) is unchanged;
   * the 1e-7 absolute floor of the flux test is the option ``fluxResidualFloor`` (default 1e-7);
   * ``multigridSetupPerIncrement=True`` sets the multigrid hierarchy up on the first solve of an increment only: the
     later corrections of the increment are preconditioned by that hierarchy (its prolongator, coarse operators,
@@ -111,11 +114,12 @@ class NISTGPU:
         "linsolverConfigFile": "",
         "tangentPrecision": "fp64",
         "pcgTolerance": 1e-10,
-        "pcgToleranceFP32": 1e-4,
+        "pcgToleranceFP32": None,
         "pcgMaxIter": 50000,
         "linsolverRefinement": 0,
         "multigridSetupPerIncrement": False,
         "multigridFinePrecision": "matrix",
+        "fp32TangentDiagonal": "rebuilt",
         "fluxResidualFloor": 1e-7,
         "device": "cuda",
     }
@@ -146,7 +150,10 @@ class NISTGPU:
     def _updateOptions(self, updatedOptions, journal):
         for k, v in updatedOptions.items():
             if k in self.NISTOptions:
-                self.options[k] = type(self.NISTOptions[k])(v)
+                default = self.NISTOptions[k]
+                # options without a default value (None) are tolerances: floats, or None for the default rule
+                self.options[k] = (None if v is None or (isinstance(v, str) and v.lower() == "none") else float(v)) \
+                    if default is None else type(default)(v)
             else:
                 raise AttributeError("Invalid option {:} for {:}".format(k, self.identification))
 
@@ -222,7 +229,10 @@ class NISTGPU:
         if tp not in ("fp64", "fp32", "fp32-mfma"):
             raise AttributeError("invalid tangentPrecision {:} (fp64, fp32, fp32-mfma)".format(tp))
         self._mfmaTangent = tp == "fp32-mfma"
-        K = self.system.newValues("fp64" if tp == "fp64" else "fp32")
+        dg = self.options["fp32TangentDiagonal"].lower()
+        if dg not in ("assembled", "rebuilt"):
+            raise AttributeError("invalid fp32TangentDiagonal {:} (assembled, rebuilt)".format(dg))
+        K = self.system.newValues("fp64" if tp == "fp64" else ("fp32-d64" if dg == "rebuilt" else "fp32"))
 
         extrapolation = self.options["extrapolation"]
         maxIter, criticalIter = step.maxIter, step.criticalIter
@@ -425,9 +435,14 @@ class NISTGPU:
             self.linearSolveIterations.append((0, 0.0))
             self.computationTimes["linear solve"] += _time.perf_counter() - tic
             return x
-        # an FP32 tangent (entries rounded to ~6e-8) makes Newton contract linearly, by ~1e-4..1e-3 per
-        # iteration at these condition numbers: its corrections are solved only to pcgToleranceFP32
-        rtol = self.options["pcgToleranceFP32"] if A.dtype == self.torch.float32 else self.options["pcgTolerance"]
+        # a plain FP32 tangent (entries rounded to ~6e-8) makes Newton contract linearly, by ~1e-4..1e-3 per
+        # iteration at these condition numbers: its corrections are solved only to 1e-4 unless pcgToleranceFP32 says
+        # otherwise; with the diagonal corrections the FP32 tangent contracts like the FP64 one
+        rtol = self.options["pcgTolerance"]
+        if A.dtype == self.torch.float32:
+            rtol = self.options["pcgToleranceFP32"]
+            if rtol is None:
+                rtol = self.options["pcgTolerance"] if A.numel() != self.system.nnz else 1e-4
         amg = self._linsolver == "amg"
         if amg and not (self.options["multigridSetupPerIncrement"] and self._mgSetUpThisIncrement):
             self.system.multigridSetup(A, rows, finePrecision=self.options["multigridFinePrecision"])

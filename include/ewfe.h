@@ -64,8 +64,26 @@ enum ewfe_mat_kind {
  * difference of large terms) and of 20-node elements are formed with FP64 FMAs and rounded when they are stored; EWFE_F32_MFMA forms them in FP32 on the matrix
  * cores (v_mfma_f32_16x16x4_f32) for 8-node elements and for 20-node elements assembled through the row-node gather
  * (20-node plans on the colour path take it as EWFE_F32). The residual, nodal forces and state stay FP64. Only ewfe_eval_assemble takes EWFE_F32_MFMA; every other
- * call takes EWFE_F32 for FP32 values. */
-enum ewfe_dtype { EWFE_F64 = 0, EWFE_F32 = 1, EWFE_F32_MFMA = 2, EWFE_F16 = 3 /* ewfe_mg_set_fine_precision only */ };
+ * call takes EWFE_F32 for FP32 values.
+ * EWFE_F32_D64 stores the EWFE_F32 values (formed the same way, bitwise equal) followed, at byte offset
+ * ewfe_vals_bytes(plan, EWFE_F32) rounded up to 256, by FP64 corrections of the diagonal blocks: 9 doubles per row
+ * node (row-major, node of first dof 3 r at block r), E = -(sum of the node rows' off-diagonal FP32 blocks) - (its
+ * FP32 diagonal block). The matrix these values stand for has the diagonal blocks FP32 block + E, i.e. minus the
+ * sums of the off-diagonal FP32 blocks: an element's tangent maps a rigid translation to zero (its shape-function
+ * gradients sum to zero); rounding the values to FP32 breaks that, the corrections restore it exactly, which keeps
+ * Newton's contraction with an FP32 tangent close to that of the FP64 one. Every product, residual, Dirichlet edit
+ * and CG iteration adds them; the multigrid hierarchy is set up from the FP32 values alone. Needs a plan whose every
+ * dof is one of 3 consecutive dofs of a node (a row node).
+ * EWFE_F32_MFMA_D64 (ewfe_eval_assemble only): the EWFE_F32_MFMA tangent stored as EWFE_F32_D64; every other call
+ * takes such values as EWFE_F32_D64. No counterpart in the reference (its tangent is FP64). */
+enum ewfe_dtype {
+    EWFE_F64 = 0,
+    EWFE_F32 = 1,
+    EWFE_F32_MFMA = 2,
+    EWFE_F16 = 3, /* ewfe_mg_set_fine_precision only */
+    EWFE_F32_D64 = 4,
+    EWFE_F32_MFMA_D64 = 5 /* ewfe_eval_assemble only */
+};
 
 /* Error codes */
 #define EWFE_OK 0
@@ -128,6 +146,11 @@ int ewfe_plan_pattern(const ewfe_plan *plan, int32_t *indptr_host, int32_t *indi
 /* Device copies of the pattern (owned by the plan, valid until destroy). */
 int ewfe_plan_pattern_dev(const ewfe_plan *plan, const int32_t **indptr_dev, const int32_t **indices_dev);
 
+/* Bytes of a CSR value array of the plan in precision vals_dtype (8 nnz for EWFE_F64, 4 nnz for EWFE_F32 and
+ * EWFE_F32_MFMA, the layout described at enum ewfe_dtype for EWFE_F32_D64); EWFE_EINVAL for another dtype or for
+ * EWFE_F32_D64 on a plan without row nodes covering every dof. No counterpart in the reference. */
+int64_t ewfe_vals_bytes(const ewfe_plan *plan, int vals_dtype);
+
 /* Element order of the plan's quadrature-point state arrays: order_host[p] = element index stored at
  * position p (state layout is [component][qp][p], structure of arrays). n_colors_out may be NULL. */
 int ewfe_plan_element_order(const ewfe_plan *plan, int64_t *order_host, int32_t *n_colors_out);
@@ -177,7 +200,8 @@ int ewfe_plan_set_materials(ewfe_plan *plan, int32_t n_mat, const double *params
  *   state_n_dev      f64[nstate * nqp * n_el] converged state, plan order; NULL if nstate == 0
  *   state_t_dev      same layout, receives the trial state
  *   P_dev, F_dev     f64[n_dof] out: P = -f_int (element "P"), F = sum |Pe|
- *   csr_vals_dev     f64 or f32 [nnz] out: the assembled tangent, vals_dtype EWFE_F64, EWFE_F32 or EWFE_F32_MFMA
+ *   csr_vals_dev     out: the assembled tangent, vals_dtype EWFE_F64 (f64 [nnz]), EWFE_F32 or EWFE_F32_MFMA
+ *                    (f32 [nnz]), EWFE_F32_D64 or EWFE_F32_MFMA_D64 (ewfe_vals_bytes(plan, EWFE_F32_D64) bytes)
  *   pnewdt_dev       f64[1] out: 1e36, or < 1 if a quadrature point requests a cutback
  */
 int ewfe_eval_assemble(const ewfe_plan *plan, int mat_kind, const double *mat_params_host,

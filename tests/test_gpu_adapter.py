@@ -118,7 +118,7 @@ def test_two_blocks_evaluation_matches_oracle():
     assert e_k < 1e-12 and e_p < 1e-12 and rel(F.cpu().numpy(), outs[0]["F"] + outs[1]["F"]) < 1e-12
 
 
-@pytest.mark.parametrize("tangent", ["fp64", "fp32"])
+@pytest.mark.parametrize("tangent", ["fp64", "fp32", "fp32 rebuilt diagonal"])
 def test_two_blocks_newton_matches_oracle(tangent):
     from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
     from edelweissfe_amd.stepping import AdaptiveStep
@@ -129,7 +129,8 @@ def test_two_blocks_newton_matches_oracle(tangent):
     acts = {"dirichlet": {"bottom": R.Dirichlet(ref.nodeSets["gen_back"], {0: 0.0, 1: 0.0, 2: 0.0}),
                           "top": R.Dirichlet(ref.nodeSets["gen_front"], {2: 6e-3})}}
     step = AdaptiveStep(1, ref, acts, maxInc=0.25, startInc=0.25, minInc=1e-4, maxIter=25)
-    solver = NISTGPU(tangentPrecision=tangent, pcgTolerance=1e-14, linsolverRefinement=3)
+    solver = NISTGPU(tangentPrecision=tangent.split()[0], pcgTolerance=1e-14, linsolverRefinement=3,
+                     fp32TangentDiagonal="rebuilt" if "rebuilt" in tangent else "assembled")
     solver.solveStep(step, ref)
     r = O.solve_step(dict(X=X, blocks=oblocks),
                      dict(maxInc=0.25, startInc=0.25, minInc=1e-4, maxNumInc=1000, maxIter=25, extrapolation="linear",

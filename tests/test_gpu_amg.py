@@ -19,9 +19,10 @@ pytestmark = pytest.mark.gpu
 J2 = [210000.0, 0.3, 250.0, 1000.0, 0.0, 0.0]
 
 
-def _plastic_system(n, tangent, elastic_modulus=None):
+def _plastic_system(n, tangent, elastic_modulus=None, dirichlet=True):
     """Fully plastic J2 tangent of an n^3 jittered cube under uniaxial strain with the bench's Dirichlet set (or, with
-    `elastic_modulus`, the linear elastic tangent for that modulus)."""
+    `elastic_modulus`, the linear elastic tangent for that modulus); `tangent` is a value precision of newValues.
+    With dirichlet=False the Dirichlet rows are returned but not applied."""
     import torch
 
     X, conn, sets = jittered_box(n, "C3D8", seed=3)
@@ -44,7 +45,8 @@ def _plastic_system(n, tangent, elastic_modulus=None):
     rows = np.unique(np.concatenate([base[sets["gen_back"]] + 2, base[sets["gen_front"]] + 2,
                                      [base[org], base[org] + 1, base[xax] + 1]])).astype(np.int32)
     rows_t = torch.as_tensor(rows, device="cuda")
-    g.asm.dirichletRows(rows_t, vals)
+    if dirichlet:
+        g.asm.dirichletRows(rows_t, vals)
     rng = np.random.default_rng(5)
     b = rng.standard_normal(nd)
     b[rows] = 0.0
@@ -86,7 +88,8 @@ def test_amg_pcg_matches_direct_solve(tangent):
 def test_amg_pcg_half_precision_fine_passes(tangent):
     """ewfe_mg_set_fine_precision(EWFE_F16): the V-cycle's fine-level smoothing and residual passes read a scaled FP16
     copy of the set-up matrix. The preconditioner changes, the solution does not: PCG to 1e-12 matches the sparse
-    direct solve, in at most twice the iterations of the default; the same problem in other units (moduli and yield
+    direct solve, in at most twice the iterations of the default (FP32 matrices: whose default V-cycle runs FP32
+    vectors, so near 1e-12 either may need more); the same problem in other units (moduli and yield
     stress 1e6 times larger: entries far beyond FP16's range, 65504, unless the copy is scaled) is solved as well;
     switching back reproduces the default bitwise."""
     import scipy.sparse as sp
@@ -105,7 +108,7 @@ def test_amg_pcg_half_precision_fine_passes(tangent):
     K = sp.csr_matrix((vals.cpu().numpy().astype(np.float64), ix, ip), shape=(nd, nd))
     xr = spla.spsolve(K.tocsc(), b)
     assert rr < 1e-12 and rel(x.cpu().numpy(), xr) < 1e-9
-    assert it0 <= it <= 2 * it0
+    assert it <= 2 * it0
     # a linear elastic system in units with 1e6 times larger stresses (E = 2.1e11): modulus and the loads on the free
     # rows scale, the prescribed displacements (unit rows) do not, so the solution is that of E = 2.1e5
     ge, ke, _, rows_e, _ = _plastic_system(20, tangent, elastic_modulus=J2[0])
@@ -208,3 +211,135 @@ def test_amg_pcg_on_20_node_mesh():
     x2 = g.dm.constructDofVector()
     it2, _ = g.asm.pcg(vals, g.vec(b), x2, rows_t, rtol=1e-12, maxit=2000, check_every=4, precond="amg")
     assert it2 == it and bool((x2 == x).all().item())
+
+
+def _host_csr(g, vals, nd):
+    import scipy.sparse as sp
+
+    ip, ix = g.asm.pattern()
+    return sp.csr_matrix((g.asm.csrValues(vals).cpu().numpy().astype(np.float64), ix, ip), shape=(nd, nd))
+
+
+def _with_diagonal_corrections(K, E):
+    """K with E [n_nodes, 9] added to its 3x3 diagonal node blocks (node r: dofs 3r .. 3r + 2)."""
+    C = K.tocoo()
+    data = C.data.copy()
+    dg = C.row // 3 == C.col // 3
+    data[dg] += E[C.row[dg] // 3, 3 * (C.row[dg] % 3) + C.col[dg] % 3]
+    return type(K)((data, (C.row, C.col)), shape=K.shape).tocsr()
+
+
+def test_fp32_values_with_fp64_diagonal_blocks():
+    """EWFE_F32_D64 ("fp32-d64" values): the FP32 values are bitwise the EWFE_F32 ones, and each node's FP64 diagonal
+    correction makes its diagonal block minus the sum of its rows' off-diagonal FP32 values, so every row of the
+    matrix sums to zero, as the rows of a tangent do for a rigid translation (checked against the same sums on the
+    host, and the row sums of the FP64 tangent). SpMV, the compensated residual, Dirichlet rows and both CG variants
+    use the corrections (against the host matrix with them); and the solution with them is much closer to the FP64
+    tangent's than without."""
+    import scipy.sparse.linalg as spla
+
+    print("fp32 values with fp64 diagonal blocks, J2 8^3", flush=True)
+    g64, v64, rows, rows_t, b = _plastic_system(8, "fp64", dirichlet=False)
+    g32, v32, _, _, _ = _plastic_system(8, "fp32", dirichlet=False)
+    gd, vd, _, _, _ = _plastic_system(8, "fp32-d64", dirichlet=False)
+    nd = gd.dm.nDof
+    nnz = gd.asm.nnz
+    assert vd.numel() > nnz and vd.dtype == gd.torch.float32
+    assert bool((gd.asm.csrValues(vd) == v32).all().item())
+    K32 = _host_csr(g32, v32, nd)
+    K64 = _host_csr(g64, v64, nd)
+    offset = (4 * nnz + 255) // 256 * 256 // 4  # float index of the FP64 blocks (256-byte aligned)
+    assert vd.numel() == offset + 18 * (nd // 3)
+    E = vd[offset:].cpu().numpy().view(np.float64).reshape(-1, 9)
+    # host: minus the off-diagonal row sums per column component, and the FP32 diagonal blocks
+    C = K32.tocoo()
+    offd = C.row // 3 != C.col // 3
+    S = np.zeros((nd, 3))
+    np.add.at(S, (C.row[offd], C.col[offd] % 3), C.data[offd])
+    D32 = np.zeros((nd, 3))
+    np.add.at(D32, (C.row[~offd], C.col[~offd] % 3), C.data[~offd])
+    scale = np.abs(K64.data).max()
+    assert np.abs(D32.reshape(-1, 9) + E + S.reshape(-1, 9)).max() <= 1e-15 * scale
+    assert np.abs(E).max() <= 1e-5 * scale  # a rounding-sized correction
+    Kd = _with_diagonal_corrections(K32, E)
+    rs = np.zeros((nd, 3))
+    Cd = Kd.tocoo()
+    np.add.at(rs, (Cd.row, Cd.col % 3), Cd.data)
+    rs64 = np.zeros((nd, 3))
+    C64 = K64.tocoo()
+    np.add.at(rs64, (C64.row, C64.col % 3), C64.data)
+    rs32 = np.zeros((nd, 3))
+    np.add.at(rs32, (C.row, C.col % 3), C.data)
+    print(f"max row sum / max entry: fp64 {np.abs(rs64).max() / scale:.1e}, fp32 {np.abs(rs32).max() / scale:.1e}, "
+          f"fp32 + fp64 blocks {np.abs(rs).max() / scale:.1e}", flush=True)
+    assert np.abs(rs).max() <= 1e-14 * scale and np.abs(rs32).max() > 1e3 * np.abs(rs).max()
+    # SpMV and residual with the blocks
+    rng = np.random.default_rng(11)
+    xh = rng.standard_normal(nd)
+    y = gd.asm.spmv(vd, gd.vec(xh), gd.dm.constructDofVector()).cpu().numpy()
+    assert rel(y, Kd @ xh) < 1e-14
+    r = gd.asm.residual(vd, gd.vec(b), gd.vec(xh), gd.dm.constructDofVector()).cpu().numpy()
+    assert np.abs(r - (b - Kd @ xh)).max() <= 1e-14 * np.abs(Kd @ xh).max()
+    # Dirichlet rows: the FP32 values as for EWFE_F32 (unit rows), the corrections' rows of those dofs zero
+    gd.asm.dirichletRows(rows_t, vd)
+    g32.asm.dirichletRows(rows_t, v32)
+    g64.asm.dirichletRows(rows_t, v64)
+    assert bool((gd.asm.csrValues(vd) == v32).all().item())
+    Ed = vd[offset:].cpu().numpy().view(np.float64).reshape(-1, 9)
+    Eexp = E.copy()
+    for row in rows:
+        Eexp[row // 3, 3 * (row % 3): 3 * (row % 3) + 3] = 0.0
+    assert np.array_equal(Ed, Eexp)
+    Kd = _with_diagonal_corrections(_host_csr(g32, v32, nd), Ed)
+    xd = spla.spsolve(Kd.tocsc(), b)
+    for precond in ("jacobi", "amg"):
+        x = gd.dm.constructDofVector()
+        it, rr = gd.asm.pcg(vd, gd.vec(b), x, rows_t, rtol=1e-12, maxit=20000, check_every=4, precond=precond)
+        print(f"pcg {precond}: {it} iterations, relres {rr:.1e}", flush=True)
+        assert rr < 1e-12 and rel(x.cpu().numpy(), xd) < 1e-9
+    # closer to the FP64 tangent's solution
+    x64 = spla.spsolve(_host_csr(g64, v64, nd).tocsc(), b)
+    x32 = spla.spsolve(_host_csr(g32, v32, nd).tocsc(), b)
+    e32, ed = rel(x32, x64), rel(xd, x64)
+    print(f"solution vs FP64 tangent: fp32 {e32:.2e}, fp32 + fp64 blocks {ed:.2e}", flush=True)
+    assert ed < e32 / 4
+
+
+@pytest.mark.parametrize("case", ["C3D8 reversed element order", "C3D20", "C3D8TL"])
+def test_fp32_d64_corrections_on_other_assembly_paths(case):
+    """The diagonal corrections of EWFE_F32_D64 from the separate rebuild pass (meshes the structured 8-node gather does
+    not take, and 20-node elements): the FP32 values bitwise the EWFE_F32 ones, diagonal block + correction = minus the
+    sum of the off-diagonal FP32 blocks of its rows."""
+    print(f"fp32-d64 corrections: {case}", flush=True)
+    el = case.split()[0]
+    X, conn, _ = jittered_box(4, el, seed=7)
+    if "reversed" in case:
+        conn = conn[::-1].copy()
+    mat, mp = ("neohookeanwa", [80.0, 200.0]) if el.endswith("TL") else ("vonmises", J2)
+    out = {}
+    for dt in ("fp32", "fp32-d64"):
+        g = GpuEval(el, mat, mp, X, conn)
+        nd = g.dm.nDof
+        u = np.zeros((len(X), 3))
+        u[:, 2] = 5e-3 * X[:, 2]
+        u[:, 0] = -0.15e-2 * X[:, 0]
+        U = g.vec(u.ravel())
+        vals = g.asm.newValues(dt)
+        P, F = g.dm.constructDofVector(), g.dm.constructDofVector()
+        g.asm.evaluate(U, U, P, F, vals)
+        g.torch.cuda.synchronize()
+        out[dt] = (g, vals)
+    g, vd = out["fp32-d64"]
+    assert bool((g.asm.csrValues(vd) == out["fp32"][1]).all().item())
+    nnz = g.asm.nnz
+    E = vd[(4 * nnz + 255) // 256 * 64:].cpu().numpy().view(np.float64).reshape(-1, 9)
+    C = _host_csr(g, vd, nd).tocoo()
+    offd = C.row // 3 != C.col // 3
+    S = np.zeros((nd, 3))
+    np.add.at(S, (C.row[offd], C.col[offd] % 3), C.data[offd])
+    D32 = np.zeros((nd, 3))
+    np.add.at(D32, (C.row[~offd], C.col[~offd] % 3), C.data[~offd])
+    scale = np.abs(C.data).max()
+    err = np.abs(D32.reshape(-1, 9) + E + S.reshape(-1, 9)).max() / scale
+    print(f"|D32 + E + S| / max|K| = {err:.1e}, max|E| / max|K| = {np.abs(E).max() / scale:.1e}", flush=True)
+    assert err <= 1e-15 and 0 < np.abs(E).max() <= 1e-5 * scale
