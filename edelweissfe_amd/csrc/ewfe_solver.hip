@@ -96,7 +96,7 @@ __device__ __forceinline__ double row_dot(const Mat &M, const VT *vals, const do
 }
 
 // --- node-block rows: (K x)[d..d+2] of row node rn, summed over the 32-lane group ----------------------------
-template <typename VT>
+template <typename VT, bool D64>
 __device__ __forceinline__ void node_dot(const Mat &M, const VT *__restrict__ vals, const double *__restrict__ x,
                                          int64_t rn, int lane32, double &y0, double &y1, double &y2)
 {
@@ -104,14 +104,13 @@ __device__ __forceinline__ void node_dot(const Mat &M, const VT *__restrict__ va
     const int32_t base = meta.y, nb0 = meta.z, deg = meta.w;
     const int32_t L = 3 * deg;
     double a0 = 0, a1 = 0, a2 = 0;
-    if (M.d64 && lane32 == 0) {  // EWFE_F32_D64: E x_node starts lane 0's sums (loaded beside the blocks)
-        const double *E = M.d64 + 9 * rn;
-        const double x0 = x[meta.x], x1 = x[meta.x + 1], x2 = x[meta.x + 2];
-        a0 = E[0] * x0 + E[1] * x1 + E[2] * x2;
-        a1 = E[3] * x0 + E[4] * x1 + E[5] * x2;
-        a2 = E[6] * x0 + E[7] * x1 + E[8] * x2;
+    // EWFE_F32_D64 (a separate instantiation): E x_node joins the sums, one load per lane issued before the blocks'
+    // (lanes 0..8: E's entries, lanes 9..11: the node's x)
+    double ev = 0.0;
+    if constexpr (D64) {
+        if (lane32 < 12) ev = lane32 < 9 ? M.d64[9 * rn + lane32] : x[meta.x + lane32 - 9];
     }
-    for (int m = lane32; m < deg; m += kLanesPerNode) {
+    auto block = [&](int m) {
         const int32_t c = M.nbcol[nb0 + m];
         const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
         const VT *v0 = vals + base + 3 * m;
@@ -120,13 +119,24 @@ __device__ __forceinline__ void node_dot(const Mat &M, const VT *__restrict__ va
         a0 += (double)v0[0] * x0 + (double)v0[1] * x1 + (double)v0[2] * x2;
         a1 += (double)v1[0] * x0 + (double)v1[1] * x1 + (double)v1[2] * x2;
         a2 += (double)v2[0] * x0 + (double)v2[1] * x1 + (double)v2[2] * x2;
+    };
+    for (int m = lane32; m < deg; m += kLanesPerNode) block(m);
+    if constexpr (D64) {  // lane t < 9: E[t] x[t % 3] into row t / 3
+        const double xk = __shfl(ev, 9 + lane32 % 3, 32);
+        const double pr = lane32 < 9 ? ev * xk : 0.0;
+        if (lane32 < 3)
+            a0 += pr;
+        else if (lane32 < 6)
+            a1 += pr;
+        else
+            a2 += pr;
     }
     y0 = group_sum32(a0);
     y1 = group_sum32(a1);
     y2 = group_sum32(a2);
 }
 
-template <typename VT>
+template <typename VT, bool D64>
 __global__ __launch_bounds__(kThreads) void k_spmv(Mat M, const VT *vals, const double *x, double *y)
 {
     if (M.blocked) {
@@ -134,7 +144,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(Mat M, const VT *vals, const 
         for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
              rn += (int64_t)gridDim.x * kNodesPerBlock) {
             double y0, y1, y2;
-            node_dot(M, vals, x, rn, lane32, y0, y1, y2);
+            node_dot<VT, D64>(M, vals, x, rn, lane32, y0, y1, y2);
             if (lane32 == 0) {
                 const int32_t d = M.rn_dof[rn];
                 y[d] = y0;
@@ -223,7 +233,7 @@ __device__ __forceinline__ void apply_block(const double *Q, const double r0, co
 }
 
 // r = b - K x, z = M^-1 r, p = z, partials of (r.z, r.r, b.b)
-template <typename VT>
+template <typename VT, bool D64>
 __global__ __launch_bounds__(kThreads) void k_pcg_residual(Mat M, const VT *vals, const double *b, const double *x,
                                                            const double *pre, double *r, double *z, double *p,
                                                            double *partials)
@@ -234,7 +244,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_residual(Mat M, const VT *vals
         for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
              rn += (int64_t)gridDim.x * kNodesPerBlock) {
             double k0, k1, k2;
-            node_dot(M, vals, x, rn, lane32, k0, k1, k2);
+            node_dot<VT, D64>(M, vals, x, rn, lane32, k0, k1, k2);
             if (lane32 == 0) {
                 const int32_t d = M.rn_dof[rn];
                 const double b0 = b[d], b1 = b[d + 1], b2 = b[d + 2];
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_residual(Mat M, const VT *vals
 }
 
 // Ap = K p, partial of p.Ap
-template <typename VT>
+template <typename VT, bool D64>
 __global__ __launch_bounds__(kThreads) void k_pcg_spmv(Mat M, const VT *vals, const double *p, double *Ap,
                                                        double *partials, const double *scal)
 {
@@ -282,7 +292,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_spmv(Mat M, const VT *vals, co
         for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
              rn += (int64_t)gridDim.x * kNodesPerBlock) {
             double y0, y1, y2;
-            node_dot(M, vals, p, rn, lane32, y0, y1, y2);
+            node_dot<VT, D64>(M, vals, p, rn, lane32, y0, y1, y2);
             if (lane32 < 3) {  // lane c: component c (every lane of the group holds the sums)
                 const int32_t d = M.rn_dof[rn] + lane32;
                 const double y = lane32 == 0 ? y0 : (lane32 == 1 ? y1 : y2);
@@ -408,7 +418,7 @@ __global__ __launch_bounds__(1024) void k_pcg_reduce(int mode, const double *par
 }
 
 // multigrid-preconditioned variant: r = b - K x with partials of (r.r, b.b) at partials + kBlocks
-template <typename VT>
+template <typename VT, bool D64>
 __global__ __launch_bounds__(kThreads) void k_pcg_residual_nz(Mat M, const VT *vals, const double *b,
                                                               const double *x, double *r, double *partials)
 {
@@ -417,7 +427,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_residual_nz(Mat M, const VT *v
     for (int64_t rn = (int64_t)blockIdx.x * kNodesPerBlock + threadIdx.x / kLanesPerNode; rn < M.n_rownodes;
          rn += (int64_t)gridDim.x * kNodesPerBlock) {
         double k0, k1, k2;
-        node_dot(M, vals, x, rn, lane32, k0, k1, k2);
+        node_dot<VT, D64>(M, vals, x, rn, lane32, k0, k1, k2);
         if (lane32 < 3) {
             const int32_t d = M.rn_dof[rn] + lane32;
             const double bc = b[d], rc = bc - (lane32 == 0 ? k0 : (lane32 == 1 ? k1 : k2));
@@ -474,7 +484,7 @@ Mat make_mat(const ewfe_plan *p, const double *d64 = nullptr)
     return M;
 }
 
-template <typename VT>
+template <typename VT, bool D64>
 int pcg(const ewfe_plan *p, const VT *vals, const double *d64, const double *b, double *x, const int32_t *rows,
         int64_t n_rows, double rtol, int maxit, int check_every, double *ws, int *iters_out, double *relres_out,
         hipStream_t s)
@@ -486,7 +496,7 @@ int pcg(const ewfe_plan *p, const VT *vals, const double *d64, const double *b, 
     const double rtol2 = rtol * rtol;
     hipLaunchKernelGGL(k_pcg_setup<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, pre, x);
     if (n_rows > 0) hipLaunchKernelGGL(k_set_rows, dim3(256), dim3(256), 0, s, rows, n_rows, b, x);
-    hipLaunchKernelGGL(k_pcg_residual<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, b, (const double *)x,
+    hipLaunchKernelGGL((k_pcg_residual<VT, D64>), dim3(kBlocks), dim3(kThreads), 0, s, M, vals, b, (const double *)x,
                        (const double *)pre, r, z, pp, partials);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 0, (const double *)partials, scal, rtol2, maxit);
     HIP_CHECK(hipGetLastError());
@@ -497,7 +507,7 @@ int pcg(const ewfe_plan *p, const VT *vals, const double *d64, const double *b, 
         HIP_CHECK(hipStreamSynchronize(s));
         if (host_scal[S_DONE] != 0.0 || it >= maxit) break;
         for (int k = 0; k < check_every && it < maxit; ++k, ++it) {
-            hipLaunchKernelGGL(k_pcg_spmv<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, (const double *)pp, Ap,
+            hipLaunchKernelGGL((k_pcg_spmv<VT, D64>), dim3(kBlocks), dim3(kThreads), 0, s, M, vals, (const double *)pp, Ap,
                                partials, (const double *)scal);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 1, (const double *)partials, scal, rtol2,
                                maxit);
@@ -520,7 +530,7 @@ int pcg(const ewfe_plan *p, const VT *vals, const double *d64, const double *b, 
     return 0;
 }
 
-template <typename VT>
+template <typename VT, bool D64>
 int pcg_mg(ewfe_mg *mg, const ewfe_plan *p, int vals_dtype, const VT *vals, const double *d64, const double *b,
            double *x, const int32_t *rows, int64_t n_rows, double rtol, int maxit, int check_every, double *ws,
            int *iters_out, double *relres_out, hipStream_t s)
@@ -533,7 +543,7 @@ int pcg_mg(ewfe_mg *mg, const ewfe_plan *p, int vals_dtype, const VT *vals, cons
     void *sv = reinterpret_cast<void *>(s);
     hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, s, n, x);
     if (n_rows > 0) hipLaunchKernelGGL(k_set_rows, dim3(256), dim3(256), 0, s, rows, n_rows, b, x);
-    hipLaunchKernelGGL(k_pcg_residual_nz<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, b, (const double *)x, r,
+    hipLaunchKernelGGL((k_pcg_residual_nz<VT, D64>), dim3(kBlocks), dim3(kThreads), 0, s, M, vals, b, (const double *)x, r,
                        partials);
     int rc = ewfe::mg_apply(mg, vals, vals_dtype, r, z, nullptr, sv);
     if (rc) return rc;
@@ -549,7 +559,7 @@ int pcg_mg(ewfe_mg *mg, const ewfe_plan *p, int vals_dtype, const VT *vals, cons
         HIP_CHECK(hipStreamSynchronize(s));
         if (host_scal[S_DONE] != 0.0 || it >= maxit) break;
         for (int k = 0; k < check_every && it < maxit; ++k, ++it) {
-            hipLaunchKernelGGL(k_pcg_spmv<VT>, dim3(kBlocks), dim3(kThreads), 0, s, M, vals, (const double *)pp, Ap,
+            hipLaunchKernelGGL((k_pcg_spmv<VT, D64>), dim3(kBlocks), dim3(kThreads), 0, s, M, vals, (const double *)pp, Ap,
                                partials, (const double *)scal);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, s, 1, (const double *)partials, scal, rtol2,
                                maxit);
@@ -589,7 +599,8 @@ bool parse_vals(const ewfe_plan *p, const void *vals, int vals_dtype, bool &f64,
 }
 
 // EWFE_F32_D64: E_r = -(sum of row node r's off-diagonal blocks) - (its diagonal block), the FP32 values summed in
-// FP64 (32 lanes per node, a fixed lane assignment and butterfly: bitwise reproducible). K_e of every element maps a
+// FP64 (32 lanes per node, a fixed lane assignment and butterfly: bitwise reproducible; such sums are exact unless the
+// values' magnitudes spread over more than ~24 binades, so any order gives the same corrections). K_e of every element maps a
 // rigid translation to zero (the shape-function gradients of an element sum to zero), so the FP64 matrix has
 // diagonal blocks equal to minus the sums of the off-diagonal ones up to its own rounding; with E added to the
 // rounded diagonal blocks every row of the stored matrix sums to zero again.
@@ -618,9 +629,7 @@ __global__ __launch_bounds__(kThreads) void k_rebuild_diag64(Mat M, const float 
         }
         double out = 0.0;
         for (int t = 0; t < 9; ++t) {
-            // the butterfly of the structured gather's fused version: xor 16, 8, 4, 2, 1
-            double x = a[t];
-            for (int off = 16; off > 0; off >>= 1) x += __shfl_xor(x, off, 32);
+            const double x = group_sum32(a[t]);
             const double d = group_sum32(dg[t]);  // the one lane holding the block: exact
             if (lane32 == t) out = -x - d;
         }
@@ -694,11 +703,14 @@ extern "C" int ewfe_pcg_mg(ewfe_mg *mg, const void *vals, int vals_dtype, const 
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     double *ws = reinterpret_cast<double *>(workspace);
     if (f64)
-        return pcg_mg<double>(mg, p, EWFE_F64, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit,
-                              check_every, ws, iters_out, relres_out, s);
-    // the V-cycle reads the FP32 values (the hierarchy is set up from them); the CG operator takes the FP64 blocks
-    return pcg_mg<float>(mg, p, EWFE_F32, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit,
-                         check_every, ws, iters_out, relres_out, s);
+        return pcg_mg<double, false>(mg, p, EWFE_F64, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol,
+                                     maxit, check_every, ws, iters_out, relres_out, s);
+    // the V-cycle reads the FP32 values (the hierarchy is set up from them); the CG operator adds the corrections
+    if (d64)
+        return pcg_mg<float, true>(mg, p, EWFE_F32, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit,
+                                   check_every, ws, iters_out, relres_out, s);
+    return pcg_mg<float, false>(mg, p, EWFE_F32, (const float *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit,
+                                check_every, ws, iters_out, relres_out, s);
 }
 
 extern "C" int64_t ewfe_pcg_workspace_size(const ewfe_plan *p)
@@ -719,9 +731,12 @@ extern "C" int ewfe_spmv(const ewfe_plan *p, const void *vals, int vals_dtype, c
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const Mat M = make_mat(p, d64);
     if (f64)
-        hipLaunchKernelGGL(k_spmv<double>, dim3(kBlocks), dim3(kThreads), 0, s, M, (const double *)vals, x, y);
+        hipLaunchKernelGGL((k_spmv<double, false>), dim3(kBlocks), dim3(kThreads), 0, s, M, (const double *)vals, x,
+                           y);
+    else if (d64)
+        hipLaunchKernelGGL((k_spmv<float, true>), dim3(kBlocks), dim3(kThreads), 0, s, M, (const float *)vals, x, y);
     else
-        hipLaunchKernelGGL(k_spmv<float>, dim3(kBlocks), dim3(kThreads), 0, s, M, (const float *)vals, x, y);
+        hipLaunchKernelGGL((k_spmv<float, false>), dim3(kBlocks), dim3(kThreads), 0, s, M, (const float *)vals, x, y);
     HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -798,10 +813,13 @@ extern "C" int ewfe_pcg(const ewfe_plan *p, const void *vals, int vals_dtype, co
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     double *ws = reinterpret_cast<double *>(workspace);
     if (f64)
-        return pcg<double>(p, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws,
-                           iters_out, relres_out, s);
-    return pcg<float>(p, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
-                      relres_out, s);
+        return pcg<double, false>(p, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit, check_every,
+                                  ws, iters_out, relres_out, s);
+    if (d64)
+        return pcg<float, true>(p, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws,
+                                iters_out, relres_out, s);
+    return pcg<float, false>(p, (const float *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws,
+                             iters_out, relres_out, s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -878,8 +896,11 @@ extern "C" int ewfe_pcg_noalloc(const ewfe_plan *p, const void *vals, int vals_d
     hipLaunchKernelGGL(k_clear_scalars, dim3(1), dim3(64), 0, s, ws + 7 * p->n_dof + 3 * kBlocks);
     HIP_CHECK(hipGetLastError());
     if (f64)
-        return pcg<double>(p, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws,
-                           iters_out, relres_out, s);
-    return pcg<float>(p, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws, iters_out,
-                      relres_out, s);
+        return pcg<double, false>(p, (const double *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit, check_every,
+                                  ws, iters_out, relres_out, s);
+    if (d64)
+        return pcg<float, true>(p, (const float *)vals, d64, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws,
+                                iters_out, relres_out, s);
+    return pcg<float, false>(p, (const float *)vals, nullptr, b, x, rows_dev, n_rows, rtol, maxit, check_every, ws,
+                             iters_out, relres_out, s);
 }
