@@ -678,7 +678,8 @@ extern "C" int64_t ewfe_vals_bytes(const ewfe_plan *p, int vals_dtype)
     }
     if (vals_dtype == EWFE_F64) return 8 * p->nnz;
     if (vals_dtype == EWFE_F32 || vals_dtype == EWFE_F32_MFMA) return 4 * p->nnz;
-    if (vals_dtype == EWFE_F32_D64 && ewfe::d64_supported(p)) return ewfe::d64_offset_bytes(p->nnz) + 72 * p->n_rownodes;
+    // a size only: answered from the host description, also for plans without device data
+    if (vals_dtype == EWFE_F32_D64 && p->rows_cover_all_dofs) return ewfe::d64_offset_bytes(p->nnz) + 72 * p->n_rownodes;
     ewfe::set_error("ewfe_vals_bytes: dtype must be EWFE_F64, EWFE_F32, EWFE_F32_MFMA or EWFE_F32_D64 (the latter on a "
                     "plan whose row nodes cover every dof)");
     return EWFE_EINVAL;

@@ -426,3 +426,35 @@ def test_reference_shaped_model_adapter():
         assert np.array_equal(nf["dU"][row], 3 * nf["U"][row])
     assert ref.time == 1.5
     assert np.array_equal(ad.readDisplacement(dm), U.numpy())
+
+
+def test_value_array_sizes_and_fp32_solver_options():
+    """ewfe_vals_bytes: 8 nnz (FP64), 4 nnz (FP32, matrix-core FP32), and for FP32 values with FP64 diagonal
+    corrections the FP32 values up to a 256-byte boundary plus 9 doubles per node; other dtypes are rejected. The GPU
+    solver's FP32 options: corrected diagonal blocks by default, the FP32 PCG tolerance derived (None) unless given,
+    given as a number or text, unknown options rejected."""
+    L = _lib.lib()
+    X, conn, _ = O.boxgen(3, 2, 4)
+    base, nDof, _ = O.dof_numbering(len(X), conn)
+    plan, nnz = ctypes.c_void_p(), ctypes.c_int64()
+    _lib.check(L.ewfe_plan_create_host(ctypes.byref(plan), 0, len(conn), _lib.ptr(np.ascontiguousarray(conn)),
+                                       len(X), _lib.ptr(np.ascontiguousarray(base)), nDof, ctypes.byref(nnz), 0))
+    try:
+        n = nnz.value
+        assert L.ewfe_vals_bytes(plan, _lib.EWFE_F64) == 8 * n
+        assert L.ewfe_vals_bytes(plan, _lib.EWFE_F32) == 4 * n == L.ewfe_vals_bytes(plan, _lib.EWFE_F32_MFMA)
+        assert L.ewfe_vals_bytes(plan, _lib.EWFE_F32_D64) == (4 * n + 255) // 256 * 256 + 72 * len(X)
+        assert L.ewfe_vals_bytes(plan, _lib.EWFE_F16) < 0 and L.ewfe_vals_bytes(plan, 9) < 0
+    finally:
+        L.ewfe_plan_destroy(plan)
+
+    from edelweissfe_amd.solvers.nonlinearimplicitstaticgpu import NISTGPU
+
+    s = NISTGPU()
+    assert s.options["fp32TangentDiagonal"] == "rebuilt" and s.options["pcgToleranceFP32"] is None
+    s = NISTGPU(pcgToleranceFP32="1e-5", fp32TangentDiagonal="assembled")
+    assert s.options["pcgToleranceFP32"] == 1e-5 and s.options["fp32TangentDiagonal"] == "assembled"
+    assert NISTGPU(pcgToleranceFP32=2e-6).options["pcgToleranceFP32"] == 2e-6
+    assert NISTGPU(pcgToleranceFP32="none").options["pcgToleranceFP32"] is None
+    with pytest.raises(AttributeError):
+        NISTGPU(fp32Diagonal="rebuilt")
