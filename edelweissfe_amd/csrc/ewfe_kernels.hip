@@ -700,16 +700,16 @@ struct RecordArgs {
 };
 
 // LDS layout of the per-quadrature-point data of k_hex8_elem, per field f: 64 values, one per (element le,
-// quadrature point q), in the precision of the tangent arithmetic (doubles, or floats for the FP32 tangent of the
-// materials other than J2). Fields: per-node gradients g (and p for the total-Lagrangian form), 3 per node, then the tangent
+// quadrature point q), in the precision of the tangent arithmetic (doubles, or floats for the FP32 tangent).
+// Fields: per-node gradients g (and p for the total-Lagrangian form), 3 per node, then the tangent
 // coefficients alpha wd, beta wd [, gamma wd for the total-Lagrangian form; gamma = beta for small strain]
 // [, c wd and the flow direction N for J2] [, the stress S wd for St. Venant-Kirchhoff, whose tangent has a
-// stress term; KEEP_S keeps the six fields as staging space for the matrix-core tangent].
-template <int FORM, int MAT, bool KEEP_S = false>
+// stress term] [, XF fields of staging space for the matrix-core tangent, from field XS].
+template <int FORM, int MAT, int XF = 0>
 struct Hex8Layout {
     // HAS_N: the per-point 6-vector slot NV (the J2 flow direction, or B = F F^T for St. Venant-Kirchhoff)
     static constexpr bool TL = FORM == FORM_TL, HAS_N = MAT == M_J2 || MAT == M_SVK;
-    static constexpr bool HAS_S = KEEP_S || MAT == M_SVK;
+    static constexpr bool HAS_S = MAT == M_SVK;
     static constexpr int G = 0;
     static constexpr int P = TL ? 24 : 0;  // aliases g for small strain
     static constexpr int AL = TL ? 48 : 24;
@@ -718,8 +718,16 @@ struct Hex8Layout {
     static constexpr int CC = TL ? AL + 3 : AL + 2;  // only with HAS_N
     static constexpr int NV = CC + 1;                // 6, only with HAS_N
     static constexpr int S = HAS_N ? NV + 6 : CC;    // 6, only with HAS_S
-    static constexpr int NF = HAS_S ? S + 6 : S;
+    static constexpr int XS = HAS_S ? S + 6 : S;
+    static constexpr int NF = XS + XF;
 };
+
+// staging fields of the matrix-core tangent: T + G and Tb in record slot order (2 x 300 floats) and S_ab (36 floats)
+template <typename PT>
+constexpr int mfma_staging_fields()
+{
+    return (int)((2 * 300 + 36) * sizeof(float) + 64 * sizeof(PT) - 1) / (int)(64 * sizeof(PT));
+}
 
 // A 64-lane ds_*_b64 is served 16 lanes at a time by 32 four-byte banks, i.e. 16 double slots. Phase 1 writes
 // a field with lanes (le, q) (fixed field), phase 2 reads node fields with lanes (le, a) at a fixed q, where
@@ -745,16 +753,16 @@ __device__ __forceinline__ int slot_node(int base, int b, int i, int q, int le)
 // products (q, j) as its inner dimension: 24 MFMAs for J2, 18 otherwise, per element. The nodal forces fa come
 // from phase 1 in FP64, exactly as in the other variants (P and F are bitwise the same). T + G and Tb are staged
 // in LDS in the record's slot order, each slot written by one lane, and summed when the record is written out
-// (deterministic). One element at a time; the stress fields of the layout (not written by phase 1) hold the
-// staging.
-template <int FORM, int MAT, typename VT>
-__device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const RecordArgs &R, double *s_qp,
+// (deterministic). One element at a time; staging fields at the end of the layout (not written by phase 1) hold the
+// staging. PT: the per-point data's precision (floats: the operands need no rounding).
+template <int FORM, int MAT, typename VT, typename PT>
+__device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const RecordArgs &R, PT *s_qp,
                                                   int32_t *s_el, int le, int t, bool active, int64_t q0, int64_t el,
                                                   const double (&fa)[3], const double (&st_t)[7])
 {
     typedef float f4 __attribute__((ext_vector_type(4)));
     constexpr int NQ = 8, EPB = 8;
-    using L = Hex8Layout<FORM, MAT, true>;
+    using L = Hex8Layout<FORM, MAT, mfma_staging_fields<PT>()>;
     constexpr int PB = L::TL ? L::P : L::G;  // the gradient the tangent's p terms use
     const int lane = threadIdx.x;
     const int a = t;
@@ -769,10 +777,11 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
     if (active && FORM == FORM_SS && MAT == M_J2)
         for (int s = 0; s < 7; ++s) A.state_t[((int64_t)s * NQ + t) * n_el + pos] = st_t[s];
     lds_barrier();  // s_el is written
-    float *s_ta = reinterpret_cast<float *>(s_qp + L::S * 64);  // T + G in record slot order (300 values)
-    float *s_tb = s_ta + kRec8;                                   // Tb in record slot order
-    float *s_sg = s_tb + kRec8;                                   // S_ab of the pairs a <= b (36 values)
-    static_assert(2 * kRec8 + kPairs8 <= 6 * 64 * 2, "the staging must fit in the stress fields");
+    float *s_ta = reinterpret_cast<float *>(s_qp + L::XS * 64);  // T + G in record slot order (300 values)
+    float *s_tb = s_ta + kRec8;                                    // Tb in record slot order
+    float *s_sg = s_tb + kRec8;                                    // S_ab of the pairs a <= b (36 values)
+    static_assert((2 * kRec8 + kPairs8) * sizeof(float) <= mfma_staging_fields<PT>() * 64 * sizeof(PT),
+                  "the staging must fit in its fields");
     static_assert((kRec8 * 4) % 16 == 0, "staged records must be 16-byte aligned");
 
     // (B) one element at a time: lane (g, c) supplies rows c and 16 + c for points g and g + 4
@@ -883,14 +892,14 @@ __device__ __forceinline__ void hex8_tangent_mfma(const EvalArgs &A, const Recor
     }
 }
 
-// Precision of the per-point data and of the tangent arithmetic of k_hex8_elem: that of the records. (The J2
+// Precision of the per-point data and of the tangent arithmetic of k_hex8_elem: that of the records, also for the
+// matrix-core tangent, whose operands are FP32 (the float fields need no rounding when they are packed). (The J2
 // consistent tangent has the deviatoric stiffness 2G - a2 - a1, about 2H/3, along the flow direction as a difference
 // of terms of size 2G, so FP32 arithmetic puts ~240 times the rounding of its terms into that soft mode; with the
 // assembled FP32 diagonal blocks the 128^3 J2 schedule needed 12 instead of 7 solves, with the FP64 diagonal
-// corrections of EWFE_F32_D64, which make every row sum to zero again, it needs the FP64 tangent's 7.) The
-// matrix-core tangent keeps FP64 point data and rounds its operands when it packs them.
+// corrections of EWFE_F32_D64, which make every row sum to zero again, it needs the FP64 tangent's 7.)
 template <int MAT, typename VT, bool MF>
-using hex8_point_t = typename std::conditional<MF, double, VT>::type;
+using hex8_point_t = VT;
 
 // occupancy the compiler is asked for: float point data and accumulators fit 3 waves per SIMD (4 waves need
 // register spills)
@@ -910,7 +919,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hex8_min_wav
 {
     constexpr int NN = 8, NQ = 8, EPB = 8;
     using PT = hex8_point_t<MAT, VT, MF>;
-    using L = Hex8Layout<FORM, MAT, MF>;
+    using L = Hex8Layout<FORM, MAT, MF ? mfma_staging_fields<PT>() : 0>;
     constexpr int QP_WORDS = L::NF * 64;
     static_assert((EPB / 2) * kRec8 * sizeof(VT) <= QP_WORDS * sizeof(PT),
                   "record staging must fit in the quadrature-point area");
@@ -1050,7 +1059,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(hex8_min_wav
 
     if constexpr (MF) {
         static_assert(sizeof(VT) == 4, "the matrix-core tangent writes FP32 records");
-        hex8_tangent_mfma<FORM, MAT, VT>(A, R, s_qp, s_el, le, t, active, q0, el, fa, st_t);
+        hex8_tangent_mfma<FORM, MAT, VT, PT>(A, R, s_qp, s_el, le, t, active, q0, el, fa, st_t);
         return;
     }
     // phase 2: lane (le, a = t) builds the blocks K_ab, b = a + j (mod 8), j < nb, in the precision PT
