@@ -210,8 +210,8 @@ class ElementAssembly(PlanLinearAlgebra):
         """Fills P (= sum Pe), F (= sum |Pe|), the CSR values and the trial state. Returns pNewDT (device
         tensor, 1e36 or < 1 when a quadrature point asks for a cutback). FP32 values: ``mfma=True`` forms the
         tangents in FP32 on the matrix cores (8-node elements, and 20-node elements assembled through the row-node
-        gather), otherwise in FP32 arithmetic on the vector units (8-node elements, materials other than J2) or
-        with FP64 FMAs rounded when stored (J2, 20-node elements); fp32-d64 values (``newValues``) then also
+        gather), otherwise in FP32 arithmetic on the vector units (8-node elements) or with FP64 FMAs rounded
+        when stored (20-node elements); fp32-d64 values (``newValues``) then also
         receive their FP64 diagonal blocks."""
         for t in (U_np, dU, P, F):
             if t.numel() != self.nDof or t.dtype != self.torch.float64 or not t.is_contiguous():

@@ -59,11 +59,10 @@ enum ewfe_mat_kind {
     EWFE_ST_VENANT_KIRCHHOFF = 3
 };
 /* Precision of the assembled tangent. EWFE_F32 and EWFE_F32_MFMA both store FP32 CSR values. EWFE_F32 forms the
- * element tangents of 8-node elements with linear elastic, Neo-Hooke or St. Venant-Kirchhoff material in FP32
- * arithmetic on the vector units (per-point data rounded to FP32); those of J2 (whose plastic tangent is a
- * difference of large terms) and of 20-node elements are formed with FP64 FMAs and rounded when they are stored; EWFE_F32_MFMA forms them in FP32 on the matrix
- * cores (v_mfma_f32_16x16x4_f32) for 8-node elements and for 20-node elements assembled through the row-node gather
- * (20-node plans on the colour path take it as EWFE_F32). The residual, nodal forces and state stay FP64. Only ewfe_eval_assemble takes EWFE_F32_MFMA; every other
+ * element tangents of 8-node elements in FP32 arithmetic on the vector units (per-point data rounded to FP32); those
+ * of 20-node elements are formed with FP64 FMAs and rounded when they are stored; EWFE_F32_MFMA forms them in FP32 on
+ * the matrix cores (v_mfma_f32_16x16x4_f32) for 8-node elements and for 20-node elements assembled through the
+ * row-node gather (20-node plans on the colour path take it as EWFE_F32). The residual, nodal forces and state stay FP64. Only ewfe_eval_assemble takes EWFE_F32_MFMA; every other
  * call takes EWFE_F32 for FP32 values.
  * EWFE_F32_D64 stores the EWFE_F32 values (formed the same way, bitwise equal) followed, at byte offset
  * ewfe_vals_bytes(plan, EWFE_F32) rounded up to 256, by FP64 corrections of the diagonal blocks: 9 doubles per row

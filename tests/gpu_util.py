@@ -37,8 +37,8 @@ class GpuEval:
         return self.torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device="cuda")
 
     def __call__(self, U_np, dU, dtype="fp64"):
-        """dtype: "fp64", "fp32" (FP32 values; FP32 tangent arithmetic for 8-node elements of the materials other
-        than J2, FP64 FMAs rounded when stored otherwise) or "fp32-mfma" (tangent on the matrix cores)."""
+        """dtype: "fp64", "fp32" (FP32 values; FP32 tangent arithmetic for 8-node elements, FP64 FMAs rounded
+        when stored for 20-node elements) or "fp32-mfma" (tangent on the matrix cores)."""
         P, F = self.dm.constructDofVector(), self.dm.constructDofVector()
         vals = self.asm.newValues("fp64" if dtype == "fp64" else "fp32")
         pdt = self.asm.evaluate(self.vec(U_np), self.vec(dU), P, F, vals, mfma=dtype == "fp32-mfma")
