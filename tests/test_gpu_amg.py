@@ -305,11 +305,14 @@ def test_fp32_values_with_fp64_diagonal_blocks():
     assert ed < e32 / 4
 
 
-@pytest.mark.parametrize("case", ["C3D8 reversed element order", "C3D20", "C3D8TL"])
+@pytest.mark.parametrize("case", ["C3D8 reversed element order", "C3D20", "C3D20 colour path", "C3D8TL",
+                                  "C3D8 4 pipeline chunks"])
 def test_fp32_d64_corrections_on_other_assembly_paths(case):
     """The diagonal corrections of EWFE_F32_D64 from the separate rebuild pass (meshes the structured 8-node gather does
-    not take, and 20-node elements): the FP32 values bitwise the EWFE_F32 ones, diagonal block + correction = minus the
-    sum of the off-diagonal FP32 blocks of its rows."""
+    not take, 20-node elements through their gather or the colour path) and from the structured gather of a chunked
+    pipeline (each chunk's rows on the second stream): the FP32 values bitwise the EWFE_F32 ones, diagonal block +
+    correction = minus the sum of the off-diagonal FP32 blocks of its rows."""
+    from edelweissfe_amd._lib import check, lib
     print(f"fp32-d64 corrections: {case}", flush=True)
     el = case.split()[0]
     X, conn, _ = jittered_box(4, el, seed=7)
@@ -318,7 +321,11 @@ def test_fp32_d64_corrections_on_other_assembly_paths(case):
     mat, mp = ("neohookeanwa", [80.0, 200.0]) if el.endswith("TL") else ("vonmises", J2)
     out = {}
     for dt in ("fp32", "fp32-d64"):
-        g = GpuEval(el, mat, mp, X, conn)
+        g = GpuEval(el, mat, mp, X, conn, pipelineChunks=4 if "chunks" in case else 0)
+        if "colour" in case:
+            check(lib().ewfe_plan_set_row_gather(g.asm._plan, 0))
+        if "chunks" in case:
+            assert g.asm.pipelineChunks == 4 and g.asm.structuredMesh
         nd = g.dm.nDof
         u = np.zeros((len(X), 3))
         u[:, 2] = 5e-3 * X[:, 2]
