@@ -1490,15 +1490,18 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
     const int lane = threadIdx.x, h = lane >> 5, hl = lane & 31;
     const int64_t idx = 2 * pair + h;
     const bool live = idx < count;  // the second half may lie past the end of the range
-    const int64_t N1 = G.E1 + 1, N2 = G.E2 + 1, N0 = G.E0 + 1;
-    const int64_t r = live ? (int64_t)rorder[q_begin + idx] : 0;
-    const int64_t i0 = r / (N1 * N2), i1 = (r / N2) % N1, i2 = r % N2;
+    // node and element indices in 32 bits (the plan's CSR is int32-indexed, so every count here fits): 32-bit
+    // division instead of the long 64-bit sequences
+    const int32_t E0 = (int32_t)G.E0, E1 = (int32_t)G.E1, E2 = (int32_t)G.E2;
+    const int32_t N1 = E1 + 1, N2 = E2 + 1, N0 = E0 + 1;
+    const int32_t r = live ? rorder[q_begin + idx] : 0;
+    const int32_t i0 = r / (N1 * N2), rem = r - i0 * (N1 * N2), i1 = rem / N2, i2 = rem - i1 * N2;
     const int lo0 = i0 > 0, lo1 = i1 > 0, lo2 = i2 > 0;
-    const int64_t n12 = G.E1 * G.E2;
+    const int32_t n12 = E1 * E2;
     auto elem = [&](int o, int64_t &e) {  // incident element at corner offset o (0..7); false outside the grid
-        const int64_t e0 = i0 - 1 + (o >> 2), e1 = i1 - 1 + ((o >> 1) & 1), e2 = i2 - 1 + (o & 1);
-        e = e0 * n12 + e1 * G.E2 + e2;
-        return e0 >= 0 && e0 < G.E0 && e1 >= 0 && e1 < G.E1 && e2 >= 0 && e2 < G.E2;
+        const int32_t e0 = i0 - 1 + (o >> 2), e1 = i1 - 1 + ((o >> 1) & 1), e2 = i2 - 1 + (o & 1);
+        e = (int64_t)(e0 * n12 + e1 * E2 + e2);
+        return e0 >= 0 && e0 < E0 && e1 >= 0 && e1 < E1 && e2 >= 0 && e2 < E2;
     };
     // 1) lane hl fetches blocks hl and hl + 32 (corner o = block >> 3, element node b = block & 7)
 #pragma unroll
