@@ -1563,22 +1563,22 @@ __global__ __launch_bounds__(64) void k_gather8s(GridArgs G, const int32_t *__re
     // through the half's block area (free once every lane has summed its contributions and the forces), then lane
     // t < 9 sums entry t over the neighbours in ascending order: sums of FP32 values, exact in FP64 unless their
     // magnitudes spread over more than ~24 binades, so the separate rebuild pass gives the same corrections
-    if (d64) {  // uniform over the grid
-        double *st = reinterpret_cast<double *>(s_blk[h]);  // 32 x 9 doubles: 2304 of the half's bytes
-        static_assert(32 * 9 * sizeof(double) <= 64 * SB * sizeof(VT), "the rounded blocks must fit the block area");
+    if (d64) {  // uniform over the grid; only FP32 values carry corrections, so the rounded blocks are floats
+        float *st = reinterpret_cast<float *>(s_blk[h]);  // 32 x 9 floats (lane stride 9 words: no bank conflicts)
+        static_assert(32 * 9 * sizeof(float) <= 64 * SB * sizeof(VT), "the rounded blocks must fit the block area");
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 9; ++t) st[9 * m + t] = (live && m < deg) ? (double)(VT)acc[t] : 0.0;  // acc = 0 beyond
+        for (int t = 0; t < 9; ++t) st[9 * m + t] = (live && m < deg) ? (float)acc[t] : 0.f;  // acc = 0 beyond
         __syncthreads();
         if (live && m < 9) {
             const int dr = (lo0 * c1 + lo1) * c2 + lo2;  // rank of the node itself (offset 0, 0, 0)
             double x = 0.0;
 #pragma unroll
             for (int k = 0; k < 32; ++k) {  // every load issued at once: no branch, the diagonal masked by a select
-                const double v = st[9 * k + m];
+                const double v = (double)st[9 * k + m];
                 x += k == dr ? 0.0 : v;
             }
-            d64[9 * r + m] = -x - st[9 * dr + m];
+            d64[9 * r + m] = -x - (double)st[9 * dr + m];
         }
     }
     // 3) the node's 3 CSR rows: FP64 staged in its half's block area and written in 16-byte units
