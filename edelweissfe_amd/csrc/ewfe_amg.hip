@@ -67,6 +67,8 @@ struct MgLevel {
     int32_t *wptr = nullptr, *wcol = nullptr;
     double *wval = nullptr;
     int32_t *rptr = nullptr, *ri = nullptr, *rt = nullptr;
+    double *rval = nullptr;  // P's blocks in restriction order, rval[q] = pval[rt[q]] (set up with P; FP32 P: floats)
+    int64_t n_r = 0;         // entries of the restriction lists
     // V-cycle vectors (3n each); level 0 uses the caller's right-hand side / result for b / x
     double *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;
 };
@@ -699,12 +701,23 @@ __global__ __launch_bounds__(kThreads) void k_mg_cheb_step(OP A, const DT *dinv,
     }
 }
 
-// b_c[I] = sum over R(I) of P_t^T r[i] (one 64-lane wave per aggregate, fixed-order butterfly)
+// rval[q] = pval[rt[q]]: P's blocks in the order restriction reads them (contiguous per aggregate)
+template <typename PT>
+__global__ void k_mg_rgather(int64_t n_r, const int32_t *__restrict__ rt, const PT *__restrict__ pval,
+                             PT *__restrict__ rval)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < 9 * n_r; k += stride)
+        rval[k] = pval[9 * (int64_t)rt[k / 9] + k % 9];
+}
+
+// b_c[I] = sum over R(I) of P_t^T r[i] (one 64-lane wave per aggregate, fixed-order butterfly); the blocks from the
+// restriction-ordered copy rval
 template <typename PT, typename XT>
 __global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t *__restrict__ rptr,
-                                                    const int32_t *__restrict__ ri, const int32_t *__restrict__ rt,
-                                                    const PT *__restrict__ pval, const XT *__restrict__ r,
-                                                    double *__restrict__ bc, const double *done)
+                                                    const int32_t *__restrict__ ri, const PT *__restrict__ rval,
+                                                    const XT *__restrict__ r, double *__restrict__ bc,
+                                                    const double *done)
 {
     if (done && *done != 0.0) return;
     const int64_t I = blockIdx.x;
@@ -712,7 +725,7 @@ __global__ __launch_bounds__(64) void k_mg_restrict(int64_t n_agg, const int32_t
     double a0 = 0, a1 = 0, a2 = 0;
     for (int32_t q = rptr[I] + lane; q < rptr[I + 1]; q += 64) {
         const int32_t i = ri[q];
-        const PT *p = pval + 9 * (int64_t)rt[q];
+        const PT *p = rval + 9 * (int64_t)q;
         const double r0 = (double)r[3 * i], r1 = (double)r[3 * i + 1], r2 = (double)r[3 * i + 2];
         a0 += (double)p[0] * r0 + (double)p[3] * r1 + (double)p[6] * r2;
         a1 += (double)p[1] * r0 + (double)p[4] * r1 + (double)p[7] * r2;
@@ -967,6 +980,9 @@ int mg_setup(ewfe_mg *mg, const void *vals, const int32_t *rows, int64_t n_rows,
                            (const int32_t *)L.ri, (const int32_t *)L.rt, (const VT *)L.pval,
                            (const int32_t *)L.wptr, (const int32_t *)L.wcol, (const double *)L.wval,
                            (const int32_t *)N.aptr, (const int32_t *)N.acol, N.aval);
+        if (L.n_r > 0)
+            hipLaunchKernelGGL(k_mg_rgather<VT>, dim3(2048), dim3(256), 0, s, L.n_r, (const int32_t *)L.rt,
+                               (const VT *)L.pval, (VT *)L.rval);
         HIP_CHECK(hipGetLastError());
     }
     RC(coarse_inverse(mg, s));
@@ -1020,7 +1036,7 @@ int vcycle(ewfe_mg *mg, int l, const void *vals, const double *done, hipStream_t
         hipLaunchKernelGGL((k_mg_resid<OP, XT, XT>), dim3(kBlocks), dim3(kThreads), 0, s, op, (const XT *)r,
                            (const XT *)dc, r, done);
         hipLaunchKernelGGL((k_mg_restrict<VT, XT>), dim3((unsigned)N.n), dim3(64), 0, s, N.n, (const int32_t *)L.rptr,
-                           (const int32_t *)L.ri, (const int32_t *)L.rt, (const VT *)L.pval, (const XT *)r, N.b,
+                           (const int32_t *)L.ri, (const VT *)L.rval, (const XT *)r, N.b,
                            done);
         RC(vcycle<VT>(mg, l + 1, vals, done, s));
         hipLaunchKernelGGL((k_mg_prolong<VT, XT>), dim3(ew), dim3(256), 0, s, L.n, (const int32_t *)L.pptr,
@@ -1176,8 +1192,10 @@ extern "C" int ewfe_mg_create(const ewfe_plan *p, int degree, double ratio, ewfe
                 (rc = upload(mg, L.pslot, h.pslot)) || (rc = alloc(mg, L.pval, 9 * h.pcol.size())) ||
                 (rc = upload(mg, L.wptr, h.wptr)) || (rc = upload(mg, L.wcol, h.wcol)) ||
                 (rc = alloc(mg, L.wval, 9 * h.wcol.size())) || (rc = upload(mg, L.rptr, h.rptr)) ||
-                (rc = upload(mg, L.ri, h.ri)) || (rc = upload(mg, L.rt, h.rt)))
+                (rc = upload(mg, L.ri, h.ri)) || (rc = upload(mg, L.rt, h.rt)) ||
+                (rc = alloc(mg, L.rval, 9 * h.rt.size())))
                 return fail(rc);
+            L.n_r = (int64_t)h.rt.size();
             // k_mg_ap keeps a row of A_l in LDS
             const int64_t max_row = l > 0 ? kCoarseMaxRow : (p->max_deg <= kFineMaxRow ? kFineMaxRow : kWideMaxRow);
             for (int64_t i = 0; i < h.n; ++i)
