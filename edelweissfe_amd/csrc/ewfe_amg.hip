@@ -744,10 +744,13 @@ __global__ void k_mg_prolong(int64_t n, const int32_t *__restrict__ pptr, const 
                              const double *done)
 {
     if (done && *done != 0.0) return;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    // 4 lanes per fine node: lane s takes blocks s, s + 4, ... of the node's row (neighbouring lanes read neighbouring
+    // blocks), a fixed 2-step butterfly sums them (blockDim is a multiple of 4, so a node's lanes share a wave)
+    const int sub = threadIdx.x & 3;
+    const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 2;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2; i < n; i += stride) {
         double a0 = 0, a1 = 0, a2 = 0;
-        for (int32_t t = pptr[i]; t < pptr[i + 1]; ++t) {
+        for (int32_t t = pptr[i] + sub; t < pptr[i + 1]; t += 4) {
             const int32_t J = 3 * pcol[t];
             const PT *p = pval + 9 * (int64_t)t;
             const double c0 = xc[J], c1 = xc[J + 1], c2 = xc[J + 2];
@@ -755,8 +758,11 @@ __global__ void k_mg_prolong(int64_t n, const int32_t *__restrict__ pptr, const 
             a1 += (double)p[3] * c0 + (double)p[4] * c1 + (double)p[5] * c2;
             a2 += (double)p[6] * c0 + (double)p[7] * c1 + (double)p[8] * c2;
         }
-        x[3 * i] = (XT)((double)x[3 * i] + a0), x[3 * i + 1] = (XT)((double)x[3 * i + 1] + a1),
-        x[3 * i + 2] = (XT)((double)x[3 * i + 2] + a2);
+        a0 += __shfl_xor(a0, 1, 4), a1 += __shfl_xor(a1, 1, 4), a2 += __shfl_xor(a2, 1, 4);
+        a0 += __shfl_xor(a0, 2, 4), a1 += __shfl_xor(a1, 2, 4), a2 += __shfl_xor(a2, 2, 4);
+        if (sub == 0)
+            x[3 * i] = (XT)((double)x[3 * i] + a0), x[3 * i + 1] = (XT)((double)x[3 * i + 1] + a1),
+            x[3 * i + 2] = (XT)((double)x[3 * i + 2] + a2);
     }
 }
 
